@@ -1,0 +1,64 @@
+"""Build step for the native parts: the gfx950 HIP library (C ABI) and the
+Fortran ISO_C_BINDING bridge.  Everything is built in-tree so the shared
+objects travel with the source tree."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+# gfx950 only; packed FP32 instructions are disabled for all device code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, cwd=ROOT):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s" % (" ".join(cmd),
+                                                          r.stdout[-4000:]))
+    return r.stdout
+
+
+def build_hip():
+    out = os.path.join(HERE, "libbats_hip.so")
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-I" + os.path.join(ROOT, "include"),
+                                "-o", tmp,
+                                os.path.join(HERE, "csrc", "bats_hip.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran():
+    """Compile the bind(C) bridge module and the demonstration driver
+    against libbats_hip.so with AMD flang, when it is installed."""
+    if not (os.path.exists(FLANG) or shutil.which("amdflang")):
+        return None
+    fc = FLANG if os.path.exists(FLANG) else shutil.which("amdflang")
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(fdir, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(fdir, "bats_fortran_driver")
+    srcs = [os.path.join(fdir, "ModHipBridge.f90"),
+            os.path.join(fdir, "bats_fortran_driver.f90")]
+    if not all(os.path.exists(s) for s in srcs):
+        return None
+    _run([fc, "-O2", "-cpp", "-fdefault-real-8", "-J", bdir, "-o", exe] +
+         srcs + ["-L" + HERE, "-lbats_hip", "-Wl,-rpath,$ORIGIN/.."])
+    return exe
+
+
+def build_all():
+    built = [build_hip()]
+    f = build_fortran()
+    if f:
+        built.append(f)
+    return built

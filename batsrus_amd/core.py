@@ -1,0 +1,332 @@
+"""Python mirror of the reference's hot-path interface over the C ABI.
+
+The reference exposes the explicit stage update as Fortran module procedures
+(``calc_face_value``, ``calc_face_flux``, ``update_state``,
+``calc_timestep``, ``set_global_timestep``, ``advance_explicit``,
+``exchange_messages``) on module-global block arrays.  :class:`Session` keeps
+those names, argument meanings and argument order, and calls the gfx950
+library ``libbats_hip.so`` through its C ABI (include/bats_hip.h).  A
+non-zero status raises :class:`StopMpi`, the analogue of the reference's
+``stop_mpi`` abort.
+
+There is no CPU fallback: if the library is missing or no gfx950 device is
+present, construction fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .grid import BlockGrid
+
+__all__ = ["StopMpi", "BatsConfig", "Session", "library_path", "load_library",
+           "LIMITERS", "FLUXES", "ABI_FUNCTIONS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+LIMITERS = {"minmod": 1, "mc": 2, "mc3": 3, "koren": 3, "beta": 4, "no": 5}
+FLUXES = {"rusanov": 1, "linde": 2, "hlld": 3}
+
+STATUS = {0: "BATS_OK", 1: "BATS_EINVAL", 2: "BATS_ENEGFAST2", 3: "BATS_EHLLD",
+          4: "BATS_ENAN", 5: "BATS_EBC", 6: "BATS_EHIP", 7: "BATS_ENOMEM",
+          8: "BATS_ENODEV", 9: "BATS_ELIMIT"}
+
+
+class StopMpi(RuntimeError):
+    """Raised where the reference would call stop_mpi."""
+
+    def __init__(self, code, where):
+        self.code = code
+        name = STATUS.get(code, str(code))
+        msg = _strerror(code)
+        super().__init__(f"{where}: {name}: {msg}")
+
+
+class BatsConfig(C.Structure):
+    _fields_ = [("nI", C.c_int), ("nJ", C.c_int), ("nK", C.c_int),
+                ("nG", C.c_int), ("nDim", C.c_int), ("nBlock", C.c_int),
+                ("nOrder", C.c_int), ("iLimiter", C.c_int),
+                ("LimiterBeta", C.c_double), ("iFlux", C.c_int),
+                ("nStage", C.c_int), ("UseHalfStep", C.c_int),
+                ("Cfl", C.c_double), ("Gamma", C.c_double),
+                ("pMin", C.c_double), ("UseDbTrick", C.c_int),
+                ("UseDtFixed", C.c_int), ("DtFixed", C.c_double),
+                ("cTiny", C.c_double), ("iDevice", C.c_int)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_hp = C.c_void_p
+
+# name -> (restype, argtypes); every entry point of include/bats_hip.h
+ABI_FUNCTIONS = {
+    "bats_init": (C.c_int, [C.POINTER(BatsConfig), C.POINTER(_hp)]),
+    "bats_finalize": (C.c_int, [_hp]),
+    "bats_set_grid": (C.c_int, [_hp, _ip, _dp]),
+    "bats_upload_state": (C.c_int, [_hp, _dp]),
+    "bats_download_state": (C.c_int, [_hp, _dp]),
+    "bats_upload_state_soa": (C.c_int, [_hp, _dp]),
+    "bats_download_state_soa": (C.c_int, [_hp, _dp]),
+    "bats_exchange_messages": (C.c_int, [_hp]),
+    "bats_set_stage": (C.c_int, [_hp, C.c_int]),
+    "bats_calc_face_value": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int]),
+    "bats_calc_face_flux": (C.c_int, [_hp, C.c_int, C.c_int]),
+    "bats_update_state": (C.c_int, [_hp, C.c_int]),
+    "bats_calc_timestep": (C.c_int, [_hp, C.c_int, C.c_int]),
+    "bats_get_face_values": (C.c_int, [_hp] + [_dp] * 6),
+    "bats_get_face_fluxes": (C.c_int, [_hp] + [_dp] * 3),
+    "bats_get_dtmax": (C.c_int, [_hp, _dp, _dp]),
+    "bats_set_global_timestep": (C.c_int, [_hp, C.c_double, _dp]),
+    "bats_advance_explicit": (C.c_int, [_hp, C.c_int]),
+    "bats_advance_stage": (C.c_int, [_hp, C.c_int, C.c_int]),
+    "bats_advance": (C.c_int, [_hp, C.c_int, C.c_double]),
+    "bats_init_timestep": (C.c_int, [_hp]),
+    "bats_get_time": (C.c_int, [_hp, _dp, _dp, _ip]),
+    "bats_synchronize": (C.c_int, [_hp]),
+    "bats_check_nan": (C.c_int, [_hp]),
+    "bats_integrate_domain": (C.c_int, [_hp, _dp]),
+    "bats_last_stage_kernel_ms": (C.c_int, [_hp, _dp, _ip]),
+    "bats_strerror": (C.c_char_p, [C.c_int]),
+}
+
+
+def library_path():
+    return os.path.join(_HERE, "libbats_hip.so")
+
+
+def load_library():
+    """Load libbats_hip.so (built by ``__graft_entry__.build()``)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} is missing: build it with __graft_entry__.build()")
+        lib = C.CDLL(path)
+        for name, (res, args) in ABI_FUNCTIONS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _strerror(code):
+    try:
+        return load_library().bats_strerror(code).decode()
+    except Exception:  # library not loadable: keep the numeric code
+        return "status %d" % code
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
+                       nStage=2, UseHalfStep=True, cfl=0.8, gamma=5.0 / 3.0,
+                       pMin=-1.0, UseDbTrick=False, UseDtFixed=False,
+                       DtFixed=0.0, cTiny=1e-6)
+
+
+def make_config(grid: BlockGrid, device=0, **scheme):
+    s = dict(SCHEME_DEFAULTS)
+    s.update(scheme)
+    c = BatsConfig()
+    c.nI, c.nJ, c.nK = grid.nI, grid.nJ, grid.nK
+    c.nG, c.nDim, c.nBlock = grid.nG, grid.nDim, grid.nBlock
+    c.nOrder = s["nOrder"]
+    lim = s["limiter"]
+    c.iLimiter = LIMITERS[lim] if isinstance(lim, str) else int(lim)
+    c.LimiterBeta = s["beta"]
+    fl = s["flux"]
+    c.iFlux = FLUXES[fl] if isinstance(fl, str) else int(fl)
+    c.nStage = s["nStage"]
+    c.UseHalfStep = int(s["UseHalfStep"])
+    c.Cfl = s["cfl"]
+    c.Gamma = s["gamma"]
+    c.pMin = s["pMin"]
+    c.UseDbTrick = int(s["UseDbTrick"])
+    c.UseDtFixed = int(s["UseDtFixed"])
+    c.DtFixed = s["DtFixed"]
+    c.cTiny = s["cTiny"]
+    c.iDevice = device
+    return c
+
+
+class Session:
+    """One GPU-resident solution: the reference's State_VGB plus the
+    hot-path routines, with reference names and argument order."""
+
+    def __init__(self, grid: BlockGrid, device=0, **scheme):
+        self.lib = load_library()
+        self.grid = grid
+        self.cfg = make_config(grid, device, **scheme)
+        h = _hp()
+        self._check(self.lib.bats_init(C.byref(self.cfg), C.byref(h)),
+                    "bats_init")
+        self.h = h
+        nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
+        cs = np.ascontiguousarray(grid.CellSize_DB, dtype=np.float64)
+        self._check(self.lib.bats_set_grid(self.h, _ptr(nei, C.c_int),
+                                           _ptr(cs)), "bats_set_grid")
+
+    # ------------------------------------------------------------------
+    def _check(self, code, where):
+        if code != 0:
+            raise StopMpi(code, where)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bats_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state transfer (sync_cpu_gpu) ---------------------------------
+    def upload_state(self, State_VGB):
+        S = np.ascontiguousarray(State_VGB, dtype=np.float64)
+        self._check(self.lib.bats_upload_state(self.h, _ptr(S)),
+                    "upload_state")
+
+    def download_state(self, out=None):
+        g = self.grid
+        gI, gJ, gK = g.ghost
+        shp = (g.nBlock, g.nK + 2 * gK, g.nJ + 2 * gJ, g.nI + 2 * gI, 8)
+        S = np.zeros(shp) if out is None else out
+        self._check(self.lib.bats_download_state(self.h, _ptr(S)),
+                    "download_state")
+        return S
+
+    def upload_soa(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self._check(self.lib.bats_upload_state_soa(self.h, _ptr(U)),
+                    "upload_state_soa")
+
+    def download_soa(self):
+        g = self.grid
+        U = np.zeros((8, g.nBlock, g.nK, g.nJ, g.nI))
+        self._check(self.lib.bats_download_state_soa(self.h, _ptr(U)),
+                    "download_state_soa")
+        return U
+
+    # -- reference-named per-block routines ----------------------------
+    def exchange_messages(self):
+        self._check(self.lib.bats_exchange_messages(self.h),
+                    "exchange_messages")
+
+    def set_stage(self, iStage):
+        self._check(self.lib.bats_set_stage(self.h, iStage), "set_stage")
+
+    def calc_face_value(self, iBlock, DoResChangeOnly=False,
+                        DoMonotoneRestrict=True):
+        self._check(self.lib.bats_calc_face_value(
+            self.h, iBlock, int(DoResChangeOnly), int(DoMonotoneRestrict)),
+            "calc_face_value")
+
+    def calc_face_flux(self, DoResChangeOnly, iBlock):
+        self._check(self.lib.bats_calc_face_flux(
+            self.h, int(DoResChangeOnly), iBlock), "calc_face_flux")
+
+    def update_state(self, iBlock):
+        self._check(self.lib.bats_update_state(self.h, iBlock),
+                    "update_state")
+
+    def calc_timestep(self, iBlock, IsPartLocal=False):
+        self._check(self.lib.bats_calc_timestep(self.h, iBlock,
+                                                int(IsPartLocal)),
+                    "calc_timestep")
+
+    def face_shapes(self):
+        g = self.grid
+        return ((g.nK, g.nJ, g.nI + 1), (g.nK, g.nJ + 1, g.nI),
+                (g.nK + 1, g.nJ, g.nI))
+
+    def get_face_values(self):
+        sx, sy, sz = self.face_shapes()
+        arrs = [np.zeros(s + (8,)) for s in (sx, sx, sy, sy, sz, sz)]
+        self._check(self.lib.bats_get_face_values(
+            self.h, *[_ptr(a) for a in arrs]), "get_face_values")
+        return arrs
+
+    def get_face_fluxes(self):
+        fl = [np.zeros(s + (14,)) for s in self.face_shapes()]
+        self._check(self.lib.bats_get_face_fluxes(
+            self.h, *[_ptr(a) for a in fl]), "get_face_fluxes")
+        return fl
+
+    def get_dtmax(self):
+        g = self.grid
+        cb = np.zeros((g.nBlock, g.nK, g.nJ, g.nI))
+        b = np.zeros(g.nBlock)
+        self._check(self.lib.bats_get_dtmax(self.h, _ptr(cb), _ptr(b)),
+                    "get_dtmax")
+        return cb, b
+
+    def get_dtmax_b(self):
+        b = np.zeros(self.grid.nBlock)
+        self._check(self.lib.bats_get_dtmax(self.h, None, _ptr(b)),
+                    "get_dtmax")
+        return b
+
+    # -- production path ------------------------------------------------
+    def set_global_timestep(self, TimeSimulationLimit=-1.0):
+        dt = C.c_double()
+        self._check(self.lib.bats_set_global_timestep(
+            self.h, TimeSimulationLimit, C.byref(dt)), "set_global_timestep")
+        return dt.value
+
+    def advance_explicit(self, DoCalcTimestep=True):
+        self._check(self.lib.bats_advance_explicit(self.h,
+                                                   int(DoCalcTimestep)),
+                    "advance_explicit")
+
+    def advance_stage(self, iStage, DoCalcTimestep=True):
+        self._check(self.lib.bats_advance_stage(self.h, iStage,
+                                                int(DoCalcTimestep)),
+                    "advance_stage")
+
+    def advance(self, nStep, TimeSimulationLimit=-1.0):
+        """BATS_advance for nStep time steps."""
+        self._check(self.lib.bats_advance(self.h, nStep, TimeSimulationLimit),
+                    "advance")
+
+    def init_timestep(self):
+        self._check(self.lib.bats_init_timestep(self.h), "init_timestep")
+
+    def synchronize(self):
+        self._check(self.lib.bats_synchronize(self.h), "synchronize")
+
+    def check_nan(self):
+        self._check(self.lib.bats_check_nan(self.h), "check_nan")
+
+    def time(self):
+        dt, t, n = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.lib.bats_get_time(self.h, C.byref(dt), C.byref(t),
+                                           C.byref(n)), "get_time")
+        return dt.value, t.value, n.value
+
+    def integrate_domain(self):
+        s = np.zeros(8)
+        self._check(self.lib.bats_integrate_domain(self.h, _ptr(s)),
+                    "integrate_domain")
+        return s
+
+    def last_stage_kernel_ms(self):
+        ms, n = C.c_double(), C.c_int()
+        self._check(self.lib.bats_last_stage_kernel_ms(self.h, C.byref(ms),
+                                                       C.byref(n)),
+                    "last_stage_kernel_ms")
+        return ms.value, n.value

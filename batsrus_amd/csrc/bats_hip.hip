@@ -1,0 +1,1182 @@
+// bats_hip.hip -- gfx950 kernels and the C ABI (include/bats_hip.h) of the
+// explicit block-adaptive ideal-MHD stage update.
+//
+// Device layout: the state is held interior-only, variable-major
+// (structure of arrays): U[((v*nBlock + b)*NB + k)*NB + j)*NB + i].  There
+// are no ghost cells in HBM; halos are read directly from the neighbouring
+// blocks' interiors through the neighbour table, which replaces the
+// reference's ghost-cell fill (exchange_messages / BATL message_pass_cell)
+// for same-level blocks.  Two state buffers ping-pong between the stages of
+// the midpoint scheme, so StateOld is never copied.
+//
+// The hot kernel k_stage fuses, per block, what the reference does in
+// calc_face_value + calc_face_flux + update_state (+ calc_timestep at the
+// last stage) (src/ModAdvanceExplicit.f90:140-220):
+//   * a workgroup of 256 threads owns one 16^3 block or four 8^3 blocks;
+//     each thread owns one (i,j) column and marches along k;
+//   * z faces: every thread keeps a 4-plane window of primitive states of
+//     its column in registers and computes the z face above its cell; the
+//     flux carries over to the next plane, so each z face is computed once;
+//   * x/y faces: the plane's primitives plus a 2-cell x/y halo are staged
+//     in LDS, all x and y faces of the plane are spread over the 256
+//     threads, and their fluxes go back to LDS;
+//   * each thread then forms the flux divergence of its cell, updates it
+//     (p -> E, U = Uold + dt*R, E -> p) and writes it; face states and
+//     fluxes never reach HBM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/bats_hip.h"
+#include "mhd_device.hpp"
+
+using namespace bats;
+
+namespace {
+
+enum { S_DTCB = 0, S_DT = 1, S_TSIM = 2, S_NSTEP = 3, S_DTMIN = 4, S_NSCAL = 8 };
+constexpr int kThreads = 256;
+
+struct StageArgs {
+  const double *Uin;   // stage input (stencils and halos)
+  const double *Uold;  // state at the start of the step (StateOld)
+  double *Uout;        // updated state (may alias Uold)
+  const int *nei;      // 6 per block
+  const double *cellsize;  // 3 per block
+  double *scal;        // device scalars (S_*)
+  double *dtmaxB;      // per block minimum of the local time step
+  double *dtmaxCB;     // optional per-cell local time step
+  int *err;
+  int nBlock;
+  long vs;             // variable stride = nBlock*NB^3
+  Phys ph;
+  double beta;
+  int lim;             // LIM_* (LIM_FIRST for nOrder = 1)
+  double DtFactor;     // (Cfl*iStage)/nStage or Cfl
+  int iStage, nStage, halfStep, dbTrick, doDt, roundTripOld, noUpdate;
+  int advanceTime;     // add Dt/nStage to tSimulation when done
+};
+
+// offset (within one variable plane) of cell (i,j,k) of block b; one index
+// may lie in the 2-cell halo: it is resolved to the neighbour block or, at
+// a float outer boundary, to the block's own boundary cell
+// (src/ModCellBoundary.f90:518-559).
+template <int NB>
+__device__ __forceinline__ long cell_off(const int *__restrict__ nei, int b,
+                                         int i, int j, int k) {
+  if (i < 0) {
+    const int n = nei[6 * b + 0];
+    if (n >= 0) { b = n; i += NB; } else { i = 0; }
+  } else if (i >= NB) {
+    const int n = nei[6 * b + 1];
+    if (n >= 0) { b = n; i -= NB; } else { i = NB - 1; }
+  }
+  if (j < 0) {
+    const int n = nei[6 * b + 2];
+    if (n >= 0) { b = n; j += NB; } else { j = 0; }
+  } else if (j >= NB) {
+    const int n = nei[6 * b + 3];
+    if (n >= 0) { b = n; j -= NB; } else { j = NB - 1; }
+  }
+  if (k < 0) {
+    const int n = nei[6 * b + 4];
+    if (n >= 0) { b = n; k += NB; } else { k = 0; }
+  } else if (k >= NB) {
+    const int n = nei[6 * b + 5];
+    if (n >= 0) { b = n; k -= NB; } else { k = NB - 1; }
+  }
+  return (((long)b * NB + k) * NB + j) * NB + i;
+}
+
+__device__ __forceinline__ void load_state(const double *__restrict__ U,
+                                           long vs, long off, double *q) {
+#pragma unroll
+  for (int v = 0; v < NVAR; v++) q[v] = U[v * vs + off];
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = fmin2(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
+template <int DIR, int FLUX, bool FULL>
+__device__ __forceinline__ int face_flux(const Phys &ph, double area, int lim,
+                                         double beta, const double *q0,
+                                         const double *q1, const double *q2,
+                                         const double *q3, double *F,
+                                         double &vdt, double &unA) {
+  double L[NVAR], R[NVAR];
+  face_states(lim, beta, q0, q1, q2, q3, L, R);
+  return numerical_flux<DIR, FLUX, FULL>(ph, area, L, R, F, vdt, unA);
+}
+
+// ---------------------------------------------------------------------
+// fused stage kernel
+// ---------------------------------------------------------------------
+template <int NB, int FLUX>
+__global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
+  constexpr int NC = NB * NB;          // columns per block
+  constexpr int BPW = kThreads / NC;   // blocks per workgroup
+  constexpr int NP = NB + 4;           // padded plane edge
+  constexpr int NFX = (NB + 1) * NB;   // x (or y) faces per plane
+  constexpr int NFS = 9;               // flux slots: 0..6, PP<-vdt, EN
+  __shared__ double sP[BPW * NVAR * NP * NP];
+  __shared__ double sF[BPW * 2 * NFS * NFX];
+  __shared__ double sRed[kThreads / 64];
+
+  const int tid = threadIdx.x;
+  const int lb = tid / NC, col = tid % NC, j = col / NB, i = col % NB;
+  const int b = blockIdx.x * BPW + lb;
+  const bool act = b < a.nBlock;
+  const int bb = act ? b : 0;
+  const double *__restrict__ U = a.Uin;
+  const long vs = a.vs;
+  const Phys ph = a.ph;
+  const int lim = a.lim;
+  const double beta = a.beta;
+  int err = 0;
+
+  auto sPi = [&](int l, int v, int jj, int ii) {
+    return ((l * NVAR + v) * NP + jj) * NP + ii;
+  };
+  auto sFi = [&](int l, int dir, int v, int q) {
+    return ((l * 2 + dir) * NFS + v) * NFX + q;
+  };
+
+  const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
+               dzb = a.cellsize[3 * bb + 2];
+  const double AreaZ = dxb * dyb;
+  const double CellVolume = dxb * dyb * dzb;
+
+  const double DtUnscaled = a.scal[S_DTCB];
+  UpdateCtl uc;
+  uc.DtLocal = a.DtFactor * DtUnscaled;
+  uc.CellVolume = CellVolume;
+  uc.halfStep = a.halfStep;
+  uc.iStage = a.iStage;
+  uc.nStage = a.nStage;
+  uc.dbTrick = a.dbTrick;
+  uc.dtZero = (a.scal[S_DT] == 0.0);
+  uc.roundTripOld = a.roundTripOld;
+
+  double Q0[NVAR], Q1[NVAR], Q2[NVAR], Q3[NVAR];
+  double FzLo[NFS], FzHi[NFS];
+  double dtmin = 1.7976931348623157e308;
+
+  auto load_prim = [&](int kk, double *q) {
+    const long off = cell_off<NB>(a.nei, bb, i, j, kk);
+    load_state(U, vs, off, q);
+    to_primitive(q);
+  };
+  auto zface = [&](double *Fz) {
+    double F[NFLUX], vdt, unA;
+    err |= face_flux<2, FLUX, false>(ph, AreaZ, lim, beta, Q0, Q1, Q2, Q3, F,
+                                     vdt, unA);
+#pragma unroll
+    for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
+    Fz[PP] = vdt;
+  };
+
+  if (act) {
+    load_prim(-2, Q0);
+    load_prim(-1, Q1);
+    load_prim(0, Q2);
+    load_prim(1, Q3);
+    zface(FzLo);
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+  }
+
+  for (int k = 0; k < NB; k++) {
+    // z face k+1 of this column
+    if (act) {
+      load_prim(k + 2, Q3);
+      zface(FzHi);
+      // own primitive of plane k
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
+    }
+    // x/y halo of plane k
+    for (int h = tid; h < BPW * 8 * NB; h += kThreads) {
+      const int lbh = h / (8 * NB);
+      const int bh = blockIdx.x * BPW + lbh;
+      if (bh >= a.nBlock) continue;
+      const int r = h % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
+      int ii, jj;
+      if (side < 2) {
+        const int layer = rem & 1, pos = rem >> 1;
+        jj = pos;
+        ii = side == 0 ? layer - 2 : NB + layer;
+      } else {
+        const int layer = rem / NB, pos = rem % NB;
+        ii = pos;
+        jj = side == 2 ? layer - 2 : NB + layer;
+      }
+      double q[NVAR];
+      load_state(U, vs, cell_off<NB>(a.nei, bh, ii, jj, k), q);
+      to_primitive(q);
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) sP[sPi(lbh, v, jj + 2, ii + 2)] = q[v];
+    }
+    __syncthreads();
+    // x and y faces of plane k
+    for (int f = tid; f < BPW * 2 * NFX; f += kThreads) {
+      const int lbf = f / (2 * NFX);
+      const int bf = blockIdx.x * BPW + lbf;
+      if (bf >= a.nBlock) continue;
+      const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
+      const int row = q / (NB + 1), fp = q % (NB + 1);
+      double c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
+      double F[NFLUX], vdt, unA;
+      const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
+                   dzf = a.cellsize[3 * bf + 2];
+      if (dir == 0) {
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) {
+          c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
+          c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
+          c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
+          c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
+        }
+        err |= face_flux<0, FLUX, false>(ph, dyf * dzf, lim, beta, c0, c1, c2,
+                                         c3, F, vdt, unA);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) {
+          c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
+          c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
+          c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
+          c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
+        }
+        err |= face_flux<1, FLUX, false>(ph, dxf * dzf, lim, beta, c0, c1, c2,
+                                         c3, F, vdt, unA);
+      }
+      F[PP] = vdt;
+#pragma unroll
+      for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
+    }
+    __syncthreads();
+    if (act) {
+      const int qx0 = j * (NB + 1) + i, qy0 = i * (NB + 1) + j;
+      double dF[NFLUX];
+#pragma unroll
+      for (int v = 0; v < NFLUX; v++) {
+        if (v == PP) { dF[v] = 0.0; continue; }
+        double d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
+        d = d + sF[sFi(lb, 1, v, qy0)] - sF[sFi(lb, 1, v, qy0 + 1)];
+        d = d + FzLo[v] - FzHi[v];
+        dF[v] = d;
+      }
+      const long off = (((long)b * NB + k) * NB + j) * NB + i;
+      if (!a.noUpdate) {
+        double Uc[NVAR], Uo[NVAR], Un[NVAR];
+        load_state(U, vs, off, Uc);
+        load_state(a.Uold, vs, off, Uo);
+        update_cell(ph, uc, dF, Uc, Uo, Un);
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) a.Uout[v * vs + off] = Un[v];
+      }
+      if (a.doDt) {
+        const double Vdt =
+            fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
+            fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
+            fmax2(FzLo[PP], FzHi[PP]);
+        const double dtc = CellVolume / Vdt;
+        if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
+        dtmin = fmin2(dtmin, dtc);
+      }
+      // shift the z window
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+#pragma unroll
+      for (int v = 0; v < NFS; v++) FzLo[v] = FzHi[v];
+    }
+  }
+
+  if (a.doDt) {
+    // block minimum of the local time step (calc_timestep :236-256)
+    double m = wave_min(act ? dtmin : 1.7976931348623157e308);
+    const int w = tid / 64;
+    if ((tid & 63) == 0) sRed[w] = m;
+    __syncthreads();
+    if (act && col == 0) {
+      constexpr int WPB = NC >= 64 ? NC / 64 : 1;  // waves per block
+      double r = sRed[(lb * NC) / 64];
+      for (int q = 1; q < WPB; q++) r = fmin2(r, sRed[(lb * NC) / 64 + q]);
+      a.dtmaxB[b] = r;
+    }
+  }
+  if (err) atomicOr(a.err, err);
+  if (a.advanceTime && blockIdx.x == 0 && tid == 0)
+    a.scal[S_TSIM] = a.scal[S_TSIM] + a.scal[S_DT] / a.nStage;
+}
+
+// ---------------------------------------------------------------------
+// per-block kernels (reference-shaped routines for parity dumps)
+// ---------------------------------------------------------------------
+struct BlockArgs {
+  const double *Usnap;   // halo source (exchange_messages snapshot)
+  double *State;         // current state
+  double *StateOld;
+  const int *nei;
+  const double *cellsize;
+  double *scal;
+  double *LX, *RX, *LY, *RY, *LZ, *RZ;  // reference face layouts
+  double *FX, *FY, *FZ;                 // Flux_V?I(14, ...)
+  double *dtmaxCB, *dtmaxB;
+  int *err;
+  int nBlock, iBlock;
+  long vs;
+  Phys ph;
+  double beta;
+  int lim, iFlux;
+  double DtFactor;
+  int iStage, nStage, halfStep, dbTrick, useDtFixed;
+  double DtFixed;
+};
+
+template <int NB>
+__global__ void k_block_face_value(BlockArgs a) {
+  const int nFx = (NB + 1) * NB * NB;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nFx) return;
+  const int dir = t / nFx, r = t % nFx;
+  int fi, fj, fk;
+  double *L, *R;
+  long fo;
+  if (dir == 0) {  // LeftState_VX(nVar, nI+1, nJ, nK)
+    fi = r % (NB + 1); fj = (r / (NB + 1)) % NB; fk = r / ((NB + 1) * NB);
+    fo = r; L = a.LX; R = a.RX;
+  } else if (dir == 1) {  // (nVar, nI, nJ+1, nK)
+    fi = r % NB; fj = (r / NB) % (NB + 1); fk = r / (NB * (NB + 1));
+    fo = r; L = a.LY; R = a.RY;
+  } else {  // (nVar, nI, nJ, nK+1)
+    fi = r % NB; fj = (r / NB) % NB; fk = r / (NB * NB);
+    fo = r; L = a.LZ; R = a.RZ;
+  }
+  double q[4][NVAR];
+  for (int s = 0; s < 4; s++) {
+    int ii = fi, jj = fj, kk = fk;
+    if (dir == 0) ii = fi - 2 + s;
+    else if (dir == 1) jj = fj - 2 + s;
+    else kk = fk - 2 + s;
+    load_state(a.Usnap, a.vs, cell_off<NB>(a.nei, a.iBlock, ii, jj, kk), q[s]);
+    to_primitive(q[s]);
+  }
+  double Lv[NVAR], Rv[NVAR];
+  face_states(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
+  for (int v = 0; v < NVAR; v++) {
+    L[fo * NVAR + v] = Lv[v];
+    R[fo * NVAR + v] = Rv[v];
+  }
+}
+
+template <int DIR>
+__device__ int block_flux_dir(const BlockArgs &a, double area, const double *L,
+                              const double *R, double *F, double &vdt,
+                              double &unA) {
+  switch (a.iFlux) {
+  case FLUX_RUSANOV:
+    return numerical_flux<DIR, FLUX_RUSANOV, true>(a.ph, area, L, R, F, vdt, unA);
+  case FLUX_LINDE:
+    return numerical_flux<DIR, FLUX_LINDE, true>(a.ph, area, L, R, F, vdt, unA);
+  default:
+    return numerical_flux<DIR, FLUX_HLLD, true>(a.ph, area, L, R, F, vdt, unA);
+  }
+}
+
+template <int NB>
+__global__ void k_block_face_flux(BlockArgs a) {
+  const int nFx = (NB + 1) * NB * NB;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nFx) return;
+  const int dir = t / nFx, r = t % nFx;
+  const double *cs = a.cellsize + 3 * a.iBlock;
+  double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+  const double *Ls = dir == 0 ? a.LX : (dir == 1 ? a.LY : a.LZ);
+  const double *Rs = dir == 0 ? a.RX : (dir == 1 ? a.RY : a.RZ);
+  double *Fo = dir == 0 ? a.FX : (dir == 1 ? a.FY : a.FZ);
+  for (int v = 0; v < NVAR; v++) { L[v] = Ls[r * NVAR + v]; R[v] = Rs[r * NVAR + v]; }
+  int err;
+  if (dir == 0) err = block_flux_dir<0>(a, cs[1] * cs[2], L, R, F, vdt, unA);
+  else if (dir == 1) err = block_flux_dir<1>(a, cs[0] * cs[2], L, R, F, vdt, unA);
+  else err = block_flux_dir<2>(a, cs[0] * cs[1], L, R, F, vdt, unA);
+  double *o = Fo + (long)r * 14;
+  for (int v = 0; v < NFLUX; v++) o[v] = F[v];
+  o[9] = unA; o[10] = 0.0; o[11] = vdt; o[12] = 0.0; o[13] = 0.0;
+  if (err) atomicOr(a.err, err);
+}
+
+template <int NB>
+__global__ void k_block_update(BlockArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= NB * NB * NB) return;
+  const int i = c % NB, j = (c / NB) % NB, k = c / (NB * NB);
+  const long off = (((long)a.iBlock * NB + k) * NB + j) * NB + i;
+  double St[NVAR], So[NVAR], Un[NVAR];
+  load_state(a.State, a.vs, off, St);
+  if (a.iStage == 1) {  // StateOld = State (src/ModUpdateState.f90:161-163)
+    for (int v = 0; v < NVAR; v++) a.StateOld[v * a.vs + off] = St[v];
+  }
+  load_state(a.StateOld, a.vs, off, So);
+  const double *cs = a.cellsize + 3 * a.iBlock;
+  UpdateCtl u;
+  u.DtLocal = a.DtFactor * a.scal[S_DTCB];
+  u.CellVolume = cs[0] * cs[1] * cs[2];
+  u.halfStep = a.halfStep;
+  u.iStage = a.iStage;
+  u.nStage = a.nStage;
+  u.dbTrick = a.dbTrick;
+  u.dtZero = (a.scal[S_DT] == 0.0);
+  u.roundTripOld = 0;  // StateOld is round-tripped in memory below
+  const long fx0 = ((long)(k * NB + j) * (NB + 1) + i) * 14;
+  const long fy0 = ((long)(k * (NB + 1) + j) * NB + i) * 14;
+  const long fz0 = ((long)(k * NB + j) * NB + i) * 14;
+  double dF[NFLUX];
+  for (int v = 0; v < NFLUX; v++) {
+    double d = a.FX[fx0 + v] - a.FX[fx0 + 14 + v];
+    d = d + a.FY[fy0 + v] - a.FY[fy0 + (long)NB * 14 + v];
+    d = d + a.FZ[fz0 + v] - a.FZ[fz0 + (long)NB * NB * 14 + v];
+    dF[v] = d;
+  }
+  update_cell(a.ph, u, dF, St, So, Un);
+  for (int v = 0; v < NVAR; v++) a.State[v * a.vs + off] = Un[v];
+  if (!u.dtZero)  // p -> E -> p of StateOld (src/ModUpdateState.f90:606,965)
+    a.StateOld[PP * a.vs + off] =
+        energy_to_pressure(a.ph, So, pressure_to_energy(a.ph, So));
+}
+
+template <int NB>
+__global__ void __launch_bounds__(512) k_block_timestep(BlockArgs a) {
+  __shared__ double sRed[16];
+  const double *cs = a.cellsize + 3 * a.iBlock;
+  const double V = cs[0] * cs[1] * cs[2];
+  double m = 1.7976931348623157e308;
+  for (int c = threadIdx.x; c < NB * NB * NB; c += blockDim.x) {
+    const int i = c % NB, j = (c / NB) % NB, k = c / (NB * NB);
+    const long fx0 = ((long)(k * NB + j) * (NB + 1) + i) * 14;
+    const long fy0 = ((long)(k * (NB + 1) + j) * NB + i) * 14;
+    const long fz0 = ((long)(k * NB + j) * NB + i) * 14;
+    const double Vdt = fmax2(a.FX[fx0 + 11], a.FX[fx0 + 14 + 11]) +
+                       fmax2(a.FY[fy0 + 11], a.FY[fy0 + (long)NB * 14 + 11]) +
+                       fmax2(a.FZ[fz0 + 11], a.FZ[fz0 + (long)NB * NB * 14 + 11]);
+    const double dtc = V / Vdt;
+    const long off = (((long)a.iBlock * NB + k) * NB + j) * NB + i;
+    m = fmin2(m, dtc);
+    a.dtmaxCB[off] = a.useDtFixed ? a.DtFixed : dtc;
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sRed[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sRed[0];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) r = fmin2(r, sRed[w]);
+    a.dtmaxB[a.iBlock] = r;
+  }
+}
+
+// set_global_timestep (src/ModTimeStepControl.f90:306-524), one workgroup
+__global__ void __launch_bounds__(1024)
+k_global_dt(const double *dtmaxB, int nBlock, double *scal, double tLimit,
+            double Cfl, int useDtFixed, double DtFixed, int incStep) {
+  __shared__ double sRed[16];
+  double m = 1.7976931348623157e308;  // huge()
+  for (int b = threadIdx.x; b < nBlock; b += blockDim.x) m = fmin2(m, dtmaxB[b]);
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sRed[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double DtMin = sRed[0];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) DtMin = fmin2(DtMin, sRed[w]);
+    const double nStep = scal[S_NSTEP] + (incStep ? 1.0 : 0.0);
+    scal[S_NSTEP] = nStep;
+    double Dt;
+    if (useDtFixed)
+      Dt = DtFixed;
+    else if (nStep < 1.0 || (nStep == 1.0 && tLimit > 0.0))
+      Dt = 0.0;
+    else
+      Dt = DtMin;
+    if (tLimit > 0.0 && scal[S_TSIM] + Cfl * Dt > tLimit)
+      Dt = (tLimit - scal[S_TSIM]) / Cfl;
+    scal[S_DTMIN] = DtMin;
+    scal[S_DTCB] = Dt;
+    scal[S_DT] = Cfl * Dt;
+  }
+}
+
+__global__ void k_check_nan(const double *U, long n, int *err) {
+  int bad = 0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    if (!isfinite(U[t])) bad = 1;
+  if (bad) atomicOr(err, ERRB_NAN);
+}
+
+// per-block sums of U*V (rho, rho*u, B, total energy) in a fixed order
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_integrate(const double *U, long vs, const double *cellsize, Phys ph,
+            int nBlock, double *partial) {
+  __shared__ double sRed[256];
+  const int b = blockIdx.x;
+  const double V = cellsize[3 * b] * cellsize[3 * b + 1] * cellsize[3 * b + 2];
+  double acc[NVAR];
+  for (int v = 0; v < NVAR; v++) acc[v] = 0.0;
+  for (int c = threadIdx.x; c < NB * NB * NB; c += 256) {
+    double q[NVAR];
+    load_state(U, vs, (long)b * NB * NB * NB + c, q);
+    q[PP] = pressure_to_energy(ph, q);
+    for (int v = 0; v < NVAR; v++) acc[v] += q[v] * V;
+  }
+  for (int v = 0; v < NVAR; v++) {
+    sRed[threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sRed[threadIdx.x] += sRed[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)b * NVAR + v] = sRed[0];
+    __syncthreads();
+  }
+}
+
+} // namespace
+
+// =====================================================================
+// host side
+// =====================================================================
+struct bats_handle {
+  bats_config cfg;
+  int NB;
+  long nCellBlk, nCell;
+  hipStream_t stream;
+  double *U[2];
+  int cur;       // buffer holding the state at the start of a step
+  int stateBuf;  // buffer holding the current State_VGB (differs from cur
+                 // between the stages of a 2-stage step)
+  double *Usnap;
+  int *nei;
+  double *cellsize;
+  double *dtmaxB;
+  double *dtmaxCB;
+  double *scal;
+  int *err;
+  double *partial;
+  double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;
+  int iStageBlock;
+  std::vector<int> hNei;
+  // kernel timing
+  std::vector<hipEvent_t> evStart, evStop;
+  int nTimed;
+  double lastStageMs;
+  int lastStageLaunches;
+};
+
+static int hip_status(hipError_t e) {
+  return e == hipSuccess ? BATS_OK : BATS_EHIP;
+}
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      fprintf(stderr, "bats_hip: %s failed: %s (%s:%d)\n", #x,           \
+              hipGetErrorString(e_), __FILE__, __LINE__);                \
+      return e_ == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;        \
+    }                                                                    \
+  } while (0)
+
+static Phys make_phys(const bats_config &c) {
+  Phys p;
+  p.gamma = c.Gamma;
+  p.gm1 = c.Gamma - 1.0;
+  p.invgm1 = 1.0 / (c.Gamma - 1.0);
+  p.cTiny = c.cTiny;
+  p.pMin = c.pMin;
+  return p;
+}
+
+static int limiter_code(const bats_config &c) {
+  return c.nOrder == 1 ? LIM_FIRST : c.iLimiter;
+}
+
+static int err_from_bits(int bits) {
+  if (bits & ERRB_NEGFAST2) return BATS_ENEGFAST2;
+  if (bits & ERRB_HLLD) return BATS_EHLLD;
+  if (bits & ERRB_NAN) return BATS_ENAN;
+  return BATS_OK;
+}
+
+extern "C" const char *bats_strerror(int code) {
+  switch (code) {
+  case BATS_OK: return "ok";
+  case BATS_EINVAL: return "invalid argument or configuration";
+  case BATS_ENEGFAST2: return "negative fast speed squared";
+  case BATS_EHLLD: return "negative HLLD density ratio";
+  case BATS_ENAN: return "NaN or Inf in the state";
+  case BATS_EBC: return "unknown boundary condition";
+  case BATS_EHIP: return "HIP runtime error";
+  case BATS_ENOMEM: return "device allocation failed";
+  case BATS_ENODEV: return "no gfx950 device available";
+  case BATS_ELIMIT: return "request beyond a fixed buffer limit";
+  default: return "unknown status";
+  }
+}
+
+extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
+  if (!cfg || !out) return BATS_EINVAL;
+  *out = nullptr;
+  const bats_config &c = *cfg;
+  if (c.nI != c.nJ || c.nI != c.nK || (c.nI != 8 && c.nI != 16)) return BATS_EINVAL;
+  if (c.nDim != 3 || c.nG != 2 || c.nBlock <= 0) return BATS_EINVAL;
+  if (c.nOrder != 1 && c.nOrder != 2) return BATS_EINVAL;
+  if (c.nOrder == 2 && (c.iLimiter < BATS_LIM_MINMOD || c.iLimiter > BATS_LIM_NO))
+    return BATS_EINVAL;
+  if (c.iFlux < BATS_FLUX_RUSANOV || c.iFlux > BATS_FLUX_HLLD) return BATS_EINVAL;
+  if (c.nStage != 1 && c.nStage != 2) return BATS_EINVAL;
+  if (!(c.Gamma > 1.0) || !(c.Cfl > 0.0)) return BATS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.iDevice || c.iDevice < 0)
+    return BATS_ENODEV;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, c.iDevice));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    fprintf(stderr, "bats_hip: device %d is %s, this library is built for gfx950\n",
+            c.iDevice, prop.gcnArchName);
+    return BATS_ENODEV;
+  }
+  HIPCHK(hipSetDevice(c.iDevice));
+  bats_handle *h = new bats_handle();
+  h->cfg = c;
+  h->NB = c.nI;
+  h->nCellBlk = (long)c.nI * c.nJ * c.nK;
+  h->nCell = h->nCellBlk * c.nBlock;
+  h->cur = 0;
+  h->stateBuf = 0;
+  h->iStageBlock = 1;
+  h->nTimed = 0;
+  h->lastStageMs = 0.0;
+  h->lastStageLaunches = 0;
+  hipError_t e = hipStreamCreate(&h->stream);
+  const size_t sz = sizeof(double) * NVAR * h->nCell;
+  if (e == hipSuccess) e = hipMalloc(&h->U[0], sz);
+  if (e == hipSuccess) e = hipMalloc(&h->U[1], sz);
+  if (e == hipSuccess) e = hipMalloc(&h->nei, sizeof(int) * 6 * c.nBlock);
+  if (e == hipSuccess) e = hipMalloc(&h->cellsize, sizeof(double) * 3 * c.nBlock);
+  if (e == hipSuccess) e = hipMalloc(&h->dtmaxB, sizeof(double) * c.nBlock);
+  if (e == hipSuccess) e = hipMalloc(&h->scal, sizeof(double) * S_NSCAL);
+  if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * NVAR * c.nBlock);
+  if (e != hipSuccess) {
+    fprintf(stderr, "bats_hip: allocation failed: %s\n", hipGetErrorString(e));
+    bats_finalize(h);
+    return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
+  }
+  h->Usnap = nullptr;
+  h->dtmaxCB = nullptr;
+  h->LX = h->RX = h->LY = h->RY = h->LZ = h->RZ = nullptr;
+  h->FX = h->FY = h->FZ = nullptr;
+  double s0[S_NSCAL] = {0};
+  HIPCHK(hipMemcpy(h->scal, s0, sizeof s0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->err, 0, sizeof(int)));
+  HIPCHK(hipMemset(h->dtmaxB, 0, sizeof(double) * c.nBlock));
+  // default grid: all sides float until bats_set_grid
+  h->hNei.assign(6 * c.nBlock, -1);
+  HIPCHK(hipMemcpy(h->nei, h->hNei.data(), sizeof(int) * 6 * c.nBlock,
+                   hipMemcpyHostToDevice));
+  *out = h;
+  return BATS_OK;
+}
+
+extern "C" int bats_finalize(bats_handle *h) {
+  if (!h) return BATS_EINVAL;
+  (void)hipStreamSynchronize(h->stream);
+  void *ptrs[] = {h->U[0], h->U[1], h->Usnap, h->nei, h->cellsize, h->dtmaxB,
+                  h->dtmaxCB, h->scal, h->err, h->partial, h->LX, h->RX,
+                  h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto &ev : h->evStart) (void)hipEventDestroy(ev);
+  for (auto &ev : h->evStop) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return BATS_OK;
+}
+
+extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
+                             const double *CellSize_DB) {
+  if (!h || !iNeighbor_EB || !CellSize_DB) return BATS_EINVAL;
+  const int nB = h->cfg.nBlock;
+  for (int t = 0; t < 6 * nB; t++)
+    if (iNeighbor_EB[t] < -1 || iNeighbor_EB[t] >= nB) return BATS_EINVAL;
+  for (int t = 0; t < 3 * nB; t++)
+    if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
+  h->hNei.assign(iNeighbor_EB, iNeighbor_EB + 6 * nB);
+  HIPCHK(hipMemcpyAsync(h->nei, iNeighbor_EB, sizeof(int) * 6 * nB,
+                        hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->cellsize, CellSize_DB, sizeof(double) * 3 * nB,
+                        hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return BATS_OK;
+}
+
+// ---- layout conversions ---------------------------------------------
+static void aos_to_soa(const bats_config &c, const double *S, double *U) {
+  const int NB = c.nI, g = c.nG, NG = NB + 2 * g;
+  const long nB = c.nBlock, vs = nB * NB * NB * NB;
+  for (long b = 0; b < nB; b++)
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++) {
+          const double *src =
+              S + ((((b * NG + (k + g)) * NG + (j + g)) * NG) + (i + g)) * NVAR;
+          const long off = ((b * NB + k) * NB + j) * NB + i;
+          for (int v = 0; v < NVAR; v++) U[v * vs + off] = src[v];
+        }
+}
+
+static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
+                       const double *U, double *S) {
+  const int NB = c.nI, g = c.nG, NG = NB + 2 * g;
+  const long nB = c.nBlock, vs = nB * NB * NB * NB;
+  auto at = [&](long b, int i, int j, int k) {
+    return S + ((((b * NG + (k + g)) * NG + (j + g)) * NG) + (i + g)) * NVAR;
+  };
+  for (long b = 0; b < nB; b++)
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++) {
+          const long off = ((b * NB + k) * NB + j) * NB + i;
+          double *dst = at(b, i, j, k);
+          for (int v = 0; v < NVAR; v++) dst[v] = U[v * vs + off];
+        }
+  // face ghost slabs, as exchange_messages + float BCs would fill them
+  for (long b = 0; b < nB; b++)
+    for (int side = 0; side < 6; side++) {
+      const int n = nei[6 * b + side], dim = side / 2, hi = side % 2;
+      for (int layer = 1; layer <= g; layer++) {
+        const int dst = hi ? NB - 1 + layer : -layer;
+        const int src = n >= 0 ? (hi ? layer - 1 : NB - layer) : (hi ? NB - 1 : 0);
+        const long sb = n >= 0 ? n : b;
+        for (int p = 0; p < NB; p++)
+          for (int q = 0; q < NB; q++) {
+            int di, dj, dk, si, sj, sk;
+            if (dim == 0) { di = dst; si = src; dj = sj = q; dk = sk = p; }
+            else if (dim == 1) { dj = dst; sj = src; di = si = q; dk = sk = p; }
+            else { dk = dst; sk = src; di = si = q; dj = sj = p; }
+            const long off = ((sb * NB + sk) * NB + sj) * NB + si;
+            double *d = at(b, di, dj, dk);
+            for (int v = 0; v < NVAR; v++) d[v] = U[v * vs + off];
+          }
+      }
+    }
+}
+
+extern "C" int bats_upload_state(bats_handle *h, const double *State_VGB) {
+  if (!h || !State_VGB) return BATS_EINVAL;
+  std::vector<double> tmp((size_t)NVAR * h->nCell);
+  aos_to_soa(h->cfg, State_VGB, tmp.data());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(h->U[h->cur], tmp.data(), sizeof(double) * tmp.size(),
+                   hipMemcpyHostToDevice));
+  h->stateBuf = h->cur;
+  return BATS_OK;
+}
+
+extern "C" int bats_download_state(bats_handle *h, double *State_VGB) {
+  if (!h || !State_VGB) return BATS_EINVAL;
+  std::vector<double> tmp((size_t)NVAR * h->nCell);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(tmp.data(), h->U[h->stateBuf], sizeof(double) * tmp.size(),
+                   hipMemcpyDeviceToHost));
+  soa_to_aos(h->cfg, h->hNei, tmp.data(), State_VGB);
+  return BATS_OK;
+}
+
+extern "C" int bats_upload_state_soa(bats_handle *h, const double *U) {
+  if (!h || !U) return BATS_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(h->U[h->cur], U, sizeof(double) * NVAR * h->nCell,
+                   hipMemcpyHostToDevice));
+  h->stateBuf = h->cur;
+  return BATS_OK;
+}
+
+extern "C" int bats_download_state_soa(bats_handle *h, double *U) {
+  if (!h || !U) return BATS_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(U, h->U[h->stateBuf], sizeof(double) * NVAR * h->nCell,
+                   hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+// ---- per-block routines ---------------------------------------------
+static int ensure_block_buffers(bats_handle *h) {
+  if (h->Usnap) return BATS_OK;
+  const int NB = h->NB;
+  const size_t nF = (size_t)(NB + 1) * NB * NB;
+  HIPCHK(hipMalloc(&h->Usnap, sizeof(double) * NVAR * h->nCell));
+  HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
+                        sizeof(double) * NVAR * h->nCell,
+                        hipMemcpyDeviceToDevice, h->stream));
+  double **fs[] = {&h->LX, &h->RX, &h->LY, &h->RY, &h->LZ, &h->RZ};
+  for (double **p : fs) {
+    HIPCHK(hipMalloc(p, sizeof(double) * NVAR * nF));
+    HIPCHK(hipMemsetAsync(*p, 0, sizeof(double) * NVAR * nF, h->stream));
+  }
+  double **fl[] = {&h->FX, &h->FY, &h->FZ};
+  for (double **p : fl) {
+    HIPCHK(hipMalloc(p, sizeof(double) * 14 * nF));
+    HIPCHK(hipMemsetAsync(*p, 0, sizeof(double) * 14 * nF, h->stream));
+  }
+  if (!h->dtmaxCB) {
+    HIPCHK(hipMalloc(&h->dtmaxCB, sizeof(double) * h->nCell));
+    HIPCHK(hipMemsetAsync(h->dtmaxCB, 0, sizeof(double) * h->nCell, h->stream));
+  }
+  return BATS_OK;
+}
+
+static BlockArgs block_args(bats_handle *h, int iBlock) {
+  BlockArgs a;
+  const bats_config &c = h->cfg;
+  a.Usnap = h->Usnap;
+  a.State = h->U[h->stateBuf];
+  a.StateOld = h->U[1 - h->stateBuf];
+  a.nei = h->nei;
+  a.cellsize = h->cellsize;
+  a.scal = h->scal;
+  a.LX = h->LX; a.RX = h->RX; a.LY = h->LY; a.RY = h->RY; a.LZ = h->LZ; a.RZ = h->RZ;
+  a.FX = h->FX; a.FY = h->FY; a.FZ = h->FZ;
+  a.dtmaxCB = h->dtmaxCB;
+  a.dtmaxB = h->dtmaxB;
+  a.err = h->err;
+  a.nBlock = c.nBlock;
+  a.iBlock = iBlock;
+  a.vs = h->nCell;
+  a.ph = make_phys(c);
+  a.beta = c.LimiterBeta;
+  a.lim = limiter_code(c);
+  a.iFlux = c.iFlux;
+  const int halfStep = c.UseHalfStep || c.nStage == 1;
+  a.DtFactor = c.UseHalfStep ? (c.Cfl * h->iStageBlock) / c.nStage : c.Cfl;
+  a.iStage = h->iStageBlock;
+  a.nStage = c.nStage;
+  a.halfStep = halfStep;
+  a.dbTrick = c.UseDbTrick;
+  a.useDtFixed = c.UseDtFixed;
+  a.DtFixed = c.DtFixed;
+  return a;
+}
+
+extern "C" int bats_exchange_messages(bats_handle *h) {
+  if (!h) return BATS_EINVAL;
+  int st = ensure_block_buffers(h);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
+                        sizeof(double) * NVAR * h->nCell,
+                        hipMemcpyDeviceToDevice, h->stream));
+  return BATS_OK;
+}
+
+extern "C" int bats_set_stage(bats_handle *h, int iStage) {
+  if (!h || iStage < 1 || iStage > h->cfg.nStage) return BATS_EINVAL;
+  h->iStageBlock = iStage;
+  return BATS_OK;
+}
+
+static int check_block(bats_handle *h, int iBlock1) {
+  if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
+  return ensure_block_buffers(h);
+}
+
+extern "C" int bats_calc_face_value(bats_handle *h, int iBlock,
+                                    int DoResChangeOnly, int DoMonotoneRestrict) {
+  (void)DoMonotoneRestrict;  // no resolution changes on a same-level grid
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  if (DoResChangeOnly) return BATS_OK;  // nothing to do without res changes
+  BlockArgs a = block_args(h, iBlock - 1);
+  const int n = 3 * (h->NB + 1) * h->NB * h->NB;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_face_value<8>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_face_value<16>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock) {
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  if (DoResChangeOnly) return BATS_OK;
+  BlockArgs a = block_args(h, iBlock - 1);
+  const int n = 3 * (h->NB + 1) * h->NB * h->NB;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_face_flux<8>, dim3((n + 127) / 128), dim3(128), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_face_flux<16>, dim3((n + 127) / 128), dim3(128), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_update_state(bats_handle *h, int iBlock) {
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  BlockArgs a = block_args(h, iBlock - 1);
+  const int n = h->NB * h->NB * h->NB;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_update<8>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_update<16>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_calc_timestep(bats_handle *h, int iBlock, int IsPartLocal) {
+  if (IsPartLocal) return BATS_EINVAL;  // local time stepping is not supported
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  BlockArgs a = block_args(h, iBlock - 1);
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_timestep<8>, dim3(1), dim3(512), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_timestep<16>, dim3(1), dim3(512), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_get_face_values(bats_handle *h, double *LeftX, double *RightX,
+                                    double *LeftY, double *RightY, double *LeftZ,
+                                    double *RightZ) {
+  if (!h || !h->LX) return BATS_EINVAL;
+  const size_t n = sizeof(double) * NVAR * (h->NB + 1) * h->NB * h->NB;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double *dst[] = {LeftX, RightX, LeftY, RightY, LeftZ, RightZ};
+  double *src[] = {h->LX, h->RX, h->LY, h->RY, h->LZ, h->RZ};
+  for (int t = 0; t < 6; t++)
+    if (dst[t]) HIPCHK(hipMemcpy(dst[t], src[t], n, hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+extern "C" int bats_get_face_fluxes(bats_handle *h, double *FluxX, double *FluxY,
+                                    double *FluxZ) {
+  if (!h || !h->FX) return BATS_EINVAL;
+  const size_t n = sizeof(double) * 14 * (h->NB + 1) * h->NB * h->NB;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double *dst[] = {FluxX, FluxY, FluxZ};
+  double *src[] = {h->FX, h->FY, h->FZ};
+  for (int t = 0; t < 3; t++)
+    if (dst[t]) HIPCHK(hipMemcpy(dst[t], src[t], n, hipMemcpyDeviceToHost));
+  int bits = 0;
+  HIPCHK(hipMemcpy(&bits, h->err, sizeof(int), hipMemcpyDeviceToHost));
+  return err_from_bits(bits);
+}
+
+extern "C" int bats_get_dtmax(bats_handle *h, double *DtMax_CB, double *DtMax_B) {
+  if (!h) return BATS_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (DtMax_CB) {
+    if (!h->dtmaxCB) return BATS_EINVAL;
+    // device order (b, k, j, i) equals DtMax_CB(nI,nJ,nK,nBlock)
+    HIPCHK(hipMemcpy(DtMax_CB, h->dtmaxCB, sizeof(double) * h->nCell,
+                     hipMemcpyDeviceToHost));
+  }
+  if (DtMax_B)
+    HIPCHK(hipMemcpy(DtMax_B, h->dtmaxB, sizeof(double) * h->cfg.nBlock,
+                     hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+// ---- production path ------------------------------------------------
+extern "C" int bats_set_global_timestep(bats_handle *h, double TimeSimulationLimit,
+                                        double *DtOut) {
+  if (!h) return BATS_EINVAL;
+  const bats_config &c = h->cfg;
+  hipLaunchKernelGGL(k_global_dt, dim3(1), dim3(1024), 0, h->stream, h->dtmaxB,
+                     c.nBlock, h->scal, TimeSimulationLimit, c.Cfl, c.UseDtFixed,
+                     c.DtFixed, 1);
+  HIPCHK(hipGetLastError());
+  if (DtOut) {
+    double s[S_NSCAL];
+    HIPCHK(hipMemcpyAsync(s, h->scal, sizeof s, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *DtOut = s[S_DT];
+  }
+  return BATS_OK;
+}
+
+static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
+                        int advanceTime, bool timed) {
+  const bats_config &c = h->cfg;
+  StageArgs a;
+  const int last = (iStage == c.nStage);
+  if (c.nStage == 1) {
+    a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
+  } else if (iStage == 1) {
+    a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
+  } else {
+    a.Uin = h->U[1 - h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[h->cur];
+  }
+  a.nei = h->nei;
+  a.cellsize = h->cellsize;
+  a.scal = h->scal;
+  a.dtmaxB = h->dtmaxB;
+  a.dtmaxCB = h->dtmaxCB;
+  a.err = h->err;
+  a.nBlock = c.nBlock;
+  a.vs = h->nCell;
+  a.ph = make_phys(c);
+  a.beta = c.LimiterBeta;
+  a.lim = limiter_code(c);
+  a.DtFactor = c.UseHalfStep ? (c.Cfl * iStage) / c.nStage : c.Cfl;
+  a.iStage = iStage;
+  a.nStage = c.nStage;
+  a.halfStep = c.UseHalfStep || c.nStage == 1;
+  a.dbTrick = c.UseDbTrick;
+  a.doDt = doDt;
+  a.roundTripOld = 1;
+  a.noUpdate = noUpdate;
+  a.advanceTime = advanceTime;
+  const int NB = h->NB;
+  const int bpw = kThreads / (NB * NB);
+  const dim3 grid((c.nBlock + bpw - 1) / bpw);
+  int ev = -1;
+  if (timed) {
+    ev = h->nTimed++;
+    while ((int)h->evStart.size() <= ev) {
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      h->evStart.push_back(e0);
+      h->evStop.push_back(e1);
+    }
+    HIPCHK(hipEventRecord(h->evStart[ev], h->stream));
+  }
+#define LAUNCH(NBv, FL)                                                    \
+  hipLaunchKernelGGL((k_stage<NBv, FL>), grid, dim3(kThreads), 0, h->stream, a)
+  if (NB == 8) {
+    if (c.iFlux == BATS_FLUX_RUSANOV) LAUNCH(8, FLUX_RUSANOV);
+    else if (c.iFlux == BATS_FLUX_LINDE) LAUNCH(8, FLUX_LINDE);
+    else LAUNCH(8, FLUX_HLLD);
+  } else {
+    if (c.iFlux == BATS_FLUX_RUSANOV) LAUNCH(16, FLUX_RUSANOV);
+    else if (c.iFlux == BATS_FLUX_LINDE) LAUNCH(16, FLUX_LINDE);
+    else LAUNCH(16, FLUX_HLLD);
+  }
+#undef LAUNCH
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));
+  if (!noUpdate) {
+    if (c.nStage == 1) {
+      h->cur = 1 - h->cur;
+      h->stateBuf = h->cur;
+    } else {
+      h->stateBuf = last ? h->cur : 1 - h->cur;
+    }
+  }
+  return BATS_OK;
+}
+
+extern "C" int bats_advance_stage(bats_handle *h, int iStage, int DoCalcTimestep) {
+  if (!h || iStage < 1 || iStage > h->cfg.nStage) return BATS_EINVAL;
+  const int doDt = DoCalcTimestep && iStage == h->cfg.nStage;
+  return launch_stage(h, iStage, doDt, 0, 1, false);
+}
+
+extern "C" int bats_advance_explicit(bats_handle *h, int DoCalcTimestep) {
+  if (!h) return BATS_EINVAL;
+  for (int s = 1; s <= h->cfg.nStage; s++) {
+    int st = launch_stage(h, s, DoCalcTimestep && s == h->cfg.nStage, 0, 1, false);
+    if (st) return st;
+  }
+  return BATS_OK;
+}
+
+extern "C" int bats_advance(bats_handle *h, int nStep, double TimeSimulationLimit) {
+  if (!h || nStep < 0) return BATS_EINVAL;
+  h->nTimed = 0;
+  for (int n = 0; n < nStep; n++) {
+    int st = bats_set_global_timestep(h, TimeSimulationLimit, nullptr);
+    if (st) return st;
+    for (int s = 1; s <= h->cfg.nStage; s++) {
+      st = launch_stage(h, s, s == h->cfg.nStage, 0, 1, true);
+      if (st) return st;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double tot = 0.0;
+  for (int e = 0; e < h->nTimed; e++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->evStart[e], h->evStop[e]));
+    tot += ms;
+  }
+  h->lastStageLaunches = h->nTimed;
+  h->lastStageMs = h->nTimed ? tot / h->nTimed : 0.0;
+  int bits = 0;
+  HIPCHK(hipMemcpy(&bits, h->err, sizeof(int), hipMemcpyDeviceToHost));
+  return err_from_bits(bits);
+}
+
+extern "C" int bats_init_timestep(bats_handle *h) {
+  if (!h) return BATS_EINVAL;
+  return launch_stage(h, h->cfg.nStage, 1, 1, 0, false);
+}
+
+extern "C" int bats_get_time(bats_handle *h, double *Dt, double *tSimulation,
+                             int *nStep) {
+  if (!h) return BATS_EINVAL;
+  double s[S_NSCAL];
+  HIPCHK(hipMemcpyAsync(s, h->scal, sizeof s, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (Dt) *Dt = s[S_DT];
+  if (tSimulation) *tSimulation = s[S_TSIM];
+  if (nStep) *nStep = (int)s[S_NSTEP];
+  return BATS_OK;
+}
+
+extern "C" int bats_synchronize(bats_handle *h) {
+  if (!h) return BATS_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  int bits = 0;
+  HIPCHK(hipMemcpy(&bits, h->err, sizeof(int), hipMemcpyDeviceToHost));
+  return err_from_bits(bits);
+}
+
+extern "C" int bats_check_nan(bats_handle *h) {
+  if (!h) return BATS_EINVAL;
+  const long n = NVAR * h->nCell;
+  hipLaunchKernelGGL(k_check_nan, dim3(1024), dim3(256), 0, h->stream,
+                     h->U[h->stateBuf], n, h->err);
+  HIPCHK(hipGetLastError());
+  return bats_synchronize(h);
+}
+
+extern "C" int bats_integrate_domain(bats_handle *h, double *Sum_V) {
+  if (!h || !Sum_V) return BATS_EINVAL;
+  const bats_config &c = h->cfg;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_integrate<8>, dim3(c.nBlock), dim3(256), 0, h->stream,
+                       h->U[h->stateBuf], h->nCell, h->cellsize, make_phys(c),
+                       c.nBlock, h->partial);
+  else
+    hipLaunchKernelGGL(k_integrate<16>, dim3(c.nBlock), dim3(256), 0, h->stream,
+                       h->U[h->stateBuf], h->nCell, h->cellsize, make_phys(c),
+                       c.nBlock, h->partial);
+  HIPCHK(hipGetLastError());
+  std::vector<double> p((size_t)NVAR * c.nBlock);
+  HIPCHK(hipMemcpyAsync(p.data(), h->partial, sizeof(double) * p.size(),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int v = 0; v < NVAR; v++) Sum_V[v] = 0.0;
+  for (int b = 0; b < c.nBlock; b++)
+    for (int v = 0; v < NVAR; v++) Sum_V[v] += p[(size_t)b * NVAR + v];
+  return BATS_OK;
+}
+
+extern "C" int bats_last_stage_kernel_ms(bats_handle *h, double *ms, int *nLaunch) {
+  if (!h) return BATS_EINVAL;
+  if (ms) *ms = h->lastStageMs;
+  if (nLaunch) *nLaunch = h->lastStageLaunches;
+  return BATS_OK;
+}

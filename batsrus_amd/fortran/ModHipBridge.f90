@@ -1,0 +1,367 @@
+!  ModHipBridge -- Fortran host binding of the gfx950 stage-update library
+!  (libbats_hip.so, C ABI in include/bats_hip.h).
+!
+!  The reference's hot path is a set of module procedures
+!    calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
+!                                          src/ModFaceValue.f90:854
+!    calc_face_flux(DoResChangeOnly, iBlock) src/ModFaceFlux.f90:483
+!    update_state(iBlock)                  src/ModUpdateState.f90:83
+!    calc_timestep(iBlock, IsPartLocal)    src/ModTimeStepControl.f90:111
+!    set_global_timestep(TimeSimulationLimit)
+!                                          src/ModTimeStepControl.f90:306
+!    advance_explicit(DoCalcTimestep)      src/ModAdvanceExplicit.f90:18
+!    exchange_messages                     src/ModMessagePass.f90:27
+!  working on module-global arrays.  This module keeps those names and
+!  argument lists; a driver shaped like advance_explicit can use it in place
+!  of the reference modules.  The device state lives behind one module-level
+!  handle (the analogue of the reference's module globals); State_VGB is
+!  exchanged only at sync points (sync_cpu_gpu,
+!  src/ModUpdateStateFast.f90:71-188).  A non-zero status from the library
+!  stops the program with the library's message, as stop_mpi would.
+module ModHipBridge
+
+  use iso_c_binding
+
+  implicit none
+
+  private
+
+  public:: bats_config
+  public:: BATS_LIM_MINMOD, BATS_LIM_MC, BATS_LIM_MC3, BATS_LIM_BETA, &
+       BATS_LIM_NO, BATS_FLUX_RUSANOV, BATS_FLUX_LINDE, BATS_FLUX_HLLD
+  ! raw C entry points
+  public:: bats_init, bats_finalize, bats_set_grid, bats_upload_state, &
+       bats_download_state, bats_upload_state_soa, bats_download_state_soa, &
+       bats_exchange_messages, bats_set_stage, bats_calc_face_value, &
+       bats_calc_face_flux, bats_update_state, bats_calc_timestep, &
+       bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
+       bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
+       bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
+       bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms
+  ! reference-named routines on the module-level handle
+  public:: init_hip_bridge, clean_hip_bridge, sync_state_to_gpu, &
+       sync_state_to_cpu, calc_face_value, calc_face_flux, update_state, &
+       calc_timestep, set_global_timestep, advance_explicit, &
+       exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
+       init_timestep, get_time
+
+  integer(c_int), parameter:: BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, &
+       BATS_LIM_MC3 = 3, BATS_LIM_BETA = 4, BATS_LIM_NO = 5
+  integer(c_int), parameter:: BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, &
+       BATS_FLUX_HLLD = 3
+
+  ! mirrors struct bats_config of include/bats_hip.h
+  type, bind(C):: bats_config
+     integer(c_int):: nI, nJ, nK, nG, nDim, nBlock, nOrder, iLimiter
+     real(c_double):: LimiterBeta
+     integer(c_int):: iFlux, nStage, UseHalfStep
+     real(c_double):: Cfl, Gamma, pMin
+     integer(c_int):: UseDbTrick, UseDtFixed
+     real(c_double):: DtFixed, cTiny
+     integer(c_int):: iDevice
+  end type bats_config
+
+  interface
+     integer(c_int) function bats_init(Config, Handle) &
+          bind(C, name="bats_init")
+       import:: c_int, c_ptr, bats_config
+       type(bats_config), intent(in):: Config
+       type(c_ptr), intent(out):: Handle
+     end function bats_init
+     integer(c_int) function bats_finalize(Handle) &
+          bind(C, name="bats_finalize")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_finalize
+     integer(c_int) function bats_set_grid(Handle, iNeighbor_EB, CellSize_DB) &
+          bind(C, name="bats_set_grid")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), intent(in):: iNeighbor_EB(*)
+       real(c_double), intent(in):: CellSize_DB(*)
+     end function bats_set_grid
+     integer(c_int) function bats_upload_state(Handle, State_VGB) &
+          bind(C, name="bats_upload_state")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(in):: State_VGB(*)
+     end function bats_upload_state
+     integer(c_int) function bats_download_state(Handle, State_VGB) &
+          bind(C, name="bats_download_state")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(inout):: State_VGB(*)
+     end function bats_download_state
+     integer(c_int) function bats_upload_state_soa(Handle, U) &
+          bind(C, name="bats_upload_state_soa")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(in):: U(*)
+     end function bats_upload_state_soa
+     integer(c_int) function bats_download_state_soa(Handle, U) &
+          bind(C, name="bats_download_state_soa")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(inout):: U(*)
+     end function bats_download_state_soa
+     integer(c_int) function bats_exchange_messages(Handle) &
+          bind(C, name="bats_exchange_messages")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_exchange_messages
+     integer(c_int) function bats_set_stage(Handle, iStage) &
+          bind(C, name="bats_set_stage")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iStage
+     end function bats_set_stage
+     integer(c_int) function bats_calc_face_value(Handle, iBlock, &
+          DoResChangeOnly, DoMonotoneRestrict) &
+          bind(C, name="bats_calc_face_value")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock, DoResChangeOnly, DoMonotoneRestrict
+     end function bats_calc_face_value
+     integer(c_int) function bats_calc_face_flux(Handle, DoResChangeOnly, &
+          iBlock) bind(C, name="bats_calc_face_flux")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: DoResChangeOnly, iBlock
+     end function bats_calc_face_flux
+     integer(c_int) function bats_update_state(Handle, iBlock) &
+          bind(C, name="bats_update_state")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock
+     end function bats_update_state
+     integer(c_int) function bats_calc_timestep(Handle, iBlock, IsPartLocal) &
+          bind(C, name="bats_calc_timestep")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock, IsPartLocal
+     end function bats_calc_timestep
+     integer(c_int) function bats_get_face_values(Handle, LeftX, RightX, &
+          LeftY, RightY, LeftZ, RightZ) bind(C, name="bats_get_face_values")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(inout):: LeftX(*), RightX(*), LeftY(*), &
+            RightY(*), LeftZ(*), RightZ(*)
+     end function bats_get_face_values
+     integer(c_int) function bats_get_face_fluxes(Handle, FluxX, FluxY, &
+          FluxZ) bind(C, name="bats_get_face_fluxes")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(inout):: FluxX(*), FluxY(*), FluxZ(*)
+     end function bats_get_face_fluxes
+     integer(c_int) function bats_get_dtmax(Handle, DtMax_CB, DtMax_B) &
+          bind(C, name="bats_get_dtmax")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(inout):: DtMax_CB(*), DtMax_B(*)
+     end function bats_get_dtmax
+     integer(c_int) function bats_set_global_timestep(Handle, &
+          TimeSimulationLimit, DtOut) bind(C, name="bats_set_global_timestep")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), value:: TimeSimulationLimit
+       real(c_double), intent(out):: DtOut
+     end function bats_set_global_timestep
+     integer(c_int) function bats_advance_explicit(Handle, DoCalcTimestep) &
+          bind(C, name="bats_advance_explicit")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: DoCalcTimestep
+     end function bats_advance_explicit
+     integer(c_int) function bats_advance_stage(Handle, iStage, &
+          DoCalcTimestep) bind(C, name="bats_advance_stage")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iStage, DoCalcTimestep
+     end function bats_advance_stage
+     integer(c_int) function bats_advance(Handle, nStep, TimeSimulationLimit) &
+          bind(C, name="bats_advance")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: nStep
+       real(c_double), value:: TimeSimulationLimit
+     end function bats_advance
+     integer(c_int) function bats_init_timestep(Handle) &
+          bind(C, name="bats_init_timestep")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_init_timestep
+     integer(c_int) function bats_get_time(Handle, Dt, tSimulation, nStep) &
+          bind(C, name="bats_get_time")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(out):: Dt, tSimulation
+       integer(c_int), intent(out):: nStep
+     end function bats_get_time
+     integer(c_int) function bats_synchronize(Handle) &
+          bind(C, name="bats_synchronize")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_synchronize
+     integer(c_int) function bats_check_nan(Handle) &
+          bind(C, name="bats_check_nan")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_check_nan
+     integer(c_int) function bats_integrate_domain(Handle, Sum_V) &
+          bind(C, name="bats_integrate_domain")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(out):: Sum_V(8)
+     end function bats_integrate_domain
+     integer(c_int) function bats_last_stage_kernel_ms(Handle, Ms, nLaunch) &
+          bind(C, name="bats_last_stage_kernel_ms")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(out):: Ms
+       integer(c_int), intent(out):: nLaunch
+     end function bats_last_stage_kernel_ms
+     type(c_ptr) function bats_strerror_c(Code) bind(C, name="bats_strerror")
+       import:: c_int, c_ptr
+       integer(c_int), value:: Code
+     end function bats_strerror_c
+  end interface
+
+  ! the module-level handle, the analogue of the reference's module arrays
+  type(c_ptr), save:: hBats = c_null_ptr
+
+contains
+  !============================================================================
+  subroutine bats_check(iError, NameWhere)
+    ! Stop with the library's message on a non-zero status (stop_mpi)
+    integer(c_int), intent(in):: iError
+    character(len=*), intent(in):: NameWhere
+    character(len=200):: String
+    character(kind=c_char), pointer:: Msg_I(:)
+    integer:: i
+    !--------------------------------------------------------------------------
+    if(iError == 0) RETURN
+    call c_f_pointer(bats_strerror_c(iError), Msg_I, [200])
+    String = ' '
+    do i = 1, 200
+       if(Msg_I(i) == c_null_char) EXIT
+       String(i:i) = Msg_I(i)
+    end do
+    write(*,'(a,i3,2a)') 'ERROR in '//NameWhere//': status ', iError, &
+         ': ', trim(String)
+    error stop 'stop_mpi'
+  end subroutine bats_check
+  !============================================================================
+  subroutine init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
+    type(bats_config), intent(in):: Config
+    integer(c_int), intent(in):: iNeighbor_EB(6,Config%nBlock)
+    real(c_double), intent(in):: CellSize_DB(3,Config%nBlock)
+    !--------------------------------------------------------------------------
+    call bats_check(bats_init(Config, hBats), 'init_hip_bridge')
+    call bats_check(bats_set_grid(hBats, iNeighbor_EB, CellSize_DB), &
+         'init_hip_bridge')
+  end subroutine init_hip_bridge
+  !============================================================================
+  subroutine clean_hip_bridge
+    !--------------------------------------------------------------------------
+    if(c_associated(hBats)) call bats_check(bats_finalize(hBats), &
+         'clean_hip_bridge')
+    hBats = c_null_ptr
+  end subroutine clean_hip_bridge
+  !============================================================================
+  subroutine sync_state_to_gpu(State_VGB)
+    real(c_double), intent(in):: State_VGB(:,:,:,:,:)
+    !--------------------------------------------------------------------------
+    call bats_check(bats_upload_state(hBats, State_VGB), 'sync_state_to_gpu')
+  end subroutine sync_state_to_gpu
+  !============================================================================
+  subroutine sync_state_to_cpu(State_VGB)
+    real(c_double), intent(inout):: State_VGB(:,:,:,:,:)
+    !--------------------------------------------------------------------------
+    call bats_check(bats_download_state(hBats, State_VGB), &
+         'sync_state_to_cpu')
+  end subroutine sync_state_to_cpu
+  !============================================================================
+  subroutine exchange_messages
+    !--------------------------------------------------------------------------
+    call bats_check(bats_exchange_messages(hBats), 'exchange_messages')
+  end subroutine exchange_messages
+  !============================================================================
+  subroutine set_stage(iStage)
+    integer, intent(in):: iStage
+    !--------------------------------------------------------------------------
+    call bats_check(bats_set_stage(hBats, iStage), 'set_stage')
+  end subroutine set_stage
+  !============================================================================
+  subroutine calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
+    integer, intent(in):: iBlock
+    logical, intent(in):: DoResChangeOnly, DoMonotoneRestrict
+    !--------------------------------------------------------------------------
+    call bats_check(bats_calc_face_value(hBats, iBlock, &
+         merge(1, 0, DoResChangeOnly), merge(1, 0, DoMonotoneRestrict)), &
+         'calc_face_value')
+  end subroutine calc_face_value
+  !============================================================================
+  subroutine calc_face_flux(DoResChangeOnly, iBlock)
+    logical, intent(in):: DoResChangeOnly
+    integer, intent(in):: iBlock
+    !--------------------------------------------------------------------------
+    call bats_check(bats_calc_face_flux(hBats, merge(1, 0, DoResChangeOnly), &
+         iBlock), 'calc_face_flux')
+  end subroutine calc_face_flux
+  !============================================================================
+  subroutine update_state(iBlock)
+    integer, intent(in):: iBlock
+    !--------------------------------------------------------------------------
+    call bats_check(bats_update_state(hBats, iBlock), 'update_state')
+  end subroutine update_state
+  !============================================================================
+  subroutine calc_timestep(iBlock, IsPartLocal)
+    integer, intent(in):: iBlock
+    logical, optional, intent(in):: IsPartLocal
+    integer(c_int):: iPart
+    !--------------------------------------------------------------------------
+    iPart = 0
+    if(present(IsPartLocal)) iPart = merge(1, 0, IsPartLocal)
+    call bats_check(bats_calc_timestep(hBats, iBlock, iPart), 'calc_timestep')
+  end subroutine calc_timestep
+  !============================================================================
+  subroutine set_global_timestep(TimeSimulationLimit, Dt)
+    real(c_double), intent(in):: TimeSimulationLimit
+    real(c_double), intent(out):: Dt
+    !--------------------------------------------------------------------------
+    call bats_check(bats_set_global_timestep(hBats, TimeSimulationLimit, Dt), &
+         'set_global_timestep')
+  end subroutine set_global_timestep
+  !============================================================================
+  subroutine advance_explicit(DoCalcTimestep)
+    logical, intent(in):: DoCalcTimestep
+    !--------------------------------------------------------------------------
+    call bats_check(bats_advance_explicit(hBats, merge(1, 0, DoCalcTimestep)),&
+         'advance_explicit')
+  end subroutine advance_explicit
+  !============================================================================
+  subroutine check_nan
+    !--------------------------------------------------------------------------
+    call bats_check(bats_check_nan(hBats), 'check_nan')
+  end subroutine check_nan
+  !============================================================================
+  subroutine init_timestep
+    ! time step limit from the current state without an update
+    !--------------------------------------------------------------------------
+    call bats_check(bats_init_timestep(hBats), 'init_timestep')
+  end subroutine init_timestep
+  !============================================================================
+  subroutine get_time(Dt, tSimulation, nStep)
+    real(c_double), intent(out):: Dt, tSimulation
+    integer(c_int), intent(out):: nStep
+    !--------------------------------------------------------------------------
+    call bats_check(bats_get_time(hBats, Dt, tSimulation, nStep), 'get_time')
+  end subroutine get_time
+  !============================================================================
+  type(c_ptr) function bridge_handle()
+    ! the module-level handle, for callers of the raw bats_* entry points
+    bridge_handle = hBats
+  end function bridge_handle
+  !============================================================================
+end module ModHipBridge
+!==============================================================================

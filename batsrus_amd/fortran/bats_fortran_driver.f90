@@ -1,0 +1,110 @@
+!  bats_fortran_driver -- a Fortran host that runs the explicit MHD update on
+!  the GPU through ModHipBridge, with the time loop shaped like the
+!  reference's BATS_advance / advance_explicit
+!  (src/ModBatsrusMethods.f90:566-653, src/ModAdvanceExplicit.f90:74-312).
+!
+!  Usage: bats_fortran_driver <run directory>
+!  The run directory holds
+!    PARAM.nml      namelist &RUN (grid shape, scheme, nStep, TypeLoop)
+!    grid.bin       iNeighbor_EB(6,nBlock) int32, CellSize_DB(3,nBlock) real8
+!    state_in.bin   State_VGB(8,-1:nI+2,-1:nJ+2,-1:nK+2,nBlock) real8
+!  and receives state_out.bin (same layout) and log.txt.
+!  TypeLoop='blocks' calls the per-block routines in the reference's order;
+!  TypeLoop='fused' calls set_global_timestep + advance_explicit.
+program bats_fortran_driver
+
+  use iso_c_binding
+  use ModHipBridge
+
+  implicit none
+
+  integer:: nRootI = 2, nRootJ = 2, nRootK = 2, nCell = 8, nStep = 2
+  integer:: nOrder = 2, nStage = 2, iLimiter = 2, iFlux = 1
+  real(c_double):: LimiterBeta = 1.5, Cfl = 0.8, Gamma = 5.0/3.0
+  character(len=10):: TypeLoop = 'blocks'
+  namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
+       iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop
+
+  type(bats_config):: Config
+  character(len=512):: NameDir
+  integer:: nBlock, iBlock, iStep, iStage, iUnit, nG
+  integer(c_int), allocatable:: iNeighbor_EB(:,:)
+  real(c_double), allocatable:: CellSize_DB(:,:), State_VGB(:,:,:,:,:)
+  real(c_double):: Dt, tSimulation
+  integer(c_int):: nStepDone
+  !----------------------------------------------------------------------------
+  call get_command_argument(1, NameDir)
+  open(newunit=iUnit, file=trim(NameDir)//'/PARAM.nml', status='old')
+  read(iUnit, nml=RUN)
+  close(iUnit)
+
+  nG = 2
+  nBlock = nRootI*nRootJ*nRootK
+  allocate(iNeighbor_EB(6,nBlock), CellSize_DB(3,nBlock))
+  allocate(State_VGB(8,1-nG:nCell+nG,1-nG:nCell+nG,1-nG:nCell+nG,nBlock))
+  open(newunit=iUnit, file=trim(NameDir)//'/grid.bin', access='stream', &
+       form='unformatted', status='old')
+  read(iUnit) iNeighbor_EB, CellSize_DB
+  close(iUnit)
+  open(newunit=iUnit, file=trim(NameDir)//'/state_in.bin', access='stream', &
+       form='unformatted', status='old')
+  read(iUnit) State_VGB
+  close(iUnit)
+
+  Config%nI = nCell; Config%nJ = nCell; Config%nK = nCell
+  Config%nG = nG; Config%nDim = 3; Config%nBlock = nBlock
+  Config%nOrder = nOrder; Config%iLimiter = iLimiter
+  Config%LimiterBeta = LimiterBeta; Config%iFlux = iFlux
+  Config%nStage = nStage; Config%UseHalfStep = 1
+  Config%Cfl = Cfl; Config%Gamma = Gamma; Config%pMin = -1.0
+  Config%UseDbTrick = 0; Config%UseDtFixed = 0; Config%DtFixed = 0.0
+  Config%cTiny = 1e-6; Config%iDevice = 0
+
+  call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
+  call sync_state_to_gpu(State_VGB)
+
+  if(TypeLoop == 'blocks')then
+     ! time step limit from the initial state (the reference's Dt=0 step)
+     call exchange_messages
+     call set_stage(nStage)
+     do iBlock = 1, nBlock
+        call calc_face_value(iBlock, .false., .true.)
+        call calc_face_flux(.false., iBlock)
+        call calc_timestep(iBlock)
+     end do
+     do iStep = 1, nStep
+        call set_global_timestep(-1.0_c_double, Dt)
+        STAGELOOP: do iStage = 1, nStage
+           call set_stage(iStage)
+           do iBlock = 1, nBlock
+              call calc_face_value(iBlock, .false., .true.)
+              call calc_face_flux(.false., iBlock)
+              call update_state(iBlock)
+              if(iStage == nStage) call calc_timestep(iBlock)
+           end do
+           call exchange_messages
+        end do STAGELOOP
+     end do
+  else
+     call init_timestep
+     do iStep = 1, nStep
+        call set_global_timestep(-1.0_c_double, Dt)
+        call advance_explicit(.true.)
+     end do
+  end if
+  call check_nan
+  call sync_state_to_cpu(State_VGB)
+  call get_time(Dt, tSimulation, nStepDone)
+
+  open(newunit=iUnit, file=trim(NameDir)//'/state_out.bin', &
+       access='stream', form='unformatted', status='replace')
+  write(iUnit) State_VGB
+  close(iUnit)
+  open(newunit=iUnit, file=trim(NameDir)//'/log.txt', status='replace')
+  write(iUnit,'(a,i6,2es24.16)') 'nStep Dt tSimulation ', nStepDone, Dt, &
+       tSimulation
+  close(iUnit)
+  call clean_hip_bridge
+
+end program bats_fortran_driver
+!==============================================================================

@@ -1,0 +1,195 @@
+/*
+ * bats_hip.h -- C ABI of the MI355X (gfx950) implementation of the BATS-R-US
+ * explicit block-adaptive ideal-MHD stage update.
+ *
+ * The reference's hot path is a set of Fortran module procedures working on
+ * module-global block arrays (State_VGB, LeftState_V*, Flux_V*I, DtMax_CB);
+ * it has no C or FFI layer of its own.  Each entry point below replaces one of
+ * those procedures and keeps its name, argument meaning and order, so that a
+ * Fortran host can bind them with ISO_C_BINDING (see the bind(C) module
+ * batsrus_amd/fortran/ModHipBridge.f90).  Differences from the reference:
+ *   - the arrays live on the GPU inside an opaque handle; the host exchanges
+ *     the reference's State_VGB layout only at sync points (upload/download),
+ *     mirroring sync_cpu_gpu (src/ModUpdateStateFast.f90:71-188);
+ *   - block indices are 1-based, as in the reference;
+ *   - errors are returned as status codes instead of calling stop_mpi; the
+ *     Fortran bridge turns a non-zero code into stop_mpi.
+ *
+ * Threading: one host thread per handle; device work is ordered on the
+ * handle's stream.  All calls are asynchronous except those that return
+ * data to the host (download, get_*, set_global_timestep with DtOut).
+ */
+#ifndef BATS_HIP_H
+#define BATS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference's stop_mpi sites they replace) */
+enum {
+  BATS_OK = 0,
+  BATS_EINVAL = 1,      /* bad argument / configuration */
+  BATS_ENEGFAST2 = 2,   /* negative fast speed squared,
+                           src/ModFaceFlux.f90:4119-4128 */
+  BATS_EHLLD = 3,       /* negative HLLD density ratio,
+                           src/ModFaceFlux.f90:2985-2997,3067-3078 */
+  BATS_ENAN = 4,        /* NaN in the state, src/ModUpdateState.f90:2003-2040 */
+  BATS_EBC = 5,         /* unknown boundary condition,
+                           src/ModUpdateStateFast.f90:1272 */
+  BATS_EHIP = 6,        /* HIP runtime error */
+  BATS_ENOMEM = 7,      /* device allocation failed */
+  BATS_ENODEV = 8,      /* no usable gfx950 device */
+  BATS_ELIMIT = 9       /* request beyond a fixed buffer limit */
+};
+
+/* TypeLimiter (src/ModFaceValue.f90:3713-3789) */
+enum { BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, BATS_LIM_MC3 = 3,
+       BATS_LIM_BETA = 4, BATS_LIM_NO = 5 };
+/* TypeFlux (src/ModFaceFlux.f90:396-404) */
+enum { BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, BATS_FLUX_HLLD = 3 };
+
+/* Scheme parameters: the subset of #SCHEME, #TIMESTEPPING, #GAMMA,
+ * #MINIMUMPRESSURE, #DBTRICK, #FIXEDTIMESTEP and the compile-time grid
+ * shape (Config.pl -g, -ng) that the ideal-MHD explicit update reads
+ * (src/ModSetParameters.f90:628-633,1742-1779,2314-2343,3873-3906). */
+typedef struct bats_config {
+  int nI, nJ, nK;       /* cells per block: 8x8x8 or 16x16x16 */
+  int nG;               /* ghost layers of the State_VGB exchange format (2) */
+  int nDim;             /* must be 3 for the GPU path */
+  int nBlock;           /* blocks held by this handle */
+  int nOrder;           /* 1 or 2 */
+  int iLimiter;         /* BATS_LIM_* (ignored for nOrder=1) */
+  double LimiterBeta;
+  int iFlux;            /* BATS_FLUX_* */
+  int nStage;           /* 1 or 2 */
+  int UseHalfStep;      /* 1: midpoint "half step" scheme (default);
+                           0: SSP-RK2 interpolation (#RK) */
+  double Cfl;
+  double Gamma;
+  double pMin;          /* pressure floor, < 0 disables (#MINIMUMPRESSURE) */
+  int UseDbTrick;       /* #DBTRICK */
+  int UseDtFixed;       /* #FIXEDTIMESTEP */
+  double DtFixed;
+  double cTiny;         /* HLLD degeneracy guard (share ModNumConst) */
+  int iDevice;          /* HIP device ordinal */
+} bats_config;
+
+typedef struct bats_handle bats_handle;
+
+/* Allocate device state for cfg->nBlock blocks.  Replaces init_mod_advance
+ * (src/ModAdvance.f90:249-364) for the hot-path arrays. */
+int bats_init(const bats_config *cfg, bats_handle **h);
+/* Release everything owned by the handle (clean_mod_advance). */
+int bats_finalize(bats_handle *h);
+
+/* Grid description for same-level blocks.  iNeighbor_EB[6*b + side]
+ * (side = x-,x+,y-,y+,z-,z+) is the 0-based index of the neighbouring
+ * block or -1 for a float (zero-gradient) outer boundary
+ * (src/ModCellBoundary.f90:518-559); periodic sides point at the wrapped
+ * neighbour.  CellSize_DB[3*b + d] = cell size of block b in dimension d.
+ * Replaces the BATL tree/geometry queries (DiLevel_EB, CellSize_DB,
+ * CellFace_DB, CellVolume_B). */
+int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
+                  const double *CellSize_DB);
+
+/* Host <-> device state in the reference layout State_VGB(nVar, MinI:MaxI,
+ * MinJ:MaxJ, MinK:MaxK, nBlock), Fortran order with the variable index
+ * fastest (src/ModAdvance.f90:296).  Upload ignores ghost cells; download
+ * writes the interior and fills the face ghost slabs the way
+ * exchange_messages would (same-level neighbours, float outer sides).
+ * Mirrors sync_cpu_gpu (src/ModUpdateStateFast.f90:71-188). */
+int bats_upload_state(bats_handle *h, const double *State_VGB);
+int bats_download_state(bats_handle *h, double *State_VGB);
+/* Same in the device layout (variable-major, interior only):
+ * U[((v*nBlock + b)*nK + k)*nJ + j)*nI + i].  No host transposition. */
+int bats_upload_state_soa(bats_handle *h, const double *U);
+int bats_download_state_soa(bats_handle *h, double *U);
+
+/* ------------------------------------------------------------------------
+ * Per-block routines with the reference's names and arguments.  They work
+ * on one block at a time through one set of face/flux buffers, like the
+ * reference's threadprivate arrays, and exist for per-stage parity dumps.
+ * Ghost values come from the snapshot taken by bats_exchange_messages. */
+
+/* exchange_messages (src/ModMessagePass.f90:27-290): snapshot the current
+ * state as the halo source of the next stage. */
+int bats_exchange_messages(bats_handle *h);
+/* iStage (ModMain) for the per-block routines */
+int bats_set_stage(bats_handle *h, int iStage);
+/* calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
+ * (src/ModFaceValue.f90:854).  DoResChangeOnly must be 0 on a same-level
+ * grid (no resolution changes). */
+int bats_calc_face_value(bats_handle *h, int iBlock, int DoResChangeOnly,
+                         int DoMonotoneRestrict);
+/* calc_face_flux(DoResChangeOnly, iBlock) -- reference argument order
+ * (src/ModFaceFlux.f90:483). */
+int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock);
+/* update_state(iBlock) (src/ModUpdateState.f90:83); at iStage==1 copies the
+ * block into StateOld first. */
+int bats_update_state(bats_handle *h, int iBlock);
+/* calc_timestep(iBlock, IsPartLocal) (src/ModTimeStepControl.f90:111);
+ * IsPartLocal must be 0. */
+int bats_calc_timestep(bats_handle *h, int iBlock, int IsPartLocal);
+
+/* Face data of the last calc_face_value / calc_face_flux block, in the
+ * reference layouts LeftState_VX(nVar,nI+1,nJ,nK), ..._VY(nVar,nI,nJ+1,nK),
+ * ..._VZ(nVar,nI,nJ,nK+1) and Flux_VXI(nFaceValue=14, ...). */
+int bats_get_face_values(bats_handle *h, double *LeftX, double *RightX,
+                         double *LeftY, double *RightY, double *LeftZ,
+                         double *RightZ);
+int bats_get_face_fluxes(bats_handle *h, double *FluxX, double *FluxY,
+                         double *FluxZ);
+/* DtMax_CB(nI,nJ,nK,nBlock) and DtMax_B(nBlock) */
+int bats_get_dtmax(bats_handle *h, double *DtMax_CB, double *DtMax_B);
+
+/* ------------------------------------------------------------------------
+ * Production entry points (all blocks, fused kernels). */
+
+/* set_global_timestep(TimeSimulationLimit)
+ * (src/ModTimeStepControl.f90:306-524): increments nStep, reduces DtMax_B
+ * on the device, applies the first-step Dt=0 rule and the stop-time clip,
+ * and sets Dt = Cfl*DtMin.  If DtOut is non-NULL the call waits and returns
+ * the CFL-scaled Dt; otherwise it stays asynchronous. */
+int bats_set_global_timestep(bats_handle *h, double TimeSimulationLimit,
+                             double *DtOut);
+/* advance_explicit(DoCalcTimestep) (src/ModAdvanceExplicit.f90:18-325):
+ * all stages of one time step as fused face-value + face-flux + update
+ * kernels over all blocks; with DoCalcTimestep the last stage also computes
+ * DtMax_B (calc_timestep), and tSimulation advances by Dt/nStage per stage. */
+int bats_advance_explicit(bats_handle *h, int DoCalcTimestep);
+/* One stage of advance_explicit (iStage = 1..nStage) over all blocks. */
+int bats_advance_stage(bats_handle *h, int iStage, int DoCalcTimestep);
+/* BATS_advance (src/ModBatsrusMethods.f90:493-759) for nStep steps:
+ * set_global_timestep + advance_explicit per step, replayed from a HIP
+ * graph after the first step. */
+int bats_advance(bats_handle *h, int nStep, double TimeSimulationLimit);
+/* Face-speed pass on the current state without an update: fills DtMax_B
+ * like the reference's first (Dt = 0) time step. */
+int bats_init_timestep(bats_handle *h);
+
+/* Scalars: Dt (CFL-scaled), tSimulation, nStep */
+int bats_get_time(bats_handle *h, double *Dt, double *tSimulation,
+                  int *nStep);
+/* Wait for queued work and return the first error raised on the device */
+int bats_synchronize(bats_handle *h);
+/* check_nan (src/ModUpdateState.f90:2003-2040): BATS_ENAN if any state
+ * value is NaN or infinite */
+int bats_check_nan(bats_handle *h);
+/* Conservation integrals: sum over cells of U*V for rho, rho*u (3), B (3)
+ * and total energy (8 values), deterministic order (integrate_domain,
+ * src/ModWriteLogSatFile.f90:1948-2020). */
+int bats_integrate_domain(bats_handle *h, double *Sum_V);
+
+/* Kernel-time measurement: average duration in ms of the fused stage
+ * kernel over the last bats_advance call, measured with HIP events on the
+ * handle's stream (0 if none). */
+int bats_last_stage_kernel_ms(bats_handle *h, double *ms, int *nLaunch);
+
+/* Human-readable text for a status code */
+const char *bats_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+/*
+ * bats_oracle.h -- CPU oracle for the explicit block-adaptive ideal-MHD stage
+ * update of BATS-R-US (face reconstruction, Rusanov/Linde/HLLD face fluxes,
+ * flux-divergence stage update, CFL time step).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * Fortran "orig" CPU path, written so that the GPU product can be checked
+ * against it.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it; the product never links or calls it.
+ *
+ * Array layouts follow the reference exactly (Fortran column-major, variable
+ * index fastest):
+ *   State_VGB(nVar, MinI:MaxI, MinJ:MaxJ, MinK:MaxK, nBlock)
+ *       -> State[(((b*nKg + k)*nJg + j)*nIg + i)*nVar + v], ghost offsets nG
+ *          in each active dimension (src/ModAdvance.f90:296).
+ *   LeftState_VX(nVar, nI+1, nJ, nK) etc.            (src/ModAdvance.f90:178-186)
+ *   Flux_VXI(nFaceValue=14, nI+1, nJ, nK)            (src/ModAdvance.f90:46-59,318)
+ *   DtMax_CB(nI, nJ, nK, nBlock)                      (src/ModAdvance.f90:124-137)
+ *
+ * Parity status of each function is stated in oracle/README-style header of
+ * bats_oracle.c (pinned vs golden fixture / closed forms / "parity unpinned").
+ */
+#ifndef BATS_ORACLE_H
+#define BATS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 0-based variable and face-value slots; reference 1-based indices are
+ * Rho_=1 .. p_=8, Energy_=9, UnFirst_=10, UnLast_=11, Vdt_=12, BnL_=13, BnR_=14
+ * (srcEquation/ModEquationMhd.f90:20-35, src/ModAdvance.f90:46-59). */
+enum {
+  OB_RHO = 0, OB_RHOUX = 1, OB_RHOUY = 2, OB_RHOUZ = 3,
+  OB_BX = 4, OB_BY = 5, OB_BZ = 6, OB_P = 7,
+  OB_NVAR = 8, OB_ENERGY = 8, OB_NFLUX = 9,
+  OB_UNFIRST = 9, OB_UNLAST = 10, OB_VDT = 11, OB_BNL = 12, OB_BNR = 13,
+  OB_NFACEVALUE = 14
+};
+
+/* TypeLimiter values (src/ModFaceValue.f90:3713-3789) */
+enum { OB_LIM_MINMOD = 1, OB_LIM_MC = 2, OB_LIM_MC3 = 3, OB_LIM_BETA = 4,
+       OB_LIM_NO = 5 };
+/* TypeFlux values (src/ModFaceFlux.f90:396-404) */
+enum { OB_FLUX_RUSANOV = 1, OB_FLUX_LINDE = 2, OB_FLUX_HLLD = 3 };
+
+/* Error codes (the reference calls stop_mpi at these points) */
+enum { OB_OK = 0, OB_ENEGFAST2 = 2, OB_EHLLD = 3, OB_EBC = 5 };
+
+typedef struct {
+  int nI, nJ, nK;      /* cells per block */
+  int nG;              /* ghost layers (2) */
+  int nDim;            /* 1, 2 or 3: faces exist only in active dimensions */
+  int nOrder;          /* 1 or 2 */
+  int iLimiter;        /* OB_LIM_* */
+  double LimiterBeta;
+  int iFlux;           /* OB_FLUX_* */
+  int nStage;          /* 1 or 2 */
+  int UseHalfStep;     /* 1: U = Uold + Source (midpoint); 0: SSP-RK interp */
+  double Cfl;
+  double Gamma;
+  double pMin;         /* < 0: no pressure floor */
+  int UseDbTrick;
+  int UseDtFixed;
+  double DtFixed;
+  double cTiny;        /* HLLD degeneracy guard (ModNumConst, absent) */
+} ob_config;
+
+/* --- point routines (exposed for unit tests) ------------------------------ */
+/* limiter on a 1-D pencil: P holds primitives for l = lMin-2 .. lMax+1 with
+ * P[(l - lOff)*8 + v]; slopes are written for l = lMin-1 .. lMax into
+ * dL/dR[(l - lOff)*8 + v].  Mirrors src/ModFaceValue.f90:3689-3791. */
+void ob_limiter(int iLimiter, int lMin, int lMax, double Beta,
+                const double *P, int lOff, double *dL, double *dR);
+
+/* physical flux of primitive state (src/ModFaceFlux.f90:1166-1410,1550-1899),
+ * iDim = 0,1,2 (Cartesian face normal) */
+void ob_physical_flux(const ob_config *c, int iDim, const double *State_V,
+                      double *StateCons_V, double *Flux_V, double *Un);
+/* speeds of state (src/ModFaceFlux.f90:3545-3722,3853-4252) */
+int ob_speed_max(const ob_config *c, int iDim, const double *State_V,
+                 double *Cmax, double *Cleft, double *Cright);
+/* numerical flux at one face: inputs are the L/R primitive face states,
+ * Area; Flux_V has nFaceValue=14 slots (flux*Area in 0..8, Un*Area in 9,
+ * Vdt in 11).  src/ModFaceFlux.f90:2130-2473 */
+int ob_numerical_flux(const ob_config *c, int iDim, double Area,
+                      const double *StateLeftIn_V, const double *StateRightIn_V,
+                      double *Flux_V);
+
+/* --- per-block routines --------------------------------------------------- */
+/* calc_face_value (src/ModFaceValue.f90:854-2165) for one ghosted block. */
+void ob_calc_face_value(const ob_config *c, const double *StateBlk,
+                        double *LeftX, double *RightX,
+                        double *LeftY, double *RightY,
+                        double *LeftZ, double *RightZ);
+/* calc_face_flux (src/ModFaceFlux.f90:483-977); CellSize = dx,dy,dz */
+int ob_calc_face_flux(const ob_config *c, const double *CellSize,
+                      double *LeftX, double *RightX,
+                      double *LeftY, double *RightY,
+                      double *LeftZ, double *RightZ,
+                      double *FluxX, double *FluxY, double *FluxZ);
+/* update_state (src/ModUpdateState.f90:83-965) for one block.  DtMax_CB is
+ * this block's (nI,nJ,nK) array; Dt is the CFL-scaled global step. */
+void ob_update_state(const ob_config *c, int iStage, double Dt,
+                     const double *CellSize, const double *DtMax_CB,
+                     const double *FluxX, const double *FluxY,
+                     const double *FluxZ, double *StateBlk,
+                     double *StateOldBlk);
+/* calc_timestep (src/ModTimeStepControl.f90:111-304) for one block */
+void ob_calc_timestep(const ob_config *c, const double *CellSize,
+                      const double *FluxX, const double *FluxY,
+                      const double *FluxZ, double *DtMax_CB, double *DtMax_B);
+
+/* --- grid-level routines -------------------------------------------------- */
+/* Ghost fill for same-level blocks: iNeighbor_EB[b*6+side] (side order
+ * x-,x+,y-,y+,z-,z+) is a 0-based block index or -1 for a float (zero
+ * gradient) outer boundary (src/ModCellBoundary.f90:518-559). */
+void ob_exchange_messages(const ob_config *c, int nBlock,
+                          const int *iNeighbor_EB, double *State);
+
+/* Whole-grid session state for the time loop */
+typedef struct {
+  ob_config c;
+  int nBlock;
+  const int *iNeighbor_EB;   /* 6*nBlock */
+  const double *CellSize_DB; /* 3*nBlock */
+  double *State;             /* ghosted, nBlock blocks */
+  double *StateOld;          /* ghosted */
+  double *DtMax_CB;          /* nI*nJ*nK*nBlock */
+  double *DtMax_B;           /* nBlock */
+  double Dt;                 /* CFL-scaled step actually taken */
+  double tSimulation;
+  int nStep;
+  int nThread;               /* OpenMP threads (<=0: default) */
+} ob_session;
+
+/* set_global_timestep (src/ModTimeStepControl.f90:306-524) */
+void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit);
+/* one stage of advance_explicit: all blocks, time advance, ghost exchange
+ * between stages (src/ModAdvanceExplicit.f90:74-312) */
+int ob_advance_stage(ob_session *s, int iStage, int DoCalcTimestep);
+/* advance_explicit (src/ModAdvanceExplicit.f90:18-325), orig CPU path */
+int ob_advance_explicit(ob_session *s, int DoCalcTimestep);
+/* BATS_advance-shaped time step (src/ModBatsrusMethods.f90:566-653) */
+int ob_advance_step(ob_session *s, double TimeSimulationLimit);
+/* face-speed pass on the current state only (no update): fills DtMax_CB and
+ * DtMax_B exactly as the reference's Dt=0 first step does */
+int ob_init_timestep(ob_session *s);
+
+/* pressure <-> energy for one cell (src/ModEnergy.f90:47-127,243-332) */
+void ob_pressure_to_energy_cell(const ob_config *c, double *State_V);
+void ob_energy_to_pressure_cell(const ob_config *c, double *State_V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` (built by ``make -C oracle``).  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import this
+module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+LIMITERS = {"minmod": 1, "mc": 2, "mc3": 3, "koren": 3, "beta": 4, "no": 5}
+FLUXES = {"rusanov": 1, "linde": 2, "hlld": 3}
+
+
+class ObConfig(C.Structure):
+    _fields_ = [("nI", C.c_int), ("nJ", C.c_int), ("nK", C.c_int),
+                ("nG", C.c_int), ("nDim", C.c_int), ("nOrder", C.c_int),
+                ("iLimiter", C.c_int), ("LimiterBeta", C.c_double),
+                ("iFlux", C.c_int), ("nStage", C.c_int),
+                ("UseHalfStep", C.c_int), ("Cfl", C.c_double),
+                ("Gamma", C.c_double), ("pMin", C.c_double),
+                ("UseDbTrick", C.c_int), ("UseDtFixed", C.c_int),
+                ("DtFixed", C.c_double), ("cTiny", C.c_double)]
+
+
+class ObSession(C.Structure):
+    _fields_ = [("c", ObConfig), ("nBlock", C.c_int),
+                ("iNeighbor_EB", C.POINTER(C.c_int)),
+                ("CellSize_DB", C.POINTER(C.c_double)),
+                ("State", C.POINTER(C.c_double)),
+                ("StateOld", C.POINTER(C.c_double)),
+                ("DtMax_CB", C.POINTER(C.c_double)),
+                ("DtMax_B", C.POINTER(C.c_double)),
+                ("Dt", C.c_double), ("tSimulation", C.c_double),
+                ("nStep", C.c_int), ("nThread", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.ob_limiter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dp,
+                                 C.c_int, dp, dp]
+        L.ob_physical_flux.argtypes = [C.POINTER(ObConfig), C.c_int, dp, dp,
+                                       dp, dp]
+        L.ob_speed_max.argtypes = [C.POINTER(ObConfig), C.c_int, dp, dp, dp,
+                                   dp]
+        L.ob_speed_max.restype = C.c_int
+        L.ob_numerical_flux.argtypes = [C.POINTER(ObConfig), C.c_int,
+                                        C.c_double, dp, dp, dp]
+        L.ob_numerical_flux.restype = C.c_int
+        L.ob_calc_face_value.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 6
+        L.ob_calc_face_flux.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 9
+        L.ob_calc_face_flux.restype = C.c_int
+        L.ob_update_state.argtypes = [C.POINTER(ObConfig), C.c_int,
+                                      C.c_double, dp, dp, dp, dp, dp, dp, dp]
+        L.ob_calc_timestep.argtypes = [C.POINTER(ObConfig), dp, dp, dp, dp,
+                                       dp, dp]
+        L.ob_exchange_messages.argtypes = [C.POINTER(ObConfig), C.c_int,
+                                           C.POINTER(C.c_int), dp]
+        L.ob_set_global_timestep.argtypes = [C.POINTER(ObSession), C.c_double]
+        L.ob_advance_explicit.argtypes = [C.POINTER(ObSession), C.c_int]
+        L.ob_advance_explicit.restype = C.c_int
+        L.ob_advance_stage.argtypes = [C.POINTER(ObSession), C.c_int, C.c_int]
+        L.ob_advance_stage.restype = C.c_int
+        L.ob_advance_step.argtypes = [C.POINTER(ObSession), C.c_double]
+        L.ob_advance_step.restype = C.c_int
+        L.ob_init_timestep.argtypes = [C.POINTER(ObSession)]
+        L.ob_init_timestep.restype = C.c_int
+        L.ob_pressure_to_energy_cell.argtypes = [C.POINTER(ObConfig), dp]
+        L.ob_energy_to_pressure_cell.argtypes = [C.POINTER(ObConfig), dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
+                beta=1.5, flux="rusanov", nStage=2, UseHalfStep=True, cfl=0.8,
+                gamma=5.0 / 3.0, pMin=-1.0, UseDbTrick=False,
+                UseDtFixed=False, DtFixed=0.0, cTiny=1e-6):
+    c = ObConfig()
+    c.nI, c.nJ, c.nK, c.nG, c.nDim, c.nOrder = nI, nJ, nK, nG, nDim, nOrder
+    c.iLimiter = LIMITERS[limiter] if isinstance(limiter, str) else limiter
+    c.LimiterBeta = beta
+    c.iFlux = FLUXES[flux] if isinstance(flux, str) else flux
+    c.nStage, c.UseHalfStep, c.Cfl, c.Gamma = nStage, int(UseHalfStep), cfl, gamma
+    c.pMin, c.UseDbTrick = pMin, int(UseDbTrick)
+    c.UseDtFixed, c.DtFixed, c.cTiny = int(UseDtFixed), DtFixed, cTiny
+    return c
+
+
+def config_from_dict(d, grid):
+    return make_config(nI=grid.nI, nJ=grid.nJ, nK=grid.nK, nG=grid.nG,
+                       nDim=grid.nDim, **d)
+
+
+# ----------------------------------------------------------------------
+# point routines
+def limiter(iLimiter, P, beta):
+    """Slopes for a pencil P of shape (n, 8) holding l = lMin-2 .. lMax+1;
+    returns dL, dR of the same shape with rows lMin-1 .. lMax valid."""
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    n = P.shape[0]
+    lMin, lMax = 2, n - 2
+    dL = np.zeros_like(P)
+    dR = np.zeros_like(P)
+    lib().ob_limiter(LIMITERS.get(iLimiter, iLimiter), lMin, lMax, beta,
+                     _dp(P), 0, _dp(dL), _dp(dR))
+    return dL, dR
+
+
+def physical_flux(cfg, iDim, state):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    cons = np.zeros(9)
+    flux = np.zeros(9)
+    un = C.c_double()
+    lib().ob_physical_flux(C.byref(cfg), iDim, _dp(s), _dp(cons), _dp(flux),
+                           C.byref(un))
+    return cons, flux, un.value
+
+
+def speed_max(cfg, iDim, state):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    err = lib().ob_speed_max(C.byref(cfg), iDim, _dp(s), C.byref(a),
+                             C.byref(b), C.byref(c))
+    return err, a.value, b.value, c.value
+
+
+def numerical_flux(cfg, iDim, area, left, right):
+    L = np.ascontiguousarray(left, dtype=np.float64)
+    R = np.ascontiguousarray(right, dtype=np.float64)
+    F = np.zeros(14)
+    err = lib().ob_numerical_flux(C.byref(cfg), iDim, area, _dp(L), _dp(R),
+                                  _dp(F))
+    return err, F
+
+
+def energy_round_trip(cfg, state):
+    s = np.array(state, dtype=np.float64)
+    lib().ob_pressure_to_energy_cell(C.byref(cfg), _dp(s))
+    e = s.copy()
+    lib().ob_energy_to_pressure_cell(C.byref(cfg), _dp(s))
+    return e, s
+
+
+# ----------------------------------------------------------------------
+# per-block routines
+def face_shapes(cfg):
+    nI, nJ, nK = cfg.nI, cfg.nJ, cfg.nK
+    return ((nK, nJ, nI + 1), (nK, nJ + 1, nI), (nK + 1, nJ, nI))
+
+
+def calc_face_value(cfg, block_state):
+    blk = np.ascontiguousarray(block_state, dtype=np.float64)
+    sx, sy, sz = face_shapes(cfg)
+    arrs = [np.zeros(s + (8,)) for s in (sx, sx, sy, sy, sz, sz)]
+    lib().ob_calc_face_value(C.byref(cfg), _dp(blk), *[_dp(a) for a in arrs])
+    return arrs
+
+
+def calc_face_flux(cfg, cell_size, faces):
+    sx, sy, sz = face_shapes(cfg)
+    fl = [np.zeros(s + (14,)) for s in (sx, sy, sz)]
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    err = lib().ob_calc_face_flux(C.byref(cfg), _dp(cs),
+                                  *[_dp(a) for a in faces],
+                                  *[_dp(a) for a in fl])
+    return err, fl
+
+
+def update_state(cfg, iStage, dt, cell_size, dtmax_cb, fluxes, block_state,
+                 block_old):
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    lib().ob_update_state(C.byref(cfg), iStage, dt, _dp(cs), _dp(dtmax_cb),
+                          *[_dp(f) for f in fluxes], _dp(block_state),
+                          _dp(block_old))
+
+
+def calc_timestep(cfg, cell_size, fluxes):
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    dtcb = np.zeros((cfg.nK, cfg.nJ, cfg.nI))
+    dtb = C.c_double()
+    lib().ob_calc_timestep(C.byref(cfg), _dp(cs), *[_dp(f) for f in fluxes],
+                           _dp(dtcb), C.byref(dtb))
+    return dtcb, dtb.value
+
+
+# ----------------------------------------------------------------------
+class Session:
+    """Whole-grid oracle run (BATS_advance-shaped time loop)."""
+
+    def __init__(self, grid, cfg, state_vgb, nThread=0):
+        self.grid = grid
+        self.cfg = cfg
+        self.nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
+        self.cs = np.ascontiguousarray(grid.CellSize_DB, dtype=np.float64)
+        self.State = np.ascontiguousarray(state_vgb, dtype=np.float64).copy()
+        self.StateOld = np.zeros_like(self.State)
+        n = grid.nBlock * cfg.nI * cfg.nJ * cfg.nK
+        self.DtMax_CB = np.zeros(n)
+        self.DtMax_B = np.zeros(grid.nBlock)
+        s = ObSession()
+        s.c = cfg
+        s.nBlock = grid.nBlock
+        s.iNeighbor_EB = self.nei.ctypes.data_as(C.POINTER(C.c_int))
+        s.CellSize_DB = _dp(self.cs)
+        s.State = _dp(self.State)
+        s.StateOld = _dp(self.StateOld)
+        s.DtMax_CB = _dp(self.DtMax_CB)
+        s.DtMax_B = _dp(self.DtMax_B)
+        s.Dt = 0.0
+        s.tSimulation = 0.0
+        s.nStep = 0
+        s.nThread = nThread
+        self.s = s
+
+    def exchange(self):
+        lib().ob_exchange_messages(C.byref(self.cfg), self.grid.nBlock,
+                                   self.nei.ctypes.data_as(C.POINTER(C.c_int)),
+                                   _dp(self.State))
+
+    def init_timestep(self):
+        return lib().ob_init_timestep(C.byref(self.s))
+
+    def set_global_timestep(self, tlimit=-1.0):
+        lib().ob_set_global_timestep(C.byref(self.s), tlimit)
+        return self.s.Dt
+
+    def advance_explicit(self, DoCalcTimestep=True):
+        return lib().ob_advance_explicit(C.byref(self.s), int(DoCalcTimestep))
+
+    def advance_stage(self, iStage, DoCalcTimestep=True):
+        return lib().ob_advance_stage(C.byref(self.s), iStage,
+                                      int(DoCalcTimestep))
+
+    def new_step(self, tlimit=-1.0):
+        """nStep += 1 and set_global_timestep, as BATS_advance starts a
+        step (src/ModBatsrusMethods.f90:566-572)"""
+        self.s.nStep += 1
+        return self.set_global_timestep(tlimit)
+
+    def advance_step(self, tlimit=-1.0):
+        return lib().ob_advance_step(C.byref(self.s), tlimit)
+
+    @property
+    def Dt(self):
+        return self.s.Dt
+
+    @property
+    def tSimulation(self):
+        return self.s.tSimulation
+
+    @property
+    def nStep(self):
+        return self.s.nStep
+
+    def soa(self):
+        return self.grid.state_vgb_to_soa(self.State)

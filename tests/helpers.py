@@ -1,0 +1,51 @@
+"""Shared helpers for the parity tests."""
+import numpy as np
+
+from batsrus_amd.grid import BlockGrid
+
+
+def random_mhd_soa(grid: BlockGrid, seed=1, rho=(0.5, 1.5), u=1.0, b=1.0,
+                   p=(0.3, 1.5)):
+    """Cell-wise random but physical MHD state (every HLLD branch and every
+    limiter case occurs), SoA layout U[var, block, k, j, i]."""
+    rng = np.random.default_rng(seed)
+    shp = (grid.nBlock, grid.nK, grid.nJ, grid.nI)
+    U = np.zeros((8,) + shp)
+    r = rng.uniform(rho[0], rho[1], shp)
+    U[0] = r
+    for d in range(3):
+        U[1 + d] = r * rng.uniform(-u, u, shp)
+        U[4 + d] = rng.uniform(-b, b, shp)
+    U[7] = rng.uniform(p[0], p[1], shp)
+    return U
+
+
+def smooth_mhd_soa(grid: BlockGrid, seed=2, amp=0.3):
+    """Smooth periodic state with random Fourier phases plus small noise."""
+    rng = np.random.default_rng(seed)
+    x, y, z = grid.cell_centers()
+    tp = 2 * np.pi
+    ph = rng.uniform(0, tp, 16)
+    U = np.zeros((8,) + x.shape)
+    rho = 1.0 + amp * np.sin(tp * x + ph[0]) * np.cos(tp * y + ph[1])
+    U[0] = rho
+    U[1] = rho * amp * np.sin(tp * y + ph[2])
+    U[2] = rho * amp * np.sin(tp * z + ph[3])
+    U[3] = rho * amp * np.sin(tp * x + ph[4])
+    U[4] = 0.5 + amp * np.cos(tp * z + ph[5])
+    U[5] = amp * np.sin(tp * x + ph[6])
+    U[6] = amp * np.cos(tp * y + ph[7])
+    U[7] = 1.0 + amp * np.cos(tp * (x + y + z) + ph[8])
+    U += 1e-4 * rng.standard_normal(U.shape) * np.array(
+        [1, 1, 1, 1, 1, 1, 1, 1.0])[:, None, None, None, None]
+    return U
+
+
+def rel_err(a, b, axis_var=0):
+    """max |a-b| per variable divided by max |b| of that variable."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    a2 = np.moveaxis(a, axis_var, 0).reshape(a.shape[axis_var], -1)
+    b2 = np.moveaxis(b, axis_var, 0).reshape(b.shape[axis_var], -1)
+    scale = np.maximum(np.abs(b2).max(axis=1), 1e-300)
+    return (np.abs(a2 - b2).max(axis=1) / scale).max()

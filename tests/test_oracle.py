@@ -1,0 +1,386 @@
+"""Pins the CPU oracle: the reference's committed golden run, closed forms
+and invariants chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails at least one check."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from batsrus_amd.grid import BlockGrid
+from batsrus_amd import ics
+from helpers import random_mhd_soa, smooth_mhd_soa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GAMMA = 5.0 / 3.0
+
+
+# ----------------------------------------------------------------------
+# golden: Param/SHOCKTUBE/TestOutput/mhdhyp_1d_ref.out
+def test_oracle_shocktube_golden():
+    """First-order Rusanov, one stage, E<->p, float boundaries: the 1-D
+    Brio-Wu run of Param/SHOCKTUBE/PARAM.in.hyp.1d (gamma=2, 1024 cells in
+    16 blocks of 64, dt=0.05, 512 steps) matches the committed reference
+    output to its print precision (and to DiffNum -r=1e-5 -a=1e-11)."""
+    ref = np.load(os.path.join(HERE, "golden", "mhdhyp_1d_ref.npz"))
+    assert int(ref["nstep"]) == 512 and float(ref["gamma"]) == 2.0
+    g = BlockGrid(nRoot=(16, 1, 1), nCell=(64, 1, 1),
+                  xyzMin=(-128, -0.125, -0.125), xyzMax=(128, 0.125, 0.125),
+                  periodic=(False, True, True), nDim=1)
+    U = ics.brio_wu(g, noise=0.0, x0=0.0, left=(1.0, 1.0, 1.0),
+                    right=(0.125, 0.1, -1.0), bx=0.75)
+    cfg = po.make_config(nI=64, nJ=1, nK=1, nDim=1, nOrder=1, flux="rusanov",
+                         nStage=1, cfl=1.0, gamma=2.0, UseDtFixed=True,
+                         DtFixed=0.05)
+    s = po.Session(g, cfg, g.soa_to_state_vgb(U))
+    s.exchange()
+    for _ in range(512):
+        assert s.advance_step(25.6) == 0
+    assert s.nStep == 512 and abs(s.tSimulation - 25.6) < 1e-12
+    out = s.soa()
+    rho = out[0].reshape(-1)
+    prim = {"Rho": rho, "Ux": out[1].reshape(-1) / rho,
+            "Uy": out[2].reshape(-1) / rho, "Uz": out[3].reshape(-1) / rho,
+            "Bx": out[4].reshape(-1), "By": out[5].reshape(-1),
+            "Bz": out[6].reshape(-1), "P": out[7].reshape(-1)}
+    names = list(ref["names"])
+    x = ref["data"][:, names.index("x")]
+    xc = g.cell_centers()[0].reshape(-1)
+    assert np.allclose(x, xc, rtol=0, atol=1e-9)
+    for nm, arr in prim.items():
+        r = ref["data"][:, names.index(nm)]
+        err = np.abs(arr - r)
+        assert np.all((err <= 1e-11) | (err <= 1e-5 * np.abs(r))), nm
+        assert err.max() <= 1e-10, (nm, err.max())
+
+
+def test_planted_mistake_fails_golden():
+    """A plausible mistake in the oracle's scheme (second-order instead of
+    first-order faces, or Linde instead of Rusanov) is caught by the golden
+    comparison: the check is sensitive, not vacuous."""
+    ref = np.load(os.path.join(HERE, "golden", "mhdhyp_1d_ref.npz"))
+    names = list(ref["names"])
+    g = BlockGrid(nRoot=(16, 1, 1), nCell=(64, 1, 1),
+                  xyzMin=(-128, -0.125, -0.125), xyzMax=(128, 0.125, 0.125),
+                  periodic=(False, True, True), nDim=1)
+    U = ics.brio_wu(g, noise=0.0, x0=0.0, left=(1.0, 1.0, 1.0),
+                    right=(0.125, 0.1, -1.0), bx=0.75)
+    for bad in (dict(flux="linde"), dict(gamma=5.0 / 3.0)):
+        kw = dict(nI=64, nJ=1, nK=1, nDim=1, nOrder=1, flux="rusanov",
+                  nStage=1, cfl=1.0, gamma=2.0, UseDtFixed=True, DtFixed=0.05)
+        kw.update(bad)
+        s = po.Session(g, po.make_config(**kw), g.soa_to_state_vgb(U))
+        s.exchange()
+        for _ in range(64):
+            s.advance_step(25.6)
+        # compare at an intermediate time against a correct run
+        good = po.Session(g, po.make_config(**dict(kw, **dict(
+            flux="rusanov", gamma=2.0))), g.soa_to_state_vgb(U))
+        good.exchange()
+        for _ in range(64):
+            good.advance_step(25.6)
+        assert np.abs(s.soa() - good.soa()).max() > 1e-3
+
+
+# ----------------------------------------------------------------------
+# limiter closed forms (src/ModFaceValue.f90:3689-3791)
+@pytest.mark.parametrize("lim", ["minmod", "mc", "mc3", "beta"])
+def test_limiter_extremum_and_linear(lim):
+    n = 12
+    # local extremum at l = 5: slope must vanish there
+    P = np.zeros((n, 8))
+    P[:, :] = np.array([0, 1, 2, 3, 4, 9, 4, 3, 2, 1, 0, -1.0])[:, None]
+    dL, dR = po.limiter(lim, P, 1.5)
+    assert np.all(dL[5] == 0) and np.all(dR[5] == 0)
+    # linear data: half slopes equal a quarter... of the difference times 2
+    P = np.outer(np.arange(n) * 0.37 + 2.0, np.ones(8))
+    dL, dR = po.limiter(lim, P, 1.5)
+    half = 0.5 * 0.37
+    assert np.allclose(dL[1:n - 2], half, rtol=1e-15, atol=1e-16)
+    assert np.allclose(dR[1:n - 2], half, rtol=1e-15, atol=1e-16)
+
+
+def test_limiter_formulas_random():
+    """Brute-force check of every limiter formula on random pencils."""
+    rng = np.random.default_rng(0)
+    n = 20
+    P = rng.standard_normal((n, 8))
+    beta = 1.5
+    for lim in ["minmod", "mc", "mc3", "beta", "no"]:
+        dL, dR = po.limiter(lim, P, beta)
+        for l in range(1, n - 1 - 1):
+            d1 = P[l] - P[l - 1]
+            d2 = P[l + 1] - P[l]
+            s = 0.25 * (np.sign(d1) + np.sign(d2))
+            if lim == "minmod":
+                eL = eR = s * np.minimum(abs(d1), abs(d2))
+            elif lim == "mc":
+                eL = eR = s * np.minimum(np.minimum(beta * abs(d1),
+                                                    beta * abs(d2)),
+                                         0.5 * abs(d1 + d2))
+            elif lim == "mc3":
+                m = np.minimum(beta * abs(d1), beta * abs(d2))
+                eR = s * np.minimum(m, abs(2 * d1 + d2) / 3)
+                eL = s * np.minimum(m, abs(d1 + 2 * d2) / 3)
+            elif lim == "beta":
+                eL = eR = s * np.maximum(np.minimum(abs(d2), beta * abs(d1)),
+                                         np.minimum(abs(d1), beta * abs(d2)))
+            else:
+                eL = eR = 0.25 * (P[l + 1] - P[l - 1])
+            assert np.allclose(dL[l], eL, rtol=1e-14, atol=1e-15), (lim, l)
+            assert np.allclose(dR[l], eR, rtol=1e-14, atol=1e-15), (lim, l)
+
+
+def test_koren_beta1_equals_minmod():
+    """mc3 with beta = 1 reduces to minmod (the 1/3 terms never win)."""
+    rng = np.random.default_rng(1)
+    P = rng.standard_normal((30, 8))
+    a = po.limiter("mc3", P, 1.0)
+    b = po.limiter("minmod", P, 1.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ----------------------------------------------------------------------
+# fluxes (src/ModFaceFlux.f90)
+def _rand_state(rng):
+    return np.array([rng.uniform(0.5, 1.5), *rng.uniform(-1, 1, 3),
+                     *rng.uniform(-1, 1, 3), rng.uniform(0.3, 1.5)])
+
+
+def _exact_flux(S, d, gamma=GAMMA):
+    """Ideal-MHD physical flux written from the textbook conservation law
+    (independent of the oracle's code)."""
+    rho, u, B, p = S[0], S[1:4], S[4:7], S[7]
+    un, bn = u[d], B[d]
+    B2 = B @ B
+    e = p / (gamma - 1) + 0.5 * rho * (u @ u) + 0.5 * B2
+    F = np.zeros(9)
+    F[0] = rho * un
+    F[1:4] = rho * un * u - bn * B
+    F[1 + d] += p + 0.5 * B2
+    F[4:7] = un * B - u * bn
+    F[7] = un * p
+    F[8] = un * (e + p + 0.5 * B2) - bn * (u @ B)
+    return F
+
+
+def test_physical_flux_textbook():
+    cfg = po.make_config()
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        S = _rand_state(rng)
+        for d in range(3):
+            cons, F, un = po.physical_flux(cfg, d, S)
+            assert np.allclose(F, _exact_flux(S, d), rtol=1e-13, atol=1e-14)
+            assert un == S[1 + d]
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "linde", "hlld"])
+def test_flux_consistency(flux):
+    """U_L = U_R gives the physical flux times the area."""
+    cfg = po.make_config(flux=flux)
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        S = _rand_state(rng)
+        for d in range(3):
+            err, F = po.numerical_flux(cfg, d, 0.25, S, S)
+            assert err == 0
+            ref = _exact_flux(S, d) * 0.25
+            sl = [0, 1, 2, 3, 4, 5, 6, 8]
+            assert np.allclose(F[sl], ref[sl], rtol=1e-12, atol=1e-14)
+
+
+def test_hlld_supersonic_upwind():
+    """sL >= 0: HLLD returns the left physical flux (:2880-2886)."""
+    cfg = po.make_config(flux="hlld")
+    L = np.array([1.0, 5.0, 0.1, -0.2, 0.3, 0.2, 0.1, 0.5])
+    R = np.array([0.8, 4.5, -0.1, 0.1, 0.3, 0.4, -0.2, 0.4])
+    err, F = po.numerical_flux(cfg, 0, 1.0, L, R)
+    assert err == 0
+    # Linde fix: Bn averaged, flux of L with Bn -> avg, then -Cmax*dB terms
+    dB = 0.5 * (R[4] - L[4])
+    assert dB == 0.0
+    ref = _exact_flux(L, 0)
+    assert np.allclose(F[[0, 1, 2, 3, 4, 5, 6, 8]], ref[[0, 1, 2, 3, 4, 5, 6, 8]],
+                       rtol=1e-13, atol=1e-14)
+
+
+def test_hlld_resolves_contact_and_rotational():
+    """Miyoshi & Kusano (2005): HLLD resolves an isolated stationary contact
+    (jump in density only, u = 0) and an isolated stationary rotational
+    discontinuity exactly, i.e. the numerical flux is the physical flux
+    with no diffusion.  Rusanov does not (it diffuses the density jump)."""
+    cfg = po.make_config(flux="hlld")
+    L = np.array([1.0, 0.0, 0.0, 0.0, 0.7, 0.3, -0.2, 0.6])
+    R = L.copy()
+    R[0] = 0.3
+    err, F = po.numerical_flux(cfg, 0, 1.0, L, R)
+    assert err == 0
+    ref = _exact_flux(L, 0)
+    assert abs(F[0]) <= 1e-15
+    assert np.allclose(F[[1, 2, 3, 4, 5, 6, 8]], ref[[1, 2, 3, 4, 5, 6, 8]],
+                       rtol=1e-13, atol=1e-14)
+    errr, Fr = po.numerical_flux(po.make_config(flux="rusanov"), 0, 1.0, L, R)
+    assert abs(Fr[0]) > 1e-3
+    # stationary rotational discontinuity: Bn=0.7, rho=1, u_t = -B_t/sqrt(rho)
+    # on each side (Alfven relation, sign for Bn>0), tangential field
+    # rotated, same |B_t|, same p
+    rho = 1.0
+    Bt1 = np.array([0.3, -0.2])
+    Bt2 = np.array([-0.2, -0.3])
+    L = np.array([rho, 0.0, *(-Bt1 / np.sqrt(rho)), 0.7, *Bt1, 0.6])
+    R = np.array([rho, 0.0, *(-Bt2 / np.sqrt(rho)), 0.7, *Bt2, 0.6])
+    # a stationary rotational discontinuity moving with the Alfven speed in
+    # the fluid frame: shift the frame so it is at rest; u_n = Bn/sqrt(rho)
+    un = 0.7 / np.sqrt(rho)
+    L[1] = un
+    R[1] = un
+    L[2:4] = un * 0 + (-Bt1 / np.sqrt(rho)) * 1.0 + 0.0
+    R[2:4] = (-Bt2 / np.sqrt(rho))
+    err, F = po.numerical_flux(cfg, 0, 1.0, L, R)
+    assert err == 0
+    FL = _exact_flux(L, 0)
+    FR = _exact_flux(R, 0)
+    # both sides have the same physical flux (Rankine-Hugoniot for a wave at
+    # rest) and HLLD returns it
+    assert np.allclose(FL[[0, 1, 4, 8]], FR[[0, 1, 4, 8]], rtol=1e-13,
+                       atol=1e-13)
+    assert np.allclose(F[[0, 1, 2, 3, 5, 6, 8]], FL[[0, 1, 2, 3, 5, 6, 8]],
+                       rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "linde", "hlld"])
+def test_flux_axis_permutation(flux):
+    """flux_y(perm(L), perm(R)) is the permuted flux_x(L, R)
+    (rotate_state_vectors / rotate_flux_vector, :4482-4592)."""
+    cfg = po.make_config(flux=flux)
+    rng = np.random.default_rng(4)
+    # x->y relabelling: new (x,y,z) components = old (z,x,y)
+    vec = [0, 3, 1, 2, 6, 4, 5, 7]
+    for _ in range(20):
+        L = _rand_state(rng)
+        R = _rand_state(rng)
+        _, Fx = po.numerical_flux(cfg, 0, 1.0, L, R)
+        _, Fy = po.numerical_flux(cfg, 1, 1.0, L[vec], R[vec])
+        assert np.allclose(Fy[:9], Fx[vec + [8]], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "linde", "hlld"])
+def test_flux_mirror_symmetry(flux):
+    """Reflecting x -> -x swaps L and R and flips normal vector components;
+    the normal-direction flux of scalars changes sign."""
+    cfg = po.make_config(flux=flux)
+    rng = np.random.default_rng(5)
+    sign = np.array([1, -1, 1, 1, -1, 1, 1, 1.0])
+    fsgn = np.array([-1, 1, -1, -1, 1, -1, -1, -1, -1.0])
+    for _ in range(20):
+        L = _rand_state(rng)
+        R = _rand_state(rng)
+        _, F = po.numerical_flux(cfg, 0, 1.0, L, R)
+        _, G = po.numerical_flux(cfg, 0, 1.0, R * sign, L * sign)
+        sl = [0, 1, 2, 3, 4, 5, 6, 8]
+        assert np.allclose(G[sl], (F[:9] * fsgn)[sl], rtol=1e-12, atol=1e-13)
+        assert np.isclose(G[11], F[11], rtol=1e-14)
+
+
+def test_speeds_closed_form():
+    cfg = po.make_config()
+    S = np.array([2.0, 0.3, -0.1, 0.2, 0.5, -0.4, 0.1, 0.8])
+    err, cmax, cl, cr = po.speed_max(cfg, 0, S)
+    cs2 = GAMMA * 0.8 / 2.0
+    va2 = (0.25 + 0.16 + 0.01) / 2.0
+    van2 = 0.25 / 2.0
+    cf = np.sqrt(0.5 * (cs2 + va2 + np.sqrt((cs2 + va2) ** 2 - 4 * cs2 * van2)))
+    assert np.isclose(cmax, 0.3 + cf, rtol=1e-14)
+    assert np.isclose(cl, 0.3 - cf, rtol=1e-14)
+    assert np.isclose(cr, 0.3 + cf, rtol=1e-14)
+
+
+def test_energy_pressure_round_trip():
+    cfg = po.make_config()
+    S = np.array([1.3, 0.4, -0.2, 0.3, 0.5, 0.6, -0.7, 0.9])
+    e, back = po.energy_round_trip(cfg, S)
+    expect = 0.9 / (GAMMA - 1) + 0.5 * (0.16 + 0.04 + 0.09) / 1.3 + \
+        0.5 * (0.25 + 0.36 + 0.49)
+    assert np.isclose(e[7], expect, rtol=1e-15)
+    assert np.isclose(back[7], 0.9, rtol=1e-14)
+    assert np.array_equal(back[:7], S[:7])
+
+
+# ----------------------------------------------------------------------
+# whole-grid invariants
+@pytest.mark.parametrize("flux,lim", [("rusanov", "mc"), ("linde", "mc3"),
+                                      ("hlld", "mc3"), ("hlld", "minmod")])
+def test_oracle_conservation(flux, lim):
+    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
+    U0 = smooth_mhd_soa(grid, seed=7)
+    cfg = po.config_from_dict(dict(flux=flux, limiter=lim, beta=1.5), grid)
+    s = po.Session(grid, cfg, grid.soa_to_state_vgb(U0))
+    s.init_timestep()
+    V = np.prod(grid.CellSize_DB[0])
+
+    def totals(U):
+        E = U[7] / (GAMMA - 1) + 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / \
+            U[0] + 0.5 * (U[4] ** 2 + U[5] ** 2 + U[6] ** 2)
+        t = U.reshape(8, -1).sum(axis=1) * V
+        t[7] = E.sum() * V
+        return t
+    t0 = totals(U0)
+    for _ in range(5):
+        assert s.advance_step() == 0
+    t1 = totals(s.soa())
+    scale = np.abs(U0).reshape(8, -1).sum(axis=1) * V
+    scale[7] = t0[7]
+    assert (np.abs(t1 - t0) / scale).max() < 1e-13
+
+
+def test_oracle_uniform_state():
+    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
+    U0 = ics.uniform_state(grid, rho=1.2, u=(0.1, 0.2, -0.3),
+                           b=(0.4, -0.1, 0.2), p=0.9)
+    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3"), grid)
+    s = po.Session(grid, cfg, grid.soa_to_state_vgb(U0))
+    s.init_timestep()
+    for _ in range(3):
+        s.advance_step()
+    U = s.soa()
+    assert np.array_equal(U[:7], U0[:7])
+    assert np.allclose(U[7], U0[7], rtol=1e-14)
+
+
+def test_oracle_dt_uniform_closed_form():
+    """Uniform state: DtMax = V / sum_d A_d (|u_d| + c_f,d)
+    (src/ModTimeStepControl.f90:157-168)."""
+    grid = BlockGrid(nRoot=(1, 1, 1), nCell=(8, 8, 8))
+    u = (0.1, 0.2, -0.3)
+    b = (0.4, -0.1, 0.2)
+    U0 = ics.uniform_state(grid, rho=1.2, u=u, b=b, p=0.9)
+    cfg = po.config_from_dict(dict(flux="rusanov", limiter="mc"), grid)
+    s = po.Session(grid, cfg, grid.soa_to_state_vgb(U0))
+    s.init_timestep()
+    dx = grid.CellSize_DB[0]
+    rho, p = 1.2, 0.9
+    B = np.array(b)
+    vdt = 0.0
+    for d in range(3):
+        cs2 = GAMMA * p / rho
+        va2 = B @ B / rho
+        van2 = B[d] ** 2 / rho
+        cf = np.sqrt(0.5 * (cs2 + va2 + np.sqrt((cs2 + va2) ** 2 -
+                                                4 * cs2 * van2)))
+        area = np.prod(dx) / dx[d]
+        vdt += area * (abs(u[d]) + cf)
+    assert np.isclose(s.DtMax_B[0], np.prod(dx) / vdt, rtol=1e-13)
+    s.s.nStep = 1
+    dt = s.set_global_timestep()
+    assert np.isclose(dt, 0.8 * np.prod(dx) / vdt, rtol=1e-13)
+
+
+def test_first_step_dt_zero_oracle():
+    grid = BlockGrid(nRoot=(1, 1, 1), nCell=(8, 8, 8))
+    U0 = smooth_mhd_soa(grid)
+    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3"), grid)
+    s = po.Session(grid, cfg, grid.soa_to_state_vgb(U0))
+    assert s.advance_step(1.0) == 0
+    assert s.Dt == 0.0 and s.tSimulation == 0.0
+    assert np.array_equal(s.soa(), U0)
