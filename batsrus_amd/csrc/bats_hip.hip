@@ -1018,6 +1018,9 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   } else {
     a.Uin = h->U[1 - h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[h->cur];
   }
+  if (noUpdate) {  // face-speed pass on the current state only
+    a.Uin = h->U[h->stateBuf]; a.Uold = a.Uin; a.Uout = nullptr;
+  }
   a.nei = h->nei;
   a.cellsize = h->cellsize;
   a.scal = h->scal;

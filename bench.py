@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size "
-                         "(for scaling HBM counters)")
+                         "(1.07 GB, for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding "
@@ -145,16 +145,16 @@ def main():
         },
     }
     if args.copy_calibration:
-        import torch
-        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        x.fill_(1.0)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = {"read": x.numel() * 8,
-                                         "write": y.numel() * 8}
+        # a plain device-to-device copy of known size (the full state,
+        # 8 x 16.8M doubles) through the library's halo snapshot, for
+        # scaling the HBM byte counters
+        with Session(grid, device=0, **SCHEME) as s2:
+            s2.exchange_messages()
+            s2.synchronize()
+            s2.exchange_messages()
+            s2.synchronize()
+        out["copy_calibration_bytes"] = {"read": ncell * 64,
+                                         "write": ncell * 64}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     else:
