@@ -104,28 +104,29 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
-template <int DIR, int FLUX, bool FULL>
-__device__ __forceinline__ int face_flux(const Phys &ph, double area, int lim,
+template <int DIR, int FLUX, int LIM>
+__device__ __forceinline__ int face_flux(const Phys &ph, double area,
                                          double beta, const double *q0,
                                          const double *q1, const double *q2,
                                          const double *q3, double *F,
                                          double &vdt, double &unA) {
   double L[NVAR], R[NVAR];
-  face_states(lim, beta, q0, q1, q2, q3, L, R);
-  return numerical_flux<DIR, FLUX, FULL>(ph, area, L, R, F, vdt, unA);
+  face_states<LIM>(beta, q0, q1, q2, q3, L, R);
+  return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
 }
 
 // ---------------------------------------------------------------------
 // fused stage kernel
 // ---------------------------------------------------------------------
-template <int NB, int FLUX>
-__global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
+template <int NB, int FLUX, int LIM>
+__global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
-  constexpr int NP = NB + 4;           // padded plane edge
+  constexpr int NR = NB + 4;           // rows of the haloed plane
+  constexpr int NS = NB + 5;           // odd row stride: no LDS bank conflicts
   constexpr int NFX = (NB + 1) * NB;   // x (or y) faces per plane
   constexpr int NFS = 9;               // flux slots: 0..6, PP<-vdt, EN
-  __shared__ double sP[BPW * NVAR * NP * NP];
+  __shared__ double sP[BPW * NVAR * NR * NS];
   __shared__ double sF[BPW * 2 * NFS * NFX];
   __shared__ double sRed[kThreads / 64];
 
@@ -137,12 +138,11 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   const double *__restrict__ U = a.Uin;
   const long vs = a.vs;
   const Phys ph = a.ph;
-  const int lim = a.lim;
   const double beta = a.beta;
   int err = 0;
 
   auto sPi = [&](int l, int v, int jj, int ii) {
-    return ((l * NVAR + v) * NP + jj) * NP + ii;
+    return ((l * NVAR + v) * NR + jj) * NS + ii;
   };
   auto sFi = [&](int l, int dir, int v, int q) {
     return ((l * 2 + dir) * NFS + v) * NFX + q;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   const double DtUnscaled = a.scal[S_DTCB];
   UpdateCtl uc;
   uc.DtLocal = a.DtFactor * DtUnscaled;
-  uc.CellVolume = CellVolume;
+  uc.InvVolume = 1.0 / CellVolume;
   uc.halfStep = a.halfStep;
   uc.iStage = a.iStage;
   uc.nStage = a.nStage;
@@ -164,7 +164,10 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
   uc.dtZero = (a.scal[S_DT] == 0.0);
   uc.roundTripOld = a.roundTripOld;
 
-  double Q0[NVAR], Q1[NVAR], Q2[NVAR], Q3[NVAR];
+  // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
+  // left-state slope dLk of cell k; each cell's slopes are computed once and
+  // serve the faces below and above it
+  double Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
   double FzLo[NFS], FzHi[NFS];
   double dtmin = 1.7976931348623157e308;
 
@@ -173,23 +176,39 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
     load_state(U, vs, off, q);
     to_primitive(q);
   };
+  // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
   auto zface = [&](double *Fz) {
-    double F[NFLUX], vdt, unA;
-    err |= face_flux<2, FLUX, false>(ph, AreaZ, lim, beta, Q0, Q1, Q2, Q3, F,
-                                     vdt, unA);
+    double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) {
+      double dl, dr;
+      slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
+      L[v] = Q1[v] + dLk[v];
+      R[v] = Q2[v] - dr;
+      dLk[v] = dl;
+    }
+    err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
 #pragma unroll
     for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
     Fz[PP] = vdt;
   };
 
   if (act) {
-    load_prim(-2, Q0);
+    // prologue: slopes of cell -1 and the face below cell 0
+    double Pm2[NVAR];
+    load_prim(-2, Pm2);
     load_prim(-1, Q1);
     load_prim(0, Q2);
     load_prim(1, Q3);
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) {
+      double dl, dr;
+      slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
+      dLk[v] = dl;
+    }
     zface(FzLo);
 #pragma unroll
-    for (int v = 0; v < NVAR; v++) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+    for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
   }
 
   for (int k = 0; k < NB; k++) {
@@ -243,8 +262,8 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
           c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
           c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
         }
-        err |= face_flux<0, FLUX, false>(ph, dyf * dzf, lim, beta, c0, c1, c2,
-                                         c3, F, vdt, unA);
+        err |= face_flux<0, FLUX, LIM>(ph, dyf * dzf, beta, c0, c1, c2, c3, F,
+                                       vdt, unA);
       } else {
 #pragma unroll
         for (int v = 0; v < NVAR; v++) {
@@ -253,8 +272,8 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
           c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
           c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
         }
-        err |= face_flux<1, FLUX, false>(ph, dxf * dzf, lim, beta, c0, c1, c2,
-                                         c3, F, vdt, unA);
+        err |= face_flux<1, FLUX, LIM>(ph, dxf * dzf, beta, c0, c1, c2, c3, F,
+                                       vdt, unA);
       }
       F[PP] = vdt;
 #pragma unroll
@@ -292,7 +311,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(StageArgs a) {
       }
       // shift the z window
 #pragma unroll
-      for (int v = 0; v < NVAR; v++) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+      for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
 #pragma unroll
       for (int v = 0; v < NFS; v++) FzLo[v] = FzHi[v];
     }
@@ -369,7 +388,7 @@ __global__ void k_block_face_value(BlockArgs a) {
     to_primitive(q[s]);
   }
   double Lv[NVAR], Rv[NVAR];
-  face_states(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
+  face_states_rt(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
   for (int v = 0; v < NVAR; v++) {
     L[fo * NVAR + v] = Lv[v];
     R[fo * NVAR + v] = Rv[v];
@@ -427,7 +446,7 @@ __global__ void k_block_update(BlockArgs a) {
   const double *cs = a.cellsize + 3 * a.iBlock;
   UpdateCtl u;
   u.DtLocal = a.DtFactor * a.scal[S_DTCB];
-  u.CellVolume = cs[0] * cs[1] * cs[2];
+  u.InvVolume = 1.0 / (cs[0] * cs[1] * cs[2]);
   u.halfStep = a.halfStep;
   u.iStage = a.iStage;
   u.nStage = a.nStage;
@@ -1056,17 +1075,28 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
     }
     HIPCHK(hipEventRecord(h->evStart[ev], h->stream));
   }
-#define LAUNCH(NBv, FL)                                                    \
-  hipLaunchKernelGGL((k_stage<NBv, FL>), grid, dim3(kThreads), 0, h->stream, a)
-  if (NB == 8) {
-    if (c.iFlux == BATS_FLUX_RUSANOV) LAUNCH(8, FLUX_RUSANOV);
-    else if (c.iFlux == BATS_FLUX_LINDE) LAUNCH(8, FLUX_LINDE);
-    else LAUNCH(8, FLUX_HLLD);
-  } else {
-    if (c.iFlux == BATS_FLUX_RUSANOV) LAUNCH(16, FLUX_RUSANOV);
-    else if (c.iFlux == BATS_FLUX_LINDE) LAUNCH(16, FLUX_LINDE);
-    else LAUNCH(16, FLUX_HLLD);
+#define LAUNCH(NBv, FL, LM)                                                \
+  hipLaunchKernelGGL((k_stage<NBv, FL, LM>), grid, dim3(kThreads), 0,        \
+                     h->stream, a)
+#define LAUNCH_LIM(NBv, FL)                                                \
+  switch (a.lim) {                                                         \
+  case LIM_MINMOD: LAUNCH(NBv, FL, LIM_MINMOD); break;                     \
+  case LIM_MC: LAUNCH(NBv, FL, LIM_MC); break;                             \
+  case LIM_MC3: LAUNCH(NBv, FL, LIM_MC3); break;                           \
+  case LIM_BETA: LAUNCH(NBv, FL, LIM_BETA); break;                         \
+  case LIM_NO: LAUNCH(NBv, FL, LIM_NO); break;                             \
+  default: LAUNCH(NBv, FL, LIM_FIRST); break;                              \
   }
+  if (NB == 8) {
+    if (c.iFlux == BATS_FLUX_RUSANOV) { LAUNCH_LIM(8, FLUX_RUSANOV); }
+    else if (c.iFlux == BATS_FLUX_LINDE) { LAUNCH_LIM(8, FLUX_LINDE); }
+    else { LAUNCH_LIM(8, FLUX_HLLD); }
+  } else {
+    if (c.iFlux == BATS_FLUX_RUSANOV) { LAUNCH_LIM(16, FLUX_RUSANOV); }
+    else if (c.iFlux == BATS_FLUX_LINDE) { LAUNCH_LIM(16, FLUX_LINDE); }
+    else { LAUNCH_LIM(16, FLUX_HLLD); }
+  }
+#undef LAUNCH_LIM
 #undef LAUNCH
   HIPCHK(hipGetLastError());
   if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));

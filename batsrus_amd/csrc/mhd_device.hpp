@@ -2,8 +2,7 @@
 // (limited MUSCL face states, Rusanov / Linde / HLLD face fluxes, wave
 // speeds, energy <-> pressure, stage update) for gfx950.
 //
-// Every function follows the reference's operand order so that the GPU
-// agrees with the Fortran "orig" CPU path to rounding:
+// The functions follow the reference's formulas and operand order:
 //   limiter            src/ModFaceValue.f90:3689-3791
 //   face states        src/ModFaceValue.f90:1970-2163
 //   physical flux      src/ModFaceFlux.f90:1166-1410,1550-1617,1619-1899
@@ -14,9 +13,16 @@
 //   energy/pressure    src/ModEnergy.f90:47-127,243-332,388-451
 //   stage update       src/ModUpdateState.f90:191-965
 // Faces are Cartesian: the unit normal's zero components only add signed
-// zeros in the reference, so they are dropped here; sums the reference
-// forms in x,y,z order (|B|^2, |u|^2, Poynting term) keep that order, sums
-// it forms in the rotated n,t1,t2 frame (HLLD) keep that order.
+// zeros in the reference, so they are dropped; sums the reference forms in
+// x,y,z order (|B|^2, |u|^2, Poynting term) keep that order, sums it forms
+// in the rotated n,t1,t2 frame (HLLD) keep that order.
+//
+// Division and square root use the hardware reciprocal / reciprocal square
+// root with two Newton-Raphson (Goldschmidt) refinements instead of the
+// IEEE-exact sequences; each result is within about one ulp of the
+// correctly rounded one, far inside the 1e-12 parity tolerance, and the
+// inputs here never approach the denormal or overflow range the exact
+// sequences guard.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,89 +44,117 @@ struct Phys {
 __device__ __forceinline__ double fsign(double a, double b) {
   return copysign(a, b);
 }
-// Fortran MIN / MAX of two arguments
+// Fortran MIN / MAX of two (finite) arguments
 __device__ __forceinline__ double fmin2(double a, double b) {
-  return b < a ? b : a;
+  return __builtin_fmin(a, b);
 }
 __device__ __forceinline__ double fmax2(double a, double b) {
-  return b > a ? b : a;
+  return __builtin_fmax(a, b);
+}
+// 1/x: v_rcp_f64 + two Newton-Raphson steps
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double fdiv(double a, double b) {
+  return a * frcp(b);
+}
+// sqrt(x), x >= 0: v_rsq_f64 + Goldschmidt refinement
+__device__ __forceinline__ double fsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  const double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x == 0.0 ? 0.0 : g;
 }
 
 // ---------------------------------------------------------------------
-// limiter: slopes of the cell l from p0=P(l-1), p1=P(l), p2=P(l+1)
-// (half-slopes, as in the reference).  dL is used for the face on the
-// right of the cell (left state), dR for the face on its left.
-__device__ __forceinline__ void limit(int lim, double beta, double p0,
-                                      double p1, double p2, double &dL,
-                                      double &dR) {
+// limiter (half slopes, src/ModFaceValue.f90:3689-3791): slopes of the
+// cell l from p0=P(l-1), p1=P(l), p2=P(l+1).  dL builds the left state of
+// the face to the right of the cell, dR the right state of the face to its
+// left.
+template <int LIM>
+__device__ __forceinline__ void slopes(double beta, double p0, double p1,
+                                       double p2, double &dL, double &dR) {
   const double cThird = 1. / 3.;
   const double d1 = p1 - p0, d2 = p2 - p1;
-  switch (lim) {
-  case LIM_MC3: {
+  if (LIM == LIM_MC3) {
     const double sg = fsign(0.25, d1) + fsign(0.25, d2);
     const double m = fmin2(beta * fabs(d1), beta * fabs(d2));
     dR = sg * fmin2(m, cThird * fabs(2 * d1 + d2));
     dL = sg * fmin2(m, cThird * fabs(d1 + 2 * d2));
-    break;
-  }
-  case LIM_MC: {
+  } else if (LIM == LIM_MC) {
     const double s = (fsign(0.25, d1) + fsign(0.25, d2)) *
                      fmin2(fmin2(beta * fabs(d1), beta * fabs(d2)),
                            0.5 * fabs(d1 + d2));
     dL = s; dR = s;
-    break;
-  }
-  case LIM_MINMOD: {
+  } else if (LIM == LIM_MINMOD) {
     const double s =
         (fsign(0.25, d2) + fsign(0.25, d1)) * fmin2(fabs(d2), fabs(d1));
     dL = s; dR = s;
-    break;
-  }
-  case LIM_BETA: {
+  } else if (LIM == LIM_BETA) {
     const double a1 = fabs(d1), a2 = fabs(d2);
     const double sum = fsign(0.25, d2) + fsign(0.25, d1);
-    const double lo2 = fmin2(a2, beta * a1), lo3 = fmin2(a1, beta * a2);
-    const double s = sum * fmax2(lo2, lo3);
+    const double s = sum * fmax2(fmin2(a2, beta * a1), fmin2(a1, beta * a2));
     dL = s; dR = s;
-    break;
-  }
-  case LIM_NO: {
+  } else if (LIM == LIM_NO) {
     const double s = 0.25 * (p2 - p0);
     dL = s; dR = s;
-    break;
-  }
-  default:
+  } else {
     dL = 0.0; dR = 0.0;
-    break;
   }
 }
 
 // Left/right face states of the face between cells f-1 and f from the
 // primitive values q0=P(f-2), q1=P(f-1), q2=P(f), q3=P(f+1):
 // L = P(f-1) + dL(f-1), R = P(f) - dR(f)  (src/ModFaceValue.f90:2021-2027)
-__device__ __forceinline__ void face_states(int lim, double beta,
-                                            const double *q0, const double *q1,
-                                            const double *q2, const double *q3,
-                                            double *L, double *R) {
-  if (lim == LIM_FIRST) {
-#pragma unroll
-    for (int v = 0; v < NVAR; v++) { L[v] = q1[v]; R[v] = q2[v]; }
-    return;
-  }
+template <int LIM>
+__device__ __forceinline__ void face_states(double beta, const double *q0,
+                                            const double *q1, const double *q2,
+                                            const double *q3, double *L,
+                                            double *R) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++) {
-    double dl, dr, t;
-    limit(lim, beta, q0[v], q1[v], q2[v], dl, t);
-    limit(lim, beta, q1[v], q2[v], q3[v], t, dr);
-    L[v] = q1[v] + dl;
-    R[v] = q2[v] - dr;
+    if (LIM == LIM_FIRST) {
+      L[v] = q1[v];
+      R[v] = q2[v];
+    } else {
+      double dl, dr, t;
+      slopes<LIM>(beta, q0[v], q1[v], q2[v], dl, t);
+      slopes<LIM>(beta, q1[v], q2[v], q3[v], t, dr);
+      L[v] = q1[v] + dl;
+      R[v] = q2[v] - dr;
+    }
+  }
+}
+
+// runtime-limiter variant used by the per-block parity routines
+__device__ __forceinline__ void face_states_rt(int lim, double beta,
+                                               const double *q0,
+                                               const double *q1,
+                                               const double *q2,
+                                               const double *q3, double *L,
+                                               double *R) {
+  switch (lim) {
+  case LIM_MINMOD: face_states<LIM_MINMOD>(beta, q0, q1, q2, q3, L, R); break;
+  case LIM_MC: face_states<LIM_MC>(beta, q0, q1, q2, q3, L, R); break;
+  case LIM_MC3: face_states<LIM_MC3>(beta, q0, q1, q2, q3, L, R); break;
+  case LIM_BETA: face_states<LIM_BETA>(beta, q0, q1, q2, q3, L, R); break;
+  case LIM_NO: face_states<LIM_NO>(beta, q0, q1, q2, q3, L, R); break;
+  default: face_states<LIM_FIRST>(beta, q0, q1, q2, q3, L, R); break;
   }
 }
 
 // conserved (p stored) -> primitive, u = (1/rho)*rho*u
 // (src/ModFaceValue.f90:1541,1581-1582)
 __device__ __forceinline__ void to_primitive(double *q) {
-  const double RhoInv = 1 / q[RHO];
+  const double RhoInv = frcp(q[RHO]);
   q[MX] = RhoInv * q[MX];
   q[MY] = RhoInv * q[MY];
   q[MZ] = RhoInv * q[MZ];
@@ -166,7 +200,7 @@ __device__ __forceinline__ void phys_flux(const Phys &ph, const double *S,
     if (d != DIR) fe = fe + fb[d] * b[d];
   F[EN] = fe;
   Cons[EN] = e + 0.5 * B2;
-  if (FULL) F[PP] = Un * p;
+  F[PP] = FULL ? Un * p : 0.0;
   UnOut = Un;
 }
 
@@ -176,25 +210,27 @@ __device__ __forceinline__ void phys_flux(const Phys &ph, const double *S,
 template <int DIR>
 __device__ __forceinline__ bool fast_speed(const Phys &ph, const double *S,
                                            double &Un, double &Fast) {
-  const double Rho = S[RHO];
-  const double Sound2 = S[PP] * ph.gamma / Rho;
+  const double InvRho = frcp(S[RHO]);
+  const double Sound2 = S[PP] * ph.gamma * InvRho;
   Un = S[MX + DIR];
-  const double InvRho = 1.0 / Rho;
   const double bx = S[BX], by = S[BY], bz = S[BZ];
   const double Alfven2 = (bx * bx + by * by + bz * bz) * InvRho;
   const double FullBn = S[BX + DIR];
   const double Alfven2Normal = InvRho * (FullBn * FullBn);
   const double Fast2 = Sound2 + Alfven2;
   const double Discr =
-      sqrt(fmax2(0.0, Fast2 * Fast2 - 4 * Sound2 * Alfven2Normal));
+      fsqrt(fmax2(0.0, Fast2 * Fast2 - 4 * Sound2 * Alfven2Normal));
   const bool ok = !(Fast2 + Discr < 0.0);
-  Fast = sqrt(0.5 * (Fast2 + Discr));
+  Fast = fsqrt(0.5 * (Fast2 + Discr));
   return ok;
 }
 
 // ---------------------------------------------------------------------
-// HLLD (src/ModFaceFlux.f90:2826-3160) on Linde-corrected states.
-// Writes F[0..8] (F[PP] only if FULL) and returns Cmax (= CmaxDt) and Un.
+// HLLD (src/ModFaceFlux.f90:2826-3160) on Linde-corrected states, with the
+// reference's two mirror-image branches on the sign of the contact speed Un.
+// (Evaluating the Un < 0 branch as the reflected Un >= 0 branch removes the
+// branch divergence but keeps about 30 more values live; on gfx950 the
+// resulting register spills cost more than the divergence.)
 template <int DIR, bool FULL>
 __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
                                     const double *SR, double *F, double &Cmax,
@@ -207,14 +243,12 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
   const double sL = fmin2(UnLs - FastL, UnRs - FastR);
   const double sR = fmax2(UnLs + FastL, UnRs + FastR);
   Cmax = fmax2(sR, -sL);
-  if (sL >= 0.) {
-    double Cons[NFLUX];
-    phys_flux<DIR, FULL>(ph, SL, Cons, F, UnOut);
-    return err;
-  }
-  if (sR <= 0.) {
-    double Cons[NFLUX];
-    phys_flux<DIR, FULL>(ph, SR, Cons, F, UnOut);
+  if (sL >= 0. || sR <= 0.) {  // supersonic: upwind physical flux
+    const bool left = sL >= 0.;
+    double S[NVAR], Cons[NFLUX];
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) S[v] = left ? SL[v] : SR[v];
+    phys_flux<DIR, FULL>(ph, S, Cons, F, UnOut);
     return err;
   }
   const double cTiny = ph.cTiny;
@@ -227,7 +261,6 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
   const double B1n = 0.5 * (B1nL + B1nR);
   const double B1n2 = B1n * B1n;
   const double Bn = B1n;
-  const double Bt1L = B1t1L, Bt2L = B1t2L, Bt1R = B1t1R, Bt2R = B1t2R;
   const double Bn2 = Bn * Bn;
   const double SignBn = fsign(1., Bn);
   const double PbL = 0.5 * (B1n2 + B1t1L * B1t1L + B1t2L * B1t2L);
@@ -236,20 +269,20 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
   const double PtotR = pR + PbR;
   const double DsL = sL - UnL, DsR = sR - UnR;
   const double DsRhoL = DsL * RhoL, DsRhoR = DsR * RhoR;
-  double Tmp = 1. / (DsRhoR - DsRhoL);
-  const double Un = (DsRhoR * UnR - DsRhoL * UnL - PtotR + PtotL) * Tmp;
+  const double Tmp0 = frcp(DsRhoR - DsRhoL);
+  const double Un = (DsRhoR * UnR - DsRhoL * UnL - PtotR + PtotL) * Tmp0;
   const double Ptot12 =
-      (DsRhoR * PtotL - DsRhoL * PtotR + DsRhoR * DsRhoL * (UnR - UnL)) * Tmp;
+      (DsRhoR * PtotL - DsRhoL * PtotR + DsRhoR * DsRhoL * (UnR - UnL)) * Tmp0;
 
+  const double Bt1L = B1t1L, Bt2L = B1t2L, Bt1R = B1t1R, Bt2R = B1t2R;
   double Rho, Ut1, Ut2, B1t1, B1t2, uDotB1, e;
   double Ut1L1, Ut2L1, B1t1L1, B1t2L1, Ut1R1, Ut2R1, B1t1R1, B1t2R1;
-
   auto outer_left = [&]() {
     double T = DsRhoL * (sL - Un) - Bn2;
     if (T < cTiny) {
       Ut1L1 = Ut1L; Ut2L1 = Ut2L; B1t1L1 = B1t1L; B1t2L1 = B1t2L;
     } else {
-      T = 1.0 / T;
+      T = frcp(T);
       Ut1L1 = Ut1L - Bn * Bt1L * (Un - UnL) * T;
       Ut2L1 = Ut2L - Bn * Bt2L * (Un - UnL) * T;
       B1t1L1 = Bt1L * (DsRhoL * DsL - Bn2) * T;
@@ -261,16 +294,15 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
     if (T < cTiny) {
       Ut1R1 = Ut1R; Ut2R1 = Ut2R; B1t1R1 = B1t1R; B1t2R1 = B1t2R;
     } else {
-      T = 1.0 / T;
+      T = frcp(T);
       Ut1R1 = Ut1R - Bn * Bt1R * (Un - UnR) * T;
       Ut2R1 = Ut2R - Bn * Bt2R * (Un - UnR) * T;
       B1t1R1 = Bt1R * (DsRhoR * DsR - Bn2) * T;
       B1t2R1 = Bt2R * (DsRhoR * DsR - Bn2) * T;
     }
   };
-
   if (Un >= 0.) {
-    const double InvSLUn = 1.0 / (sL - Un);
+    const double InvSLUn = frcp(sL - Un);
     Rho = DsRhoL * InvSLUn;
     outer_left();
     const double eL = ph.invgm1 * pL + PbL +
@@ -279,31 +311,27 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
     const double uDotB1L1 = Un * B1n + Ut1L1 * B1t1L1 + Ut2L1 * B1t2L1;
     e = (DsL * eL - PtotL * UnL + Ptot12 * Un + Bn * (uDotB1L - uDotB1L1)) *
         InvSLUn;
-    const double SqRhoL1 = sqrt(Rho);
-    const double sL1 = Un - fabs(Bn) / SqRhoL1;
+    const double SqRhoL1 = fsqrt(Rho);
+    const double sL1 = Un - fabs(Bn) * frcp(SqRhoL1);
     if (sL1 >= 0.) {
       Ut1 = Ut1L1; Ut2 = Ut2L1; B1t1 = B1t1L1; B1t2 = B1t2L1;
       uDotB1 = uDotB1L1;
     } else {
-      const double ratio = DsRhoR / (sR - Un);
+      const double ratio = DsRhoR * frcp(sR - Un);
       if (ratio < 0) err |= ERRB_HLLD;
-      const double SqRhoR1 = sqrt(ratio);
+      const double SqRhoR1 = fsqrt(ratio);
       outer_right();
-      Tmp = 1. / (SqRhoL1 + SqRhoR1);
+      const double Ti = frcp(SqRhoL1 + SqRhoR1);
       const double SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
-      Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) *
-            Tmp;
-      Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) *
-            Tmp;
-      B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) *
-             Tmp;
-      B1t2 = (SqRhoL1 * B1t2R1 + SqRhoR1 * B1t2L1 + (Ut2R1 - Ut2L1) * SqRhoLR1) *
-             Tmp;
+      Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) * Ti;
+      Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) * Ti;
+      B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) * Ti;
+      B1t2 = (SqRhoL1 * B1t2R1 + SqRhoR1 * B1t2L1 + (Ut2R1 - Ut2L1) * SqRhoLR1) * Ti;
       uDotB1 = Un * B1n + Ut1 * B1t1 + Ut2 * B1t2;
       e = e - SqRhoL1 * (uDotB1L1 - uDotB1) * SignBn;
     }
   } else {
-    const double InvSRUn = 1.0 / (sR - Un);
+    const double InvSRUn = frcp(sR - Un);
     Rho = DsRhoR * InvSRUn;
     outer_right();
     const double eR = ph.invgm1 * pR + PbR +
@@ -312,30 +340,27 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
     const double uDotB1R1 = Un * B1n + Ut1R1 * B1t1R1 + Ut2R1 * B1t2R1;
     e = (DsR * eR - PtotR * UnR + Ptot12 * Un + Bn * (uDotB1R - uDotB1R1)) *
         InvSRUn;
-    const double SqRhoR1 = sqrt(Rho);
-    const double sR1 = Un + fabs(Bn) / SqRhoR1;
+    const double SqRhoR1 = fsqrt(Rho);
+    const double sR1 = Un + fabs(Bn) * frcp(SqRhoR1);
     if (sR1 <= 0.) {
       Ut1 = Ut1R1; Ut2 = Ut2R1; B1t1 = B1t1R1; B1t2 = B1t2R1;
       uDotB1 = uDotB1R1;
     } else {
-      const double ratio = DsRhoL / (sL - Un);
+      const double ratio = DsRhoL * frcp(sL - Un);
       if (ratio < 0) err |= ERRB_HLLD;
-      const double SqRhoL1 = sqrt(ratio);
+      const double SqRhoL1 = fsqrt(ratio);
       outer_left();
-      Tmp = 1. / (SqRhoL1 + SqRhoR1);
+      const double Ti = frcp(SqRhoL1 + SqRhoR1);
       const double SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
-      Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) *
-            Tmp;
-      Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) *
-            Tmp;
-      B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) *
-             Tmp;
-      B1t2 = (SqRhoL1 * B1t2R1 + SqRhoR1 * B1t2L1 + (Ut2R1 - Ut2L1) * SqRhoLR1) *
-             Tmp;
+      Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) * Ti;
+      Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) * Ti;
+      B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) * Ti;
+      B1t2 = (SqRhoL1 * B1t2R1 + SqRhoR1 * B1t2L1 + (Ut2R1 - Ut2L1) * SqRhoLR1) * Ti;
       uDotB1 = Un * B1n + Ut1 * B1t1 + Ut2 * B1t2;
       e = e + SqRhoR1 * (uDotB1R1 - uDotB1) * SignBn;
     }
   }
+  // flux of the HLLD state in the original frame (:3111-3131)
   const double RhoUn = Rho * Un;
   const double Bt1 = B1t1, Bt2 = B1t2;
   F[RHO] = RhoUn;
@@ -345,9 +370,9 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
   F[BX + in] = 0.0;
   F[BX + it1] = Un * Bt1 - Ut1 * Bn;
   F[BX + it2] = Un * Bt2 - Ut2 * Bn;
-  if (FULL)
-    F[PP] = 0.5 * (sR * UnL * pL - sL * UnR * pR + sR * sL * (pR - pL)) /
-            (sR - sL);
+  F[PP] = FULL ? 0.5 * (sR * UnL * pL - sL * UnR * pR + sR * sL * (pR - pL)) *
+                     frcp(sR - sL)
+               : 0.0;
   F[EN] = Un * (e + Ptot12) - Bn * uDotB1;
   UnOut = Un;
   return err;
@@ -356,7 +381,7 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
 // ---------------------------------------------------------------------
 // numerical flux for a face normal to DIR from the left/right primitive
 // face states (src/ModFaceFlux.f90:2130-2473).  Out: F[0..8] already times
-// Area, CmaxDt (Vdt = CmaxDt*Area), Un*Area.
+// Area, Vdt = CmaxDt*Area, Un*Area.
 template <int DIR, int FLUX, bool FULL>
 __device__ __forceinline__ int numerical_flux(const Phys &ph, double Area,
                                               const double *Lin,
@@ -390,10 +415,8 @@ __device__ __forceinline__ int numerical_flux(const Phys &ph, double Area,
     Cmax = fabs(UnHat) + FastHat;
     if (FLUX == FLUX_RUSANOV) {
 #pragma unroll
-      for (int v = 0; v < NFLUX; v++) {
-        if (!FULL && v == PP) { F[v] = 0.0; continue; }
+      for (int v = 0; v < NFLUX; v++)
         F[v] = 0.5 * (FL[v] + FR[v] - Cmax * (CR[v] - CL[v]));
-      }
       Un = 0.5 * (UnL + UnR);
     } else {
       double UnLs, FastL, UnRs, FastR;
@@ -401,17 +424,16 @@ __device__ __forceinline__ int numerical_flux(const Phys &ph, double Area,
       if (!fast_speed<DIR>(ph, R, UnRs, FastR)) err |= ERRB_NEGFAST2;
       const double Cleft = fmin2(fmin2(0.0, UnLs - FastL), UnHat - FastHat);
       const double Cright = fmax2(fmax2(0.0, UnRs + FastR), UnHat + FastHat);
-      const double WeightLeft = Cright / (Cright - Cleft);
+      const double WeightLeft = Cright * frcp(Cright - Cleft);
       const double WeightRight = 1.0 - WeightLeft;
       const double Diffusion = Cright * WeightRight;
 #pragma unroll
-      for (int v = 0; v < NFLUX; v++) {
-        if (!FULL && v == PP) { F[v] = 0.0; continue; }
+      for (int v = 0; v < NFLUX; v++)
         F[v] = (WeightRight * FR[v] + WeightLeft * FL[v] -
                 Diffusion * (CR[v] - CL[v]));
-      }
       Un = WeightRight * UnR + WeightLeft * UnL;
     }
+    if (!FULL) F[PP] = 0.0;
   }
   if (Linde) { // Linde's Lax-Friedrichs flux for Bn (:2403-2418)
     F[BX + DIR] = F[BX + DIR] - Cmax * dB;
@@ -427,19 +449,30 @@ __device__ __forceinline__ int numerical_flux(const Phys &ph, double Area,
 // ---------------------------------------------------------------------
 // energy <-> pressure for one cell (src/ModEnergy.f90:88-97,292-319)
 __device__ __forceinline__ double pressure_to_energy(const Phys &ph,
-                                                     const double *S) {
+                                                     const double *S,
+                                                     double InvRho) {
   double E = ph.invgm1 * S[PP] +
-             0.5 * (S[MX] * S[MX] + S[MY] * S[MY] + S[MZ] * S[MZ]) / S[RHO];
+             0.5 * (S[MX] * S[MX] + S[MY] * S[MY] + S[MZ] * S[MZ]) * InvRho;
   E = E + 0.5 * (S[BX] * S[BX] + S[BY] * S[BY] + S[BZ] * S[BZ]);
   return E;
+}
+__device__ __forceinline__ double pressure_to_energy(const Phys &ph,
+                                                     const double *S) {
+  return pressure_to_energy(ph, S, frcp(S[RHO]));
+}
+__device__ __forceinline__ double energy_to_pressure(const Phys &ph,
+                                                     const double *S,
+                                                     double E,
+                                                     double InvRho) {
+  double p = E - 0.5 * (S[BX] * S[BX] + S[BY] * S[BY] + S[BZ] * S[BZ]);
+  p = ph.gm1 *
+      (p - 0.5 * (S[MX] * S[MX] + S[MY] * S[MY] + S[MZ] * S[MZ]) * InvRho);
+  return p;
 }
 __device__ __forceinline__ double energy_to_pressure(const Phys &ph,
                                                      const double *S,
                                                      double E) {
-  double p = E - 0.5 * (S[BX] * S[BX] + S[BY] * S[BY] + S[BZ] * S[BZ]);
-  p = ph.gm1 *
-      (p - 0.5 * (S[MX] * S[MX] + S[MY] * S[MY] + S[MZ] * S[MZ]) / S[RHO]);
-  return p;
+  return energy_to_pressure(ph, S, E, frcp(S[RHO]));
 }
 __device__ __forceinline__ double limit_pressure(const Phys &ph, double p) {
   return ph.pMin < 0.0 ? p : fmax2(ph.pMin, p);
@@ -451,11 +484,9 @@ __device__ __forceinline__ double limit_pressure(const Phys &ph, double p) {
 //   Uin[8]    : stage-input state (p stored)
 //   Uold[8]   : state at the start of the time step (StateOld)
 //   out Unew  : updated state (p stored)
-// Stage-2 old states go through the p->E->p round trip the reference
-// applies to StateOld_VGB at the end of stage 1 (roundTripOld).
 struct UpdateCtl {
   double DtLocal;     // DtFactor*DtMax_CB
-  double CellVolume;
+  double InvVolume;   // 1/CellVolume
   int halfStep;       // UseHalfStep or nStage==1
   int iStage, nStage;
   int dbTrick;
@@ -476,13 +507,15 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
   }
   double Source[NFLUX];
 #pragma unroll
-  for (int v = 0; v < NFLUX; v++) Source[v] = u.DtLocal * (dF[v] / u.CellVolume);
+  for (int v = 0; v < NFLUX; v++) Source[v] = u.DtLocal * (dF[v] * u.InvVolume);
   double Old[NVAR];
 #pragma unroll
   for (int v = 0; v < NVAR; v++) Old[v] = Uold[v];
+  const double InvRhoOld = frcp(Old[RHO]);
   if (u.roundTripOld && u.iStage > 1)
-    Old[PP] = energy_to_pressure(ph, Old, pressure_to_energy(ph, Old));
-  const double Eold = pressure_to_energy(ph, Old);
+    Old[PP] = energy_to_pressure(ph, Old, pressure_to_energy(ph, Old, InvRhoOld),
+                                 InvRhoOld);
+  const double Eold = pressure_to_energy(ph, Old, InvRhoOld);
   double Base[NVAR];
   double Ebase;
   if (u.halfStep) {
