@@ -6,6 +6,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -13,9 +14,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
 
 # gfx950 only; packed FP32 instructions are disabled for all device code.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
+
+# the fused stage kernel, one object per (block size, face flux)
+STAGE_UNITS = [(nb, flux) for nb in (8, 16)
+               for flux in ("rusanov", "linde", "hlld")]
 
 
 def _run(cmd, cwd=ROOT):
@@ -27,12 +32,30 @@ def _run(cmd, cwd=ROOT):
     return r.stdout
 
 
-def build_hip():
+def build_hip(jobs=None):
+    """Compile every HIP translation unit for gfx950 (in parallel) and link
+    batsrus_amd/libbats_hip.so."""
+    csrc = os.path.join(HERE, "csrc")
+    odir = os.path.join(csrc, "_build")
+    os.makedirs(odir, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + csrc]
+    units = [(os.path.join(csrc, "bats_hip.hip"),
+              os.path.join(odir, "bats_hip.o"), [])]
+    for nb, flux in STAGE_UNITS:
+        units.append((os.path.join(csrc, "stage_inst.hip"),
+                      os.path.join(odir, "stage_%d_%s.o" % (nb, flux)),
+                      ["-DBATS_NB=%d" % nb, "-DBATS_FLUX=%s" % flux]))
+    jobs = jobs or min(len(units), os.cpu_count() or 1, 16)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_run, [HIPCC] + HIP_FLAGS + inc + defs +
+                          ["-c", src, "-o", obj])
+                for src, obj, defs in units]
+        for f in futs:
+            f.result()
     out = os.path.join(HERE, "libbats_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-I" + os.path.join(ROOT, "include"),
-                                "-o", tmp,
-                                os.path.join(HERE, "csrc", "bats_hip.hip")])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] +
+         [obj for _, obj, _ in units])
     os.replace(tmp, out)
     return out
 

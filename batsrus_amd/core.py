@@ -54,7 +54,8 @@ class BatsConfig(C.Structure):
                 ("Cfl", C.c_double), ("Gamma", C.c_double),
                 ("pMin", C.c_double), ("UseDbTrick", C.c_int),
                 ("UseDtFixed", C.c_int), ("DtFixed", C.c_double),
-                ("cTiny", C.c_double), ("iDevice", C.c_int)]
+                ("cTiny", C.c_double), ("iDevice", C.c_int),
+                ("UseDivbSource", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -74,6 +75,7 @@ ABI_FUNCTIONS = {
     "bats_set_stage": (C.c_int, [_hp, C.c_int]),
     "bats_calc_face_value": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int]),
     "bats_calc_face_flux": (C.c_int, [_hp, C.c_int, C.c_int]),
+    "bats_calc_source": (C.c_int, [_hp, C.c_int]),
     "bats_update_state": (C.c_int, [_hp, C.c_int]),
     "bats_calc_timestep": (C.c_int, [_hp, C.c_int, C.c_int]),
     "bats_get_face_values": (C.c_int, [_hp] + [_dp] * 6),
@@ -128,7 +130,7 @@ def _ptr(a, ct=C.c_double):
 SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        nStage=2, UseHalfStep=True, cfl=0.8, gamma=5.0 / 3.0,
                        pMin=-1.0, UseDbTrick=False, UseDtFixed=False,
-                       DtFixed=0.0, cTiny=1e-6)
+                       DtFixed=0.0, cTiny=1e-6, UseDivbSource=False)
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):
@@ -153,6 +155,7 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.DtFixed = s["DtFixed"]
     c.cTiny = s["cTiny"]
     c.iDevice = device
+    c.UseDivbSource = int(s["UseDivbSource"])
     return c
 
 
@@ -239,6 +242,9 @@ class Session:
     def calc_face_flux(self, DoResChangeOnly, iBlock):
         self._check(self.lib.bats_calc_face_flux(
             self.h, int(DoResChangeOnly), iBlock), "calc_face_flux")
+
+    def calc_source(self, iBlock):
+        self._check(self.lib.bats_calc_source(self.h, iBlock), "calc_source")
 
     def update_state(self, iBlock):
         self._check(self.lib.bats_update_state(self.h, iBlock),

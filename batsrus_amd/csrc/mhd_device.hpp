@@ -114,24 +114,32 @@ __device__ __forceinline__ void slopes(double beta, double p0, double p1,
 // Left/right face states of the face between cells f-1 and f from the
 // primitive values q0=P(f-2), q1=P(f-1), q2=P(f), q3=P(f+1):
 // L = P(f-1) + dL(f-1), R = P(f) - dR(f)  (src/ModFaceValue.f90:2021-2027)
+// left and right states of one variable at the face between cells 1 and 2
+// of the 4-cell stencil q0..q3 (src/ModFaceValue.f90:2021-2027)
+template <int LIM>
+__device__ __forceinline__ void face_lr(double beta, double q0, double q1,
+                                        double q2, double q3, double &L,
+                                        double &R) {
+  if (LIM == LIM_FIRST) {
+    L = q1;
+    R = q2;
+  } else {
+    double dl, dr, t;
+    slopes<LIM>(beta, q0, q1, q2, dl, t);
+    slopes<LIM>(beta, q1, q2, q3, t, dr);
+    L = q1 + dl;
+    R = q2 - dr;
+  }
+}
+
 template <int LIM>
 __device__ __forceinline__ void face_states(double beta, const double *q0,
                                             const double *q1, const double *q2,
                                             const double *q3, double *L,
                                             double *R) {
 #pragma unroll
-  for (int v = 0; v < NVAR; v++) {
-    if (LIM == LIM_FIRST) {
-      L[v] = q1[v];
-      R[v] = q2[v];
-    } else {
-      double dl, dr, t;
-      slopes<LIM>(beta, q0[v], q1[v], q2[v], dl, t);
-      slopes<LIM>(beta, q1[v], q2[v], q3[v], t, dr);
-      L[v] = q1[v] + dl;
-      R[v] = q2[v] - dr;
-    }
-  }
+  for (int v = 0; v < NVAR; v++)
+    face_lr<LIM>(beta, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
 }
 
 // runtime-limiter variant used by the per-block parity routines
@@ -498,7 +506,8 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
                                             const double *dF,
                                             const double *Uin,
                                             const double *Uold,
-                                            double *Unew) {
+                                            double *Unew,
+                                            const double *Src = nullptr) {
   if (u.dtZero) {
 #pragma unroll
     for (int v = 0; v < NVAR; v++) Unew[v] = Uin[v];
@@ -507,7 +516,9 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
   }
   double Source[NFLUX];
 #pragma unroll
-  for (int v = 0; v < NFLUX; v++) Source[v] = u.DtLocal * (dF[v] * u.InvVolume);
+  for (int v = 0; v < NFLUX; v++)
+    Source[v] = Src ? u.DtLocal * (Src[v] + dF[v] * u.InvVolume)
+                    : u.DtLocal * (dF[v] * u.InvVolume);
   double Old[NVAR];
 #pragma unroll
   for (int v = 0; v < NVAR; v++) Old[v] = Uold[v];
@@ -543,6 +554,41 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
                                  Source[BY] * Source[BY] +
                                  Source[BZ] * Source[BZ]);
   Unew[PP] = limit_pressure(ph, energy_to_pressure(ph, Unew, Unew[PP]));
+}
+
+// 8-wave (Powell) div B source of one cell on a Cartesian grid without B0
+// (src/ModCalcSource.f90:1694-1889 with DoCorrectFace false, :930-951,
+// :1016-1022).  B?lo/B?hi are the normal-field left (L) and right (R) face
+// states at the lower and upper faces of the cell in each direction;
+// InvHalf = 0.5/CellSize; St is the cell's stage-input state.
+__device__ __forceinline__ void powell_source(
+    const double *InvHalf, double LxLo, double RxLo, double LxHi, double RxHi,
+    double LyLo, double RyLo, double LyHi, double RyHi, double LzLo,
+    double RzLo, double LzHi, double RzHi, const double *St, double *Src) {
+  const double dB1 = InvHalf[0] * (RxLo - LxLo);
+  const double dB2 = InvHalf[0] * (RxHi - LxHi);
+  const double dB3 = InvHalf[1] * (RyLo - LyLo);
+  const double dB4 = InvHalf[1] * (RyHi - LyHi);
+  const double dB5 = InvHalf[2] * (RzLo - LzLo);
+  const double dB6 = InvHalf[2] * (RzHi - LzHi);
+  double Internal = 2 * InvHalf[0] * (LxHi - RxLo);
+  Internal = Internal + 2 * InvHalf[1] * (LyHi - RyLo);
+  Internal = Internal + 2 * InvHalf[2] * (LzHi - RzLo);
+  double DivB = Internal + dB1 + dB2;
+  DivB = DivB + dB3 + dB4;
+  DivB = DivB + dB5 + dB6;
+  const double RhoInv = frcp(St[RHO]);
+  const double ux = RhoInv * St[MX], uy = RhoInv * St[MY],
+               uz = RhoInv * St[MZ];
+  Src[RHO] = 0.0;
+  Src[MX] = -(DivB * St[BX]);
+  Src[MY] = -(DivB * St[BY]);
+  Src[MZ] = -(DivB * St[BZ]);
+  Src[BX] = -(DivB * ux);
+  Src[BY] = -(DivB * uy);
+  Src[BZ] = -(DivB * uz);
+  Src[PP] = 0.0;
+  Src[EN] = -(DivB * (St[BX] * ux + St[BY] * uy + St[BZ] * uz));
 }
 
 } // namespace bats

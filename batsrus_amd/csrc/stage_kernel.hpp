@@ -1,0 +1,406 @@
+// stage_kernel.hpp -- the fused stage kernel k_stage and the device helpers
+// it shares with the per-block routines.
+//
+// Device layout: the state is held interior-only, variable-major
+// (structure of arrays): U[((v*nBlock + b)*NB + k)*NB + j)*NB + i].  There
+// are no ghost cells in HBM; halos are read directly from the neighbouring
+// blocks' interiors through the neighbour table, which replaces the
+// reference's ghost-cell fill (exchange_messages / BATL message_pass_cell)
+// for same-level blocks.  Two state buffers ping-pong between the stages of
+// the midpoint scheme, so StateOld is never copied.
+//
+// k_stage fuses, per block, what the reference does in calc_face_value +
+// calc_face_flux + calc_source + update_state (+ calc_timestep at the last
+// stage) (src/ModAdvanceExplicit.f90:140-220):
+//   * a workgroup of 256 threads owns one 16^3 block or four 8^3 blocks;
+//     each thread owns one (i,j) column and marches along k;
+//   * z faces: every thread keeps a window of primitive states of its column
+//     in registers and computes the z face above its cell; the flux carries
+//     over to the next plane, so each z face is computed once;
+//   * x/y faces: the plane's primitives plus a 2-cell x/y halo are staged
+//     in LDS, all x and y faces of the plane are spread over the 256
+//     threads, and their fluxes go back to LDS;
+//   * each thread then forms the flux divergence of its cell (and, with
+//     the 8-wave scheme, the div B source from the face states), updates it
+//     (p -> E, U = Uold + dt*R, E -> p) and writes it; face states and
+//     fluxes never reach HBM.
+//
+// The kernel is instantiated per (block size, flux) in its own translation
+// unit (stage_inst.hip) so the instantiations compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mhd_device.hpp"
+
+namespace bats {
+
+enum { S_DTCB = 0, S_DT = 1, S_TSIM = 2, S_NSTEP = 3, S_DTMIN = 4, S_NSCAL = 8 };
+constexpr int kThreads = 256;
+
+struct StageArgs {
+  const double *Uin;   // stage input (stencils and halos)
+  const double *Uold;  // state at the start of the step (StateOld)
+  double *Uout;        // updated state (may alias Uold)
+  const int *nei;      // 6 per block
+  const double *cellsize;  // 3 per block
+  double *scal;        // device scalars (S_*)
+  double *dtmaxB;      // per block minimum of the local time step
+  double *dtmaxCB;     // optional per-cell local time step
+  int *err;
+  int nBlock;
+  long vs;             // variable stride = nBlock*NB^3
+  Phys ph;
+  double beta;
+  int lim;             // LIM_* (LIM_FIRST for nOrder = 1)
+  double DtFactor;     // (Cfl*iStage)/nStage or Cfl
+  int iStage, nStage, halfStep, dbTrick, doDt, roundTripOld, noUpdate;
+  int advanceTime;     // add Dt/nStage to tSimulation when done
+};
+
+// offset (within one variable plane) of cell (i,j,k) of block b; one index
+// may lie in the 2-cell halo: it is resolved to the neighbour block or, at
+// a float outer boundary, to the block's own boundary cell
+// (src/ModCellBoundary.f90:518-559).
+template <int NB>
+__device__ __forceinline__ long cell_off(const int *__restrict__ nei, int b,
+                                         int i, int j, int k) {
+  if (i < 0) {
+    const int n = nei[6 * b + 0];
+    if (n >= 0) { b = n; i += NB; } else { i = 0; }
+  } else if (i >= NB) {
+    const int n = nei[6 * b + 1];
+    if (n >= 0) { b = n; i -= NB; } else { i = NB - 1; }
+  }
+  if (j < 0) {
+    const int n = nei[6 * b + 2];
+    if (n >= 0) { b = n; j += NB; } else { j = 0; }
+  } else if (j >= NB) {
+    const int n = nei[6 * b + 3];
+    if (n >= 0) { b = n; j -= NB; } else { j = NB - 1; }
+  }
+  if (k < 0) {
+    const int n = nei[6 * b + 4];
+    if (n >= 0) { b = n; k += NB; } else { k = 0; }
+  } else if (k >= NB) {
+    const int n = nei[6 * b + 5];
+    if (n >= 0) { b = n; k -= NB; } else { k = NB - 1; }
+  }
+  return (((long)b * NB + k) * NB + j) * NB + i;
+}
+
+__device__ __forceinline__ void load_state(const double *__restrict__ U,
+                                           long vs, long off, double *q) {
+#pragma unroll
+  for (int v = 0; v < NVAR; v++) q[v] = U[v * vs + off];
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = fmin2(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
+template <int DIR, int FLUX, int LIM>
+__device__ __forceinline__ int face_flux(const Phys &ph, double area,
+                                         double beta, const double *q0,
+                                         const double *q1, const double *q2,
+                                         const double *q3, double *F,
+                                         double &vdt, double &unA) {
+  double L[NVAR], R[NVAR];
+  face_states<LIM>(beta, q0, q1, q2, q3, L, R);
+  return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
+}
+
+// ---------------------------------------------------------------------
+// fused stage kernel
+// ---------------------------------------------------------------------
+template <int NB, int FLUX, int LIM, bool POWELL>
+__global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
+  constexpr int NC = NB * NB;          // columns per block
+  constexpr int BPW = kThreads / NC;   // blocks per workgroup
+  constexpr int NR = NB + 4;           // rows of the haloed plane
+  constexpr int NS = NB + 5;           // odd row stride: no LDS bank conflicts
+  constexpr int NFX = (NB + 1) * NB;   // x (or y) faces per plane
+  constexpr int NFS = 9;               // flux slots: 0..6, PP<-vdt, EN
+  __shared__ double sP[BPW * NVAR * NR * NS];
+  __shared__ double sF[BPW * 2 * NFS * NFX];
+  __shared__ double sRed[kThreads / 64];
+
+  const int tid = threadIdx.x;
+  const int lb = tid / NC, col = tid % NC, j = col / NB, i = col % NB;
+  const int blk0 = blockIdx.x * BPW;
+  const int b = blk0 + lb;
+  const bool act = b < a.nBlock;
+  const int bb = act ? b : 0;
+  const double *__restrict__ U = a.Uin;
+  const long vs = a.vs;
+  const Phys ph = a.ph;
+  const double beta = a.beta;
+  int err = 0;
+
+  auto sPi = [&](int l, int v, int jj, int ii) {
+    return ((l * NVAR + v) * NR + jj) * NS + ii;
+  };
+  auto sFi = [&](int l, int dir, int v, int q) {
+    return ((l * 2 + dir) * NFS + v) * NFX + q;
+  };
+
+  const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
+               dzb = a.cellsize[3 * bb + 2];
+  const double AreaZ = dxb * dyb;
+  const double CellVolume = dxb * dyb * dzb;
+  double InvHalf[3];
+  if (POWELL) {
+    InvHalf[0] = 0.5 / dxb;
+    InvHalf[1] = 0.5 / dyb;
+    InvHalf[2] = 0.5 / dzb;
+  }
+
+  const double DtUnscaled = a.scal[S_DTCB];
+  UpdateCtl uc;
+  uc.DtLocal = a.DtFactor * DtUnscaled;
+  uc.InvVolume = 1.0 / CellVolume;
+  uc.halfStep = a.halfStep;
+  uc.iStage = a.iStage;
+  uc.nStage = a.nStage;
+  uc.dbTrick = a.dbTrick;
+  uc.dtZero = (a.scal[S_DT] == 0.0);
+  uc.roundTripOld = a.roundTripOld;
+
+  // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
+  // left-state slope dLk of cell k; each cell's slopes are computed once and
+  // serve the faces below and above it
+  double Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
+  // z-face fluxes of the lower and upper face of the cell; with the 8-wave
+  // scheme two more slots carry the normal-field face states Bz_L, Bz_R
+  constexpr int NFZ = POWELL ? NFS + 2 : NFS;
+  double FzLo[NFZ], FzHi[NFZ];
+  double dtmin = 1.7976931348623157e308;
+
+  auto load_prim = [&](int kk, double *q) {
+    const long off = cell_off<NB>(a.nei, bb, i, j, kk);
+    load_state(U, vs, off, q);
+    to_primitive(q);
+  };
+  // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
+  auto zface = [&](double *Fz) {
+    double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) {
+      double dl, dr;
+      slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
+      L[v] = Q1[v] + dLk[v];
+      R[v] = Q2[v] - dr;
+      dLk[v] = dl;
+    }
+    err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
+#pragma unroll
+    for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
+    Fz[PP] = vdt;
+    if (POWELL) { Fz[NFZ - 2] = L[BZ]; Fz[NFZ - 1] = R[BZ]; }
+  };
+
+  if (act) {
+    // prologue: slopes of cell -1 and the face below cell 0
+    double Pm2[NVAR];
+    load_prim(-2, Pm2);
+    load_prim(-1, Q1);
+    load_prim(0, Q2);
+    load_prim(1, Q3);
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) {
+      double dl, dr;
+      slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
+      dLk[v] = dl;
+    }
+    zface(FzLo);
+#pragma unroll
+    for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+  }
+
+  for (int k = 0; k < NB; k++) {
+    // z face k+1 of this column
+    if (act) {
+      load_prim(k + 2, Q3);
+      zface(FzHi);
+      // own primitive of plane k
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
+    }
+    // x/y halo of plane k
+    for (int h = tid; h < BPW * 8 * NB; h += kThreads) {
+      const int lbh = h / (8 * NB);
+      const int bh = blk0 + lbh;
+      if (bh >= a.nBlock) continue;
+      const int r = h % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
+      int ii, jj;
+      if (side < 2) {
+        const int layer = rem & 1, pos = rem >> 1;
+        jj = pos;
+        ii = side == 0 ? layer - 2 : NB + layer;
+      } else {
+        const int layer = rem / NB, pos = rem % NB;
+        ii = pos;
+        jj = side == 2 ? layer - 2 : NB + layer;
+      }
+      double q[NVAR];
+      load_state(U, vs, cell_off<NB>(a.nei, bh, ii, jj, k), q);
+      to_primitive(q);
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) sP[sPi(lbh, v, jj + 2, ii + 2)] = q[v];
+    }
+    __syncthreads();
+    // x and y faces of plane k
+    for (int f = tid; f < BPW * 2 * NFX; f += kThreads) {
+      const int lbf = f / (2 * NFX);
+      const int bf = blk0 + lbf;
+      if (bf >= a.nBlock) continue;
+      const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
+      const int row = q / (NB + 1), fp = q % (NB + 1);
+      double c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
+      double F[NFLUX], vdt, unA;
+      const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
+                   dzf = a.cellsize[3 * bf + 2];
+      if (dir == 0) {
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) {
+          c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
+          c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
+          c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
+          c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
+        }
+        err |= face_flux<0, FLUX, LIM>(ph, dyf * dzf, beta, c0, c1, c2, c3, F,
+                                       vdt, unA);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) {
+          c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
+          c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
+          c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
+          c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
+        }
+        err |= face_flux<1, FLUX, LIM>(ph, dxf * dzf, beta, c0, c1, c2, c3, F,
+                                       vdt, unA);
+      }
+      F[PP] = vdt;
+#pragma unroll
+      for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
+    }
+    __syncthreads();
+    if (act) {
+      const int qx0 = j * (NB + 1) + i, qy0 = i * (NB + 1) + j;
+      double dF[NFLUX];
+#pragma unroll
+      for (int v = 0; v < NFLUX; v++) {
+        if (v == PP) { dF[v] = 0.0; continue; }
+        double d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
+        d = d + sF[sFi(lb, 1, v, qy0)] - sF[sFi(lb, 1, v, qy0 + 1)];
+        d = d + FzLo[v] - FzHi[v];
+        dF[v] = d;
+      }
+      const long off = (((long)b * NB + k) * NB + j) * NB + i;
+      if (!a.noUpdate) {
+        double Uc[NVAR], Uo[NVAR], Un[NVAR];
+        load_state(a.Uold, vs, off, Uo);
+        if (POWELL) {
+          // calc_source: the div B source from the normal-field face states
+          // of the six faces (x and y recomputed from the LDS plane exactly
+          // as the face loop computed them) and the stage-input cell state
+          load_state(U, vs, off, Uc);
+          double p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
+#pragma unroll
+          for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
+          face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
+          face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
+#pragma unroll
+          for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
+          face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
+          face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
+          double Src[NFLUX];
+          powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
+                        RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
+                        FzHi[NFZ - 1], Uc, Src);
+          update_cell(ph, uc, dF, Uc, Uo, Un, Src);
+        } else {
+          // the stage input at the cell is used only by the SSP-RK2
+          // interpolation and for a zero time step; otherwise re-read the
+          // just-loaded StateOld lines (cache hits) instead of the U* stream
+          load_state((uc.dtZero || !uc.halfStep) ? U : a.Uold, vs, off, Uc);
+          update_cell(ph, uc, dF, Uc, Uo, Un);
+        }
+#pragma unroll
+        for (int v = 0; v < NVAR; v++) a.Uout[v * vs + off] = Un[v];
+      }
+      if (a.doDt) {
+        const double Vdt =
+            fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
+            fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
+            fmax2(FzLo[PP], FzHi[PP]);
+        const double dtc = CellVolume / Vdt;
+        if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
+        dtmin = fmin2(dtmin, dtc);
+      }
+      // shift the z window
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+#pragma unroll
+      for (int v = 0; v < NFZ; v++) FzLo[v] = FzHi[v];
+    }
+    // the div B source reads neighbouring cells of the LDS plane, which the
+    // next plane overwrites
+    if (POWELL) __syncthreads();
+  }
+
+  if (a.doDt) {
+    // block minimum of the local time step (calc_timestep :236-256)
+    double m = wave_min(act ? dtmin : 1.7976931348623157e308);
+    const int w = tid / 64;
+    if ((tid & 63) == 0) sRed[w] = m;
+    __syncthreads();
+    if (act && col == 0) {
+      constexpr int WPB = NC >= 64 ? NC / 64 : 1;  // waves per block
+      double r = sRed[(lb * NC) / 64];
+      for (int q = 1; q < WPB; q++) r = fmin2(r, sRed[(lb * NC) / 64 + q]);
+      a.dtmaxB[b] = r;
+    }
+  }
+  if (err) atomicOr(a.err, err);
+  if (a.advanceTime && blockIdx.x == 0 && tid == 0)
+    a.scal[S_TSIM] = a.scal[S_TSIM] + a.scal[S_DT] / a.nStage;
+}
+
+// Launch k_stage for one block size and flux; the limiter and the 8-wave
+// switch are dispatched here.
+template <int NB, int FLUX>
+inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
+                                      hipStream_t stream) {
+  const int bpw = kThreads / (NB * NB);
+  const dim3 grid((a.nBlock + bpw - 1) / bpw), block(kThreads);
+#define BATS_LAUNCH(LM)                                                     \
+  if (powell)                                                               \
+    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, true>), grid, block, 0,       \
+                       stream, a);                                          \
+  else                                                                      \
+    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, false>), grid, block, 0,      \
+                       stream, a)
+  switch (a.lim) {
+  case LIM_MINMOD: BATS_LAUNCH(LIM_MINMOD); break;
+  case LIM_MC: BATS_LAUNCH(LIM_MC); break;
+  case LIM_MC3: BATS_LAUNCH(LIM_MC3); break;
+  case LIM_BETA: BATS_LAUNCH(LIM_BETA); break;
+  case LIM_NO: BATS_LAUNCH(LIM_NO); break;
+  default: BATS_LAUNCH(LIM_FIRST); break;
+  }
+#undef BATS_LAUNCH
+  return hipGetLastError();
+}
+
+// one launcher per (block size, flux) translation unit (stage_inst.hip)
+hipError_t launch_stage_8_rusanov(const StageArgs &, bool, hipStream_t);
+hipError_t launch_stage_8_linde(const StageArgs &, bool, hipStream_t);
+hipError_t launch_stage_8_hlld(const StageArgs &, bool, hipStream_t);
+hipError_t launch_stage_16_rusanov(const StageArgs &, bool, hipStream_t);
+hipError_t launch_stage_16_linde(const StageArgs &, bool, hipStream_t);
+hipError_t launch_stage_16_hlld(const StageArgs &, bool, hipStream_t);
+
+} // namespace bats

@@ -5,6 +5,7 @@
 !    calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
 !                                          src/ModFaceValue.f90:854
 !    calc_face_flux(DoResChangeOnly, iBlock) src/ModFaceFlux.f90:483
+!    calc_source(iBlock)                   src/ModCalcSource.f90:245
 !    update_state(iBlock)                  src/ModUpdateState.f90:83
 !    calc_timestep(iBlock, IsPartLocal)    src/ModTimeStepControl.f90:111
 !    set_global_timestep(TimeSimulationLimit)
@@ -33,14 +34,16 @@ module ModHipBridge
   public:: bats_init, bats_finalize, bats_set_grid, bats_upload_state, &
        bats_download_state, bats_upload_state_soa, bats_download_state_soa, &
        bats_exchange_messages, bats_set_stage, bats_calc_face_value, &
-       bats_calc_face_flux, bats_update_state, bats_calc_timestep, &
+       bats_calc_face_flux, bats_calc_source, bats_update_state, &
+       bats_calc_timestep, &
        bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms
   ! reference-named routines on the module-level handle
   public:: init_hip_bridge, clean_hip_bridge, sync_state_to_gpu, &
-       sync_state_to_cpu, calc_face_value, calc_face_flux, update_state, &
+       sync_state_to_cpu, calc_face_value, calc_face_flux, calc_source, &
+       update_state, &
        calc_timestep, set_global_timestep, advance_explicit, &
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
        init_timestep, get_time
@@ -59,6 +62,7 @@ module ModHipBridge
      integer(c_int):: UseDbTrick, UseDtFixed
      real(c_double):: DtFixed, cTiny
      integer(c_int):: iDevice
+     integer(c_int):: UseDivbSource
   end type bats_config
 
   interface
@@ -128,6 +132,12 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: DoResChangeOnly, iBlock
      end function bats_calc_face_flux
+     integer(c_int) function bats_calc_source(Handle, iBlock) &
+          bind(C, name="bats_calc_source")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock
+     end function bats_calc_source
      integer(c_int) function bats_update_state(Handle, iBlock) &
           bind(C, name="bats_update_state")
        import:: c_int, c_ptr
@@ -308,6 +318,12 @@ contains
     call bats_check(bats_calc_face_flux(hBats, merge(1, 0, DoResChangeOnly), &
          iBlock), 'calc_face_flux')
   end subroutine calc_face_flux
+  !============================================================================
+  subroutine calc_source(iBlock)
+    integer, intent(in):: iBlock
+    !--------------------------------------------------------------------------
+    call bats_check(bats_calc_source(hBats, iBlock), 'calc_source')
+  end subroutine calc_source
   !============================================================================
   subroutine update_state(iBlock)
     integer, intent(in):: iBlock

@@ -22,8 +22,9 @@ program bats_fortran_driver
   integer:: nOrder = 2, nStage = 2, iLimiter = 2, iFlux = 1
   real(c_double):: LimiterBeta = 1.5, Cfl = 0.8, Gamma = 5.0/3.0
   character(len=10):: TypeLoop = 'blocks'
+  logical:: UseDivbSource = .false.
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
-       iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop
+       iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -59,6 +60,7 @@ program bats_fortran_driver
   Config%Cfl = Cfl; Config%Gamma = Gamma; Config%pMin = -1.0
   Config%UseDbTrick = 0; Config%UseDtFixed = 0; Config%DtFixed = 0.0
   Config%cTiny = 1e-6; Config%iDevice = 0
+  Config%UseDivbSource = merge(1, 0, UseDivbSource)
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   call sync_state_to_gpu(State_VGB)
@@ -79,6 +81,7 @@ program bats_fortran_driver
            do iBlock = 1, nBlock
               call calc_face_value(iBlock, .false., .true.)
               call calc_face_flux(.false., iBlock)
+              call calc_source(iBlock)
               call update_state(iBlock)
               if(iStage == nStage) call calc_timestep(iBlock)
            end do

@@ -50,7 +50,7 @@ enum { BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, BATS_LIM_MC3 = 3,
 enum { BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, BATS_FLUX_HLLD = 3 };
 
 /* Scheme parameters: the subset of #SCHEME, #TIMESTEPPING, #GAMMA,
- * #MINIMUMPRESSURE, #DBTRICK, #FIXEDTIMESTEP and the compile-time grid
+ * #MINIMUMPRESSURE, #DBTRICK, #FIXEDTIMESTEP, #DIVB and the compile-time grid
  * shape (Config.pl -g, -ng) that the ideal-MHD explicit update reads
  * (src/ModSetParameters.f90:628-633,1742-1779,2314-2343,3873-3906). */
 typedef struct bats_config {
@@ -73,6 +73,8 @@ typedef struct bats_config {
   double DtFixed;
   double cTiny;         /* HLLD degeneracy guard (share ModNumConst) */
   int iDevice;          /* HIP device ordinal */
+  int UseDivbSource;    /* 8-wave (Powell) div B source (#DIVB,
+                           src/ModMain.f90:204; 0 = #DIVB F) */
 } bats_config;
 
 typedef struct bats_handle bats_handle;
@@ -125,8 +127,15 @@ int bats_calc_face_value(bats_handle *h, int iBlock, int DoResChangeOnly,
 /* calc_face_flux(DoResChangeOnly, iBlock) -- reference argument order
  * (src/ModFaceFlux.f90:483). */
 int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock);
+/* calc_source(iBlock) (src/ModCalcSource.f90:245-1025): the 8-wave div B
+ * source from the last calc_face_value block's face states; zero unless
+ * UseDivbSource.  Call it between calc_face_flux and update_state, as
+ * advance_explicit does (src/ModAdvanceExplicit.f90:166). */
+int bats_calc_source(bats_handle *h, int iBlock);
 /* update_state(iBlock) (src/ModUpdateState.f90:83); at iStage==1 copies the
- * block into StateOld first. */
+ * block into StateOld first.  With UseDivbSource it adds the source of the
+ * preceding bats_calc_source call for the same block (BATS_EINVAL if there
+ * was none). */
 int bats_update_state(bats_handle *h, int iBlock);
 /* calc_timestep(iBlock, IsPartLocal) (src/ModTimeStepControl.f90:111);
  * IsPartLocal must be 0. */

@@ -29,6 +29,11 @@
  *        pinned only by invariants (conservation, uniform state, symmetry,
  *        dt closed form); the reference's own fixtures do not exercise
  *        these combinations (parity partially pinned).
+ *   Powell div B source (calc_source)
+ *        pinned: div B of a linear field is exact (closed form), a
+ *        divergence-free field gives a zero source, and the source equals
+ *        -divB*(0, B, u, u.B) with div B from calc_divb's formula
+ *        (test_oracle_powell_*).
  *   HLLD guard constant cTiny (share/ModNumConst, absent): parity unpinned;
  *        configurable, default 1e-6.
  */
@@ -769,13 +774,98 @@ static inline void limit_pressure_cell(const ob_config *c, double *S) {
 }
 
 /* ---------------------------------------------------------------------- */
+/* calc_source -- src/ModCalcSource.f90:245-262, 862-1025: the 8-wave       */
+/* (Powell) div B source on a Cartesian grid without B0 (calc_divb_source  */
+/* :1694-1889 with DoCorrectFace false, then :908-951 and :1016-1022).     */
+/* ---------------------------------------------------------------------- */
+void ob_calc_source(const ob_config *c, const double *CellSize,
+                    const double *StateBlk, const double *LeftX,
+                    const double *RightX, const double *LeftY,
+                    const double *RightY, const double *LeftZ,
+                    const double *RightZ, double *Source_VC) {
+  ob_shape s = shape_of(c);
+  const int nI = s.nI, nJ = s.nJ, nK = s.nK;
+  const long nCell = (long)nI * nJ * nK;
+  int i, j, k, v;
+  for (v = 0; v < nCell * OB_NFLUX; v++) Source_VC[v] = 0; /* :254-256 */
+  if (!c->UseDivbSource) return;
+  {
+    /* :1706-1708 */
+    const double DxInvHalf = 0.5 / CellSize[0];
+    const double DyInvHalf = 0.5 / CellSize[1];
+    const double DzInvHalf = 0.5 / CellSize[2];
+    for (k = 1; k <= nK; k++)
+      for (j = 1; j <= nJ; j++)
+        for (i = 1; i <= nI; i++) {
+          const long c0 = ((long)(k - 1) * nJ + (j - 1)) * nI + (i - 1);
+          const long fx = ((long)(k - 1) * nJ + (j - 1)) * (nI + 1) + (i - 1);
+          double dB1nFace1, dB1nFace2, dB1nFace3 = 0, dB1nFace4 = 0;
+          double dB1nFace5 = 0, dB1nFace6 = 0, DivBInternal, DivB1;
+          /* :1716-1741 */
+          dB1nFace1 = DxInvHalf *
+                      (RightX[fx * NV + OB_BX] - LeftX[fx * NV + OB_BX]);
+          dB1nFace2 = DxInvHalf * (RightX[(fx + 1) * NV + OB_BX] -
+                                   LeftX[(fx + 1) * NV + OB_BX]);
+          /* :1743-1753 */
+          DivBInternal = 2 * DxInvHalf * (LeftX[(fx + 1) * NV + OB_BX] -
+                                          RightX[fx * NV + OB_BX]);
+          if (c->nDim >= 2) {
+            const long fy =
+                ((long)(k - 1) * (nJ + 1) + (j - 1)) * nI + (i - 1);
+            const long fy1 = fy + nI;
+            dB1nFace3 = DyInvHalf *
+                        (RightY[fy * NV + OB_BY] - LeftY[fy * NV + OB_BY]);
+            dB1nFace4 = DyInvHalf *
+                        (RightY[fy1 * NV + OB_BY] - LeftY[fy1 * NV + OB_BY]);
+            DivBInternal = DivBInternal +
+                           2 * DyInvHalf * (LeftY[fy1 * NV + OB_BY] -
+                                            RightY[fy * NV + OB_BY]);
+          }
+          if (c->nDim >= 3) {
+            const long fz = c0, fz1 = c0 + (long)nI * nJ;
+            dB1nFace5 = DzInvHalf *
+                        (RightZ[fz * NV + OB_BZ] - LeftZ[fz * NV + OB_BZ]);
+            dB1nFace6 = DzInvHalf *
+                        (RightZ[fz1 * NV + OB_BZ] - LeftZ[fz1 * NV + OB_BZ]);
+            DivBInternal = DivBInternal +
+                           2 * DzInvHalf * (LeftZ[fz1 * NV + OB_BZ] -
+                                            RightZ[fz * NV + OB_BZ]);
+          }
+          /* :1865-1873 */
+          DivB1 = DivBInternal + dB1nFace1 + dB1nFace2;
+          if (c->nDim >= 2) DivB1 = DivB1 + dB1nFace3 + dB1nFace4;
+          if (c->nDim >= 3) DivB1 = DivB1 + dB1nFace5 + dB1nFace6;
+          {
+            /* :930-951: single ion fluid, UseDivFullBSource false */
+            const double *St = StateBlk + cidx(&s, i, j, k);
+            double *Src = Source_VC + c0 * OB_NFLUX;
+            const double RhoInv = 1.0 / St[OB_RHO];
+            double u_D[3], SourceMhd[3];
+            for (v = 0; v < 3; v++) u_D[v] = RhoInv * St[OB_RHOUX + v];
+            for (v = 0; v < 3; v++)
+              Src[OB_BX + v] = Src[OB_BX + v] - DivB1 * u_D[v];
+            for (v = 0; v < 3; v++)
+              SourceMhd[v] = 0 - DivB1 * St[OB_BX + v];
+            Src[OB_ENERGY] =
+                Src[OB_ENERGY] - DivB1 * (St[OB_BX] * u_D[0] +
+                                          St[OB_BY] * u_D[1] +
+                                          St[OB_BZ] * u_D[2]);
+            /* :1016-1022 */
+            for (v = 0; v < 3; v++)
+              Src[OB_RHOUX + v] = Src[OB_RHOUX + v] + SourceMhd[v];
+          }
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
 /* update_state -- src/ModUpdateState.f90:83-965                           */
 /* ---------------------------------------------------------------------- */
 void ob_update_state(const ob_config *c, int iStage, double Dt,
                      const double *CellSize, const double *DtMax_CB,
                      const double *FluxX, const double *FluxY,
-                     const double *FluxZ, double *StateBlk,
-                     double *StateOldBlk) {
+                     const double *FluxZ, const double *Source_VC,
+                     double *StateBlk, double *StateOldBlk) {
   ob_shape s = shape_of(c);
   const int nI = s.nI, nJ = s.nJ, nK = s.nK;
   const double CellVolume = CellSize[0] * CellSize[1] * CellSize[2];
@@ -821,7 +911,8 @@ void ob_update_state(const ob_config *c, int iStage, double Dt,
           double d = fx0[v] - fx1[v];
           if (c->nDim >= 2) d = d + fy0[v] - fy1[v];
           if (c->nDim >= 3) d = d + fz0[v] - fz1[v];
-          Source[v] = DtLocal * (0.0 + d / CellVolume);
+          Source[v] = DtLocal * ((Source_VC ? Source_VC[c0 * OB_NFLUX + v]
+                                            : 0.0) + d / CellVolume);
         }
         /* update_explicit (:453-965) */
         ob_pressure_to_energy_cell(c, So);               /* :606 */
@@ -972,7 +1063,7 @@ void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit) {
 /* advance_explicit -- src/ModAdvanceExplicit.f90:18-325 (orig path)       */
 /* ---------------------------------------------------------------------- */
 typedef struct {
-  double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;
+  double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ, *Src;
 } ob_scratch;
 
 static void scratch_alloc(const ob_config *c, ob_scratch *w) {
@@ -985,10 +1076,11 @@ static void scratch_alloc(const ob_config *c, ob_scratch *w) {
   w->FX = (double *)calloc(nfx * OB_NFACEVALUE, 8);
   w->FY = (double *)calloc(nfy * OB_NFACEVALUE, 8);
   w->FZ = (double *)calloc(nfz * OB_NFACEVALUE, 8);
+  w->Src = (double *)calloc((long)c->nI * c->nJ * c->nK * OB_NFLUX, 8);
 }
 static void scratch_free(ob_scratch *w) {
   free(w->LX); free(w->RX); free(w->LY); free(w->RY); free(w->LZ);
-  free(w->RZ); free(w->FX); free(w->FY); free(w->FZ);
+  free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
 }
 
 static int run_blocks(ob_session *s, int iStage, int DoUpdate,
@@ -1022,9 +1114,15 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
 #endif
         if (!err) err = e;
       }
-      if (DoUpdate)
+      if (DoUpdate) {
+        /* calc_source (src/ModAdvanceExplicit.f90:166) */
+        if (c->UseDivbSource)
+          ob_calc_source(c, cs, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
+                         w.Src);
         ob_update_state(c, iStage, s->Dt, cs, s->DtMax_CB + nCell * b, w.FX,
-                        w.FY, w.FZ, Blk, Old);
+                        w.FY, w.FZ, c->UseDivbSource ? w.Src : NULL, Blk,
+                        Old);
+      }
       if (DoCalcTimestep)
         ob_calc_timestep(c, cs, w.FX, w.FY, w.FZ, s->DtMax_CB + nCell * b,
                          s->DtMax_B + b);

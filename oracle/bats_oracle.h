@@ -64,6 +64,7 @@ typedef struct {
   int UseDtFixed;
   double DtFixed;
   double cTiny;        /* HLLD degeneracy guard (ModNumConst, absent) */
+  int UseDivbSource;   /* 8-wave (Powell) div B source, #DIVB */
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */
@@ -99,13 +100,22 @@ int ob_calc_face_flux(const ob_config *c, const double *CellSize,
                       double *LeftY, double *RightY,
                       double *LeftZ, double *RightZ,
                       double *FluxX, double *FluxY, double *FluxZ);
+/* calc_source (src/ModCalcSource.f90:245-262,862-1025): the Powell div B
+ * source on a Cartesian grid without B0.  Source_VC(nFlux=9, nI, nJ, nK),
+ * variable fastest; zero unless UseDivbSource. */
+void ob_calc_source(const ob_config *c, const double *CellSize,
+                    const double *StateBlk, const double *LeftX,
+                    const double *RightX, const double *LeftY,
+                    const double *RightY, const double *LeftZ,
+                    const double *RightZ, double *Source_VC);
 /* update_state (src/ModUpdateState.f90:83-965) for one block.  DtMax_CB is
- * this block's (nI,nJ,nK) array; Dt is the CFL-scaled global step. */
+ * this block's (nI,nJ,nK) array; Dt is the CFL-scaled global step;
+ * Source_VC from calc_source or NULL (no source terms). */
 void ob_update_state(const ob_config *c, int iStage, double Dt,
                      const double *CellSize, const double *DtMax_CB,
                      const double *FluxX, const double *FluxY,
-                     const double *FluxZ, double *StateBlk,
-                     double *StateOldBlk);
+                     const double *FluxZ, const double *Source_VC,
+                     double *StateBlk, double *StateOldBlk);
 /* calc_timestep (src/ModTimeStepControl.f90:111-304) for one block */
 void ob_calc_timestep(const ob_config *c, const double *CellSize,
                       const double *FluxX, const double *FluxY,

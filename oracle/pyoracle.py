@@ -26,7 +26,8 @@ class ObConfig(C.Structure):
                 ("UseHalfStep", C.c_int), ("Cfl", C.c_double),
                 ("Gamma", C.c_double), ("pMin", C.c_double),
                 ("UseDbTrick", C.c_int), ("UseDtFixed", C.c_int),
-                ("DtFixed", C.c_double), ("cTiny", C.c_double)]
+                ("DtFixed", C.c_double), ("cTiny", C.c_double),
+                ("UseDivbSource", C.c_int)]
 
 
 class ObSession(C.Structure):
@@ -63,7 +64,9 @@ def lib():
         L.ob_calc_face_flux.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 9
         L.ob_calc_face_flux.restype = C.c_int
         L.ob_update_state.argtypes = [C.POINTER(ObConfig), C.c_int,
-                                      C.c_double, dp, dp, dp, dp, dp, dp, dp]
+                                      C.c_double, dp, dp, dp, dp, dp, dp, dp,
+                                      dp]
+        L.ob_calc_source.argtypes = [C.POINTER(ObConfig), dp, dp] + [dp] * 7
         L.ob_calc_timestep.argtypes = [C.POINTER(ObConfig), dp, dp, dp, dp,
                                        dp, dp]
         L.ob_exchange_messages.argtypes = [C.POINTER(ObConfig), C.c_int,
@@ -91,7 +94,8 @@ def _dp(a):
 def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 beta=1.5, flux="rusanov", nStage=2, UseHalfStep=True, cfl=0.8,
                 gamma=5.0 / 3.0, pMin=-1.0, UseDbTrick=False,
-                UseDtFixed=False, DtFixed=0.0, cTiny=1e-6):
+                UseDtFixed=False, DtFixed=0.0, cTiny=1e-6,
+                UseDivbSource=False):
     c = ObConfig()
     c.nI, c.nJ, c.nK, c.nG, c.nDim, c.nOrder = nI, nJ, nK, nG, nDim, nOrder
     c.iLimiter = LIMITERS[limiter] if isinstance(limiter, str) else limiter
@@ -100,6 +104,7 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.nStage, c.UseHalfStep, c.Cfl, c.Gamma = nStage, int(UseHalfStep), cfl, gamma
     c.pMin, c.UseDbTrick = pMin, int(UseDbTrick)
     c.UseDtFixed, c.DtFixed, c.cTiny = int(UseDtFixed), DtFixed, cTiny
+    c.UseDivbSource = int(UseDivbSource)
     return c
 
 
@@ -183,12 +188,25 @@ def calc_face_flux(cfg, cell_size, faces):
     return err, fl
 
 
-def update_state(cfg, iStage, dt, cell_size, dtmax_cb, fluxes, block_state,
-                 block_old):
+def calc_source(cfg, cell_size, block_state, faces):
+    """Source_VC of shape (nK, nJ, nI, 9) (zero unless UseDivbSource)."""
     cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    blk = np.ascontiguousarray(block_state, dtype=np.float64)
+    src = np.zeros((cfg.nK, cfg.nJ, cfg.nI, 9))
+    lib().ob_calc_source(C.byref(cfg), _dp(cs), _dp(blk),
+                         *[_dp(np.ascontiguousarray(a)) for a in faces],
+                         _dp(src))
+    return src
+
+
+def update_state(cfg, iStage, dt, cell_size, dtmax_cb, fluxes, block_state,
+                 block_old, source=None):
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    src = None if source is None else np.ascontiguousarray(source)
     lib().ob_update_state(C.byref(cfg), iStage, dt, _dp(cs), _dp(dtmax_cb),
-                          *[_dp(f) for f in fluxes], _dp(block_state),
-                          _dp(block_old))
+                          *[_dp(f) for f in fluxes],
+                          None if src is None else _dp(src),
+                          _dp(block_state), _dp(block_old))
 
 
 def calc_timestep(cfg, cell_size, fluxes):

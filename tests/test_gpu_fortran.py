@@ -18,9 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "batsrus_amd", "fortran", "bats_fortran_driver")
 
 
+@pytest.mark.parametrize("powell", [False, True])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
-def test_fortran_driver_matches_oracle(loop):
-    print(f"case fortran_driver loop={loop}", flush=True)
+def test_fortran_driver_matches_oracle(loop, powell):
+    print(f"case fortran_driver loop={loop} powell={powell}", flush=True)
     if not os.path.exists(EXE):
         pytest.fail("Fortran driver not built (build() compiles it)")
     grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
@@ -32,8 +33,9 @@ def test_fortran_driver_matches_oracle(loop):
         with open(os.path.join(d, "PARAM.nml"), "w") as f:
             f.write("&RUN nRootI=2, nRootJ=2, nRootK=2, nCell=8, nStep=%d, "
                     "nOrder=2, nStage=2, iLimiter=3, iFlux=3, "
-                    "LimiterBeta=1.5, Cfl=0.8, TypeLoop='%s' /\n"
-                    % (nstep, loop))
+                    "LimiterBeta=1.5, Cfl=0.8, TypeLoop='%s', "
+                    "UseDivbSource=%s /\n"
+                    % (nstep, loop, ".true." if powell else ".false."))
         with open(os.path.join(d, "grid.bin"), "wb") as f:
             f.write(np.ascontiguousarray(grid.iNeighbor_EB, np.int32).tobytes())
             f.write(np.ascontiguousarray(grid.CellSize_DB).tobytes())
@@ -44,7 +46,7 @@ def test_fortran_driver_matches_oracle(loop):
         assert r.returncode == 0, r.stdout + r.stderr
         S = np.fromfile(os.path.join(d, "state_out.bin")).reshape(S0.shape)
     cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3", beta=1.5,
-                                   cfl=0.8), grid)
+                                   cfl=0.8, UseDivbSource=powell), grid)
     o = po.Session(grid, cfg, S0)
     o.init_timestep()
     for _ in range(nstep):

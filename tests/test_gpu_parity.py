@@ -200,3 +200,86 @@ def test_download_state_vgb_ghosts():
     mask[g:g + n, :, g:g + n] = True
     mask[:, g:g + n, g:g + n] = True
     assert np.array_equal(Sg[:, mask], ref[:, mask])
+
+
+@pytest.mark.parametrize("NB", [8, 16])
+@pytest.mark.parametrize("flux,limiter", [("rusanov", "mc"),
+                                          ("hlld", "mc3"),
+                                          ("linde", "minmod")])
+def test_block_routines_powell(flux, limiter, NB):
+    """calc_face_value -> calc_face_flux -> calc_source -> update_state per
+    block with the 8-wave div B source, against the oracle's routines."""
+    print(f"case block_powell flux={flux} lim={limiter} NB={NB}", flush=True)
+    grid = BlockGrid(nRoot=(2, 2, 1) if NB == 16 else (2, 2, 2),
+                     nCell=(NB, NB, NB), periodic=(True, False, True))
+    U0 = random_mhd_soa(grid, seed=30 + NB)
+    scheme = dict(flux=flux, limiter=limiter, beta=1.5, UseDivbSource=True)
+    cfg = po.config_from_dict(scheme, grid)
+    S0 = _ghosted(grid, U0)
+    o = po.Session(grid, cfg, S0)
+    o.init_timestep()
+    o.s.nStep = 1
+    dto = o.set_global_timestep()
+    dt_unscaled = o.DtMax_CB[0]
+    with Session(grid, **scheme) as s:
+        s.upload_soa(U0)
+        s.init_timestep()
+        s.set_global_timestep()
+        s.exchange_messages()
+        s.set_stage(1)
+        S_or = S0.copy()
+        for b in range(grid.nBlock):
+            s.calc_face_value(b + 1)
+            s.calc_face_flux(False, b + 1)
+            with pytest.raises(Exception):
+                s.update_state(b + 1)      # calc_source not called yet
+            s.calc_source(b + 1)
+            s.update_state(b + 1)
+            ofv = po.calc_face_value(cfg, S0[b])
+            src = po.calc_source(cfg, grid.CellSize_DB[b], S0[b], ofv)
+            _, off = po.calc_face_flux(cfg, grid.CellSize_DB[b],
+                                       [f.copy() for f in ofv])
+            dtcb = np.full(grid.nI * grid.nJ * grid.nK, dt_unscaled)
+            blk = np.ascontiguousarray(S_or[b])
+            old = np.zeros_like(blk)
+            po.update_state(cfg, 1, dto, grid.CellSize_DB[b], dtcb, off, blk,
+                            old, source=src)
+            S_or[b] = blk
+        Sg = s.download_soa()
+    assert rel_err(Sg, grid.state_vgb_to_soa(S_or)) <= TOL_STAGE
+
+
+@pytest.mark.parametrize("NB", [8, 16])
+@pytest.mark.parametrize("flux,limiter,nOrder", [
+    ("rusanov", "mc", 2), ("linde", "mc3", 2), ("hlld", "mc3", 2),
+    ("hlld", "minmod", 1)])
+def test_fused_stages_powell(flux, limiter, nOrder, NB):
+    """Fused stage kernel with the 8-wave source, stage by stage over two
+    steps, against the oracle."""
+    print(f"case fused_powell flux={flux} lim={limiter} order={nOrder} "
+          f"NB={NB}", flush=True)
+    grid = BlockGrid(nRoot=(3, 2, 2) if NB == 8 else (2, 2, 2),
+                     nCell=(NB, NB, NB), periodic=(False, True, True),
+                     order="morton")
+    U0 = smooth_mhd_soa(grid, seed=40 + NB)   # div B != 0 (random phases)
+    scheme = dict(flux=flux, limiter=limiter, nOrder=nOrder, beta=1.5,
+                  cfl=0.8, UseDivbSource=True)
+    cfg = po.config_from_dict(scheme, grid)
+    o = po.Session(grid, cfg, _ghosted(grid, U0))
+    o.init_timestep()
+    with Session(grid, **scheme) as s:
+        s.upload_soa(U0)
+        s.init_timestep()
+        for step in range(2):
+            o.s.nStep += 1
+            dto = o.set_global_timestep()
+            dtg = s.set_global_timestep()
+            assert abs(dtg - dto) <= TOL_STAGE * dto, (step, dtg, dto)
+            for stage in (1, 2):
+                assert o.advance_stage(stage) == 0
+                if stage == 2:
+                    o.exchange()
+                s.advance_stage(stage)
+                e = rel_err(s.download_soa(), o.soa())
+                print(f"  step {step} stage {stage} rel err {e:.2e}")
+                assert e <= TOL_STAGE, (step, stage, e)
