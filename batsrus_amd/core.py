@@ -88,6 +88,9 @@ ABI_FUNCTIONS = {
     "bats_init_timestep": (C.c_int, [_hp]),
     "bats_get_time": (C.c_int, [_hp, _dp, _dp, _ip]),
     "bats_synchronize": (C.c_int, [_hp]),
+    "bats_set_runs": (C.c_int, [_hp, C.c_int, _ip]),
+    "bats_get_run_times": (C.c_int, [_hp, _dp, _dp, _ip]),
+    "bats_set_persistent_grid": (C.c_int, [_hp, C.c_int]),
     "bats_check_nan": (C.c_int, [_hp]),
     "bats_integrate_domain": (C.c_int, [_hp, _dp]),
     "bats_last_stage_kernel_ms": (C.c_int, [_hp, _dp, _ip]),
@@ -323,6 +326,27 @@ class Session:
         self._check(self.lib.bats_get_time(self.h, C.byref(dt), C.byref(t),
                                            C.byref(n)), "get_time")
         return dt.value, t.value, n.value
+
+    def set_runs(self, iBlockFirst_R):
+        """Independent runs over contiguous block ranges (0-based offsets,
+        nRun+1 entries)."""
+        f = np.ascontiguousarray(iBlockFirst_R, dtype=np.int32)
+        self._check(self.lib.bats_set_runs(self.h, len(f) - 1,
+                                           _ptr(f, C.c_int)), "set_runs")
+        self.nRun = len(f) - 1
+
+    def run_times(self):
+        n = getattr(self, "nRun", 1)
+        dt, t = np.zeros(n), np.zeros(n)
+        ns = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.bats_get_run_times(self.h, _ptr(dt), _ptr(t),
+                                                _ptr(ns, C.c_int)),
+                    "get_run_times")
+        return dt, t, ns
+
+    def set_persistent_grid(self, nWorkgroup):
+        self._check(self.lib.bats_set_persistent_grid(self.h, nWorkgroup),
+                    "set_persistent_grid")
 
     def integrate_domain(self):
         s = np.zeros(8)

@@ -214,12 +214,18 @@ __global__ void __launch_bounds__(512) k_block_timestep(BlockArgs a) {
 }
 
 // set_global_timestep (src/ModTimeStepControl.f90:306-524), one workgroup
+// per run; run r owns blocks runFirst[r] .. runFirst[r+1]-1 and the scalars
+// scal[r*S_NSCAL ...]
 __global__ void __launch_bounds__(1024)
-k_global_dt(const double *dtmaxB, int nBlock, double *scal, double tLimit,
-            double Cfl, int useDtFixed, double DtFixed, int incStep) {
+k_global_dt(const double *dtmaxB, const int *runFirst, double *scalAll,
+            double tLimit, double Cfl, int useDtFixed, double DtFixed,
+            int incStep) {
   __shared__ double sRed[16];
+  const int r = blockIdx.x;
+  double *scal = scalAll + (long)r * S_NSCAL;
   double m = 1.7976931348623157e308;  // huge()
-  for (int b = threadIdx.x; b < nBlock; b += blockDim.x) m = fmin2(m, dtmaxB[b]);
+  for (int b = runFirst[r] + threadIdx.x; b < runFirst[r + 1]; b += blockDim.x)
+    m = fmin2(m, dtmaxB[b]);
   m = wave_min(m);
   if ((threadIdx.x & 63) == 0) sRed[threadIdx.x / 64] = m;
   __syncthreads();
@@ -298,7 +304,12 @@ struct bats_handle {
   double *cellsize;
   double *dtmaxB;
   double *dtmaxCB;
-  double *scal;
+  double *scal;      // S_NSCAL scalars per run
+  int nRun;
+  std::vector<int> hRunFirst, hRunOfBlock;
+  int *runFirst;     // nRun+1 block offsets
+  int *runOfBlock;   // nBlock
+  int nWorkgroupMax; // > 0: persistent grid of at most this many workgroups
   int *err;
   double *partial;
   double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;
@@ -406,6 +417,8 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (e == hipSuccess) e = hipMalloc(&h->cellsize, sizeof(double) * 3 * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->dtmaxB, sizeof(double) * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->scal, sizeof(double) * S_NSCAL);
+  if (e == hipSuccess) e = hipMalloc(&h->runFirst, sizeof(int) * 2);
+  if (e == hipSuccess) e = hipMalloc(&h->runOfBlock, sizeof(int) * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * NVAR * c.nBlock);
   if (e != hipSuccess) {
@@ -419,6 +432,13 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   h->FX = h->FY = h->FZ = nullptr;
   double s0[S_NSCAL] = {0};
   HIPCHK(hipMemcpy(h->scal, s0, sizeof s0, hipMemcpyHostToDevice));
+  h->nRun = 1;
+  h->hRunFirst = {0, c.nBlock};
+  h->hRunOfBlock.assign(c.nBlock, 0);
+  h->nWorkgroupMax = 0;
+  HIPCHK(hipMemcpy(h->runFirst, h->hRunFirst.data(), sizeof(int) * 2,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->runOfBlock, 0, sizeof(int) * c.nBlock));
   HIPCHK(hipMemset(h->err, 0, sizeof(int)));
   HIPCHK(hipMemset(h->dtmaxB, 0, sizeof(double) * c.nBlock));
   // default grid: all sides float until bats_set_grid
@@ -433,7 +453,8 @@ extern "C" int bats_finalize(bats_handle *h) {
   if (!h) return BATS_EINVAL;
   (void)hipStreamSynchronize(h->stream);
   void *ptrs[] = {h->U[0], h->U[1], h->Usnap, h->nei, h->cellsize, h->dtmaxB,
-                  h->dtmaxCB, h->scal, h->err, h->partial, h->LX, h->RX,
+                  h->dtmaxCB, h->scal, h->runFirst, h->runOfBlock, h->err,
+                  h->partial, h->LX, h->RX,
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -452,6 +473,10 @@ extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
     if (iNeighbor_EB[t] < -1 || iNeighbor_EB[t] >= nB) return BATS_EINVAL;
   for (int t = 0; t < 3 * nB; t++)
     if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
+  for (int t = 0; t < 6 * nB; t++)  // runs stay independent domains
+    if (iNeighbor_EB[t] >= 0 &&
+        h->hRunOfBlock[iNeighbor_EB[t]] != h->hRunOfBlock[t / 6])
+      return BATS_EINVAL;
   h->hNei.assign(iNeighbor_EB, iNeighbor_EB + 6 * nB);
   HIPCHK(hipMemcpyAsync(h->nei, iNeighbor_EB, sizeof(int) * 6 * nB,
                         hipMemcpyHostToDevice, h->stream));
@@ -587,7 +612,7 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.StateOld = h->U[1 - h->stateBuf];
   a.nei = h->nei;
   a.cellsize = h->cellsize;
-  a.scal = h->scal;
+  a.scal = h->scal + (long)h->hRunOfBlock[iBlock] * S_NSCAL;
   a.LX = h->LX; a.RX = h->RX; a.LY = h->LY; a.RY = h->RY; a.LZ = h->LZ; a.RZ = h->RZ;
   a.FX = h->FX; a.FY = h->FY; a.FZ = h->FZ;
   a.Src = h->Src;
@@ -748,9 +773,9 @@ extern "C" int bats_set_global_timestep(bats_handle *h, double TimeSimulationLim
                                         double *DtOut) {
   if (!h) return BATS_EINVAL;
   const bats_config &c = h->cfg;
-  hipLaunchKernelGGL(k_global_dt, dim3(1), dim3(1024), 0, h->stream, h->dtmaxB,
-                     c.nBlock, h->scal, TimeSimulationLimit, c.Cfl, c.UseDtFixed,
-                     c.DtFixed, 1);
+  hipLaunchKernelGGL(k_global_dt, dim3(h->nRun), dim3(1024), 0, h->stream,
+                     h->dtmaxB, h->runFirst, h->scal, TimeSimulationLimit,
+                     c.Cfl, c.UseDtFixed, c.DtFixed, 1);
   HIPCHK(hipGetLastError());
   if (DtOut) {
     double s[S_NSCAL];
@@ -779,6 +804,8 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   a.nei = h->nei;
   a.cellsize = h->cellsize;
   a.scal = h->scal;
+  a.runOfBlock = h->nRun > 1 ? h->runOfBlock : nullptr;
+  a.nRun = h->nRun;
   a.dtmaxB = h->dtmaxB;
   a.dtmaxCB = h->dtmaxCB;
   a.err = h->err;
@@ -797,6 +824,8 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   a.noUpdate = noUpdate;
   a.advanceTime = advanceTime;
   const int NB = h->NB;
+  a.nGroup = (c.nBlock + kThreads / (NB * NB) - 1) / (kThreads / (NB * NB));
+  a.nWorkgroup = h->nWorkgroupMax > 0 ? h->nWorkgroupMax : a.nGroup;
   int ev = -1;
   if (timed) {
     ev = h->nTimed++;
@@ -886,6 +915,62 @@ extern "C" int bats_get_time(bats_handle *h, double *Dt, double *tSimulation,
   if (Dt) *Dt = s[S_DT];
   if (tSimulation) *tSimulation = s[S_TSIM];
   if (nStep) *nStep = (int)s[S_NSTEP];
+  return BATS_OK;
+}
+
+extern "C" int bats_set_runs(bats_handle *h, int nRun, const int *iBlockFirst_R) {
+  if (!h || nRun < 1 || !iBlockFirst_R) return BATS_EINVAL;
+  const int nB = h->cfg.nBlock;
+  if (iBlockFirst_R[0] != 0 || iBlockFirst_R[nRun] != nB) return BATS_EINVAL;
+  for (int r = 0; r < nRun; r++)
+    if (iBlockFirst_R[r + 1] <= iBlockFirst_R[r]) return BATS_EINVAL;
+  std::vector<int> runOf(nB);
+  for (int r = 0; r < nRun; r++)
+    for (int b = iBlockFirst_R[r]; b < iBlockFirst_R[r + 1]; b++) runOf[b] = r;
+  // runs are independent domains: no neighbour across a run boundary
+  for (int b = 0; b < nB; b++)
+    for (int s = 0; s < 6; s++) {
+      const int n = h->hNei[6 * b + s];
+      if (n >= 0 && runOf[n] != runOf[b]) return BATS_EINVAL;
+    }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double *scal = nullptr;
+  int *first = nullptr;
+  HIPCHK(hipMalloc(&scal, sizeof(double) * S_NSCAL * nRun));
+  HIPCHK(hipMalloc(&first, sizeof(int) * (nRun + 1)));
+  HIPCHK(hipMemset(scal, 0, sizeof(double) * S_NSCAL * nRun));
+  HIPCHK(hipMemcpy(first, iBlockFirst_R, sizeof(int) * (nRun + 1),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->runOfBlock, runOf.data(), sizeof(int) * nB,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipFree(h->scal));
+  HIPCHK(hipFree(h->runFirst));
+  h->scal = scal;
+  h->runFirst = first;
+  h->nRun = nRun;
+  h->hRunFirst.assign(iBlockFirst_R, iBlockFirst_R + nRun + 1);
+  h->hRunOfBlock = runOf;
+  return BATS_OK;
+}
+
+extern "C" int bats_get_run_times(bats_handle *h, double *Dt_R,
+                                  double *tSimulation_R, int *nStep_R) {
+  if (!h) return BATS_EINVAL;
+  std::vector<double> s((size_t)S_NSCAL * h->nRun);
+  HIPCHK(hipMemcpyAsync(s.data(), h->scal, sizeof(double) * s.size(),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int r = 0; r < h->nRun; r++) {
+    if (Dt_R) Dt_R[r] = s[(size_t)r * S_NSCAL + S_DT];
+    if (tSimulation_R) tSimulation_R[r] = s[(size_t)r * S_NSCAL + S_TSIM];
+    if (nStep_R) nStep_R[r] = (int)s[(size_t)r * S_NSCAL + S_NSTEP];
+  }
+  return BATS_OK;
+}
+
+extern "C" int bats_set_persistent_grid(bats_handle *h, int nWorkgroup) {
+  if (!h || nWorkgroup < 0) return BATS_EINVAL;
+  h->nWorkgroupMax = nWorkgroup;
   return BATS_OK;
 }
 

@@ -43,7 +43,9 @@ struct StageArgs {
   double *Uout;        // updated state (may alias Uold)
   const int *nei;      // 6 per block
   const double *cellsize;  // 3 per block
-  double *scal;        // device scalars (S_*)
+  double *scal;        // device scalars, S_NSCAL per run (S_*)
+  const int *runOfBlock;  // run index of each block (null: one run)
+  int nRun;
   double *dtmaxB;      // per block minimum of the local time step
   double *dtmaxCB;     // optional per-cell local time step
   int *err;
@@ -55,6 +57,8 @@ struct StageArgs {
   double DtFactor;     // (Cfl*iStage)/nStage or Cfl
   int iStage, nStage, halfStep, dbTrick, doDt, roundTripOld, noUpdate;
   int advanceTime;     // add Dt/nStage to tSimulation when done
+  int nGroup;          // workgroup-sized groups of blocks (work items)
+  int nWorkgroup;      // launched workgroups (< nGroup: persistent grid)
 };
 
 // offset (within one variable plane) of cell (i,j,k) of block b; one index
@@ -128,245 +132,256 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
 
   const int tid = threadIdx.x;
   const int lb = tid / NC, col = tid % NC, j = col / NB, i = col % NB;
-  const int blk0 = blockIdx.x * BPW;
-  const int b = blk0 + lb;
-  const bool act = b < a.nBlock;
-  const int bb = act ? b : 0;
-  const double *__restrict__ U = a.Uin;
-  const long vs = a.vs;
-  const Phys ph = a.ph;
-  const double beta = a.beta;
   int err = 0;
 
-  auto sPi = [&](int l, int v, int jj, int ii) {
-    return ((l * NVAR + v) * NR + jj) * NS + ii;
-  };
-  auto sFi = [&](int l, int dir, int v, int q) {
-    return ((l * 2 + dir) * NFS + v) * NFX + q;
-  };
+  // a persistent grid walks the flat work list of block groups; with one
+  // workgroup per group the loop runs once
+  for (int grp = blockIdx.x; grp < a.nGroup; grp += gridDim.x) {
+    const int blk0 = grp * BPW;
+    const int b = blk0 + lb;
+    const bool act = b < a.nBlock;
+    const int bb = act ? b : 0;
+    const double *__restrict__ U = a.Uin;
+    const long vs = a.vs;
+    const Phys ph = a.ph;
+    const double beta = a.beta;
 
-  const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
-               dzb = a.cellsize[3 * bb + 2];
-  const double AreaZ = dxb * dyb;
-  const double CellVolume = dxb * dyb * dzb;
-  double InvHalf[3];
-  if (POWELL) {
-    InvHalf[0] = 0.5 / dxb;
-    InvHalf[1] = 0.5 / dyb;
-    InvHalf[2] = 0.5 / dzb;
-  }
+    auto sPi = [&](int l, int v, int jj, int ii) {
+      return ((l * NVAR + v) * NR + jj) * NS + ii;
+    };
+    auto sFi = [&](int l, int dir, int v, int q) {
+      return ((l * 2 + dir) * NFS + v) * NFX + q;
+    };
 
-  const double DtUnscaled = a.scal[S_DTCB];
-  UpdateCtl uc;
-  uc.DtLocal = a.DtFactor * DtUnscaled;
-  uc.InvVolume = 1.0 / CellVolume;
-  uc.halfStep = a.halfStep;
-  uc.iStage = a.iStage;
-  uc.nStage = a.nStage;
-  uc.dbTrick = a.dbTrick;
-  uc.dtZero = (a.scal[S_DT] == 0.0);
-  uc.roundTripOld = a.roundTripOld;
-
-  // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
-  // left-state slope dLk of cell k; each cell's slopes are computed once and
-  // serve the faces below and above it
-  double Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
-  // z-face fluxes of the lower and upper face of the cell; with the 8-wave
-  // scheme two more slots carry the normal-field face states Bz_L, Bz_R
-  constexpr int NFZ = POWELL ? NFS + 2 : NFS;
-  double FzLo[NFZ], FzHi[NFZ];
-  double dtmin = 1.7976931348623157e308;
-
-  auto load_prim = [&](int kk, double *q) {
-    const long off = cell_off<NB>(a.nei, bb, i, j, kk);
-    load_state(U, vs, off, q);
-    to_primitive(q);
-  };
-  // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
-  auto zface = [&](double *Fz) {
-    double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
-#pragma unroll
-    for (int v = 0; v < NVAR; v++) {
-      double dl, dr;
-      slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
-      L[v] = Q1[v] + dLk[v];
-      R[v] = Q2[v] - dr;
-      dLk[v] = dl;
+    const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
+                 dzb = a.cellsize[3 * bb + 2];
+    const double AreaZ = dxb * dyb;
+    const double CellVolume = dxb * dyb * dzb;
+    double InvHalf[3];
+    if (POWELL) {
+      InvHalf[0] = 0.5 / dxb;
+      InvHalf[1] = 0.5 / dyb;
+      InvHalf[2] = 0.5 / dzb;
     }
-    err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
-#pragma unroll
-    for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
-    Fz[PP] = vdt;
-    if (POWELL) { Fz[NFZ - 2] = L[BZ]; Fz[NFZ - 1] = R[BZ]; }
-  };
 
-  if (act) {
-    // prologue: slopes of cell -1 and the face below cell 0
-    double Pm2[NVAR];
-    load_prim(-2, Pm2);
-    load_prim(-1, Q1);
-    load_prim(0, Q2);
-    load_prim(1, Q3);
-#pragma unroll
-    for (int v = 0; v < NVAR; v++) {
-      double dl, dr;
-      slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
-      dLk[v] = dl;
-    }
-    zface(FzLo);
-#pragma unroll
-    for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
-  }
+    // this block's run: its own time step (per-run CFL dt)
+    const double *rs = a.scal + (a.runOfBlock ? a.runOfBlock[bb] : 0) * S_NSCAL;
+    const double DtUnscaled = rs[S_DTCB];
+    UpdateCtl uc;
+    uc.DtLocal = a.DtFactor * DtUnscaled;
+    uc.InvVolume = 1.0 / CellVolume;
+    uc.halfStep = a.halfStep;
+    uc.iStage = a.iStage;
+    uc.nStage = a.nStage;
+    uc.dbTrick = a.dbTrick;
+    uc.dtZero = (rs[S_DT] == 0.0);
+    uc.roundTripOld = a.roundTripOld;
 
-  for (int k = 0; k < NB; k++) {
-    // z face k+1 of this column
-    if (act) {
-      load_prim(k + 2, Q3);
-      zface(FzHi);
-      // own primitive of plane k
-#pragma unroll
-      for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
-    }
-    // x/y halo of plane k
-    for (int h = tid; h < BPW * 8 * NB; h += kThreads) {
-      const int lbh = h / (8 * NB);
-      const int bh = blk0 + lbh;
-      if (bh >= a.nBlock) continue;
-      const int r = h % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
-      int ii, jj;
-      if (side < 2) {
-        const int layer = rem & 1, pos = rem >> 1;
-        jj = pos;
-        ii = side == 0 ? layer - 2 : NB + layer;
-      } else {
-        const int layer = rem / NB, pos = rem % NB;
-        ii = pos;
-        jj = side == 2 ? layer - 2 : NB + layer;
-      }
-      double q[NVAR];
-      load_state(U, vs, cell_off<NB>(a.nei, bh, ii, jj, k), q);
+    // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
+    // left-state slope dLk of cell k; each cell's slopes are computed once and
+    // serve the faces below and above it
+    double Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
+    // z-face fluxes of the lower and upper face of the cell; with the 8-wave
+    // scheme two more slots carry the normal-field face states Bz_L, Bz_R
+    constexpr int NFZ = POWELL ? NFS + 2 : NFS;
+    double FzLo[NFZ], FzHi[NFZ];
+    double dtmin = 1.7976931348623157e308;
+
+    auto load_prim = [&](int kk, double *q) {
+      const long off = cell_off<NB>(a.nei, bb, i, j, kk);
+      load_state(U, vs, off, q);
       to_primitive(q);
-#pragma unroll
-      for (int v = 0; v < NVAR; v++) sP[sPi(lbh, v, jj + 2, ii + 2)] = q[v];
-    }
-    __syncthreads();
-    // x and y faces of plane k
-    for (int f = tid; f < BPW * 2 * NFX; f += kThreads) {
-      const int lbf = f / (2 * NFX);
-      const int bf = blk0 + lbf;
-      if (bf >= a.nBlock) continue;
-      const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
-      const int row = q / (NB + 1), fp = q % (NB + 1);
-      double c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
-      double F[NFLUX], vdt, unA;
-      const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
-                   dzf = a.cellsize[3 * bf + 2];
-      if (dir == 0) {
-#pragma unroll
-        for (int v = 0; v < NVAR; v++) {
-          c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
-          c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
-          c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
-          c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
-        }
-        err |= face_flux<0, FLUX, LIM>(ph, dyf * dzf, beta, c0, c1, c2, c3, F,
-                                       vdt, unA);
-      } else {
-#pragma unroll
-        for (int v = 0; v < NVAR; v++) {
-          c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
-          c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
-          c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
-          c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
-        }
-        err |= face_flux<1, FLUX, LIM>(ph, dxf * dzf, beta, c0, c1, c2, c3, F,
-                                       vdt, unA);
+    };
+    // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
+    auto zface = [&](double *Fz) {
+      double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+  #pragma unroll
+      for (int v = 0; v < NVAR; v++) {
+        double dl, dr;
+        slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
+        L[v] = Q1[v] + dLk[v];
+        R[v] = Q2[v] - dr;
+        dLk[v] = dl;
       }
-      F[PP] = vdt;
-#pragma unroll
-      for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
-    }
-    __syncthreads();
-    if (act) {
-      const int qx0 = j * (NB + 1) + i, qy0 = i * (NB + 1) + j;
-      double dF[NFLUX];
-#pragma unroll
-      for (int v = 0; v < NFLUX; v++) {
-        if (v == PP) { dF[v] = 0.0; continue; }
-        double d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
-        d = d + sF[sFi(lb, 1, v, qy0)] - sF[sFi(lb, 1, v, qy0 + 1)];
-        d = d + FzLo[v] - FzHi[v];
-        dF[v] = d;
-      }
-      const long off = (((long)b * NB + k) * NB + j) * NB + i;
-      if (!a.noUpdate) {
-        double Uc[NVAR], Uo[NVAR], Un[NVAR];
-        load_state(a.Uold, vs, off, Uo);
-        if (POWELL) {
-          // calc_source: the div B source from the normal-field face states
-          // of the six faces (x and y recomputed from the LDS plane exactly
-          // as the face loop computed them) and the stage-input cell state
-          load_state(U, vs, off, Uc);
-          double p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
-#pragma unroll
-          for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
-          face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
-          face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
-#pragma unroll
-          for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
-          face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
-          face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
-          double Src[NFLUX];
-          powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
-                        RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
-                        FzHi[NFZ - 1], Uc, Src);
-          update_cell(ph, uc, dF, Uc, Uo, Un, Src);
-        } else {
-          // the stage input at the cell is used only by the SSP-RK2
-          // interpolation and for a zero time step; otherwise re-read the
-          // just-loaded StateOld lines (cache hits) instead of the U* stream
-          load_state((uc.dtZero || !uc.halfStep) ? U : a.Uold, vs, off, Uc);
-          update_cell(ph, uc, dF, Uc, Uo, Un);
-        }
-#pragma unroll
-        for (int v = 0; v < NVAR; v++) a.Uout[v * vs + off] = Un[v];
-      }
-      if (a.doDt) {
-        const double Vdt =
-            fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
-            fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
-            fmax2(FzLo[PP], FzHi[PP]);
-        const double dtc = CellVolume / Vdt;
-        if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
-        dtmin = fmin2(dtmin, dtc);
-      }
-      // shift the z window
-#pragma unroll
-      for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
-#pragma unroll
-      for (int v = 0; v < NFZ; v++) FzLo[v] = FzHi[v];
-    }
-    // the div B source reads neighbouring cells of the LDS plane, which the
-    // next plane overwrites
-    if (POWELL) __syncthreads();
-  }
+      err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
+  #pragma unroll
+      for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
+      Fz[PP] = vdt;
+      if (POWELL) { Fz[NFZ - 2] = L[BZ]; Fz[NFZ - 1] = R[BZ]; }
+    };
 
-  if (a.doDt) {
-    // block minimum of the local time step (calc_timestep :236-256)
-    double m = wave_min(act ? dtmin : 1.7976931348623157e308);
-    const int w = tid / 64;
-    if ((tid & 63) == 0) sRed[w] = m;
-    __syncthreads();
-    if (act && col == 0) {
-      constexpr int WPB = NC >= 64 ? NC / 64 : 1;  // waves per block
-      double r = sRed[(lb * NC) / 64];
-      for (int q = 1; q < WPB; q++) r = fmin2(r, sRed[(lb * NC) / 64 + q]);
-      a.dtmaxB[b] = r;
+    if (act) {
+      // prologue: slopes of cell -1 and the face below cell 0
+      double Pm2[NVAR];
+      load_prim(-2, Pm2);
+      load_prim(-1, Q1);
+      load_prim(0, Q2);
+      load_prim(1, Q3);
+  #pragma unroll
+      for (int v = 0; v < NVAR; v++) {
+        double dl, dr;
+        slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
+        dLk[v] = dl;
+      }
+      zface(FzLo);
+  #pragma unroll
+      for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
     }
-  }
+
+    for (int k = 0; k < NB; k++) {
+      // z face k+1 of this column
+      if (act) {
+        load_prim(k + 2, Q3);
+        zface(FzHi);
+        // own primitive of plane k
+  #pragma unroll
+        for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
+      }
+      // x/y halo of plane k
+      for (int h = tid; h < BPW * 8 * NB; h += kThreads) {
+        const int lbh = h / (8 * NB);
+        const int bh = blk0 + lbh;
+        if (bh >= a.nBlock) continue;
+        const int r = h % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
+        int ii, jj;
+        if (side < 2) {
+          const int layer = rem & 1, pos = rem >> 1;
+          jj = pos;
+          ii = side == 0 ? layer - 2 : NB + layer;
+        } else {
+          const int layer = rem / NB, pos = rem % NB;
+          ii = pos;
+          jj = side == 2 ? layer - 2 : NB + layer;
+        }
+        double q[NVAR];
+        load_state(U, vs, cell_off<NB>(a.nei, bh, ii, jj, k), q);
+        to_primitive(q);
+  #pragma unroll
+        for (int v = 0; v < NVAR; v++) sP[sPi(lbh, v, jj + 2, ii + 2)] = q[v];
+      }
+      __syncthreads();
+      // x and y faces of plane k
+      for (int f = tid; f < BPW * 2 * NFX; f += kThreads) {
+        const int lbf = f / (2 * NFX);
+        const int bf = blk0 + lbf;
+        if (bf >= a.nBlock) continue;
+        const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
+        const int row = q / (NB + 1), fp = q % (NB + 1);
+        double c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
+        double F[NFLUX], vdt, unA;
+        const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
+                     dzf = a.cellsize[3 * bf + 2];
+        if (dir == 0) {
+  #pragma unroll
+          for (int v = 0; v < NVAR; v++) {
+            c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
+            c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
+            c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
+            c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
+          }
+          err |= face_flux<0, FLUX, LIM>(ph, dyf * dzf, beta, c0, c1, c2, c3, F,
+                                         vdt, unA);
+        } else {
+  #pragma unroll
+          for (int v = 0; v < NVAR; v++) {
+            c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
+            c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
+            c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
+            c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
+          }
+          err |= face_flux<1, FLUX, LIM>(ph, dxf * dzf, beta, c0, c1, c2, c3, F,
+                                         vdt, unA);
+        }
+        F[PP] = vdt;
+  #pragma unroll
+        for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
+      }
+      __syncthreads();
+      if (act) {
+        const int qx0 = j * (NB + 1) + i, qy0 = i * (NB + 1) + j;
+        double dF[NFLUX];
+  #pragma unroll
+        for (int v = 0; v < NFLUX; v++) {
+          if (v == PP) { dF[v] = 0.0; continue; }
+          double d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
+          d = d + sF[sFi(lb, 1, v, qy0)] - sF[sFi(lb, 1, v, qy0 + 1)];
+          d = d + FzLo[v] - FzHi[v];
+          dF[v] = d;
+        }
+        const long off = (((long)b * NB + k) * NB + j) * NB + i;
+        if (!a.noUpdate) {
+          double Uc[NVAR], Uo[NVAR], Un[NVAR];
+          load_state(a.Uold, vs, off, Uo);
+          if (POWELL) {
+            // calc_source: the div B source from the normal-field face states
+            // of the six faces (x and y recomputed from the LDS plane exactly
+            // as the face loop computed them) and the stage-input cell state
+            load_state(U, vs, off, Uc);
+            double p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
+  #pragma unroll
+            for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
+            face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
+            face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
+  #pragma unroll
+            for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
+            face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
+            face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
+            double Src[NFLUX];
+            powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
+                          RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
+                          FzHi[NFZ - 1], Uc, Src);
+            update_cell(ph, uc, dF, Uc, Uo, Un, Src);
+          } else {
+            // the stage input at the cell is used only by the SSP-RK2
+            // interpolation and for a zero time step; otherwise re-read the
+            // just-loaded StateOld lines (cache hits) instead of the U* stream
+            load_state((uc.dtZero || !uc.halfStep) ? U : a.Uold, vs, off, Uc);
+            update_cell(ph, uc, dF, Uc, Uo, Un);
+          }
+  #pragma unroll
+          for (int v = 0; v < NVAR; v++) a.Uout[v * vs + off] = Un[v];
+        }
+        if (a.doDt) {
+          const double Vdt =
+              fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
+              fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
+              fmax2(FzLo[PP], FzHi[PP]);
+          const double dtc = CellVolume / Vdt;
+          if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
+          dtmin = fmin2(dtmin, dtc);
+        }
+        // shift the z window
+  #pragma unroll
+        for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+  #pragma unroll
+        for (int v = 0; v < NFZ; v++) FzLo[v] = FzHi[v];
+      }
+      // the div B source reads neighbouring cells of the LDS plane, which the
+      // next plane overwrites
+      if (POWELL) __syncthreads();
+    }
+
+    if (a.doDt) {
+      // block minimum of the local time step (calc_timestep :236-256)
+      double m = wave_min(act ? dtmin : 1.7976931348623157e308);
+      const int w = tid / 64;
+      if ((tid & 63) == 0) sRed[w] = m;
+      __syncthreads();
+      if (act && col == 0) {
+        constexpr int WPB = NC >= 64 ? NC / 64 : 1;  // waves per block
+        double r = sRed[(lb * NC) / 64];
+        for (int q = 1; q < WPB; q++) r = fmin2(r, sRed[(lb * NC) / 64 + q]);
+        a.dtmaxB[b] = r;
+      }
+    }
+    __syncthreads();  // LDS is reused by the next group
+  }  // group loop
   if (err) atomicOr(a.err, err);
-  if (a.advanceTime && blockIdx.x == 0 && tid == 0)
-    a.scal[S_TSIM] = a.scal[S_TSIM] + a.scal[S_DT] / a.nStage;
+  if (a.advanceTime && blockIdx.x == 0)
+    for (int r = tid; r < a.nRun; r += kThreads) {
+      double *rs = a.scal + r * S_NSCAL;
+      rs[S_TSIM] = rs[S_TSIM] + rs[S_DT] / a.nStage;
+    }
 }
 
 // Launch k_stage for one block size and flux; the limiter and the 8-wave
@@ -374,8 +389,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
 template <int NB, int FLUX>
 inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
                                       hipStream_t stream) {
-  const int bpw = kThreads / (NB * NB);
-  const dim3 grid((a.nBlock + bpw - 1) / bpw), block(kThreads);
+  const dim3 grid(a.nGroup < a.nWorkgroup ? a.nGroup : a.nWorkgroup),
+      block(kThreads);
 #define BATS_LAUNCH(LM)                                                     \
   if (powell)                                                               \
     hipLaunchKernelGGL((k_stage<NB, FLUX, LM, true>), grid, block, 0,       \

@@ -39,7 +39,8 @@ module ModHipBridge
        bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
-       bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms
+       bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
+       bats_set_runs, bats_get_run_times, bats_set_persistent_grid
   ! reference-named routines on the module-level handle
   public:: init_hip_bridge, clean_hip_bridge, sync_state_to_gpu, &
        sync_state_to_cpu, calc_face_value, calc_face_flux, calc_source, &
@@ -212,6 +213,26 @@ module ModHipBridge
        import:: c_int, c_ptr
        type(c_ptr), value:: Handle
      end function bats_synchronize
+     integer(c_int) function bats_set_runs(Handle, nRun, iBlockFirst_R) &
+          bind(C, name="bats_set_runs")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: nRun
+       integer(c_int), intent(in):: iBlockFirst_R(*)
+     end function bats_set_runs
+     integer(c_int) function bats_get_run_times(Handle, Dt_R, &
+          tSimulation_R, nStep_R) bind(C, name="bats_get_run_times")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(out):: Dt_R(*), tSimulation_R(*)
+       integer(c_int), intent(out):: nStep_R(*)
+     end function bats_get_run_times
+     integer(c_int) function bats_set_persistent_grid(Handle, nWorkgroup) &
+          bind(C, name="bats_set_persistent_grid")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: nWorkgroup
+     end function bats_set_persistent_grid
      integer(c_int) function bats_check_nan(Handle) &
           bind(C, name="bats_check_nan")
        import:: c_int, c_ptr

@@ -182,6 +182,23 @@ int bats_get_time(bats_handle *h, double *Dt, double *tSimulation,
                   int *nStep);
 /* Wait for queued work and return the first error raised on the device */
 int bats_synchronize(bats_handle *h);
+
+/* ------------------------------------------------------------------------
+ * Ensembles.  No counterpart in the reference, which advances one solution
+ * per process: here many independent runs share the handle's flat block
+ * list and advance together in every stage launch, each with its own CFL
+ * time step (set_global_timestep reduces per run), simulation time and step
+ * counter.  Run r owns the blocks iBlockFirst_R[r] .. iBlockFirst_R[r+1]-1
+ * (0-based; iBlockFirst_R[0] = 0, iBlockFirst_R[nRun] = nBlock); no
+ * neighbour link may cross a run boundary.  Resets every run's time. */
+int bats_set_runs(bats_handle *h, int nRun, const int *iBlockFirst_R);
+/* Per-run Dt (CFL-scaled), simulation time and step count (nRun each). */
+int bats_get_run_times(bats_handle *h, double *Dt_R, double *tSimulation_R,
+                       int *nStep_R);
+/* Launch shape of the stage kernel: 0 (default) launches one workgroup per
+ * group of blocks; n > 0 launches at most n workgroups that walk the flat
+ * block work list persistently. */
+int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
 /* check_nan (src/ModUpdateState.f90:2003-2040): BATS_ENAN if any state
  * value is NaN or infinite */
 int bats_check_nan(bats_handle *h);
