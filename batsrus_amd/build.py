@@ -18,9 +18,11 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 
-# the fused stage kernel, one object per (block size, face flux)
-STAGE_UNITS = [(nb, flux) for nb in (8, 16)
-               for flux in ("rusanov", "linde", "hlld")]
+# the fused stage kernel, one object per (block size, face flux, storage
+# type)
+STAGE_UNITS = [(nb, flux, real) for nb in (8, 16)
+               for flux in ("rusanov", "linde", "hlld")
+               for real in ("f64", "f32")]
 
 
 def _run(cmd, cwd=ROOT):
@@ -41,10 +43,11 @@ def build_hip(jobs=None):
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + csrc]
     units = [(os.path.join(csrc, "bats_hip.hip"),
               os.path.join(odir, "bats_hip.o"), [])]
-    for nb, flux in STAGE_UNITS:
+    for nb, flux, real in STAGE_UNITS:
         units.append((os.path.join(csrc, "stage_inst.hip"),
-                      os.path.join(odir, "stage_%d_%s.o" % (nb, flux)),
-                      ["-DBATS_NB=%d" % nb, "-DBATS_FLUX=%s" % flux]))
+                      os.path.join(odir, "stage_%d_%s_%s.o" % (nb, flux, real)),
+                      ["-DBATS_NB=%d" % nb, "-DBATS_FLUX=%s" % flux,
+                       "-DBATS_REAL=%s" % real]))
     jobs = jobs or min(len(units), os.cpu_count() or 1, 16)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(_run, [HIPCC] + HIP_FLAGS + inc + defs +

@@ -55,7 +55,7 @@ class BatsConfig(C.Structure):
                 ("pMin", C.c_double), ("UseDbTrick", C.c_int),
                 ("UseDtFixed", C.c_int), ("DtFixed", C.c_double),
                 ("cTiny", C.c_double), ("iDevice", C.c_int),
-                ("UseDivbSource", C.c_int)]
+                ("UseDivbSource", C.c_int), ("StatePrecision", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -133,7 +133,8 @@ def _ptr(a, ct=C.c_double):
 SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        nStage=2, UseHalfStep=True, cfl=0.8, gamma=5.0 / 3.0,
                        pMin=-1.0, UseDbTrick=False, UseDtFixed=False,
-                       DtFixed=0.0, cTiny=1e-6, UseDivbSource=False)
+                       DtFixed=0.0, cTiny=1e-6, UseDivbSource=False,
+                       precision="f64")
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):
@@ -159,6 +160,10 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.cTiny = s["cTiny"]
     c.iDevice = device
     c.UseDivbSource = int(s["UseDivbSource"])
+    prec = s["precision"]
+    if prec not in ("f64", "f32"):
+        raise ValueError("precision must be 'f64' or 'f32'")
+    c.StatePrecision = 8 if prec == "f64" else 4
     return c
 
 

@@ -249,7 +249,8 @@ k_global_dt(const double *dtmaxB, const int *runFirst, double *scalAll,
   }
 }
 
-__global__ void k_check_nan(const double *U, long n, int *err) {
+template <typename T>
+__global__ void k_check_nan(const T *U, long n, int *err) {
   int bad = 0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
        t += (long)gridDim.x * blockDim.x)
@@ -258,9 +259,9 @@ __global__ void k_check_nan(const double *U, long n, int *err) {
 }
 
 // per-block sums of U*V (rho, rho*u, B, total energy) in a fixed order
-template <int NB>
+template <int NB, typename T>
 __global__ void __launch_bounds__(256)
-k_integrate(const double *U, long vs, const double *cellsize, Phys ph,
+k_integrate(const T *U, long vs, const double *cellsize, Phys ph,
             int nBlock, double *partial) {
   __shared__ double sRed[256];
   const int b = blockIdx.x;
@@ -268,8 +269,10 @@ k_integrate(const double *U, long vs, const double *cellsize, Phys ph,
   double acc[NVAR];
   for (int v = 0; v < NVAR; v++) acc[v] = 0.0;
   for (int c = threadIdx.x; c < NB * NB * NB; c += 256) {
+    T qt[NVAR];
     double q[NVAR];
-    load_state(U, vs, (long)b * NB * NB * NB + c, q);
+    load_state(U, vs, (long)b * NB * NB * NB + c, qt);
+    for (int v = 0; v < NVAR; v++) q[v] = double(qt[v]);
     q[PP] = pressure_to_energy(ph, q);
     for (int v = 0; v < NVAR; v++) acc[v] += q[v] * V;
   }
@@ -295,7 +298,8 @@ struct bats_handle {
   int NB;
   long nCellBlk, nCell;
   hipStream_t stream;
-  double *U[2];
+  void *U[2];    // state buffers of elemSize-byte reals
+  int elemSize;  // 8: FP64 state, 4: FP32 state
   int cur;       // buffer holding the state at the start of a step
   int stateBuf;  // buffer holding the current State_VGB (differs from cur
                  // between the stages of a 2-stage step)
@@ -387,6 +391,8 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (c.nStage != 1 && c.nStage != 2) return BATS_EINVAL;
   if (!(c.Gamma > 1.0) || !(c.Cfl > 0.0)) return BATS_EINVAL;
   if (c.UseDivbSource != 0 && c.UseDivbSource != 1) return BATS_EINVAL;
+  if (c.StatePrecision != 0 && c.StatePrecision != 4 && c.StatePrecision != 8)
+    return BATS_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.iDevice || c.iDevice < 0)
     return BATS_ENODEV;
@@ -410,7 +416,8 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   h->lastStageMs = 0.0;
   h->lastStageLaunches = 0;
   hipError_t e = hipStreamCreate(&h->stream);
-  const size_t sz = sizeof(double) * NVAR * h->nCell;
+  h->elemSize = (c.StatePrecision == 4) ? 4 : 8;
+  const size_t sz = (size_t)h->elemSize * NVAR * h->nCell;
   if (e == hipSuccess) e = hipMalloc(&h->U[0], sz);
   if (e == hipSuccess) e = hipMalloc(&h->U[1], sz);
   if (e == hipSuccess) e = hipMalloc(&h->nei, sizeof(int) * 6 * c.nBlock);
@@ -538,42 +545,57 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
     }
 }
 
+// host SoA (double) <-> device state buffer, converting to the state type
+static int put_soa(bats_handle *h, const double *U) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t n = (size_t)NVAR * h->nCell;
+  if (h->elemSize == 8) {
+    HIPCHK(hipMemcpy(h->U[h->cur], U, sizeof(double) * n,
+                     hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(U, U + n);
+    HIPCHK(hipMemcpy(h->U[h->cur], f.data(), sizeof(float) * n,
+                     hipMemcpyHostToDevice));
+  }
+  h->stateBuf = h->cur;
+  return BATS_OK;
+}
+static int get_soa(bats_handle *h, double *U) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t n = (size_t)NVAR * h->nCell;
+  if (h->elemSize == 8) {
+    HIPCHK(hipMemcpy(U, h->U[h->stateBuf], sizeof(double) * n,
+                     hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpy(f.data(), h->U[h->stateBuf], sizeof(float) * n,
+                     hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < n; t++) U[t] = f[t];
+  }
+  return BATS_OK;
+}
+
 extern "C" int bats_upload_state(bats_handle *h, const double *State_VGB) {
   if (!h || !State_VGB) return BATS_EINVAL;
   std::vector<double> tmp((size_t)NVAR * h->nCell);
   aos_to_soa(h->cfg, State_VGB, tmp.data());
-  HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(h->U[h->cur], tmp.data(), sizeof(double) * tmp.size(),
-                   hipMemcpyHostToDevice));
-  h->stateBuf = h->cur;
-  return BATS_OK;
+  return put_soa(h, tmp.data());
 }
-
 extern "C" int bats_download_state(bats_handle *h, double *State_VGB) {
   if (!h || !State_VGB) return BATS_EINVAL;
   std::vector<double> tmp((size_t)NVAR * h->nCell);
-  HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(tmp.data(), h->U[h->stateBuf], sizeof(double) * tmp.size(),
-                   hipMemcpyDeviceToHost));
+  int st = get_soa(h, tmp.data());
+  if (st) return st;
   soa_to_aos(h->cfg, h->hNei, tmp.data(), State_VGB);
   return BATS_OK;
 }
-
 extern "C" int bats_upload_state_soa(bats_handle *h, const double *U) {
   if (!h || !U) return BATS_EINVAL;
-  HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(h->U[h->cur], U, sizeof(double) * NVAR * h->nCell,
-                   hipMemcpyHostToDevice));
-  h->stateBuf = h->cur;
-  return BATS_OK;
+  return put_soa(h, U);
 }
-
 extern "C" int bats_download_state_soa(bats_handle *h, double *U) {
   if (!h || !U) return BATS_EINVAL;
-  HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(U, h->U[h->stateBuf], sizeof(double) * NVAR * h->nCell,
-                   hipMemcpyDeviceToHost));
-  return BATS_OK;
+  return get_soa(h, U);
 }
 
 // ---- per-block routines ---------------------------------------------
@@ -581,9 +603,9 @@ static int ensure_block_buffers(bats_handle *h) {
   if (h->Usnap) return BATS_OK;
   const int NB = h->NB;
   const size_t nF = (size_t)(NB + 1) * NB * NB;
-  HIPCHK(hipMalloc(&h->Usnap, sizeof(double) * NVAR * h->nCell));
+  HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->nCell));
   HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
-                        sizeof(double) * NVAR * h->nCell,
+                        (size_t)h->elemSize * NVAR * h->nCell,
                         hipMemcpyDeviceToDevice, h->stream));
   double **fs[] = {&h->LX, &h->RX, &h->LY, &h->RY, &h->LZ, &h->RZ};
   for (double **p : fs) {
@@ -608,8 +630,8 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   BlockArgs a;
   const bats_config &c = h->cfg;
   a.Usnap = h->Usnap;
-  a.State = h->U[h->stateBuf];
-  a.StateOld = h->U[1 - h->stateBuf];
+  a.State = static_cast<double *>(h->U[h->stateBuf]);
+  a.StateOld = static_cast<double *>(h->U[1 - h->stateBuf]);
   a.nei = h->nei;
   a.cellsize = h->cellsize;
   a.scal = h->scal + (long)h->hRunOfBlock[iBlock] * S_NSCAL;
@@ -643,7 +665,7 @@ extern "C" int bats_exchange_messages(bats_handle *h) {
   int st = ensure_block_buffers(h);
   if (st) return st;
   HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
-                        sizeof(double) * NVAR * h->nCell,
+                        (size_t)h->elemSize * NVAR * h->nCell,
                         hipMemcpyDeviceToDevice, h->stream));
   return BATS_OK;
 }
@@ -656,6 +678,8 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
+  // the reference-shaped per-block routines run on FP64 state only
+  if (h->elemSize != 8) return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
 
@@ -839,15 +863,20 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
     HIPCHK(hipEventRecord(h->evStart[ev], h->stream));
   }
   const bool powell = c.UseDivbSource != 0;
+  const bool f32 = h->elemSize == 4;
   hipError_t e;
+#define BATS_PICK(NBv, FL)                                                   \
+  (f32 ? launch_stage_##NBv##_##FL##_f32(a, powell, h->stream)               \
+       : launch_stage_##NBv##_##FL##_f64(a, powell, h->stream))
   if (NB == 8)
-    e = c.iFlux == BATS_FLUX_RUSANOV ? launch_stage_8_rusanov(a, powell, h->stream)
-      : c.iFlux == BATS_FLUX_LINDE   ? launch_stage_8_linde(a, powell, h->stream)
-                                     : launch_stage_8_hlld(a, powell, h->stream);
+    e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(8, rusanov)
+      : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(8, linde)
+                                     : BATS_PICK(8, hlld);
   else
-    e = c.iFlux == BATS_FLUX_RUSANOV ? launch_stage_16_rusanov(a, powell, h->stream)
-      : c.iFlux == BATS_FLUX_LINDE   ? launch_stage_16_linde(a, powell, h->stream)
-                                     : launch_stage_16_hlld(a, powell, h->stream);
+    e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(16, rusanov)
+      : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(16, linde)
+                                     : BATS_PICK(16, hlld);
+#undef BATS_PICK
   HIPCHK(e);
   if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));
   if (!noUpdate) {
@@ -985,8 +1014,14 @@ extern "C" int bats_synchronize(bats_handle *h) {
 extern "C" int bats_check_nan(bats_handle *h) {
   if (!h) return BATS_EINVAL;
   const long n = NVAR * h->nCell;
-  hipLaunchKernelGGL(k_check_nan, dim3(1024), dim3(256), 0, h->stream,
-                     h->U[h->stateBuf], n, h->err);
+  if (h->elemSize == 8)
+    hipLaunchKernelGGL(k_check_nan<double>, dim3(1024), dim3(256), 0,
+                       h->stream, static_cast<const double *>(h->U[h->stateBuf]),
+                       n, h->err);
+  else
+    hipLaunchKernelGGL(k_check_nan<float>, dim3(1024), dim3(256), 0,
+                       h->stream, static_cast<const float *>(h->U[h->stateBuf]),
+                       n, h->err);
   HIPCHK(hipGetLastError());
   return bats_synchronize(h);
 }
@@ -994,14 +1029,22 @@ extern "C" int bats_check_nan(bats_handle *h) {
 extern "C" int bats_integrate_domain(bats_handle *h, double *Sum_V) {
   if (!h || !Sum_V) return BATS_EINVAL;
   const bats_config &c = h->cfg;
-  if (h->NB == 8)
-    hipLaunchKernelGGL(k_integrate<8>, dim3(c.nBlock), dim3(256), 0, h->stream,
-                       h->U[h->stateBuf], h->nCell, h->cellsize, make_phys(c),
-                       c.nBlock, h->partial);
+  const void *U = h->U[h->stateBuf];
+  const double *Ud = static_cast<const double *>(U);
+  const float *Uf = static_cast<const float *>(U);
+  const dim3 g(c.nBlock), t(256);
+  if (h->NB == 8 && h->elemSize == 8)
+    hipLaunchKernelGGL((k_integrate<8, double>), g, t, 0, h->stream, Ud,
+                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+  else if (h->NB == 8)
+    hipLaunchKernelGGL((k_integrate<8, float>), g, t, 0, h->stream, Uf,
+                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+  else if (h->elemSize == 8)
+    hipLaunchKernelGGL((k_integrate<16, double>), g, t, 0, h->stream, Ud,
+                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
   else
-    hipLaunchKernelGGL(k_integrate<16>, dim3(c.nBlock), dim3(256), 0, h->stream,
-                       h->U[h->stateBuf], h->nCell, h->cellsize, make_phys(c),
-                       c.nBlock, h->partial);
+    hipLaunchKernelGGL((k_integrate<16, float>), g, t, 0, h->stream, Uf,
+                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
   HIPCHK(hipGetLastError());
   std::vector<double> p((size_t)NVAR * c.nBlock);
   HIPCHK(hipMemcpyAsync(p.data(), h->partial, sizeof(double) * p.size(),

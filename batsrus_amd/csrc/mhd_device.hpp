@@ -1,4 +1,4 @@
-// mhd_device.hpp -- FP64 device arithmetic of the ideal-MHD stage update
+// mhd_device.hpp -- device arithmetic of the ideal-MHD stage update
 // (limited MUSCL face states, Rusanov / Linde / HLLD face fluxes, wave
 // speeds, energy <-> pressure, stage update) for gfx950.
 //
@@ -12,6 +12,11 @@
 //   HLLD               src/ModFaceFlux.f90:2826-3160
 //   energy/pressure    src/ModEnergy.f90:47-127,243-332,388-451
 //   stage update       src/ModUpdateState.f90:191-965
+// The face arithmetic (limiter, face states, fluxes, speeds) is templated
+// on its real type: double for the plain path, float for the FP32-state
+// path; the stage update itself (energy <-> pressure, U = Uold + dt*R,
+// pressure floor) always runs in double.
+//
 // Faces are Cartesian: the unit normal's zero components only add signed
 // zeros in the reference, so they are dropped; sums the reference forms in
 // x,y,z order (|B|^2, |u|^2, Poynting term) keep that order, sums it forms
@@ -36,20 +41,38 @@ enum { LIM_FIRST = 0, LIM_MINMOD = 1, LIM_MC = 2, LIM_MC3 = 3, LIM_BETA = 4,
 // device error bits
 enum { ERRB_NEGFAST2 = 1, ERRB_HLLD = 2, ERRB_NAN = 4 };
 
-struct Phys {
-  double gamma, gm1, invgm1, cTiny, pMin;
+// physical constants in the arithmetic type of the face computation
+template <typename Real> struct PhysT {
+  Real gamma, gm1, invgm1, cTiny, pMin;
 };
+using Phys = PhysT<double>;
+template <typename Real>
+__device__ __forceinline__ PhysT<Real> phys_as(const Phys &p) {
+  return PhysT<Real>{Real(p.gamma), Real(p.gm1), Real(p.invgm1),
+                     Real(p.cTiny), Real(p.pMin)};
+}
 
 // Fortran SIGN(a, b) for a > 0
 __device__ __forceinline__ double fsign(double a, double b) {
-  return copysign(a, b);
+  return __builtin_copysign(a, b);
 }
+__device__ __forceinline__ float fsign(float a, float b) {
+  return __builtin_copysignf(a, b);
+}
+__device__ __forceinline__ double fabsr(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float fabsr(float a) { return __builtin_fabsf(a); }
 // Fortran MIN / MAX of two (finite) arguments
 __device__ __forceinline__ double fmin2(double a, double b) {
   return __builtin_fmin(a, b);
 }
 __device__ __forceinline__ double fmax2(double a, double b) {
   return __builtin_fmax(a, b);
+}
+__device__ __forceinline__ float fmin2(float a, float b) {
+  return __builtin_fminf(a, b);
+}
+__device__ __forceinline__ float fmax2(float a, float b) {
+  return __builtin_fmaxf(a, b);
 }
 // 1/x: v_rcp_f64 + two Newton-Raphson steps
 __device__ __forceinline__ double frcp(double x) {
@@ -59,7 +82,13 @@ __device__ __forceinline__ double frcp(double x) {
   e = __builtin_fma(-x, r, 1.0);
   return __builtin_fma(r, e, r);
 }
-__device__ __forceinline__ double fdiv(double a, double b) {
+// 1/x in FP32: v_rcp_f32 + one Newton-Raphson step
+__device__ __forceinline__ float frcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+template <typename Real> __device__ __forceinline__ Real fdiv(Real a, Real b) {
   return a * frcp(b);
 }
 // sqrt(x), x >= 0: v_rsq_f64 + Goldschmidt refinement
@@ -73,41 +102,45 @@ __device__ __forceinline__ double fsqrt(double x) {
   g = __builtin_fma(d, h, g);
   return x == 0.0 ? 0.0 : g;
 }
+// sqrt(x) in FP32: v_sqrt_f32
+__device__ __forceinline__ float fsqrt(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
 
 // ---------------------------------------------------------------------
 // limiter (half slopes, src/ModFaceValue.f90:3689-3791): slopes of the
 // cell l from p0=P(l-1), p1=P(l), p2=P(l+1).  dL builds the left state of
 // the face to the right of the cell, dR the right state of the face to its
 // left.
-template <int LIM>
-__device__ __forceinline__ void slopes(double beta, double p0, double p1,
-                                       double p2, double &dL, double &dR) {
-  const double cThird = 1. / 3.;
-  const double d1 = p1 - p0, d2 = p2 - p1;
+template <int LIM, typename Real>
+__device__ __forceinline__ void slopes(Real beta, Real p0, Real p1,
+                                       Real p2, Real &dL, Real &dR) {
+  const Real cThird = Real(1.) / Real(3.);
+  const Real d1 = p1 - p0, d2 = p2 - p1;
   if (LIM == LIM_MC3) {
-    const double sg = fsign(0.25, d1) + fsign(0.25, d2);
-    const double m = fmin2(beta * fabs(d1), beta * fabs(d2));
-    dR = sg * fmin2(m, cThird * fabs(2 * d1 + d2));
-    dL = sg * fmin2(m, cThird * fabs(d1 + 2 * d2));
+    const Real sg = fsign(Real(0.25), d1) + fsign(Real(0.25), d2);
+    const Real m = fmin2(beta * fabsr(d1), beta * fabsr(d2));
+    dR = sg * fmin2(m, cThird * fabsr(2 * d1 + d2));
+    dL = sg * fmin2(m, cThird * fabsr(d1 + 2 * d2));
   } else if (LIM == LIM_MC) {
-    const double s = (fsign(0.25, d1) + fsign(0.25, d2)) *
-                     fmin2(fmin2(beta * fabs(d1), beta * fabs(d2)),
-                           0.5 * fabs(d1 + d2));
+    const Real s = (fsign(Real(0.25), d1) + fsign(Real(0.25), d2)) *
+                     fmin2(fmin2(beta * fabsr(d1), beta * fabsr(d2)),
+                           Real(0.5) * fabsr(d1 + d2));
     dL = s; dR = s;
   } else if (LIM == LIM_MINMOD) {
-    const double s =
-        (fsign(0.25, d2) + fsign(0.25, d1)) * fmin2(fabs(d2), fabs(d1));
+    const Real s =
+        (fsign(Real(0.25), d2) + fsign(Real(0.25), d1)) * fmin2(fabsr(d2), fabsr(d1));
     dL = s; dR = s;
   } else if (LIM == LIM_BETA) {
-    const double a1 = fabs(d1), a2 = fabs(d2);
-    const double sum = fsign(0.25, d2) + fsign(0.25, d1);
-    const double s = sum * fmax2(fmin2(a2, beta * a1), fmin2(a1, beta * a2));
+    const Real a1 = fabsr(d1), a2 = fabsr(d2);
+    const Real sum = fsign(Real(0.25), d2) + fsign(Real(0.25), d1);
+    const Real s = sum * fmax2(fmin2(a2, beta * a1), fmin2(a1, beta * a2));
     dL = s; dR = s;
   } else if (LIM == LIM_NO) {
-    const double s = 0.25 * (p2 - p0);
+    const Real s = Real(0.25) * (p2 - p0);
     dL = s; dR = s;
   } else {
-    dL = 0.0; dR = 0.0;
+    dL = Real(0.0); dR = Real(0.0);
   }
 }
 
@@ -116,15 +149,15 @@ __device__ __forceinline__ void slopes(double beta, double p0, double p1,
 // L = P(f-1) + dL(f-1), R = P(f) - dR(f)  (src/ModFaceValue.f90:2021-2027)
 // left and right states of one variable at the face between cells 1 and 2
 // of the 4-cell stencil q0..q3 (src/ModFaceValue.f90:2021-2027)
-template <int LIM>
-__device__ __forceinline__ void face_lr(double beta, double q0, double q1,
-                                        double q2, double q3, double &L,
-                                        double &R) {
+template <int LIM, typename Real>
+__device__ __forceinline__ void face_lr(Real beta, Real q0, Real q1,
+                                        Real q2, Real q3, Real &L,
+                                        Real &R) {
   if (LIM == LIM_FIRST) {
     L = q1;
     R = q2;
   } else {
-    double dl, dr, t;
+    Real dl, dr, t;
     slopes<LIM>(beta, q0, q1, q2, dl, t);
     slopes<LIM>(beta, q1, q2, q3, t, dr);
     L = q1 + dl;
@@ -132,23 +165,24 @@ __device__ __forceinline__ void face_lr(double beta, double q0, double q1,
   }
 }
 
-template <int LIM>
-__device__ __forceinline__ void face_states(double beta, const double *q0,
-                                            const double *q1, const double *q2,
-                                            const double *q3, double *L,
-                                            double *R) {
+template <int LIM, typename Real>
+__device__ __forceinline__ void face_states(Real beta, const Real *q0,
+                                            const Real *q1, const Real *q2,
+                                            const Real *q3, Real *L,
+                                            Real *R) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++)
     face_lr<LIM>(beta, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
 }
 
 // runtime-limiter variant used by the per-block parity routines
-__device__ __forceinline__ void face_states_rt(int lim, double beta,
-                                               const double *q0,
-                                               const double *q1,
-                                               const double *q2,
-                                               const double *q3, double *L,
-                                               double *R) {
+template <typename Real>
+__device__ __forceinline__ void face_states_rt(int lim, Real beta,
+                                               const Real *q0,
+                                               const Real *q1,
+                                               const Real *q2,
+                                               const Real *q3, Real *L,
+                                               Real *R) {
   switch (lim) {
   case LIM_MINMOD: face_states<LIM_MINMOD>(beta, q0, q1, q2, q3, L, R); break;
   case LIM_MC: face_states<LIM_MC>(beta, q0, q1, q2, q3, L, R); break;
@@ -161,8 +195,9 @@ __device__ __forceinline__ void face_states_rt(int lim, double beta,
 
 // conserved (p stored) -> primitive, u = (1/rho)*rho*u
 // (src/ModFaceValue.f90:1541,1581-1582)
-__device__ __forceinline__ void to_primitive(double *q) {
-  const double RhoInv = frcp(q[RHO]);
+template <typename Real>
+__device__ __forceinline__ void to_primitive(Real *q) {
+  const Real RhoInv = frcp(q[RHO]);
   q[MX] = RhoInv * q[MX];
   q[MY] = RhoInv * q[MY];
   q[MZ] = RhoInv * q[MZ];
@@ -172,64 +207,64 @@ __device__ __forceinline__ void to_primitive(double *q) {
 // physical flux of a primitive state for a face normal to axis DIR
 // (src/ModFaceFlux.f90:1194-1212,1649-1876,1574-1601).  Cons[9] is the
 // conserved state with total energy in slot 8.
-template <int DIR, bool FULL>
-__device__ __forceinline__ void phys_flux(const Phys &ph, const double *S,
-                                          double *Cons, double *F,
-                                          double &UnOut) {
-  const double Rho = S[RHO], p = S[PP];
-  const double u[3] = {S[MX], S[MY], S[MZ]};
-  const double b[3] = {S[BX], S[BY], S[BZ]};
-  const double Un = u[DIR], Bn = b[DIR];
+template <int DIR, bool FULL, typename Real>
+__device__ __forceinline__ void phys_flux(const PhysT<Real> &ph, const Real *S,
+                                          Real *Cons, Real *F,
+                                          Real &UnOut) {
+  const Real Rho = S[RHO], p = S[PP];
+  const Real u[3] = {S[MX], S[MY], S[MZ]};
+  const Real b[3] = {S[BX], S[BY], S[BZ]};
+  const Real Un = u[DIR], Bn = b[DIR];
   F[RHO] = Rho * Un;
   Cons[RHO] = Rho;
   Cons[MX] = Rho * u[0]; Cons[MY] = Rho * u[1]; Cons[MZ] = Rho * u[2];
   Cons[BX] = b[0]; Cons[BY] = b[1]; Cons[BZ] = b[2];
   Cons[PP] = p;
-  const double e = ph.invgm1 * p + 0.5 * Rho * (u[0] * u[0] + u[1] * u[1] +
+  const Real e = ph.invgm1 * p + Real(0.5) * Rho * (u[0] * u[0] + u[1] * u[1] +
                                                 u[2] * u[2]);
-  const double B2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-  const double pTotal = 0.5 * B2;
-  const double UnRho = Un * Rho;
+  const Real B2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  const Real pTotal = Real(0.5) * B2;
+  const Real UnRho = Un * Rho;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    double f = UnRho * u[d];
+    Real f = UnRho * u[d];
     if (d == DIR) f = f + p;
-    double mf = -Bn * b[d];
+    Real mf = -Bn * b[d];
     if (d == DIR) mf = mf + pTotal;
     F[MX + d] = f + mf;
   }
-  double fb[3];
+  Real fb[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) fb[d] = (d == DIR) ? 0.0 : Un * b[d] - u[d] * Bn;
+  for (int d = 0; d < 3; d++) fb[d] = (d == DIR) ? Real(0.0) : Un * b[d] - u[d] * Bn;
   F[BX] = fb[0]; F[BY] = fb[1]; F[BZ] = fb[2];
-  double fe = Un * (e + p);
+  Real fe = Un * (e + p);
 #pragma unroll
   for (int d = 0; d < 3; d++)
     if (d != DIR) fe = fe + fb[d] * b[d];
   F[EN] = fe;
-  Cons[EN] = e + 0.5 * B2;
-  F[PP] = FULL ? Un * p : 0.0;
+  Cons[EN] = e + Real(0.5) * B2;
+  F[PP] = FULL ? Un * p : Real(0.0);
   UnOut = Un;
 }
 
 // fast magnetosonic speed of state S for a face normal to DIR
 // (src/ModFaceFlux.f90:3900-3921,3990-4134).  Returns false on a negative
 // fast speed squared.
-template <int DIR>
-__device__ __forceinline__ bool fast_speed(const Phys &ph, const double *S,
-                                           double &Un, double &Fast) {
-  const double InvRho = frcp(S[RHO]);
-  const double Sound2 = S[PP] * ph.gamma * InvRho;
+template <int DIR, typename Real>
+__device__ __forceinline__ bool fast_speed(const PhysT<Real> &ph, const Real *S,
+                                           Real &Un, Real &Fast) {
+  const Real InvRho = frcp(S[RHO]);
+  const Real Sound2 = S[PP] * ph.gamma * InvRho;
   Un = S[MX + DIR];
-  const double bx = S[BX], by = S[BY], bz = S[BZ];
-  const double Alfven2 = (bx * bx + by * by + bz * bz) * InvRho;
-  const double FullBn = S[BX + DIR];
-  const double Alfven2Normal = InvRho * (FullBn * FullBn);
-  const double Fast2 = Sound2 + Alfven2;
-  const double Discr =
-      fsqrt(fmax2(0.0, Fast2 * Fast2 - 4 * Sound2 * Alfven2Normal));
-  const bool ok = !(Fast2 + Discr < 0.0);
-  Fast = fsqrt(0.5 * (Fast2 + Discr));
+  const Real bx = S[BX], by = S[BY], bz = S[BZ];
+  const Real Alfven2 = (bx * bx + by * by + bz * bz) * InvRho;
+  const Real FullBn = S[BX + DIR];
+  const Real Alfven2Normal = InvRho * (FullBn * FullBn);
+  const Real Fast2 = Sound2 + Alfven2;
+  const Real Discr =
+      fsqrt(fmax2(Real(0.0), Fast2 * Fast2 - 4 * Sound2 * Alfven2Normal));
+  const bool ok = !(Fast2 + Discr < Real(0.0));
+  Fast = fsqrt(Real(0.5) * (Fast2 + Discr));
   return ok;
 }
 
@@ -239,54 +274,54 @@ __device__ __forceinline__ bool fast_speed(const Phys &ph, const double *S,
 // (Evaluating the Un < 0 branch as the reflected Un >= 0 branch removes the
 // branch divergence but keeps about 30 more values live; on gfx950 the
 // resulting register spills cost more than the divergence.)
-template <int DIR, bool FULL>
-__device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
-                                    const double *SR, double *F, double &Cmax,
-                                    double &UnOut) {
+template <int DIR, bool FULL, typename Real>
+__device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
+                                    const Real *SR, Real *F, Real &Cmax,
+                                    Real &UnOut) {
   constexpr int in = DIR, it1 = (DIR + 1) % 3, it2 = (DIR + 2) % 3;
   int err = 0;
-  double UnLs, FastL, UnRs, FastR;
+  Real UnLs, FastL, UnRs, FastR;
   if (!fast_speed<DIR>(ph, SL, UnLs, FastL)) err |= ERRB_NEGFAST2;
   if (!fast_speed<DIR>(ph, SR, UnRs, FastR)) err |= ERRB_NEGFAST2;
-  const double sL = fmin2(UnLs - FastL, UnRs - FastR);
-  const double sR = fmax2(UnLs + FastL, UnRs + FastR);
+  const Real sL = fmin2(UnLs - FastL, UnRs - FastR);
+  const Real sR = fmax2(UnLs + FastL, UnRs + FastR);
   Cmax = fmax2(sR, -sL);
-  if (sL >= 0. || sR <= 0.) {  // supersonic: upwind physical flux
-    const bool left = sL >= 0.;
-    double S[NVAR], Cons[NFLUX];
+  if (sL >= Real(0.) || sR <= Real(0.)) {  // supersonic: upwind physical flux
+    const bool left = sL >= Real(0.);
+    Real S[NVAR], Cons[NFLUX];
 #pragma unroll
     for (int v = 0; v < NVAR; v++) S[v] = left ? SL[v] : SR[v];
     phys_flux<DIR, FULL>(ph, S, Cons, F, UnOut);
     return err;
   }
-  const double cTiny = ph.cTiny;
-  const double RhoL = SL[RHO], pL = SL[PP], RhoR = SR[RHO], pR = SR[PP];
-  const double UnL = SL[MX + in], Ut1L = SL[MX + it1], Ut2L = SL[MX + it2];
-  const double B1nL = SL[BX + in], B1t1L = SL[BX + it1], B1t2L = SL[BX + it2];
-  const double UnR = SR[MX + in], Ut1R = SR[MX + it1], Ut2R = SR[MX + it2];
-  const double B1nR = SR[BX + in], B1t1R = SR[BX + it1], B1t2R = SR[BX + it2];
+  const Real cTiny = ph.cTiny;
+  const Real RhoL = SL[RHO], pL = SL[PP], RhoR = SR[RHO], pR = SR[PP];
+  const Real UnL = SL[MX + in], Ut1L = SL[MX + it1], Ut2L = SL[MX + it2];
+  const Real B1nL = SL[BX + in], B1t1L = SL[BX + it1], B1t2L = SL[BX + it2];
+  const Real UnR = SR[MX + in], Ut1R = SR[MX + it1], Ut2R = SR[MX + it2];
+  const Real B1nR = SR[BX + in], B1t1R = SR[BX + it1], B1t2R = SR[BX + it2];
 
-  const double B1n = 0.5 * (B1nL + B1nR);
-  const double B1n2 = B1n * B1n;
-  const double Bn = B1n;
-  const double Bn2 = Bn * Bn;
-  const double SignBn = fsign(1., Bn);
-  const double PbL = 0.5 * (B1n2 + B1t1L * B1t1L + B1t2L * B1t2L);
-  const double PbR = 0.5 * (B1n2 + B1t1R * B1t1R + B1t2R * B1t2R);
-  const double PtotL = pL + PbL;
-  const double PtotR = pR + PbR;
-  const double DsL = sL - UnL, DsR = sR - UnR;
-  const double DsRhoL = DsL * RhoL, DsRhoR = DsR * RhoR;
-  const double Tmp0 = frcp(DsRhoR - DsRhoL);
-  const double Un = (DsRhoR * UnR - DsRhoL * UnL - PtotR + PtotL) * Tmp0;
-  const double Ptot12 =
+  const Real B1n = Real(0.5) * (B1nL + B1nR);
+  const Real B1n2 = B1n * B1n;
+  const Real Bn = B1n;
+  const Real Bn2 = Bn * Bn;
+  const Real SignBn = fsign(Real(1.), Bn);
+  const Real PbL = Real(0.5) * (B1n2 + B1t1L * B1t1L + B1t2L * B1t2L);
+  const Real PbR = Real(0.5) * (B1n2 + B1t1R * B1t1R + B1t2R * B1t2R);
+  const Real PtotL = pL + PbL;
+  const Real PtotR = pR + PbR;
+  const Real DsL = sL - UnL, DsR = sR - UnR;
+  const Real DsRhoL = DsL * RhoL, DsRhoR = DsR * RhoR;
+  const Real Tmp0 = frcp(DsRhoR - DsRhoL);
+  const Real Un = (DsRhoR * UnR - DsRhoL * UnL - PtotR + PtotL) * Tmp0;
+  const Real Ptot12 =
       (DsRhoR * PtotL - DsRhoL * PtotR + DsRhoR * DsRhoL * (UnR - UnL)) * Tmp0;
 
-  const double Bt1L = B1t1L, Bt2L = B1t2L, Bt1R = B1t1R, Bt2R = B1t2R;
-  double Rho, Ut1, Ut2, B1t1, B1t2, uDotB1, e;
-  double Ut1L1, Ut2L1, B1t1L1, B1t2L1, Ut1R1, Ut2R1, B1t1R1, B1t2R1;
+  const Real Bt1L = B1t1L, Bt2L = B1t2L, Bt1R = B1t1R, Bt2R = B1t2R;
+  Real Rho, Ut1, Ut2, B1t1, B1t2, uDotB1, e;
+  Real Ut1L1, Ut2L1, B1t1L1, B1t2L1, Ut1R1, Ut2R1, B1t1R1, B1t2R1;
   auto outer_left = [&]() {
-    double T = DsRhoL * (sL - Un) - Bn2;
+    Real T = DsRhoL * (sL - Un) - Bn2;
     if (T < cTiny) {
       Ut1L1 = Ut1L; Ut2L1 = Ut2L; B1t1L1 = B1t1L; B1t2L1 = B1t2L;
     } else {
@@ -298,7 +333,7 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
     }
   };
   auto outer_right = [&]() {
-    double T = DsRhoR * (sR - Un) - Bn2;
+    Real T = DsRhoR * (sR - Un) - Bn2;
     if (T < cTiny) {
       Ut1R1 = Ut1R; Ut2R1 = Ut2R; B1t1R1 = B1t1R; B1t2R1 = B1t2R;
     } else {
@@ -309,28 +344,28 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
       B1t2R1 = Bt2R * (DsRhoR * DsR - Bn2) * T;
     }
   };
-  if (Un >= 0.) {
-    const double InvSLUn = frcp(sL - Un);
+  if (Un >= Real(0.)) {
+    const Real InvSLUn = frcp(sL - Un);
     Rho = DsRhoL * InvSLUn;
     outer_left();
-    const double eL = ph.invgm1 * pL + PbL +
-                      0.5 * RhoL * (UnL * UnL + Ut1L * Ut1L + Ut2L * Ut2L);
-    const double uDotB1L = UnL * B1n + Ut1L * B1t1L + Ut2L * B1t2L;
-    const double uDotB1L1 = Un * B1n + Ut1L1 * B1t1L1 + Ut2L1 * B1t2L1;
+    const Real eL = ph.invgm1 * pL + PbL +
+                      Real(0.5) * RhoL * (UnL * UnL + Ut1L * Ut1L + Ut2L * Ut2L);
+    const Real uDotB1L = UnL * B1n + Ut1L * B1t1L + Ut2L * B1t2L;
+    const Real uDotB1L1 = Un * B1n + Ut1L1 * B1t1L1 + Ut2L1 * B1t2L1;
     e = (DsL * eL - PtotL * UnL + Ptot12 * Un + Bn * (uDotB1L - uDotB1L1)) *
         InvSLUn;
-    const double SqRhoL1 = fsqrt(Rho);
-    const double sL1 = Un - fabs(Bn) * frcp(SqRhoL1);
-    if (sL1 >= 0.) {
+    const Real SqRhoL1 = fsqrt(Rho);
+    const Real sL1 = Un - fabsr(Bn) * frcp(SqRhoL1);
+    if (sL1 >= Real(0.)) {
       Ut1 = Ut1L1; Ut2 = Ut2L1; B1t1 = B1t1L1; B1t2 = B1t2L1;
       uDotB1 = uDotB1L1;
     } else {
-      const double ratio = DsRhoR * frcp(sR - Un);
+      const Real ratio = DsRhoR * frcp(sR - Un);
       if (ratio < 0) err |= ERRB_HLLD;
-      const double SqRhoR1 = fsqrt(ratio);
+      const Real SqRhoR1 = fsqrt(ratio);
       outer_right();
-      const double Ti = frcp(SqRhoL1 + SqRhoR1);
-      const double SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
+      const Real Ti = frcp(SqRhoL1 + SqRhoR1);
+      const Real SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
       Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) * Ti;
       Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) * Ti;
       B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) * Ti;
@@ -339,27 +374,27 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
       e = e - SqRhoL1 * (uDotB1L1 - uDotB1) * SignBn;
     }
   } else {
-    const double InvSRUn = frcp(sR - Un);
+    const Real InvSRUn = frcp(sR - Un);
     Rho = DsRhoR * InvSRUn;
     outer_right();
-    const double eR = ph.invgm1 * pR + PbR +
-                      0.5 * RhoR * (UnR * UnR + Ut1R * Ut1R + Ut2R * Ut2R);
-    const double uDotB1R = UnR * B1n + Ut1R * B1t1R + Ut2R * B1t2R;
-    const double uDotB1R1 = Un * B1n + Ut1R1 * B1t1R1 + Ut2R1 * B1t2R1;
+    const Real eR = ph.invgm1 * pR + PbR +
+                      Real(0.5) * RhoR * (UnR * UnR + Ut1R * Ut1R + Ut2R * Ut2R);
+    const Real uDotB1R = UnR * B1n + Ut1R * B1t1R + Ut2R * B1t2R;
+    const Real uDotB1R1 = Un * B1n + Ut1R1 * B1t1R1 + Ut2R1 * B1t2R1;
     e = (DsR * eR - PtotR * UnR + Ptot12 * Un + Bn * (uDotB1R - uDotB1R1)) *
         InvSRUn;
-    const double SqRhoR1 = fsqrt(Rho);
-    const double sR1 = Un + fabs(Bn) * frcp(SqRhoR1);
-    if (sR1 <= 0.) {
+    const Real SqRhoR1 = fsqrt(Rho);
+    const Real sR1 = Un + fabsr(Bn) * frcp(SqRhoR1);
+    if (sR1 <= Real(0.)) {
       Ut1 = Ut1R1; Ut2 = Ut2R1; B1t1 = B1t1R1; B1t2 = B1t2R1;
       uDotB1 = uDotB1R1;
     } else {
-      const double ratio = DsRhoL * frcp(sL - Un);
+      const Real ratio = DsRhoL * frcp(sL - Un);
       if (ratio < 0) err |= ERRB_HLLD;
-      const double SqRhoL1 = fsqrt(ratio);
+      const Real SqRhoL1 = fsqrt(ratio);
       outer_left();
-      const double Ti = frcp(SqRhoL1 + SqRhoR1);
-      const double SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
+      const Real Ti = frcp(SqRhoL1 + SqRhoR1);
+      const Real SqRhoLR1 = SqRhoL1 * SqRhoR1 * SignBn;
       Ut1 = (SqRhoL1 * Ut1L1 + SqRhoR1 * Ut1R1 + (B1t1R1 - B1t1L1) * SignBn) * Ti;
       Ut2 = (SqRhoL1 * Ut2L1 + SqRhoR1 * Ut2R1 + (B1t2R1 - B1t2L1) * SignBn) * Ti;
       B1t1 = (SqRhoL1 * B1t1R1 + SqRhoR1 * B1t1L1 + (Ut1R1 - Ut1L1) * SqRhoLR1) * Ti;
@@ -369,18 +404,18 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
     }
   }
   // flux of the HLLD state in the original frame (:3111-3131)
-  const double RhoUn = Rho * Un;
-  const double Bt1 = B1t1, Bt2 = B1t2;
+  const Real RhoUn = Rho * Un;
+  const Real Bt1 = B1t1, Bt2 = B1t2;
   F[RHO] = RhoUn;
   F[MX + in] = RhoUn * Un - B1n * Bn + Ptot12;
   F[MX + it1] = RhoUn * Ut1 - B1n * Bt1;
   F[MX + it2] = RhoUn * Ut2 - B1n * Bt2;
-  F[BX + in] = 0.0;
+  F[BX + in] = Real(0.0);
   F[BX + it1] = Un * Bt1 - Ut1 * Bn;
   F[BX + it2] = Un * Bt2 - Ut2 * Bn;
-  F[PP] = FULL ? 0.5 * (sR * UnL * pL - sL * UnR * pR + sR * sL * (pR - pL)) *
+  F[PP] = FULL ? Real(0.5) * (sR * UnL * pL - sL * UnR * pR + sR * sL * (pR - pL)) *
                      frcp(sR - sL)
-               : 0.0;
+               : Real(0.0);
   F[EN] = Un * (e + Ptot12) - Bn * uDotB1;
   UnOut = Un;
   return err;
@@ -390,58 +425,58 @@ __device__ __forceinline__ int hlld(const Phys &ph, const double *SL,
 // numerical flux for a face normal to DIR from the left/right primitive
 // face states (src/ModFaceFlux.f90:2130-2473).  Out: F[0..8] already times
 // Area, Vdt = CmaxDt*Area, Un*Area.
-template <int DIR, int FLUX, bool FULL>
-__device__ __forceinline__ int numerical_flux(const Phys &ph, double Area,
-                                              const double *Lin,
-                                              const double *Rin, double *F,
-                                              double &Vdt, double &UnArea) {
-  double L[NVAR], R[NVAR];
+template <int DIR, int FLUX, bool FULL, typename Real>
+__device__ __forceinline__ int numerical_flux(const PhysT<Real> &ph, Real Area,
+                                              const Real *Lin,
+                                              const Real *Rin, Real *F,
+                                              Real &Vdt, Real &UnArea) {
+  Real L[NVAR], R[NVAR];
 #pragma unroll
   for (int v = 0; v < NVAR; v++) { L[v] = Lin[v]; R[v] = Rin[v]; }
   int err = 0;
-  double Cmax, Un = 0.0;
-  double dB = 0.0, DiffE = 0.0;
+  Real Cmax, Un = Real(0.0);
+  Real dB = Real(0.0), DiffE = Real(0.0);
   constexpr bool Linde = (FLUX == FLUX_LINDE || FLUX == FLUX_HLLD);
   if (Linde) {
     // remove the jump of the normal field (:2302-2320)
-    dB = 0.5 * (R[BX + DIR] - L[BX + DIR]);
+    dB = Real(0.5) * (R[BX + DIR] - L[BX + DIR]);
     L[BX + DIR] = L[BX + DIR] + dB;
     R[BX + DIR] = R[BX + DIR] - dB;
-    DiffE = 0.5 * ((R[BX + DIR] + L[BX + DIR]) * dB);
+    DiffE = Real(0.5) * ((R[BX + DIR] + L[BX + DIR]) * dB);
   }
   if (FLUX == FLUX_HLLD) {
     err |= hlld<DIR, FULL>(ph, L, R, F, Cmax, Un);
   } else {
-    double S[NVAR];
+    Real S[NVAR];
 #pragma unroll
-    for (int v = 0; v < NVAR; v++) S[v] = 0.5 * (L[v] + R[v]);
-    double CL[NFLUX], CR[NFLUX], FL[NFLUX], FR[NFLUX], UnL, UnR;
+    for (int v = 0; v < NVAR; v++) S[v] = Real(0.5) * (L[v] + R[v]);
+    Real CL[NFLUX], CR[NFLUX], FL[NFLUX], FR[NFLUX], UnL, UnR;
     phys_flux<DIR, FULL>(ph, L, CL, FL, UnL);
     phys_flux<DIR, FULL>(ph, R, CR, FR, UnR);
-    double UnHat, FastHat;
+    Real UnHat, FastHat;
     if (!fast_speed<DIR>(ph, S, UnHat, FastHat)) err |= ERRB_NEGFAST2;
-    Cmax = fabs(UnHat) + FastHat;
+    Cmax = fabsr(UnHat) + FastHat;
     if (FLUX == FLUX_RUSANOV) {
 #pragma unroll
       for (int v = 0; v < NFLUX; v++)
-        F[v] = 0.5 * (FL[v] + FR[v] - Cmax * (CR[v] - CL[v]));
-      Un = 0.5 * (UnL + UnR);
+        F[v] = Real(0.5) * (FL[v] + FR[v] - Cmax * (CR[v] - CL[v]));
+      Un = Real(0.5) * (UnL + UnR);
     } else {
-      double UnLs, FastL, UnRs, FastR;
+      Real UnLs, FastL, UnRs, FastR;
       if (!fast_speed<DIR>(ph, L, UnLs, FastL)) err |= ERRB_NEGFAST2;
       if (!fast_speed<DIR>(ph, R, UnRs, FastR)) err |= ERRB_NEGFAST2;
-      const double Cleft = fmin2(fmin2(0.0, UnLs - FastL), UnHat - FastHat);
-      const double Cright = fmax2(fmax2(0.0, UnRs + FastR), UnHat + FastHat);
-      const double WeightLeft = Cright * frcp(Cright - Cleft);
-      const double WeightRight = 1.0 - WeightLeft;
-      const double Diffusion = Cright * WeightRight;
+      const Real Cleft = fmin2(fmin2(Real(0.0), UnLs - FastL), UnHat - FastHat);
+      const Real Cright = fmax2(fmax2(Real(0.0), UnRs + FastR), UnHat + FastHat);
+      const Real WeightLeft = Cright * frcp(Cright - Cleft);
+      const Real WeightRight = Real(1.0) - WeightLeft;
+      const Real Diffusion = Cright * WeightRight;
 #pragma unroll
       for (int v = 0; v < NFLUX; v++)
         F[v] = (WeightRight * FR[v] + WeightLeft * FL[v] -
                 Diffusion * (CR[v] - CL[v]));
       Un = WeightRight * UnR + WeightLeft * UnL;
     }
-    if (!FULL) F[PP] = 0.0;
+    if (!FULL) F[PP] = Real(0.0);
   }
   if (Linde) { // Linde's Lax-Friedrichs flux for Bn (:2403-2418)
     F[BX + DIR] = F[BX + DIR] - Cmax * dB;
@@ -561,33 +596,34 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
 // :1016-1022).  B?lo/B?hi are the normal-field left (L) and right (R) face
 // states at the lower and upper faces of the cell in each direction;
 // InvHalf = 0.5/CellSize; St is the cell's stage-input state.
+template <typename Real>
 __device__ __forceinline__ void powell_source(
-    const double *InvHalf, double LxLo, double RxLo, double LxHi, double RxHi,
-    double LyLo, double RyLo, double LyHi, double RyHi, double LzLo,
-    double RzLo, double LzHi, double RzHi, const double *St, double *Src) {
-  const double dB1 = InvHalf[0] * (RxLo - LxLo);
-  const double dB2 = InvHalf[0] * (RxHi - LxHi);
-  const double dB3 = InvHalf[1] * (RyLo - LyLo);
-  const double dB4 = InvHalf[1] * (RyHi - LyHi);
-  const double dB5 = InvHalf[2] * (RzLo - LzLo);
-  const double dB6 = InvHalf[2] * (RzHi - LzHi);
-  double Internal = 2 * InvHalf[0] * (LxHi - RxLo);
+    const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
+    Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
+    Real LzHi, Real RzHi, const Real *St, Real *Src) {
+  const Real dB1 = InvHalf[0] * (RxLo - LxLo);
+  const Real dB2 = InvHalf[0] * (RxHi - LxHi);
+  const Real dB3 = InvHalf[1] * (RyLo - LyLo);
+  const Real dB4 = InvHalf[1] * (RyHi - LyHi);
+  const Real dB5 = InvHalf[2] * (RzLo - LzLo);
+  const Real dB6 = InvHalf[2] * (RzHi - LzHi);
+  Real Internal = 2 * InvHalf[0] * (LxHi - RxLo);
   Internal = Internal + 2 * InvHalf[1] * (LyHi - RyLo);
   Internal = Internal + 2 * InvHalf[2] * (LzHi - RzLo);
-  double DivB = Internal + dB1 + dB2;
+  Real DivB = Internal + dB1 + dB2;
   DivB = DivB + dB3 + dB4;
   DivB = DivB + dB5 + dB6;
-  const double RhoInv = frcp(St[RHO]);
-  const double ux = RhoInv * St[MX], uy = RhoInv * St[MY],
-               uz = RhoInv * St[MZ];
-  Src[RHO] = 0.0;
+  const Real RhoInv = frcp(St[RHO]);
+  const Real ux = RhoInv * St[MX], uy = RhoInv * St[MY],
+             uz = RhoInv * St[MZ];
+  Src[RHO] = Real(0);
   Src[MX] = -(DivB * St[BX]);
   Src[MY] = -(DivB * St[BY]);
   Src[MZ] = -(DivB * St[BZ]);
   Src[BX] = -(DivB * ux);
   Src[BY] = -(DivB * uy);
   Src[BZ] = -(DivB * uz);
-  Src[PP] = 0.0;
+  Src[PP] = Real(0);
   Src[EN] = -(DivB * (St[BX] * ux + St[BY] * uy + St[BZ] * uz));
 }
 

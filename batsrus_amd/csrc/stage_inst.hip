@@ -1,26 +1,29 @@
-// stage_inst.hip -- instantiates the fused stage kernel for one block size
-// and one face flux.  Built once per (BATS_NB, BATS_FLUX) pair, e.g.
-//   hipcc -DBATS_NB=16 -DBATS_FLUX=hlld -c stage_inst.hip
+// stage_inst.hip -- instantiates the fused stage kernel for one block size,
+// one face flux and one storage type.  Built once per combination, e.g.
+//   hipcc -DBATS_NB=16 -DBATS_FLUX=hlld -DBATS_REAL=f64 -c stage_inst.hip
 // so that the heavy instantiations compile in parallel.
 #include "stage_kernel.hpp"
 
-#if !defined(BATS_NB) || !defined(BATS_FLUX)
-#error "define BATS_NB (8 or 16) and BATS_FLUX (rusanov, linde or hlld)"
+#if !defined(BATS_NB) || !defined(BATS_FLUX) || !defined(BATS_REAL)
+#error "define BATS_NB (8, 16), BATS_FLUX (rusanov, linde, hlld), BATS_REAL (f64, f32)"
 #endif
 
-#define BATS_CAT_(a, b, c) a##b##_##c
-#define BATS_CAT(a, b, c) BATS_CAT_(a, b, c)
 #define BATS_FLUX_rusanov FLUX_RUSANOV
 #define BATS_FLUX_linde FLUX_LINDE
 #define BATS_FLUX_hlld FLUX_HLLD
-#define BATS_FLUX_CODE_(f) BATS_FLUX_##f
-#define BATS_FLUX_CODE(f) BATS_FLUX_CODE_(f)
+#define BATS_REAL_f64 double
+#define BATS_REAL_f32 float
+#define BATS_PASTE_(a, b) a##b
+#define BATS_PASTE(a, b) BATS_PASTE_(a, b)
+#define BATS_NAME_(nb, fl, ty) launch_stage_##nb##_##fl##_##ty
+#define BATS_NAME(nb, fl, ty) BATS_NAME_(nb, fl, ty)
 
 namespace bats {
-hipError_t BATS_CAT(launch_stage_, BATS_NB, BATS_FLUX)(const StageArgs &a,
-                                                       bool powell,
-                                                       hipStream_t stream) {
-  return launch_stage_kernel<BATS_NB, BATS_FLUX_CODE(BATS_FLUX)>(a, powell,
-                                                                 stream);
+hipError_t BATS_NAME(BATS_NB, BATS_FLUX, BATS_REAL)(const StageArgs &a,
+                                                    bool powell,
+                                                    hipStream_t stream) {
+  return launch_stage_kernel<BATS_NB, BATS_PASTE(BATS_FLUX_, BATS_FLUX),
+                             BATS_PASTE(BATS_REAL_, BATS_REAL)>(a, powell,
+                                                                stream);
 }
 } // namespace bats

@@ -38,9 +38,10 @@ enum { S_DTCB = 0, S_DT = 1, S_TSIM = 2, S_NSTEP = 3, S_DTMIN = 4, S_NSCAL = 8 }
 constexpr int kThreads = 256;
 
 struct StageArgs {
-  const double *Uin;   // stage input (stencils and halos)
-  const double *Uold;  // state at the start of the step (StateOld)
-  double *Uout;        // updated state (may alias Uold)
+  // state buffers, double or float (the kernel's storage type T)
+  const void *Uin;     // stage input (stencils and halos)
+  const void *Uold;    // state at the start of the step (StateOld)
+  void *Uout;          // updated state (may alias Uold)
   const int *nei;      // 6 per block
   const double *cellsize;  // 3 per block
   double *scal;        // device scalars, S_NSCAL per run (S_*)
@@ -92,8 +93,9 @@ __device__ __forceinline__ long cell_off(const int *__restrict__ nei, int b,
   return (((long)b * NB + k) * NB + j) * NB + i;
 }
 
-__device__ __forceinline__ void load_state(const double *__restrict__ U,
-                                           long vs, long off, double *q) {
+template <typename T>
+__device__ __forceinline__ void load_state(const T *__restrict__ U, long vs,
+                                           long off, T *q) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++) q[v] = U[v * vs + off];
 }
@@ -104,35 +106,72 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
-template <int DIR, int FLUX, int LIM>
-__device__ __forceinline__ int face_flux(const Phys &ph, double area,
-                                         double beta, const double *q0,
-                                         const double *q1, const double *q2,
-                                         const double *q3, double *F,
-                                         double &vdt, double &unA) {
-  double L[NVAR], R[NVAR];
+template <int DIR, int FLUX, int LIM, typename T>
+__device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T beta,
+                                         const T *q0, const T *q1,
+                                         const T *q2, const T *q3, T *F,
+                                         T &vdt, T &unA) {
+  T L[NVAR], R[NVAR];
   face_states<LIM>(beta, q0, q1, q2, q3, L, R);
   return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
 }
 
 // ---------------------------------------------------------------------
-// fused stage kernel
+// fused stage kernel.  T is the storage and face-arithmetic type (double:
+// plain path; float: FP32-state path); the stage update, the pressure fix
+// and the time step stay in double.
 // ---------------------------------------------------------------------
-template <int NB, int FLUX, int LIM, bool POWELL>
-__global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
+template <int NB, int FLUX, int LIM, bool POWELL, typename T>
+__global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
+    k_stage(StageArgs a) {
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
   constexpr int NS = NB + 5;           // odd row stride: no LDS bank conflicts
   constexpr int NFX = (NB + 1) * NB;   // x (or y) faces per plane
   constexpr int NFS = 9;               // flux slots: 0..6, PP<-vdt, EN
-  __shared__ double sP[BPW * NVAR * NR * NS];
-  __shared__ double sF[BPW * 2 * NFS * NFX];
+  // z-face fluxes of the lower and upper face of the cell; with the 8-wave
+  // scheme two more slots carry the normal-field face states Bz_L, Bz_R
+  constexpr int NFZ = POWELL ? NFS + 2 : NFS;
+  __shared__ T sP[BPW * NVAR * NR * NS];
+  __shared__ T sF[BPW * 2 * NFS * NFX];
   __shared__ double sRed[kThreads / 64];
 
   const int tid = threadIdx.x;
   const int lb = tid / NC, col = tid % NC, j = col / NB, i = col % NB;
+  // this thread's cell of the 2-deep x/y halo ring of each plane (at most
+  // one: the ring of a group's blocks has NH <= 256 cells)
+  constexpr int NH = BPW * 8 * NB;
+  static_assert(NH <= kThreads, "one halo cell per thread");
+  const bool hasH = tid < NH;
+  int hlb = 0, hii = 0, hjj = 0;
+  if (hasH) {
+    hlb = tid / (8 * NB);
+    const int r = tid % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
+    if (side < 2) {
+      const int layer = rem & 1, pos = rem >> 1;
+      hjj = pos;
+      hii = side == 0 ? layer - 2 : NB + layer;
+    } else {
+      const int layer = rem / NB, pos = rem % NB;
+      hii = pos;
+      hjj = side == 2 ? layer - 2 : NB + layer;
+    }
+  }
+  const T *__restrict__ U = static_cast<const T *>(a.Uin);
+  const T *Uold = static_cast<const T *>(a.Uold);
+  T *Uout = static_cast<T *>(a.Uout);
+  const long vs = a.vs;
+  const PhysT<T> ph = phys_as<T>(a.ph);
+  const T beta = T(a.beta);
   int err = 0;
+
+  auto sPi = [&](int l, int v, int jj, int ii) {
+    return ((l * NVAR + v) * NR + jj) * NS + ii;
+  };
+  auto sFi = [&](int l, int dir, int v, int q) {
+    return ((l * 2 + dir) * NFS + v) * NFX + q;
+  };
 
   // a persistent grid walks the flat work list of block groups; with one
   // workgroup per group the loop runs once
@@ -141,34 +180,25 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
     const int b = blk0 + lb;
     const bool act = b < a.nBlock;
     const int bb = act ? b : 0;
-    const double *__restrict__ U = a.Uin;
-    const long vs = a.vs;
-    const Phys ph = a.ph;
-    const double beta = a.beta;
-
-    auto sPi = [&](int l, int v, int jj, int ii) {
-      return ((l * NVAR + v) * NR + jj) * NS + ii;
-    };
-    auto sFi = [&](int l, int dir, int v, int q) {
-      return ((l * 2 + dir) * NFS + v) * NFX + q;
-    };
+    const int hb = blk0 + hlb;
+    const bool hAct = hasH && hb < a.nBlock;
 
     const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
                  dzb = a.cellsize[3 * bb + 2];
-    const double AreaZ = dxb * dyb;
+    const T AreaZ = T(dxb * dyb);
     const double CellVolume = dxb * dyb * dzb;
-    double InvHalf[3];
+    T InvHalf[3];
     if (POWELL) {
-      InvHalf[0] = 0.5 / dxb;
-      InvHalf[1] = 0.5 / dyb;
-      InvHalf[2] = 0.5 / dzb;
+      InvHalf[0] = T(0.5 / dxb);
+      InvHalf[1] = T(0.5 / dyb);
+      InvHalf[2] = T(0.5 / dzb);
     }
 
     // this block's run: its own time step (per-run CFL dt)
-    const double *rs = a.scal + (a.runOfBlock ? a.runOfBlock[bb] : 0) * S_NSCAL;
-    const double DtUnscaled = rs[S_DTCB];
+    const double *rs =
+        a.scal + (a.runOfBlock ? a.runOfBlock[bb] : 0) * S_NSCAL;
     UpdateCtl uc;
-    uc.DtLocal = a.DtFactor * DtUnscaled;
+    uc.DtLocal = a.DtFactor * rs[S_DTCB];
     uc.InvVolume = 1.0 / CellVolume;
     uc.halfStep = a.halfStep;
     uc.iStage = a.iStage;
@@ -178,33 +208,30 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
     uc.roundTripOld = a.roundTripOld;
 
     // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
-    // left-state slope dLk of cell k; each cell's slopes are computed once and
-    // serve the faces below and above it
-    double Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
-    // z-face fluxes of the lower and upper face of the cell; with the 8-wave
-    // scheme two more slots carry the normal-field face states Bz_L, Bz_R
-    constexpr int NFZ = POWELL ? NFS + 2 : NFS;
-    double FzLo[NFZ], FzHi[NFZ];
+    // left-state slope dLk of cell k; each cell's slopes are computed once
+    // and serve the faces below and above it
+    T Q1[NVAR], Q2[NVAR], Q3[NVAR], dLk[NVAR];
+    T FzLo[NFZ], FzHi[NFZ];
     double dtmin = 1.7976931348623157e308;
 
-    auto load_prim = [&](int kk, double *q) {
+    auto load_prim = [&](int kk, T *q) {
       const long off = cell_off<NB>(a.nei, bb, i, j, kk);
       load_state(U, vs, off, q);
       to_primitive(q);
     };
     // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
-    auto zface = [&](double *Fz) {
-      double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
-  #pragma unroll
+    auto zface = [&](T *Fz) {
+      T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+#pragma unroll
       for (int v = 0; v < NVAR; v++) {
-        double dl, dr;
+        T dl, dr;
         slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
         L[v] = Q1[v] + dLk[v];
         R[v] = Q2[v] - dr;
         dLk[v] = dl;
       }
       err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
-  #pragma unroll
+#pragma unroll
       for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
       Fz[PP] = vdt;
       if (POWELL) { Fz[NFZ - 2] = L[BZ]; Fz[NFZ - 1] = R[BZ]; }
@@ -212,19 +239,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
 
     if (act) {
       // prologue: slopes of cell -1 and the face below cell 0
-      double Pm2[NVAR];
+      T Pm2[NVAR];
       load_prim(-2, Pm2);
       load_prim(-1, Q1);
       load_prim(0, Q2);
       load_prim(1, Q3);
-  #pragma unroll
+#pragma unroll
       for (int v = 0; v < NVAR; v++) {
-        double dl, dr;
+        T dl, dr;
         slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
         dLk[v] = dl;
       }
       zface(FzLo);
-  #pragma unroll
+#pragma unroll
       for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
     }
 
@@ -234,30 +261,17 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
         load_prim(k + 2, Q3);
         zface(FzHi);
         // own primitive of plane k
-  #pragma unroll
+#pragma unroll
         for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
       }
-      // x/y halo of plane k
-      for (int h = tid; h < BPW * 8 * NB; h += kThreads) {
-        const int lbh = h / (8 * NB);
-        const int bh = blk0 + lbh;
-        if (bh >= a.nBlock) continue;
-        const int r = h % (8 * NB), side = r / (2 * NB), rem = r % (2 * NB);
-        int ii, jj;
-        if (side < 2) {
-          const int layer = rem & 1, pos = rem >> 1;
-          jj = pos;
-          ii = side == 0 ? layer - 2 : NB + layer;
-        } else {
-          const int layer = rem / NB, pos = rem % NB;
-          ii = pos;
-          jj = side == 2 ? layer - 2 : NB + layer;
-        }
-        double q[NVAR];
-        load_state(U, vs, cell_off<NB>(a.nei, bh, ii, jj, k), q);
+      // this thread's x/y halo cell of plane k
+      if (hAct) {
+        T q[NVAR];
+        load_state(U, vs, cell_off<NB>(a.nei, hb, hii, hjj, k), q);
         to_primitive(q);
-  #pragma unroll
-        for (int v = 0; v < NVAR; v++) sP[sPi(lbh, v, jj + 2, ii + 2)] = q[v];
+#pragma unroll
+        for (int v = 0; v < NVAR; v++)
+          sP[sPi(hlb, v, hjj + 2, hii + 2)] = q[v];
       }
       __syncthreads();
       // x and y faces of plane k
@@ -267,97 +281,114 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
         if (bf >= a.nBlock) continue;
         const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
         const int row = q / (NB + 1), fp = q % (NB + 1);
-        double c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
-        double F[NFLUX], vdt, unA;
+        T c0[NVAR], c1[NVAR], c2[NVAR], c3[NVAR];
+        T F[NFLUX], vdt, unA;
         const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
                      dzf = a.cellsize[3 * bf + 2];
         if (dir == 0) {
-  #pragma unroll
+#pragma unroll
           for (int v = 0; v < NVAR; v++) {
             c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
             c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
             c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
             c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
           }
-          err |= face_flux<0, FLUX, LIM>(ph, dyf * dzf, beta, c0, c1, c2, c3, F,
-                                         vdt, unA);
+          err |= face_flux<0, FLUX, LIM>(ph, T(dyf * dzf), beta, c0, c1, c2,
+                                         c3, F, vdt, unA);
         } else {
-  #pragma unroll
+#pragma unroll
           for (int v = 0; v < NVAR; v++) {
             c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
             c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
             c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
             c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
           }
-          err |= face_flux<1, FLUX, LIM>(ph, dxf * dzf, beta, c0, c1, c2, c3, F,
-                                         vdt, unA);
+          err |= face_flux<1, FLUX, LIM>(ph, T(dxf * dzf), beta, c0, c1, c2,
+                                         c3, F, vdt, unA);
         }
         F[PP] = vdt;
-  #pragma unroll
+#pragma unroll
         for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
       }
       __syncthreads();
       if (act) {
         const int qx0 = j * (NB + 1) + i, qy0 = i * (NB + 1) + j;
+        // flux divergence in T, handed to the update in double
         double dF[NFLUX];
-  #pragma unroll
+#pragma unroll
         for (int v = 0; v < NFLUX; v++) {
           if (v == PP) { dF[v] = 0.0; continue; }
-          double d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
+          T d = sF[sFi(lb, 0, v, qx0)] - sF[sFi(lb, 0, v, qx0 + 1)];
           d = d + sF[sFi(lb, 1, v, qy0)] - sF[sFi(lb, 1, v, qy0 + 1)];
           d = d + FzLo[v] - FzHi[v];
-          dF[v] = d;
+          dF[v] = double(d);
         }
         const long off = (((long)b * NB + k) * NB + j) * NB + i;
         if (!a.noUpdate) {
-          double Uc[NVAR], Uo[NVAR], Un[NVAR];
-          load_state(a.Uold, vs, off, Uo);
+          T Uc[NVAR], Uo[NVAR];
+          double Ucd[NVAR], Uod[NVAR], Un[NVAR];
+          load_state(Uold, vs, off, Uo);
           if (POWELL) {
-            // calc_source: the div B source from the normal-field face states
-            // of the six faces (x and y recomputed from the LDS plane exactly
-            // as the face loop computed them) and the stage-input cell state
+            // calc_source: the div B source from the normal-field face
+            // states of the six faces (x and y recomputed from the LDS plane
+            // exactly as the face loop computed them) and the stage-input
+            // cell state
             load_state(U, vs, off, Uc);
-            double p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
-  #pragma unroll
+            T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
+#pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
             face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
             face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
-  #pragma unroll
+#pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
             face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
             face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
-            double Src[NFLUX];
+            T Src[NFLUX];
             powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
                           RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
                           FzHi[NFZ - 1], Uc, Src);
-            update_cell(ph, uc, dF, Uc, Uo, Un, Src);
+            double Srcd[NFLUX];
+#pragma unroll
+            for (int v = 0; v < NVAR; v++) {
+              Ucd[v] = double(Uc[v]);
+              Uod[v] = double(Uo[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < NFLUX; v++) Srcd[v] = double(Src[v]);
+            update_cell(a.ph, uc, dF, Ucd, Uod, Un, Srcd);
           } else {
             // the stage input at the cell is used only by the SSP-RK2
             // interpolation and for a zero time step; otherwise re-read the
-            // just-loaded StateOld lines (cache hits) instead of the U* stream
-            load_state((uc.dtZero || !uc.halfStep) ? U : a.Uold, vs, off, Uc);
-            update_cell(ph, uc, dF, Uc, Uo, Un);
+            // just-loaded StateOld lines (cache hits) instead of the U*
+            // stream
+            load_state((uc.dtZero || !uc.halfStep) ? U : Uold, vs, off, Uc);
+#pragma unroll
+            for (int v = 0; v < NVAR; v++) {
+              Ucd[v] = double(Uc[v]);
+              Uod[v] = double(Uo[v]);
+            }
+            update_cell(a.ph, uc, dF, Ucd, Uod, Un);
           }
-  #pragma unroll
-          for (int v = 0; v < NVAR; v++) a.Uout[v * vs + off] = Un[v];
+#pragma unroll
+          for (int v = 0; v < NVAR; v++) Uout[v * vs + off] = T(Un[v]);
         }
         if (a.doDt) {
-          const double Vdt =
+          const T Vdt =
               fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
               fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
               fmax2(FzLo[PP], FzHi[PP]);
-          const double dtc = CellVolume / Vdt;
+          const double dtc = CellVolume / double(Vdt);
           if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
           dtmin = fmin2(dtmin, dtc);
         }
         // shift the z window
-  #pragma unroll
+#pragma unroll
         for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
-  #pragma unroll
+#pragma unroll
         for (int v = 0; v < NFZ; v++) FzLo[v] = FzHi[v];
       }
-      // the div B source reads neighbouring cells of the LDS plane, which the
-      // next plane overwrites
+      // the div B source reads neighbouring cells of the LDS plane, which
+      // the next plane overwrites
       if (POWELL) __syncthreads();
     }
 
@@ -379,24 +410,24 @@ __global__ void __launch_bounds__(kThreads, 2) k_stage(StageArgs a) {
   if (err) atomicOr(a.err, err);
   if (a.advanceTime && blockIdx.x == 0)
     for (int r = tid; r < a.nRun; r += kThreads) {
-      double *rs = a.scal + r * S_NSCAL;
-      rs[S_TSIM] = rs[S_TSIM] + rs[S_DT] / a.nStage;
+      double *rsr = a.scal + r * S_NSCAL;
+      rsr[S_TSIM] = rsr[S_TSIM] + rsr[S_DT] / a.nStage;
     }
 }
 
 // Launch k_stage for one block size and flux; the limiter and the 8-wave
 // switch are dispatched here.
-template <int NB, int FLUX>
+template <int NB, int FLUX, typename T>
 inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
                                       hipStream_t stream) {
   const dim3 grid(a.nGroup < a.nWorkgroup ? a.nGroup : a.nWorkgroup),
       block(kThreads);
 #define BATS_LAUNCH(LM)                                                     \
   if (powell)                                                               \
-    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, true>), grid, block, 0,       \
+    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, true, T>), grid, block, 0,    \
                        stream, a);                                          \
   else                                                                      \
-    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, false>), grid, block, 0,      \
+    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, false, T>), grid, block, 0,   \
                        stream, a)
   switch (a.lim) {
   case LIM_MINMOD: BATS_LAUNCH(LIM_MINMOD); break;
@@ -410,12 +441,20 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
   return hipGetLastError();
 }
 
-// one launcher per (block size, flux) translation unit (stage_inst.hip)
-hipError_t launch_stage_8_rusanov(const StageArgs &, bool, hipStream_t);
-hipError_t launch_stage_8_linde(const StageArgs &, bool, hipStream_t);
-hipError_t launch_stage_8_hlld(const StageArgs &, bool, hipStream_t);
-hipError_t launch_stage_16_rusanov(const StageArgs &, bool, hipStream_t);
-hipError_t launch_stage_16_linde(const StageArgs &, bool, hipStream_t);
-hipError_t launch_stage_16_hlld(const StageArgs &, bool, hipStream_t);
+// one launcher per (block size, flux, storage type) translation unit
+// (stage_inst.hip)
+#define BATS_DECLARE_LAUNCH(NB, FL, TY)                                     \
+  hipError_t launch_stage_##NB##_##FL##_##TY(const StageArgs &, bool,       \
+                                             hipStream_t);
+#define BATS_DECLARE_LAUNCH_FLUX(NB, TY)                                    \
+  BATS_DECLARE_LAUNCH(NB, rusanov, TY)                                      \
+  BATS_DECLARE_LAUNCH(NB, linde, TY)                                        \
+  BATS_DECLARE_LAUNCH(NB, hlld, TY)
+BATS_DECLARE_LAUNCH_FLUX(8, f64)
+BATS_DECLARE_LAUNCH_FLUX(16, f64)
+BATS_DECLARE_LAUNCH_FLUX(8, f32)
+BATS_DECLARE_LAUNCH_FLUX(16, f32)
+#undef BATS_DECLARE_LAUNCH_FLUX
+#undef BATS_DECLARE_LAUNCH
 
 } // namespace bats

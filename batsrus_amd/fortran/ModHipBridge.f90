@@ -64,6 +64,7 @@ module ModHipBridge
      real(c_double):: DtFixed, cTiny
      integer(c_int):: iDevice
      integer(c_int):: UseDivbSource
+     integer(c_int):: StatePrecision
   end type bats_config
 
   interface

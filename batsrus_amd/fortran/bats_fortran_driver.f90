@@ -61,6 +61,7 @@ program bats_fortran_driver
   Config%UseDbTrick = 0; Config%UseDtFixed = 0; Config%DtFixed = 0.0
   Config%cTiny = 1e-6; Config%iDevice = 0
   Config%UseDivbSource = merge(1, 0, UseDivbSource)
+  Config%StatePrecision = 8
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   call sync_state_to_gpu(State_VGB)

@@ -75,6 +75,12 @@ typedef struct bats_config {
   int iDevice;          /* HIP device ordinal */
   int UseDivbSource;    /* 8-wave (Powell) div B source (#DIVB,
                            src/ModMain.f90:204; 0 = #DIVB F) */
+  int StatePrecision;   /* bytes per state value on the device: 8 (or 0)
+                           FP64; 4 = FP32 state and FP32 face arithmetic
+                           with the stage update, the pressure fix and the
+                           time step in FP64 (no counterpart in the
+                           reference, which runs FP64 throughout); the
+                           per-block routines need FP64 */
 } bats_config;
 
 typedef struct bats_handle bats_handle;
