@@ -41,8 +41,9 @@ def build_hip(jobs=None):
     odir = os.path.join(csrc, "_build")
     os.makedirs(odir, exist_ok=True)
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + csrc]
-    units = [(os.path.join(csrc, "bats_hip.hip"),
-              os.path.join(odir, "bats_hip.o"), [])]
+    units = [(os.path.join(csrc, name + ".hip"),
+              os.path.join(odir, name + ".o"), [])
+             for name in ("bats_hip", "mlp", "surrogate")]
     for nb, flux, real in STAGE_UNITS:
         units.append((os.path.join(csrc, "stage_inst.hip"),
                       os.path.join(odir, "stage_%d_%s_%s.o" % (nb, flux, real)),

@@ -59,6 +59,7 @@ class BatsConfig(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 _hp = C.c_void_p
 
@@ -91,6 +92,21 @@ ABI_FUNCTIONS = {
     "bats_set_runs": (C.c_int, [_hp, C.c_int, _ip]),
     "bats_get_run_times": (C.c_int, [_hp, _dp, _dp, _ip]),
     "bats_set_persistent_grid": (C.c_int, [_hp, C.c_int]),
+    "bats_mlp_create": (C.c_int, [C.POINTER(_hp)]),
+    "bats_mlp_free": (C.c_int, [_hp]),
+    "bats_mlp_set_params": (C.c_int, [_hp, _fp]),
+    "bats_mlp_get_params": (C.c_int, [_hp, _fp, _fp]),
+    "bats_mlp_forward": (C.c_int, [_hp, _fp, _fp, C.c_int]),
+    "bats_mlp_loss_grad": (C.c_int, [_hp, _fp, _fp, C.c_int, _dp, _fp]),
+    "bats_mlp_set_data": (C.c_int, [_hp, _fp, _fp, C.c_long]),
+    "bats_mlp_get_data": (C.c_int, [_hp, C.POINTER(C.c_long), _fp, _fp]),
+    "bats_mlp_harvest": (C.c_int, [_hp, _hp, C.c_double, C.c_double,
+                                   C.c_ulonglong, C.POINTER(C.c_long)]),
+    "bats_mlp_train": (C.c_int, [_hp, C.c_int, C.c_int, C.c_double,
+                                 C.c_double, C.c_ulonglong, _dp, _dp]),
+    "bats_set_surrogate": (C.c_int, [_hp, _hp, C.c_int, C.c_double]),
+    "bats_surrogate_counts": (C.c_int, [_hp, C.POINTER(C.c_long),
+                                        C.POINTER(C.c_long)]),
     "bats_check_nan": (C.c_int, [_hp]),
     "bats_integrate_domain": (C.c_int, [_hp, _dp]),
     "bats_last_stage_kernel_ms": (C.c_int, [_hp, _dp, _ip]),
@@ -353,6 +369,23 @@ class Session:
         self._check(self.lib.bats_set_persistent_grid(self.h, nWorkgroup),
                     "set_persistent_grid")
 
+    def set_surrogate(self, model, mode="network", BetaMin=0.0):
+        """Gated face fluxes on an FP32 handle: mode 'network' (the trained
+        model on smooth faces), 'rusanov' (the same gate, Rusanov flux) or
+        'off'."""
+        code = {"off": 0, "network": 1, "rusanov": 2}[mode]
+        self._check(self.lib.bats_set_surrogate(
+            self.h, model.h if model is not None else None, code,
+            float(BetaMin)), "set_surrogate")
+        self._surrogate = model
+
+    def surrogate_counts(self):
+        nf, nn = C.c_long(), C.c_long()
+        self._check(self.lib.bats_surrogate_counts(self.h, C.byref(nf),
+                                                   C.byref(nn)),
+                    "surrogate_counts")
+        return nf.value, nn.value
+
     def integrate_domain(self):
         s = np.zeros(8)
         self._check(self.lib.bats_integrate_domain(self.h, _ptr(s)),
@@ -365,3 +398,103 @@ class Session:
                                                        C.byref(n)),
                     "last_stage_kernel_ms")
         return ms.value, n.value
+
+
+class Mlp:
+    """The face-flux surrogate network on the GPU (bats_mlp_* entry points):
+    FP32 master weights, BF16 matrix-core forward/backward, Adam."""
+
+    NPARAM = 27464
+
+    def __init__(self, params=None):
+        self.lib = load_library()
+        h = _hp()
+        st = self.lib.bats_mlp_create(C.byref(h))
+        if st != 0:
+            raise StopMpi("bats_mlp_create: " + _strerror(st))
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def _check(self, st, what):
+        if st != 0:
+            raise StopMpi("%s: %s" % (what, _strerror(st)))
+
+    def close(self):
+        if self.h:
+            self.lib.bats_mlp_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.NPARAM
+        self._check(self.lib.bats_mlp_set_params(self.h, _ptr(w, C.c_float)),
+                    "mlp_set_params")
+
+    def params(self):
+        w = np.zeros(self.NPARAM, np.float32)
+        g = np.zeros(self.NPARAM, np.float32)
+        self._check(self.lib.bats_mlp_get_params(self.h, _ptr(w, C.c_float),
+                                                 _ptr(g, C.c_float)),
+                    "mlp_get_params")
+        return w, g
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 8), np.float32)
+        self._check(self.lib.bats_mlp_forward(self.h, _ptr(X, C.c_float),
+                                              _ptr(Y, C.c_float), X.shape[0]),
+                    "mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.bats_mlp_loss_grad(
+            self.h, _ptr(X, C.c_float), _ptr(T, C.c_float), X.shape[0],
+            C.byref(loss), _ptr(g, C.c_float)), "mlp_loss_grad")
+        return loss.value, g
+
+    def set_data(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        self._check(self.lib.bats_mlp_set_data(self.h, _ptr(X, C.c_float),
+                                               _ptr(T, C.c_float), X.shape[0]),
+                    "mlp_set_data")
+
+    def data(self):
+        n = C.c_long()
+        self._check(self.lib.bats_mlp_get_data(self.h, C.byref(n), None, None),
+                    "mlp_get_data")
+        X = np.zeros((n.value, 16), np.float32)
+        T = np.zeros((n.value, 8), np.float32)
+        self._check(self.lib.bats_mlp_get_data(self.h, C.byref(n),
+                                               _ptr(X, C.c_float),
+                                               _ptr(T, C.c_float)),
+                    "mlp_get_data")
+        return X, T
+
+    def harvest(self, session, fraction, BetaMin=0.0, seed=1):
+        n = C.c_long()
+        self._check(self.lib.bats_mlp_harvest(self.h, session.h,
+                                              float(fraction), float(BetaMin),
+                                              int(seed), C.byref(n)),
+                    "mlp_harvest")
+        return n.value
+
+    def train(self, nEpoch, batch=65536, lr=1e-3, hold=0.1, seed=1):
+        loss = C.c_double()
+        err = np.zeros(9)
+        self._check(self.lib.bats_mlp_train(self.h, int(nEpoch), int(batch),
+                                            float(lr), float(hold), int(seed),
+                                            C.byref(loss), _ptr(err)),
+                    "mlp_train")
+        return loss.value, err

@@ -13,6 +13,7 @@
 
 #include "../../include/bats_hip.h"
 #include "stage_kernel.hpp"
+#include "handle.hpp"
 
 using namespace bats;
 
@@ -293,75 +294,6 @@ k_integrate(const T *U, long vs, const double *cellsize, Phys ph,
 // =====================================================================
 // host side
 // =====================================================================
-struct bats_handle {
-  bats_config cfg;
-  int NB;
-  long nCellBlk, nCell;
-  hipStream_t stream;
-  void *U[2];    // state buffers of elemSize-byte reals
-  int elemSize;  // 8: FP64 state, 4: FP32 state
-  int cur;       // buffer holding the state at the start of a step
-  int stateBuf;  // buffer holding the current State_VGB (differs from cur
-                 // between the stages of a 2-stage step)
-  double *Usnap;
-  int *nei;
-  double *cellsize;
-  double *dtmaxB;
-  double *dtmaxCB;
-  double *scal;      // S_NSCAL scalars per run
-  int nRun;
-  std::vector<int> hRunFirst, hRunOfBlock;
-  int *runFirst;     // nRun+1 block offsets
-  int *runOfBlock;   // nBlock
-  int nWorkgroupMax; // > 0: persistent grid of at most this many workgroups
-  int *err;
-  double *partial;
-  double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;
-  double *Src;   // Source_VC of the last calc_source block
-  int srcBlock;  // that block (0-based), -1: none since the last update
-  int iStageBlock;
-  std::vector<int> hNei;
-  // kernel timing
-  std::vector<hipEvent_t> evStart, evStop;
-  int nTimed;
-  double lastStageMs;
-  int lastStageLaunches;
-};
-
-static int hip_status(hipError_t e) {
-  return e == hipSuccess ? BATS_OK : BATS_EHIP;
-}
-#define HIPCHK(x)                                                        \
-  do {                                                                   \
-    hipError_t e_ = (x);                                                 \
-    if (e_ != hipSuccess) {                                              \
-      fprintf(stderr, "bats_hip: %s failed: %s (%s:%d)\n", #x,           \
-              hipGetErrorString(e_), __FILE__, __LINE__);                \
-      return e_ == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;        \
-    }                                                                    \
-  } while (0)
-
-static Phys make_phys(const bats_config &c) {
-  Phys p;
-  p.gamma = c.Gamma;
-  p.gm1 = c.Gamma - 1.0;
-  p.invgm1 = 1.0 / (c.Gamma - 1.0);
-  p.cTiny = c.cTiny;
-  p.pMin = c.pMin;
-  return p;
-}
-
-static int limiter_code(const bats_config &c) {
-  return c.nOrder == 1 ? LIM_FIRST : c.iLimiter;
-}
-
-static int err_from_bits(int bits) {
-  if (bits & ERRB_NEGFAST2) return BATS_ENEGFAST2;
-  if (bits & ERRB_HLLD) return BATS_EHLLD;
-  if (bits & ERRB_NAN) return BATS_ENAN;
-  return BATS_OK;
-}
-
 extern "C" const char *bats_strerror(int code) {
   switch (code) {
   case BATS_OK: return "ok";
@@ -462,7 +394,9 @@ extern "C" int bats_finalize(bats_handle *h) {
   void *ptrs[] = {h->U[0], h->U[1], h->Usnap, h->nei, h->cellsize, h->dtmaxB,
                   h->dtmaxCB, h->scal, h->runFirst, h->runOfBlock, h->err,
                   h->partial, h->LX, h->RX,
-                  h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src};
+                  h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
+                  h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
+                  h->sur.counters};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &ev : h->evStart) (void)hipEventDestroy(ev);
@@ -864,7 +798,12 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   }
   const bool powell = c.UseDivbSource != 0;
   const bool f32 = h->elemSize == 4;
-  hipError_t e;
+  hipError_t e = hipSuccess;
+  if (h->sur.mode != BATS_SUR_OFF) {
+    // gated face fluxes with the network on smooth faces
+    const int st = sur_stage(h, a);
+    if (st) return st;
+  } else {
 #define BATS_PICK(NBv, FL)                                                   \
   (f32 ? launch_stage_##NBv##_##FL##_f32(a, powell, h->stream)               \
        : launch_stage_##NBv##_##FL##_f64(a, powell, h->stream))
@@ -877,6 +816,7 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
       : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(16, linde)
                                      : BATS_PICK(16, hlld);
 #undef BATS_PICK
+  }
   HIPCHK(e);
   if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));
   if (!noUpdate) {

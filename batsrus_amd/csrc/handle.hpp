@@ -1,0 +1,101 @@
+// handle.hpp -- the device-side state behind a bats_handle, shared by the
+// C ABI (bats_hip.hip) and the surrogate stage (surrogate.hip).  Internal.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/bats_hip.h"
+#include "stage_kernel.hpp"
+
+using namespace bats;
+
+// buffers of the surrogate stage (allocated on first use)
+struct SurrogateState {
+  bats_mlp *model;     // network (not owned); null: plain stages
+  int mode;            // BATS_SUR_*
+  double betaMin;      // gate: minimum plasma beta for the network
+  float *face;         // face fluxes, 9 slots per face, all blocks
+  float *rowX;         // network inputs of the gated faces (rows x 16)
+  float *rowY;         // network outputs (rows x 8)
+  int *rowFace;        // per row: global face index
+  float *rowScale;     // per row: density scale, speed scale
+  int *counters;       // [0] rows this stage, [1] rows since set_surrogate,
+                       // [2] faces since set_surrogate
+  long faceCap;
+};
+
+struct bats_handle {
+  bats_config cfg;
+  int NB;
+  long nCellBlk, nCell;
+  hipStream_t stream;
+  void *U[2];    // state buffers of elemSize-byte reals
+  int elemSize;  // 8: FP64 state, 4: FP32 state
+  int cur;       // buffer holding the state at the start of a step
+  int stateBuf;  // buffer holding the current State_VGB (differs from cur
+                 // between the stages of a 2-stage step)
+  double *Usnap;
+  int *nei;
+  double *cellsize;
+  double *dtmaxB;
+  double *dtmaxCB;
+  double *scal;      // S_NSCAL scalars per run
+  int nRun;
+  std::vector<int> hRunFirst, hRunOfBlock;
+  int *runFirst;     // nRun+1 block offsets
+  int *runOfBlock;   // nBlock
+  int nWorkgroupMax; // > 0: persistent grid of at most this many workgroups
+  int *err;
+  double *partial;
+  double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;
+  double *Src;   // Source_VC of the last calc_source block
+  int srcBlock;  // that block (0-based), -1: none since the last update
+  int iStageBlock;
+  std::vector<int> hNei;
+  // kernel timing
+  std::vector<hipEvent_t> evStart, evStop;
+  int nTimed;
+  double lastStageMs;
+  int lastStageLaunches;
+  // learned face-flux surrogate (surrogate.hip), FP32 state only
+  SurrogateState sur;
+};
+
+inline int hip_status(hipError_t e) {
+  return e == hipSuccess ? BATS_OK : BATS_EHIP;
+}
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      fprintf(stderr, "bats_hip: %s failed: %s (%s:%d)\n", #x,           \
+              hipGetErrorString(e_), __FILE__, __LINE__);                \
+      return e_ == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;        \
+    }                                                                    \
+  } while (0)
+
+inline Phys make_phys(const bats_config &c) {
+  Phys p;
+  p.gamma = c.Gamma;
+  p.gm1 = c.Gamma - 1.0;
+  p.invgm1 = 1.0 / (c.Gamma - 1.0);
+  p.cTiny = c.cTiny;
+  p.pMin = c.pMin;
+  return p;
+}
+
+inline int limiter_code(const bats_config &c) {
+  return c.nOrder == 1 ? LIM_FIRST : c.iLimiter;
+}
+
+inline int err_from_bits(int bits) {
+  if (bits & ERRB_NEGFAST2) return BATS_ENEGFAST2;
+  if (bits & ERRB_HLLD) return BATS_EHLLD;
+  if (bits & ERRB_NAN) return BATS_ENAN;
+  return BATS_OK;
+}
+
+// one stage of the update through the surrogate kernels (surrogate.hip)
+int sur_stage(bats_handle *h, const StageArgs &a);

@@ -1,0 +1,663 @@
+// mlp.hip -- training and inference of the face-flux surrogate network on
+// gfx950 matrix cores (see mlp.hpp for the network and operand layouts).
+//
+// Training step on a minibatch of B rows (all kernels hand-written):
+//   forward   Z_l = A_{l-1} W_l + b_l, A_l = bf16(silu(Z_l))   (k_gemm FWD)
+//   loss      dY = bf16(2 (Y - T) / (8 B)), per-row squared error (k_loss)
+//   backward  dZ_{l-1} = bf16((dZ_l W_l^T) * silu'(Z_{l-1}))     (k_gemm BWD)
+//             partial dW_l = A_{l-1}^T dZ_l over row chunks        (k_gemm WGRAD)
+//             partial db_l = column sums of dZ_l over row chunks   (k_colsum)
+//   update    sum the chunk partials in a fixed order, Adam on the FP32
+//             master weights, refresh the BF16 copies           (k_adam)
+// Every reduction has a fixed order, so a training run is deterministic.
+// Inference (k_infer) keeps all 58 KiB of fragment-ordered BF16 weights in
+// LDS and pushes 16-row tiles through the four layers per wavefront.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/bats_hip.h"
+#include "mlp.hpp"
+#include "mlp_host.hpp"
+
+using namespace bats::mlp;
+
+namespace {
+
+enum { EPI_FWD = 0, EPI_OUT = 1, EPI_BWD = 2, EPI_WGRAD = 3 };
+
+struct GemmArgs {
+  int M, N, K;
+  const __bf16 *A;  // A(m,k) = A[m*sAm + k*sAk]
+  long sAm, sAk;
+  const __bf16 *B;  // B(k,n) = B[k*sBk + n*sBn]
+  long sBk, sBn;
+  const float *bias;     // FWD/OUT: [N]
+  float *Cf;             // FWD: Z, OUT: Y, WGRAD: partial (ld = ldc)
+  long ldc;
+  __bf16 *Cb;            // FWD: A = silu(Z), BWD: dZ (ld = ldc)
+  const float *Zprev;    // BWD: pre-activation of the layer below (ld = ldc)
+  int kChunk;            // WGRAD: rows per split (blockIdx.y)
+  long partialStride;    // WGRAD: floats between splits
+};
+
+// one 16x16 output tile per wavefront, K in steps of 32
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nTn = (g.N + 15) / 16;
+  const int m0 = (tile / nTn) * 16, n0 = (tile % nTn) * 16;
+  if (m0 >= g.M) return;  // wave-uniform
+  int kBeg = 0, kEnd = g.K;
+  if (EPI == EPI_WGRAD) {
+    kBeg = blockIdx.y * g.kChunk;
+    kEnd = min(g.K, kBeg + g.kChunk);
+  }
+  const int r = lane & 15, kq = 8 * (lane >> 4);
+  const int am = m0 + r, bn = n0 + r;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kBeg; k0 < kEnd; k0 += 32) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int k = k0 + kq + t;
+      const bool ok = k < kEnd;
+      a[t] = (ok && am < g.M) ? g.A[am * g.sAm + k * g.sAk] : (__bf16)0.0f;
+      b[t] = (ok && bn < g.N) ? g.B[k * g.sBk + bn * g.sBn] : (__bf16)0.0f;
+    }
+    acc = mfma(a, b, acc);
+  }
+  const int n = n0 + r;
+  if (n >= g.N) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int m = m0 + 4 * (lane >> 4) + i;
+    if (m >= g.M) continue;
+    const float v = acc[i];
+    if (EPI == EPI_FWD) {
+      const float z = v + g.bias[n];
+      g.Cf[m * g.ldc + n] = z;
+      g.Cb[m * g.ldc + n] = (__bf16)silu(z);
+    } else if (EPI == EPI_OUT) {
+      g.Cf[m * g.ldc + n] = v + g.bias[n];
+    } else if (EPI == EPI_BWD) {
+      g.Cb[m * g.ldc + n] = (__bf16)(v * dsilu(g.Zprev[m * g.ldc + n]));
+    } else {
+      g.Cf[blockIdx.y * g.partialStride + m * g.ldc + n] = v;
+    }
+  }
+}
+
+// dY = bf16(2 (Y - T) / (8 B)); per-row squared error sums
+__global__ void k_loss(const float *Y, const float *T, int B, __bf16 *dY,
+                       float *rowErr) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= B) return;
+  float s = 0.f;
+  const float sc = 2.0f / (8.0f * (float)B);
+  for (int j = 0; j < 8; j++) {
+    const float d = Y[row * 8 + j] - T[row * 8 + j];
+    s += d * d;
+    dY[row * 8 + j] = (__bf16)(sc * d);
+  }
+  rowErr[row] = s;
+}
+
+// partial column sums of a BF16 [rows x N] matrix over row chunks
+__global__ void k_colsum(const __bf16 *X, int rows, int N, int chunk,
+                         float *partial, long partialStride, int off) {
+  const int n = threadIdx.x;  // N <= 256
+  if (n >= N) return;
+  const int r0 = blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
+  float s = 0.f;
+  for (int r = r0; r < r1; r++) s += (float)X[(long)r * N + n];
+  partial[blockIdx.x * partialStride + off + n] = s;
+}
+
+// gradient = sum of the row-chunk partials, in chunk order
+__global__ void k_sum_partials(const float *partial, int nSplit, float *grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  float g = 0.f;
+  for (int s = 0; s < nSplit; s++) g += partial[(long)s * NPARAM + p];
+  grad[p] = g;
+}
+
+struct AdamArgs {
+  const float *grad;
+  float *w, *m, *v;
+  __bf16 *wb;            // BF16 copy, flat layout
+  float lr, b1, b2, eps, bc1, bc2;  // bc = 1 - beta^t
+};
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights
+__global__ void k_adam(AdamArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  const float g = a.grad[p];
+  const float m = a.b1 * a.m[p] + (1.f - a.b1) * g;
+  const float v = a.b2 * a.v[p] + (1.f - a.b2) * g * g;
+  a.m[p] = m;
+  a.v[p] = v;
+  const float w = a.w[p] - a.lr * (m / a.bc1) / (sqrtf(v / a.bc2) + a.eps);
+  a.w[p] = w;
+  a.wb[p] = (__bf16)w;
+}
+
+__global__ void k_to_bf16(const float *x, __bf16 *y, long n) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (__bf16)x[t];
+}
+
+// fragment-ordered, zero-padded BF16 weights for k_infer
+__global__ void k_pack_frag(const float *w, __bf16 *frag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NFRAG) return;
+  const int tile = t / TILE, lane = (t % TILE) / 8, e = t % 8;
+  int l = 0;
+  while (l + 1 < NL && tile >= FT[l + 1]) l++;
+  const int lt = tile - FT[l], nNb = NP[l] / 16;
+  const int kb = lt / nNb, nb = lt % nNb;
+  const int k = kb * 32 + 8 * (lane >> 4) + e, n = nb * 16 + (lane & 15);
+  const float v = (k < DIM[l] && n < DIM[l + 1])
+                      ? w[OW[l] + k * DIM[l + 1] + n] : 0.0f;
+  frag[t] = (__bf16)v;
+}
+
+}  // namespace
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
+        int rows) {
+  __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
+  __shared__ __attribute__((aligned(16))) __bf16 sAct[4][16 * ACT_LD];
+  for (int t = threadIdx.x; t < NFRAG / 8; t += 256)
+    reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const int nTile = (rows + 15) / 16;
+  auto out = [&](int row, int col, float v) { Y[(long)row * 8 + col] = v; };
+  for (int tile = blockIdx.x * 4 + wave; tile < nTile; tile += gridDim.x * 4)
+    infer_tile(sW, w, sAct[wave], X, rows, tile * 16, out);
+}
+
+// minibatch gather: rows idx[0..B) of the dataset into X (BF16) and T
+__global__ void k_gather_rows(const float *dataX, const float *dataT,
+                              const int *idx, int B, __bf16 *X, float *T) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B) return;
+  const long s = idx[t];
+  for (int j = 0; j < 16; j++) X[t * 16 + j] = (__bf16)dataX[s * 16 + j];
+  for (int j = 0; j < 8; j++) T[t * 8 + j] = dataT[s * 8 + j];
+}
+__global__ void k_gather_rows_f32(const float *dataX, const float *dataT,
+                                  const int *idx, int B, float *X, float *T) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B) return;
+  const long s = idx[t];
+  for (int j = 0; j < 16; j++) X[(long)t * 16 + j] = dataX[s * 16 + j];
+  for (int j = 0; j < 8; j++) T[(long)t * 8 + j] = dataT[s * 8 + j];
+}
+
+}  // namespace
+
+// =====================================================================
+// host side: the model object
+// =====================================================================
+struct bats_mlp {
+  hipStream_t stream;
+  float *w, *m, *v, *grad;   // FP32 master weights, Adam moments, gradient
+  __bf16 *wb;                // BF16 copy of w (flat layout)
+  __bf16 *frag;              // fragment-ordered BF16 weights for inference
+  int step;                  // Adam steps taken
+  int fragStale;             // frag lags w (repacked before inference)
+  // training buffers for a minibatch of cap rows
+  int cap;
+  __bf16 *X, *A1, *A2, *A3, *dZ1, *dZ2, *dZ3, *dY;
+  float *Z1, *Z2, *Z3, *Y, *T, *rowErr, *partial;
+  int nSplit;
+  // dataset (device): inputs (n x 16) and targets (n x 8), FP32
+  float *dataX, *dataT;
+  long nData, capData;
+};
+
+#define MCHK(x)                                                          \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      fprintf(stderr, "bats_mlp: %s failed: %s (%s:%d)\n", #x,           \
+              hipGetErrorString(e_), __FILE__, __LINE__);                \
+      return e_ == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;        \
+    }                                                                    \
+  } while (0)
+
+static const int kChunkRows = 256;  // rows per weight-gradient split
+
+static int repack(bats_mlp *m) {
+  if (!m->fragStale) return BATS_OK;
+  hipLaunchKernelGGL(k_pack_frag, dim3((NFRAG + 255) / 256), dim3(256), 0,
+                     m->stream, m->w, m->frag);
+  MCHK(hipGetLastError());
+  m->fragStale = 0;
+  return BATS_OK;
+}
+
+static int refresh_copies(bats_mlp *m) {
+  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0,
+                     m->stream, m->w, m->wb, (long)NPARAM);
+  hipLaunchKernelGGL(k_pack_frag, dim3((NFRAG + 255) / 256), dim3(256), 0,
+                     m->stream, m->w, m->frag);
+  MCHK(hipGetLastError());
+  return BATS_OK;
+}
+
+extern "C" int bats_mlp_create(bats_mlp **out) {
+  if (!out) return BATS_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return BATS_ENODEV;
+  bats_mlp *m = new bats_mlp();
+  memset(m, 0, sizeof *m);
+  MCHK(hipStreamCreate(&m->stream));
+  MCHK(hipMalloc(&m->w, sizeof(float) * NPARAM));
+  MCHK(hipMalloc(&m->m, sizeof(float) * NPARAM));
+  MCHK(hipMalloc(&m->v, sizeof(float) * NPARAM));
+  MCHK(hipMalloc(&m->grad, sizeof(float) * NPARAM));
+  MCHK(hipMalloc(&m->wb, sizeof(__bf16) * NPARAM));
+  MCHK(hipMalloc(&m->frag, sizeof(__bf16) * NFRAG));
+  MCHK(hipMemset(m->w, 0, sizeof(float) * NPARAM));
+  MCHK(hipMemset(m->m, 0, sizeof(float) * NPARAM));
+  MCHK(hipMemset(m->v, 0, sizeof(float) * NPARAM));
+  MCHK(hipMemset(m->grad, 0, sizeof(float) * NPARAM));
+  *out = m;
+  return refresh_copies(m);
+}
+
+static void free_batch(bats_mlp *m) {
+  void *p[] = {m->X, m->A1, m->A2, m->A3, m->dZ1, m->dZ2, m->dZ3, m->dY,
+               m->Z1, m->Z2, m->Z3, m->Y, m->T, m->rowErr, m->partial};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  m->X = m->A1 = m->A2 = m->A3 = m->dZ1 = m->dZ2 = m->dZ3 = m->dY = nullptr;
+  m->Z1 = m->Z2 = m->Z3 = m->Y = m->T = m->rowErr = m->partial = nullptr;
+  m->cap = 0;
+}
+
+extern "C" int bats_mlp_free(bats_mlp *m) {
+  if (!m) return BATS_EINVAL;
+  (void)hipStreamSynchronize(m->stream);
+  free_batch(m);
+  void *p[] = {m->w, m->m, m->v, m->grad, m->wb, m->frag, m->dataX, m->dataT};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  (void)hipStreamDestroy(m->stream);
+  delete m;
+  return BATS_OK;
+}
+
+extern "C" int bats_mlp_set_params(bats_mlp *m, const float *w) {
+  if (!m || !w) return BATS_EINVAL;
+  MCHK(hipMemcpyAsync(m->w, w, sizeof(float) * NPARAM, hipMemcpyHostToDevice,
+                      m->stream));
+  MCHK(hipMemsetAsync(m->m, 0, sizeof(float) * NPARAM, m->stream));
+  MCHK(hipMemsetAsync(m->v, 0, sizeof(float) * NPARAM, m->stream));
+  m->step = 0;
+  int st = refresh_copies(m);
+  if (st) return st;
+  MCHK(hipStreamSynchronize(m->stream));
+  return BATS_OK;
+}
+
+extern "C" int bats_mlp_get_params(bats_mlp *m, float *w, float *grad) {
+  if (!m) return BATS_EINVAL;
+  MCHK(hipStreamSynchronize(m->stream));
+  if (w) MCHK(hipMemcpy(w, m->w, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  if (grad)
+    MCHK(hipMemcpy(grad, m->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+static int ensure_batch(bats_mlp *m, int B) {
+  if (B <= m->cap) return BATS_OK;
+  free_batch(m);
+  const size_t b = (size_t)B;
+  MCHK(hipMalloc(&m->X, sizeof(__bf16) * b * 16));
+  MCHK(hipMalloc(&m->A1, sizeof(__bf16) * b * 128));
+  MCHK(hipMalloc(&m->A2, sizeof(__bf16) * b * 128));
+  MCHK(hipMalloc(&m->A3, sizeof(__bf16) * b * 64));
+  MCHK(hipMalloc(&m->dZ1, sizeof(__bf16) * b * 128));
+  MCHK(hipMalloc(&m->dZ2, sizeof(__bf16) * b * 128));
+  MCHK(hipMalloc(&m->dZ3, sizeof(__bf16) * b * 64));
+  MCHK(hipMalloc(&m->dY, sizeof(__bf16) * b * 8));
+  MCHK(hipMalloc(&m->Z1, sizeof(float) * b * 128));
+  MCHK(hipMalloc(&m->Z2, sizeof(float) * b * 128));
+  MCHK(hipMalloc(&m->Z3, sizeof(float) * b * 64));
+  MCHK(hipMalloc(&m->Y, sizeof(float) * b * 8));
+  MCHK(hipMalloc(&m->T, sizeof(float) * b * 8));
+  MCHK(hipMalloc(&m->rowErr, sizeof(float) * b));
+  m->nSplit = (B + kChunkRows - 1) / kChunkRows;
+  MCHK(hipMalloc(&m->partial, sizeof(float) * (size_t)m->nSplit * NPARAM));
+  m->cap = B;
+  return BATS_OK;
+}
+
+static GemmArgs gemm0() {
+  GemmArgs g;
+  memset(&g, 0, sizeof g);
+  return g;
+}
+
+static void launch_gemm(int epi, const GemmArgs &g, hipStream_t s, int splits) {
+  const int tiles = ((g.M + 15) / 16) * ((g.N + 15) / 16);
+  const dim3 grid((tiles + 3) / 4, splits), blk(256);
+  switch (epi) {
+  case EPI_FWD: hipLaunchKernelGGL(k_gemm<EPI_FWD>, grid, blk, 0, s, g); break;
+  case EPI_OUT: hipLaunchKernelGGL(k_gemm<EPI_OUT>, grid, blk, 0, s, g); break;
+  case EPI_BWD: hipLaunchKernelGGL(k_gemm<EPI_BWD>, grid, blk, 0, s, g); break;
+  default: hipLaunchKernelGGL(k_gemm<EPI_WGRAD>, grid, blk, 0, s, g); break;
+  }
+}
+
+// forward of the minibatch in m->X (B rows) into Z1..Z3, A1..A3, Y
+static void forward_batch(bats_mlp *m, int B) {
+  const __bf16 *in[4] = {m->X, m->A1, m->A2, m->A3};
+  float *Z[4] = {m->Z1, m->Z2, m->Z3, m->Y};
+  __bf16 *Aout[3] = {m->A1, m->A2, m->A3};
+  for (int l = 0; l < NL; l++) {
+    GemmArgs g = gemm0();
+    g.M = B; g.N = DIM[l + 1]; g.K = DIM[l];
+    g.A = in[l]; g.sAm = DIM[l]; g.sAk = 1;
+    g.B = m->wb + OW[l]; g.sBk = DIM[l + 1]; g.sBn = 1;
+    g.bias = m->w + OB[l];
+    g.Cf = Z[l]; g.ldc = DIM[l + 1];
+    if (l < NL - 1) {
+      g.Cb = Aout[l];
+      launch_gemm(EPI_FWD, g, m->stream, 1);
+    } else {
+      launch_gemm(EPI_OUT, g, m->stream, 1);
+    }
+  }
+}
+
+// gradient of the mean-square loss of the minibatch (X, T already on the
+// device) into m->grad and, if lr > 0, one Adam step; rowErr filled
+static int grad_step(bats_mlp *m, int B, float lr, bool update) {
+  forward_batch(m, B);
+  hipLaunchKernelGGL(k_loss, dim3((B + 255) / 256), dim3(256), 0, m->stream,
+                     m->Y, m->T, B, m->dY, m->rowErr);
+  const __bf16 *Ain[4] = {m->X, m->A1, m->A2, m->A3};
+  __bf16 *dZ[4] = {m->dZ1, m->dZ2, m->dZ3, m->dY};
+  const float *Zs[3] = {m->Z1, m->Z2, m->Z3};
+  const int S = (B + kChunkRows - 1) / kChunkRows;
+  for (int l = NL - 1; l >= 0; l--) {
+    // weight gradient partials: dW_l = A_{l-1}^T dZ_l over row chunks
+    GemmArgs g = gemm0();
+    g.M = DIM[l]; g.N = DIM[l + 1]; g.K = B;
+    g.A = Ain[l]; g.sAm = 1; g.sAk = DIM[l];
+    g.B = dZ[l]; g.sBk = DIM[l + 1]; g.sBn = 1;
+    g.Cf = m->partial + OW[l]; g.ldc = DIM[l + 1];
+    g.kChunk = kChunkRows; g.partialStride = NPARAM;
+    launch_gemm(EPI_WGRAD, g, m->stream, S);
+    hipLaunchKernelGGL(k_colsum, dim3(S), dim3(256), 0, m->stream, dZ[l], B,
+                       DIM[l + 1], kChunkRows, m->partial, (long)NPARAM, OB[l]);
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1})
+      GemmArgs d = gemm0();
+      d.M = B; d.N = DIM[l]; d.K = DIM[l + 1];
+      d.A = dZ[l]; d.sAm = DIM[l + 1]; d.sAk = 1;
+      d.B = m->wb + OW[l]; d.sBk = 1; d.sBn = DIM[l + 1];
+      d.Cb = dZ[l - 1]; d.ldc = DIM[l]; d.Zprev = Zs[l - 1];
+      launch_gemm(EPI_BWD, d, m->stream, 1);
+    }
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3((NPARAM + 255) / 256), dim3(256), 0,
+                     m->stream, m->partial, S, m->grad);
+  if (update) {
+    m->step++;
+    AdamArgs a;
+    a.grad = m->grad; a.w = m->w; a.m = m->m; a.v = m->v; a.wb = m->wb;
+    a.lr = lr; a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-8f;
+    a.bc1 = 1.f - powf(a.b1, (float)m->step);
+    a.bc2 = 1.f - powf(a.b2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0,
+                       m->stream, a);
+    m->fragStale = 1;
+  }
+  MCHK(hipGetLastError());
+  return BATS_OK;
+}
+
+
+extern "C" int bats_mlp_loss_grad(bats_mlp *m, const float *X, const float *T,
+                                  int B, double *Loss, float *grad) {
+  if (!m || !X || !T || B <= 0) return BATS_EINVAL;
+  int st = ensure_batch(m, B);
+  if (st) return st;
+  std::vector<__bf16> xb((size_t)B * 16);
+  for (size_t t = 0; t < xb.size(); t++) xb[t] = (__bf16)X[t];
+  MCHK(hipMemcpyAsync(m->X, xb.data(), sizeof(__bf16) * xb.size(),
+                      hipMemcpyHostToDevice, m->stream));
+  MCHK(hipMemcpyAsync(m->T, T, sizeof(float) * (size_t)B * 8,
+                      hipMemcpyHostToDevice, m->stream));
+  MCHK(hipStreamSynchronize(m->stream));
+  st = grad_step(m, B, 0.f, false);
+  if (st) return st;
+  std::vector<float> e(B);
+  MCHK(hipMemcpyAsync(e.data(), m->rowErr, sizeof(float) * B,
+                      hipMemcpyDeviceToHost, m->stream));
+  MCHK(hipStreamSynchronize(m->stream));
+  double s = 0.0;
+  for (int r = 0; r < B; r++) s += e[r];
+  if (Loss) *Loss = s / (8.0 * B);
+  if (grad)
+    MCHK(hipMemcpy(grad, m->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+extern "C" int bats_mlp_forward(bats_mlp *m, const float *X, float *Y, int rows) {
+  if (!m || !X || !Y || rows <= 0) return BATS_EINVAL;
+  float *dX = nullptr, *dY = nullptr;
+  MCHK(hipMalloc(&dX, sizeof(float) * (size_t)rows * 16));
+  MCHK(hipMalloc(&dY, sizeof(float) * (size_t)rows * 8));
+  MCHK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)rows * 16,
+                      hipMemcpyHostToDevice, m->stream));
+  int st = bats::mlp::forward_device(m, dX, dY, rows, m->stream);
+  if (st == BATS_OK)
+    MCHK(hipMemcpyAsync(Y, dY, sizeof(float) * (size_t)rows * 8,
+                        hipMemcpyDeviceToHost, m->stream));
+  MCHK(hipStreamSynchronize(m->stream));
+  (void)hipFree(dX);
+  (void)hipFree(dY);
+  return st;
+}
+
+int bats::mlp::forward_device(bats_mlp *m, const float *dX, float *dY,
+                              int rows, hipStream_t stream) {
+  if (!m || rows <= 0) return BATS_EINVAL;
+  // the weights must be current on this stream
+  int st = repack(m);
+  if (st) return st;
+  MCHK(hipStreamSynchronize(m->stream));
+  const int nTile = (rows + 15) / 16;
+  int grid = (nTile + 3) / 4;
+  if (grid > 2048) grid = 2048;
+  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(256), 0, stream, m->frag, m->w,
+                     dX, dY, rows);
+  MCHK(hipGetLastError());
+  return BATS_OK;
+}
+
+// ---- dataset and training loop ----------------------------------------
+static int ensure_data(bats_mlp *m, long n) {
+  if (n <= m->capData) return BATS_OK;
+  float *x = nullptr, *t = nullptr;
+  MCHK(hipMalloc(&x, sizeof(float) * (size_t)n * 16));
+  MCHK(hipMalloc(&t, sizeof(float) * (size_t)n * 8));
+  if (m->nData > 0) {
+    MCHK(hipMemcpyAsync(x, m->dataX, sizeof(float) * (size_t)m->nData * 16,
+                        hipMemcpyDeviceToDevice, m->stream));
+    MCHK(hipMemcpyAsync(t, m->dataT, sizeof(float) * (size_t)m->nData * 8,
+                        hipMemcpyDeviceToDevice, m->stream));
+  }
+  MCHK(hipStreamSynchronize(m->stream));
+  if (m->dataX) (void)hipFree(m->dataX);
+  if (m->dataT) (void)hipFree(m->dataT);
+  m->dataX = x;
+  m->dataT = t;
+  m->capData = n;
+  return BATS_OK;
+}
+
+int bats::mlp::reserve_data(bats_mlp *m, long nMore, float **X, float **T) {
+  int st = ensure_data(m, m->nData + nMore);
+  if (st) return st;
+  *X = m->dataX + (size_t)m->nData * 16;
+  *T = m->dataT + (size_t)m->nData * 8;
+  return BATS_OK;
+}
+
+void bats::mlp::commit_data(bats_mlp *m, long nAdded) { m->nData += nAdded; }
+
+bats::mlp::Weights bats::mlp::weights(bats_mlp *m) {
+  (void)repack(m);
+  (void)hipStreamSynchronize(m->stream);  // weights current
+  return Weights{m->frag, m->w};
+}
+
+extern "C" int bats_mlp_set_data(bats_mlp *m, const float *X, const float *T,
+                                 long n) {
+  if (!m || n < 0 || (n > 0 && (!X || !T))) return BATS_EINVAL;
+  m->nData = 0;
+  int st = ensure_data(m, n);
+  if (st) return st;
+  if (n > 0) {
+    MCHK(hipMemcpy(m->dataX, X, sizeof(float) * (size_t)n * 16,
+                   hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(m->dataT, T, sizeof(float) * (size_t)n * 8,
+                   hipMemcpyHostToDevice));
+  }
+  m->nData = n;
+  return BATS_OK;
+}
+
+extern "C" int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T) {
+  if (!m) return BATS_EINVAL;
+  MCHK(hipStreamSynchronize(m->stream));
+  if (n) *n = m->nData;
+  if (X && m->nData)
+    MCHK(hipMemcpy(X, m->dataX, sizeof(float) * (size_t)m->nData * 16,
+                   hipMemcpyDeviceToHost));
+  if (T && m->nData)
+    MCHK(hipMemcpy(T, m->dataT, sizeof(float) * (size_t)m->nData * 8,
+                   hipMemcpyDeviceToHost));
+  return BATS_OK;
+}
+
+// splitmix64: the seeded shuffles of the training loop
+static uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static void shuffle(std::vector<int> &v, uint64_t &s) {
+  for (size_t i = v.size(); i > 1; i--) {
+    const size_t j = splitmix64(s) % i;
+    std::swap(v[i - 1], v[j]);
+  }
+}
+
+extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
+                              double HoldFrac, unsigned long long Seed,
+                              double *TrainLoss, double *HeldOutRelErr_V) {
+  if (!m || nEpoch < 0 || Batch <= 0 || !(Lr0 >= 0.0) || HoldFrac < 0.0 ||
+      HoldFrac >= 1.0 || m->nData < 2)
+    return BATS_EINVAL;
+  const long n = m->nData;
+  if (n > 0x7fffffffL) return BATS_EINVAL;
+  const long nHold = (long)(HoldFrac * (double)n);
+  const long nTrain = n - nHold;
+  if (nTrain < 1) return BATS_EINVAL;
+  const int B = (int)(Batch < nTrain ? Batch : nTrain);
+  int st = ensure_batch(m, B);
+  if (st) return st;
+  // seeded split: the first nTrain of a permutation train, the rest are held
+  uint64_t s = Seed;
+  std::vector<int> perm(n);
+  for (long i = 0; i < n; i++) perm[i] = (int)i;
+  shuffle(perm, s);
+  std::vector<int> train(perm.begin(), perm.begin() + nTrain);
+  std::vector<int> hold(perm.begin() + nTrain, perm.end());
+  int *dIdx = nullptr;
+  MCHK(hipMalloc(&dIdx, sizeof(int) * (size_t)(nTrain > nHold ? nTrain : nHold)));
+  const long stepsPerEpoch = nTrain / B;
+  const long total = stepsPerEpoch * nEpoch;
+  long step = 0;
+  double lastLoss = 0.0;
+  std::vector<float> e(B);
+  for (int ep = 0; ep < nEpoch; ep++) {
+    shuffle(train, s);
+    MCHK(hipMemcpyAsync(dIdx, train.data(), sizeof(int) * train.size(),
+                        hipMemcpyHostToDevice, m->stream));
+    double epLoss = 0.0;
+    for (long b = 0; b < stepsPerEpoch; b++, step++) {
+      const double lr = Lr0 * 0.5 * (1.0 + cos(M_PI * (double)step / (double)total));
+      hipLaunchKernelGGL(k_gather_rows, dim3((B + 255) / 256), dim3(256), 0,
+                         m->stream, m->dataX, m->dataT, dIdx + b * B, B, m->X,
+                         m->T);
+      st = grad_step(m, B, (float)lr, true);
+      if (st) { (void)hipFree(dIdx); return st; }
+      if (b == stepsPerEpoch - 1) {  // loss of the epoch's last minibatch
+        MCHK(hipMemcpyAsync(e.data(), m->rowErr, sizeof(float) * B,
+                            hipMemcpyDeviceToHost, m->stream));
+        MCHK(hipStreamSynchronize(m->stream));
+        for (int r = 0; r < B; r++) epLoss += e[r];
+        lastLoss = epLoss / (8.0 * B);
+      }
+    }
+  }
+  if (TrainLoss) *TrainLoss = lastLoss;
+  // held-out relative error: ||Y - T|| / ||T|| overall and per output
+  if (HeldOutRelErr_V && nHold > 0) {
+    float *hx = nullptr, *ht = nullptr, *hy = nullptr;
+    MCHK(hipMalloc(&hx, sizeof(float) * (size_t)nHold * 16));
+    MCHK(hipMalloc(&ht, sizeof(float) * (size_t)nHold * 8));
+    MCHK(hipMalloc(&hy, sizeof(float) * (size_t)nHold * 8));
+    MCHK(hipMemcpyAsync(dIdx, hold.data(), sizeof(int) * hold.size(),
+                        hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_gather_rows_f32, dim3((nHold + 255) / 256), dim3(256),
+                       0, m->stream, m->dataX, m->dataT, dIdx, (int)nHold, hx,
+                       ht);
+    MCHK(hipStreamSynchronize(m->stream));
+    st = bats::mlp::forward_device(m, hx, hy, (int)nHold, m->stream);
+    std::vector<float> Y((size_t)nHold * 8), T((size_t)nHold * 8);
+    MCHK(hipMemcpyAsync(Y.data(), hy, sizeof(float) * Y.size(),
+                        hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipMemcpyAsync(T.data(), ht, sizeof(float) * T.size(),
+                        hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    double num[9] = {0}, den[9] = {0};
+    for (long r = 0; r < nHold; r++)
+      for (int j = 0; j < 8; j++) {
+        const double d = (double)Y[r * 8 + j] - (double)T[r * 8 + j];
+        const double t = T[r * 8 + j];
+        num[j + 1] += d * d;
+        den[j + 1] += t * t;
+        num[0] += d * d;
+        den[0] += t * t;
+      }
+    for (int j = 0; j < 9; j++)
+      HeldOutRelErr_V[j] = den[j] > 0 ? sqrt(num[j] / den[j]) : sqrt(num[j]);
+    (void)hipFree(hx);
+    (void)hipFree(ht);
+    (void)hipFree(hy);
+  }
+  (void)hipFree(dIdx);
+  return st;
+}

@@ -1,0 +1,664 @@
+// surrogate.hip -- the learned face-flux surrogate in the stage update, and
+// the harvesting of its training samples from the plain solver.
+//
+// Face frame and scaling (shared by harvest and inference).  For a face
+// normal to axis d the left and right primitive face states are rotated to
+// (rho, u_n, u_t1, u_t2, B_n, B_t1, B_t2, p) with (n, t1, t2) = (d, d+1,
+// d+2 mod 3), and scaled by the face's density rho_s = (rho_L + rho_R)/2
+// and speed c_s = max(c_fL, c_fR) (normal fast magnetosonic speeds):
+//   inputs  rho/rho_s, u/c_s, B/(c_s sqrt(rho_s)), p/(rho_s c_s^2)  (L, R)
+//   outputs F_rho/(rho_s c_s), F_m/(rho_s c_s^2), F_B/(c_s^2 sqrt(rho_s)),
+//           F_E/(rho_s c_s^3)
+// where F is the numerical flux per unit area of the plain path (HLLD with
+// the Linde correction of the normal field, src/ModFaceFlux.f90:2826-3160,
+// 2302-2320, 2403-2418) in the rotated frame.
+//
+// Smoothness gate: a face goes to the network when, for every variable and
+// for both cells next to the face, the MC limiter would leave the slope
+// unclipped (the two one-sided differences agree in sign and their mean is
+// within beta times the smaller one, or both vanish), and the plasma beta
+// 2p/B^2 of both face states is at least BetaMin.  Every other face takes
+// the exact flux.
+//
+// Surrogate stage (FP32 state):
+//   k_sur_faces   face states, gate, Vdt; exact flux of the gated-out faces
+//                 into the face array; compacted network input rows
+//   k_sur_infer   the network on the rows (matrix cores), fluxes scattered
+//                 back to the face array in the original frame, times area
+//   k_sur_update  flux divergence, stage update in FP64, local time step
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "handle.hpp"
+#include "mlp.hpp"
+#include "mlp_host.hpp"
+
+namespace {
+
+constexpr int NSLOT = 9;  // face slots: flux 0..6, 7 <- Vdt, 8 energy
+
+template <int NB> struct FaceGeom {
+  static constexpr int NF = (NB + 1) * NB * NB;  // faces per direction
+  // local face r of direction dir -> the 4-cell stencil start (i,j,k) of
+  // cell "f-2" and the unit step
+  __device__ static void decode(int dir, int r, int &fi, int &fj, int &fk) {
+    if (dir == 0) {
+      fi = r % (NB + 1); fj = (r / (NB + 1)) % NB; fk = r / ((NB + 1) * NB);
+    } else if (dir == 1) {
+      fi = r % NB; fj = (r / NB) % (NB + 1); fk = r / (NB * (NB + 1));
+    } else {
+      fi = r % NB; fj = (r / NB) % NB; fk = r / (NB * NB);
+    }
+  }
+};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename R>
+__device__ __forceinline__ bool var_smooth(R p0, R p1, R p2, R beta) {
+  const R d1 = p1 - p0, d2 = p2 - p1;
+  if (d1 == R(0) && d2 == R(0)) return true;
+  return d1 * d2 > R(0) &&
+         R(0.5) * fabsr(d1 + d2) <= beta * fmin2(fabsr(d1), fabsr(d2));
+}
+
+template <typename R>
+__device__ __forceinline__ bool face_smooth(const R *q0, const R *q1,
+                                            const R *q2, const R *q3,
+                                            const R *L, const R *Rs, R beta,
+                                            R betaMin) {
+  bool ok = true;
+#pragma unroll
+  for (int v = 0; v < NVAR; v++)
+    ok = ok && var_smooth(q0[v], q1[v], q2[v], beta) &&
+         var_smooth(q1[v], q2[v], q3[v], beta);
+  const R b2L = L[BX] * L[BX] + L[BY] * L[BY] + L[BZ] * L[BZ];
+  const R b2R = Rs[BX] * Rs[BX] + Rs[BY] * Rs[BY] + Rs[BZ] * Rs[BZ];
+  ok = ok && R(2) * L[PP] >= betaMin * b2L && R(2) * Rs[PP] >= betaMin * b2R;
+  return ok;
+}
+
+// rotated component index c of the 8-vector (rho, m_n, m_t1, m_t2, B_n,
+// B_t1, B_t2, E) -> flux slot in the original frame
+__device__ __forceinline__ int rot_slot(int dir, int c) {
+  const int it1 = (dir + 1) % 3, it2 = (dir + 2) % 3;
+  switch (c) {
+  case 0: return RHO;
+  case 1: return MX + dir;
+  case 2: return MX + it1;
+  case 3: return MX + it2;
+  case 4: return BX + dir;
+  case 5: return BX + it1;
+  case 6: return BX + it2;
+  default: return EN;
+  }
+}
+
+// the 16 scaled, rotated inputs of a face; returns false on a negative fast
+// speed squared
+template <int DIR, typename R>
+__device__ __forceinline__ bool face_inputs(const PhysT<R> &ph, const R *L,
+                                            const R *Rs, float *x, R &rhos,
+                                            R &cs) {
+  constexpr int it1 = (DIR + 1) % 3, it2 = (DIR + 2) % 3;
+  R unL, cfL, unR, cfR;
+  const bool ok = fast_speed<DIR>(ph, L, unL, cfL) & fast_speed<DIR>(ph, Rs, unR, cfR);
+  rhos = R(0.5) * (L[RHO] + Rs[RHO]);
+  cs = fmax2(cfL, cfR);
+  const R iu = R(1) / cs, ib = iu / fsqrt(rhos), ip = iu * iu / rhos,
+          ir = R(1) / rhos;
+  const R *S[2] = {L, Rs};
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    float *o = x + 8 * s;
+    o[0] = float(S[s][RHO] * ir);
+    o[1] = float(S[s][MX + DIR] * iu);
+    o[2] = float(S[s][MX + it1] * iu);
+    o[3] = float(S[s][MX + it2] * iu);
+    o[4] = float(S[s][BX + DIR] * ib);
+    o[5] = float(S[s][BX + it1] * ib);
+    o[6] = float(S[s][BX + it2] * ib);
+    o[7] = float(S[s][PP] * ip);
+  }
+  return ok;
+}
+
+template <typename R>
+__device__ __forceinline__ R out_scale(int c, R rhos, R cs) {
+  switch (c) {
+  case 0: return rhos * cs;
+  case 1: case 2: case 3: return rhos * cs * cs;
+  case 4: case 5: case 6: return cs * cs * fsqrt(rhos);
+  default: return rhos * cs * cs * cs;
+  }
+}
+
+// ---------------------------------------------------------------------
+// harvest (FP64 state): one workgroup per block, faces in chunks of 256,
+// deterministic placement from per-block counts (pass 1) and a workgroup
+// scan (pass 2)
+// ---------------------------------------------------------------------
+struct HarvestArgs {
+  const double *U;
+  const int *nei;
+  const double *cellsize;
+  long vs;
+  int nBlock;
+  Phys ph;
+  double beta, betaMin, frac;
+  unsigned long long seed;
+  long *count;        // pass 1: selected faces per block
+  const long *offset; // pass 2: exclusive prefix of count
+  float *X, *T;       // pass 2: sample rows
+};
+
+template <int NB, int LIM, bool WRITE>
+__global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
+  using G = FaceGeom<NB>;
+  __shared__ int sScan[256];
+  const int b = blockIdx.x;
+  if (b >= a.nBlock) return;
+  long base = WRITE ? a.offset[b] : 0;
+  long cnt = 0;
+  for (int c0 = 0; c0 < 3 * G::NF; c0 += 256) {
+    const int f = c0 + threadIdx.x;
+    bool sel = false;
+    float x[16], t[8];
+    if (f < 3 * G::NF) {
+      const int dir = f / G::NF, r = f % G::NF;
+      const unsigned long long gid =
+          ((unsigned long long)b * 3 + dir) * G::NF + r;
+      const double u = (double)(mix64(a.seed ^ mix64(gid + 0x9E3779B97F4A7C15ull)) >> 11) *
+                       (1.0 / 9007199254740992.0);
+      if (u < a.frac) {
+        int fi, fj, fk;
+        G::decode(dir, r, fi, fj, fk);
+        double q[4][NVAR];
+        for (int s = 0; s < 4; s++) {
+          int ii = fi, jj = fj, kk = fk;
+          if (dir == 0) ii += s - 2; else if (dir == 1) jj += s - 2; else kk += s - 2;
+          load_state(a.U, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk), q[s]);
+          to_primitive(q[s]);
+        }
+        double L[NVAR], R[NVAR];
+        face_states<LIM>(a.beta, q[0], q[1], q[2], q[3], L, R);
+        if (face_smooth(q[0], q[1], q[2], q[3], L, R, a.beta, a.betaMin)) {
+          double F[NFLUX], vdt, unA, rhos, cs;
+          bool ok;
+          if (dir == 0) {
+            ok = face_inputs<0>(a.ph, L, R, x, rhos, cs);
+            ok = ok && !numerical_flux<0, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+          } else if (dir == 1) {
+            ok = face_inputs<1>(a.ph, L, R, x, rhos, cs);
+            ok = ok && !numerical_flux<1, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+          } else {
+            ok = face_inputs<2>(a.ph, L, R, x, rhos, cs);
+            ok = ok && !numerical_flux<2, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+          }
+          if (ok && cs > 0.0) {
+            sel = true;
+            for (int c = 0; c < 8; c++)
+              t[c] = float(F[rot_slot(dir, c)] / out_scale(c, rhos, cs));
+          }
+        }
+      }
+    }
+    if (!WRITE) {
+      cnt += sel ? 1 : 0;
+      continue;
+    }
+    // workgroup exclusive scan of the selection flags
+    sScan[threadIdx.x] = sel ? 1 : 0;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const int v = threadIdx.x >= o ? sScan[threadIdx.x - o] : 0;
+      __syncthreads();
+      sScan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int incl = sScan[threadIdx.x], total = sScan[255];
+    if (sel) {
+      const long row = base + incl - 1;
+      for (int c = 0; c < 16; c++) a.X[row * 16 + c] = x[c];
+      for (int c = 0; c < 8; c++) a.T[row * 8 + c] = t[c];
+    }
+    base += total;
+    __syncthreads();
+  }
+  if (!WRITE) {
+    sScan[threadIdx.x] = (int)cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long s = 0;
+      for (int q = 0; q < 256; q++) s += sScan[q];
+      a.count[b] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// surrogate stage kernels (FP32 state)
+// ---------------------------------------------------------------------
+struct SurArgs {
+  const float *Uin, *Uold;
+  float *Uout;
+  const int *nei;
+  const double *cellsize;
+  double *scal;
+  const int *runOfBlock;
+  int nRun;
+  double *dtmaxB, *dtmaxCB;
+  int *err;
+  int nBlock;
+  long vs;
+  Phys ph;
+  double beta, betaMin, DtFactor;
+  int lim, mode, iStage, nStage, halfStep, dbTrick, doDt, roundTripOld,
+      noUpdate, advanceTime;
+  float *face;     // NSLOT x nFaceTot
+  long nFaceTot;   // 3 * nBlock * NF
+  float *rowX;
+  int *rowFace;
+  float *rowScale;
+  int *counters;
+};
+
+template <int NB, int LIM>
+__global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
+  using G = FaceGeom<NB>;
+  const PhysT<float> ph = phys_as<float>(a.ph);
+  const float beta = float(a.beta), betaMin = float(a.betaMin);
+  int err = 0;
+  long nNet = 0;
+  const long nTot = a.nFaceTot;
+  for (long gf = blockIdx.x * (long)blockDim.x + threadIdx.x;
+       gf - threadIdx.x < nTot; gf += (long)gridDim.x * blockDim.x) {
+    const bool valid = gf < nTot;
+    bool net = false;
+    float x[16];
+    float rhos = 0.f, cs = 0.f;
+    if (valid) {
+      const int dir = (int)(gf / ((long)a.nBlock * G::NF));
+      const long rem = gf % ((long)a.nBlock * G::NF);
+      const int b = (int)(rem / G::NF), r = (int)(rem % G::NF);
+      int fi, fj, fk;
+      G::decode(dir, r, fi, fj, fk);
+      float q[4][NVAR];
+      for (int s = 0; s < 4; s++) {
+        int ii = fi, jj = fj, kk = fk;
+        if (dir == 0) ii += s - 2; else if (dir == 1) jj += s - 2; else kk += s - 2;
+        load_state(a.Uin, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk), q[s]);
+        to_primitive(q[s]);
+      }
+      float L[NVAR], R[NVAR];
+      face_states<LIM>(beta, q[0], q[1], q[2], q[3], L, R);
+      const double *cz = a.cellsize + 3 * b;
+      const float area = float(dir == 0 ? cz[1] * cz[2]
+                               : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
+      const bool smooth =
+          a.mode != BATS_SUR_OFF && face_smooth(q[0], q[1], q[2], q[3], L, R,
+                                                beta, betaMin);
+      float F[NFLUX], vdt, unA;
+      if (smooth && a.mode == BATS_SUR_NETWORK) {
+        // exact Vdt from the HLLD speeds; the flux comes from the network
+        bool ok;
+        if (dir == 0) ok = face_inputs<0>(ph, L, R, x, rhos, cs);
+        else if (dir == 1) ok = face_inputs<1>(ph, L, R, x, rhos, cs);
+        else ok = face_inputs<2>(ph, L, R, x, rhos, cs);
+        if (!ok) err |= ERRB_NEGFAST2;
+        // HLLD's Cmax = max(sR, -sL) of the Linde-corrected states
+        float Lc[NVAR], Rc[NVAR];
+        for (int v = 0; v < NVAR; v++) { Lc[v] = L[v]; Rc[v] = R[v]; }
+        const float dB = 0.5f * (Rc[BX + dir] - Lc[BX + dir]);
+        Lc[BX + dir] += dB;
+        Rc[BX + dir] -= dB;
+        float uL, cL, uR, cR;
+        if (dir == 0) { fast_speed<0>(ph, Lc, uL, cL); fast_speed<0>(ph, Rc, uR, cR); }
+        else if (dir == 1) { fast_speed<1>(ph, Lc, uL, cL); fast_speed<1>(ph, Rc, uR, cR); }
+        else { fast_speed<2>(ph, Lc, uL, cL); fast_speed<2>(ph, Rc, uR, cR); }
+        const float sL = fmin2(uL - cL, uR - cR), sR = fmax2(uL + cL, uR + cR);
+        a.face[PP * nTot + gf] = fmax2(sR, -sL) * area;
+        net = true;
+      } else {
+        if (smooth) {  // the method without the network: Rusanov
+          if (dir == 0) err |= numerical_flux<0, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+          else if (dir == 1) err |= numerical_flux<1, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+          else err |= numerical_flux<2, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+        } else {
+          if (dir == 0) err |= numerical_flux<0, FLUX_HLLD, false>(ph, area, L, R, F, vdt, unA);
+          else if (dir == 1) err |= numerical_flux<1, FLUX_HLLD, false>(ph, area, L, R, F, vdt, unA);
+          else err |= numerical_flux<2, FLUX_HLLD, false>(ph, area, L, R, F, vdt, unA);
+        }
+        F[PP] = vdt;
+        for (int v = 0; v < NSLOT; v++) a.face[v * nTot + gf] = F[v];
+      }
+    }
+    // wave-aggregated row allocation for the network faces
+    const unsigned long long mask = __ballot(net);
+    if (mask) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)mask) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&a.counters[0], __popcll(mask));
+      base = __shfl(base, leader, 64);
+      if (net) {
+        const int row = base + __popcll(mask & ((1ull << lane) - 1ull));
+        for (int c = 0; c < 16; c++) a.rowX[(long)row * 16 + c] = x[c];
+        a.rowFace[row] = (int)gf;
+        a.rowScale[2 * row] = rhos;
+        a.rowScale[2 * row + 1] = cs;
+      }
+      nNet += net ? 1 : 0;
+    }
+  }
+  if (err) atomicOr(a.err, err);
+  (void)nNet;
+}
+
+// the network on the compacted rows; fluxes back to the face array
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
+  using namespace bats::mlp;
+  using G = FaceGeom<NB>;
+  __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
+  __shared__ __attribute__((aligned(16))) __bf16 sAct[4][16 * ACT_LD];
+  for (int t = threadIdx.x; t < NFRAG / 8; t += 256)
+    reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
+  __syncthreads();
+  const int rows = a.counters[0];
+  const int wave = threadIdx.x >> 6;
+  const long nTot = a.nFaceTot;
+  auto out = [&](int row, int c, float y) {
+    const int gf = a.rowFace[row];
+    const int dir = gf / (a.nBlock * G::NF);
+    const int b = (gf % (a.nBlock * G::NF)) / G::NF;
+    const double *cz = a.cellsize + 3 * b;
+    const float area = float(dir == 0 ? cz[1] * cz[2]
+                             : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
+    const float v = y * out_scale(c, a.rowScale[2 * row], a.rowScale[2 * row + 1]);
+    a.face[(long)rot_slot(dir, c) * nTot + gf] = v * area;
+  };
+  const int nTile = (rows + 15) / 16;
+  for (int tile = blockIdx.x * 4 + wave; tile < nTile; tile += gridDim.x * 4)
+    infer_tile(sW, w, sAct[wave], a.rowX, rows, tile * 16, out);
+}
+
+// flux divergence and stage update (FP64), local time step; one workgroup
+// per block
+template <int NB>
+__global__ void __launch_bounds__(256) k_sur_update(SurArgs a) {
+  using G = FaceGeom<NB>;
+  __shared__ double sRed[4];
+  const int b = blockIdx.x;
+  if (b >= a.nBlock) return;
+  const long nTot = a.nFaceTot;
+  const long dOff = (long)a.nBlock * G::NF;
+  const double *cz = a.cellsize + 3 * b;
+  const double CellVolume = cz[0] * cz[1] * cz[2];
+  const double *rs = a.scal + (a.runOfBlock ? a.runOfBlock[b] : 0) * S_NSCAL;
+  UpdateCtl uc;
+  uc.DtLocal = a.DtFactor * rs[S_DTCB];
+  uc.InvVolume = 1.0 / CellVolume;
+  uc.halfStep = a.halfStep;
+  uc.iStage = a.iStage;
+  uc.nStage = a.nStage;
+  uc.dbTrick = a.dbTrick;
+  uc.dtZero = (rs[S_DT] == 0.0);
+  uc.roundTripOld = a.roundTripOld;
+  double dtmin = 1.7976931348623157e308;
+  for (int c = threadIdx.x; c < NB * NB * NB; c += 256) {
+    const int i = c % NB, j = (c / NB) % NB, k = c / (NB * NB);
+    const long fx = 0 * dOff + (long)b * G::NF + (k * NB + j) * (NB + 1) + i;
+    const long fy = 1 * dOff + (long)b * G::NF + (k * (NB + 1) + j) * NB + i;
+    const long fz = 2 * dOff + (long)b * G::NF + (k * NB + j) * NB + i;
+    double dF[NFLUX];
+    for (int v = 0; v < NFLUX; v++) {
+      if (v == PP) { dF[v] = 0.0; continue; }
+      const float *Fv = a.face + (long)v * nTot;
+      float d = Fv[fx] - Fv[fx + 1];
+      d = d + Fv[fy] - Fv[fy + NB];
+      d = d + Fv[fz] - Fv[fz + NB * NB];
+      dF[v] = double(d);
+    }
+    const long off = (((long)b * NB + k) * NB + j) * NB + i;
+    if (!a.noUpdate) {
+      float Uc[NVAR], Uo[NVAR];
+      double Ucd[NVAR], Uod[NVAR], Un[NVAR];
+      load_state(a.Uold, a.vs, off, Uo);
+      load_state(a.Uin, a.vs, off, Uc);
+      for (int v = 0; v < NVAR; v++) { Ucd[v] = Uc[v]; Uod[v] = Uo[v]; }
+      update_cell(a.ph, uc, dF, Ucd, Uod, Un);
+      for (int v = 0; v < NVAR; v++) a.Uout[v * a.vs + off] = float(Un[v]);
+    }
+    if (a.doDt) {
+      const float *Fv = a.face + (long)PP * nTot;
+      const float Vdt = fmax2(Fv[fx], Fv[fx + 1]) + fmax2(Fv[fy], Fv[fy + NB]) +
+                        fmax2(Fv[fz], Fv[fz + NB * NB]);
+      const double dtc = CellVolume / double(Vdt);
+      if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
+      dtmin = fmin2(dtmin, dtc);
+    }
+  }
+  if (a.doDt) {
+    double m = wave_min(dtmin);
+    if ((threadIdx.x & 63) == 0) sRed[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      a.dtmaxB[b] = fmin2(fmin2(sRed[0], sRed[1]), fmin2(sRed[2], sRed[3]));
+  }
+  if (a.advanceTime && b == 0)
+    for (int r = threadIdx.x; r < a.nRun; r += 256) {
+      double *rr = a.scal + r * S_NSCAL;
+      rr[S_TSIM] = rr[S_TSIM] + rr[S_DT] / a.nStage;
+    }
+}
+
+__global__ void k_count_faces(int *counters, long nFace) {
+  // rows of this stage -> running totals (single thread)
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    atomicAdd(&counters[1], counters[0]);
+    atomicAdd(reinterpret_cast<unsigned long long *>(counters + 2),
+              (unsigned long long)nFace);
+    counters[0] = 0;
+  }
+}
+
+}  // namespace
+
+// =====================================================================
+// host side
+// =====================================================================
+template <int NB, bool W>
+static void launch_harvest(int lim, int nB, hipStream_t st, const HarvestArgs &a) {
+  const dim3 g(nB), t(256);
+  switch (lim) {
+  case LIM_MINMOD: hipLaunchKernelGGL((k_harvest<NB, LIM_MINMOD, W>), g, t, 0, st, a); break;
+  case LIM_MC: hipLaunchKernelGGL((k_harvest<NB, LIM_MC, W>), g, t, 0, st, a); break;
+  case LIM_MC3: hipLaunchKernelGGL((k_harvest<NB, LIM_MC3, W>), g, t, 0, st, a); break;
+  case LIM_BETA: hipLaunchKernelGGL((k_harvest<NB, LIM_BETA, W>), g, t, 0, st, a); break;
+  case LIM_NO: hipLaunchKernelGGL((k_harvest<NB, LIM_NO, W>), g, t, 0, st, a); break;
+  default: hipLaunchKernelGGL((k_harvest<NB, LIM_FIRST, W>), g, t, 0, st, a); break;
+  }
+}
+
+template <int NB>
+static void launch_sur_faces(int lim, dim3 g, hipStream_t st, const SurArgs &a) {
+  const dim3 t(256);
+  switch (lim) {
+  case LIM_MINMOD: hipLaunchKernelGGL((k_sur_faces<NB, LIM_MINMOD>), g, t, 0, st, a); break;
+  case LIM_MC: hipLaunchKernelGGL((k_sur_faces<NB, LIM_MC>), g, t, 0, st, a); break;
+  case LIM_MC3: hipLaunchKernelGGL((k_sur_faces<NB, LIM_MC3>), g, t, 0, st, a); break;
+  case LIM_BETA: hipLaunchKernelGGL((k_sur_faces<NB, LIM_BETA>), g, t, 0, st, a); break;
+  case LIM_NO: hipLaunchKernelGGL((k_sur_faces<NB, LIM_NO>), g, t, 0, st, a); break;
+  default: hipLaunchKernelGGL((k_sur_faces<NB, LIM_FIRST>), g, t, 0, st, a); break;
+  }
+}
+
+extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
+                                double BetaMin, unsigned long long Seed,
+                                long *nAdded) {
+  if (!m || !h || !(Fraction >= 0.0) || Fraction > 1.0 || !(BetaMin >= 0.0))
+    return BATS_EINVAL;
+  if (h->elemSize != 8) return BATS_EINVAL;  // samples come from FP64 runs
+  const bats_config &c = h->cfg;
+  const int nB = c.nBlock, lim = limiter_code(c);
+  HarvestArgs a;
+  a.U = static_cast<const double *>(h->U[h->stateBuf]);
+  a.nei = h->nei;
+  a.cellsize = h->cellsize;
+  a.vs = h->nCell;
+  a.nBlock = nB;
+  a.ph = make_phys(c);
+  a.beta = c.LimiterBeta;
+  a.betaMin = BetaMin;
+  a.frac = Fraction;
+  a.seed = Seed;
+  a.X = a.T = nullptr;
+  long *dCount = nullptr, *dOff = nullptr;
+  HIPCHK(hipMalloc(&dCount, sizeof(long) * nB));
+  HIPCHK(hipMalloc(&dOff, sizeof(long) * nB));
+  a.count = dCount;
+  a.offset = dOff;
+  if (h->NB == 8) launch_harvest<8, false>(lim, nB, h->stream, a);
+  else launch_harvest<16, false>(lim, nB, h->stream, a);
+  HIPCHK(hipGetLastError());
+  std::vector<long> cnt(nB), off(nB);
+  HIPCHK(hipMemcpyAsync(cnt.data(), dCount, sizeof(long) * nB,
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  long total = 0;
+  for (int b = 0; b < nB; b++) { off[b] = total; total += cnt[b]; }
+  int st = BATS_OK;
+  if (total > 0) {
+    float *X = nullptr, *T = nullptr;
+    st = bats::mlp::reserve_data(m, total, &X, &T);
+    if (st == BATS_OK) {
+      HIPCHK(hipMemcpyAsync(dOff, off.data(), sizeof(long) * nB,
+                            hipMemcpyHostToDevice, h->stream));
+      a.X = X;
+      a.T = T;
+      if (h->NB == 8) launch_harvest<8, true>(lim, nB, h->stream, a);
+      else launch_harvest<16, true>(lim, nB, h->stream, a);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(h->stream));
+      bats::mlp::commit_data(m, total);
+    }
+  }
+  (void)hipFree(dCount);
+  (void)hipFree(dOff);
+  if (nAdded) *nAdded = st == BATS_OK ? total : 0;
+  return st;
+}
+
+extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
+                                  double BetaMin) {
+  if (!h || Mode < BATS_SUR_OFF || Mode > BATS_SUR_RUSANOV || !(BetaMin >= 0.0))
+    return BATS_EINVAL;
+  if (Mode == BATS_SUR_NETWORK && !m) return BATS_EINVAL;
+  if (Mode != BATS_SUR_OFF && h->elemSize != 4) return BATS_EINVAL;
+  SurrogateState &s = h->sur;
+  if (Mode != BATS_SUR_OFF && !s.face) {
+    const long nF = (long)(h->NB + 1) * h->NB * h->NB;
+    const long nTot = 3 * (long)h->cfg.nBlock * nF;
+    HIPCHK(hipMalloc(&s.face, sizeof(float) * NSLOT * nTot));
+    HIPCHK(hipMalloc(&s.rowX, sizeof(float) * 16 * nTot));
+    HIPCHK(hipMalloc(&s.rowFace, sizeof(int) * nTot));
+    HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 2 * nTot));
+    HIPCHK(hipMalloc(&s.counters, sizeof(int) * 4));
+    s.faceCap = nTot;
+  }
+  if (s.counters) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemset(s.counters, 0, sizeof(int) * 4));
+  }
+  s.model = m;
+  s.mode = Mode;
+  s.betaMin = BetaMin;
+  return BATS_OK;
+}
+
+extern "C" int bats_surrogate_counts(bats_handle *h, long *nFace, long *nNetwork) {
+  if (!h) return BATS_EINVAL;
+  int c[4] = {0, 0, 0, 0};
+  if (h->sur.counters) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(c, h->sur.counters, sizeof c, hipMemcpyDeviceToHost));
+  }
+  unsigned long long f = 0;
+  memcpy(&f, c + 2, sizeof f);
+  if (nFace) *nFace = (long)f;
+  if (nNetwork) *nNetwork = c[1];
+  return BATS_OK;
+}
+
+// one stage through the surrogate kernels (called by the stage launcher)
+int sur_stage(bats_handle *h, const StageArgs &sa) {
+  SurrogateState &s = h->sur;
+  const bats_config &c = h->cfg;
+  SurArgs a;
+  a.Uin = static_cast<const float *>(sa.Uin);
+  a.Uold = static_cast<const float *>(sa.Uold);
+  a.Uout = static_cast<float *>(sa.Uout);
+  a.nei = sa.nei;
+  a.cellsize = sa.cellsize;
+  a.scal = sa.scal;
+  a.runOfBlock = sa.runOfBlock;
+  a.nRun = sa.nRun;
+  a.dtmaxB = sa.dtmaxB;
+  a.dtmaxCB = sa.dtmaxCB;
+  a.err = sa.err;
+  a.nBlock = sa.nBlock;
+  a.vs = sa.vs;
+  a.ph = sa.ph;
+  a.beta = sa.beta;
+  a.betaMin = s.betaMin;
+  a.DtFactor = sa.DtFactor;
+  a.lim = sa.lim;
+  a.mode = s.mode;
+  a.iStage = sa.iStage;
+  a.nStage = sa.nStage;
+  a.halfStep = sa.halfStep;
+  a.dbTrick = sa.dbTrick;
+  a.doDt = sa.doDt;
+  a.roundTripOld = sa.roundTripOld;
+  a.noUpdate = sa.noUpdate;
+  a.advanceTime = sa.advanceTime;
+  a.face = s.face;
+  a.nFaceTot = s.faceCap;
+  a.rowX = s.rowX;
+  a.rowFace = s.rowFace;
+  a.rowScale = s.rowScale;
+  a.counters = s.counters;
+  const long nTot = s.faceCap;
+  long nWg = (nTot + 255) / 256;
+  if (nWg > 16384) nWg = 16384;
+  const int lim = sa.lim;
+  if (h->NB == 8) launch_sur_faces<8>(lim, dim3((unsigned)nWg), h->stream, a);
+  else launch_sur_faces<16>(lim, dim3((unsigned)nWg), h->stream, a);
+  if (s.mode == BATS_SUR_NETWORK) {
+    const bats::mlp::Weights wts = bats::mlp::weights(s.model);
+    if (h->NB == 8)
+      hipLaunchKernelGGL(k_sur_infer<8>, dim3(1024), dim3(256), 0, h->stream,
+                         wts.frag, wts.w, a);
+    else
+      hipLaunchKernelGGL(k_sur_infer<16>, dim3(1024), dim3(256), 0, h->stream,
+                         wts.frag, wts.w, a);
+  }
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_sur_update<8>, dim3(c.nBlock), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_sur_update<16>, dim3(c.nBlock), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(k_count_faces, dim3(1), dim3(64), 0, h->stream, s.counters,
+                     nTot);
+  HIPCHK(hipGetLastError());
+  return BATS_OK;
+}

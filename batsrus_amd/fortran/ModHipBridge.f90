@@ -40,7 +40,11 @@ module ModHipBridge
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
-       bats_set_runs, bats_get_run_times, bats_set_persistent_grid
+       bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
+       bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
+       bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
+       bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
+       bats_mlp_train, bats_set_surrogate, bats_surrogate_counts
   ! reference-named routines on the module-level handle
   public:: init_hip_bridge, clean_hip_bridge, sync_state_to_gpu, &
        sync_state_to_cpu, calc_face_value, calc_face_flux, calc_source, &
@@ -234,6 +238,91 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: nWorkgroup
      end function bats_set_persistent_grid
+     ! learned face-flux surrogate (no counterpart in the reference)
+     integer(c_int) function bats_mlp_create(Model) &
+          bind(C, name="bats_mlp_create")
+       import:: c_int, c_ptr
+       type(c_ptr), intent(out):: Model
+     end function bats_mlp_create
+     integer(c_int) function bats_mlp_free(Model) &
+          bind(C, name="bats_mlp_free")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Model
+     end function bats_mlp_free
+     integer(c_int) function bats_mlp_set_params(Model, W) &
+          bind(C, name="bats_mlp_set_params")
+       import:: c_int, c_ptr, c_float
+       type(c_ptr), value:: Model
+       real(c_float), intent(in):: W(*)
+     end function bats_mlp_set_params
+     integer(c_int) function bats_mlp_get_params(Model, W, Grad) &
+          bind(C, name="bats_mlp_get_params")
+       import:: c_int, c_ptr, c_float
+       type(c_ptr), value:: Model
+       real(c_float), intent(out):: W(*), Grad(*)
+     end function bats_mlp_get_params
+     integer(c_int) function bats_mlp_forward(Model, X, Y, nRow) &
+          bind(C, name="bats_mlp_forward")
+       import:: c_int, c_ptr, c_float
+       type(c_ptr), value:: Model
+       real(c_float), intent(in):: X(*)
+       real(c_float), intent(out):: Y(*)
+       integer(c_int), value:: nRow
+     end function bats_mlp_forward
+     integer(c_int) function bats_mlp_loss_grad(Model, X, T, nRow, Loss, &
+          Grad) bind(C, name="bats_mlp_loss_grad")
+       import:: c_int, c_ptr, c_float, c_double
+       type(c_ptr), value:: Model
+       real(c_float), intent(in):: X(*), T(*)
+       integer(c_int), value:: nRow
+       real(c_double), intent(out):: Loss
+       real(c_float), intent(out):: Grad(*)
+     end function bats_mlp_loss_grad
+     integer(c_int) function bats_mlp_set_data(Model, X, T, n) &
+          bind(C, name="bats_mlp_set_data")
+       import:: c_int, c_ptr, c_float, c_long
+       type(c_ptr), value:: Model
+       real(c_float), intent(in):: X(*), T(*)
+       integer(c_long), value:: n
+     end function bats_mlp_set_data
+     integer(c_int) function bats_mlp_get_data(Model, n, X, T) &
+          bind(C, name="bats_mlp_get_data")
+       import:: c_int, c_ptr, c_float, c_long
+       type(c_ptr), value:: Model
+       integer(c_long), intent(out):: n
+       real(c_float), intent(out):: X(*), T(*)
+     end function bats_mlp_get_data
+     integer(c_int) function bats_mlp_harvest(Model, Handle, Fraction, &
+          BetaMin, Seed, nAdded) bind(C, name="bats_mlp_harvest")
+       import:: c_int, c_ptr, c_double, c_long, c_long_long
+       type(c_ptr), value:: Model, Handle
+       real(c_double), value:: Fraction, BetaMin
+       integer(c_long_long), value:: Seed
+       integer(c_long), intent(out):: nAdded
+     end function bats_mlp_harvest
+     integer(c_int) function bats_mlp_train(Model, nEpoch, Batch, Lr0, &
+          HoldFrac, Seed, TrainLoss, HeldOutRelErr_V) &
+          bind(C, name="bats_mlp_train")
+       import:: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value:: Model
+       integer(c_int), value:: nEpoch, Batch
+       real(c_double), value:: Lr0, HoldFrac
+       integer(c_long_long), value:: Seed
+       real(c_double), intent(out):: TrainLoss, HeldOutRelErr_V(9)
+     end function bats_mlp_train
+     integer(c_int) function bats_set_surrogate(Handle, Model, Mode, &
+          BetaMin) bind(C, name="bats_set_surrogate")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle, Model
+       integer(c_int), value:: Mode
+       real(c_double), value:: BetaMin
+     end function bats_set_surrogate
+     integer(c_int) function bats_surrogate_counts(Handle, nFace, &
+          nNetwork) bind(C, name="bats_surrogate_counts")
+       import:: c_int, c_ptr, c_long
+       type(c_ptr), value:: Handle
+       integer(c_long), intent(out):: nFace, nNetwork
+     end function bats_surrogate_counts
      integer(c_int) function bats_check_nan(Handle) &
           bind(C, name="bats_check_nan")
        import:: c_int, c_ptr

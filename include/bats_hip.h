@@ -205,6 +205,58 @@ int bats_get_run_times(bats_handle *h, double *Dt_R, double *tSimulation_R,
  * group of blocks; n > 0 launches at most n workgroups that walk the flat
  * block work list persistently. */
 int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
+/* ------------------------------------------------------------------------
+ * Learned face-flux surrogate.  No counterpart in the reference.  A small
+ * network (16 -> 128 -> 128 -> 64 -> 8, SiLU; FP32 master weights, BF16
+ * weights and activations on the matrix cores, FP32 accumulation) maps the
+ * left/right primitive face states, rotated to the face normal and scaled
+ * by the face's mean density and larger fast speed, to the face's
+ * numerical flux in the same frame and scaling (8 values: mass, normal and
+ * two tangential momenta, normal and two tangential field, energy).  The
+ * parameter vector is flat: W1 (16 x 128, row-major [in][out]), b1, W2
+ * (128 x 128), b2, W3 (128 x 64), b3, W4 (64 x 8), b4 -- 27,464 floats. */
+typedef struct bats_mlp bats_mlp;
+enum { BATS_MLP_NPARAM = 27464, BATS_MLP_NIN = 16, BATS_MLP_NOUT = 8 };
+int bats_mlp_create(bats_mlp **m);
+int bats_mlp_free(bats_mlp *m);
+/* FP32 master weights in (resets the Adam moments) / out; Grad receives
+ * the gradient of the last bats_mlp_loss_grad or training step. */
+int bats_mlp_set_params(bats_mlp *m, const float *W);
+int bats_mlp_get_params(bats_mlp *m, float *W, float *Grad);
+/* Y(nRow x 8) = network(X(nRow x 16)), host arrays */
+int bats_mlp_forward(bats_mlp *m, const float *X, float *Y, int nRow);
+/* mean-square loss of the network on (X, T) and its gradient (27,464) */
+int bats_mlp_loss_grad(bats_mlp *m, const float *X, const float *T, int nRow,
+                       double *Loss, float *Grad);
+/* training samples held on the device: inputs n x 16, targets n x 8 */
+int bats_mlp_set_data(bats_mlp *m, const float *X, const float *T, long n);
+int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T);
+/* Harvest samples from a solver's current state (FP64 handle): a seeded
+ * random fraction of all faces that pass the smoothness gate, each with its
+ * FP64 HLLD (Linde) flux as target; appended to the model's samples. */
+int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
+                     double BetaMin, unsigned long long Seed, long *nAdded);
+/* Adam with cosine learning-rate decay over nEpoch passes of minibatches;
+ * HoldFrac of the samples (seeded split) are held out; returns the last
+ * minibatch loss and the held-out relative L2 error ||Y-T||/||T|| overall
+ * and per output (9 values). */
+int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
+                   double HoldFrac, unsigned long long Seed, double *TrainLoss,
+                   double *HeldOutRelErr_V);
+
+/* Face fluxes of the stage update on an FP32-state handle.  Mode
+ * BATS_SUR_OFF: the plain fused stage kernel.  BATS_SUR_NETWORK: faces that
+ * pass the smoothness gate (the limiter leaves both cells' slopes unclipped
+ * for every variable, and the plasma beta of both states is at least
+ * BetaMin) take the network's flux, all others the exact HLLD (Linde) flux.
+ * BATS_SUR_RUSANOV: the same gate with the Rusanov flux on the smooth faces
+ * (the method without the network).  Every face still carries one flux, so
+ * the update stays conservative.  The model must outlive its use. */
+enum { BATS_SUR_OFF = 0, BATS_SUR_NETWORK = 1, BATS_SUR_RUSANOV = 2 };
+int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode, double BetaMin);
+/* faces evaluated and faces given to the network since bats_set_surrogate */
+int bats_surrogate_counts(bats_handle *h, long *nFace, long *nNetwork);
+
 /* check_nan (src/ModUpdateState.f90:2003-2040): BATS_ENAN if any state
  * value is NaN or infinite */
 int bats_check_nan(bats_handle *h);

@@ -1,0 +1,101 @@
+"""The surrogate network's GPU kernels (matrix-core forward, backward, Adam)
+against the CPU oracle of the same network with the same BF16 rounding
+points.  The oracle forms products and sums in FP64, the GPU accumulates in
+FP32, so results agree to FP32 accumulation error except where an
+intermediate lands on the other side of a BF16 rounding boundary."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+from batsrus_amd.core import Mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    w = mo.init_params(seed=seed).astype(np.float64)
+    w += rng.normal(0, 0.02, w.size)            # non-zero biases
+    return w.astype(np.float32)
+
+
+@pytest.mark.parametrize("rows", [1, 16, 1003])
+def test_forward_matches_oracle(rows):
+    print(f"case mlp forward rows={rows}", flush=True)
+    rng = np.random.default_rng(rows)
+    w = _params(1)
+    X = rng.normal(0, 1, (rows, 16)).astype(np.float32)
+    with Mlp(w) as m:
+        Y = m.forward(X)
+    Yo = mo.forward(w, X, rounding=True)
+    err = np.abs(Y - Yo)
+    scale = np.abs(Yo).max()
+    assert np.median(err) <= 1e-4 * scale
+    assert err.max() <= 2e-2 * scale
+
+
+def test_gradient_matches_oracle():
+    print("case mlp gradient", flush=True)
+    rng = np.random.default_rng(5)
+    w = _params(2)
+    X = rng.normal(0, 1, (700, 16)).astype(np.float32)
+    T = rng.normal(0, 1, (700, 8)).astype(np.float32)
+    with Mlp(w) as m:
+        loss, g = m.loss_grad(X, T)
+    lo, go = mo.loss_and_grad(w, X, T, rounding=True)
+    assert abs(loss - lo) <= 1e-4 * lo
+    # per parameter block: relative L2 error and direction
+    for name, shp in mo.param_shapes():
+        o = 0
+        for n2, s2 in mo.param_shapes():
+            if n2 == name:
+                break
+            o += int(np.prod(s2))
+        n = int(np.prod(shp))
+        a, b = g[o:o + n].astype(np.float64), go[o:o + n]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        print(f"  {name}: rel L2 {rel:.2e} cos {cos:.6f}")
+        assert rel < 2e-2 and cos > 0.9995, name
+
+
+def test_adam_step_matches_oracle():
+    """One training step on the whole set (no hold-out): the new weights are
+    the oracle's Adam update of the old ones with the GPU's gradient."""
+    print("case mlp adam", flush=True)
+    rng = np.random.default_rng(6)
+    w0 = _params(3)
+    X = rng.normal(0, 1, (512, 16)).astype(np.float32)
+    T = rng.normal(0, 1, (512, 8)).astype(np.float32)
+    with Mlp(w0) as m:
+        m.set_data(X, T)
+        m.train(1, batch=512, lr=1e-3, hold=0.0, seed=1)
+        w1, g = m.params()
+    wo = w0.astype(np.float64).copy()
+    mom, vel = np.zeros_like(wo), np.zeros_like(wo)
+    mo.adam_step(wo, mom, vel, g.astype(np.float64), 1, 1e-3)
+    assert np.abs(w1 - wo).max() <= 1e-6
+    # and the gradient of that step is the oracle's gradient
+    _, go = mo.loss_and_grad(w0, X, T, rounding=True)
+    assert np.linalg.norm(g - go) < 2e-2 * np.linalg.norm(go)
+
+
+def test_training_learns_a_smooth_map():
+    """Training on a smooth synthetic map reduces the held-out error far
+    below the untrained network's, deterministically for a fixed seed."""
+    print("case mlp training", flush=True)
+    rng = np.random.default_rng(7)
+    n = 40000
+    X = rng.uniform(-1, 1, (n, 16)).astype(np.float32)
+    A = rng.normal(0, 0.5, (16, 8))
+    T = (np.tanh(X @ A) + 0.1 * (X[:, :8] ** 2)).astype(np.float32)
+    res = []
+    for _ in range(2):
+        with Mlp(mo.init_params(seed=9)) as m:
+            m.set_data(X, T)
+            _, e0 = m.train(0, batch=4096, hold=0.1, seed=3)
+            loss, e1 = m.train(30, batch=4096, lr=3e-3, hold=0.1, seed=3)
+            res.append((e0[0], e1[0], m.params()[0]))
+    print(f"  held-out rel L2: untrained {res[0][0]:.3f} trained {res[0][1]:.4f}")
+    assert res[0][1] < 0.1 * res[0][0] and res[0][1] < 0.05
+    assert np.array_equal(res[0][2], res[1][2])   # deterministic

@@ -1,0 +1,187 @@
+"""The surrogate stage on the GPU: harvested samples equal the oracle's face
+states, gate and FP64 HLLD fluxes in the network's frame and scaling; the
+gated stage reproduces the plain FP32 path when no face passes the gate;
+the update stays conservative with the network in the loop; and a trained
+network is closer to the FP64 HLLD run than the same gate with the Rusanov
+flux (what the network gains over the method without it)."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import pyoracle as po
+from batsrus_amd.core import Session, Mlp
+from batsrus_amd.grid import BlockGrid
+from helpers import smooth_mhd_soa
+
+pytestmark = pytest.mark.gpu
+GAMMA = 5.0 / 3.0
+
+
+def _prim(S):
+    P = S.copy()
+    P[..., 1:4] = S[..., 1:4] / S[..., :1]
+    return P
+
+
+def _cf(S, d):
+    rho, p = S[..., 0], S[..., 7]
+    B = S[..., 4:7]
+    cs2 = GAMMA * p / rho
+    va2 = (B * B).sum(-1) / rho
+    vn2 = B[..., d] ** 2 / rho
+    return np.sqrt(0.5 * (cs2 + va2 + np.sqrt(np.maximum(
+        0.0, (cs2 + va2) ** 2 - 4 * cs2 * vn2))))
+
+
+def _smooth(p0, p1, p2, beta):
+    d1, d2 = p1 - p0, p2 - p1
+    return ((d1 == 0) & (d2 == 0)) | (
+        (d1 * d2 > 0) & (0.5 * np.abs(d1 + d2) <= beta * np.minimum(
+            np.abs(d1), np.abs(d2))))
+
+
+def _rot(d):
+    t1, t2 = (d + 1) % 3, (d + 2) % 3
+    return [0, 1 + d, 1 + t1, 1 + t2, 4 + d, 4 + t1, 4 + t2]
+
+
+def _oracle_samples(grid, S, cfg, beta):
+    """All smooth faces of all blocks in the harvest order."""
+    X, T = [], []
+    nb = grid.nI
+    for b in range(grid.nBlock):
+        blk = np.ascontiguousarray(S[b])
+        P = _prim(blk)
+        faces = po.calc_face_value(cfg, blk)
+        for d in range(3):
+            L, R = faces[2 * d], faces[2 * d + 1]
+            shp = L.shape[:3]
+            for idx in np.ndindex(*shp):
+                k, j, i = idx
+                # stencil cells f-2..f+1 along d in ghosted indices
+                g = [k + 2, j + 2, i + 2]
+                ax = 2 - d
+                cells = []
+                for s in range(4):
+                    gg = list(g)
+                    gg[ax] += s - 2
+                    cells.append(P[tuple(gg)])
+                ok = all(_smooth(cells[0][v], cells[1][v], cells[2][v], beta) and
+                         _smooth(cells[1][v], cells[2][v], cells[3][v], beta)
+                         for v in range(8))
+                if not ok:
+                    continue
+                l, r = L[idx], R[idx]
+                _, F = po.numerical_flux(cfg, d, 1.0, l, r)
+                rhos = 0.5 * (l[0] + r[0])
+                cs = max(_cf(l, d), _cf(r, d))
+                rot = _rot(d) + [7]
+                x = []
+                for st in (l, r):
+                    v = st[rot]
+                    x += [v[0] / rhos] + list(v[1:4] / cs) + \
+                         list(v[4:7] / (cs * np.sqrt(rhos))) + \
+                         [v[7] / (rhos * cs * cs)]
+                fr = [F[i_] for i_ in _rot(d)] + [F[8]]
+                sc = [rhos * cs] + [rhos * cs * cs] * 3 + \
+                     [cs * cs * np.sqrt(rhos)] * 3 + [rhos * cs ** 3]
+                X.append(x)
+                T.append(np.array(fr) / np.array(sc))
+    return np.array(X), np.array(T)
+
+
+def test_harvest_matches_oracle():
+    print("case harvest", flush=True)
+    grid = BlockGrid(nRoot=(2, 1, 1), nCell=(8, 8, 8))
+    U0 = smooth_mhd_soa(grid, seed=61, amp=0.2)
+    scheme = dict(flux="hlld", limiter="mc", beta=1.5)
+    cfg = po.config_from_dict(scheme, grid)
+    S = grid.soa_to_state_vgb(U0)
+    grid.fill_ghosts(S)
+    Xo, To = _oracle_samples(grid, S, cfg, 1.5)
+    with Session(grid, **scheme) as s, Mlp(mo.init_params()) as m:
+        s.upload_soa(U0)
+        n = m.harvest(s, 1.0, BetaMin=0.0, seed=5)
+        X, T = m.data()
+    print(f"  {n} smooth faces of {3 * 9 * 64 * 2}")
+    assert n == len(Xo) and n > 100
+    assert np.allclose(X, Xo, rtol=2e-6, atol=2e-6 * np.abs(Xo).max())
+    assert np.allclose(T, To, rtol=1e-5, atol=1e-5 * np.abs(To).max())
+
+
+def test_all_faces_gated_out_equals_plain_fp32():
+    """No face passes the gate (BetaMin huge): every face takes the exact
+    HLLD flux and the gated stage is the plain FP32 stage."""
+    print("case gated-out", flush=True)
+    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
+    U0 = smooth_mhd_soa(grid, seed=62)
+    scheme = dict(flux="hlld", limiter="mc", beta=1.5, precision="f32")
+    out = []
+    for sur in (False, True):
+        with Session(grid, **scheme) as s:
+            if sur:
+                s.set_surrogate(None, "rusanov", BetaMin=1e30)
+            s.upload_soa(U0)
+            s.init_timestep()
+            s.advance(3)
+            out.append(s.download_soa())
+            if sur:
+                nf, nn = s.surrogate_counts()
+    assert nn == 0 and nf == 7 * 3 * 9 * 64 * 8
+    d = np.abs(out[1] - out[0]).reshape(8, -1).max(1)
+    assert np.all(d <= 2e-6 * np.abs(out[0]).reshape(8, -1).max(1))
+
+
+def test_network_in_the_loop_is_conservative():
+    print("case network conservation", flush=True)
+    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
+    U0 = smooth_mhd_soa(grid, seed=63, amp=0.1)
+    with Session(grid, flux="hlld", limiter="mc", beta=1.5, cfl=0.05,
+                 precision="f32") as s, Mlp(mo.init_params(seed=2)) as m:
+        s.set_surrogate(m, "network", BetaMin=0.0)
+        s.upload_soa(U0)
+        s.init_timestep()
+        t0 = s.integrate_domain()
+        s.advance(1)
+        t1 = s.integrate_domain()
+        nf, nn = s.surrogate_counts()
+    print(f"  network faces {nn} of {nf}")
+    assert nn > 0
+    scale = np.abs(t0)
+    scale[1:7] = np.maximum(scale[1:7], abs(t0[0]))
+    assert (np.abs(t1 - t0) / scale)[:7].max() < 1e-6
+
+
+def test_trained_network_beats_rusanov_on_smooth_faces():
+    """Train on harvested faces of an FP64 run, then compare three FP32 runs
+    against the FP64 HLLD run: the network on the smooth faces is closer to
+    it than the Rusanov flux on the same faces."""
+    print("case network vs rusanov", flush=True)
+    grid = BlockGrid(nRoot=(4, 4, 4), nCell=(8, 8, 8), order="morton")
+    U0 = smooth_mhd_soa(grid, seed=64, amp=0.2)
+    scheme = dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8)
+    nstep = 6
+    with Session(grid, **scheme) as s, Mlp(mo.init_params(seed=4)) as m:
+        s.upload_soa(U0)
+        s.init_timestep()
+        for k in range(nstep):
+            m.harvest(s, 1.0, BetaMin=0.0, seed=100 + k)
+            s.advance(1)
+        ref = s.download_soa()
+        loss, err = m.train(60, batch=8192, lr=2e-3, hold=0.1, seed=1)
+        print(f"  samples {m.data()[0].shape[0]}, held-out rel L2 {err[0]:.4f}")
+        res = {}
+        for mode in ("network", "rusanov"):
+            with Session(grid, precision="f32", **scheme) as s2:
+                s2.set_surrogate(m, mode, BetaMin=0.0)
+                s2.upload_soa(U0)
+                s2.init_timestep()
+                s2.advance(nstep)
+                U = s2.download_soa()
+                nf, nn = s2.surrogate_counts()
+            res[mode] = (np.abs(U - ref).reshape(8, -1).sum(1) /
+                         np.abs(ref).reshape(8, -1).sum(1))
+            print(f"  {mode}: rel L1 max {res[mode].max():.2e}, "
+                  f"network faces {nn}/{nf}")
+    assert err[0] < 0.05
+    assert res["network"].max() < 0.5 * res["rusanov"].max()
