@@ -68,6 +68,7 @@ ABI_FUNCTIONS = {
     "bats_init": (C.c_int, [C.POINTER(BatsConfig), C.POINTER(_hp)]),
     "bats_finalize": (C.c_int, [_hp]),
     "bats_set_grid": (C.c_int, [_hp, _ip, _dp]),
+    "bats_set_amr_grid": (C.c_int, [_hp, _ip, _dp, C.c_int, _ip]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
     "bats_download_state": (C.c_int, [_hp, _dp]),
     "bats_upload_state_soa": (C.c_int, [_hp, _dp]),
@@ -197,8 +198,15 @@ class Session:
         self.h = h
         nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
         cs = np.ascontiguousarray(grid.CellSize_DB, dtype=np.float64)
-        self._check(self.lib.bats_set_grid(self.h, _ptr(nei, C.c_int),
-                                           _ptr(cs)), "bats_set_grid")
+        slabs = getattr(grid, "iSlab_IS", None)
+        if slabs is not None and len(slabs):
+            sl = np.ascontiguousarray(slabs, dtype=np.int32)
+            self._check(self.lib.bats_set_amr_grid(
+                self.h, _ptr(nei, C.c_int), _ptr(cs), len(sl),
+                _ptr(sl, C.c_int)), "bats_set_amr_grid")
+        else:
+            self._check(self.lib.bats_set_grid(self.h, _ptr(nei, C.c_int),
+                                               _ptr(cs)), "bats_set_grid")
 
     # ------------------------------------------------------------------
     def _check(self, code, where):
@@ -411,14 +419,14 @@ class Mlp:
         h = _hp()
         st = self.lib.bats_mlp_create(C.byref(h))
         if st != 0:
-            raise StopMpi("bats_mlp_create: " + _strerror(st))
+            raise StopMpi(st, "bats_mlp_create")
         self.h = h
         if params is not None:
             self.set_params(params)
 
     def _check(self, st, what):
         if st != 0:
-            raise StopMpi("%s: %s" % (what, _strerror(st)))
+            raise StopMpi(st, what)
 
     def close(self):
         if self.h:

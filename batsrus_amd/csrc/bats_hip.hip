@@ -36,7 +36,7 @@ struct BlockArgs {
   double *dtmaxCB, *dtmaxB;
   int *err;
   int nBlock, iBlock;
-  long vs;
+  long vs, nCellTot;
   Phys ph;
   double beta;
   int lim, iFlux;
@@ -70,7 +70,7 @@ __global__ void k_block_face_value(BlockArgs a) {
     if (dir == 0) ii = fi - 2 + s;
     else if (dir == 1) jj = fj - 2 + s;
     else kk = fk - 2 + s;
-    load_state(a.Usnap, a.vs, cell_off<NB>(a.nei, a.iBlock, ii, jj, kk), q[s]);
+    load_state(a.Usnap, a.vs, cell_off<NB>(a.nei, a.iBlock, ii, jj, kk, a.nCellTot), q[s]);
     to_primitive(q[s]);
   }
   double Lv[NVAR], Rv[NVAR];
@@ -341,6 +341,7 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   h->NB = c.nI;
   h->nCellBlk = (long)c.nI * c.nJ * c.nK;
   h->nCell = h->nCellBlk * c.nBlock;
+  h->vs = h->nCell;
   h->cur = 0;
   h->stateBuf = 0;
   h->iStageBlock = 1;
@@ -396,7 +397,8 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->partial, h->LX, h->RX,
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
-                  h->sur.counters};
+                  h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
+                  h->corr};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &ev : h->evStart) (void)hipEventDestroy(ev);
@@ -482,13 +484,17 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
 // host SoA (double) <-> device state buffer, converting to the state type
 static int put_soa(bats_handle *h, const double *U) {
   HIPCHK(hipStreamSynchronize(h->stream));
-  const size_t n = (size_t)NVAR * h->nCell;
-  if (h->elemSize == 8) {
-    HIPCHK(hipMemcpy(h->U[h->cur], U, sizeof(double) * n,
-                     hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> f(U, U + n);
-    HIPCHK(hipMemcpy(h->U[h->cur], f.data(), sizeof(float) * n,
+  const size_t n = (size_t)h->nCell, es = h->elemSize;
+  char *dst = static_cast<char *>(h->U[h->cur]);
+  std::vector<float> f(h->elemSize == 4 ? n : 0);
+  for (int v = 0; v < NVAR; v++) {
+    const double *src = U + v * n;
+    const void *p = src;
+    if (es == 4) {
+      for (size_t t = 0; t < n; t++) f[t] = (float)src[t];
+      p = f.data();
+    }
+    HIPCHK(hipMemcpy(dst + (size_t)v * h->vs * es, p, es * n,
                      hipMemcpyHostToDevice));
   }
   h->stateBuf = h->cur;
@@ -496,15 +502,19 @@ static int put_soa(bats_handle *h, const double *U) {
 }
 static int get_soa(bats_handle *h, double *U) {
   HIPCHK(hipStreamSynchronize(h->stream));
-  const size_t n = (size_t)NVAR * h->nCell;
-  if (h->elemSize == 8) {
-    HIPCHK(hipMemcpy(U, h->U[h->stateBuf], sizeof(double) * n,
-                     hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> f(n);
-    HIPCHK(hipMemcpy(f.data(), h->U[h->stateBuf], sizeof(float) * n,
-                     hipMemcpyDeviceToHost));
-    for (size_t t = 0; t < n; t++) U[t] = f[t];
+  const size_t n = (size_t)h->nCell, es = h->elemSize;
+  const char *src = static_cast<const char *>(h->U[h->stateBuf]);
+  std::vector<float> f(h->elemSize == 4 ? n : 0);
+  for (int v = 0; v < NVAR; v++) {
+    double *dst = U + v * n;
+    if (es == 8) {
+      HIPCHK(hipMemcpy(dst, src + (size_t)v * h->vs * es, es * n,
+                       hipMemcpyDeviceToHost));
+    } else {
+      HIPCHK(hipMemcpy(f.data(), src + (size_t)v * h->vs * es, es * n,
+                       hipMemcpyDeviceToHost));
+      for (size_t t = 0; t < n; t++) dst[t] = f[t];
+    }
   }
   return BATS_OK;
 }
@@ -537,9 +547,9 @@ static int ensure_block_buffers(bats_handle *h) {
   if (h->Usnap) return BATS_OK;
   const int NB = h->NB;
   const size_t nF = (size_t)(NB + 1) * NB * NB;
-  HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->nCell));
+  HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->vs));
   HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
-                        (size_t)h->elemSize * NVAR * h->nCell,
+                        (size_t)h->elemSize * NVAR * h->vs,
                         hipMemcpyDeviceToDevice, h->stream));
   double **fs[] = {&h->LX, &h->RX, &h->LY, &h->RY, &h->LZ, &h->RZ};
   for (double **p : fs) {
@@ -578,7 +588,8 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.err = h->err;
   a.nBlock = c.nBlock;
   a.iBlock = iBlock;
-  a.vs = h->nCell;
+  a.vs = h->vs;
+  a.nCellTot = h->nCell;
   a.ph = make_phys(c);
   a.beta = c.LimiterBeta;
   a.lim = limiter_code(c);
@@ -599,7 +610,7 @@ extern "C" int bats_exchange_messages(bats_handle *h) {
   int st = ensure_block_buffers(h);
   if (st) return st;
   HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
-                        (size_t)h->elemSize * NVAR * h->nCell,
+                        (size_t)h->elemSize * NVAR * h->vs,
                         hipMemcpyDeviceToDevice, h->stream));
   return BATS_OK;
 }
@@ -612,6 +623,8 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
+  // the per-block routines work on same-level grids
+  if (h->nSlab > 0) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only
   if (h->elemSize != 8) return BATS_EINVAL;
   return ensure_block_buffers(h);
@@ -768,7 +781,10 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   a.dtmaxCB = h->dtmaxCB;
   a.err = h->err;
   a.nBlock = c.nBlock;
-  a.vs = h->nCell;
+  a.vs = h->vs;
+  a.nCellTot = h->nCell;
+  a.slabKind = h->slabKind;
+  a.corr = nullptr;
   a.ph = make_phys(c);
   a.beta = c.LimiterBeta;
   a.lim = limiter_code(c);
@@ -799,6 +815,33 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   const bool powell = c.UseDivbSource != 0;
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
+  if (h->nSlab > 0) {
+    // resolution changes: ghost slabs of the stage input, then (plain
+    // path) the fine face fluxes that replace the coarse ones
+    int st = amr_fill_ghosts(h, a.Uin);
+    if (st) return st;
+    if (h->sur.mode == BATS_SUR_OFF && h->nFine > 0) {
+      AmrArgs m;
+      m.U = a.Uin; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
+      m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
+      m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = a.ph;
+      m.beta = a.beta; m.lim = a.lim; m.corr = h->corr; m.err = h->err;
+#define BATS_FPICK(NBv, FL)                                                  \
+  (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
+       : launch_fineflux_##NBv##_##FL##_f64(m, h->stream))
+      if (NB == 8)
+        e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_FPICK(8, rusanov)
+          : c.iFlux == BATS_FLUX_LINDE   ? BATS_FPICK(8, linde)
+                                         : BATS_FPICK(8, hlld);
+      else
+        e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_FPICK(16, rusanov)
+          : c.iFlux == BATS_FLUX_LINDE   ? BATS_FPICK(16, linde)
+                                         : BATS_FPICK(16, hlld);
+#undef BATS_FPICK
+      HIPCHK(e);
+      a.corr = h->corr;
+    }
+  }
   if (h->sur.mode != BATS_SUR_OFF) {
     // gated face fluxes with the network on smooth faces
     const int st = sur_stage(h, a);
@@ -943,6 +986,105 @@ extern "C" int bats_set_persistent_grid(bats_handle *h, int nWorkgroup) {
   return BATS_OK;
 }
 
+int amr_fill_ghosts(bats_handle *h, const void *U) {
+  AmrArgs m;
+  m.U = U; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
+  m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
+  m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);
+  m.beta = h->cfg.LimiterBeta; m.lim = limiter_code(h->cfg);
+  m.corr = h->corr; m.err = h->err;
+  const long n = 2L * h->NB * h->NB * h->nSlab;
+  const dim3 g((unsigned)((n + 255) / 256)), t(256);
+  if (h->elemSize == 8)
+    hipLaunchKernelGGL(k_amr_ghosts<double>, g, t, 0, h->stream, m, h->NB);
+  else
+    hipLaunchKernelGGL(k_amr_ghosts<float>, g, t, 0, h->stream, m, h->NB);
+  HIPCHK(hipGetLastError());
+  return BATS_OK;
+}
+
+extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
+                                 const double *CellSize_DB, int nSlab,
+                                 const int *iSlab_IS) {
+  if (!h || !iNeighbor_EB || !CellSize_DB || nSlab < 0 ||
+      (nSlab > 0 && !iSlab_IS))
+    return BATS_EINVAL;
+  const int nB = h->cfg.nBlock, NB = h->NB, H = NB / 2;
+  if (nSlab > 0 && h->cfg.UseDivbSource) return BATS_EINVAL;
+  for (int t = 0; t < 3 * nB; t++)
+    if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
+  for (int b = 0; b < nB; b++)
+    for (int s = 0; s < 6; s++) {
+      const int n = iNeighbor_EB[6 * b + s];
+      if (n >= nB || (n <= -2 && -2 - n >= nSlab)) return BATS_EINVAL;
+      if (n >= 0 && h->hRunOfBlock[n] != h->hRunOfBlock[b]) return BATS_EINVAL;
+      if (n <= -2) {
+        const int *sl = iSlab_IS + 8 * (-2 - n);
+        if (sl[0] != b || sl[1] != s) return BATS_EINVAL;
+      }
+    }
+  std::vector<int> kind(nSlab), fine;
+  for (int t = 0; t < nSlab; t++) {
+    const int *sl = iSlab_IS + 8 * t;
+    if (sl[0] < 0 || sl[0] >= nB || sl[1] < 0 || sl[1] > 5) return BATS_EINVAL;
+    if (iNeighbor_EB[6 * sl[0] + sl[1]] != -2 - t) return BATS_EINVAL;
+    if (sl[2] == 0) {
+      if (sl[3] < 0 || sl[3] >= nB || (sl[4] != 0 && sl[4] != H) ||
+          (sl[5] != 0 && sl[5] != H) || sl[7] < 0 || sl[7] >= nSlab ||
+          iSlab_IS[8 * sl[7] + 2] != 1)
+        return BATS_EINVAL;
+      fine.push_back(t);
+    } else if (sl[2] == 1) {
+      for (int q = 0; q < 4; q++)
+        if (sl[3 + q] < 0 || sl[3 + q] >= nB) return BATS_EINVAL;
+    } else {
+      return BATS_EINVAL;
+    }
+    kind[t] = sl[2];
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  // state buffers with the slab region after the interior cells
+  const long vs = h->nCell + 2L * NB * NB * nSlab;
+  for (int u = 0; u < 2; u++) {
+    HIPCHK(hipFree(h->U[u]));
+    h->U[u] = nullptr;
+    HIPCHK(hipMalloc(&h->U[u], (size_t)h->elemSize * NVAR * vs));
+    HIPCHK(hipMemset(h->U[u], 0, (size_t)h->elemSize * NVAR * vs));
+  }
+  if (h->Usnap) { HIPCHK(hipFree(h->Usnap)); h->Usnap = nullptr; }
+  void *old[] = {h->slabs, h->slabKind, h->fineSlabs, h->corr};
+  for (void *p : old)
+    if (p) HIPCHK(hipFree(p));
+  h->slabs = h->slabKind = h->fineSlabs = nullptr;
+  h->corr = nullptr;
+  h->vs = vs;
+  h->nSlab = nSlab;
+  h->nFine = (int)fine.size();
+  h->hSlabs.assign(iSlab_IS, iSlab_IS + 8 * nSlab);
+  if (nSlab > 0) {
+    HIPCHK(hipMalloc(&h->slabs, sizeof(int) * 8 * nSlab));
+    HIPCHK(hipMalloc(&h->slabKind, sizeof(int) * nSlab));
+    HIPCHK(hipMemcpy(h->slabs, iSlab_IS, sizeof(int) * 8 * nSlab,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->slabKind, kind.data(), sizeof(int) * nSlab,
+                     hipMemcpyHostToDevice));
+    const size_t nc = (size_t)nSlab * NB * NB * 4 * 9;
+    HIPCHK(hipMalloc(&h->corr, (size_t)h->elemSize * nc));
+    HIPCHK(hipMemset(h->corr, 0, (size_t)h->elemSize * nc));
+    if (!fine.empty()) {
+      HIPCHK(hipMalloc(&h->fineSlabs, sizeof(int) * fine.size()));
+      HIPCHK(hipMemcpy(h->fineSlabs, fine.data(), sizeof(int) * fine.size(),
+                       hipMemcpyHostToDevice));
+    }
+  }
+  h->hNei.assign(iNeighbor_EB, iNeighbor_EB + 6 * nB);
+  HIPCHK(hipMemcpy(h->nei, iNeighbor_EB, sizeof(int) * 6 * nB,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->cellsize, CellSize_DB, sizeof(double) * 3 * nB,
+                   hipMemcpyHostToDevice));
+  return BATS_OK;
+}
+
 extern "C" int bats_synchronize(bats_handle *h) {
   if (!h) return BATS_EINVAL;
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -953,15 +1095,19 @@ extern "C" int bats_synchronize(bats_handle *h) {
 
 extern "C" int bats_check_nan(bats_handle *h) {
   if (!h) return BATS_EINVAL;
-  const long n = NVAR * h->nCell;
-  if (h->elemSize == 8)
-    hipLaunchKernelGGL(k_check_nan<double>, dim3(1024), dim3(256), 0,
-                       h->stream, static_cast<const double *>(h->U[h->stateBuf]),
-                       n, h->err);
-  else
-    hipLaunchKernelGGL(k_check_nan<float>, dim3(1024), dim3(256), 0,
-                       h->stream, static_cast<const float *>(h->U[h->stateBuf]),
-                       n, h->err);
+  const long n = h->nCell;  // interior cells of each variable
+  for (int v = 0; v < NVAR; v++) {
+    if (h->elemSize == 8)
+      hipLaunchKernelGGL(k_check_nan<double>, dim3(1024), dim3(256), 0,
+                         h->stream,
+                         static_cast<const double *>(h->U[h->stateBuf]) + v * h->vs,
+                         n, h->err);
+    else
+      hipLaunchKernelGGL(k_check_nan<float>, dim3(1024), dim3(256), 0,
+                         h->stream,
+                         static_cast<const float *>(h->U[h->stateBuf]) + v * h->vs,
+                         n, h->err);
+  }
   HIPCHK(hipGetLastError());
   return bats_synchronize(h);
 }
@@ -975,16 +1121,16 @@ extern "C" int bats_integrate_domain(bats_handle *h, double *Sum_V) {
   const dim3 g(c.nBlock), t(256);
   if (h->NB == 8 && h->elemSize == 8)
     hipLaunchKernelGGL((k_integrate<8, double>), g, t, 0, h->stream, Ud,
-                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+                       h->vs, h->cellsize, make_phys(c), c.nBlock, h->partial);
   else if (h->NB == 8)
     hipLaunchKernelGGL((k_integrate<8, float>), g, t, 0, h->stream, Uf,
-                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+                       h->vs, h->cellsize, make_phys(c), c.nBlock, h->partial);
   else if (h->elemSize == 8)
     hipLaunchKernelGGL((k_integrate<16, double>), g, t, 0, h->stream, Ud,
-                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+                       h->vs, h->cellsize, make_phys(c), c.nBlock, h->partial);
   else
     hipLaunchKernelGGL((k_integrate<16, float>), g, t, 0, h->stream, Uf,
-                       h->nCell, h->cellsize, make_phys(c), c.nBlock, h->partial);
+                       h->vs, h->cellsize, make_phys(c), c.nBlock, h->partial);
   HIPCHK(hipGetLastError());
   std::vector<double> p((size_t)NVAR * c.nBlock);
   HIPCHK(hipMemcpyAsync(p.data(), h->partial, sizeof(double) * p.size(),

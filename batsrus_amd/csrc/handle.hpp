@@ -31,7 +31,8 @@ struct bats_handle {
   int NB;
   long nCellBlk, nCell;
   hipStream_t stream;
-  void *U[2];    // state buffers of elemSize-byte reals
+  void *U[2];    // state buffers of elemSize-byte reals, vs per variable
+  long vs;       // variable stride: nCell interior cells + ghost slabs
   int elemSize;  // 8: FP64 state, 4: FP32 state
   int cur;       // buffer holding the state at the start of a step
   int stateBuf;  // buffer holding the current State_VGB (differs from cur
@@ -59,6 +60,11 @@ struct bats_handle {
   int nTimed;
   double lastStageMs;
   int lastStageLaunches;
+  // two-level grids: resolution-change slabs (iNeighbor_EB = -2 - slab)
+  int nSlab, nFine;
+  std::vector<int> hSlabs;  // 8 ints per slab
+  int *slabs, *slabKind, *fineSlabs;
+  void *corr;               // summed fine fluxes of the finer sides
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };
@@ -99,3 +105,5 @@ inline int err_from_bits(int bits) {
 
 // one stage of the update through the surrogate kernels (surrogate.hip)
 int sur_stage(bats_handle *h, const StageArgs &a);
+// fill the ghost slabs of the state buffer U (two-level grids)
+int amr_fill_ghosts(bats_handle *h, const void *U);

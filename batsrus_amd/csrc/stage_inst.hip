@@ -26,4 +26,11 @@ hipError_t BATS_NAME(BATS_NB, BATS_FLUX, BATS_REAL)(const StageArgs &a,
                              BATS_PASTE(BATS_REAL_, BATS_REAL)>(a, powell,
                                                                 stream);
 }
+#define BATS_FNAME_(nb, fl, ty) launch_fineflux_##nb##_##fl##_##ty
+#define BATS_FNAME(nb, fl, ty) BATS_FNAME_(nb, fl, ty)
+hipError_t BATS_FNAME(BATS_NB, BATS_FLUX, BATS_REAL)(const AmrArgs &a,
+                                                     hipStream_t stream) {
+  return launch_amr_fineflux<BATS_NB, BATS_PASTE(BATS_FLUX_, BATS_FLUX),
+                             BATS_PASTE(BATS_REAL_, BATS_REAL)>(a, stream);
+}
 } // namespace bats

@@ -58,37 +58,60 @@ struct StageArgs {
   double DtFactor;     // (Cfl*iStage)/nStage or Cfl
   int iStage, nStage, halfStep, dbTrick, doDt, roundTripOld, noUpdate;
   int advanceTime;     // add Dt/nStage to tSimulation when done
+  long nCellTot;       // interior cells; ghost slabs follow in each plane
+  const int *slabKind; // resolution-change slabs: 0 coarser, 1 finer side
+  const void *corr;    // summed fine fluxes of the finer sides (type T),
+                       // [slab][q][p][sub 0..3][9 slots]; null: no slabs
   int nGroup;          // workgroup-sized groups of blocks (work items)
   int nWorkgroup;      // launched workgroups (< nGroup: persistent grid)
 };
 
 // offset (within one variable plane) of cell (i,j,k) of block b; one index
-// may lie in the 2-cell halo: it is resolved to the neighbour block or, at
-// a float outer boundary, to the block's own boundary cell
-// (src/ModCellBoundary.f90:518-559).
+// may lie in the 2-cell halo: it is resolved to the neighbour block, at a
+// float outer boundary to the block's own boundary cell
+// (src/ModCellBoundary.f90:518-559), and at a resolution change to the
+// side's ghost slab stored after the nCellTot interior cells (two layers of
+// NB x NB cells per slab, in-plane order (p, q) = (j,k) / (i,k) / (i,j) on
+// x / y / z sides).  Only one index is ever outside the block: the stencils
+// are dimension-split.
 template <int NB>
 __device__ __forceinline__ long cell_off(const int *__restrict__ nei, int b,
-                                         int i, int j, int k) {
+                                         int i, int j, int k, long nCellTot) {
+  auto slab = [&](int n, int layer, int p, int q) {
+    return nCellTot + (((long)(-2 - n) * 2 + layer) * NB + q) * NB + p;
+  };
   if (i < 0) {
     const int n = nei[6 * b + 0];
-    if (n >= 0) { b = n; i += NB; } else { i = 0; }
+    if (n >= 0) { b = n; i += NB; }
+    else if (n == -1) { i = 0; }
+    else return slab(n, -1 - i, j, k);
   } else if (i >= NB) {
     const int n = nei[6 * b + 1];
-    if (n >= 0) { b = n; i -= NB; } else { i = NB - 1; }
+    if (n >= 0) { b = n; i -= NB; }
+    else if (n == -1) { i = NB - 1; }
+    else return slab(n, i - NB, j, k);
   }
   if (j < 0) {
     const int n = nei[6 * b + 2];
-    if (n >= 0) { b = n; j += NB; } else { j = 0; }
+    if (n >= 0) { b = n; j += NB; }
+    else if (n == -1) { j = 0; }
+    else return slab(n, -1 - j, i, k);
   } else if (j >= NB) {
     const int n = nei[6 * b + 3];
-    if (n >= 0) { b = n; j -= NB; } else { j = NB - 1; }
+    if (n >= 0) { b = n; j -= NB; }
+    else if (n == -1) { j = NB - 1; }
+    else return slab(n, j - NB, i, k);
   }
   if (k < 0) {
     const int n = nei[6 * b + 4];
-    if (n >= 0) { b = n; k += NB; } else { k = 0; }
+    if (n >= 0) { b = n; k += NB; }
+    else if (n == -1) { k = 0; }
+    else return slab(n, -1 - k, i, j);
   } else if (k >= NB) {
     const int n = nei[6 * b + 5];
-    if (n >= 0) { b = n; k -= NB; } else { k = NB - 1; }
+    if (n >= 0) { b = n; k -= NB; }
+    else if (n == -1) { k = NB - 1; }
+    else return slab(n, k - NB, i, j);
   }
   return (((long)b * NB + k) * NB + j) * NB + i;
 }
@@ -169,6 +192,17 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   auto sPi = [&](int l, int v, int jj, int ii) {
     return ((l * NVAR + v) * NR + jj) * NS + ii;
   };
+  // a face on a side with finer neighbours takes the sum of the four fine
+  // face fluxes (apply_cons_flux, src/ModConserveFlux.f90:260-441)
+  auto corrected = [&](int blk, int side, int p, int q, T *F) {
+    const int n = a.nei[6 * blk + side];
+    if (n > -2 || a.slabKind[-2 - n] != 1) return;
+    const T *c = static_cast<const T *>(a.corr) +
+                 ((((long)(-2 - n)) * NB + q) * NB + p) * 4 * NFS;
+#pragma unroll
+    for (int v = 0; v < NFS; v++)
+      F[v] = ((c[v] + c[NFS + v]) + c[2 * NFS + v]) + c[3 * NFS + v];
+  };
   auto sFi = [&](int l, int dir, int v, int q) {
     return ((l * 2 + dir) * NFS + v) * NFX + q;
   };
@@ -215,7 +249,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     double dtmin = 1.7976931348623157e308;
 
     auto load_prim = [&](int kk, T *q) {
-      const long off = cell_off<NB>(a.nei, bb, i, j, kk);
+      const long off = cell_off<NB>(a.nei, bb, i, j, kk, a.nCellTot);
       load_state(U, vs, off, q);
       to_primitive(q);
     };
@@ -251,6 +285,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         dLk[v] = dl;
       }
       zface(FzLo);
+      if (a.corr) corrected(bb, 4, i, j, FzLo);
 #pragma unroll
       for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
     }
@@ -260,6 +295,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       if (act) {
         load_prim(k + 2, Q3);
         zface(FzHi);
+        if (a.corr && k == NB - 1) corrected(bb, 5, i, j, FzHi);
         // own primitive of plane k
 #pragma unroll
         for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
@@ -267,7 +303,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       // this thread's x/y halo cell of plane k
       if (hAct) {
         T q[NVAR];
-        load_state(U, vs, cell_off<NB>(a.nei, hb, hii, hjj, k), q);
+        load_state(U, vs, cell_off<NB>(a.nei, hb, hii, hjj, k, a.nCellTot), q);
         to_primitive(q);
 #pragma unroll
         for (int v = 0; v < NVAR; v++)
@@ -307,6 +343,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
                                          c3, F, vdt, unA);
         }
         F[PP] = vdt;
+        if (a.corr && (fp == 0 || fp == NB))
+          corrected(bf, 2 * dir + (fp == NB), row, k, F);
 #pragma unroll
         for (int v = 0; v < NFS; v++) sF[sFi(lbf, dir, v, q)] = F[v];
       }
@@ -415,6 +453,159 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     }
 }
 
+// ---------------------------------------------------------------------
+// two-level grids: ghost slabs at resolution changes and the fine face
+// fluxes that replace the coarse ones
+// ---------------------------------------------------------------------
+struct AmrArgs {
+  const void *U;        // stage input (type T), slabs filled in place
+  long vs, nCellTot;
+  const int *nei;
+  const double *cellsize;
+  const int *slabs;     // 8 ints per slab (block, side, kind, n0..n3, target)
+  int nSlab;
+  const int *fineSlabs; // the kind-0 slabs
+  int nFine;
+  Phys ph;
+  double beta;
+  int lim;
+  void *corr;           // type T, [slab][q][p][sub][9]
+  int *err;
+};
+
+__device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
+                                         int &y, int &z) {
+  if (d == 0) { x = n; y = p; z = q; }
+  else if (d == 1) { x = p; y = n; z = q; }
+  else { x = p; y = q; z = n; }
+}
+
+template <typename T>
+__device__ __forceinline__ T minmod2(T a, T b) {
+  if (a * b <= T(0)) return T(0);
+  return fabsr(a) < fabsr(b) ? a : b;
+}
+
+// Ghost slabs of the resolution-change sides: the coarse ghost is the mean
+// of the 2x2x2 fine cells it covers; the fine ghost is the coarse cell plus
+// its slopes (minmod of the coarse block's one-sided differences, the one
+// available difference at the block edge) times the +-1/4 offsets of the
+// fine centre.  One thread per ghost cell.
+template <typename T>
+__global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per = 2L * NB * NB;
+  if (t >= per * a.nSlab) return;
+  const int slab = (int)(t / per), r = (int)(t % per);
+  const int L = r / (NB * NB), q = (r / NB) % NB, p = r % NB;
+  const int *sl = a.slabs + 8 * slab;
+  const int side = sl[1], kind = sl[2], d = side / 2, hi = side % 2, H = NB / 2;
+  const T *U = static_cast<const T *>(a.U);
+  T val[NVAR];
+  auto cell = [&](int blk, int x, int y, int z) {
+    return (((long)blk * NB + z) * NB + y) * NB + x;
+  };
+  if (kind == 1) {
+    const int F = sl[3 + (p / H) + 2 * (q / H)];
+    for (int v = 0; v < NVAR; v++) val[v] = T(0);
+    for (int ai = 0; ai < 2; ai++)
+      for (int bq = 0; bq < 2; bq++)
+        for (int cp = 0; cp < 2; cp++) {
+          const int nf = hi ? 2 * L + ai : NB - 1 - 2 * L - ai;
+          int x, y, z;
+          side_xyz(d, nf, 2 * (p % H) + cp, 2 * (q % H) + bq, x, y, z);
+          const long o = cell(F, x, y, z);
+          for (int v = 0; v < NVAR; v++) val[v] += U[v * a.vs + o];
+        }
+    for (int v = 0; v < NVAR; v++) val[v] = T(0.125) * val[v];
+  } else {
+    const int C = sl[3];
+    int cc[3];
+    side_xyz(d, hi ? 0 : NB - 1, sl[4] + p / 2, sl[5] + q / 2, cc[0], cc[1],
+             cc[2]);
+    const T dn = hi ? (L == 0 ? T(-0.25) : T(0.25)) : (L == 0 ? T(0.25) : T(-0.25));
+    const T dp = (p % 2 == 0) ? T(-0.25) : T(0.25);
+    const T dq = (q % 2 == 0) ? T(-0.25) : T(0.25);
+    T delta[3];
+    if (d == 0) { delta[0] = dn; delta[1] = dp; delta[2] = dq; }
+    else if (d == 1) { delta[0] = dp; delta[1] = dn; delta[2] = dq; }
+    else { delta[0] = dp; delta[1] = dq; delta[2] = dn; }
+    const long o0 = cell(C, cc[0], cc[1], cc[2]);
+    for (int v = 0; v < NVAR; v++) val[v] = U[v * a.vs + o0];
+    for (int dd = 0; dd < 3; dd++) {
+      int lo[3] = {cc[0], cc[1], cc[2]}, up[3] = {cc[0], cc[1], cc[2]};
+      lo[dd] -= 1;
+      up[dd] += 1;
+      const bool hasLo = cc[dd] > 0, hasUp = cc[dd] < NB - 1;
+      const long ol = hasLo ? cell(C, lo[0], lo[1], lo[2]) : o0;
+      const long ou = hasUp ? cell(C, up[0], up[1], up[2]) : o0;
+      for (int v = 0; v < NVAR; v++) {
+        const T u0 = U[v * a.vs + o0];
+        T g = T(0);
+        if (hasLo && hasUp) g = minmod2(u0 - U[v * a.vs + ol], U[v * a.vs + ou] - u0);
+        else if (hasLo) g = u0 - U[v * a.vs + ol];
+        else if (hasUp) g = U[v * a.vs + ou] - u0;
+        val[v] = val[v] + g * delta[dd];
+      }
+    }
+  }
+  T *W = const_cast<T *>(U);
+  const long o = a.nCellTot + ((long)slab * 2 + L) * NB * NB + (long)q * NB + p;
+  for (int v = 0; v < NVAR; v++) W[v * a.vs + o] = val[v];
+}
+
+// save_cons_flux (src/ModConserveFlux.f90:79-187): the face fluxes of the
+// fine blocks at their coarser sides, into the coarse slab's store at
+// (coarse face, sub-face) -- one thread per fine face
+template <int NB, int FLUX, int LIM, typename T>
+__global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nFine * NB * NB) return;
+  const int slab = a.fineSlabs[t / (NB * NB)];
+  const int q = (t / NB) % NB, p = t % NB;
+  const int *sl = a.slabs + 8 * slab;
+  const int b = sl[0], side = sl[1], d = side / 2, hi = side % 2;
+  const T *U = static_cast<const T *>(a.U);
+  const PhysT<T> ph = phys_as<T>(a.ph);
+  T c[4][NVAR];
+  for (int s4 = 0; s4 < 4; s4++) {
+    // face between normal cells f-1 and f; f = 0 (low side) or NB (high)
+    const int nrm = (hi ? NB : 0) + s4 - 2;
+    int x, y, z;
+    side_xyz(d, nrm, p, q, x, y, z);
+    load_state(U, a.vs, cell_off<NB>(a.nei, b, x, y, z, a.nCellTot), c[s4]);
+    to_primitive(c[s4]);
+  }
+  const double *cz = a.cellsize + 3 * b;
+  const T area = T(d == 0 ? cz[1] * cz[2] : d == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
+  T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+  face_states<LIM>(T(a.beta), c[0], c[1], c[2], c[3], L, R);
+  int err;
+  if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  else if (d == 1) err = numerical_flux<1, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  else err = numerical_flux<2, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  F[PP] = vdt;
+  const int pc = sl[4] + p / 2, qc = sl[5] + q / 2, sub = (p % 2) + 2 * (q % 2);
+  T *dst = static_cast<T *>(a.corr) +
+           ((((long)sl[7]) * NB + qc) * NB + pc) * 4 * 9 + sub * 9;
+  for (int v = 0; v < 9; v++) dst[v] = F[v];
+  if (err) atomicOr(a.err, err);
+}
+
+template <int NB, int FLUX, typename T>
+inline hipError_t launch_amr_fineflux(const AmrArgs &a, hipStream_t stream) {
+  const dim3 grid((a.nFine * NB * NB + 255) / 256), block(256);
+  switch (a.lim) {
+  case LIM_MINMOD: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_MINMOD, T>), grid, block, 0, stream, a); break;
+  case LIM_MC: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_MC, T>), grid, block, 0, stream, a); break;
+  case LIM_MC3: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_MC3, T>), grid, block, 0, stream, a); break;
+  case LIM_BETA: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_BETA, T>), grid, block, 0, stream, a); break;
+  case LIM_NO: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_NO, T>), grid, block, 0, stream, a); break;
+  default: hipLaunchKernelGGL((k_amr_fineflux<NB, FLUX, LIM_FIRST, T>), grid, block, 0, stream, a); break;
+  }
+  return hipGetLastError();
+}
+
 // Launch k_stage for one block size and flux; the limiter and the 8-wave
 // switch are dispatched here.
 template <int NB, int FLUX, typename T>
@@ -450,6 +641,18 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
   BATS_DECLARE_LAUNCH(NB, rusanov, TY)                                      \
   BATS_DECLARE_LAUNCH(NB, linde, TY)                                        \
   BATS_DECLARE_LAUNCH(NB, hlld, TY)
+#define BATS_DECLARE_FINE(NB, FL, TY)                                       \
+  hipError_t launch_fineflux_##NB##_##FL##_##TY(const AmrArgs &, hipStream_t);
+#define BATS_DECLARE_FINE_FLUX(NB, TY)                                      \
+  BATS_DECLARE_FINE(NB, rusanov, TY)                                        \
+  BATS_DECLARE_FINE(NB, linde, TY)                                          \
+  BATS_DECLARE_FINE(NB, hlld, TY)
+BATS_DECLARE_FINE_FLUX(8, f64)
+BATS_DECLARE_FINE_FLUX(16, f64)
+BATS_DECLARE_FINE_FLUX(8, f32)
+BATS_DECLARE_FINE_FLUX(16, f32)
+#undef BATS_DECLARE_FINE_FLUX
+#undef BATS_DECLARE_FINE
 BATS_DECLARE_LAUNCH_FLUX(8, f64)
 BATS_DECLARE_LAUNCH_FLUX(16, f64)
 BATS_DECLARE_LAUNCH_FLUX(8, f32)

@@ -150,7 +150,7 @@ struct HarvestArgs {
   const double *U;
   const int *nei;
   const double *cellsize;
-  long vs;
+  long vs, nCellTot;
   int nBlock;
   Phys ph;
   double beta, betaMin, frac;
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
         for (int s = 0; s < 4; s++) {
           int ii = fi, jj = fj, kk = fk;
           if (dir == 0) ii += s - 2; else if (dir == 1) jj += s - 2; else kk += s - 2;
-          load_state(a.U, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk), q[s]);
+          load_state(a.U, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk, a.nCellTot), q[s]);
           to_primitive(q[s]);
         }
         double L[NVAR], R[NVAR];
@@ -258,7 +258,7 @@ struct SurArgs {
   double *dtmaxB, *dtmaxCB;
   int *err;
   int nBlock;
-  long vs;
+  long vs, nCellTot;
   Phys ph;
   double beta, betaMin, DtFactor;
   int lim, mode, iStage, nStage, halfStep, dbTrick, doDt, roundTripOld,
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
       for (int s = 0; s < 4; s++) {
         int ii = fi, jj = fj, kk = fk;
         if (dir == 0) ii += s - 2; else if (dir == 1) jj += s - 2; else kk += s - 2;
-        load_state(a.Uin, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk), q[s]);
+        load_state(a.Uin, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk, a.nCellTot), q[s]);
         to_primitive(q[s]);
       }
       float L[NVAR], R[NVAR];
@@ -462,6 +462,40 @@ __global__ void __launch_bounds__(256) k_sur_update(SurArgs a) {
     }
 }
 
+// two-level grids: the coarse faces of the finer sides take the sum of the
+// four fine faces' fluxes as the fine blocks use them (network or exact),
+// summed in sub-face order -- one thread per coarse face
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_sur_amr_correct(float *face, long nTot, int nBlock, const int *slabs,
+                  int nSlab) {
+  using G = FaceGeom<NB>;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nSlab * NB * NB) return;
+  const int slab = (int)(t / (NB * NB)), q = (int)((t / NB) % NB),
+            p = (int)(t % NB);
+  const int *sl = slabs + 8 * slab;
+  if (sl[2] != 1) return;
+  const int b = sl[0], side = sl[1], d = side / 2, H = NB / 2;
+  const int F = sl[3 + (p / H) + 2 * (q / H)];
+  auto gface = [&](int blk, int fn, int fp, int fq) {
+    long r;
+    if (d == 0) r = ((long)fq * NB + fp) * (NB + 1) + fn;
+    else if (d == 1) r = ((long)fq * (NB + 1) + fn) * NB + fp;
+    else r = ((long)fn * NB + fq) * NB + fp;
+    return (long)d * nBlock * G::NF + (long)blk * G::NF + r;
+  };
+  const long gc = gface(b, (side % 2) ? NB : 0, p, q);
+  const int fn = (side % 2) ? 0 : NB;  // the fine block's face on its other side
+  long gf[4];
+  for (int sub = 0; sub < 4; sub++)
+    gf[sub] = gface(F, fn, 2 * (p % H) + (sub & 1), 2 * (q % H) + (sub >> 1));
+  for (int v = 0; v < NSLOT; v++) {
+    const float *Fv = face + (long)v * nTot;
+    face[(long)v * nTot + gc] = ((Fv[gf[0]] + Fv[gf[1]]) + Fv[gf[2]]) + Fv[gf[3]];
+  }
+}
+
 __global__ void k_count_faces(int *counters, long nFace) {
   // rows of this stage -> running totals (single thread)
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -515,7 +549,8 @@ extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
   a.U = static_cast<const double *>(h->U[h->stateBuf]);
   a.nei = h->nei;
   a.cellsize = h->cellsize;
-  a.vs = h->nCell;
+  a.vs = h->vs;
+  a.nCellTot = h->nCell;
   a.nBlock = nB;
   a.ph = make_phys(c);
   a.beta = c.LimiterBeta;
@@ -618,6 +653,7 @@ int sur_stage(bats_handle *h, const StageArgs &sa) {
   a.err = sa.err;
   a.nBlock = sa.nBlock;
   a.vs = sa.vs;
+  a.nCellTot = sa.nCellTot;
   a.ph = sa.ph;
   a.beta = sa.beta;
   a.betaMin = s.betaMin;
@@ -652,6 +688,16 @@ int sur_stage(bats_handle *h, const StageArgs &sa) {
     else
       hipLaunchKernelGGL(k_sur_infer<16>, dim3(1024), dim3(256), 0, h->stream,
                          wts.frag, wts.w, a);
+  }
+  if (h->nSlab > 0) {
+    const long n = (long)h->nSlab * h->NB * h->NB;
+    const dim3 g((unsigned)((n + 255) / 256)), t(256);
+    if (h->NB == 8)
+      hipLaunchKernelGGL(k_sur_amr_correct<8>, g, t, 0, h->stream, s.face,
+                         nTot, c.nBlock, h->slabs, h->nSlab);
+    else
+      hipLaunchKernelGGL(k_sur_amr_correct<16>, g, t, 0, h->stream, s.face,
+                         nTot, c.nBlock, h->slabs, h->nSlab);
   }
   if (h->NB == 8)
     hipLaunchKernelGGL(k_sur_update<8>, dim3(c.nBlock), dim3(256), 0, h->stream, a);

@@ -41,6 +41,7 @@ module ModHipBridge
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
        bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
+       bats_set_amr_grid, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
        bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
        bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
@@ -238,6 +239,14 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: nWorkgroup
      end function bats_set_persistent_grid
+     integer(c_int) function bats_set_amr_grid(Handle, iNeighbor_EB, &
+          CellSize_DB, nSlab, iSlab_IS) bind(C, name="bats_set_amr_grid")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), intent(in):: iNeighbor_EB(*), iSlab_IS(*)
+       real(c_double), intent(in):: CellSize_DB(*)
+       integer(c_int), value:: nSlab
+     end function bats_set_amr_grid
      ! learned face-flux surrogate (no counterpart in the reference)
      integer(c_int) function bats_mlp_create(Model) &
           bind(C, name="bats_mlp_create")

@@ -101,6 +101,25 @@ int bats_finalize(bats_handle *h);
 int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
                   const double *CellSize_DB);
 
+/* Two-level block-adaptive grid: as bats_set_grid, plus resolution-change
+ * sides.  iNeighbor_EB[6*b + side] = -2 - t names slab t of iSlab_IS
+ * (8 ints per slab: block, side, kind, n0, n1, n2, n3, target).  Kind 0
+ * (the neighbour is coarser): n0 = the coarse block, (n1, n2) = offset of
+ * this block's face within the coarse face in coarse cells (0 or nI/2),
+ * target = the coarse block's kind-1 slab.  Kind 1 (finer neighbours):
+ * n0..n3 = the fine blocks covering the face's quadrants, in-plane
+ * coordinates (j,k) / (i,k) / (i,j) on x / y / z sides, quadrant index
+ * (p >= nI/2) + 2*(q >= nI/2).  Before every stage the ghost slabs are
+ * filled (mean of the 2x2x2 fine cells for coarse ghosts, limited linear
+ * prolongation for fine ghosts) and the coarse faces of kind-1 sides take
+ * the sum of the four fine face fluxes, as save_cons_flux / apply_cons_flux
+ * do (src/ModConserveFlux.f90:79-441, src/ModAdvanceExplicit.f90:95-128).
+ * Reallocates the state: call before uploading it.  Not with the 8-wave
+ * source; the per-block routines need a same-level grid. */
+int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
+                      const double *CellSize_DB, int nSlab,
+                      const int *iSlab_IS);
+
 /* Host <-> device state in the reference layout State_VGB(nVar, MinI:MaxI,
  * MinJ:MaxJ, MinK:MaxK, nBlock), Fortran order with the variable index
  * fastest (src/ModAdvance.f90:296).  Upload ignores ghost cells; download

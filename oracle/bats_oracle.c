@@ -1003,6 +1003,7 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
     for (side = 0; side < 2 * c->nDim; side++) {
       int nb = iNeighbor_EB[b * 6 + side];
       const double *Nbr = nb >= 0 ? State + (long)nb * len : NULL;
+      if (nb <= -2) continue; /* resolution change: ob_exchange_amr */
       int dim = side / 2, hi = side % 2;
       int nG = c->nG;
       for (g = 1; g <= nG; g++) {
@@ -1031,6 +1032,171 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
       }
     }
   }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Two-level ghost cells at resolution changes (the role of BATL's         */
+/* message_pass_cell restriction/prolongation, called by exchange_messages, */
+/* src/ModMessagePass.f90:157-217).  BATL is absent, so these formulas are */
+/* this project's own (parity unpinned against BATL):                      */
+/*   coarse ghost = mean of the 2x2x2 fine cells it covers (summed in       */
+/*     normal, q, p order);                                                */
+/*   fine ghost = coarse cell + sum over x, y, z of g_d * delta_d, where    */
+/*     delta_d = +-1/4 is the fine centre's offset in coarse cells and g_d  */
+/*     the coarse cell's slope from the coarse block's own cells: minmod   */
+/*     of the two one-sided differences, the one available difference at   */
+/*     the block edge.                                                     */
+/* ---------------------------------------------------------------------- */
+static inline double mm2(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  return fabs(a) < fabs(b) ? a : b;
+}
+
+/* interior 0-based (x, y, z) of a side's (normal, p, q) coordinates */
+static inline void side_xyz(int d, int n, int p, int q, int *x, int *y,
+                            int *z) {
+  if (d == 0) { *x = n; *y = p; *z = q; }
+  else if (d == 1) { *x = p; *y = n; *z = q; }
+  else { *x = p; *y = q; *z = n; }
+}
+
+void ob_exchange_amr(const ob_config *c, int nBlock, int nSlab,
+                     const int *iSlab_IS, double *State) {
+  ob_shape s = shape_of(c);
+  const long len = block_len(&s);
+  const int NB = c->nI, H = NB / 2;
+  int t;
+  (void)nBlock;
+#pragma omp parallel for schedule(static)
+  for (t = 0; t < nSlab; t++) {
+    const int *sl = iSlab_IS + 8 * t;
+    const int b = sl[0], side = sl[1], kind = sl[2], d = side / 2,
+              hi = side % 2;
+    double *Blk = State + (long)b * len;
+    int L, p, q, v;
+    for (L = 0; L < 2; L++)
+      for (q = 0; q < NB; q++)
+        for (p = 0; p < NB; p++) {
+          int x, y, z;
+          double val[NV];
+          side_xyz(d, hi ? NB + L : -1 - L, p, q, &x, &y, &z);
+          if (kind == 1) { /* restriction from the fine neighbour */
+            const int F = sl[3 + (p / H) + 2 * (q / H)];
+            const double *Fb = State + (long)F * len;
+            int a, bq, cp;
+            for (v = 0; v < NV; v++) val[v] = 0.0;
+            for (a = 0; a < 2; a++)
+              for (bq = 0; bq < 2; bq++)
+                for (cp = 0; cp < 2; cp++) {
+                  const int nf = hi ? 2 * L + a : NB - 1 - 2 * L - a;
+                  const int pf = 2 * (p % H) + cp, qf = 2 * (q % H) + bq;
+                  int fx, fy, fz;
+                  side_xyz(d, nf, pf, qf, &fx, &fy, &fz);
+                  const double *src = Fb + cidx(&s, fx + 1, fy + 1, fz + 1);
+                  for (v = 0; v < NV; v++) val[v] += src[v];
+                }
+            for (v = 0; v < NV; v++) val[v] = 0.125 * val[v];
+          } else { /* prolongation from the coarse neighbour */
+            const int C = sl[3], p0 = sl[4], q0 = sl[5];
+            const double *Cb = State + (long)C * len;
+            int cx, cy, cz, dd;
+            side_xyz(d, hi ? 0 : NB - 1, p0 + p / 2, q0 + q / 2, &cx, &cy, &cz);
+            const int cc[3] = {cx, cy, cz};
+            double delta[3];
+            const double dn = hi ? (L == 0 ? -0.25 : 0.25)
+                                 : (L == 0 ? 0.25 : -0.25);
+            const double dp = (p % 2 == 0) ? -0.25 : 0.25;
+            const double dq = (q % 2 == 0) ? -0.25 : 0.25;
+            if (d == 0) { delta[0] = dn; delta[1] = dp; delta[2] = dq; }
+            else if (d == 1) { delta[0] = dp; delta[1] = dn; delta[2] = dq; }
+            else { delta[0] = dp; delta[1] = dq; delta[2] = dn; }
+            const double *U0 = Cb + cidx(&s, cx + 1, cy + 1, cz + 1);
+            for (v = 0; v < NV; v++) val[v] = U0[v];
+            for (dd = 0; dd < 3; dd++) {
+              int lo[3] = {cc[0], cc[1], cc[2]}, up[3] = {cc[0], cc[1], cc[2]};
+              lo[dd] -= 1;
+              up[dd] += 1;
+              const int hasLo = cc[dd] > 0, hasUp = cc[dd] < NB - 1;
+              const double *Ul = hasLo ? Cb + cidx(&s, lo[0] + 1, lo[1] + 1, lo[2] + 1) : NULL;
+              const double *Uu = hasUp ? Cb + cidx(&s, up[0] + 1, up[1] + 1, up[2] + 1) : NULL;
+              for (v = 0; v < NV; v++) {
+                double g = 0.0;
+                if (hasLo && hasUp) g = mm2(U0[v] - Ul[v], Uu[v] - U0[v]);
+                else if (hasLo) g = U0[v] - Ul[v];
+                else if (hasUp) g = Uu[v] - U0[v];
+                val[v] = val[v] + g * delta[dd];
+              }
+            }
+          }
+          memcpy(Blk + cidx(&s, x + 1, y + 1, z + 1), val, NV * 8);
+        }
+  }
+}
+
+/* flux slot of the boundary face (p, q) of side `side` in the block's face
+ * flux arrays (src/ModConserveFlux.f90:129-187 lFaceFrom / lFaceTo) */
+static double *side_face_flux(const ob_config *c, int side, int p, int q,
+                              double *FX, double *FY, double *FZ) {
+  const int NB = c->nI, d = side / 2, f = (side % 2) ? NB : 0;
+  long idx;
+  if (d == 0) { idx = ((long)q * NB + p) * (NB + 1) + f; return FX + idx * OB_NFACEVALUE; }
+  if (d == 1) { idx = ((long)q * (NB + 1) + f) * NB + p; return FY + idx * OB_NFACEVALUE; }
+  idx = ((long)f * NB + q) * NB + p;
+  return FZ + idx * OB_NFACEVALUE;
+}
+
+/* save_cons_flux (src/ModConserveFlux.f90:79-187) for one fine block: its
+ * face fluxes at coarser sides go to the coarse slab's corrected-flux store
+ * Corr[(slab*NB*NB + qc*NB + pc)*4 + sub][0..Vdt] */
+static void save_cons_flux(const ob_session *s, int b, double *FX, double *FY,
+                           double *FZ) {
+  const ob_config *c = &s->c;
+  const int NB = c->nI;
+  int side, p, q, v;
+  for (side = 0; side < 6; side++) {
+    const int n = s->iNeighbor_EB[6 * b + side];
+    if (n > -2) continue;
+    const int *sl = s->iSlab_IS + 8 * (-2 - n);
+    if (sl[2] != 0) continue;
+    for (q = 0; q < NB; q++)
+      for (p = 0; p < NB; p++) {
+        const double *F = side_face_flux(c, side, p, q, FX, FY, FZ);
+        const int pc = sl[4] + p / 2, qc = sl[5] + q / 2;
+        const int sub = (p % 2) + 2 * (q % 2);
+        double *dst = s->Corr + (((long)sl[7] * NB + qc) * NB + pc) * 4 * OB_NFACEVALUE +
+                      sub * OB_NFACEVALUE;
+        for (v = 0; v <= OB_VDT; v++) dst[v] = F[v];
+      }
+  }
+}
+
+/* apply_cons_flux (src/ModConserveFlux.f90:260-441) for one coarse block:
+ * the face fluxes at finer sides become the sums of the four fine ones */
+static void apply_cons_flux(const ob_session *s, int b, double *FX, double *FY,
+                            double *FZ) {
+  const ob_config *c = &s->c;
+  const int NB = c->nI;
+  int side, p, q, v;
+  for (side = 0; side < 6; side++) {
+    const int n = s->iNeighbor_EB[6 * b + side];
+    if (n > -2) continue;
+    const int t = -2 - n;
+    if (s->iSlab_IS[8 * t + 2] != 1) continue;
+    for (q = 0; q < NB; q++)
+      for (p = 0; p < NB; p++) {
+        double *F = side_face_flux(c, side, p, q, FX, FY, FZ);
+        const double *src = s->Corr + (((long)t * NB + q) * NB + p) * 4 * OB_NFACEVALUE;
+        for (v = 0; v <= OB_VDT; v++)
+          F[v] = ((src[v] + src[OB_NFACEVALUE + v]) + src[2 * OB_NFACEVALUE + v]) +
+                 src[3 * OB_NFACEVALUE + v];
+      }
+  }
+}
+
+static void exchange_all(ob_session *s) {
+  ob_exchange_messages(&s->c, s->nBlock, s->iNeighbor_EB, s->State);
+  if (s->nSlab > 0)
+    ob_exchange_amr(&s->c, s->nBlock, s->nSlab, s->iSlab_IS, s->State);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1083,6 +1249,15 @@ static void scratch_free(ob_scratch *w) {
   free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
 }
 
+static int has_kind(const ob_session *s, int b, int kind) {
+  int side;
+  for (side = 0; side < 6; side++) {
+    const int n = s->iNeighbor_EB[6 * b + side];
+    if (n <= -2 && s->iSlab_IS[8 * (-2 - n) + 2] == kind) return 1;
+  }
+  return 0;
+}
+
 static int run_blocks(ob_session *s, int iStage, int DoUpdate,
                       int DoCalcTimestep) {
   const ob_config *c = &s->c;
@@ -1090,6 +1265,38 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
   const long len = block_len(&sh);
   const long nCell = (long)c->nI * c->nJ * c->nK;
   int err = OB_OK;
+  if (s->nSlab > 0) {
+    /* fine blocks next to coarser ones: face fluxes for the correction
+     * (src/ModAdvanceExplicit.f90:95-128) */
+#ifdef _OPENMP
+    int nt0 = s->nThread > 0 ? s->nThread : omp_get_max_threads();
+#pragma omp parallel num_threads(nt0)
+#endif
+    {
+      ob_scratch w;
+      int b, e;
+      scratch_alloc(c, &w);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (b = 0; b < s->nBlock; b++) {
+        if (!has_kind(s, b, 0)) continue;
+        const double *cs = s->CellSize_DB + 3L * b;
+        ob_calc_face_value(c, s->State + (long)b * len, w.LX, w.RX, w.LY,
+                           w.RY, w.LZ, w.RZ);
+        e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
+                              w.FX, w.FY, w.FZ);
+        if (e) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+          if (!err) err = e;
+        }
+        save_cons_flux(s, b, w.FX, w.FY, w.FZ);
+      }
+      scratch_free(&w);
+    }
+  }
 #ifdef _OPENMP
   int nt = s->nThread > 0 ? s->nThread : omp_get_max_threads();
 #pragma omp parallel num_threads(nt)
@@ -1108,6 +1315,7 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       ob_calc_face_value(c, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
       e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
                             w.FY, w.FZ);
+      if (s->nSlab > 0) apply_cons_flux(s, b, w.FX, w.FY, w.FZ);
       if (e) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -1137,7 +1345,7 @@ int ob_advance_stage(ob_session *s, int iStage, int DoCalcTimestep) {
   int err = run_blocks(s, iStage, 1, DoCalcTimestep && iStage == c->nStage);
   s->tSimulation = s->tSimulation + s->Dt / c->nStage;        /* :274 */
   if (iStage < c->nStage)                                      /* :309-312 */
-    ob_exchange_messages(c, s->nBlock, s->iNeighbor_EB, s->State);
+    exchange_all(s);
   return err;
 }
 
@@ -1155,11 +1363,11 @@ int ob_advance_step(ob_session *s, double TimeSimulationLimit) {
   s->nStep += 1;                                    /* ModBatsrusMethods :566 */
   ob_set_global_timestep(s, TimeSimulationLimit);   /* :572 */
   err = ob_advance_explicit(s, 1);                  /* :606 */
-  ob_exchange_messages(&s->c, s->nBlock, s->iNeighbor_EB, s->State); /* :653 */
+  exchange_all(s);                                  /* :653 */
   return err;
 }
 
 int ob_init_timestep(ob_session *s) {
-  ob_exchange_messages(&s->c, s->nBlock, s->iNeighbor_EB, s->State);
+  exchange_all(s);
   return run_blocks(s, s->c.nStage, 0, 1);
 }

@@ -128,6 +128,15 @@ void ob_calc_timestep(const ob_config *c, const double *CellSize,
 void ob_exchange_messages(const ob_config *c, int nBlock,
                           const int *iNeighbor_EB, double *State);
 
+/* Ghost cells at resolution changes of a two-level grid: restriction to
+ * coarse ghosts, limited linear prolongation to fine ghosts (this
+ * project's formulas; BATL's are absent).  iSlab_IS: 8 ints per slab
+ * [block, side, kind, n0, n1, n2, n3, target] (kind 0: coarser neighbour
+ * n0 at in-plane offset (n1, n2) coarse cells; kind 1: finer neighbours
+ * n0..n3 by quadrant). */
+void ob_exchange_amr(const ob_config *c, int nBlock, int nSlab,
+                     const int *iSlab_IS, double *State);
+
 /* Whole-grid session state for the time loop */
 typedef struct {
   ob_config c;
@@ -142,6 +151,12 @@ typedef struct {
   double tSimulation;
   int nStep;
   int nThread;               /* OpenMP threads (<=0: default) */
+  /* two-level grids: resolution-change slabs (iNeighbor_EB = -2 - slab)
+   * and the corrected-flux store of the coarse slabs
+   * (nSlab*NB*NB*4*nFaceValue); nSlab = 0 for a same-level grid */
+  int nSlab;
+  const int *iSlab_IS;
+  double *Corr;
 } ob_session;
 
 /* set_global_timestep (src/ModTimeStepControl.f90:306-524) */

@@ -39,7 +39,9 @@ class ObSession(C.Structure):
                 ("DtMax_CB", C.POINTER(C.c_double)),
                 ("DtMax_B", C.POINTER(C.c_double)),
                 ("Dt", C.c_double), ("tSimulation", C.c_double),
-                ("nStep", C.c_int), ("nThread", C.c_int)]
+                ("nStep", C.c_int), ("nThread", C.c_int),
+                ("nSlab", C.c_int), ("iSlab_IS", C.POINTER(C.c_int)),
+                ("Corr", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -80,6 +82,8 @@ def lib():
         L.ob_advance_step.restype = C.c_int
         L.ob_init_timestep.argtypes = [C.POINTER(ObSession)]
         L.ob_init_timestep.restype = C.c_int
+        L.ob_exchange_amr.argtypes = [C.POINTER(ObConfig), C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), dp]
         L.ob_pressure_to_energy_cell.argtypes = [C.POINTER(ObConfig), dp]
         L.ob_energy_to_pressure_cell.argtypes = [C.POINTER(ObConfig), dp]
         _LIB = L
@@ -245,12 +249,25 @@ class Session:
         s.tSimulation = 0.0
         s.nStep = 0
         s.nThread = nThread
+        slabs = getattr(grid, "iSlab_IS", None)
+        self.nSlab = 0 if slabs is None else len(slabs)
+        if self.nSlab:
+            self.slabs = np.ascontiguousarray(slabs, dtype=np.int32)
+            self.Corr = np.zeros(self.nSlab * cfg.nI * cfg.nJ * 4 * 14)
+            s.nSlab = self.nSlab
+            s.iSlab_IS = self.slabs.ctypes.data_as(C.POINTER(C.c_int))
+            s.Corr = _dp(self.Corr)
         self.s = s
 
     def exchange(self):
         lib().ob_exchange_messages(C.byref(self.cfg), self.grid.nBlock,
                                    self.nei.ctypes.data_as(C.POINTER(C.c_int)),
                                    _dp(self.State))
+        if self.nSlab:
+            lib().ob_exchange_amr(C.byref(self.cfg), self.grid.nBlock,
+                                  self.nSlab,
+                                  self.slabs.ctypes.data_as(C.POINTER(C.c_int)),
+                                  _dp(self.State))
 
     def init_timestep(self):
         return lib().ob_init_timestep(C.byref(self.s))
