@@ -55,7 +55,10 @@ class BatsConfig(C.Structure):
                 ("pMin", C.c_double), ("UseDbTrick", C.c_int),
                 ("UseDtFixed", C.c_int), ("DtFixed", C.c_double),
                 ("cTiny", C.c_double), ("iDevice", C.c_int),
-                ("UseDivbSource", C.c_int), ("StatePrecision", C.c_int)]
+                ("UseDivbSource", C.c_int), ("StatePrecision", C.c_int),
+                ("UseAdaptiveLowOrder", C.c_int), ("nLowOrder", C.c_int),
+                ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
+                ("VelCrit", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -69,6 +72,8 @@ ABI_FUNCTIONS = {
     "bats_finalize": (C.c_int, [_hp]),
     "bats_set_grid": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_amr_grid": (C.c_int, [_hp, _ip, _dp, C.c_int, _ip]),
+    "bats_set_low_order_face": (C.c_int, [_hp]),
+    "bats_get_low_order_crit": (C.c_int, [_hp, C.c_int, _dp, _dp, _dp]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
     "bats_download_state": (C.c_int, [_hp, _dp]),
     "bats_upload_state_soa": (C.c_int, [_hp, _dp]),
@@ -100,12 +105,16 @@ ABI_FUNCTIONS = {
     "bats_mlp_forward": (C.c_int, [_hp, _fp, _fp, C.c_int]),
     "bats_mlp_loss_grad": (C.c_int, [_hp, _fp, _fp, C.c_int, _dp, _fp]),
     "bats_mlp_set_data": (C.c_int, [_hp, _fp, _fp, C.c_long]),
-    "bats_mlp_get_data": (C.c_int, [_hp, C.POINTER(C.c_long), _fp, _fp]),
+    "bats_mlp_get_data": (C.c_int, [_hp, C.POINTER(C.c_long), _fp, _fp,
+                                    _fp]),
     "bats_mlp_harvest": (C.c_int, [_hp, _hp, C.c_double, C.c_double,
-                                   C.c_ulonglong, C.POINTER(C.c_long)]),
+                                   C.c_double, C.c_ulonglong,
+                                   C.POINTER(C.c_long)]),
     "bats_mlp_train": (C.c_int, [_hp, C.c_int, C.c_int, C.c_double,
-                                 C.c_double, C.c_ulonglong, _dp, _dp]),
-    "bats_set_surrogate": (C.c_int, [_hp, _hp, C.c_int, C.c_double]),
+                                 C.c_double, C.c_int, C.c_ulonglong, _dp,
+                                 _dp]),
+    "bats_set_surrogate": (C.c_int, [_hp, _hp, C.c_int, C.c_double,
+                                     C.c_double]),
     "bats_surrogate_counts": (C.c_int, [_hp, C.POINTER(C.c_long),
                                         C.POINTER(C.c_long)]),
     "bats_check_nan": (C.c_int, [_hp]),
@@ -151,7 +160,8 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        nStage=2, UseHalfStep=True, cfl=0.8, gamma=5.0 / 3.0,
                        pMin=-1.0, UseDbTrick=False, UseDtFixed=False,
                        DtFixed=0.0, cTiny=1e-6, UseDivbSource=False,
-                       precision="f64")
+                       precision="f64", UseAdaptiveLowOrder=False,
+                       nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0)
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):
@@ -181,6 +191,10 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     if prec not in ("f64", "f32"):
         raise ValueError("precision must be 'f64' or 'f32'")
     c.StatePrecision = 8 if prec == "f64" else 4
+    c.UseAdaptiveLowOrder = int(s["UseAdaptiveLowOrder"])
+    c.nLowOrder = int(s["nLowOrder"])
+    c.pCritLow, c.pCritHigh = s["pCritLow"], s["pCritHigh"]
+    c.VelCrit = s["VelCrit"]
     return c
 
 
@@ -270,6 +284,19 @@ class Session:
         self._check(self.lib.bats_calc_face_value(
             self.h, iBlock, int(DoResChangeOnly), int(DoMonotoneRestrict)),
             "calc_face_value")
+
+    def set_low_order_face(self):
+        """set_low_order_face on the last exchanged state (all blocks)."""
+        self._check(self.lib.bats_set_low_order_face(self.h),
+                    "set_low_order_face")
+
+    def get_low_order_crit(self, iBlock):
+        """LowOrderCrit_[XYZ]B of block iBlock (1-based) in C order:
+        (nK, nJ, nI+1), (nK, nJ+1, nI), (nK+1, nJ, nI)."""
+        out = [np.zeros(s) for s in self.face_shapes()]
+        self._check(self.lib.bats_get_low_order_crit(
+            self.h, iBlock, *[_ptr(a) for a in out]), "get_low_order_crit")
+        return out
 
     def calc_face_flux(self, DoResChangeOnly, iBlock):
         self._check(self.lib.bats_calc_face_flux(
@@ -377,14 +404,14 @@ class Session:
         self._check(self.lib.bats_set_persistent_grid(self.h, nWorkgroup),
                     "set_persistent_grid")
 
-    def set_surrogate(self, model, mode="network", BetaMin=0.0):
+    def set_surrogate(self, model, mode="network", BetaMin=0.0, JumpTol=0.05):
         """Gated face fluxes on an FP32 handle: mode 'network' (the trained
         model on smooth faces), 'rusanov' (the same gate, Rusanov flux) or
         'off'."""
         code = {"off": 0, "network": 1, "rusanov": 2}[mode]
         self._check(self.lib.bats_set_surrogate(
             self.h, model.h if model is not None else None, code,
-            float(BetaMin)), "set_surrogate")
+            float(BetaMin), float(JumpTol)), "set_surrogate")
         self._surrogate = model
 
     def surrogate_counts(self):
@@ -479,30 +506,38 @@ class Mlp:
                     "mlp_set_data")
 
     def data(self):
+        """Samples held on the device: inputs, targets, baseline fluxes."""
         n = C.c_long()
-        self._check(self.lib.bats_mlp_get_data(self.h, C.byref(n), None, None),
-                    "mlp_get_data")
+        self._check(self.lib.bats_mlp_get_data(self.h, C.byref(n), None, None,
+                                               None), "mlp_get_data")
         X = np.zeros((n.value, 16), np.float32)
         T = np.zeros((n.value, 8), np.float32)
+        B = np.zeros((n.value, 8), np.float32)
         self._check(self.lib.bats_mlp_get_data(self.h, C.byref(n),
                                                _ptr(X, C.c_float),
-                                               _ptr(T, C.c_float)),
+                                               _ptr(T, C.c_float),
+                                               _ptr(B, C.c_float)),
                     "mlp_get_data")
-        return X, T
+        return X, T, B
 
-    def harvest(self, session, fraction, BetaMin=0.0, seed=1):
+    def harvest(self, session, fraction, BetaMin=0.0, JumpTol=0.05, seed=1):
         n = C.c_long()
         self._check(self.lib.bats_mlp_harvest(self.h, session.h,
                                               float(fraction), float(BetaMin),
-                                              int(seed), C.byref(n)),
+                                              float(JumpTol), int(seed),
+                                              C.byref(n)),
                     "mlp_harvest")
         return n.value
 
-    def train(self, nEpoch, batch=65536, lr=1e-3, hold=0.1, seed=1):
+    def train(self, nEpoch, batch=65536, lr=1e-3, hold=0.1, seed=1,
+              standardize=False):
+        """Returns the last minibatch loss and the held-out relative L2
+        errors (18: flux overall + 8 outputs, correction overall + 8)."""
         loss = C.c_double()
-        err = np.zeros(9)
+        err = np.zeros(18)
         self._check(self.lib.bats_mlp_train(self.h, int(nEpoch), int(batch),
-                                            float(lr), float(hold), int(seed),
+                                            float(lr), float(hold),
+                                            int(standardize), int(seed),
                                             C.byref(loss), _ptr(err)),
                     "mlp_train")
         return loss.value, err

@@ -22,6 +22,71 @@ namespace {
 // ---------------------------------------------------------------------
 // per-block kernels (reference-shaped routines for parity dumps)
 // ---------------------------------------------------------------------
+// ---------------------------------------------------------------------
+// set_phys_based_low_order_face (src/ModFaceValue.f90:2839-2911): one
+// thread per face, the weight from the four cells f-2 .. f+1 along the
+// face normal through low_order_face_criteria (:2913-2963), written in
+// crit_index order (thread t writes entry t)
+// ---------------------------------------------------------------------
+struct LowOrderArgs {
+  const void *U;          // state (type T) with its halos
+  long vs, nCellTot;
+  const int *nei;
+  int nBlock;
+  double Gamma, pCritLow, pCritHigh, VelCrit;
+  void *crit;             // type T
+};
+
+template <int NB, typename T>
+__global__ void __launch_bounds__(256) k_low_order_crit(LowOrderArgs a) {
+  constexpr int NFX = (NB + 1) * NB, NFD = NB * NFX;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)a.nBlock * 3 * NFD) return;
+  const int b = (int)(t / (3 * NFD)), r = (int)(t % (3 * NFD));
+  const int dir = r / NFD, s = r % NFD;
+  int i, j, k;  // face coordinates, the dir component is the face index
+  if (dir == 0) { k = s / NFX; j = (s % NFX) / (NB + 1); i = s % (NB + 1); }
+  else if (dir == 1) { k = s / NFX; i = (s % NFX) / (NB + 1); j = s % (NB + 1); }
+  else { k = s / (NB * NB); j = (s / NB) % NB; i = s % NB; }
+  const T *U = static_cast<const T *>(a.U);
+  double pTotal[4], VelMax = 0, VelMin = 0, SoundMin = 0;
+  for (int l = 0; l < 4; l++) {
+    int ci = i, cj = j, ck = k;
+    if (dir == 0) ci = i - 2 + l;
+    else if (dir == 1) cj = j - 2 + l;
+    else ck = k - 2 + l;
+    const long off = cell_off<NB>(a.nei, b, ci, cj, ck, a.nCellTot);
+    const double Rho = double(U[RHO * a.vs + off]);
+    const double p = double(U[PP * a.vs + off]);
+    const double Vel = double(U[(MX + dir) * a.vs + off]) / Rho;
+    const double Sound = sqrt(p / Rho * a.Gamma);
+    pTotal[l] = p;
+    if (l == 0) {
+      VelMax = Vel; VelMin = Vel; SoundMin = Sound;
+    } else {
+      VelMax = fmax2(VelMax, Vel);
+      VelMin = fmin2(VelMin, Vel);
+      SoundMin = fmin2(SoundMin, Sound);
+    }
+    const double bx = double(U[BX * a.vs + off]), by = double(U[BY * a.vs + off]),
+                 bz = double(U[BZ * a.vs + off]);
+    pTotal[l] = pTotal[l] + 0.5 * (bx * bx + by * by + bz * bz);
+  }
+  double Crit = 0;
+  if (fmax2(0.0, (VelMax - VelMin) / SoundMin) > a.VelCrit) {
+    double pMax = pTotal[0], pMin = pTotal[0];
+    for (int l = 1; l < 4; l++) {
+      pMax = fmax2(pMax, pTotal[l]);
+      pMin = fmin2(pMin, pTotal[l]);
+    }
+    const double pRatio = pMax / pMin;
+    if (pRatio >= a.pCritLow) Crit = 1;
+    else if (pRatio < a.pCritHigh) Crit = 0;
+    else Crit = (pRatio - a.pCritHigh) / (a.pCritLow - a.pCritHigh);
+  }
+  static_cast<T *>(a.crit)[t] = T(Crit);
+}
+
 struct BlockArgs {
   const double *Usnap;   // halo source (exchange_messages snapshot)
   double *State;         // current state
@@ -33,6 +98,7 @@ struct BlockArgs {
   double *FX, *FY, *FZ;                 // Flux_V?I(14, ...)
   double *Src;                          // Source_VC(9, nI, nJ, nK)
   int useSrc;
+  const double *lowCrit;                // low order weights or null
   double *dtmaxCB, *dtmaxB;
   int *err;
   int nBlock, iBlock;
@@ -74,7 +140,12 @@ __global__ void k_block_face_value(BlockArgs a) {
     to_primitive(q[s]);
   }
   double Lv[NVAR], Rv[NVAR];
-  face_states_rt(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
+  if (a.lowCrit)
+    face_states_w_rt(a.lim, a.beta,
+                     1.0 - a.lowCrit[crit_index<NB>(a.iBlock, dir, fi, fj, fk)],
+                     q[0], q[1], q[2], q[3], Lv, Rv);
+  else
+    face_states_rt(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
   for (int v = 0; v < NVAR; v++) {
     L[fo * NVAR + v] = Lv[v];
     R[fo * NVAR + v] = Rv[v];
@@ -325,6 +396,13 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (c.UseDivbSource != 0 && c.UseDivbSource != 1) return BATS_EINVAL;
   if (c.StatePrecision != 0 && c.StatePrecision != 4 && c.StatePrecision != 8)
     return BATS_EINVAL;
+  // #ADAPTIVELOWORDER checks (src/ModFaceValue.f90:222-225)
+  if (c.UseAdaptiveLowOrder != 0 && c.UseAdaptiveLowOrder != 1)
+    return BATS_EINVAL;
+  if (c.UseAdaptiveLowOrder &&
+      ((c.nLowOrder != 1 && c.nLowOrder != 2) ||
+       c.pCritLow < c.pCritHigh - 1e-15 || !(c.VelCrit >= 0.0)))
+    return BATS_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.iDevice || c.iDevice < 0)
     return BATS_ENODEV;
@@ -361,6 +439,11 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (e == hipSuccess) e = hipMalloc(&h->runOfBlock, sizeof(int) * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * NVAR * c.nBlock);
+  h->lowCrit = nullptr;
+  const size_t critSz =
+      (size_t)h->elemSize * 3 * (c.nI + 1) * c.nI * c.nI * c.nBlock;
+  if (e == hipSuccess && c.UseAdaptiveLowOrder) e = hipMalloc(&h->lowCrit, critSz);
+  if (e == hipSuccess && h->lowCrit) e = hipMemset(h->lowCrit, 0, critSz);
   if (e != hipSuccess) {
     fprintf(stderr, "bats_hip: allocation failed: %s\n", hipGetErrorString(e));
     bats_finalize(h);
@@ -398,7 +481,7 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
                   h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
-                  h->corr};
+                  h->corr, h->lowCrit};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &ev : h->evStart) (void)hipEventDestroy(ev);
@@ -570,6 +653,40 @@ static int ensure_block_buffers(bats_handle *h) {
   return BATS_OK;
 }
 
+// the adaptive low order weights are set (second order scheme) and change
+// the face values (nLowOrder = 1), src/ModFaceValue.f90:2726-2735,2017-2019
+static bool low_order_set(const bats_config &c) {
+  return c.UseAdaptiveLowOrder && c.nOrder == 2;
+}
+static bool low_order_used(const bats_config &c) {
+  return low_order_set(c) && c.nLowOrder == 1;
+}
+
+static int launch_low_order_crit(bats_handle *h, const void *U) {
+  LowOrderArgs a;
+  a.U = U;
+  a.vs = h->vs;
+  a.nCellTot = h->nCell;
+  a.nei = h->nei;
+  a.nBlock = h->cfg.nBlock;
+  a.Gamma = h->cfg.Gamma;
+  a.pCritLow = h->cfg.pCritLow;
+  a.pCritHigh = h->cfg.pCritHigh;
+  a.VelCrit = h->cfg.VelCrit;
+  a.crit = h->lowCrit;
+  const int NB = h->NB;
+  const long n = 3L * (NB + 1) * NB * NB * h->cfg.nBlock;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (NB == 8) {
+    if (h->elemSize == 4) hipLaunchKernelGGL((k_low_order_crit<8, float>), grid, block, 0, h->stream, a);
+    else hipLaunchKernelGGL((k_low_order_crit<8, double>), grid, block, 0, h->stream, a);
+  } else {
+    if (h->elemSize == 4) hipLaunchKernelGGL((k_low_order_crit<16, float>), grid, block, 0, h->stream, a);
+    else hipLaunchKernelGGL((k_low_order_crit<16, double>), grid, block, 0, h->stream, a);
+  }
+  return hip_status(hipGetLastError());
+}
+
 static BlockArgs block_args(bats_handle *h, int iBlock) {
   BlockArgs a;
   const bats_config &c = h->cfg;
@@ -583,6 +700,8 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.FX = h->FX; a.FY = h->FY; a.FZ = h->FZ;
   a.Src = h->Src;
   a.useSrc = c.UseDivbSource;
+  a.lowCrit = low_order_used(c) ? static_cast<const double *>(h->lowCrit)
+                                : nullptr;
   a.dtmaxCB = h->dtmaxCB;
   a.dtmaxB = h->dtmaxB;
   a.err = h->err;
@@ -643,6 +762,49 @@ extern "C" int bats_calc_face_value(bats_handle *h, int iBlock,
   else
     hipLaunchKernelGGL(k_block_face_value<16>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
   return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_set_low_order_face(bats_handle *h) {
+  if (!h || !h->cfg.UseAdaptiveLowOrder || !h->lowCrit) return BATS_EINVAL;
+  int st = check_block(h, 1);
+  if (st) return st;
+  // first order scheme: every face is low order, no weights (:2727-2728)
+  if (!low_order_set(h->cfg)) return BATS_OK;
+  return launch_low_order_crit(h, h->Usnap);
+}
+
+extern "C" int bats_get_low_order_crit(bats_handle *h, int iBlock,
+                                       double *CritX, double *CritY,
+                                       double *CritZ) {
+  if (!h || !h->lowCrit || !CritX || !CritY || !CritZ || iBlock < 1 ||
+      iBlock > h->cfg.nBlock)
+    return BATS_EINVAL;
+  const int NB = h->NB;
+  const long n = 3L * (NB + 1) * NB * NB;
+  std::vector<double> w(n);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->elemSize == 8) {
+    HIPCHK(hipMemcpy(w.data(), static_cast<double *>(h->lowCrit) + n * (iBlock - 1),
+                     sizeof(double) * n, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpy(f.data(), static_cast<float *>(h->lowCrit) + n * (iBlock - 1),
+                     sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (long t = 0; t < n; t++) w[t] = f[t];
+  }
+  auto at = [&](int dir, int i, int j, int k) {
+    return NB == 8 ? w[crit_index<8>(0, dir, i, j, k)]
+                   : w[crit_index<16>(0, dir, i, j, k)];
+  };
+  // X(i,j,k) i = 0..NB; Y(i,j,k) j = 0..NB; Z(i,j,k) k = 0..NB
+  for (int a = 0; a < NB; a++)
+    for (int b = 0; b < NB; b++)
+      for (int f = 0; f <= NB; f++) {
+        CritX[((long)a * NB + b) * (NB + 1) + f] = at(0, f, b, a);
+        CritY[((long)a * (NB + 1) + f) * NB + b] = at(1, b, f, a);
+        CritZ[((long)f * NB + a) * NB + b] = at(2, b, a, f);
+      }
+  return BATS_OK;
 }
 
 extern "C" int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock) {
@@ -760,7 +922,7 @@ extern "C" int bats_set_global_timestep(bats_handle *h, double TimeSimulationLim
 static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
                         int advanceTime, bool timed) {
   const bats_config &c = h->cfg;
-  StageArgs a;
+  StageArgs a{};
   const int last = (iStage == c.nStage);
   if (c.nStage == 1) {
     a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
@@ -812,20 +974,35 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
     }
     HIPCHK(hipEventRecord(h->evStart[ev], h->stream));
   }
-  const bool powell = c.UseDivbSource != 0;
+  const bool lowUse = low_order_used(c);
+  // the surrogate's face pass has no low order weights
+  if (lowUse && h->sur.mode != BATS_SUR_OFF) return BATS_EINVAL;
+  a.lowCrit = lowUse ? h->lowCrit : nullptr;
+  const int opt = (c.UseDivbSource ? OPT_POWELL : 0) |
+                  (lowUse ? OPT_LOWORDER : 0);
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
+  // resolution changes: ghost slabs of the stage input
   if (h->nSlab > 0) {
-    // resolution changes: ghost slabs of the stage input, then (plain
-    // path) the fine face fluxes that replace the coarse ones
     int st = amr_fill_ghosts(h, a.Uin);
     if (st) return st;
+  }
+  // low order weights at the first stage of the step (and for the initial
+  // time step pass, which stands for a first step with Dt = 0),
+  // src/ModAdvanceExplicit.f90:87-91
+  if (low_order_set(c) && (iStage == 1 || noUpdate)) {
+    int st = launch_low_order_crit(h, a.Uin);
+    if (st) return st;
+  }
+  if (h->nSlab > 0) {
+    // (plain path) the fine face fluxes that replace the coarse ones
     if (h->sur.mode == BATS_SUR_OFF && h->nFine > 0) {
-      AmrArgs m;
+      AmrArgs m{};
       m.U = a.Uin; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
       m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
       m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = a.ph;
       m.beta = a.beta; m.lim = a.lim; m.corr = h->corr; m.err = h->err;
+      m.lowCrit = a.lowCrit;
 #define BATS_FPICK(NBv, FL)                                                  \
   (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
        : launch_fineflux_##NBv##_##FL##_f64(m, h->stream))
@@ -848,8 +1025,8 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
     if (st) return st;
   } else {
 #define BATS_PICK(NBv, FL)                                                   \
-  (f32 ? launch_stage_##NBv##_##FL##_f32(a, powell, h->stream)               \
-       : launch_stage_##NBv##_##FL##_f64(a, powell, h->stream))
+  (f32 ? launch_stage_##NBv##_##FL##_f32(a, opt, h->stream)                  \
+       : launch_stage_##NBv##_##FL##_f64(a, opt, h->stream))
   if (NB == 8)
     e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(8, rusanov)
       : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(8, linde)
@@ -987,7 +1164,7 @@ extern "C" int bats_set_persistent_grid(bats_handle *h, int nWorkgroup) {
 }
 
 int amr_fill_ghosts(bats_handle *h, const void *U) {
-  AmrArgs m;
+  AmrArgs m{};
   m.U = U; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
   m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
   m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);

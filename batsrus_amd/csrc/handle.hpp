@@ -16,6 +16,7 @@ struct SurrogateState {
   bats_mlp *model;     // network (not owned); null: plain stages
   int mode;            // BATS_SUR_*
   double betaMin;      // gate: minimum plasma beta for the network
+  double jumpTol;      // gate: largest scaled jump counted as smooth
   float *face;         // face fluxes, 9 slots per face, all blocks
   float *rowX;         // network inputs of the gated faces (rows x 16)
   float *rowY;         // network outputs (rows x 8)
@@ -38,6 +39,8 @@ struct bats_handle {
   int stateBuf;  // buffer holding the current State_VGB (differs from cur
                  // between the stages of a 2-stage step)
   double *Usnap;
+  void *lowCrit;  // low order face weights (elemSize-byte reals,
+                  // 3*NB^2*(NB+1) per block, crit_index order) or null
   int *nei;
   double *cellsize;
   double *dtmaxB;

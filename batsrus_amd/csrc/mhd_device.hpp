@@ -165,6 +165,34 @@ __device__ __forceinline__ void face_lr(Real beta, Real q0, Real q1,
   }
 }
 
+// face_lr with the low order weight of the face: the limited slopes scaled
+// by w = 1 - LowOrderCrit (src/ModFaceValue.f90:2021-2027)
+template <int LIM, typename Real>
+__device__ __forceinline__ void face_lr_w(Real beta, Real w, Real q0, Real q1,
+                                          Real q2, Real q3, Real &L,
+                                          Real &R) {
+  if (LIM == LIM_FIRST) {
+    L = q1;
+    R = q2;
+  } else {
+    Real dl, dr, t;
+    slopes<LIM>(beta, q0, q1, q2, dl, t);
+    slopes<LIM>(beta, q1, q2, q3, t, dr);
+    L = q1 + dl * w;
+    R = q2 - dr * w;
+  }
+}
+
+template <int LIM, typename Real>
+__device__ __forceinline__ void face_states_w(Real beta, Real w,
+                                              const Real *q0, const Real *q1,
+                                              const Real *q2, const Real *q3,
+                                              Real *L, Real *R) {
+#pragma unroll
+  for (int v = 0; v < NVAR; v++)
+    face_lr_w<LIM>(beta, w, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
+}
+
 template <int LIM, typename Real>
 __device__ __forceinline__ void face_states(Real beta, const Real *q0,
                                             const Real *q1, const Real *q2,
@@ -190,6 +218,22 @@ __device__ __forceinline__ void face_states_rt(int lim, Real beta,
   case LIM_BETA: face_states<LIM_BETA>(beta, q0, q1, q2, q3, L, R); break;
   case LIM_NO: face_states<LIM_NO>(beta, q0, q1, q2, q3, L, R); break;
   default: face_states<LIM_FIRST>(beta, q0, q1, q2, q3, L, R); break;
+  }
+}
+template <typename Real>
+__device__ __forceinline__ void face_states_w_rt(int lim, Real beta, Real w,
+                                                 const Real *q0,
+                                                 const Real *q1,
+                                                 const Real *q2,
+                                                 const Real *q3, Real *L,
+                                                 Real *R) {
+  switch (lim) {
+  case LIM_MINMOD: face_states_w<LIM_MINMOD>(beta, w, q0, q1, q2, q3, L, R); break;
+  case LIM_MC: face_states_w<LIM_MC>(beta, w, q0, q1, q2, q3, L, R); break;
+  case LIM_MC3: face_states_w<LIM_MC3>(beta, w, q0, q1, q2, q3, L, R); break;
+  case LIM_BETA: face_states_w<LIM_BETA>(beta, w, q0, q1, q2, q3, L, R); break;
+  case LIM_NO: face_states_w<LIM_NO>(beta, w, q0, q1, q2, q3, L, R); break;
+  default: face_states_w<LIM_FIRST>(beta, w, q0, q1, q2, q3, L, R); break;
   }
 }
 

@@ -148,6 +148,33 @@ __global__ void k_adam(AdamArgs a) {
   a.wb[p] = (__bf16)w;
 }
 
+// per-block partial sums of squares of the 8 target columns (fixed order)
+__global__ void k_col_sumsq(const float *T, long n, double *partial) {
+  __shared__ double s[256][8];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long r = blockIdx.x * 256L + threadIdx.x; r < n; r += gridDim.x * 256L)
+    for (int j = 0; j < 8; j++) acc[j] += (double)T[r * 8 + j] * T[r * 8 + j];
+  for (int j = 0; j < 8; j++) s[threadIdx.x][j] = acc[j];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double t = 0.0;
+    for (int q = 0; q < 256; q++) t += s[q][threadIdx.x];
+    partial[blockIdx.x * 8 + threadIdx.x] = t;
+  }
+}
+
+// last layer times (mul) or over (div) the per-output scale
+__global__ void k_scale_last(float *w, const float *scale, int div) {
+  const int t = threadIdx.x;  // 64*8 + 8 entries
+  if (t < 64 * 8) {
+    const int n = t % 8;
+    w[OW4 + t] = div ? w[OW4 + t] / scale[n] : w[OW4 + t] * scale[n];
+  } else if (t < 64 * 8 + 8) {
+    const int n = t - 64 * 8;
+    w[OB4 + n] = div ? w[OB4 + n] / scale[n] : w[OB4 + n] * scale[n];
+  }
+}
+
 __global__ void k_to_bf16(const float *x, __bf16 *y, long n) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t < n) y[t] = (__bf16)x[t];
@@ -187,22 +214,28 @@ k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
     infer_tile(sW, w, sAct[wave], X, rows, tile * 16, out);
 }
 
-// minibatch gather: rows idx[0..B) of the dataset into X (BF16) and T
+// minibatch gather: rows idx[0..B) of the dataset into X (BF16) and the
+// standardized targets T / scale
 __global__ void k_gather_rows(const float *dataX, const float *dataT,
-                              const int *idx, int B, __bf16 *X, float *T) {
+                              const int *idx, int B, __bf16 *X, float *T,
+                              const float *scale) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B) return;
   const long s = idx[t];
   for (int j = 0; j < 16; j++) X[t * 16 + j] = (__bf16)dataX[s * 16 + j];
-  for (int j = 0; j < 8; j++) T[t * 8 + j] = dataT[s * 8 + j];
+  for (int j = 0; j < 8; j++) T[t * 8 + j] = dataT[s * 8 + j] / scale[j];
 }
 __global__ void k_gather_rows_f32(const float *dataX, const float *dataT,
-                                  const int *idx, int B, float *X, float *T) {
+                                  const float *dataB, const float *dataJ,
+                                  const int *idx, int B, float *X, float *T,
+                                  float *Bo, float *Jo) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B) return;
   const long s = idx[t];
   for (int j = 0; j < 16; j++) X[(long)t * 16 + j] = dataX[s * 16 + j];
   for (int j = 0; j < 8; j++) T[(long)t * 8 + j] = dataT[s * 8 + j];
+  for (int j = 0; j < 8; j++) Bo[(long)t * 8 + j] = dataB[s * 8 + j];
+  Jo[t] = dataJ[s];
 }
 
 }  // namespace
@@ -222,8 +255,9 @@ struct bats_mlp {
   __bf16 *X, *A1, *A2, *A3, *dZ1, *dZ2, *dZ3, *dY;
   float *Z1, *Z2, *Z3, *Y, *T, *rowErr, *partial;
   int nSplit;
-  // dataset (device): inputs (n x 16) and targets (n x 8), FP32
-  float *dataX, *dataT;
+  // dataset (device): inputs (n x 16), targets (n x 8) and the baseline
+  // flux the target corrects (n x 8, zero unless harvested), FP32
+  float *dataX, *dataT, *dataB, *dataJ;  // + weight of each sample's target
   long nData, capData;
 };
 
@@ -293,7 +327,8 @@ extern "C" int bats_mlp_free(bats_mlp *m) {
   if (!m) return BATS_EINVAL;
   (void)hipStreamSynchronize(m->stream);
   free_batch(m);
-  void *p[] = {m->w, m->m, m->v, m->grad, m->wb, m->frag, m->dataX, m->dataT};
+  void *p[] = {m->w, m->m, m->v, m->grad, m->wb, m->frag, m->dataX, m->dataT,
+               m->dataB, m->dataJ};
   for (void *q : p)
     if (q) (void)hipFree(q);
   (void)hipStreamDestroy(m->stream);
@@ -496,30 +531,48 @@ int bats::mlp::forward_device(bats_mlp *m, const float *dX, float *dY,
 // ---- dataset and training loop ----------------------------------------
 static int ensure_data(bats_mlp *m, long n) {
   if (n <= m->capData) return BATS_OK;
-  float *x = nullptr, *t = nullptr;
+  float *x = nullptr, *t = nullptr, *b = nullptr, *jw = nullptr;
   MCHK(hipMalloc(&x, sizeof(float) * (size_t)n * 16));
   MCHK(hipMalloc(&t, sizeof(float) * (size_t)n * 8));
+  MCHK(hipMalloc(&b, sizeof(float) * (size_t)n * 8));
+  MCHK(hipMalloc(&jw, sizeof(float) * (size_t)n));
   if (m->nData > 0) {
     MCHK(hipMemcpyAsync(x, m->dataX, sizeof(float) * (size_t)m->nData * 16,
                         hipMemcpyDeviceToDevice, m->stream));
     MCHK(hipMemcpyAsync(t, m->dataT, sizeof(float) * (size_t)m->nData * 8,
                         hipMemcpyDeviceToDevice, m->stream));
+    MCHK(hipMemcpyAsync(b, m->dataB, sizeof(float) * (size_t)m->nData * 8,
+                        hipMemcpyDeviceToDevice, m->stream));
+    MCHK(hipMemcpyAsync(jw, m->dataJ, sizeof(float) * (size_t)m->nData,
+                        hipMemcpyDeviceToDevice, m->stream));
   }
   MCHK(hipStreamSynchronize(m->stream));
   if (m->dataX) (void)hipFree(m->dataX);
   if (m->dataT) (void)hipFree(m->dataT);
+  if (m->dataB) (void)hipFree(m->dataB);
+  if (m->dataJ) (void)hipFree(m->dataJ);
   m->dataX = x;
   m->dataT = t;
+  m->dataB = b;
+  m->dataJ = jw;
   m->capData = n;
   return BATS_OK;
 }
 
-int bats::mlp::reserve_data(bats_mlp *m, long nMore, float **X, float **T) {
+int bats::mlp::reserve_data(bats_mlp *m, long nMore, float **X, float **T,
+                            float **B, float **J) {
   int st = ensure_data(m, m->nData + nMore);
   if (st) return st;
   *X = m->dataX + (size_t)m->nData * 16;
   *T = m->dataT + (size_t)m->nData * 8;
+  *B = m->dataB + (size_t)m->nData * 8;
+  *J = m->dataJ + (size_t)m->nData;
   return BATS_OK;
+}
+
+__global__ void k_fill(float *x, long n, float v) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
 }
 
 void bats::mlp::commit_data(bats_mlp *m, long nAdded) { m->nData += nAdded; }
@@ -541,12 +594,17 @@ extern "C" int bats_mlp_set_data(bats_mlp *m, const float *X, const float *T,
                    hipMemcpyHostToDevice));
     MCHK(hipMemcpy(m->dataT, T, sizeof(float) * (size_t)n * 8,
                    hipMemcpyHostToDevice));
+    MCHK(hipMemset(m->dataB, 0, sizeof(float) * (size_t)n * 8));
+    hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, m->stream,
+                       m->dataJ, n, 1.0f);
+    MCHK(hipStreamSynchronize(m->stream));
   }
   m->nData = n;
   return BATS_OK;
 }
 
-extern "C" int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T) {
+extern "C" int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T,
+                                 float *B) {
   if (!m) return BATS_EINVAL;
   MCHK(hipStreamSynchronize(m->stream));
   if (n) *n = m->nData;
@@ -555,6 +613,9 @@ extern "C" int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T) {
                    hipMemcpyDeviceToHost));
   if (T && m->nData)
     MCHK(hipMemcpy(T, m->dataT, sizeof(float) * (size_t)m->nData * 8,
+                   hipMemcpyDeviceToHost));
+  if (B && m->nData)
+    MCHK(hipMemcpy(B, m->dataB, sizeof(float) * (size_t)m->nData * 8,
                    hipMemcpyDeviceToHost));
   return BATS_OK;
 }
@@ -574,8 +635,9 @@ static void shuffle(std::vector<int> &v, uint64_t &s) {
 }
 
 extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
-                              double HoldFrac, unsigned long long Seed,
-                              double *TrainLoss, double *HeldOutRelErr_V) {
+                              double HoldFrac, int Standardize,
+                              unsigned long long Seed, double *TrainLoss,
+                              double *HeldOutRelErr_V) {
   if (!m || nEpoch < 0 || Batch <= 0 || !(Lr0 >= 0.0) || HoldFrac < 0.0 ||
       HoldFrac >= 1.0 || m->nData < 2)
     return BATS_EINVAL;
@@ -596,6 +658,40 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
   std::vector<int> hold(perm.begin() + nTrain, perm.end());
   int *dIdx = nullptr;
   MCHK(hipMalloc(&dIdx, sizeof(int) * (size_t)(nTrain > nHold ? nTrain : nHold)));
+  // optionally standardize the targets: each output divided by its RMS
+  // over the samples while training; the last layer carries the scale in
+  // between calls, so the network always outputs physical targets
+  float *dScale = nullptr;
+  MCHK(hipMalloc(&dScale, sizeof(float) * 8));
+  {
+    const float one[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    MCHK(hipMemcpy(dScale, one, sizeof one, hipMemcpyHostToDevice));
+  }
+  if (Standardize) {
+    const int nb = 256;
+    double *dPart = nullptr;
+    MCHK(hipMalloc(&dPart, sizeof(double) * nb * 8));
+    hipLaunchKernelGGL(k_col_sumsq, dim3(nb), dim3(256), 0, m->stream,
+                       m->dataT, n, dPart);
+    std::vector<double> part(nb * 8);
+    MCHK(hipMemcpyAsync(part.data(), dPart, sizeof(double) * part.size(),
+                        hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    (void)hipFree(dPart);
+    float sc[8];
+    for (int j = 0; j < 8; j++) {
+      double t = 0.0;
+      for (int q = 0; q < nb; q++) t += part[q * 8 + j];
+      const double rms = sqrt(t / (double)n);
+      sc[j] = rms > 0.0 ? (float)rms : 1.0f;
+    }
+    MCHK(hipMemcpyAsync(dScale, sc, sizeof sc, hipMemcpyHostToDevice,
+                        m->stream));
+    hipLaunchKernelGGL(k_scale_last, dim3(1), dim3(64 * 8 + 8), 0, m->stream,
+                       m->w, dScale, 1);
+    hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0,
+                       m->stream, m->w, m->wb, (long)NPARAM);
+  }
   const long stepsPerEpoch = nTrain / B;
   const long total = stepsPerEpoch * nEpoch;
   long step = 0;
@@ -610,7 +706,7 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
       const double lr = Lr0 * 0.5 * (1.0 + cos(M_PI * (double)step / (double)total));
       hipLaunchKernelGGL(k_gather_rows, dim3((B + 255) / 256), dim3(256), 0,
                          m->stream, m->dataX, m->dataT, dIdx + b * B, B, m->X,
-                         m->T);
+                         m->T, dScale);
       st = grad_step(m, B, (float)lr, true);
       if (st) { (void)hipFree(dIdx); return st; }
       if (b == stepsPerEpoch - 1) {  // loss of the epoch's last minibatch
@@ -622,41 +718,68 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
       }
     }
   }
+  // back to physical targets (a no-op scale of one without Standardize)
+  hipLaunchKernelGGL(k_scale_last, dim3(1), dim3(64 * 8 + 8), 0, m->stream,
+                     m->w, dScale, 0);
+  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0,
+                     m->stream, m->w, m->wb, (long)NPARAM);
+  m->fragStale = 1;
+  MCHK(hipStreamSynchronize(m->stream));
+  (void)hipFree(dScale);
   if (TrainLoss) *TrainLoss = lastLoss;
   // held-out relative error: ||Y - T|| / ||T|| overall and per output
   if (HeldOutRelErr_V && nHold > 0) {
-    float *hx = nullptr, *ht = nullptr, *hy = nullptr;
+    float *hx = nullptr, *ht = nullptr, *hy = nullptr, *hb = nullptr,
+          *hj = nullptr;
     MCHK(hipMalloc(&hx, sizeof(float) * (size_t)nHold * 16));
     MCHK(hipMalloc(&ht, sizeof(float) * (size_t)nHold * 8));
     MCHK(hipMalloc(&hy, sizeof(float) * (size_t)nHold * 8));
+    MCHK(hipMalloc(&hb, sizeof(float) * (size_t)nHold * 8));
+    MCHK(hipMalloc(&hj, sizeof(float) * (size_t)nHold));
     MCHK(hipMemcpyAsync(dIdx, hold.data(), sizeof(int) * hold.size(),
                         hipMemcpyHostToDevice, m->stream));
     hipLaunchKernelGGL(k_gather_rows_f32, dim3((nHold + 255) / 256), dim3(256),
-                       0, m->stream, m->dataX, m->dataT, dIdx, (int)nHold, hx,
-                       ht);
+                       0, m->stream, m->dataX, m->dataT, m->dataB, m->dataJ,
+                       dIdx, (int)nHold, hx, ht, hb, hj);
     MCHK(hipStreamSynchronize(m->stream));
     st = bats::mlp::forward_device(m, hx, hy, (int)nHold, m->stream);
-    std::vector<float> Y((size_t)nHold * 8), T((size_t)nHold * 8);
+    std::vector<float> Y((size_t)nHold * 8), T((size_t)nHold * 8),
+        Bv((size_t)nHold * 8), Jv((size_t)nHold);
     MCHK(hipMemcpyAsync(Y.data(), hy, sizeof(float) * Y.size(),
                         hipMemcpyDeviceToHost, m->stream));
     MCHK(hipMemcpyAsync(T.data(), ht, sizeof(float) * T.size(),
                         hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipMemcpyAsync(Bv.data(), hb, sizeof(float) * Bv.size(),
+                        hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipMemcpyAsync(Jv.data(), hj, sizeof(float) * Jv.size(),
+                        hipMemcpyDeviceToHost, m->stream));
     MCHK(hipStreamSynchronize(m->stream));
-    double num[9] = {0}, den[9] = {0};
+    double num[9] = {0}, den[9] = {0}, denT[9] = {0};
     for (long r = 0; r < nHold; r++)
       for (int j = 0; j < 8; j++) {
-        const double d = (double)Y[r * 8 + j] - (double)T[r * 8 + j];
-        const double t = T[r * 8 + j];
+        // predicted flux base + w Y against the full flux base + w T,
+        // w the sample's target weight (the jump norm for harvested faces)
+        const double w = Jv[r];
+        const double d = w * ((double)Y[r * 8 + j] - (double)T[r * 8 + j]);
+        const double t = w * (double)T[r * 8 + j] + (double)Bv[r * 8 + j];
+        const double tt = w * (double)T[r * 8 + j];
         num[j + 1] += d * d;
         den[j + 1] += t * t;
+        denT[j + 1] += tt * tt;
         num[0] += d * d;
         den[0] += t * t;
+        denT[0] += tt * tt;
       }
-    for (int j = 0; j < 9; j++)
+    for (int j = 0; j < 9; j++) {
       HeldOutRelErr_V[j] = den[j] > 0 ? sqrt(num[j] / den[j]) : sqrt(num[j]);
+      HeldOutRelErr_V[9 + j] =
+          denT[j] > 0 ? sqrt(num[j] / denT[j]) : sqrt(num[j]);
+    }
     (void)hipFree(hx);
     (void)hipFree(ht);
     (void)hipFree(hy);
+    (void)hipFree(hb);
+    (void)hipFree(hj);
   }
   (void)hipFree(dIdx);
   return st;

@@ -11,8 +11,10 @@ namespace mlp {
 int forward_device(bats_mlp *m, const float *dX, float *dY, int rows,
                    hipStream_t stream);
 // room for nMore samples at the end of the model's device dataset (inputs
-// n x 16, targets n x 8, FP32); commit_data then adds the ones written
-int reserve_data(bats_mlp *m, long nMore, float **X, float **T);
+// n x 16, targets n x 8, baseline fluxes n x 8, target weights n, FP32);
+// commit_data then adds the ones written
+int reserve_data(bats_mlp *m, long nMore, float **X, float **T, float **B,
+                 float **J);
 void commit_data(bats_mlp *m, long nAdded);
 // device weights for fused use: fragment-ordered BF16 and FP32 flat
 struct Weights {

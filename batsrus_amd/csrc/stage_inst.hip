@@ -20,10 +20,10 @@
 
 namespace bats {
 hipError_t BATS_NAME(BATS_NB, BATS_FLUX, BATS_REAL)(const StageArgs &a,
-                                                    bool powell,
+                                                    int opt,
                                                     hipStream_t stream) {
   return launch_stage_kernel<BATS_NB, BATS_PASTE(BATS_FLUX_, BATS_FLUX),
-                             BATS_PASTE(BATS_REAL_, BATS_REAL)>(a, powell,
+                             BATS_PASTE(BATS_REAL_, BATS_REAL)>(a, opt,
                                                                 stream);
 }
 #define BATS_FNAME_(nb, fl, ty) launch_fineflux_##nb##_##fl##_##ty

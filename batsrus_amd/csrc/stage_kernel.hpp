@@ -64,7 +64,27 @@ struct StageArgs {
                        // [slab][q][p][sub 0..3][9 slots]; null: no slabs
   int nGroup;          // workgroup-sized groups of blocks (work items)
   int nWorkgroup;      // launched workgroups (< nGroup: persistent grid)
+  const void *lowCrit; // low order weight of every face (type T, see
+                       // crit_index); read only by the low order variant
 };
+
+// Stage kernel variants: the 8-wave div B source and the adaptive low order
+// face values are compile-time options so the plain kernel carries neither.
+enum { OPT_POWELL = 1, OPT_LOWORDER = 2 };
+
+// index of the low order weight of a face of block b: the face normal to
+// dir at face coordinates (i, j, k), whose dir component is the face index
+// 0..NB (face between cells f-1 and f).  x and y faces are stored plane by
+// plane in the stage kernel's face order, z faces by face then column.
+template <int NB>
+__host__ __device__ __forceinline__ long crit_index(int b, int dir, int i,
+                                                    int j, int k) {
+  constexpr int NFX = (NB + 1) * NB;
+  const long base = ((long)b * 3 + dir) * NB * NFX;
+  if (dir == 0) return base + (long)k * NFX + j * (NB + 1) + i;
+  if (dir == 1) return base + (long)k * NFX + i * (NB + 1) + j;
+  return base + ((long)k * NB + j) * NB + i;
+}
 
 // offset (within one variable plane) of cell (i,j,k) of block b; one index
 // may lie in the 2-cell halo: it is resolved to the neighbour block, at a
@@ -130,6 +150,16 @@ __device__ __forceinline__ double wave_min(double x) {
 }
 
 template <int DIR, int FLUX, int LIM, typename T>
+__device__ __forceinline__ int face_flux_w(const PhysT<T> &ph, T area, T beta,
+                                           T w, const T *q0, const T *q1,
+                                           const T *q2, const T *q3, T *F,
+                                           T &vdt, T &unA) {
+  T L[NVAR], R[NVAR];
+  face_states_w<LIM>(beta, w, q0, q1, q2, q3, L, R);
+  return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
+}
+
+template <int DIR, int FLUX, int LIM, typename T>
 __device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T beta,
                                          const T *q0, const T *q1,
                                          const T *q2, const T *q3, T *F,
@@ -144,9 +174,12 @@ __device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T beta,
 // plain path; float: FP32-state path); the stage update, the pressure fix
 // and the time step stay in double.
 // ---------------------------------------------------------------------
-template <int NB, int FLUX, int LIM, bool POWELL, typename T>
+template <int NB, int FLUX, int LIM, int OPT, typename T>
 __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     k_stage(StageArgs a) {
+  constexpr bool POWELL = (OPT & OPT_POWELL) != 0;
+  // adaptive low order faces: slopes scaled by 1 - weight of the face
+  constexpr bool LOWORD = (OPT & OPT_LOWORDER) != 0 && LIM != LIM_FIRST;
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -187,6 +220,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   const long vs = a.vs;
   const PhysT<T> ph = phys_as<T>(a.ph);
   const T beta = T(a.beta);
+  const T *crit = static_cast<const T *>(a.lowCrit);
   int err = 0;
 
   auto sPi = [&](int l, int v, int jj, int ii) {
@@ -253,15 +287,22 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       load_state(U, vs, off, q);
       to_primitive(q);
     };
-    // z face between Q1 (=P(k)) and Q2 (=P(k+1)); Q3 = P(k+2)
-    auto zface = [&](T *Fz) {
+    // z face fz (between Q1 = P(fz-1) and Q2 = P(fz)); Q3 = P(fz+1)
+    auto zface = [&](T *Fz, int fz) {
       T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+      T w = T(1);
+      if (LOWORD) w = T(1) - crit[crit_index<NB>(bb, 2, i, j, fz)];
 #pragma unroll
       for (int v = 0; v < NVAR; v++) {
         T dl, dr;
         slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
-        L[v] = Q1[v] + dLk[v];
-        R[v] = Q2[v] - dr;
+        if (LOWORD) {
+          L[v] = Q1[v] + dLk[v] * w;
+          R[v] = Q2[v] - dr * w;
+        } else {
+          L[v] = Q1[v] + dLk[v];
+          R[v] = Q2[v] - dr;
+        }
         dLk[v] = dl;
       }
       err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
@@ -284,7 +325,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
         dLk[v] = dl;
       }
-      zface(FzLo);
+      zface(FzLo, 0);
       if (a.corr) corrected(bb, 4, i, j, FzLo);
 #pragma unroll
       for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
@@ -294,7 +335,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       // z face k+1 of this column
       if (act) {
         load_prim(k + 2, Q3);
-        zface(FzHi);
+        zface(FzHi, k + 1);
         if (a.corr && k == NB - 1) corrected(bb, 5, i, j, FzHi);
         // own primitive of plane k
 #pragma unroll
@@ -329,8 +370,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
             c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
           }
-          err |= face_flux<0, FLUX, LIM>(ph, T(dyf * dzf), beta, c0, c1, c2,
-                                         c3, F, vdt, unA);
+          if (LOWORD)
+            err |= face_flux_w<0, FLUX, LIM>(
+                ph, T(dyf * dzf), beta,
+                T(1) - crit[crit_index<NB>(bf, 0, fp, row, k)], c0, c1, c2,
+                c3, F, vdt, unA);
+          else
+            err |= face_flux<0, FLUX, LIM>(ph, T(dyf * dzf), beta, c0, c1,
+                                           c2, c3, F, vdt, unA);
         } else {
 #pragma unroll
           for (int v = 0; v < NVAR; v++) {
@@ -339,8 +386,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
             c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
           }
-          err |= face_flux<1, FLUX, LIM>(ph, T(dxf * dzf), beta, c0, c1, c2,
-                                         c3, F, vdt, unA);
+          if (LOWORD)
+            err |= face_flux_w<1, FLUX, LIM>(
+                ph, T(dxf * dzf), beta,
+                T(1) - crit[crit_index<NB>(bf, 1, row, fp, k)], c0, c1, c2,
+                c3, F, vdt, unA);
+          else
+            err |= face_flux<1, FLUX, LIM>(ph, T(dxf * dzf), beta, c0, c1,
+                                           c2, c3, F, vdt, unA);
         }
         F[PP] = vdt;
         if (a.corr && (fp == 0 || fp == NB))
@@ -375,12 +428,28 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
 #pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
-            face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
-            face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
+            if (LOWORD) {
+              face_lr_w<LIM>(beta, T(1) - crit[crit_index<NB>(b, 0, i, j, k)],
+                             p[0], p[1], p[2], p[3], LxLo, RxLo);
+              face_lr_w<LIM>(beta,
+                             T(1) - crit[crit_index<NB>(b, 0, i + 1, j, k)],
+                             p[1], p[2], p[3], p[4], LxHi, RxHi);
+            } else {
+              face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
+              face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
+            }
 #pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
-            face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
-            face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
+            if (LOWORD) {
+              face_lr_w<LIM>(beta, T(1) - crit[crit_index<NB>(b, 1, i, j, k)],
+                             p[0], p[1], p[2], p[3], LyLo, RyLo);
+              face_lr_w<LIM>(beta,
+                             T(1) - crit[crit_index<NB>(b, 1, i, j + 1, k)],
+                             p[1], p[2], p[3], p[4], LyHi, RyHi);
+            } else {
+              face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
+              face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
+            }
             T Src[NFLUX];
             powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
                           RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
@@ -471,6 +540,7 @@ struct AmrArgs {
   int lim;
   void *corr;           // type T, [slab][q][p][sub][9]
   int *err;
+  const void *lowCrit;  // low order face weights (type T) or null
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -579,7 +649,15 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
   const double *cz = a.cellsize + 3 * b;
   const T area = T(d == 0 ? cz[1] * cz[2] : d == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
   T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
-  face_states<LIM>(T(a.beta), c[0], c[1], c[2], c[3], L, R);
+  if (a.lowCrit) {
+    int x, y, z;
+    side_xyz(d, hi ? NB : 0, p, q, x, y, z);
+    const T w = T(1) - static_cast<const T *>(a.lowCrit)[crit_index<NB>(
+                           b, d, x, y, z)];
+    face_states_w<LIM>(T(a.beta), w, c[0], c[1], c[2], c[3], L, R);
+  } else {
+    face_states<LIM>(T(a.beta), c[0], c[1], c[2], c[3], L, R);
+  }
   int err;
   if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
   else if (d == 1) err = numerical_flux<1, FLUX, false>(ph, area, L, R, F, vdt, unA);
@@ -606,20 +684,24 @@ inline hipError_t launch_amr_fineflux(const AmrArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-// Launch k_stage for one block size and flux; the limiter and the 8-wave
-// switch are dispatched here.
+// Launch k_stage for one block size and flux; the limiter and the options
+// (OPT_* bits) are dispatched here.
 template <int NB, int FLUX, typename T>
-inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
+inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
                                       hipStream_t stream) {
   const dim3 grid(a.nGroup < a.nWorkgroup ? a.nGroup : a.nWorkgroup),
       block(kThreads);
 #define BATS_LAUNCH(LM)                                                     \
-  if (powell)                                                               \
-    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, true, T>), grid, block, 0,    \
-                       stream, a);                                          \
-  else                                                                      \
-    hipLaunchKernelGGL((k_stage<NB, FLUX, LM, false, T>), grid, block, 0,   \
-                       stream, a)
+  switch (opt) {                                                            \
+  case 0: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 0, T>), grid, block, 0, \
+                             stream, a); break;                             \
+  case 1: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 1, T>), grid, block, 0, \
+                             stream, a); break;                             \
+  case 2: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 2, T>), grid, block, 0, \
+                             stream, a); break;                             \
+  default: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 3, T>), grid, block,   \
+                              0, stream, a); break;                         \
+  }
   switch (a.lim) {
   case LIM_MINMOD: BATS_LAUNCH(LIM_MINMOD); break;
   case LIM_MC: BATS_LAUNCH(LIM_MC); break;
@@ -635,7 +717,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, bool powell,
 // one launcher per (block size, flux, storage type) translation unit
 // (stage_inst.hip)
 #define BATS_DECLARE_LAUNCH(NB, FL, TY)                                     \
-  hipError_t launch_stage_##NB##_##FL##_##TY(const StageArgs &, bool,       \
+  hipError_t launch_stage_##NB##_##FL##_##TY(const StageArgs &, int,        \
                                              hipStream_t);
 #define BATS_DECLARE_LAUNCH_FLUX(NB, TY)                                    \
   BATS_DECLARE_LAUNCH(NB, rusanov, TY)                                      \

@@ -6,19 +6,29 @@
 // (rho, u_n, u_t1, u_t2, B_n, B_t1, B_t2, p) with (n, t1, t2) = (d, d+1,
 // d+2 mod 3), and scaled by the face's density rho_s = (rho_L + rho_R)/2
 // and speed c_s = max(c_fL, c_fR) (normal fast magnetosonic speeds):
-//   inputs  rho/rho_s, u/c_s, B/(c_s sqrt(rho_s)), p/(rho_s c_s^2)  (L, R)
-//   outputs F_rho/(rho_s c_s), F_m/(rho_s c_s^2), F_B/(c_s^2 sqrt(rho_s)),
+//   states  rho/rho_s, u/c_s, B/(c_s sqrt(rho_s)), p/(rho_s c_s^2)
+//   fluxes  F_rho/(rho_s c_s), F_m/(rho_s c_s^2), F_B/(c_s^2 sqrt(rho_s)),
 //           F_E/(rho_s c_s^3)
-// where F is the numerical flux per unit area of the plain path (HLLD with
-// the Linde correction of the normal field, src/ModFaceFlux.f90:2826-3160,
-// 2302-2320, 2403-2418) in the rotated frame.
+// The 16 inputs are the mean of the scaled L and R states and the unit
+// direction of their difference j = s_R - s_L; the 8 outputs are the
+// scaled flux correction divided by |j|.  (An invertible re-encoding of the
+// scaled L/R pair: on smooth faces the jump is far below the BF16 input
+// resolution of the raw states, and the correction is proportional to it.)
+// The network learns the correction that takes the Rusanov flux of the face
+// (src/ModFaceFlux.f90:2513-2559) to its HLLD flux with the Linde
+// correction of the normal field (src/ModFaceFlux.f90:2826-3160, 2302-2320,
+// 2403-2418): target = (F_HLLD - F_Rusanov) in the rotated frame and the
+// output scaling above, per unit area.  In the stage, a network face gets
+// F_Rusanov + the network's correction, so its error scales with the
+// correction rather than with the flux.
 //
 // Smoothness gate: a face goes to the network when, for every variable and
-// for both cells next to the face, the MC limiter would leave the slope
-// unclipped (the two one-sided differences agree in sign and their mean is
-// within beta times the smaller one, or both vanish), and the plasma beta
-// 2p/B^2 of both face states is at least BetaMin.  Every other face takes
-// the exact flux.
+// for both cells next to the face, either the MC limiter would leave the
+// slope unclipped (the two one-sided differences agree in sign and their
+// mean is within beta times the smaller one) or the jumps are small (the
+// two one-sided differences, scaled like the network inputs, add up to at
+// most JumpTol), and the plasma beta 2p/B^2 of both face states is at least
+// BetaMin.  Every other face takes the exact flux.
 //
 // Surrogate stage (FP32 state):
 //   k_sur_faces   face states, gate, Vdt; exact flux of the gated-out faces
@@ -63,23 +73,30 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
 }
 
 template <typename R>
-__device__ __forceinline__ bool var_smooth(R p0, R p1, R p2, R beta) {
+__device__ __forceinline__ bool var_smooth(R p0, R p1, R p2, R beta,
+                                           R tol) {
   const R d1 = p1 - p0, d2 = p2 - p1;
-  if (d1 == R(0) && d2 == R(0)) return true;
+  if (fabsr(d1) + fabsr(d2) <= tol) return true;
   return d1 * d2 > R(0) &&
          R(0.5) * fabsr(d1 + d2) <= beta * fmin2(fabsr(d1), fabsr(d2));
 }
 
+// tol: JumpTol in the network's scaling (rho_s, c_s) of each variable
 template <typename R>
 __device__ __forceinline__ bool face_smooth(const R *q0, const R *q1,
                                             const R *q2, const R *q3,
                                             const R *L, const R *Rs, R beta,
-                                            R betaMin) {
+                                            R betaMin, R jumpTol, R rhos,
+                                            R cs) {
+  const R sRho = jumpTol * rhos, sU = jumpTol * cs,
+          sB = jumpTol * cs * fsqrt(rhos), sP = jumpTol * rhos * cs * cs;
   bool ok = true;
 #pragma unroll
-  for (int v = 0; v < NVAR; v++)
-    ok = ok && var_smooth(q0[v], q1[v], q2[v], beta) &&
-         var_smooth(q1[v], q2[v], q3[v], beta);
+  for (int v = 0; v < NVAR; v++) {
+    const R tol = v == RHO ? sRho : v < BX ? sU : v < PP ? sB : sP;
+    ok = ok && var_smooth(q0[v], q1[v], q2[v], beta, tol) &&
+         var_smooth(q1[v], q2[v], q3[v], beta, tol);
+  }
   const R b2L = L[BX] * L[BX] + L[BY] * L[BY] + L[BZ] * L[BZ];
   const R b2R = Rs[BX] * Rs[BX] + Rs[BY] * Rs[BY] + Rs[BZ] * Rs[BZ];
   ok = ok && R(2) * L[PP] >= betaMin * b2L && R(2) * Rs[PP] >= betaMin * b2R;
@@ -102,12 +119,14 @@ __device__ __forceinline__ int rot_slot(int dir, int c) {
   }
 }
 
-// the 16 scaled, rotated inputs of a face; returns false on a negative fast
-// speed squared
+// The 16 network inputs of a face: with s_L, s_R the scaled, rotated face
+// states, the mean m = (s_L + s_R)/2 and the jump direction j/|j| of
+// j = s_R - s_L (|j| the Euclidean norm, returned as jn; j/|j| = 0 when
+// |j| = 0).  Returns false on a negative fast speed squared.
 template <int DIR, typename R>
 __device__ __forceinline__ bool face_inputs(const PhysT<R> &ph, const R *L,
                                             const R *Rs, float *x, R &rhos,
-                                            R &cs) {
+                                            R &cs, R &jn) {
   constexpr int it1 = (DIR + 1) % 3, it2 = (DIR + 2) % 3;
   R unL, cfL, unR, cfR;
   const bool ok = fast_speed<DIR>(ph, L, unL, cfL) & fast_speed<DIR>(ph, Rs, unR, cfR);
@@ -115,18 +134,31 @@ __device__ __forceinline__ bool face_inputs(const PhysT<R> &ph, const R *L,
   cs = fmax2(cfL, cfR);
   const R iu = R(1) / cs, ib = iu / fsqrt(rhos), ip = iu * iu / rhos,
           ir = R(1) / rhos;
+  R sv[2][8];
   const R *S[2] = {L, Rs};
 #pragma unroll
   for (int s = 0; s < 2; s++) {
-    float *o = x + 8 * s;
-    o[0] = float(S[s][RHO] * ir);
-    o[1] = float(S[s][MX + DIR] * iu);
-    o[2] = float(S[s][MX + it1] * iu);
-    o[3] = float(S[s][MX + it2] * iu);
-    o[4] = float(S[s][BX + DIR] * ib);
-    o[5] = float(S[s][BX + it1] * ib);
-    o[6] = float(S[s][BX + it2] * ib);
-    o[7] = float(S[s][PP] * ip);
+    sv[s][0] = S[s][RHO] * ir;
+    sv[s][1] = S[s][MX + DIR] * iu;
+    sv[s][2] = S[s][MX + it1] * iu;
+    sv[s][3] = S[s][MX + it2] * iu;
+    sv[s][4] = S[s][BX + DIR] * ib;
+    sv[s][5] = S[s][BX + it1] * ib;
+    sv[s][6] = S[s][BX + it2] * ib;
+    sv[s][7] = S[s][PP] * ip;
+  }
+  R j2 = R(0);
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const R d = sv[1][c] - sv[0][c];
+    j2 += d * d;
+  }
+  jn = fsqrt(j2);
+  const R ij = jn > R(0) ? R(1) / jn : R(0);
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    x[c] = float(R(0.5) * (sv[0][c] + sv[1][c]));
+    x[8 + c] = float((sv[1][c] - sv[0][c]) * ij);
   }
   return ok;
 }
@@ -153,11 +185,12 @@ struct HarvestArgs {
   long vs, nCellTot;
   int nBlock;
   Phys ph;
-  double beta, betaMin, frac;
+  double beta, betaMin, jumpTol, frac;
   unsigned long long seed;
   long *count;        // pass 1: selected faces per block
   const long *offset; // pass 2: exclusive prefix of count
-  float *X, *T;       // pass 2: sample rows
+  float *X, *T, *B;   // pass 2: sample rows (inputs, targets, baseline)
+  float *J;           // pass 2: jump norm of each sample
 };
 
 template <int NB, int LIM, bool WRITE>
@@ -171,7 +204,7 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
   for (int c0 = 0; c0 < 3 * G::NF; c0 += 256) {
     const int f = c0 + threadIdx.x;
     bool sel = false;
-    float x[16], t[8];
+    float x[16], t[8], bl[8], wj = 0.f;
     if (f < 3 * G::NF) {
       const int dir = f / G::NF, r = f % G::NF;
       const unsigned long long gid =
@@ -188,25 +221,35 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
           load_state(a.U, a.vs, cell_off<NB>(a.nei, b, ii, jj, kk, a.nCellTot), q[s]);
           to_primitive(q[s]);
         }
-        double L[NVAR], R[NVAR];
+        double L[NVAR], R[NVAR], rhos, cs, jn;
         face_states<LIM>(a.beta, q[0], q[1], q[2], q[3], L, R);
-        if (face_smooth(q[0], q[1], q[2], q[3], L, R, a.beta, a.betaMin)) {
-          double F[NFLUX], vdt, unA, rhos, cs;
-          bool ok;
+        bool ok;
+        if (dir == 0) ok = face_inputs<0>(a.ph, L, R, x, rhos, cs, jn);
+        else if (dir == 1) ok = face_inputs<1>(a.ph, L, R, x, rhos, cs, jn);
+        else ok = face_inputs<2>(a.ph, L, R, x, rhos, cs, jn);
+        if (ok && cs > 0.0 && jn > 0.0 &&
+            face_smooth(q[0], q[1], q[2], q[3], L, R, a.beta, a.betaMin,
+                        a.jumpTol, rhos, cs)) {
+          double F[NFLUX], Fr[NFLUX], vdt, unA;
           if (dir == 0) {
-            ok = face_inputs<0>(a.ph, L, R, x, rhos, cs);
-            ok = ok && !numerical_flux<0, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = !numerical_flux<0, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = ok && !numerical_flux<0, FLUX_RUSANOV, false>(a.ph, 1.0, L, R, Fr, vdt, unA);
           } else if (dir == 1) {
-            ok = face_inputs<1>(a.ph, L, R, x, rhos, cs);
-            ok = ok && !numerical_flux<1, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = !numerical_flux<1, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = ok && !numerical_flux<1, FLUX_RUSANOV, false>(a.ph, 1.0, L, R, Fr, vdt, unA);
           } else {
-            ok = face_inputs<2>(a.ph, L, R, x, rhos, cs);
-            ok = ok && !numerical_flux<2, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = !numerical_flux<2, FLUX_HLLD, false>(a.ph, 1.0, L, R, F, vdt, unA);
+            ok = ok && !numerical_flux<2, FLUX_RUSANOV, false>(a.ph, 1.0, L, R, Fr, vdt, unA);
           }
-          if (ok && cs > 0.0) {
+          if (ok) {
             sel = true;
-            for (int c = 0; c < 8; c++)
-              t[c] = float(F[rot_slot(dir, c)] / out_scale(c, rhos, cs));
+            for (int c = 0; c < 8; c++) {
+              const double sc = out_scale(c, rhos, cs);
+              const int sl = rot_slot(dir, c);
+              t[c] = float((F[sl] - Fr[sl]) / (sc * jn));
+              bl[c] = float(Fr[sl] / sc);
+            }
+            wj = float(jn);
           }
         }
       }
@@ -229,6 +272,8 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
       const long row = base + incl - 1;
       for (int c = 0; c < 16; c++) a.X[row * 16 + c] = x[c];
       for (int c = 0; c < 8; c++) a.T[row * 8 + c] = t[c];
+      for (int c = 0; c < 8; c++) a.B[row * 8 + c] = bl[c];
+      a.J[row] = wj;
     }
     base += total;
     __syncthreads();
@@ -260,7 +305,7 @@ struct SurArgs {
   int nBlock;
   long vs, nCellTot;
   Phys ph;
-  double beta, betaMin, DtFactor;
+  double beta, betaMin, jumpTol, DtFactor;
   int lim, mode, iStage, nStage, halfStep, dbTrick, doDt, roundTripOld,
       noUpdate, advanceTime;
   float *face;     // NSLOT x nFaceTot
@@ -284,7 +329,7 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
     const bool valid = gf < nTot;
     bool net = false;
     float x[16];
-    float rhos = 0.f, cs = 0.f;
+    float rhos = 0.f, cs = 0.f, jn = 0.f;
     if (valid) {
       const int dir = (int)(gf / ((long)a.nBlock * G::NF));
       const long rem = gf % ((long)a.nBlock * G::NF);
@@ -303,17 +348,17 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
       const double *cz = a.cellsize + 3 * b;
       const float area = float(dir == 0 ? cz[1] * cz[2]
                                : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
-      const bool smooth =
-          a.mode != BATS_SUR_OFF && face_smooth(q[0], q[1], q[2], q[3], L, R,
-                                                beta, betaMin);
+      bool ok;
+      if (dir == 0) ok = face_inputs<0>(ph, L, R, x, rhos, cs, jn);
+      else if (dir == 1) ok = face_inputs<1>(ph, L, R, x, rhos, cs, jn);
+      else ok = face_inputs<2>(ph, L, R, x, rhos, cs, jn);
+      if (!ok) err |= ERRB_NEGFAST2;
+      const bool smooth = ok && cs > 0.f &&
+                          face_smooth(q[0], q[1], q[2], q[3], L, R, beta,
+                                      betaMin, float(a.jumpTol), rhos, cs);
       float F[NFLUX], vdt, unA;
       if (smooth && a.mode == BATS_SUR_NETWORK) {
         // exact Vdt from the HLLD speeds; the flux comes from the network
-        bool ok;
-        if (dir == 0) ok = face_inputs<0>(ph, L, R, x, rhos, cs);
-        else if (dir == 1) ok = face_inputs<1>(ph, L, R, x, rhos, cs);
-        else ok = face_inputs<2>(ph, L, R, x, rhos, cs);
-        if (!ok) err |= ERRB_NEGFAST2;
         // HLLD's Cmax = max(sR, -sL) of the Linde-corrected states
         float Lc[NVAR], Rc[NVAR];
         for (int v = 0; v < NVAR; v++) { Lc[v] = L[v]; Rc[v] = R[v]; }
@@ -325,8 +370,13 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
         else if (dir == 1) { fast_speed<1>(ph, Lc, uL, cL); fast_speed<1>(ph, Rc, uR, cR); }
         else { fast_speed<2>(ph, Lc, uL, cL); fast_speed<2>(ph, Rc, uR, cR); }
         const float sL = fmin2(uL - cL, uR - cR), sR = fmax2(uL + cL, uR + cR);
-        a.face[PP * nTot + gf] = fmax2(sR, -sL) * area;
-        net = true;
+        // the Rusanov flux; the network adds its correction
+        if (dir == 0) err |= numerical_flux<0, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+        else if (dir == 1) err |= numerical_flux<1, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+        else err |= numerical_flux<2, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
+        F[PP] = fmax2(sR, -sL) * area;
+        for (int v = 0; v < NSLOT; v++) a.face[v * nTot + gf] = F[v];
+        net = jn > 0.f;  // no correction where the face states agree
       } else {
         if (smooth) {  // the method without the network: Rusanov
           if (dir == 0) err |= numerical_flux<0, FLUX_RUSANOV, false>(ph, area, L, R, F, vdt, unA);
@@ -353,8 +403,9 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
         const int row = base + __popcll(mask & ((1ull << lane) - 1ull));
         for (int c = 0; c < 16; c++) a.rowX[(long)row * 16 + c] = x[c];
         a.rowFace[row] = (int)gf;
-        a.rowScale[2 * row] = rhos;
-        a.rowScale[2 * row + 1] = cs;
+        a.rowScale[3 * row] = rhos;
+        a.rowScale[3 * row + 1] = cs;
+        a.rowScale[3 * row + 2] = jn;
       }
       nNet += net ? 1 : 0;
     }
@@ -384,8 +435,10 @@ k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
     const double *cz = a.cellsize + 3 * b;
     const float area = float(dir == 0 ? cz[1] * cz[2]
                              : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
-    const float v = y * out_scale(c, a.rowScale[2 * row], a.rowScale[2 * row + 1]);
-    a.face[(long)rot_slot(dir, c) * nTot + gf] = v * area;
+    const float v = y * a.rowScale[3 * row + 2] *
+                    out_scale(c, a.rowScale[3 * row], a.rowScale[3 * row + 1]);
+    float &f = a.face[(long)rot_slot(dir, c) * nTot + gf];
+    f = f + v * area;
   };
   const int nTile = (rows + 15) / 16;
   for (int tile = blockIdx.x * 4 + wave; tile < nTile; tile += gridDim.x * 4)
@@ -538,9 +591,10 @@ static void launch_sur_faces(int lim, dim3 g, hipStream_t st, const SurArgs &a) 
 }
 
 extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
-                                double BetaMin, unsigned long long Seed,
-                                long *nAdded) {
-  if (!m || !h || !(Fraction >= 0.0) || Fraction > 1.0 || !(BetaMin >= 0.0))
+                                double BetaMin, double JumpTol,
+                                unsigned long long Seed, long *nAdded) {
+  if (!m || !h || !(Fraction >= 0.0) || Fraction > 1.0 || !(BetaMin >= 0.0) ||
+      !(JumpTol >= 0.0))
     return BATS_EINVAL;
   if (h->elemSize != 8) return BATS_EINVAL;  // samples come from FP64 runs
   const bats_config &c = h->cfg;
@@ -555,9 +609,10 @@ extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
   a.ph = make_phys(c);
   a.beta = c.LimiterBeta;
   a.betaMin = BetaMin;
+  a.jumpTol = JumpTol;
   a.frac = Fraction;
   a.seed = Seed;
-  a.X = a.T = nullptr;
+  a.X = a.T = a.B = a.J = nullptr;
   long *dCount = nullptr, *dOff = nullptr;
   HIPCHK(hipMalloc(&dCount, sizeof(long) * nB));
   HIPCHK(hipMalloc(&dOff, sizeof(long) * nB));
@@ -574,13 +629,15 @@ extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
   for (int b = 0; b < nB; b++) { off[b] = total; total += cnt[b]; }
   int st = BATS_OK;
   if (total > 0) {
-    float *X = nullptr, *T = nullptr;
-    st = bats::mlp::reserve_data(m, total, &X, &T);
+    float *X = nullptr, *T = nullptr, *Bl = nullptr, *Jn = nullptr;
+    st = bats::mlp::reserve_data(m, total, &X, &T, &Bl, &Jn);
     if (st == BATS_OK) {
       HIPCHK(hipMemcpyAsync(dOff, off.data(), sizeof(long) * nB,
                             hipMemcpyHostToDevice, h->stream));
       a.X = X;
       a.T = T;
+      a.B = Bl;
+      a.J = Jn;
       if (h->NB == 8) launch_harvest<8, true>(lim, nB, h->stream, a);
       else launch_harvest<16, true>(lim, nB, h->stream, a);
       HIPCHK(hipGetLastError());
@@ -595,11 +652,16 @@ extern "C" int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
 }
 
 extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
-                                  double BetaMin) {
-  if (!h || Mode < BATS_SUR_OFF || Mode > BATS_SUR_RUSANOV || !(BetaMin >= 0.0))
+                                  double BetaMin, double JumpTol) {
+  if (!h || Mode < BATS_SUR_OFF || Mode > BATS_SUR_RUSANOV ||
+      !(BetaMin >= 0.0) || !(JumpTol >= 0.0))
     return BATS_EINVAL;
   if (Mode == BATS_SUR_NETWORK && !m) return BATS_EINVAL;
   if (Mode != BATS_SUR_OFF && h->elemSize != 4) return BATS_EINVAL;
+  // the gated face pass carries no adaptive low order weights
+  if (Mode != BATS_SUR_OFF && h->cfg.UseAdaptiveLowOrder &&
+      h->cfg.nLowOrder == 1 && h->cfg.nOrder == 2)
+    return BATS_EINVAL;
   SurrogateState &s = h->sur;
   if (Mode != BATS_SUR_OFF && !s.face) {
     const long nF = (long)(h->NB + 1) * h->NB * h->NB;
@@ -607,7 +669,7 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
     HIPCHK(hipMalloc(&s.face, sizeof(float) * NSLOT * nTot));
     HIPCHK(hipMalloc(&s.rowX, sizeof(float) * 16 * nTot));
     HIPCHK(hipMalloc(&s.rowFace, sizeof(int) * nTot));
-    HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 2 * nTot));
+    HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 3 * nTot));
     HIPCHK(hipMalloc(&s.counters, sizeof(int) * 4));
     s.faceCap = nTot;
   }
@@ -618,6 +680,7 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
   s.model = m;
   s.mode = Mode;
   s.betaMin = BetaMin;
+  s.jumpTol = JumpTol;
   return BATS_OK;
 }
 
@@ -657,6 +720,7 @@ int sur_stage(bats_handle *h, const StageArgs &sa) {
   a.ph = sa.ph;
   a.beta = sa.beta;
   a.betaMin = s.betaMin;
+  a.jumpTol = s.jumpTol;
   a.DtFactor = sa.DtFactor;
   a.lim = sa.lim;
   a.mode = s.mode;

@@ -41,7 +41,7 @@ module ModHipBridge
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
        bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
-       bats_set_amr_grid, &
+       bats_set_amr_grid, bats_set_low_order_face, bats_get_low_order_crit, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
        bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
        bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
@@ -52,6 +52,7 @@ module ModHipBridge
        update_state, &
        calc_timestep, set_global_timestep, advance_explicit, &
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
+       set_low_order_face, &
        init_timestep, get_time
 
   integer(c_int), parameter:: BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, &
@@ -70,6 +71,8 @@ module ModHipBridge
      integer(c_int):: iDevice
      integer(c_int):: UseDivbSource
      integer(c_int):: StatePrecision
+     integer(c_int):: UseAdaptiveLowOrder, nLowOrder
+     real(c_double):: pCritLow, pCritHigh, VelCrit
   end type bats_config
 
   interface
@@ -247,6 +250,20 @@ module ModHipBridge
        real(c_double), intent(in):: CellSize_DB(*)
        integer(c_int), value:: nSlab
      end function bats_set_amr_grid
+     integer(c_int) function bats_set_low_order_face(Handle) &
+          bind(C, name="bats_set_low_order_face")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_set_low_order_face
+     integer(c_int) function bats_get_low_order_crit(Handle, iBlock, &
+          LowOrderCrit_X, LowOrderCrit_Y, LowOrderCrit_Z) &
+          bind(C, name="bats_get_low_order_crit")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock
+       real(c_double), intent(out):: LowOrderCrit_X(*), LowOrderCrit_Y(*), &
+            LowOrderCrit_Z(*)
+     end function bats_get_low_order_crit
      ! learned face-flux surrogate (no counterpart in the reference)
      integer(c_int) function bats_mlp_create(Model) &
           bind(C, name="bats_mlp_create")
@@ -294,37 +311,37 @@ module ModHipBridge
        real(c_float), intent(in):: X(*), T(*)
        integer(c_long), value:: n
      end function bats_mlp_set_data
-     integer(c_int) function bats_mlp_get_data(Model, n, X, T) &
+     integer(c_int) function bats_mlp_get_data(Model, n, X, T, B) &
           bind(C, name="bats_mlp_get_data")
        import:: c_int, c_ptr, c_float, c_long
        type(c_ptr), value:: Model
        integer(c_long), intent(out):: n
-       real(c_float), intent(out):: X(*), T(*)
+       real(c_float), intent(out):: X(*), T(*), B(*)
      end function bats_mlp_get_data
      integer(c_int) function bats_mlp_harvest(Model, Handle, Fraction, &
-          BetaMin, Seed, nAdded) bind(C, name="bats_mlp_harvest")
+          BetaMin, JumpTol, Seed, nAdded) bind(C, name="bats_mlp_harvest")
        import:: c_int, c_ptr, c_double, c_long, c_long_long
        type(c_ptr), value:: Model, Handle
-       real(c_double), value:: Fraction, BetaMin
+       real(c_double), value:: Fraction, BetaMin, JumpTol
        integer(c_long_long), value:: Seed
        integer(c_long), intent(out):: nAdded
      end function bats_mlp_harvest
      integer(c_int) function bats_mlp_train(Model, nEpoch, Batch, Lr0, &
-          HoldFrac, Seed, TrainLoss, HeldOutRelErr_V) &
+          HoldFrac, Standardize, Seed, TrainLoss, HeldOutRelErr_V) &
           bind(C, name="bats_mlp_train")
        import:: c_int, c_ptr, c_double, c_long_long
        type(c_ptr), value:: Model
-       integer(c_int), value:: nEpoch, Batch
+       integer(c_int), value:: nEpoch, Batch, Standardize
        real(c_double), value:: Lr0, HoldFrac
        integer(c_long_long), value:: Seed
-       real(c_double), intent(out):: TrainLoss, HeldOutRelErr_V(9)
+       real(c_double), intent(out):: TrainLoss, HeldOutRelErr_V(18)
      end function bats_mlp_train
      integer(c_int) function bats_set_surrogate(Handle, Model, Mode, &
-          BetaMin) bind(C, name="bats_set_surrogate")
+          BetaMin, JumpTol) bind(C, name="bats_set_surrogate")
        import:: c_int, c_ptr, c_double
        type(c_ptr), value:: Handle, Model
        integer(c_int), value:: Mode
-       real(c_double), value:: BetaMin
+       real(c_double), value:: BetaMin, JumpTol
      end function bats_set_surrogate
      integer(c_int) function bats_surrogate_counts(Handle, nFace, &
           nNetwork) bind(C, name="bats_surrogate_counts")
@@ -430,6 +447,12 @@ contains
          merge(1, 0, DoResChangeOnly), merge(1, 0, DoMonotoneRestrict)), &
          'calc_face_value')
   end subroutine calc_face_value
+  !============================================================================
+  subroutine set_low_order_face
+    ! low order face weights of all blocks (src/ModFaceValue.f90:2699)
+    !--------------------------------------------------------------------------
+    call bats_check(bats_set_low_order_face(hBats), 'set_low_order_face')
+  end subroutine set_low_order_face
   !============================================================================
   subroutine calc_face_flux(DoResChangeOnly, iBlock)
     logical, intent(in):: DoResChangeOnly

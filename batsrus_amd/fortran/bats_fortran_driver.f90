@@ -23,8 +23,12 @@ program bats_fortran_driver
   real(c_double):: LimiterBeta = 1.5, Cfl = 0.8, Gamma = 5.0/3.0
   character(len=10):: TypeLoop = 'blocks'
   logical:: UseDivbSource = .false.
+  logical:: UseAdaptiveLowOrder = .false.
+  integer:: nLowOrder = 1
+  real(c_double):: pCritLow = 2.0, pCritHigh = 1.5, VelCrit = 2.0
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
-       iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource
+       iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
+       UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -62,6 +66,10 @@ program bats_fortran_driver
   Config%cTiny = 1e-6; Config%iDevice = 0
   Config%UseDivbSource = merge(1, 0, UseDivbSource)
   Config%StatePrecision = 8
+  Config%UseAdaptiveLowOrder = merge(1, 0, UseAdaptiveLowOrder)
+  Config%nLowOrder = nLowOrder
+  Config%pCritLow = pCritLow; Config%pCritHigh = pCritHigh
+  Config%VelCrit = VelCrit
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   call sync_state_to_gpu(State_VGB)
@@ -69,6 +77,7 @@ program bats_fortran_driver
   if(TypeLoop == 'blocks')then
      ! time step limit from the initial state (the reference's Dt=0 step)
      call exchange_messages
+     if(UseAdaptiveLowOrder) call set_low_order_face
      call set_stage(nStage)
      do iBlock = 1, nBlock
         call calc_face_value(iBlock, .false., .true.)
@@ -79,6 +88,8 @@ program bats_fortran_driver
         call set_global_timestep(-1.0_c_double, Dt)
         STAGELOOP: do iStage = 1, nStage
            call set_stage(iStage)
+           ! src/ModAdvanceExplicit.f90:87-91
+           if(iStage == 1 .and. UseAdaptiveLowOrder) call set_low_order_face
            do iBlock = 1, nBlock
               call calc_face_value(iBlock, .false., .true.)
               call calc_face_flux(.false., iBlock)

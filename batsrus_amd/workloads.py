@@ -1,0 +1,149 @@
+"""The benchmark workloads and the surrogate's training pipeline.
+
+  shock_tube_64     64 blocks of 8^3, Brio-Wu along x, MC beta 1.5,
+                    Rusanov, 2 stages, CFL 0.8, 20 steps
+  orszag_tang_256   4096 blocks of 16^3 (256^3 cells), Orszag-Tang-z with
+                    seeded phases, Koren (mc3) beta 1.5, HLLD + Linde, 2 stages
+  blast_ensemble    48 independent blast-wave runs, 8..4096 blocks of 8^3
+                    each (seeded log-uniform), MC, HLLD, per-run dt
+  kh_two_level      Kelvin-Helmholtz shear layer on a 28^3 root grid of 8^3
+                    blocks refined inside r < 0.25 of the centre (32,256
+                    blocks), MC beta 1.5, HLLD + Linde, 200 steps
+The surrogate is trained on faces harvested from FP64 runs of the
+Orszag-Tang and ensemble workloads, then runs in the two-level
+Kelvin-Helmholtz workload on FP32 state against the plain FP64 HLLD run.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import ics
+from .amr import AmrGrid, sphere_refinement
+from .core import Mlp, Session
+from .ensemble import blast_ensemble
+from .grid import BlockGrid
+
+OT_SCHEME = dict(flux="hlld", limiter="mc3", beta=1.5, nStage=2, cfl=0.8)
+ENS_SCHEME = dict(flux="hlld", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
+KH_SCHEME = dict(flux="hlld", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
+TUBE_SCHEME = dict(flux="rusanov", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
+
+
+def shock_tube_64():
+    g = BlockGrid(nRoot=(4, 4, 4), nCell=(8, 8, 8), order="morton")
+    return g, ics.brio_wu(g)
+
+
+def orszag_tang_256(nRoot=16):
+    g = BlockGrid(nRoot=(nRoot,) * 3, nCell=(16, 16, 16), order="morton")
+    return g, ics.orszag_tang_z(g)
+
+
+def kh_two_level(nRoot=28, radius=0.25):
+    g = AmrGrid((nRoot,) * 3, sphere_refinement((nRoot,) * 3, radius))
+    return g, ics.kelvin_helmholtz(g)
+
+
+def harvest_training_set(model, n_target, seed=1, betaMin=0.0, jumpTol=0.05,
+                         log=None, ot_root=16, ens_runs=48):
+    """Faces passing the gate, with their FP64 HLLD fluxes, from FP64 runs of
+    the Orszag-Tang and ensemble workloads at several times, until about
+    n_target samples are held."""
+    got = 0
+    t0 = time.perf_counter()
+    sources = []
+    g, U = orszag_tang_256(ot_root)
+    sources.append(("orszag_tang", g, U, OT_SCHEME, None))
+    eg, Ue = blast_ensemble(nRun=ens_runs)
+    sources.append(("blast_ensemble", eg, Ue, ENS_SCHEME, eg.iBlockFirst_R))
+    per = n_target / len(sources)
+    for si, (name, grid, U0, scheme, runs) in enumerate(sources):
+        with Session(grid, **scheme) as s:
+            if runs is not None:
+                s.set_runs(runs)
+            s.upload_soa(U0)
+            s.init_timestep()
+            nsnap = 6
+            # estimate the smooth fraction on the first snapshot
+            frac = 0.05
+            have = 0
+            for k in range(nsnap):
+                want = (per - have) / (nsnap - k)
+                n = model.harvest(s, frac, BetaMin=betaMin, JumpTol=jumpTol,
+                                  seed=seed * 1000 + si * 100 + k)
+                have += n
+                if n > 0:
+                    frac = min(1.0, frac * want / n) if k == 0 else frac
+                s.advance(5)
+            got += have
+            if log:
+                log("harvest %s: %d samples" % (name, have))
+    if log:
+        log("harvest: %d samples in %.1f s" % (got, time.perf_counter() - t0))
+    return got
+
+
+def init_weights(seed=0):
+    """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases, flat
+    layout W1, b1, ..., W4, b4 (27,464 floats)."""
+    dims = (16, 128, 128, 64, 8)
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i in range(4):
+        lim = 1.0 / np.sqrt(dims[i])
+        parts.append(rng.uniform(-lim, lim, dims[i] * dims[i + 1]))
+        parts.append(np.zeros(dims[i + 1]))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def build_surrogate(n_samples=20_000_000, epochs=4, batch=65536, lr=1e-3,
+                    hold=0.1, seed=1, log=None, **harvest_kw):
+    """Harvest n_samples faces and train the network on the GPU; returns the
+    model and a dict of statistics (samples, held-out relative L2 error
+    overall and per output, seconds)."""
+    m = Mlp(init_weights(seed))
+    t0 = time.perf_counter()
+    n = harvest_training_set(m, n_samples, seed=seed, log=log, **harvest_kw)
+    t1 = time.perf_counter()
+    loss, err = m.train(epochs, batch=batch, lr=lr, hold=hold, seed=seed)
+    t2 = time.perf_counter()
+    stats = dict(samples=n, harvest_s=t1 - t0, train_s=t2 - t1,
+                 epochs=epochs, batch=batch, lr=lr, hold=hold,
+                 train_loss=loss, heldout_rel_l2=float(err[0]),
+                 heldout_rel_l2_per_output=[float(e) for e in err[1:]])
+    if log:
+        log("train: %d epochs in %.1f s, held-out rel L2 %.4f" %
+            (epochs, t2 - t1, err[0]))
+    return m, stats
+
+
+def run_kh(grid, U0, steps, precision="f64", model=None, mode="off",
+           betaMin=0.0, jumpTol=0.05, warm=0):
+    """Advance the two-level Kelvin-Helmholtz workload; returns (state,
+    cell-updates/s over the timed steps, network face fraction)."""
+    with Session(grid, precision=precision, **KH_SCHEME) as s:
+        if mode != "off":
+            s.set_surrogate(model, mode, BetaMin=betaMin, JumpTol=jumpTol)
+        s.upload_soa(U0)
+        s.init_timestep()
+        if warm:
+            s.advance(warm)
+        s.synchronize()
+        t = time.perf_counter()
+        s.advance(steps)
+        s.synchronize()
+        el = time.perf_counter() - t
+        U = s.download_soa()
+        frac = None
+        if mode != "off":
+            nf, nn = s.surrogate_counts()
+            frac = nn / nf if nf else 0.0
+    return U, grid.nCellTotal * steps / el, frac
+
+
+def rel_l1(U, ref):
+    """Relative L1 error per variable: sum|U-ref| / sum|ref|."""
+    d = np.abs(U - ref).reshape(U.shape[0], -1).sum(1)
+    return d / np.abs(ref).reshape(U.shape[0], -1).sum(1)

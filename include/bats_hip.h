@@ -81,6 +81,15 @@ typedef struct bats_config {
                            time step in FP64 (no counterpart in the
                            reference, which runs FP64 throughout); the
                            per-block routines need FP64 */
+  /* #ADAPTIVELOWORDER (src/ModFaceValue.f90:214-225): at the first stage of
+   * each step every face gets a low order weight from the normal velocity
+   * jump and the total-pressure ratio of the four cells across it
+   * (set_low_order_face, :2699-2965); with nLowOrder = 1 the second order
+   * face values use the limited slopes times (1 - weight); nLowOrder = 2
+   * leaves the second order scheme unchanged.  pCritLow >= pCritHigh. */
+  int UseAdaptiveLowOrder;
+  int nLowOrder;        /* 1 or 2 */
+  double pCritLow, pCritHigh, VelCrit;
 } bats_config;
 
 typedef struct bats_handle bats_handle;
@@ -119,6 +128,19 @@ int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
 int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const double *CellSize_DB, int nSlab,
                       const int *iSlab_IS);
+
+/* set_low_order_face (src/ModFaceValue.f90:2699-2965) for every block from
+ * the state of the last bats_exchange_messages (ghost cells included): the
+ * weights the per-block bats_calc_face_value applies.  The whole-grid
+ * entry points set them themselves at the first stage of each step.
+ * Needs UseAdaptiveLowOrder, FP64 state and a same-level grid. */
+int bats_set_low_order_face(bats_handle *h);
+/* The low order weights of block iBlock (1-based) as LowOrderCrit_XB
+ * (nI+1, nJ, nK), LowOrderCrit_YB (nI, nJ+1, nK), LowOrderCrit_ZB
+ * (nI, nJ, nK+1), Fortran order (src/ModAdvance.f90 LowOrderCrit_[XYZ]B),
+ * as last set by bats_set_low_order_face or a whole-grid step. */
+int bats_get_low_order_crit(bats_handle *h, int iBlock, double *CritX,
+                            double *CritY, double *CritZ);
 
 /* Host <-> device state in the reference layout State_VGB(nVar, MinI:MaxI,
  * MinJ:MaxJ, MinK:MaxK, nBlock), Fortran order with the variable index
@@ -229,9 +251,12 @@ int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
  * network (16 -> 128 -> 128 -> 64 -> 8, SiLU; FP32 master weights, BF16
  * weights and activations on the matrix cores, FP32 accumulation) maps the
  * left/right primitive face states, rotated to the face normal and scaled
- * by the face's mean density and larger fast speed, to the face's
- * numerical flux in the same frame and scaling (8 values: mass, normal and
- * two tangential momenta, normal and two tangential field, energy).  The
+ * by the face's mean density and larger fast speed (16 inputs: the mean of
+ * the two scaled states and the unit direction of their difference j), to
+ * the correction that takes the face's Rusanov flux to its HLLD (Linde)
+ * flux, in the same frame and scaling and divided by |j| (8 values: mass,
+ * normal and two tangential momenta, normal and two tangential field,
+ * energy).  The
  * parameter vector is flat: W1 (16 x 128, row-major [in][out]), b1, W2
  * (128 x 128), b2, W3 (128 x 64), b3, W4 (64 x 8), b4 -- 27,464 floats. */
 typedef struct bats_mlp bats_mlp;
@@ -247,32 +272,43 @@ int bats_mlp_forward(bats_mlp *m, const float *X, float *Y, int nRow);
 /* mean-square loss of the network on (X, T) and its gradient (27,464) */
 int bats_mlp_loss_grad(bats_mlp *m, const float *X, const float *T, int nRow,
                        double *Loss, float *Grad);
-/* training samples held on the device: inputs n x 16, targets n x 8 */
+/* training samples held on the device: inputs n x 16, targets n x 8 and
+ * the baseline each target corrects (n x 8; zero for bats_mlp_set_data) */
 int bats_mlp_set_data(bats_mlp *m, const float *X, const float *T, long n);
-int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T);
+int bats_mlp_get_data(bats_mlp *m, long *n, float *X, float *T, float *B);
 /* Harvest samples from a solver's current state (FP64 handle): a seeded
- * random fraction of all faces that pass the smoothness gate, each with its
- * FP64 HLLD (Linde) flux as target; appended to the model's samples. */
+ * random fraction of all faces that pass the smoothness gate, each with the
+ * FP64 correction from its Rusanov to its HLLD (Linde) flux as target and
+ * its Rusanov flux as baseline; appended to the model's samples. */
 int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
-                     double BetaMin, unsigned long long Seed, long *nAdded);
+                     double BetaMin, double JumpTol, unsigned long long Seed,
+                     long *nAdded);
 /* Adam with cosine learning-rate decay over nEpoch passes of minibatches;
- * HoldFrac of the samples (seeded split) are held out; returns the last
- * minibatch loss and the held-out relative L2 error ||Y-T||/||T|| overall
- * and per output (9 values). */
+ * HoldFrac of the samples (seeded split) are held out.  With Standardize
+ * the outputs are trained against targets divided by their RMS (the scale
+ * is folded back into the last layer afterwards).  Returns the last
+ * minibatch loss and the held-out relative L2 errors, overall and per
+ * output, with w each sample's target weight (the jump norm |j| for
+ * harvested faces, one otherwise): [0..8] of the predicted flux,
+ * ||w(Y-T)|| / ||B+wT||, and [9..17] of the correction, ||w(Y-T)|| /
+ * ||wT|| (18 values). */
 int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
-                   double HoldFrac, unsigned long long Seed, double *TrainLoss,
-                   double *HeldOutRelErr_V);
+                   double HoldFrac, int Standardize, unsigned long long Seed,
+                   double *TrainLoss, double *HeldOutRelErr_V);
 
 /* Face fluxes of the stage update on an FP32-state handle.  Mode
  * BATS_SUR_OFF: the plain fused stage kernel.  BATS_SUR_NETWORK: faces that
- * pass the smoothness gate (the limiter leaves both cells' slopes unclipped
- * for every variable, and the plasma beta of both states is at least
- * BetaMin) take the network's flux, all others the exact HLLD (Linde) flux.
+ * pass the smoothness gate take the network's flux, all others the exact
+ * HLLD (Linde) flux.  The gate: for every variable and both cells next to
+ * the face, the MC limiter leaves the slope unclipped or the two one-sided
+ * jumps, scaled like the network inputs, add up to at most JumpTol; and the
+ * plasma beta 2p/B^2 of both face states is at least BetaMin.
  * BATS_SUR_RUSANOV: the same gate with the Rusanov flux on the smooth faces
  * (the method without the network).  Every face still carries one flux, so
  * the update stays conservative.  The model must outlive its use. */
 enum { BATS_SUR_OFF = 0, BATS_SUR_NETWORK = 1, BATS_SUR_RUSANOV = 2 };
-int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode, double BetaMin);
+int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode, double BetaMin,
+                       double JumpTol);
 /* faces evaluated and faces given to the network since bats_set_surrogate */
 int bats_surrogate_counts(bats_handle *h, long *nFace, long *nNetwork);
 

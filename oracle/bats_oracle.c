@@ -174,11 +174,102 @@ void ob_limiter(int iLimiter, int lMin, int lMax, double Beta,
 /* ---------------------------------------------------------------------- */
 /* calc_face_value -- src/ModFaceValue.f90:854-2165                        */
 /* ---------------------------------------------------------------------- */
+int ob_low_order_active(const ob_config *c) {
+  return c->UseAdaptiveLowOrder && c->nLowOrder == 1 && c->nOrder == 2;
+}
+
+/* low_order_face_criteria (src/ModFaceValue.f90:2913-2963) for one fluid,
+ * no B0: St[l] are the conserved states (pressure stored) of the cells
+ * iFace-2 .. iFace+1 along the face normal iDim, whose normal velocities
+ * are RhoU/Rho (calc_cell_norm_velocity, :2966-3040, Cartesian) */
+static double low_order_face_criteria(const ob_config *c, int iDim,
+                                      const double *const St[4]) {
+  double pTotal[4], VelMax = 0, VelMin = 0, SoundMin = 0, VelRatio, Crit = 0;
+  int l;
+  for (l = 0; l < 4; l++) {
+    const double *q = St[l];
+    const double Vel = q[OB_RHOUX + iDim] / q[OB_RHO];
+    const double Sound = sqrt(q[OB_P] / q[OB_RHO] * c->Gamma);
+    pTotal[l] = 0 + q[OB_P];
+    if (l == 0) {
+      VelMax = Vel; VelMin = Vel; SoundMin = Sound;
+    } else {
+      VelMax = fmax2(VelMax, Vel);
+      VelMin = fmin2(VelMin, Vel);
+      SoundMin = fmin2(SoundMin, Sound);
+    }
+  }
+  VelRatio = fmax2(0.0, (VelMax - VelMin) / SoundMin);
+  if (VelRatio > c->VelCrit) {
+    double pMax, pMin, pRatio;
+    for (l = 0; l < 4; l++) {
+      const double *q = St[l];
+      pTotal[l] = pTotal[l] + 0.5 * (q[OB_BX] * q[OB_BX] + q[OB_BY] * q[OB_BY] +
+                                     q[OB_BZ] * q[OB_BZ]);
+    }
+    pMax = pTotal[0]; pMin = pTotal[0];
+    for (l = 1; l < 4; l++) {
+      pMax = fmax2(pMax, pTotal[l]);
+      pMin = fmin2(pMin, pTotal[l]);
+    }
+    pRatio = pMax / pMin;
+    if (pRatio >= c->pCritLow)
+      Crit = 1;
+    else if (pRatio < c->pCritHigh)
+      Crit = 0;
+    else
+      Crit = (pRatio - c->pCritHigh) / (c->pCritLow - c->pCritHigh);
+  }
+  return Crit;
+}
+
+void ob_set_low_order_face(const ob_config *c, const double *StateBlk,
+                           double *CritX, double *CritY, double *CritZ) {
+  ob_shape s = shape_of(c);
+  const int nI = s.nI, nJ = s.nJ, nK = s.nK;
+  const double *St[4];
+  int i, j, k, l;
+  for (k = 1; k <= nK; k++)
+    for (j = 1; j <= nJ; j++)
+      for (i = 1; i <= nI + 1; i++) {
+        for (l = 0; l < 4; l++) St[l] = StateBlk + cidx(&s, i - 2 + l, j, k);
+        CritX[(((long)k - 1) * nJ + j - 1) * (nI + 1) + i - 1] =
+            low_order_face_criteria(c, 0, St);
+      }
+  if (c->nDim >= 2)
+    for (k = 1; k <= nK; k++)
+      for (j = 1; j <= nJ + 1; j++)
+        for (i = 1; i <= nI; i++) {
+          for (l = 0; l < 4; l++) St[l] = StateBlk + cidx(&s, i, j - 2 + l, k);
+          CritY[(((long)k - 1) * (nJ + 1) + j - 1) * nI + i - 1] =
+              low_order_face_criteria(c, 1, St);
+        }
+  if (c->nDim >= 3)
+    for (k = 1; k <= nK + 1; k++)
+      for (j = 1; j <= nJ; j++)
+        for (i = 1; i <= nI; i++) {
+          for (l = 0; l < 4; l++) St[l] = StateBlk + cidx(&s, i, j, k - 2 + l);
+          CritZ[(((long)k - 1) * nJ + j - 1) * nI + i - 1] =
+              low_order_face_criteria(c, 2, St);
+        }
+}
+
 void ob_calc_face_value(const ob_config *c, const double *StateBlk,
                         double *LeftX, double *RightX,
                         double *LeftY, double *RightY,
                         double *LeftZ, double *RightZ) {
+  ob_calc_face_value_lo(c, StateBlk, NULL, NULL, NULL, LeftX, RightX, LeftY,
+                        RightY, LeftZ, RightZ);
+}
+
+void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
+                           const double *CritX, const double *CritY,
+                           const double *CritZ, double *LeftX,
+                           double *RightX, double *LeftY, double *RightY,
+                           double *LeftZ, double *RightZ) {
   ob_shape s = shape_of(c);
+  /* LowOrderCrit_I = 0 unless UseLowOrder and nLowOrder == 1 (:2017-2019) */
+  if (!ob_low_order_active(c)) CritX = CritY = CritZ = NULL;
   const int nI = s.nI, nJ = s.nJ, nK = s.nK;
   const int nStencil = c->nOrder; /* :1000 (nOrder /= 5) */
   long n = block_len(&s);
@@ -256,13 +347,18 @@ void ob_calc_face_value(const ob_config *c, const double *StateBlk,
           for (v = 0; v < NV; v++)
             Pen[(l + 1) * NV + v] = Prim[cidx(&s, l, j, k) + v];
         ob_limiter(c->iLimiter, 1, nI + 1, c->LimiterBeta, Pen, -1, dL, dR);
-        for (i = 1; i <= nI + 1; i++)
+        for (i = 1; i <= nI + 1; i++) {
+          const double W =
+              1 - (CritX ? CritX[(((long)k - 1) * nJ + j - 1) * (nI + 1) +
+                                 i - 1]
+                         : 0.0);
           for (v = 0; v < NV; v++) {
             FX(LeftX, i, j, k)[v] =
-                Pen[(i - 1 + 1) * NV + v] + dL[(i - 1 + 1) * NV + v];
+                Pen[(i - 1 + 1) * NV + v] + dL[(i - 1 + 1) * NV + v] * W;
             FX(RightX, i, j, k)[v] =
-                Pen[(i + 1) * NV + v] - dR[(i + 1) * NV + v];
+                Pen[(i + 1) * NV + v] - dR[(i + 1) * NV + v] * W;
           }
+        }
       }
     /* get_facey_second (:2035-2098) */
     if (c->nDim >= 2)
@@ -272,13 +368,18 @@ void ob_calc_face_value(const ob_config *c, const double *StateBlk,
             for (v = 0; v < NV; v++)
               Pen[(l + 1) * NV + v] = Prim[cidx(&s, i, l, k) + v];
           ob_limiter(c->iLimiter, 1, nJ + 1, c->LimiterBeta, Pen, -1, dL, dR);
-          for (j = 1; j <= nJ + 1; j++)
+          for (j = 1; j <= nJ + 1; j++) {
+            const double W =
+                1 - (CritY ? CritY[(((long)k - 1) * (nJ + 1) + j - 1) * nI +
+                                   i - 1]
+                           : 0.0);
             for (v = 0; v < NV; v++) {
               FY(LeftY, i, j, k)[v] =
-                  Pen[(j - 1 + 1) * NV + v] + dL[(j - 1 + 1) * NV + v];
+                  Pen[(j - 1 + 1) * NV + v] + dL[(j - 1 + 1) * NV + v] * W;
               FY(RightY, i, j, k)[v] =
-                  Pen[(j + 1) * NV + v] - dR[(j + 1) * NV + v];
+                  Pen[(j + 1) * NV + v] - dR[(j + 1) * NV + v] * W;
             }
+          }
         }
     /* get_facez_second (:2100-2163) */
     if (c->nDim >= 3)
@@ -288,13 +389,17 @@ void ob_calc_face_value(const ob_config *c, const double *StateBlk,
             for (v = 0; v < NV; v++)
               Pen[(l + 1) * NV + v] = Prim[cidx(&s, i, j, l) + v];
           ob_limiter(c->iLimiter, 1, nK + 1, c->LimiterBeta, Pen, -1, dL, dR);
-          for (k = 1; k <= nK + 1; k++)
+          for (k = 1; k <= nK + 1; k++) {
+            const double W =
+                1 - (CritZ ? CritZ[(((long)k - 1) * nJ + j - 1) * nI + i - 1]
+                           : 0.0);
             for (v = 0; v < NV; v++) {
               FZ(LeftZ, i, j, k)[v] =
-                  Pen[(k - 1 + 1) * NV + v] + dL[(k - 1 + 1) * NV + v];
+                  Pen[(k - 1 + 1) * NV + v] + dL[(k - 1 + 1) * NV + v] * W;
               FZ(RightZ, i, j, k)[v] =
-                  Pen[(k + 1) * NV + v] - dR[(k + 1) * NV + v];
+                  Pen[(k + 1) * NV + v] - dR[(k + 1) * NV + v] * W;
             }
+          }
         }
   }
   free(Prim); free(Pen); free(dL); free(dR);
@@ -1258,6 +1363,23 @@ static int has_kind(const ob_session *s, int b, int kind) {
   return 0;
 }
 
+/* the block's low order weights (X, Y, Z) in the session store */
+static long crit_len(const ob_config *c) {
+  return (long)(c->nI + 1) * c->nJ * c->nK + (long)c->nI * (c->nJ + 1) * c->nK +
+         (long)c->nI * c->nJ * (c->nK + 1);
+}
+static void block_crit(const ob_session *s, int b, const double **X,
+                       const double **Y, const double **Z) {
+  const ob_config *c = &s->c;
+  if (b < 0 || !(ob_low_order_active(c) && s->LowOrderCrit)) {
+    *X = *Y = *Z = NULL;
+    return;
+  }
+  *X = s->LowOrderCrit + crit_len(c) * b;
+  *Y = *X + (long)(c->nI + 1) * c->nJ * c->nK;
+  *Z = *Y + (long)c->nI * (c->nJ + 1) * c->nK;
+}
+
 static int run_blocks(ob_session *s, int iStage, int DoUpdate,
                       int DoCalcTimestep) {
   const ob_config *c = &s->c;
@@ -1265,6 +1387,24 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
   const long len = block_len(&sh);
   const long nCell = (long)c->nI * c->nJ * c->nK;
   int err = OB_OK;
+  if (ob_low_order_active(c) && s->LowOrderCrit && (iStage == 1 || !DoUpdate)) {
+    /* set_low_order_face at the first stage of the step, from the state
+     * and ghost cells at its start (src/ModAdvanceExplicit.f90:87-91); the
+     * initial time step pass stands for a first step with Dt = 0, whose
+     * first stage sets them from the same state */
+    int b;
+#ifdef _OPENMP
+    int ntc = s->nThread > 0 ? s->nThread : omp_get_max_threads();
+#pragma omp parallel for num_threads(ntc) schedule(static)
+#endif
+    for (b = 0; b < s->nBlock; b++) {
+      double *X = s->LowOrderCrit + crit_len(c) * b;
+      double *Y = X + (long)(c->nI + 1) * c->nJ * c->nK;
+      double *Z = Y + (long)c->nI * (c->nJ + 1) * c->nK;
+      ob_set_low_order_face(c, s->State + (long)b * len, X, Y, Z);
+    }
+  }
+  const int UseCrit = 1;
   if (s->nSlab > 0) {
     /* fine blocks next to coarser ones: face fluxes for the correction
      * (src/ModAdvanceExplicit.f90:95-128) */
@@ -1282,7 +1422,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       for (b = 0; b < s->nBlock; b++) {
         if (!has_kind(s, b, 0)) continue;
         const double *cs = s->CellSize_DB + 3L * b;
-        ob_calc_face_value(c, s->State + (long)b * len, w.LX, w.RX, w.LY,
+        const double *CX, *CY, *CZ;
+        block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
+        ob_calc_face_value_lo(c, s->State + (long)b * len, CX, CY, CZ, w.LX,
+                              w.RX, w.LY,
                            w.RY, w.LZ, w.RZ);
         e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                               w.FX, w.FY, w.FZ);
@@ -1312,7 +1455,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       double *Blk = s->State + (long)b * len;
       double *Old = s->StateOld + (long)b * len;
       const double *cs = s->CellSize_DB + 3L * b;
-      ob_calc_face_value(c, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
+      const double *CX, *CY, *CZ;
+      block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
+      ob_calc_face_value_lo(c, Blk, CX, CY, CZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                            w.RZ);
       e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
                             w.FY, w.FZ);
       if (s->nSlab > 0) apply_cons_flux(s, b, w.FX, w.FY, w.FZ);

@@ -65,6 +65,13 @@ typedef struct {
   double DtFixed;
   double cTiny;        /* HLLD degeneracy guard (ModNumConst, absent) */
   int UseDivbSource;   /* 8-wave (Powell) div B source, #DIVB */
+  /* #ADAPTIVELOWORDER (src/ModFaceValue.f90:214-225): per-face weight of
+   * the low order face value from the velocity jump and total-pressure
+   * ratio; with the second order scheme it scales the limited slopes by
+   * (1 - weight) when nLowOrder == 1 */
+  int UseAdaptiveLowOrder;
+  int nLowOrder;       /* 1 or 2 */
+  double pCritLow, pCritHigh, VelCrit;
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */
@@ -94,6 +101,24 @@ void ob_calc_face_value(const ob_config *c, const double *StateBlk,
                         double *LeftX, double *RightX,
                         double *LeftY, double *RightY,
                         double *LeftZ, double *RightZ);
+/* calc_face_value with the low order weights of the block's faces
+ * (LowOrderCrit_[XYZ]B; NULL: all zero), applied as
+ * Left = P(i-1) + dL(i-1)*(1-w), Right = P(i) - dR(i)*(1-w) when the
+ * adaptive low order scheme is on with nLowOrder == 1 and nOrder == 2
+ * (src/ModFaceValue.f90:2017-2027,2082-2092,2147-2157) */
+void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
+                           const double *CritX, const double *CritY,
+                           const double *CritZ, double *LeftX,
+                           double *RightX, double *LeftY, double *RightY,
+                           double *LeftZ, double *RightZ);
+/* set_phys_based_low_order_face (src/ModFaceValue.f90:2839-2911) for one
+ * ghosted block: CritX (nI+1,nJ,nK), CritY (nI,nJ+1,nK), CritZ
+ * (nI,nJ,nK+1), each face from the four cells iFace-2 .. iFace+1 along its
+ * normal through low_order_face_criteria (:2913-2963) */
+void ob_set_low_order_face(const ob_config *c, const double *StateBlk,
+                           double *CritX, double *CritY, double *CritZ);
+/* the adaptive low order weights change the second order face values */
+int ob_low_order_active(const ob_config *c);
 /* calc_face_flux (src/ModFaceFlux.f90:483-977); CellSize = dx,dy,dz */
 int ob_calc_face_flux(const ob_config *c, const double *CellSize,
                       double *LeftX, double *RightX,
@@ -157,6 +182,10 @@ typedef struct {
   int nSlab;
   const int *iSlab_IS;
   double *Corr;
+  /* low order weights of every block's faces, (nfx + nfy + nfz) per block
+   * (X, Y, Z); set at stage 1 of each step when the adaptive low order
+   * scheme is on (src/ModAdvanceExplicit.f90:87-91), else unused */
+  double *LowOrderCrit;
 } ob_session;
 
 /* set_global_timestep (src/ModTimeStepControl.f90:306-524) */

@@ -27,7 +27,10 @@ class ObConfig(C.Structure):
                 ("Gamma", C.c_double), ("pMin", C.c_double),
                 ("UseDbTrick", C.c_int), ("UseDtFixed", C.c_int),
                 ("DtFixed", C.c_double), ("cTiny", C.c_double),
-                ("UseDivbSource", C.c_int)]
+                ("UseDivbSource", C.c_int),
+                ("UseAdaptiveLowOrder", C.c_int), ("nLowOrder", C.c_int),
+                ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
+                ("VelCrit", C.c_double)]
 
 
 class ObSession(C.Structure):
@@ -41,7 +44,8 @@ class ObSession(C.Structure):
                 ("Dt", C.c_double), ("tSimulation", C.c_double),
                 ("nStep", C.c_int), ("nThread", C.c_int),
                 ("nSlab", C.c_int), ("iSlab_IS", C.POINTER(C.c_int)),
-                ("Corr", C.POINTER(C.c_double))]
+                ("Corr", C.POINTER(C.c_double)),
+                ("LowOrderCrit", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -63,6 +67,10 @@ def lib():
                                         C.c_double, dp, dp, dp]
         L.ob_numerical_flux.restype = C.c_int
         L.ob_calc_face_value.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 6
+        L.ob_calc_face_value_lo.argtypes = [C.POINTER(ObConfig), dp] + \
+            [dp] * 9
+        L.ob_set_low_order_face.argtypes = [C.POINTER(ObConfig), dp, dp, dp,
+                                            dp]
         L.ob_calc_face_flux.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 9
         L.ob_calc_face_flux.restype = C.c_int
         L.ob_update_state.argtypes = [C.POINTER(ObConfig), C.c_int,
@@ -99,7 +107,8 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 beta=1.5, flux="rusanov", nStage=2, UseHalfStep=True, cfl=0.8,
                 gamma=5.0 / 3.0, pMin=-1.0, UseDbTrick=False,
                 UseDtFixed=False, DtFixed=0.0, cTiny=1e-6,
-                UseDivbSource=False):
+                UseDivbSource=False, UseAdaptiveLowOrder=False, nLowOrder=1,
+                pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0):
     c = ObConfig()
     c.nI, c.nJ, c.nK, c.nG, c.nDim, c.nOrder = nI, nJ, nK, nG, nDim, nOrder
     c.iLimiter = LIMITERS[limiter] if isinstance(limiter, str) else limiter
@@ -109,6 +118,8 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.pMin, c.UseDbTrick = pMin, int(UseDbTrick)
     c.UseDtFixed, c.DtFixed, c.cTiny = int(UseDtFixed), DtFixed, cTiny
     c.UseDivbSource = int(UseDivbSource)
+    c.UseAdaptiveLowOrder, c.nLowOrder = int(UseAdaptiveLowOrder), nLowOrder
+    c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit
     return c
 
 
@@ -174,12 +185,31 @@ def face_shapes(cfg):
     return ((nK, nJ, nI + 1), (nK, nJ + 1, nI), (nK + 1, nJ, nI))
 
 
-def calc_face_value(cfg, block_state):
+def calc_face_value(cfg, block_state, crit=None):
+    """Face states (LX, RX, LY, RY, LZ, RZ); crit = the block's low order
+    weights (X, Y, Z) from set_low_order_face, or None."""
     blk = np.ascontiguousarray(block_state, dtype=np.float64)
     sx, sy, sz = face_shapes(cfg)
     arrs = [np.zeros(s + (8,)) for s in (sx, sx, sy, sy, sz, sz)]
-    lib().ob_calc_face_value(C.byref(cfg), _dp(blk), *[_dp(a) for a in arrs])
+    if crit is None:
+        lib().ob_calc_face_value(C.byref(cfg), _dp(blk),
+                                 *[_dp(a) for a in arrs])
+    else:
+        cr = [np.ascontiguousarray(a, dtype=np.float64) for a in crit]
+        lib().ob_calc_face_value_lo(C.byref(cfg), _dp(blk),
+                                    *[_dp(a) for a in cr],
+                                    *[_dp(a) for a in arrs])
     return arrs
+
+
+def set_low_order_face(cfg, block_state):
+    """Low order weights of the block's faces, shapes X (nK, nJ, nI+1),
+    Y (nK, nJ+1, nI), Z (nK+1, nJ, nI)."""
+    blk = np.ascontiguousarray(block_state, dtype=np.float64)
+    out = [np.zeros(s) for s in face_shapes(cfg)]
+    lib().ob_set_low_order_face(C.byref(cfg), _dp(blk),
+                                *[_dp(a) for a in out])
+    return out
 
 
 def calc_face_flux(cfg, cell_size, faces):
@@ -257,6 +287,11 @@ class Session:
             s.nSlab = self.nSlab
             s.iSlab_IS = self.slabs.ctypes.data_as(C.POINTER(C.c_int))
             s.Corr = _dp(self.Corr)
+        if cfg.UseAdaptiveLowOrder:
+            sx, sy, sz = face_shapes(cfg)
+            self.LowOrderCrit = np.zeros(
+                grid.nBlock * sum(int(np.prod(a)) for a in (sx, sy, sz)))
+            s.LowOrderCrit = _dp(self.LowOrderCrit)
         self.s = s
 
     def exchange(self):

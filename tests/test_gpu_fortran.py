@@ -18,10 +18,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "batsrus_amd", "fortran", "bats_fortran_driver")
 
 
-@pytest.mark.parametrize("powell", [False, True])
+# adaptive low order faces with thresholds that give the smooth state
+# weights between 0 and 1 on many faces
+LOW = dict(UseAdaptiveLowOrder=True, nLowOrder=1, pCritLow=1.3,
+           pCritHigh=1.0, VelCrit=0.0)
+
+
+@pytest.mark.parametrize("option", ["none", "powell", "loworder"])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
-def test_fortran_driver_matches_oracle(loop, powell):
-    print(f"case fortran_driver loop={loop} powell={powell}", flush=True)
+def test_fortran_driver_matches_oracle(loop, option):
+    print(f"case fortran_driver loop={loop} option={option}", flush=True)
+    powell = option == "powell"
+    low = LOW if option == "loworder" else {}
     if not os.path.exists(EXE):
         pytest.fail("Fortran driver not built (build() compiles it)")
     grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
@@ -34,8 +42,11 @@ def test_fortran_driver_matches_oracle(loop, powell):
             f.write("&RUN nRootI=2, nRootJ=2, nRootK=2, nCell=8, nStep=%d, "
                     "nOrder=2, nStage=2, iLimiter=3, iFlux=3, "
                     "LimiterBeta=1.5, Cfl=0.8, TypeLoop='%s', "
-                    "UseDivbSource=%s /\n"
-                    % (nstep, loop, ".true." if powell else ".false."))
+                    "UseDivbSource=%s%s /\n"
+                    % (nstep, loop, ".true." if powell else ".false.",
+                       (", UseAdaptiveLowOrder=.true., nLowOrder=1, "
+                        "pCritLow=1.3, pCritHigh=1.0, VelCrit=0.0")
+                       if low else ""))
         with open(os.path.join(d, "grid.bin"), "wb") as f:
             f.write(np.ascontiguousarray(grid.iNeighbor_EB, np.int32).tobytes())
             f.write(np.ascontiguousarray(grid.CellSize_DB).tobytes())
@@ -46,7 +57,8 @@ def test_fortran_driver_matches_oracle(loop, powell):
         assert r.returncode == 0, r.stdout + r.stderr
         S = np.fromfile(os.path.join(d, "state_out.bin")).reshape(S0.shape)
     cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3", beta=1.5,
-                                   cfl=0.8, UseDivbSource=powell), grid)
+                                   cfl=0.8, UseDivbSource=powell, **low),
+                              grid)
     o = po.Session(grid, cfg, S0)
     o.init_timestep()
     for _ in range(nstep):

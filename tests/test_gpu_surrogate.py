@@ -33,9 +33,12 @@ def _cf(S, d):
         0.0, (cs2 + va2) ** 2 - 4 * cs2 * vn2))))
 
 
-def _smooth(p0, p1, p2, beta):
+JUMP_TOL = 0.05
+
+
+def _smooth(p0, p1, p2, beta, tol):
     d1, d2 = p1 - p0, p2 - p1
-    return ((d1 == 0) & (d2 == 0)) | (
+    return (np.abs(d1) + np.abs(d2) <= tol) | (
         (d1 * d2 > 0) & (0.5 * np.abs(d1 + d2) <= beta * np.minimum(
             np.abs(d1), np.abs(d2))))
 
@@ -46,8 +49,11 @@ def _rot(d):
 
 
 def _oracle_samples(grid, S, cfg, beta):
-    """All smooth faces of all blocks in the harvest order."""
-    X, T = [], []
+    """All smooth faces of all blocks in the harvest order: inputs, targets
+    (HLLD minus Rusanov flux) and baselines (Rusanov flux), scaled."""
+    X, T, B = [], [], []
+    cfg_r = po.make_config(nI=cfg.nI, nJ=cfg.nJ, nK=cfg.nK, flux="rusanov",
+                           limiter="mc", beta=beta)
     nb = grid.nI
     for b in range(grid.nBlock):
         blk = np.ascontiguousarray(S[b])
@@ -66,28 +72,40 @@ def _oracle_samples(grid, S, cfg, beta):
                     gg = list(g)
                     gg[ax] += s - 2
                     cells.append(P[tuple(gg)])
-                ok = all(_smooth(cells[0][v], cells[1][v], cells[2][v], beta) and
-                         _smooth(cells[1][v], cells[2][v], cells[3][v], beta)
+                l, r = L[idx], R[idx]
+                rhos = 0.5 * (l[0] + r[0])
+                cs = max(_cf(l, d), _cf(r, d))
+                tol = JUMP_TOL * np.array(
+                    [rhos] + [cs] * 3 + [cs * np.sqrt(rhos)] * 3 +
+                    [rhos * cs * cs])
+                ok = all(_smooth(cells[0][v], cells[1][v], cells[2][v], beta,
+                                 tol[v]) and
+                         _smooth(cells[1][v], cells[2][v], cells[3][v], beta,
+                                 tol[v])
                          for v in range(8))
                 if not ok:
                     continue
-                l, r = L[idx], R[idx]
                 _, F = po.numerical_flux(cfg, d, 1.0, l, r)
-                rhos = 0.5 * (l[0] + r[0])
-                cs = max(_cf(l, d), _cf(r, d))
+                _, Fr = po.numerical_flux(cfg_r, d, 1.0, l, r)
                 rot = _rot(d) + [7]
-                x = []
+                sv = []
                 for st in (l, r):
                     v = st[rot]
-                    x += [v[0] / rhos] + list(v[1:4] / cs) + \
-                         list(v[4:7] / (cs * np.sqrt(rhos))) + \
-                         [v[7] / (rhos * cs * cs)]
-                fr = [F[i_] for i_ in _rot(d)] + [F[8]]
-                sc = [rhos * cs] + [rhos * cs * cs] * 3 + \
-                     [cs * cs * np.sqrt(rhos)] * 3 + [rhos * cs ** 3]
+                    sv.append(np.array([v[0] / rhos] + list(v[1:4] / cs) +
+                                       list(v[4:7] / (cs * np.sqrt(rhos))) +
+                                       [v[7] / (rhos * cs * cs)]))
+                jv = sv[1] - sv[0]
+                jn = np.sqrt(jv @ jv)
+                if jn == 0:
+                    continue
+                x = list(0.5 * (sv[0] + sv[1])) + list(jv / jn)
+                rot = _rot(d) + [8]
+                sc = np.array([rhos * cs] + [rhos * cs * cs] * 3 +
+                              [cs * cs * np.sqrt(rhos)] * 3 + [rhos * cs ** 3])
                 X.append(x)
-                T.append(np.array(fr) / np.array(sc))
-    return np.array(X), np.array(T)
+                T.append((F[rot] - Fr[rot]) / (sc * jn))
+                B.append(Fr[rot] / sc)
+    return np.array(X), np.array(T), np.array(B)
 
 
 def test_harvest_matches_oracle():
@@ -98,15 +116,16 @@ def test_harvest_matches_oracle():
     cfg = po.config_from_dict(scheme, grid)
     S = grid.soa_to_state_vgb(U0)
     grid.fill_ghosts(S)
-    Xo, To = _oracle_samples(grid, S, cfg, 1.5)
+    Xo, To, Bo = _oracle_samples(grid, S, cfg, 1.5)
     with Session(grid, **scheme) as s, Mlp(mo.init_params()) as m:
         s.upload_soa(U0)
-        n = m.harvest(s, 1.0, BetaMin=0.0, seed=5)
-        X, T = m.data()
+        n = m.harvest(s, 1.0, BetaMin=0.0, JumpTol=JUMP_TOL, seed=5)
+        X, T, B = m.data()
     print(f"  {n} smooth faces of {3 * 9 * 64 * 2}")
     assert n == len(Xo) and n > 100
     assert np.allclose(X, Xo, rtol=2e-6, atol=2e-6 * np.abs(Xo).max())
-    assert np.allclose(T, To, rtol=1e-5, atol=1e-5 * np.abs(To).max())
+    assert np.allclose(T, To, rtol=1e-4, atol=1e-4 * np.abs(To).max())
+    assert np.allclose(B, Bo, rtol=1e-5, atol=1e-5 * np.abs(Bo).max())
 
 
 def test_all_faces_gated_out_equals_plain_fp32():
@@ -138,7 +157,7 @@ def test_network_in_the_loop_is_conservative():
     U0 = smooth_mhd_soa(grid, seed=63, amp=0.1)
     with Session(grid, flux="hlld", limiter="mc", beta=1.5, cfl=0.05,
                  precision="f32") as s, Mlp(mo.init_params(seed=2)) as m:
-        s.set_surrogate(m, "network", BetaMin=0.0)
+        s.set_surrogate(m, "network", BetaMin=0.0, JumpTol=JUMP_TOL)
         s.upload_soa(U0)
         s.init_timestep()
         t0 = s.integrate_domain()
@@ -165,15 +184,16 @@ def test_trained_network_beats_rusanov_on_smooth_faces():
         s.upload_soa(U0)
         s.init_timestep()
         for k in range(nstep):
-            m.harvest(s, 1.0, BetaMin=0.0, seed=100 + k)
+            m.harvest(s, 1.0, BetaMin=0.0, JumpTol=JUMP_TOL, seed=100 + k)
             s.advance(1)
         ref = s.download_soa()
-        loss, err = m.train(60, batch=8192, lr=2e-3, hold=0.1, seed=1)
-        print(f"  samples {m.data()[0].shape[0]}, held-out rel L2 {err[0]:.4f}")
+        loss, err = m.train(20, batch=1024, lr=2e-3, hold=0.1, seed=1)
+        print(f"  samples {m.data()[0].shape[0]}, held-out rel L2 of the "
+              f"flux {err[0]:.2e}, of the correction {err[9]:.2e}")
         res = {}
         for mode in ("network", "rusanov"):
             with Session(grid, precision="f32", **scheme) as s2:
-                s2.set_surrogate(m, mode, BetaMin=0.0)
+                s2.set_surrogate(m, mode, BetaMin=0.0, JumpTol=JUMP_TOL)
                 s2.upload_soa(U0)
                 s2.init_timestep()
                 s2.advance(nstep)
@@ -181,7 +201,8 @@ def test_trained_network_beats_rusanov_on_smooth_faces():
                 nf, nn = s2.surrogate_counts()
             res[mode] = (np.abs(U - ref).reshape(8, -1).sum(1) /
                          np.abs(ref).reshape(8, -1).sum(1))
-            print(f"  {mode}: rel L1 max {res[mode].max():.2e}, "
-                  f"network faces {nn}/{nf}")
-    assert err[0] < 0.05
+            print(f"  {mode}: rel L1 "
+                  f"{np.array2string(res[mode], formatter={'float': '{:.2e}'.format})}"
+                  f", network faces {nn}/{nf}")
+    assert err[0] < 0.01 and err[9] < 0.5
     assert res["network"].max() < 0.5 * res["rusanov"].max()
