@@ -73,6 +73,7 @@ ABI_FUNCTIONS = {
     "bats_set_grid": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_amr_grid": (C.c_int, [_hp, _ip, _dp, C.c_int, _ip]),
     "bats_set_low_order_face": (C.c_int, [_hp]),
+    "bats_set_step_graph": (C.c_int, [_hp, C.c_int]),
     "bats_get_low_order_crit": (C.c_int, [_hp, C.c_int, _dp, _dp, _dp]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
     "bats_download_state": (C.c_int, [_hp, _dp]),
@@ -284,6 +285,11 @@ class Session:
         self._check(self.lib.bats_calc_face_value(
             self.h, iBlock, int(DoResChangeOnly), int(DoMonotoneRestrict)),
             "calc_face_value")
+
+    def set_step_graph(self, use=True):
+        """bats_advance replays one captured hipGraph per time step."""
+        self._check(self.lib.bats_set_step_graph(self.h, int(bool(use))),
+                    "set_step_graph")
 
     def set_low_order_face(self):
         """set_low_order_face on the last exchanged state (all blocks)."""

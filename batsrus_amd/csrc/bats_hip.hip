@@ -484,6 +484,7 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->corr, h->lowCrit};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  drop_graphs(h);
   for (auto &ev : h->evStart) (void)hipEventDestroy(ev);
   for (auto &ev : h->evStop) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(h->stream);
@@ -494,6 +495,7 @@ extern "C" int bats_finalize(bats_handle *h) {
 extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
                              const double *CellSize_DB) {
   if (!h || !iNeighbor_EB || !CellSize_DB) return BATS_EINVAL;
+  drop_graphs(h);
   const int nB = h->cfg.nBlock;
   for (int t = 0; t < 6 * nB; t++)
     if (iNeighbor_EB[t] < -1 || iNeighbor_EB[t] >= nB) return BATS_EINVAL;
@@ -1065,9 +1067,81 @@ extern "C" int bats_advance_explicit(bats_handle *h, int DoCalcTimestep) {
   return BATS_OK;
 }
 
+// capture one time step (global time step, every stage) on the handle's
+// stream into the graph of the current buffer parity; the host-side buffer
+// bookkeeping the stage launcher did while capturing is undone, since
+// nothing ran yet
+static int capture_step(bats_handle *h, double TimeSimulationLimit) {
+  const int par = h->cur, cur0 = h->cur, buf0 = h->stateBuf;
+  if (h->graphValid[par]) {
+    (void)hipGraphExecDestroy(h->graphExec[par]);
+    h->graphValid[par] = 0;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int st = bats_set_global_timestep(h, TimeSimulationLimit, nullptr);
+  for (int s = 1; !st && s <= h->cfg.nStage; s++)
+    st = launch_stage(h, s, s == h->cfg.nStage, 0, 1, false);
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->cur = cur0;
+  h->stateBuf = buf0;
+  if (st || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st ? st : BATS_EHIP;
+  }
+  const hipError_t ei = hipGraphInstantiate(&h->graphExec[par], g, nullptr,
+                                            nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(ei);
+  h->graphValid[par] = 1;
+  h->graphTLimit[par] = TimeSimulationLimit;
+  return BATS_OK;
+}
+
+extern "C" int bats_set_step_graph(bats_handle *h, int UseGraph) {
+  if (!h || (UseGraph != 0 && UseGraph != 1)) return BATS_EINVAL;
+  drop_graphs(h);
+  h->useGraph = UseGraph;
+  return BATS_OK;
+}
+
 extern "C" int bats_advance(bats_handle *h, int nStep, double TimeSimulationLimit) {
   if (!h || nStep < 0) return BATS_EINVAL;
   h->nTimed = 0;
+  if (h->useGraph && nStep > 0) {
+    // one graph launch per step; the timing is per step (all kernels)
+    // divided by the stages.  The network's weights are synchronized with
+    // its own stream at every eager launch, which a replay cannot do.
+    if (h->sur.mode == BATS_SUR_NETWORK) return BATS_EINVAL;
+    if (h->evStart.empty()) {
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      h->evStart.push_back(e0);
+      h->evStop.push_back(e1);
+    }
+    HIPCHK(hipEventRecord(h->evStart[0], h->stream));
+    for (int n = 0; n < nStep; n++) {
+      const int par = h->cur;
+      if (!h->graphValid[par] || h->graphTLimit[par] != TimeSimulationLimit) {
+        const int st = capture_step(h, TimeSimulationLimit);
+        if (st) return st;
+      }
+      HIPCHK(hipGraphLaunch(h->graphExec[par], h->stream));
+      // the buffer bookkeeping of the step's stages
+      if (h->cfg.nStage == 1) h->cur = 1 - h->cur;
+      h->stateBuf = h->cur;
+    }
+    HIPCHK(hipEventRecord(h->evStop[0], h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->evStart[0], h->evStop[0]));
+    h->lastStageLaunches = nStep * h->cfg.nStage;
+    h->lastStageMs = ms / h->lastStageLaunches;
+    int bits = 0;
+    HIPCHK(hipMemcpy(&bits, h->err, sizeof(int), hipMemcpyDeviceToHost));
+    return err_from_bits(bits);
+  }
   for (int n = 0; n < nStep; n++) {
     int st = bats_set_global_timestep(h, TimeSimulationLimit, nullptr);
     if (st) return st;
@@ -1109,6 +1183,7 @@ extern "C" int bats_get_time(bats_handle *h, double *Dt, double *tSimulation,
 
 extern "C" int bats_set_runs(bats_handle *h, int nRun, const int *iBlockFirst_R) {
   if (!h || nRun < 1 || !iBlockFirst_R) return BATS_EINVAL;
+  drop_graphs(h);
   const int nB = h->cfg.nBlock;
   if (iBlockFirst_R[0] != 0 || iBlockFirst_R[nRun] != nB) return BATS_EINVAL;
   for (int r = 0; r < nRun; r++)
@@ -1159,6 +1234,7 @@ extern "C" int bats_get_run_times(bats_handle *h, double *Dt_R,
 
 extern "C" int bats_set_persistent_grid(bats_handle *h, int nWorkgroup) {
   if (!h || nWorkgroup < 0) return BATS_EINVAL;
+  drop_graphs(h);
   h->nWorkgroupMax = nWorkgroup;
   return BATS_OK;
 }
@@ -1186,6 +1262,7 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
   if (!h || !iNeighbor_EB || !CellSize_DB || nSlab < 0 ||
       (nSlab > 0 && !iSlab_IS))
     return BATS_EINVAL;
+  drop_graphs(h);
   const int nB = h->cfg.nBlock, NB = h->NB, H = NB / 2;
   if (nSlab > 0 && h->cfg.UseDivbSource) return BATS_EINVAL;
   for (int t = 0; t < 3 * nB; t++)

@@ -63,6 +63,12 @@ struct bats_handle {
   int nTimed;
   double lastStageMs;
   int lastStageLaunches;
+  // bats_advance as one captured graph per time step (global time step and
+  // every stage), one per state-buffer parity; captured for tLimit
+  int useGraph;
+  hipGraphExec_t graphExec[2];
+  int graphValid[2];
+  double graphTLimit[2];
   // two-level grids: resolution-change slabs (iNeighbor_EB = -2 - slab)
   int nSlab, nFine;
   std::vector<int> hSlabs;  // 8 ints per slab
@@ -71,6 +77,15 @@ struct bats_handle {
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };
+
+// drop captured step graphs: any change of grid, runs, launch shape, state
+// buffers or surrogate invalidates them
+inline void drop_graphs(bats_handle *h) {
+  for (int p = 0; p < 2; p++) {
+    if (h->graphValid[p]) (void)hipGraphExecDestroy(h->graphExec[p]);
+    h->graphValid[p] = 0;
+  }
+}
 
 inline int hip_status(hipError_t e) {
   return e == hipSuccess ? BATS_OK : BATS_EHIP;

@@ -657,6 +657,7 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
       !(BetaMin >= 0.0) || !(JumpTol >= 0.0))
     return BATS_EINVAL;
   if (Mode == BATS_SUR_NETWORK && !m) return BATS_EINVAL;
+  drop_graphs(h);
   if (Mode != BATS_SUR_OFF && h->elemSize != 4) return BATS_EINVAL;
   // the gated face pass carries no adaptive low order weights
   if (Mode != BATS_SUR_OFF && h->cfg.UseAdaptiveLowOrder &&

@@ -42,6 +42,7 @@ module ModHipBridge
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
        bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
        bats_set_amr_grid, bats_set_low_order_face, bats_get_low_order_crit, &
+       bats_set_step_graph, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
        bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
        bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
@@ -250,6 +251,12 @@ module ModHipBridge
        real(c_double), intent(in):: CellSize_DB(*)
        integer(c_int), value:: nSlab
      end function bats_set_amr_grid
+     integer(c_int) function bats_set_step_graph(Handle, UseGraph) &
+          bind(C, name="bats_set_step_graph")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: UseGraph
+     end function bats_set_step_graph
      integer(c_int) function bats_set_low_order_face(Handle) &
           bind(C, name="bats_set_low_order_face")
        import:: c_int, c_ptr

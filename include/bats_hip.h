@@ -129,6 +129,17 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const double *CellSize_DB, int nSlab,
                       const int *iSlab_IS);
 
+/* With UseGraph = 1, bats_advance captures one time step (the global time
+ * step and every stage's kernels) into a hipGraph on its first step and
+ * replays it for every later step: one graph launch per step instead of
+ * 2-6 kernel launches per stage.  No counterpart in the reference (its
+ * GPU path launches every kernel); for launch-bound small grids and
+ * ensembles.  Recaptured after any grid, run, launch-shape or surrogate
+ * change and when TimeSimulationLimit changes; bats_last_stage_kernel_ms
+ * then reports the whole step's device time divided by nStage.  Not with
+ * the network surrogate (BATS_EINVAL from bats_advance). */
+int bats_set_step_graph(bats_handle *h, int UseGraph);
+
 /* set_low_order_face (src/ModFaceValue.f90:2699-2965) for every block from
  * the state of the last bats_exchange_messages (ghost cells included): the
  * weights the per-block bats_calc_face_value applies.  The whole-grid
