@@ -120,7 +120,9 @@ def test_two_level_grid():
 
 def test_low_order_everywhere_is_the_first_order_run():
     """pCritLow = pCritHigh = 1, VelCrit = 0: every face is low order and
-    the fused kernel reproduces its own first order run bit for bit."""
+    the fused kernel reproduces its own first order run (to rounding: the
+    two kernel variants are compiled separately and may contract
+    multiply-adds differently)."""
     print("case low_order everywhere", flush=True)
     grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8), order="morton")
     U0 = random_mhd_soa(grid, seed=77)
@@ -134,7 +136,9 @@ def test_low_order_everywhere_is_the_first_order_run():
             s.init_timestep()
             s.advance(3)
             out.append(s.download_soa())
-    assert np.array_equal(out[0], out[1])
+    e = rel_err(out[0], out[1])
+    print(f"  rel err {e:.2e}")
+    assert e <= 1e-12
 
 
 def test_fp32_state_close_to_fp64_oracle():
