@@ -536,14 +536,15 @@ class Mlp:
         return n.value
 
     def train(self, nEpoch, batch=65536, lr=1e-3, hold=0.1, seed=1,
-              standardize=False):
+              standardize=False, jump_weight=False):
         """Returns the last minibatch loss and the held-out relative L2
         errors (18: flux overall + 8 outputs, correction overall + 8)."""
         loss = C.c_double()
         err = np.zeros(18)
         self._check(self.lib.bats_mlp_train(self.h, int(nEpoch), int(batch),
                                             float(lr), float(hold),
-                                            int(standardize), int(seed),
+                                            int(standardize) |
+                                            2 * int(jump_weight), int(seed),
                                             C.byref(loss), _ptr(err)),
                     "mlp_train")
         return loss.value, err

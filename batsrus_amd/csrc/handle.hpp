@@ -21,9 +21,10 @@ struct SurrogateState {
   float *rowX;         // network inputs of the gated faces (rows x 16)
   float *rowY;         // network outputs (rows x 8)
   int *rowFace;        // per row: global face index
-  float *rowScale;     // per row: density scale, speed scale
-  int *counters;       // [0] rows this stage, [1] rows since set_surrogate,
-                       // [2] faces since set_surrogate
+  float *rowScale;     // per row (float4): density scale, speed scale,
+                       // jump norm, face area
+  int *counters;       // [0] rows this stage; 64-bit at [2] faces and at
+                       // [4] network rows since set_surrogate
   long faceCap;
 };
 

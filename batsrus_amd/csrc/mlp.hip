@@ -93,18 +93,21 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 }
 
 // dY = bf16(2 (Y - T) / (8 B)); per-row squared error sums
-__global__ void k_loss(const float *Y, const float *T, int B, __bf16 *dY,
-                       float *rowErr) {
+// loss (1/(8B)) sum_r W_r |Y_r - T_r|^2 (W null: all one), its gradient
+// dY and the per-row squared errors W_r |Y_r - T_r|^2
+__global__ void k_loss(const float *Y, const float *T, const float *W, int B,
+                       __bf16 *dY, float *rowErr) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= B) return;
   float s = 0.f;
-  const float sc = 2.0f / (8.0f * (float)B);
+  const float wr = W ? W[row] : 1.0f;
+  const float sc = 2.0f * wr / (8.0f * (float)B);
   for (int j = 0; j < 8; j++) {
     const float d = Y[row * 8 + j] - T[row * 8 + j];
     s += d * d;
     dY[row * 8 + j] = (__bf16)(sc * d);
   }
-  rowErr[row] = s;
+  rowErr[row] = wr * s;
 }
 
 // partial column sums of a BF16 [rows x N] matrix over row chunks
@@ -199,31 +202,38 @@ __global__ void k_pack_frag(const float *w, __bf16 *frag) {
 
 namespace {
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kInferThreads)
 k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
         int rows) {
   __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
-  __shared__ __attribute__((aligned(16))) __bf16 sAct[4][16 * ACT_LD];
-  for (int t = threadIdx.x; t < NFRAG / 8; t += 256)
+  __shared__ __attribute__((aligned(16))) __bf16 sAct[kInferWaves][16 * ACT_LD];
+  __shared__ float sBias[NBIAS];
+  for (int t = threadIdx.x; t < NFRAG / 8; t += kInferThreads)
     reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
+  load_biases(w, sBias);
   __syncthreads();
   const int wave = threadIdx.x >> 6;
   const int nTile = (rows + 15) / 16;
   auto out = [&](int row, int col, float v) { Y[(long)row * 8 + col] = v; };
-  for (int tile = blockIdx.x * 4 + wave; tile < nTile; tile += gridDim.x * 4)
-    infer_tile(sW, w, sAct[wave], X, rows, tile * 16, out);
+  for (int tile = blockIdx.x * kInferWaves + wave; tile < nTile;
+       tile += gridDim.x * kInferWaves)
+    infer_tile(sW, sBias, sAct[wave], X, rows, tile * 16, out);
 }
 
 // minibatch gather: rows idx[0..B) of the dataset into X (BF16) and the
 // standardized targets T / scale
+// a minibatch: inputs (BF16), targets divided by scale, and (W non-null)
+// the loss weights dataJ / meanJ
 __global__ void k_gather_rows(const float *dataX, const float *dataT,
                               const int *idx, int B, __bf16 *X, float *T,
-                              const float *scale) {
+                              const float *scale, const float *dataJ,
+                              float invMeanJ, float *W) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B) return;
   const long s = idx[t];
   for (int j = 0; j < 16; j++) X[t * 16 + j] = (__bf16)dataX[s * 16 + j];
   for (int j = 0; j < 8; j++) T[t * 8 + j] = dataT[s * 8 + j] / scale[j];
+  if (W) W[t] = dataJ[s] * invMeanJ;
 }
 __global__ void k_gather_rows_f32(const float *dataX, const float *dataT,
                                   const float *dataB, const float *dataJ,
@@ -254,6 +264,7 @@ struct bats_mlp {
   int cap;
   __bf16 *X, *A1, *A2, *A3, *dZ1, *dZ2, *dZ3, *dY;
   float *Z1, *Z2, *Z3, *Y, *T, *rowErr, *partial;
+  float *rowW;               // loss weights of the minibatch rows or null
   int nSplit;
   // dataset (device): inputs (n x 16), targets (n x 8) and the baseline
   // flux the target corrects (n x 8, zero unless harvested), FP32
@@ -315,11 +326,13 @@ extern "C" int bats_mlp_create(bats_mlp **out) {
 
 static void free_batch(bats_mlp *m) {
   void *p[] = {m->X, m->A1, m->A2, m->A3, m->dZ1, m->dZ2, m->dZ3, m->dY,
-               m->Z1, m->Z2, m->Z3, m->Y, m->T, m->rowErr, m->partial};
+               m->Z1, m->Z2, m->Z3, m->Y, m->T, m->rowErr, m->partial,
+               m->rowW};
   for (void *q : p)
     if (q) (void)hipFree(q);
   m->X = m->A1 = m->A2 = m->A3 = m->dZ1 = m->dZ2 = m->dZ3 = m->dY = nullptr;
   m->Z1 = m->Z2 = m->Z3 = m->Y = m->T = m->rowErr = m->partial = nullptr;
+  m->rowW = nullptr;
   m->cap = 0;
 }
 
@@ -376,6 +389,7 @@ static int ensure_batch(bats_mlp *m, int B) {
   MCHK(hipMalloc(&m->Y, sizeof(float) * b * 8));
   MCHK(hipMalloc(&m->T, sizeof(float) * b * 8));
   MCHK(hipMalloc(&m->rowErr, sizeof(float) * b));
+  MCHK(hipMalloc(&m->rowW, sizeof(float) * b));
   m->nSplit = (B + kChunkRows - 1) / kChunkRows;
   MCHK(hipMalloc(&m->partial, sizeof(float) * (size_t)m->nSplit * NPARAM));
   m->cap = B;
@@ -422,10 +436,11 @@ static void forward_batch(bats_mlp *m, int B) {
 
 // gradient of the mean-square loss of the minibatch (X, T already on the
 // device) into m->grad and, if lr > 0, one Adam step; rowErr filled
-static int grad_step(bats_mlp *m, int B, float lr, bool update) {
+static int grad_step(bats_mlp *m, int B, float lr, bool update,
+                     const float *W = nullptr) {
   forward_batch(m, B);
   hipLaunchKernelGGL(k_loss, dim3((B + 255) / 256), dim3(256), 0, m->stream,
-                     m->Y, m->T, B, m->dY, m->rowErr);
+                     m->Y, m->T, W, B, m->dY, m->rowErr);
   const __bf16 *Ain[4] = {m->X, m->A1, m->A2, m->A3};
   __bf16 *dZ[4] = {m->dZ1, m->dZ2, m->dZ3, m->dY};
   const float *Zs[3] = {m->Z1, m->Z2, m->Z3};
@@ -520,9 +535,9 @@ int bats::mlp::forward_device(bats_mlp *m, const float *dX, float *dY,
   if (st) return st;
   MCHK(hipStreamSynchronize(m->stream));
   const int nTile = (rows + 15) / 16;
-  int grid = (nTile + 3) / 4;
-  if (grid > 2048) grid = 2048;
-  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(256), 0, stream, m->frag, m->w,
+  int grid = (nTile + kInferWaves - 1) / kInferWaves;
+  if (grid > 256) grid = 256;
+  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(kInferThreads), 0, stream, m->frag, m->w,
                      dX, dY, rows);
   MCHK(hipGetLastError());
   return BATS_OK;
@@ -635,9 +650,13 @@ static void shuffle(std::vector<int> &v, uint64_t &s) {
 }
 
 extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
-                              double HoldFrac, int Standardize,
+                              double HoldFrac, int Flags,
                               unsigned long long Seed, double *TrainLoss,
                               double *HeldOutRelErr_V) {
+  const bool Standardize = (Flags & BATS_MLP_STANDARDIZE) != 0;
+  const bool JumpWeight = (Flags & BATS_MLP_JUMP_WEIGHT) != 0;
+  if (Flags & ~(BATS_MLP_STANDARDIZE | BATS_MLP_JUMP_WEIGHT))
+    return BATS_EINVAL;
   if (!m || nEpoch < 0 || Batch <= 0 || !(Lr0 >= 0.0) || HoldFrac < 0.0 ||
       HoldFrac >= 1.0 || m->nData < 2)
     return BATS_EINVAL;
@@ -692,6 +711,18 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
     hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0,
                        m->stream, m->w, m->wb, (long)NPARAM);
   }
+  // jump weights: each sample's squared error weighted by its jump norm
+  // over the mean jump norm of the training samples
+  float invMeanJ = 1.0f;
+  if (JumpWeight) {
+    std::vector<float> hj(n);
+    MCHK(hipStreamSynchronize(m->stream));
+    MCHK(hipMemcpy(hj.data(), m->dataJ, sizeof(float) * n,
+                   hipMemcpyDeviceToHost));
+    double sj = 0.0;
+    for (long i = 0; i < nTrain; i++) sj += hj[train[i]];
+    invMeanJ = sj > 0.0 ? (float)((double)nTrain / sj) : 1.0f;
+  }
   const long stepsPerEpoch = nTrain / B;
   const long total = stepsPerEpoch * nEpoch;
   long step = 0;
@@ -706,8 +737,9 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
       const double lr = Lr0 * 0.5 * (1.0 + cos(M_PI * (double)step / (double)total));
       hipLaunchKernelGGL(k_gather_rows, dim3((B + 255) / 256), dim3(256), 0,
                          m->stream, m->dataX, m->dataT, dIdx + b * B, B, m->X,
-                         m->T, dScale);
-      st = grad_step(m, B, (float)lr, true);
+                         m->T, dScale, m->dataJ, invMeanJ,
+                         JumpWeight ? m->rowW : nullptr);
+      st = grad_step(m, B, (float)lr, true, JumpWeight ? m->rowW : nullptr);
       if (st) { (void)hipFree(dIdx); return st; }
       if (b == stepsPerEpoch - 1) {  // loss of the epoch's last minibatch
         MCHK(hipMemcpyAsync(e.data(), m->rowErr, sizeof(float) * B,

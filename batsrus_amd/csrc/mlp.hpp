@@ -43,13 +43,28 @@ constexpr int FT3 = FT2 + (KP[2] / 32) * (NP[2] / 16);
 constexpr int NTILE = FT3 + (KP[3] / 32) * (NP[3] / 16);   // 58
 constexpr int FT[NL] = {FT0, FT1, FT2, FT3};
 constexpr int NFRAG = NTILE * TILE;         // 29696 bf16 = 58 KiB
+// the biases packed for the inference kernel's LDS copy
+constexpr int CB1 = 0, CB2 = 128, CB3 = 256, CB4 = 320, NBIAS = 328;
+
+// copy the four bias vectors of the flat parameter vector w into sBias
+__device__ __forceinline__ void load_biases(const float *w, float *sBias) {
+  for (int t = threadIdx.x; t < NBIAS; t += blockDim.x) {
+    const int o = t < CB2 ? OB1 + t
+                : t < CB3 ? OB2 + (t - CB2)
+                : t < CB4 ? OB3 + (t - CB3)
+                          : OB4 + (t - CB4);
+    sBias[t] = w[o];
+  }
+}
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// v_exp_f32 and v_rcp_f32 (about 1 ulp each) instead of the IEEE division
+// sequence: the activation is the inference kernel's VALU cost
 __device__ __forceinline__ float sigmoid(float z) {
-  return 1.0f / (1.0f + __expf(-z));
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
 }
 __device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z) {
@@ -62,9 +77,13 @@ __device__ __forceinline__ float dsilu(float z) {
 // BF16 weights (flat FP32 biases); a workgroup walks 64-row groups
 // =====================================================================
 constexpr int ACT_LD = 128 + 8;  // bf16 row stride of the activation tile
+// inference workgroups: 16 waves share one LDS copy of the weights (58 KiB
+// + 16 activation tiles of 4.3 KiB), 4 waves per SIMD
+constexpr int kInferThreads = 1024, kInferWaves = kInferThreads / 64;
 
 // one 16-row tile through the network; X rows are FP32, Y rows FP32.  sW:
-// fragments in LDS, sAct: this wave's 16 x ACT_LD activation tile
+// fragments in LDS, bias: the packed biases (load_biases) in LDS, sAct:
+// this wave's 16 x ACT_LD activation tile
 template <typename OUT>
 __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
                                            __bf16 *sAct, const float *X,
@@ -78,20 +97,24 @@ __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
     const int k = kq + t, row = r0 + r;
     x[t] = (k < 16 && row < rows) ? (__bf16)X[(long)row * 16 + k] : (__bf16)0.f;
   }
+  // accumulators start at the bias of their column (lane l holds column
+  // l%16 of each 16-column block)
+  auto bias4 = [&](const float *b, int nb) {
+    const float v = b[nb * 16 + r];
+    return f32x4{v, v, v, v};
+  };
   f32x4 acc[8];
 #pragma unroll
-  for (int nb = 0; nb < 8; nb++) {
-    acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    acc[nb] = mfma(x, W[(FT0 + nb) * 64 + lane], acc[nb]);
-  }
-  auto store_act = [&](f32x4 *a, int nNb, const float *b) {
+  for (int nb = 0; nb < 8; nb++)
+    acc[nb] = mfma(x, W[(FT0 + nb) * 64 + lane], bias4(bias + CB1, nb));
+  auto store_act = [&](f32x4 *a, int nNb) {
 #pragma unroll
     for (int nb = 0; nb < 8; nb++) {
       if (nb >= nNb) break;
       const int n = nb * 16 + r;
 #pragma unroll
       for (int i = 0; i < 4; i++)
-        sAct[(4 * (lane >> 4) + i) * ACT_LD + n] = (__bf16)silu(a[nb][i] + b[n]);
+        sAct[(4 * (lane >> 4) + i) * ACT_LD + n] = (__bf16)silu(a[nb][i]);
     }
   };
   auto load_act = [&](int kb) {
@@ -99,7 +122,7 @@ __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
         reinterpret_cast<const bf16x8 *>(sAct + r * ACT_LD + kb * 32 + kq);
     return *p;
   };
-  store_act(acc, 8, bias + OB1);
+  store_act(acc, 8);
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS writes done
   __builtin_amdgcn_wave_barrier();
   // layer 2: 128 -> 128
@@ -108,14 +131,14 @@ __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
   for (int kb = 0; kb < 4; kb++) a4[kb] = load_act(kb);
 #pragma unroll
   for (int nb = 0; nb < 8; nb++) {
-    acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    acc[nb] = bias4(bias + CB2, nb);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++)
       acc[nb] = mfma(a4[kb], W[(FT1 + kb * 8 + nb) * 64 + lane], acc[nb]);
   }
   __builtin_amdgcn_s_waitcnt(0xc07f);
   __builtin_amdgcn_wave_barrier();
-  store_act(acc, 8, bias + OB2);
+  store_act(acc, 8);
   __builtin_amdgcn_s_waitcnt(0xc07f);
   __builtin_amdgcn_wave_barrier();
   // layer 3: 128 -> 64
@@ -123,14 +146,14 @@ __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
   for (int kb = 0; kb < 4; kb++) a4[kb] = load_act(kb);
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
-    acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    acc[nb] = bias4(bias + CB3, nb);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++)
       acc[nb] = mfma(a4[kb], W[(FT2 + kb * 4 + nb) * 64 + lane], acc[nb]);
   }
   __builtin_amdgcn_s_waitcnt(0xc07f);
   __builtin_amdgcn_wave_barrier();
-  store_act(acc, 4, bias + OB3);
+  store_act(acc, 4);
   __builtin_amdgcn_s_waitcnt(0xc07f);
   __builtin_amdgcn_wave_barrier();
   // layer 4: 64 -> 8 (padded to 16 columns)
@@ -145,7 +168,7 @@ __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int row = r0 + 4 * (lane >> 4) + i;
-      if (row < rows) out(row, r, y[i] + bias[OB4 + r]);
+      if (row < rows) out(row, r, y[i] + bias[CB4 + r]);
     }
   }
 }

@@ -72,6 +72,11 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   return z ^ (z >> 31);
 }
 
+// Faces whose scaled jump |j| is below this are not harvested: their target
+// (F_HLLD - F_Rusanov)/|j| would carry the FP64 rounding of the two fluxes
+// magnified by 1/|j|, and their correction is negligible anyway.
+constexpr double kHarvestJumpMin = 1e-6;
+
 template <typename R>
 __device__ __forceinline__ bool var_smooth(R p0, R p1, R p2, R beta,
                                            R tol) {
@@ -227,7 +232,7 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
         if (dir == 0) ok = face_inputs<0>(a.ph, L, R, x, rhos, cs, jn);
         else if (dir == 1) ok = face_inputs<1>(a.ph, L, R, x, rhos, cs, jn);
         else ok = face_inputs<2>(a.ph, L, R, x, rhos, cs, jn);
-        if (ok && cs > 0.0 && jn > 0.0 &&
+        if (ok && cs > 0.0 && jn >= kHarvestJumpMin &&
             face_smooth(q[0], q[1], q[2], q[3], L, R, a.beta, a.betaMin,
                         a.jumpTol, rhos, cs)) {
           double F[NFLUX], Fr[NFLUX], vdt, unA;
@@ -324,12 +329,18 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
   int err = 0;
   long nNet = 0;
   const long nTot = a.nFaceTot;
+  // per-wave row counts and the workgroup's base row, double-buffered by
+  // iteration so a wave can start the next iteration while another still
+  // reads this one's
+  __shared__ int sCountBuf[2][5];
+  int par = 0;
   for (long gf = blockIdx.x * (long)blockDim.x + threadIdx.x;
-       gf - threadIdx.x < nTot; gf += (long)gridDim.x * blockDim.x) {
+       gf - threadIdx.x < nTot; gf += (long)gridDim.x * blockDim.x, par ^= 1) {
+    int *sCount = sCountBuf[par];
     const bool valid = gf < nTot;
     bool net = false;
     float x[16];
-    float rhos = 0.f, cs = 0.f, jn = 0.f;
+    float rhos = 0.f, cs = 0.f, jn = 0.f, area = 0.f;
     if (valid) {
       const int dir = (int)(gf / ((long)a.nBlock * G::NF));
       const long rem = gf % ((long)a.nBlock * G::NF);
@@ -346,8 +357,8 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
       float L[NVAR], R[NVAR];
       face_states<LIM>(beta, q[0], q[1], q[2], q[3], L, R);
       const double *cz = a.cellsize + 3 * b;
-      const float area = float(dir == 0 ? cz[1] * cz[2]
-                               : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
+      area = float(dir == 0 ? cz[1] * cz[2]
+                            : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
       bool ok;
       if (dir == 0) ok = face_inputs<0>(ph, L, R, x, rhos, cs, jn);
       else if (dir == 1) ok = face_inputs<1>(ph, L, R, x, rhos, cs, jn);
@@ -391,21 +402,32 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
         for (int v = 0; v < NSLOT; v++) a.face[v * nTot + gf] = F[v];
       }
     }
-    // wave-aggregated row allocation for the network faces
+    // workgroup-aggregated row allocation for the network faces: one
+    // atomic per workgroup and iteration (the loop trip count is uniform
+    // over the workgroup)
     const unsigned long long mask = __ballot(net);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sCount[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < 4; w++) tot += sCount[w];
+      sCount[4] = tot ? atomicAdd(&a.counters[0], tot) : 0;
+    }
+    __syncthreads();
     if (mask) {
-      const int lane = threadIdx.x & 63;
-      const int leader = __ffsll((long long)mask) - 1;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&a.counters[0], __popcll(mask));
-      base = __shfl(base, leader, 64);
+      int base = sCount[4];
+      for (int w = 0; w < wave; w++) base += sCount[w];
       if (net) {
         const int row = base + __popcll(mask & ((1ull << lane) - 1ull));
-        for (int c = 0; c < 16; c++) a.rowX[(long)row * 16 + c] = x[c];
+        // 16-byte stores: the row's 64 bytes of inputs, then its scales
+        float4 *xr = reinterpret_cast<float4 *>(a.rowX + (long)row * 16);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          xr[c] = make_float4(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
         a.rowFace[row] = (int)gf;
-        a.rowScale[3 * row] = rhos;
-        a.rowScale[3 * row + 1] = cs;
-        a.rowScale[3 * row + 2] = jn;
+        reinterpret_cast<float4 *>(a.rowScale)[row] =
+            make_float4(rhos, cs, jn, area);
       }
       nNet += net ? 1 : 0;
     }
@@ -416,14 +438,16 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
 
 // the network on the compacted rows; fluxes back to the face array
 template <int NB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(bats::mlp::kInferThreads)
 k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
   using namespace bats::mlp;
   using G = FaceGeom<NB>;
   __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
-  __shared__ __attribute__((aligned(16))) __bf16 sAct[4][16 * ACT_LD];
-  for (int t = threadIdx.x; t < NFRAG / 8; t += 256)
+  __shared__ __attribute__((aligned(16))) __bf16 sAct[kInferWaves][16 * ACT_LD];
+  __shared__ float sBias[NBIAS];
+  for (int t = threadIdx.x; t < NFRAG / 8; t += kInferThreads)
     reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
+  load_biases(w, sBias);
   __syncthreads();
   const int rows = a.counters[0];
   const int wave = threadIdx.x >> 6;
@@ -431,18 +455,16 @@ k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
   auto out = [&](int row, int c, float y) {
     const int gf = a.rowFace[row];
     const int dir = gf / (a.nBlock * G::NF);
-    const int b = (gf % (a.nBlock * G::NF)) / G::NF;
-    const double *cz = a.cellsize + 3 * b;
-    const float area = float(dir == 0 ? cz[1] * cz[2]
-                             : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
-    const float v = y * a.rowScale[3 * row + 2] *
-                    out_scale(c, a.rowScale[3 * row], a.rowScale[3 * row + 1]);
+    const float4 sc = reinterpret_cast<const float4 *>(a.rowScale)[row];
+    const float area = sc.w;
+    const float v = y * sc.z * out_scale(c, sc.x, sc.y);
     float &f = a.face[(long)rot_slot(dir, c) * nTot + gf];
     f = f + v * area;
   };
   const int nTile = (rows + 15) / 16;
-  for (int tile = blockIdx.x * 4 + wave; tile < nTile; tile += gridDim.x * 4)
-    infer_tile(sW, w, sAct[wave], a.rowX, rows, tile * 16, out);
+  for (int tile = blockIdx.x * kInferWaves + wave; tile < nTile;
+       tile += gridDim.x * kInferWaves)
+    infer_tile(sW, sBias, sAct[wave], a.rowX, rows, tile * 16, out);
 }
 
 // flux divergence and stage update (FP64), local time step; one workgroup
@@ -549,10 +571,14 @@ k_sur_amr_correct(float *face, long nTot, int nBlock, const int *slabs,
   }
 }
 
+// counters: [0] rows of the current stage, [2..3] running total of faces
+// and [4..5] of network rows (64-bit), [1] unused
+constexpr int NCOUNTER = 6;
 __global__ void k_count_faces(int *counters, long nFace) {
   // rows of this stage -> running totals (single thread)
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    atomicAdd(&counters[1], counters[0]);
+    atomicAdd(reinterpret_cast<unsigned long long *>(counters + 4),
+              (unsigned long long)counters[0]);
     atomicAdd(reinterpret_cast<unsigned long long *>(counters + 2),
               (unsigned long long)nFace);
     counters[0] = 0;
@@ -670,13 +696,13 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
     HIPCHK(hipMalloc(&s.face, sizeof(float) * NSLOT * nTot));
     HIPCHK(hipMalloc(&s.rowX, sizeof(float) * 16 * nTot));
     HIPCHK(hipMalloc(&s.rowFace, sizeof(int) * nTot));
-    HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 3 * nTot));
-    HIPCHK(hipMalloc(&s.counters, sizeof(int) * 4));
+    HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 4 * nTot));
+    HIPCHK(hipMalloc(&s.counters, sizeof(int) * NCOUNTER));
     s.faceCap = nTot;
   }
   if (s.counters) {
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemset(s.counters, 0, sizeof(int) * 4));
+    HIPCHK(hipMemset(s.counters, 0, sizeof(int) * NCOUNTER));
   }
   s.model = m;
   s.mode = Mode;
@@ -687,15 +713,16 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
 
 extern "C" int bats_surrogate_counts(bats_handle *h, long *nFace, long *nNetwork) {
   if (!h) return BATS_EINVAL;
-  int c[4] = {0, 0, 0, 0};
+  int c[NCOUNTER] = {0, 0, 0, 0, 0, 0};
   if (h->sur.counters) {
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(c, h->sur.counters, sizeof c, hipMemcpyDeviceToHost));
   }
-  unsigned long long f = 0;
+  unsigned long long f = 0, n = 0;
   memcpy(&f, c + 2, sizeof f);
+  memcpy(&n, c + 4, sizeof n);
   if (nFace) *nFace = (long)f;
-  if (nNetwork) *nNetwork = c[1];
+  if (nNetwork) *nNetwork = (long)n;
   return BATS_OK;
 }
 
@@ -748,10 +775,10 @@ int sur_stage(bats_handle *h, const StageArgs &sa) {
   if (s.mode == BATS_SUR_NETWORK) {
     const bats::mlp::Weights wts = bats::mlp::weights(s.model);
     if (h->NB == 8)
-      hipLaunchKernelGGL(k_sur_infer<8>, dim3(1024), dim3(256), 0, h->stream,
+      hipLaunchKernelGGL(k_sur_infer<8>, dim3(256), dim3(bats::mlp::kInferThreads), 0, h->stream,
                          wts.frag, wts.w, a);
     else
-      hipLaunchKernelGGL(k_sur_infer<16>, dim3(1024), dim3(256), 0, h->stream,
+      hipLaunchKernelGGL(k_sur_infer<16>, dim3(256), dim3(bats::mlp::kInferThreads), 0, h->stream,
                          wts.frag, wts.w, a);
   }
   if (h->nSlab > 0) {

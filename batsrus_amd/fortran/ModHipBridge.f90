@@ -334,11 +334,11 @@ module ModHipBridge
        integer(c_long), intent(out):: nAdded
      end function bats_mlp_harvest
      integer(c_int) function bats_mlp_train(Model, nEpoch, Batch, Lr0, &
-          HoldFrac, Standardize, Seed, TrainLoss, HeldOutRelErr_V) &
+          HoldFrac, Flags, Seed, TrainLoss, HeldOutRelErr_V) &
           bind(C, name="bats_mlp_train")
        import:: c_int, c_ptr, c_double, c_long_long
        type(c_ptr), value:: Model
-       integer(c_int), value:: nEpoch, Batch, Standardize
+       integer(c_int), value:: nEpoch, Batch, Flags
        real(c_double), value:: Lr0, HoldFrac
        integer(c_long_long), value:: Seed
        real(c_double), intent(out):: TrainLoss, HeldOutRelErr_V(18)

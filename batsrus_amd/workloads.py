@@ -46,18 +46,35 @@ def kh_two_level(nRoot=28, radius=0.25):
     return g, ics.kelvin_helmholtz(g)
 
 
+def kh_uniform(nRoot=28, seed=7):
+    """A Kelvin-Helmholtz layer like kh_two_level's on a uniform grid of 8^3
+    blocks with its own noise seed (a run of the same family, not the
+    evaluated one)."""
+    g = BlockGrid(nRoot=(nRoot,) * 3, nCell=(8, 8, 8), order="morton")
+    return g, ics.kelvin_helmholtz(g, seed=seed)
+
+
 def harvest_training_set(model, n_target, seed=1, betaMin=0.0, jumpTol=0.05,
-                         log=None, ot_root=16, ens_runs=48):
+                         log=None, ot_root=16, ens_runs=48,
+                         sources=("orszag_tang", "blast_ensemble")):
     """Faces passing the gate, with their FP64 HLLD fluxes, from FP64 runs of
-    the Orszag-Tang and ensemble workloads at several times, until about
-    n_target samples are held."""
+    the named workloads (Orszag-Tang, the blast ensemble and optionally a
+    Kelvin-Helmholtz run of the same family as kh_two_level, "kh_family")
+    at several times, until about n_target samples are held."""
     got = 0
     t0 = time.perf_counter()
+    wanted = tuple(sources)
     sources = []
-    g, U = orszag_tang_256(ot_root)
-    sources.append(("orszag_tang", g, U, OT_SCHEME, None))
-    eg, Ue = blast_ensemble(nRun=ens_runs)
-    sources.append(("blast_ensemble", eg, Ue, ENS_SCHEME, eg.iBlockFirst_R))
+    if "orszag_tang" in wanted:
+        g, U = orszag_tang_256(ot_root)
+        sources.append(("orszag_tang", g, U, OT_SCHEME, None))
+    if "blast_ensemble" in wanted:
+        eg, Ue = blast_ensemble(nRun=ens_runs)
+        sources.append(("blast_ensemble", eg, Ue, ENS_SCHEME,
+                        eg.iBlockFirst_R))
+    if "kh_family" in wanted:
+        g, U = kh_uniform()
+        sources.append(("kh_family", g, U, KH_SCHEME, None))
     per = n_target / len(sources)
     for si, (name, grid, U0, scheme, runs) in enumerate(sources):
         with Session(grid, **scheme) as s:
@@ -99,7 +116,8 @@ def init_weights(seed=0):
 
 
 def build_surrogate(n_samples=20_000_000, epochs=4, batch=65536, lr=1e-3,
-                    hold=0.1, seed=1, log=None, **harvest_kw):
+                    hold=0.1, seed=1, log=None, standardize=False,
+                    jump_weight=True, **harvest_kw):
     """Harvest n_samples faces and train the network on the GPU; returns the
     model and a dict of statistics (samples, held-out relative L2 error
     overall and per output, seconds)."""
@@ -107,12 +125,16 @@ def build_surrogate(n_samples=20_000_000, epochs=4, batch=65536, lr=1e-3,
     t0 = time.perf_counter()
     n = harvest_training_set(m, n_samples, seed=seed, log=log, **harvest_kw)
     t1 = time.perf_counter()
-    loss, err = m.train(epochs, batch=batch, lr=lr, hold=hold, seed=seed)
+    loss, err = m.train(epochs, batch=batch, lr=lr, hold=hold, seed=seed,
+                        standardize=standardize, jump_weight=jump_weight)
     t2 = time.perf_counter()
     stats = dict(samples=n, harvest_s=t1 - t0, train_s=t2 - t1,
                  epochs=epochs, batch=batch, lr=lr, hold=hold,
                  train_loss=loss, heldout_rel_l2=float(err[0]),
-                 heldout_rel_l2_per_output=[float(e) for e in err[1:]])
+                 heldout_rel_l2_per_output=[float(e) for e in err[1:9]],
+                 heldout_correction_rel_l2=float(err[9]),
+                 heldout_correction_rel_l2_per_output=[float(e)
+                                                       for e in err[10:]])
     if log:
         log("train: %d epochs in %.1f s, held-out rel L2 %.4f" %
             (epochs, t2 - t1, err[0]))
@@ -147,3 +169,39 @@ def rel_l1(U, ref):
     """Relative L1 error per variable: sum|U-ref| / sum|ref|."""
     d = np.abs(U - ref).reshape(U.shape[0], -1).sum(1)
     return d / np.abs(ref).reshape(U.shape[0], -1).sum(1)
+
+
+def main(argv=None):
+    """python -m batsrus_amd.workloads [--samples N] [--epochs E]
+    [--steps S]: train the surrogate, then run the two-level
+    Kelvin-Helmholtz workload plain FP64, plain FP32, FP32 with the Rusanov
+    flux on the smooth faces and FP32 with the network, printing the rates
+    and the relative L1 error against the FP64 run."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m batsrus_amd.workloads")
+    ap.add_argument("--samples", type=float, default=2e7)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warm", type=int, default=0)
+    args = ap.parse_args(argv)
+
+    def log(s):
+        print(s, flush=True)
+    m, st = build_surrogate(n_samples=int(args.samples), epochs=args.epochs,
+                            sources=("orszag_tang", "blast_ensemble",
+                                     "kh_family"), log=log)
+    try:
+        g, U0 = kh_two_level()
+        ref, r64, _ = run_kh(g, U0, args.steps, "f64", warm=args.warm)
+        log("f64 plain: %.3e cell-updates/s" % r64)
+        for mode in ("off", "rusanov", "network"):
+            U, r, frac = run_kh(g, U0, args.steps, "f32", model=m,
+                                mode=mode, warm=args.warm)
+            log("f32 %s: %.3e cell-updates/s, network faces %s, rel L1 "
+                "max %.2e" % (mode, r, frac, rel_l1(U, ref).max()))
+    finally:
+        m.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -16,12 +16,26 @@ against the launch durations measured with HIP events on the kernel's own
 stream.  The CPU baseline times the CPU oracle (a C restatement of the
 reference's Fortran path) on a bounded sample of the same workload.
 
+The surrogate leg (on by default, --no-surrogate skips it) runs the
+two-level Kelvin-Helmholtz workload (32,256 blocks of 8^3, 200 steps) three
+ways: plain FP64 HLLD (the judge), FP32 state with the trained network on
+the smooth faces, and FP32 state with the Rusanov flux on the same faces
+(the method without the network); it reports each rate, the network's
+relative L1/L-infinity errors against the FP64 run, and the training
+statistics.  The network is trained first, on the GPU, on faces harvested
+from FP64 runs.
+
 usage: python bench.py [--gpus 1] [--steps 50] [--warmup 3]
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -70,6 +84,111 @@ def cpu_baseline(target_seconds):
                       "%.1f s" % (n, el)}
 
 
+def measure_traffic(timeout=150):
+    """HBM bytes per stage launch: two rocprofv3 --pmc child runs of this
+    script (2 steps, FETCH_SIZE then WRITE_SIZE, each counter in a run of its
+    own), each counter scaled by what it reads for the plain 1.07 GB device
+    copy of the same run.  Returns (bytes per launch, details) or (None,
+    reason)."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    known_kib = 16777216 * 64 / 1024.0        # the copy: 8 x 16.8M doubles
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="bench_pmc_")
+        try:
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d,
+                   "-o", "run", "--", sys.executable,
+                   os.path.abspath(__file__), "--steps", "2", "--warmup",
+                   "0", "--no-cpu", "--no-surrogate", "--no-traffic",
+                   "--copy-calibration"]
+            r = subprocess.run(cmd, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, timeout=timeout)
+            if r.returncode != 0:
+                return None, "%s pass exited with %d" % (counter,
+                                                         r.returncode)
+            files = glob.glob(os.path.join(d, "**",
+                                           "*counter_collection.csv"),
+                              recursive=True)
+            if not files:
+                return None, "%s pass wrote no counters" % counter
+            with open(files[0]) as f:
+                rows = [x for x in csv.DictReader(f)
+                        if x["Counter_Name"] == counter]
+        except subprocess.TimeoutExpired:
+            return None, "%s pass timed out" % counter
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        copy = [float(x["Counter_Value"]) for x in rows
+                if "copyBuffer" in x["Kernel_Name"]
+                and int(x["Grid_Size"]) == 131072]
+        stage = [float(x["Counter_Value"]) for x in rows
+                 if "k_stage" in x["Kernel_Name"]]
+        if not copy or len(stage) < 2:
+            return None, "%s: calibration or stage dispatches missing" % (
+                counter)
+        scale = known_kib / (sum(copy) / len(copy))
+        launches = stage[1:]          # the first is the dt-only pass
+        per[counter] = (scale, sum(launches) / len(launches) * 1024.0 * scale)
+    total = per["FETCH_SIZE"][1] + per["WRITE_SIZE"][1]
+    return total, {"fetch_bytes": per["FETCH_SIZE"][1],
+                   "write_bytes": per["WRITE_SIZE"][1],
+                   "fetch_scale": per["FETCH_SIZE"][0],
+                   "write_scale": per["WRITE_SIZE"][0],
+                   "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                             "4 stage launches, scaled by a 1.07 GB copy"}
+
+
+def surrogate_leg(steps=200, samples=20_000_000, epochs=60):
+    """Train the face-flux network, then run the two-level KH workload in
+    FP64 (plain HLLD) and FP32 with and without the network."""
+    import numpy as np
+    from batsrus_amd import workloads as W
+    sources = ("orszag_tang", "blast_ensemble", "kh_family")
+    m, st = W.build_surrogate(n_samples=samples, epochs=epochs,
+                              batch=65536, lr=1e-3, sources=sources,
+                              jump_weight=True)
+    try:
+        g, U0 = W.kh_two_level()
+        ref, r64, _ = W.run_kh(g, U0, steps, "f64")
+        res = {}
+        for mode in ("network", "rusanov"):
+            U, r, frac = W.run_kh(g, U0, steps, "f32", model=m, mode=mode)
+            l1 = W.rel_l1(U, ref)
+            linf = (np.abs(U - ref).reshape(8, -1).max(1) /
+                    np.abs(ref).reshape(8, -1).max(1))
+            res[mode] = {
+                "value": r, "vs_plain_fp64": r / r64,
+                "hbm_frac": r * 160.0 / 1e9 / HBM_PEAK_GBS,
+                "network_face_fraction": frac,
+                "rel_l1_all_vars": float(np.abs(U - ref).sum() /
+                                         np.abs(ref).sum()),
+                "rel_l1_per_var": [float(x) for x in l1],
+                "rel_linf_per_var": [float(x) for x in linf],
+            }
+    finally:
+        m.close()
+    return {
+        "workload": "two-level Kelvin-Helmholtz: %d blocks of 8^3 (%d "
+                    "cells), fine sphere r<0.25, MC beta=1.5, HLLD+Linde, "
+                    "2-stage, CFL 0.8, %d steps" % (g.nBlock, g.nCellTotal,
+                                                    steps),
+        "unit": "cell-updates/s",
+        "plain_fp64": r64,
+        "surrogate_fp32": res["network"],
+        "without_network_fp32": res["rusanov"],
+        "error_note": "rel_l1 = sum|U-U_fp64| / sum|U_fp64| per conserved "
+                      "variable (rho, rho u, B, p) and over all of them",
+        "training": {"sources": list(sources), "samples": st["samples"],
+                     "epochs": st["epochs"], "batch": st["batch"],
+                     "harvest_s": st["harvest_s"], "train_s": st["train_s"],
+                     "heldout_flux_rel_l2": st["heldout_rel_l2"],
+                     "heldout_correction_rel_l2":
+                         st["heldout_correction_rel_l2"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +196,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that "
+                         "measure HBM bytes per stage launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size "
                          "(1.07 GB, for scaling HBM byte counters)")
@@ -85,6 +208,11 @@ def main():
         print("bench.py: this build runs on one GPU only; multi-GPU sharding "
               "is designed but not built (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    # HBM traffic first, in child processes, before this process touches
+    # the GPU
+    traffic, traffic_info = (None, "skipped") if args.no_traffic else \
+        measure_traffic()
 
     from batsrus_amd import ics
     from batsrus_amd.core import Session
@@ -138,7 +266,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_detail": traffic_info,
             "kernel": "k_stage<16,HLLD> (fused face value + flux + update)",
             "kernel_ms_avg": kms, "launches": nlaunch,
             "bytes_per_launch": ncell * BYTES_PER_CELL_LAUNCH,
@@ -155,6 +284,11 @@ def main():
             s2.synchronize()
         out["copy_calibration_bytes"] = {"read": ncell * 64,
                                          "write": ncell * 64}
+    if not args.no_surrogate:
+        try:
+            out["surrogate"] = surrogate_leg()
+        except Exception as e:            # reported, not fatal to the line
+            out["surrogate"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     else:

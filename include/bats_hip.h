@@ -272,6 +272,8 @@ int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
  * (128 x 128), b2, W3 (128 x 64), b3, W4 (64 x 8), b4 -- 27,464 floats. */
 typedef struct bats_mlp bats_mlp;
 enum { BATS_MLP_NPARAM = 27464, BATS_MLP_NIN = 16, BATS_MLP_NOUT = 8 };
+/* bats_mlp_train flags */
+enum { BATS_MLP_STANDARDIZE = 1, BATS_MLP_JUMP_WEIGHT = 2 };
 int bats_mlp_create(bats_mlp **m);
 int bats_mlp_free(bats_mlp *m);
 /* FP32 master weights in (resets the Adam moments) / out; Grad receives
@@ -295,16 +297,19 @@ int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
                      double BetaMin, double JumpTol, unsigned long long Seed,
                      long *nAdded);
 /* Adam with cosine learning-rate decay over nEpoch passes of minibatches;
- * HoldFrac of the samples (seeded split) are held out.  With Standardize
- * the outputs are trained against targets divided by their RMS (the scale
- * is folded back into the last layer afterwards).  Returns the last
+ * HoldFrac of the samples (seeded split) are held out.  Flags: with
+ * BATS_MLP_STANDARDIZE the outputs are trained against targets divided by
+ * their RMS (the scale is folded back into the last layer afterwards);
+ * with BATS_MLP_JUMP_WEIGHT each sample's squared error is weighted by its
+ * target weight w (the jump norm of a harvested face) over the mean w, so
+ * the loss follows the error of the flux correction w*T itself.  Returns the last
  * minibatch loss and the held-out relative L2 errors, overall and per
  * output, with w each sample's target weight (the jump norm |j| for
  * harvested faces, one otherwise): [0..8] of the predicted flux,
  * ||w(Y-T)|| / ||B+wT||, and [9..17] of the correction, ||w(Y-T)|| /
  * ||wT|| (18 values). */
 int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
-                   double HoldFrac, int Standardize, unsigned long long Seed,
+                   double HoldFrac, int Flags, unsigned long long Seed,
                    double *TrainLoss, double *HeldOutRelErr_V);
 
 /* Face fluxes of the stage update on an FP32-state handle.  Mode

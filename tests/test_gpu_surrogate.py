@@ -96,7 +96,7 @@ def _oracle_samples(grid, S, cfg, beta):
                                        [v[7] / (rhos * cs * cs)]))
                 jv = sv[1] - sv[0]
                 jn = np.sqrt(jv @ jv)
-                if jn == 0:
+                if jn < 1e-6:       # not harvested: rounding-dominated
                     continue
                 x = list(0.5 * (sv[0] + sv[1])) + list(jv / jn)
                 rot = _rot(d) + [8]
