@@ -206,7 +206,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding "
-              "is designed but not built (see DESIGN.md)", file=sys.stderr)
+              "is designed but not built", file=sys.stderr)
         sys.exit(2)
 
     # HBM traffic first, in child processes, before this process touches
