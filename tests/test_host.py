@@ -117,3 +117,44 @@ def test_fill_ghosts_matches_oracle_exchange():
     s = po.Session(g, cfg, g.soa_to_state_vgb(U))
     s.exchange()
     assert np.array_equal(S1, s.State)
+
+
+@pytest.mark.parametrize("field,value", [
+    ("nI", 12), ("nDim", 2), ("nG", 1), ("nBlock", 0), ("nOrder", 3),
+    ("iLimiter", 9), ("iFlux", 0), ("nStage", 3), ("Gamma", 1.0),
+    ("Cfl", 0.0), ("UseDivbSource", 2), ("StatePrecision", 2),
+    ("UseAdaptiveLowOrder", 2)])
+def test_bad_config_rejected_before_device_use(field, value):
+    """Configurations the reference stops on (or this path cannot run) give
+    BATS_EINVAL from bats_init without touching a device."""
+    lib = core.load_library()
+    cfg = core.make_config(BlockGrid(nRoot=(1, 1, 1), nCell=(8, 8, 8)))
+    setattr(cfg, field, value)
+    h = C.c_void_p()
+    assert lib.bats_init(C.byref(cfg), C.byref(h)) == 1, field
+    assert not h.value
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nLowOrder=3), dict(pCritLow=1.2, pCritHigh=1.5),
+    dict(VelCrit=-1.0)])
+def test_bad_low_order_config_rejected(kw):
+    """#ADAPTIVELOWORDER checks (src/ModFaceValue.f90:222-225)."""
+    lib = core.load_library()
+    cfg = core.make_config(BlockGrid(nRoot=(1, 1, 1), nCell=(8, 8, 8)),
+                           UseAdaptiveLowOrder=True, **kw)
+    h = C.c_void_p()
+    assert lib.bats_init(C.byref(cfg), C.byref(h)) == 1
+
+
+def test_null_handle_rejected():
+    lib = core.load_library()
+    d = C.c_double()
+    assert lib.bats_advance(None, 1, -1.0) == 1
+    assert lib.bats_advance_stage(None, 1, 1) == 1
+    assert lib.bats_advance_explicit(None, 1) == 1
+    assert lib.bats_set_global_timestep(None, -1.0, C.byref(d)) == 1
+    assert lib.bats_calc_face_value(None, 1, 0, 1) == 1
+    assert lib.bats_set_low_order_face(None) == 1
+    assert lib.bats_set_step_graph(None, 1) == 1
+    assert lib.bats_set_runs(None, 1, None) == 1
