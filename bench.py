@@ -55,15 +55,13 @@ SCHEME = dict(flux="hlld", limiter="mc3", beta=1.5, nStage=2, cfl=0.8,
               nOrder=2)
 
 
-def cpu_baseline(target_seconds):
-    """Time the CPU oracle on a bounded sample of the same workload
-    (a slab of the 256^3 box), all host threads."""
+def _time_oracle(nroot_z, nthreads, target_seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from batsrus_amd.grid import BlockGrid
     from batsrus_amd import ics
-    nthreads = min(os.cpu_count() or 1, 16)
-    g = BlockGrid(nRoot=(16, 16, 2), nCell=(16, 16, 16), order="morton")
+    g = BlockGrid(nRoot=(16, 16, nroot_z), nCell=(16, 16, 16),
+                  order="morton")
     U = ics.orszag_tang_z(g)
     cfg = po.config_from_dict(dict(SCHEME), g)
     s = po.Session(g, cfg, g.soa_to_state_vgb(U), nThread=nthreads)
@@ -76,12 +74,38 @@ def cpu_baseline(target_seconds):
         el = time.perf_counter() - t0
         if n >= 2 and el >= target_seconds:
             break
+    return g, n, el
+
+
+def cpu_baseline(target_seconds):
+    """Time the CPU oracle on a bounded sample of the same workload
+    (a slab of the 256^3 box) on all host threads (up to 16), and on one
+    thread for a smaller slab."""
+    nthreads = min(os.cpu_count() or 1, 16)
+    g, n, el = _time_oracle(2, nthreads, target_seconds)
+    g1, n1, el1 = _time_oracle(1, 1, target_seconds / 2)
     return {"value": g.nCellTotal * n / el, "unit": "cell-updates/s",
             "cores": nthreads, "kind": "port",
             "sample": "CPU oracle (C restatement of the reference Fortran "
                       "orig path, OpenMP over blocks) on 512 of the 4096 "
                       "blocks (16x16x2 slab, 2.1M cells), %d steps, "
-                      "%.1f s" % (n, el)}
+                      "%.1f s" % (n, el),
+            "single_thread": {"value": g1.nCellTotal * n1 / el1,
+                              "sample": "256 blocks (16x16x1 slab), %d "
+                                        "steps, %.1f s" % (n1, el1)}}
+
+
+def copy_bandwidth(session, nbytes, reps=10):
+    """Achievable HBM bandwidth of a plain device-to-device copy of the
+    state (the library's halo snapshot), read + write bytes per second."""
+    session.exchange_messages()           # allocates the snapshot
+    session.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        session.exchange_messages()
+    session.synchronize()
+    el = time.perf_counter() - t0
+    return 2.0 * nbytes * reps / el / 1e9
 
 
 def measure_traffic(timeout=150):
@@ -234,6 +258,7 @@ def main():
         kms, nlaunch = s.last_stage_kernel_ms()
         dt, tsim, nstep = s.time()
         s.check_nan()
+        copy_gbs = copy_bandwidth(s, ncell * 64)
     value = ncell * args.steps / el
     achieved = ncell * BYTES_PER_CELL_LAUNCH / (kms * 1e-3) / 1e9
     out = {
@@ -271,6 +296,8 @@ def main():
             "kernel": "k_stage<16,HLLD> (fused face value + flux + update)",
             "kernel_ms_avg": kms, "launches": nlaunch,
             "bytes_per_launch": ncell * BYTES_PER_CELL_LAUNCH,
+            "copy_peak_measured": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs,
         },
     }
     if args.copy_calibration:
