@@ -190,9 +190,9 @@ __global__ void k_pack_frag(const float *w, __bf16 *frag) {
   const int tile = t / TILE, lane = (t % TILE) / 8, e = t % 8;
   int l = 0;
   while (l + 1 < NL && tile >= FT[l + 1]) l++;
-  const int lt = tile - FT[l], nNb = NP[l] / 16;
-  const int kb = lt / nNb, nb = lt % nNb;
-  const int k = kb * 32 + 8 * (lane >> 4) + e, n = nb * 16 + (lane & 15);
+  const int lt = tile - FT[l], nKb = KP[l] / 32;
+  const int nb = lt / nKb, kb = lt % nKb;
+  const int k = frag_k(l, kb, lane >> 4, e), n = nb * 16 + (lane & 15);
   const float v = (k < DIM[l] && n < DIM[l + 1])
                       ? w[OW[l] + k * DIM[l + 1] + n] : 0.0f;
   frag[t] = (__bf16)v;
@@ -202,11 +202,10 @@ __global__ void k_pack_frag(const float *w, __bf16 *frag) {
 
 namespace {
 
-__global__ void __launch_bounds__(kInferThreads)
+__global__ void __launch_bounds__(kInferThreads, 8)
 k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
         int rows) {
   __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
-  __shared__ __attribute__((aligned(16))) __bf16 sAct[kInferWaves][16 * ACT_LD];
   __shared__ float sBias[NBIAS];
   for (int t = threadIdx.x; t < NFRAG / 8; t += kInferThreads)
     reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
@@ -217,7 +216,7 @@ k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
   auto out = [&](int row, int col, float v) { Y[(long)row * 8 + col] = v; };
   for (int tile = blockIdx.x * kInferWaves + wave; tile < nTile;
        tile += gridDim.x * kInferWaves)
-    infer_tile(sW, sBias, sAct[wave], X, rows, tile * 16, out);
+    infer_tile(sW, sBias, X, rows, tile * 16, out);
 }
 
 // minibatch gather: rows idx[0..B) of the dataset into X (BF16) and the
@@ -536,7 +535,7 @@ int bats::mlp::forward_device(bats_mlp *m, const float *dX, float *dY,
   MCHK(hipStreamSynchronize(m->stream));
   const int nTile = (rows + 15) / 16;
   int grid = (nTile + kInferWaves - 1) / kInferWaves;
-  if (grid > 256) grid = 256;
+  if (grid > kInferGrid) grid = kInferGrid;
   hipLaunchKernelGGL(k_infer, dim3(grid), dim3(kInferThreads), 0, stream, m->frag, m->w,
                      dX, dY, rows);
   MCHK(hipGetLastError());

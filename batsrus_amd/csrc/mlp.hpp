@@ -30,9 +30,16 @@ constexpr int NPARAM = OB4 + 8;             // 27464
 constexpr int OW[NL] = {OW1, OW2, OW3, OW4};
 constexpr int OB[NL] = {OB1, OB2, OB3, OB4};
 
-// fragment-ordered BF16 weights for the inference kernel: layer l is padded
-// to KP[l] x NP[l] (multiples of 32 x 16) and stored tile by tile
-// (k-block major, then n-block), 64 lanes x 8 values per 32x16 tile
+// fragment-ordered BF16 weights for the inference kernel.  The network runs
+// transposed, Z^T = W^T A^T: a weight tile is the A operand (16 outputs n x
+// 32 inputs k) and the activations are the B operand, taken straight from
+// the previous layer's accumulators.  Lane l of a layer's output block nb
+// holds outputs n = 16 nb + 4 (l/16) + i, i = 0..3, of activation row l%16;
+// slot t of its B fragment for input block kb therefore carries input
+// k = frag_k(layer, kb, l/16, t): the two output blocks 2 kb and 2 kb + 1
+// of the previous layer, four values each (layer 1 reads its inputs as
+// k = 8 (l/16) + t).  Layer l is padded to KP[l] x NP[l] and stored tile by
+// tile (output block major, then input block), 64 lanes x 8 values each.
 constexpr int KP[NL] = {32, 128, 128, 64};
 constexpr int NP[NL] = {128, 128, 64, 16};
 constexpr int TILE = 64 * 8;                // bf16 values per tile
@@ -43,6 +50,10 @@ constexpr int FT3 = FT2 + (KP[2] / 32) * (NP[2] / 16);
 constexpr int NTILE = FT3 + (KP[3] / 32) * (NP[3] / 16);   // 58
 constexpr int FT[NL] = {FT0, FT1, FT2, FT3};
 constexpr int NFRAG = NTILE * TILE;         // 29696 bf16 = 58 KiB
+__host__ __device__ inline int frag_k(int layer, int kb, int g, int t) {
+  if (layer == 0) return kb * 32 + 8 * g + t;
+  return 32 * kb + (t < 4 ? 4 * g + t : 16 + 4 * g + (t - 4));
+}
 // the biases packed for the inference kernel's LDS copy
 constexpr int CB1 = 0, CB2 = 128, CB3 = 256, CB4 = 320, NBIAS = 328;
 
@@ -76,99 +87,87 @@ __device__ __forceinline__ float dsilu(float z) {
 // batched inference: Y[rows x 8] = MLP(X[rows x 16]) with fragment-ordered
 // BF16 weights (flat FP32 biases); a workgroup walks 64-row groups
 // =====================================================================
-constexpr int ACT_LD = 128 + 8;  // bf16 row stride of the activation tile
-// inference workgroups: 16 waves share one LDS copy of the weights (58 KiB
-// + 16 activation tiles of 4.3 KiB), 4 waves per SIMD
+// inference workgroups: 16 waves share one LDS copy of the weights (59 KiB);
+// two workgroups per CU, 8 waves per SIMD (at most 64 VGPRs)
 constexpr int kInferThreads = 1024, kInferWaves = kInferThreads / 64;
+constexpr int kInferGrid = 2 * 256;
 
 // one 16-row tile through the network; X rows are FP32, Y rows FP32.  sW:
-// fragments in LDS, bias: the packed biases (load_biases) in LDS, sAct:
-// this wave's 16 x ACT_LD activation tile
+// fragments in LDS, bias: the packed biases (load_biases) in LDS.  The
+// activations never leave the registers: each layer's accumulators, through
+// SiLU and BF16, are the next layer's B fragments (see frag_k).
 template <typename OUT>
 __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
-                                           __bf16 *sAct, const float *X,
-                                           int rows, int r0, OUT out) {
-  const int lane = threadIdx.x & 63, r = lane & 15, kq = 8 * (lane >> 4);
+                                           const float *X, int rows, int r0,
+                                           OUT out) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const bf16x8 *W = reinterpret_cast<const bf16x8 *>(sW);
-  // layer 1: A fragment straight from X (k >= 16 is padding)
+  // layer 1: B fragment straight from X (inputs k >= 16 are padding)
   bf16x8 x;
 #pragma unroll
   for (int t = 0; t < 8; t++) {
-    const int k = kq + t, row = r0 + r;
+    const int k = 8 * g + t, row = r0 + r;
     x[t] = (k < 16 && row < rows) ? (__bf16)X[(long)row * 16 + k] : (__bf16)0.f;
   }
-  // accumulators start at the bias of their column (lane l holds column
-  // l%16 of each 16-column block)
+  // accumulators start at the bias of their outputs 16 nb + 4 g + i
   auto bias4 = [&](const float *b, int nb) {
-    const float v = b[nb * 16 + r];
-    return f32x4{v, v, v, v};
+    const float *p = b + nb * 16 + 4 * g;
+    return f32x4{p[0], p[1], p[2], p[3]};
+  };
+  // SiLU of output blocks 2 kb and 2 kb + 1 -> B fragment kb
+  auto act = [&](const f32x4 *a, int kb) {
+    bf16x8 v;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      v[t] = (__bf16)silu(a[2 * kb][t]);
+      v[4 + t] = (__bf16)silu(a[2 * kb + 1][t]);
+    }
+    return v;
   };
   f32x4 acc[8];
 #pragma unroll
-  for (int nb = 0; nb < 8; nb++)
-    acc[nb] = mfma(x, W[(FT0 + nb) * 64 + lane], bias4(bias + CB1, nb));
-  auto store_act = [&](f32x4 *a, int nNb) {
+  for (int nb = 0; nb < 8; nb++) {
+    acc[nb] = mfma(W[(FT0 + nb) * 64 + lane], x, bias4(bias + CB1, nb));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  bf16x8 b4[4];
 #pragma unroll
-    for (int nb = 0; nb < 8; nb++) {
-      if (nb >= nNb) break;
-      const int n = nb * 16 + r;
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-        sAct[(4 * (lane >> 4) + i) * ACT_LD + n] = (__bf16)silu(a[nb][i]);
-    }
-  };
-  auto load_act = [&](int kb) {
-    const bf16x8 *p =
-        reinterpret_cast<const bf16x8 *>(sAct + r * ACT_LD + kb * 32 + kq);
-    return *p;
-  };
-  store_act(acc, 8);
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS writes done
-  __builtin_amdgcn_wave_barrier();
+  for (int kb = 0; kb < 4; kb++) b4[kb] = act(acc, kb);
   // layer 2: 128 -> 128
-  bf16x8 a4[4];
-#pragma unroll
-  for (int kb = 0; kb < 4; kb++) a4[kb] = load_act(kb);
 #pragma unroll
   for (int nb = 0; nb < 8; nb++) {
     acc[nb] = bias4(bias + CB2, nb);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++)
-      acc[nb] = mfma(a4[kb], W[(FT1 + kb * 8 + nb) * 64 + lane], acc[nb]);
+      acc[nb] = mfma(W[(FT1 + nb * 4 + kb) * 64 + lane], b4[kb], acc[nb]);
+    // keep the weight fragments of one output block live at a time
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  store_act(acc, 8);
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  // layer 3: 128 -> 64
 #pragma unroll
-  for (int kb = 0; kb < 4; kb++) a4[kb] = load_act(kb);
+  for (int kb = 0; kb < 4; kb++) b4[kb] = act(acc, kb);
+  // layer 3: 128 -> 64
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
     acc[nb] = bias4(bias + CB3, nb);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++)
-      acc[nb] = mfma(a4[kb], W[(FT2 + kb * 4 + nb) * 64 + lane], acc[nb]);
+      acc[nb] = mfma(W[(FT2 + nb * 4 + kb) * 64 + lane], b4[kb], acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  store_act(acc, 4);
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  // layer 4: 64 -> 8 (padded to 16 columns)
-  a4[0] = load_act(0);
-  a4[1] = load_act(1);
-  f32x4 y = {0.f, 0.f, 0.f, 0.f};
-  y = mfma(a4[0], W[(FT3 + 0) * 64 + lane], y);
-  y = mfma(a4[1], W[(FT3 + 1) * 64 + lane], y);
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  if (r < 8) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int row = r0 + 4 * (lane >> 4) + i;
-      if (row < rows) out(row, r, y[i] + bias[CB4 + r]);
+  for (int kb = 0; kb < 2; kb++) b4[kb] = act(acc, kb);
+  // layer 4: 64 -> 8 (padded to 16 outputs; lanes with g < 2 hold 0..7)
+  f32x4 y = {0.f, 0.f, 0.f, 0.f};
+  y = mfma(W[(FT3 + 0) * 64 + lane], b4[0], y);
+  y = mfma(W[(FT3 + 1) * 64 + lane], b4[1], y);
+  if (g < 2) {
+    const int row = r0 + r;
+    if (row < rows) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int n = 4 * g + i;
+        out(row, n, y[i] + bias[CB4 + n]);
+      }
     }
   }
 }

@@ -438,12 +438,11 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
 
 // the network on the compacted rows; fluxes back to the face array
 template <int NB>
-__global__ void __launch_bounds__(bats::mlp::kInferThreads)
+__global__ void __launch_bounds__(bats::mlp::kInferThreads, 8)
 k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
   using namespace bats::mlp;
   using G = FaceGeom<NB>;
   __shared__ __attribute__((aligned(16))) __bf16 sW[NFRAG];
-  __shared__ __attribute__((aligned(16))) __bf16 sAct[kInferWaves][16 * ACT_LD];
   __shared__ float sBias[NBIAS];
   for (int t = threadIdx.x; t < NFRAG / 8; t += kInferThreads)
     reinterpret_cast<bf16x8 *>(sW)[t] = reinterpret_cast<const bf16x8 *>(frag)[t];
@@ -464,7 +463,7 @@ k_sur_infer(const __bf16 *frag, const float *w, SurArgs a) {
   const int nTile = (rows + 15) / 16;
   for (int tile = blockIdx.x * kInferWaves + wave; tile < nTile;
        tile += gridDim.x * kInferWaves)
-    infer_tile(sW, sBias, sAct[wave], a.rowX, rows, tile * 16, out);
+    infer_tile(sW, sBias, a.rowX, rows, tile * 16, out);
 }
 
 // flux divergence and stage update (FP64), local time step; one workgroup
@@ -693,6 +692,8 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
   if (Mode != BATS_SUR_OFF && !s.face) {
     const long nF = (long)(h->NB + 1) * h->NB * h->NB;
     const long nTot = 3 * (long)h->cfg.nBlock * nF;
+    // rows and face indices are 32-bit (about 160k blocks of 16^3)
+    if (nTot > 0x7fffffffL) return BATS_ELIMIT;
     HIPCHK(hipMalloc(&s.face, sizeof(float) * NSLOT * nTot));
     HIPCHK(hipMalloc(&s.rowX, sizeof(float) * 16 * nTot));
     HIPCHK(hipMalloc(&s.rowFace, sizeof(int) * nTot));
@@ -775,10 +776,10 @@ int sur_stage(bats_handle *h, const StageArgs &sa) {
   if (s.mode == BATS_SUR_NETWORK) {
     const bats::mlp::Weights wts = bats::mlp::weights(s.model);
     if (h->NB == 8)
-      hipLaunchKernelGGL(k_sur_infer<8>, dim3(256), dim3(bats::mlp::kInferThreads), 0, h->stream,
+      hipLaunchKernelGGL(k_sur_infer<8>, dim3(bats::mlp::kInferGrid), dim3(bats::mlp::kInferThreads), 0, h->stream,
                          wts.frag, wts.w, a);
     else
-      hipLaunchKernelGGL(k_sur_infer<16>, dim3(256), dim3(bats::mlp::kInferThreads), 0, h->stream,
+      hipLaunchKernelGGL(k_sur_infer<16>, dim3(bats::mlp::kInferGrid), dim3(bats::mlp::kInferThreads), 0, h->stream,
                          wts.frag, wts.w, a);
   }
   if (h->nSlab > 0) {
