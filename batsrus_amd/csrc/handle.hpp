@@ -18,7 +18,7 @@ struct SurrogateState {
   double betaMin;      // gate: minimum plasma beta for the network
   double jumpTol;      // gate: largest scaled jump counted as smooth
   float *face;         // face fluxes, 9 slots per face, all blocks
-  float *rowX;         // network inputs of the gated faces (rows x 16)
+  __bf16 *rowX;        // network inputs of the gated faces (rows x 16)
   float *rowY;         // network outputs (rows x 8)
   int *rowFace;        // per row: global face index
   float *rowScale;     // per row (float4): density scale, speed scale,

@@ -96,18 +96,28 @@ constexpr int kInferGrid = 2 * 256;
 // fragments in LDS, bias: the packed biases (load_biases) in LDS.  The
 // activations never leave the registers: each layer's accumulators, through
 // SiLU and BF16, are the next layer's B fragments (see frag_k).
-template <typename OUT>
+// X rows are FP32 (rounded to BF16 here) or already BF16.
+template <typename XT, typename OUT>
 __device__ __forceinline__ void infer_tile(const __bf16 *sW, const float *bias,
-                                           const float *X, int rows, int r0,
+                                           const XT *X, int rows, int r0,
                                            OUT out) {
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   const bf16x8 *W = reinterpret_cast<const bf16x8 *>(sW);
   // layer 1: B fragment straight from X (inputs k >= 16 are padding)
   bf16x8 x;
+  const int row0 = r0 + r;
+  if (sizeof(XT) == 2) {
+    // 16-byte load of inputs 8g .. 8g+7 of the row
+    x = (g < 2 && row0 < rows)
+            ? *reinterpret_cast<const bf16x8 *>(X + (long)row0 * 16 + 8 * g)
+            : bf16x8{};
+  } else {
 #pragma unroll
-  for (int t = 0; t < 8; t++) {
-    const int k = 8 * g + t, row = r0 + r;
-    x[t] = (k < 16 && row < rows) ? (__bf16)X[(long)row * 16 + k] : (__bf16)0.f;
+    for (int t = 0; t < 8; t++) {
+      const int k = 8 * g + t;
+      x[t] = (k < 16 && row0 < rows) ? (__bf16)X[(long)row0 * 16 + k]
+                                     : (__bf16)0.f;
+    }
   }
   // accumulators start at the bias of their outputs 16 nb + 4 g + i
   auto bias4 = [&](const float *b, int nb) {

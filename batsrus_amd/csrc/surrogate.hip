@@ -315,7 +315,7 @@ struct SurArgs {
       noUpdate, advanceTime;
   float *face;     // NSLOT x nFaceTot
   long nFaceTot;   // 3 * nBlock * NF
-  float *rowX;
+  __bf16 *rowX;
   int *rowFace;
   float *rowScale;
   int *counters;
@@ -420,11 +420,18 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
       for (int w = 0; w < wave; w++) base += sCount[w];
       if (net) {
         const int row = base + __popcll(mask & ((1ull << lane) - 1ull));
-        // 16-byte stores: the row's 64 bytes of inputs, then its scales
-        float4 *xr = reinterpret_cast<float4 *>(a.rowX + (long)row * 16);
+        // the row's inputs in BF16 (the network's input precision; two
+        // 16-byte stores), then its scales
+        bats::mlp::bf16x8 x0, x1;
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-          xr[c] = make_float4(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
+        for (int c = 0; c < 8; c++) {
+          x0[c] = (__bf16)x[c];
+          x1[c] = (__bf16)x[8 + c];
+        }
+        bats::mlp::bf16x8 *xr =
+            reinterpret_cast<bats::mlp::bf16x8 *>(a.rowX + (long)row * 16);
+        xr[0] = x0;
+        xr[1] = x1;
         a.rowFace[row] = (int)gf;
         reinterpret_cast<float4 *>(a.rowScale)[row] =
             make_float4(rhos, cs, jn, area);
@@ -695,7 +702,7 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
     // rows and face indices are 32-bit (about 160k blocks of 16^3)
     if (nTot > 0x7fffffffL) return BATS_ELIMIT;
     HIPCHK(hipMalloc(&s.face, sizeof(float) * NSLOT * nTot));
-    HIPCHK(hipMalloc(&s.rowX, sizeof(float) * 16 * nTot));
+    HIPCHK(hipMalloc(&s.rowX, sizeof(__bf16) * 16 * nTot));
     HIPCHK(hipMalloc(&s.rowFace, sizeof(int) * nTot));
     HIPCHK(hipMalloc(&s.rowScale, sizeof(float) * 4 * nTot));
     HIPCHK(hipMalloc(&s.counters, sizeof(int) * NCOUNTER));
