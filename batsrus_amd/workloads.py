@@ -180,7 +180,9 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="python -m batsrus_amd.workloads")
     ap.add_argument("--samples", type=float, default=2e7)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warm", type=int, default=0)
     args = ap.parse_args(argv)
@@ -188,6 +190,7 @@ def main(argv=None):
     def log(s):
         print(s, flush=True)
     m, st = build_surrogate(n_samples=int(args.samples), epochs=args.epochs,
+                            batch=args.batch, lr=args.lr,
                             sources=("orszag_tang", "blast_ensemble",
                                      "kh_family"), log=log)
     try:

@@ -164,14 +164,15 @@ def measure_traffic(timeout=150):
                              "4 stage launches, scaled by a 1.07 GB copy"}
 
 
-def surrogate_leg(steps=200, samples=20_000_000, epochs=60):
+def surrogate_leg(steps=200, samples=20_000_000, epochs=100, batch=8192,
+                  lr=2e-3):
     """Train the face-flux network, then run the two-level KH workload in
     FP64 (plain HLLD) and FP32 with and without the network."""
     import numpy as np
     from batsrus_amd import workloads as W
     sources = ("orszag_tang", "blast_ensemble", "kh_family")
     m, st = W.build_surrogate(n_samples=samples, epochs=epochs,
-                              batch=65536, lr=1e-3, sources=sources,
+                              batch=batch, lr=lr, sources=sources,
                               jump_weight=True)
     try:
         g, U0 = W.kh_two_level()
@@ -206,6 +207,7 @@ def surrogate_leg(steps=200, samples=20_000_000, epochs=60):
                       "variable (rho, rho u, B, p) and over all of them",
         "training": {"sources": list(sources), "samples": st["samples"],
                      "epochs": st["epochs"], "batch": st["batch"],
+                     "lr": st["lr"],
                      "harvest_s": st["harvest_s"], "train_s": st["train_s"],
                      "heldout_flux_rel_l2": st["heldout_rel_l2"],
                      "heldout_correction_rel_l2":
