@@ -34,6 +34,19 @@
  *        divergence-free field gives a zero source, and the source equals
  *        -divB*(0, B, u, u.B) with div B from calc_divb's formula
  *        (test_oracle_powell_*).
+ *   adaptive low order faces (set_low_order_face, low order weights in the
+ *        second order face values)
+ *        pinned: closed forms on hand-built four-cell stencils (every
+ *        threshold branch, minimum sound speed, gamma, normal velocity, B^2/2
+ *        in the total pressure, stencil position), weight 1 == first order
+ *        faces, fractional weight == blend of first and second order faces,
+ *        all-low-order run == first order run (test_low_order_oracle).
+ *   two-level grids: coarse faces take the sum of the four fine subface
+ *        fluxes (save/apply_cons_flux) -- pinned by conservation to rounding;
+ *        resolution-change ghost cells (restriction = mean of 8 fine cells,
+ *        prolongation = coarse cell + minmod slopes x 1/4) stand in for the
+ *        absent BATL message passing -- pinned by closed forms only (exact on
+ *        linear data, uniform state kept): parity unpinned against BATL.
  *   HLLD guard constant cTiny (share/ModNumConst, absent): parity unpinned;
  *        configurable, default 1e-6.
  */
