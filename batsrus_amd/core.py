@@ -536,7 +536,7 @@ class Mlp:
         return n.value
 
     def train(self, nEpoch, batch=65536, lr=1e-3, hold=0.1, seed=1,
-              standardize=False, jump_weight=False):
+              standardize=False, jump_weight=False, graph=True):
         """Returns the last minibatch loss and the held-out relative L2
         errors (18: flux overall + 8 outputs, correction overall + 8)."""
         loss = C.c_double()
@@ -544,7 +544,8 @@ class Mlp:
         self._check(self.lib.bats_mlp_train(self.h, int(nEpoch), int(batch),
                                             float(lr), float(hold),
                                             int(standardize) |
-                                            2 * int(jump_weight), int(seed),
+                                            2 * int(jump_weight) |
+                                            4 * int(not graph), int(seed),
                                             C.byref(loss), _ptr(err)),
                     "mlp_train")
         return loss.value, err

@@ -135,18 +135,27 @@ struct AdamArgs {
   float *w, *m, *v;
   __bf16 *wb;            // BF16 copy, flat layout
   float lr, b1, b2, eps, bc1, bc2;  // bc = 1 - beta^t
+  // with a schedule (graph-captured training): lr, bc1, bc2 of step
+  // *step are sched[3 * *step + 0..2]
+  const float *sched;
+  const long *step;
 };
 
 // Adam (Kingma & Ba 2015) on the FP32 master weights
 __global__ void k_adam(AdamArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NPARAM) return;
+  float lr = a.lr, bc1 = a.bc1, bc2 = a.bc2;
+  if (a.sched) {
+    const float *q = a.sched + 3 * *a.step;
+    lr = q[0]; bc1 = q[1]; bc2 = q[2];
+  }
   const float g = a.grad[p];
   const float m = a.b1 * a.m[p] + (1.f - a.b1) * g;
   const float v = a.b2 * a.v[p] + (1.f - a.b2) * g * g;
   a.m[p] = m;
   a.v[p] = v;
-  const float w = a.w[p] - a.lr * (m / a.bc1) / (sqrtf(v / a.bc2) + a.eps);
+  const float w = a.w[p] - lr * (m / bc1) / (sqrtf(v / bc2) + a.eps);
   a.w[p] = w;
   a.wb[p] = (__bf16)w;
 }
@@ -222,14 +231,16 @@ k_infer(const __bf16 *frag, const float *w, const float *X, float *Y,
 // minibatch gather: rows idx[0..B) of the dataset into X (BF16) and the
 // standardized targets T / scale
 // a minibatch: inputs (BF16), targets divided by scale, and (W non-null)
-// the loss weights dataJ / meanJ
+// the loss weights dataJ / meanJ.  With step non-null the rows are
+// idx[(*step % perEpoch) * B + t] (graph-captured training).
 __global__ void k_gather_rows(const float *dataX, const float *dataT,
                               const int *idx, int B, __bf16 *X, float *T,
                               const float *scale, const float *dataJ,
-                              float invMeanJ, float *W) {
+                              float invMeanJ, float *W, const long *step,
+                              long perEpoch) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B) return;
-  const long s = idx[t];
+  const long s = step ? idx[(*step % perEpoch) * B + t] : idx[t];
   for (int j = 0; j < 16; j++) X[t * 16 + j] = (__bf16)dataX[s * 16 + j];
   for (int j = 0; j < 8; j++) T[t * 8 + j] = dataT[s * 8 + j] / scale[j];
   if (W) W[t] = dataJ[s] * invMeanJ;
@@ -435,8 +446,14 @@ static void forward_batch(bats_mlp *m, int B) {
 
 // gradient of the mean-square loss of the minibatch (X, T already on the
 // device) into m->grad and, if lr > 0, one Adam step; rowErr filled
+// the step counter of graph-captured training, advanced after each step
+__global__ void k_step_inc(long *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
 static int grad_step(bats_mlp *m, int B, float lr, bool update,
-                     const float *W = nullptr) {
+                     const float *W = nullptr, const float *sched = nullptr,
+                     const long *dStep = nullptr) {
   forward_batch(m, B);
   hipLaunchKernelGGL(k_loss, dim3((B + 255) / 256), dim3(256), 0, m->stream,
                      m->Y, m->T, W, B, m->dY, m->rowErr);
@@ -468,12 +485,19 @@ static int grad_step(bats_mlp *m, int B, float lr, bool update,
   hipLaunchKernelGGL(k_sum_partials, dim3((NPARAM + 255) / 256), dim3(256), 0,
                      m->stream, m->partial, S, m->grad);
   if (update) {
-    m->step++;
     AdamArgs a;
     a.grad = m->grad; a.w = m->w; a.m = m->m; a.v = m->v; a.wb = m->wb;
-    a.lr = lr; a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-8f;
-    a.bc1 = 1.f - powf(a.b1, (float)m->step);
-    a.bc2 = 1.f - powf(a.b2, (float)m->step);
+    a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-8f;
+    a.sched = sched;
+    a.step = dStep;
+    if (!sched) {
+      m->step++;
+      a.lr = lr;
+      a.bc1 = 1.f - powf(a.b1, (float)m->step);
+      a.bc2 = 1.f - powf(a.b2, (float)m->step);
+    } else {
+      a.lr = a.bc1 = a.bc2 = 0.f;
+    }
     hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0,
                        m->stream, a);
     m->fragStale = 1;
@@ -654,7 +678,8 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
                               double *HeldOutRelErr_V) {
   const bool Standardize = (Flags & BATS_MLP_STANDARDIZE) != 0;
   const bool JumpWeight = (Flags & BATS_MLP_JUMP_WEIGHT) != 0;
-  if (Flags & ~(BATS_MLP_STANDARDIZE | BATS_MLP_JUMP_WEIGHT))
+  const bool UseGraph = (Flags & BATS_MLP_NO_GRAPH) == 0;
+  if (Flags & ~(BATS_MLP_STANDARDIZE | BATS_MLP_JUMP_WEIGHT | BATS_MLP_NO_GRAPH))
     return BATS_EINVAL;
   if (!m || nEpoch < 0 || Batch <= 0 || !(Lr0 >= 0.0) || HoldFrac < 0.0 ||
       HoldFrac >= 1.0 || m->nData < 2)
@@ -724,30 +749,86 @@ extern "C" int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
   }
   const long stepsPerEpoch = nTrain / B;
   const long total = stepsPerEpoch * nEpoch;
-  long step = 0;
   double lastLoss = 0.0;
   std::vector<float> e(B);
-  for (int ep = 0; ep < nEpoch; ep++) {
-    shuffle(train, s);
-    MCHK(hipMemcpyAsync(dIdx, train.data(), sizeof(int) * train.size(),
-                        hipMemcpyHostToDevice, m->stream));
-    double epLoss = 0.0;
-    for (long b = 0; b < stepsPerEpoch; b++, step++) {
-      const double lr = Lr0 * 0.5 * (1.0 + cos(M_PI * (double)step / (double)total));
-      hipLaunchKernelGGL(k_gather_rows, dim3((B + 255) / 256), dim3(256), 0,
-                         m->stream, m->dataX, m->dataT, dIdx + b * B, B, m->X,
-                         m->T, dScale, m->dataJ, invMeanJ,
-                         JumpWeight ? m->rowW : nullptr);
-      st = grad_step(m, B, (float)lr, true, JumpWeight ? m->rowW : nullptr);
-      if (st) { (void)hipFree(dIdx); return st; }
-      if (b == stepsPerEpoch - 1) {  // loss of the epoch's last minibatch
-        MCHK(hipMemcpyAsync(e.data(), m->rowErr, sizeof(float) * B,
-                            hipMemcpyDeviceToHost, m->stream));
-        MCHK(hipStreamSynchronize(m->stream));
-        for (int r = 0; r < B; r++) epLoss += e[r];
-        lastLoss = epLoss / (8.0 * B);
-      }
+  if (total > 0) {
+    // the learning rate (cosine decay) and Adam's bias corrections of every
+    // step, read on the device: one training step is captured once in a
+    // hipGraph and replayed (the minibatch of step q is rows q % perEpoch of
+    // the epoch's permutation; a device counter advances after each step)
+    std::vector<float> sched(3 * (size_t)total);
+    for (long q = 0; q < total; q++) {
+      const double lr = Lr0 * 0.5 * (1.0 + cos(M_PI * (double)q / (double)total));
+      const float t = (float)(m->step + q + 1);
+      sched[3 * q] = (float)lr;
+      sched[3 * q + 1] = 1.f - powf(0.9f, t);
+      sched[3 * q + 2] = 1.f - powf(0.999f, t);
     }
+    float *dSched = nullptr;
+    long *dStep = nullptr;
+    MCHK(hipMalloc(&dSched, sizeof(float) * sched.size()));
+    MCHK(hipMalloc(&dStep, sizeof(long)));
+    MCHK(hipMemcpyAsync(dSched, sched.data(), sizeof(float) * sched.size(),
+                        hipMemcpyHostToDevice, m->stream));
+    MCHK(hipMemsetAsync(dStep, 0, sizeof(long), m->stream));
+    float *W = JumpWeight ? m->rowW : nullptr;
+    // one training step: the same kernels whether captured or launched
+    auto one_step = [&]() {
+      hipLaunchKernelGGL(k_gather_rows, dim3((B + 255) / 256), dim3(256), 0,
+                         m->stream, m->dataX, m->dataT, dIdx, B, m->X, m->T,
+                         dScale, m->dataJ, invMeanJ, W, dStep, stepsPerEpoch);
+      const int r = grad_step(m, B, 0.f, true, W, dSched, dStep);
+      hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, m->stream, dStep);
+      return r;
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (UseGraph) {
+      MCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+      st = one_step();
+      const hipError_t ec = hipStreamEndCapture(m->stream, &graph);
+      if (st == BATS_OK && ec == hipSuccess)
+        st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess
+                 ? BATS_OK : BATS_EHIP;
+      else if (st == BATS_OK)
+        st = BATS_EHIP;
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+    for (int ep = 0; st == BATS_OK && ep < nEpoch; ep++) {
+      shuffle(train, s);
+      if (hipMemcpyAsync(dIdx, train.data(), sizeof(int) * train.size(),
+                         hipMemcpyHostToDevice, m->stream) != hipSuccess) {
+        st = BATS_EHIP;
+        break;
+      }
+      for (long b = 0; st == BATS_OK && b < stepsPerEpoch; b++) {
+        if (UseGraph)
+          st = hipGraphLaunch(exec, m->stream) == hipSuccess ? BATS_OK : BATS_EHIP;
+        else
+          st = one_step();
+      }
+      if (st != BATS_OK) break;
+      // loss of the epoch's last minibatch
+      if (hipMemcpyAsync(e.data(), m->rowErr, sizeof(float) * B,
+                         hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+          hipStreamSynchronize(m->stream) != hipSuccess) {
+        st = BATS_EHIP;
+        break;
+      }
+      double epLoss = 0.0;
+      for (int r = 0; r < B; r++) epLoss += e[r];
+      lastLoss = epLoss / (8.0 * B);
+    }
+    (void)hipStreamSynchronize(m->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipFree(dSched);
+    (void)hipFree(dStep);
+    if (st) {
+      (void)hipFree(dIdx);
+      (void)hipFree(dScale);
+      return st;
+    }
+    m->step += total;
   }
   // back to physical targets (a no-op scale of one without Standardize)
   hipLaunchKernelGGL(k_scale_last, dim3(1), dim3(64 * 8 + 8), 0, m->stream,

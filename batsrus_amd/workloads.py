@@ -117,7 +117,7 @@ def init_weights(seed=0):
 
 def build_surrogate(n_samples=20_000_000, epochs=4, batch=65536, lr=1e-3,
                     hold=0.1, seed=1, log=None, standardize=False,
-                    jump_weight=True, **harvest_kw):
+                    jump_weight=True, graph=True, **harvest_kw):
     """Harvest n_samples faces and train the network on the GPU; returns the
     model and a dict of statistics (samples, held-out relative L2 error
     overall and per output, seconds)."""
@@ -126,7 +126,8 @@ def build_surrogate(n_samples=20_000_000, epochs=4, batch=65536, lr=1e-3,
     n = harvest_training_set(m, n_samples, seed=seed, log=log, **harvest_kw)
     t1 = time.perf_counter()
     loss, err = m.train(epochs, batch=batch, lr=lr, hold=hold, seed=seed,
-                        standardize=standardize, jump_weight=jump_weight)
+                        standardize=standardize, jump_weight=jump_weight,
+                        graph=graph)
     t2 = time.perf_counter()
     stats = dict(samples=n, harvest_s=t1 - t0, train_s=t2 - t1,
                  epochs=epochs, batch=batch, lr=lr, hold=hold,
@@ -185,12 +186,16 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warm", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="train without replaying a captured step (for "
+                         "profilers that do not follow graph launches)")
     args = ap.parse_args(argv)
 
     def log(s):
         print(s, flush=True)
     m, st = build_surrogate(n_samples=int(args.samples), epochs=args.epochs,
                             batch=args.batch, lr=args.lr,
+                            graph=not args.no_graph,
                             sources=("orszag_tang", "blast_ensemble",
                                      "kh_family"), log=log)
     try:

@@ -273,7 +273,8 @@ int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
 typedef struct bats_mlp bats_mlp;
 enum { BATS_MLP_NPARAM = 27464, BATS_MLP_NIN = 16, BATS_MLP_NOUT = 8 };
 /* bats_mlp_train flags */
-enum { BATS_MLP_STANDARDIZE = 1, BATS_MLP_JUMP_WEIGHT = 2 };
+enum { BATS_MLP_STANDARDIZE = 1, BATS_MLP_JUMP_WEIGHT = 2,
+       BATS_MLP_NO_GRAPH = 4 };
 int bats_mlp_create(bats_mlp **m);
 int bats_mlp_free(bats_mlp *m);
 /* FP32 master weights in (resets the Adam moments) / out; Grad receives
@@ -302,7 +303,11 @@ int bats_mlp_harvest(bats_mlp *m, bats_handle *h, double Fraction,
  * their RMS (the scale is folded back into the last layer afterwards);
  * with BATS_MLP_JUMP_WEIGHT each sample's squared error is weighted by its
  * target weight w (the jump norm of a harvested face) over the mean w, so
- * the loss follows the error of the flux correction w*T itself.  Returns the last
+ * the loss follows the error of the flux correction w*T itself.  Each
+ * training step is captured once as a hipGraph and replayed per minibatch
+ * (the learning rate and Adam's bias corrections are read on the device);
+ * BATS_MLP_NO_GRAPH launches the same kernels one by one instead, for
+ * tools that do not follow graph launches.  Returns the last
  * minibatch loss and the held-out relative L2 errors, overall and per
  * output, with w each sample's target weight (the jump norm |j| for
  * harvested faces, one otherwise): [0..8] of the predicted flux,

@@ -100,3 +100,59 @@ def test_training_learns_a_smooth_map():
     # (a CPU torch run of the same network and schedule reaches 0.029)
     assert res[0][1] < 0.1 * res[0][0] and res[0][1] < 0.06
     assert np.array_equal(res[0][2], res[1][2])   # deterministic
+
+
+def test_graph_replayed_training_matches_eager_steps():
+    """Training replays one captured step per minibatch, with the learning
+    rate and Adam's bias corrections read on the device.  Five full-batch
+    steps (one per epoch, so the host uploads a permutation and reads the
+    loss between replays) equal five eager steps: the library's gradient
+    of the current weights and the oracle's Adam update with the cosine
+    schedule."""
+    print("case mlp graph training", flush=True)
+    rng = np.random.default_rng(11)
+    w0 = _params(5)
+    X = rng.normal(0, 1, (512, 16)).astype(np.float32)
+    T = rng.normal(0, 1, (512, 8)).astype(np.float32)
+    K, lr0 = 5, 2e-3
+    with Mlp(w0) as m:
+        m.set_data(X, T)
+        m.train(K, batch=512, lr=lr0, hold=0.0, seed=2)
+        wg = m.params()[0]
+    w = w0.astype(np.float64).copy()
+    mom, vel = np.zeros_like(w), np.zeros_like(w)
+    with Mlp(w0) as m:
+        for q in range(K):
+            m.set_params(w.astype(np.float32))
+            _, g = m.loss_grad(X, T)
+            mo.adam_step(w, mom, vel, g.astype(np.float64), q + 1,
+                         mo.cosine_lr(lr0, q, K))
+    d = np.abs(wg - w)
+    bad = float(np.mean(d > 1e-5))
+    print(f"  |graph - eager|: median {np.median(d):.2e}, max {d.max():.2e}, "
+          f"fraction above 1e-5 {bad:.1e} (step size {lr0:.0e})")
+    # the minibatch rows are gathered in another order, so the FP32 gradient
+    # sums differ at rounding level; Adam's normalised step amplifies that
+    # only for components whose gradient is itself at rounding level.  A
+    # wrong learning rate, bias correction or step count moves every
+    # parameter by a fraction of the step size.
+    assert np.median(d) <= 1e-7 and bad < 1e-3
+
+
+def test_graph_training_equals_kernel_by_kernel_training():
+    """The replayed step graph and the same kernels launched one by one give
+    the same bits over several epochs of minibatches (host work between the
+    epochs' replays)."""
+    print("case mlp graph vs eager training", flush=True)
+    rng = np.random.default_rng(12)
+    X = rng.uniform(-1, 1, (6000, 16)).astype(np.float32)
+    T = np.tanh(X[:, :8] + 0.5 * X[:, 8:]).astype(np.float32)
+    out = []
+    for graph in (True, False):
+        with Mlp(mo.init_params(seed=13)) as m:
+            m.set_data(X, T)
+            loss, err = m.train(3, batch=1024, lr=2e-3, hold=0.1, seed=4,
+                                graph=graph)
+            out.append((m.params()[0], loss, err))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][2], out[1][2])
