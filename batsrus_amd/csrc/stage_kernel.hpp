@@ -194,7 +194,12 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   __shared__ double sRed[kThreads / 64];
 
   const int tid = threadIdx.x;
-  const int lb = tid / NC, col = tid % NC, j = col / NB, i = col % NB;
+  // a wave never spans two blocks (NC >= 64): the block index, and with it
+  // every per-block value (cell size, run, time step), is wave-uniform and
+  // lives in scalar registers
+  static_assert(NC % 64 == 0, "one block per wave");
+  const int lb = BPW == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid / NC);
+  const int col = tid % NC, j = col / NB, i = col % NB;
   // this thread's cell of the 2-deep x/y halo ring of each plane (at most
   // one: the ring of a group's blocks has NH <= 256 cells)
   constexpr int NH = BPW * 8 * NB;
@@ -353,7 +358,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       __syncthreads();
       // x and y faces of plane k
       for (int f = tid; f < BPW * 2 * NFX; f += kThreads) {
-        const int lbf = f / (2 * NFX);
+        const int lbf = BPW == 1 ? 0 : f / (2 * NFX);
         const int bf = blk0 + lbf;
         if (bf >= a.nBlock) continue;
         const int r = f % (2 * NFX), dir = r / NFX, q = r % NFX;
