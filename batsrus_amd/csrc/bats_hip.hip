@@ -337,6 +337,7 @@ k_integrate(const T *U, long vs, const double *cellsize, Phys ph,
             int nBlock, double *partial) {
   __shared__ double sRed[256];
   const int b = blockIdx.x;
+  if (b >= nBlock) return;  // one workgroup per block
   const double V = cellsize[3 * b] * cellsize[3 * b + 1] * cellsize[3 * b + 2];
   double acc[NVAR];
   for (int v = 0; v < NVAR; v++) acc[v] = 0.0;

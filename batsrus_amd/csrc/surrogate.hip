@@ -134,7 +134,9 @@ __device__ __forceinline__ bool face_inputs(const PhysT<R> &ph, const R *L,
                                             R &cs, R &jn) {
   constexpr int it1 = (DIR + 1) % 3, it2 = (DIR + 2) % 3;
   R unL, cfL, unR, cfR;
-  const bool ok = fast_speed<DIR>(ph, L, unL, cfL) & fast_speed<DIR>(ph, Rs, unR, cfR);
+  const bool okL = fast_speed<DIR>(ph, L, unL, cfL);   // both speeds are
+  const bool okR = fast_speed<DIR>(ph, Rs, unR, cfR);  // needed either way
+  const bool ok = okL && okR;
   rhos = R(0.5) * (L[RHO] + Rs[RHO]);
   cs = fmax2(cfL, cfR);
   const R iu = R(1) / cs, ib = iu / fsqrt(rhos), ip = iu * iu / rhos,
@@ -267,7 +269,7 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
     sScan[threadIdx.x] = sel ? 1 : 0;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {
-      const int v = threadIdx.x >= o ? sScan[threadIdx.x - o] : 0;
+      const int v = (int)threadIdx.x >= o ? sScan[threadIdx.x - o] : 0;
       __syncthreads();
       sScan[threadIdx.x] += v;
       __syncthreads();
