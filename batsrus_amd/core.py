@@ -73,6 +73,7 @@ ABI_FUNCTIONS = {
     "bats_set_grid": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_amr_grid": (C.c_int, [_hp, _ip, _dp, C.c_int, _ip]),
     "bats_set_low_order_face": (C.c_int, [_hp]),
+    "bats_set_cell_boundary": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_step_graph": (C.c_int, [_hp, C.c_int]),
     "bats_get_low_order_crit": (C.c_int, [_hp, C.c_int, _dp, _dp, _dp]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
@@ -165,6 +166,10 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0)
 
 
+# iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
+BC_TYPES = {"periodic": 4, "float": 5, "fixed": 9, "inflow": 10}
+
+
 def make_config(grid: BlockGrid, device=0, **scheme):
     s = dict(SCHEME_DEFAULTS)
     s.update(scheme)
@@ -211,6 +216,10 @@ class Session:
         self._check(self.lib.bats_init(C.byref(self.cfg), C.byref(h)),
                     "bats_init")
         self.h = h
+        if scheme.get("TypeCellBc") is not None:
+            # stored now, applied with the grid below
+            self.set_cell_boundary(scheme["TypeCellBc"],
+                                   scheme.get("CellState_VI"))
         nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
         cs = np.ascontiguousarray(grid.CellSize_DB, dtype=np.float64)
         slabs = getattr(grid, "iSlab_IS", None)
@@ -285,6 +294,24 @@ class Session:
         self._check(self.lib.bats_calc_face_value(
             self.h, iBlock, int(DoResChangeOnly), int(DoMonotoneRestrict)),
             "calc_face_value")
+
+    def set_cell_boundary(self, TypeCellBc, CellState_VI=None):
+        """Outer boundary types of the sides x-, x+, y-, y+, z-, z+ (names
+        of BC_TYPES or iTypeCellBc codes) and the (6, 8) ghost states of the
+        fixed/inflow sides (rho, rho u, B, p).  Reallocates the state when
+        a side is fixed: upload the state afterwards."""
+        t = np.array([BC_TYPES[x] if isinstance(x, str) else int(x)
+                      for x in TypeCellBc], dtype=np.int32)
+        if t.shape != (6,):
+            raise ValueError("TypeCellBc needs 6 sides")
+        st = None
+        if CellState_VI is not None:
+            st = np.ascontiguousarray(CellState_VI, dtype=np.float64)
+            if st.shape != (6, 8):
+                raise ValueError("CellState_VI must have shape (6, 8)")
+        self._check(self.lib.bats_set_cell_boundary(
+            self.h, _ptr(t, C.c_int), None if st is None else _ptr(st)),
+            "set_cell_boundary")
 
     def set_step_graph(self, use=True):
         """bats_advance replays one captured hipGraph per time step."""

@@ -465,6 +465,10 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   HIPCHK(hipMemset(h->runOfBlock, 0, sizeof(int) * c.nBlock));
   HIPCHK(hipMemset(h->err, 0, sizeof(int)));
   HIPCHK(hipMemset(h->dtmaxB, 0, sizeof(double) * c.nBlock));
+  // outer boundaries float until bats_set_cell_boundary
+  for (int side = 0; side < 6; side++) h->bcType[side] = BATS_BC_FLOAT;
+  HIPCHK(hipMalloc(&h->dBcState, sizeof h->bcState));
+  HIPCHK(hipMemset(h->dBcState, 0, sizeof h->bcState));
   // default grid: all sides float until bats_set_grid
   h->hNei.assign(6 * c.nBlock, -1);
   HIPCHK(hipMemcpy(h->nei, h->hNei.data(), sizeof(int) * 6 * c.nBlock,
@@ -482,7 +486,7 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
                   h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
-                  h->corr, h->lowCrit};
+                  h->corr, h->lowCrit, h->dBcState};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   drop_graphs(h);
@@ -492,6 +496,10 @@ extern "C" int bats_finalize(bats_handle *h) {
   delete h;
   return BATS_OK;
 }
+
+static int apply_grid(bats_handle *h, const std::vector<int> &nei,
+                      const std::vector<double> &cs,
+                      const std::vector<int> &slabs);
 
 extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
                              const double *CellSize_DB) {
@@ -506,13 +514,9 @@ extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
     if (iNeighbor_EB[t] >= 0 &&
         h->hRunOfBlock[iNeighbor_EB[t]] != h->hRunOfBlock[t / 6])
       return BATS_EINVAL;
-  h->hNei.assign(iNeighbor_EB, iNeighbor_EB + 6 * nB);
-  HIPCHK(hipMemcpyAsync(h->nei, iNeighbor_EB, sizeof(int) * 6 * nB,
-                        hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->cellsize, CellSize_DB, sizeof(double) * 3 * nB,
-                        hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return BATS_OK;
+  return apply_grid(h, std::vector<int>(iNeighbor_EB, iNeighbor_EB + 6 * nB),
+                    std::vector<double>(CellSize_DB, CellSize_DB + 3 * nB),
+                    std::vector<int>());
 }
 
 // ---- layout conversions ---------------------------------------------
@@ -531,6 +535,7 @@ static void aos_to_soa(const bats_config &c, const double *S, double *U) {
 }
 
 static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
+                       const std::vector<int> &slabs, const double *bcState,
                        const double *U, double *S) {
   const int NB = c.nI, g = c.nG, NG = NB + 2 * g;
   const long nB = c.nBlock, vs = nB * NB * NB * NB;
@@ -545,10 +550,12 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
           double *dst = at(b, i, j, k);
           for (int v = 0; v < NVAR; v++) dst[v] = U[v * vs + off];
         }
-  // face ghost slabs, as exchange_messages + float BCs would fill them
+  // face ghost slabs, as exchange_messages + float or fixed BCs would fill
+  // them (resolution-change sides as float)
   for (long b = 0; b < nB; b++)
     for (int side = 0; side < 6; side++) {
       const int n = nei[6 * b + side], dim = side / 2, hi = side % 2;
+      const bool fixedSide = n <= -2 && slabs[8 * (-2 - n) + 2] == 2;
       for (int layer = 1; layer <= g; layer++) {
         const int dst = hi ? NB - 1 + layer : -layer;
         const int src = n >= 0 ? (hi ? layer - 1 : NB - layer) : (hi ? NB - 1 : 0);
@@ -561,7 +568,8 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
             else { dk = dst; sk = src; di = si = q; dj = sj = p; }
             const long off = ((sb * NB + sk) * NB + sj) * NB + si;
             double *d = at(b, di, dj, dk);
-            for (int v = 0; v < NVAR; v++) d[v] = U[v * vs + off];
+            for (int v = 0; v < NVAR; v++)
+              d[v] = fixedSide ? bcState[side * NVAR + v] : U[v * vs + off];
           }
       }
     }
@@ -616,7 +624,7 @@ extern "C" int bats_download_state(bats_handle *h, double *State_VGB) {
   std::vector<double> tmp((size_t)NVAR * h->nCell);
   int st = get_soa(h, tmp.data());
   if (st) return st;
-  soa_to_aos(h->cfg, h->hNei, tmp.data(), State_VGB);
+  soa_to_aos(h->cfg, h->hNei, h->hSlabs, h->bcState, tmp.data(), State_VGB);
   return BATS_OK;
 }
 extern "C" int bats_upload_state_soa(bats_handle *h, const double *U) {
@@ -745,8 +753,8 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
-  // the per-block routines work on same-level grids
-  if (h->nSlab > 0) return BATS_EINVAL;
+  // the per-block routines work on grids without resolution changes
+  if (h->nResSlab > 0) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only
   if (h->elemSize != 8) return BATS_EINVAL;
   return ensure_block_buffers(h);
@@ -1246,7 +1254,7 @@ int amr_fill_ghosts(bats_handle *h, const void *U) {
   m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
   m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);
   m.beta = h->cfg.LimiterBeta; m.lim = limiter_code(h->cfg);
-  m.corr = h->corr; m.err = h->err;
+  m.corr = h->corr; m.err = h->err; m.bcState = h->dBcState;
   const long n = 2L * h->NB * h->NB * h->nSlab;
   const dim3 g((unsigned)((n + 255) / 256)), t(256);
   if (h->elemSize == 8)
@@ -1278,7 +1286,6 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
         if (sl[0] != b || sl[1] != s) return BATS_EINVAL;
       }
     }
-  std::vector<int> kind(nSlab), fine;
   for (int t = 0; t < nSlab; t++) {
     const int *sl = iSlab_IS + 8 * t;
     if (sl[0] < 0 || sl[0] >= nB || sl[1] < 0 || sl[1] > 5) return BATS_EINVAL;
@@ -1288,14 +1295,30 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
           (sl[5] != 0 && sl[5] != H) || sl[7] < 0 || sl[7] >= nSlab ||
           iSlab_IS[8 * sl[7] + 2] != 1)
         return BATS_EINVAL;
-      fine.push_back(t);
     } else if (sl[2] == 1) {
       for (int q = 0; q < 4; q++)
         if (sl[3 + q] < 0 || sl[3 + q] >= nB) return BATS_EINVAL;
     } else {
       return BATS_EINVAL;
     }
-    kind[t] = sl[2];
+  }
+  return apply_grid(
+      h, std::vector<int>(iNeighbor_EB, iNeighbor_EB + 6 * nB),
+      std::vector<double>(CellSize_DB, CellSize_DB + 3 * nB),
+      std::vector<int>(iSlab_IS, iSlab_IS + 8 * (size_t)nSlab));
+}
+
+// Allocate the state buffers with nSlab ghost slabs after the interior
+// cells and install the (already checked) neighbour and slab lists: kinds
+// 0 and 1 at resolution changes, kind 2 at fixed outer sides.
+static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
+                             const double *CellSize_DB, int nSlab,
+                             const int *iSlab_IS, int nResSlab) {
+  const int nB = h->cfg.nBlock, NB = h->NB;
+  std::vector<int> kind(nSlab), fine;
+  for (int t = 0; t < nSlab; t++) {
+    kind[t] = iSlab_IS[8 * t + 2];
+    if (kind[t] == 0) fine.push_back(t);
   }
   HIPCHK(hipStreamSynchronize(h->stream));
   // state buffers with the slab region after the interior cells
@@ -1314,6 +1337,7 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
   h->corr = nullptr;
   h->vs = vs;
   h->nSlab = nSlab;
+  h->nResSlab = nResSlab;
   h->nFine = (int)fine.size();
   h->hSlabs.assign(iSlab_IS, iSlab_IS + 8 * nSlab);
   if (nSlab > 0) {
@@ -1338,6 +1362,89 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
   HIPCHK(hipMemcpy(h->cellsize, CellSize_DB, sizeof(double) * 3 * nB,
                    hipMemcpyHostToDevice));
   return BATS_OK;
+}
+
+
+// Install the caller's grid (outer sides -1, resolution-change slabs) with
+// the outer boundary types: every -1 side of a fixed/inflow type becomes a
+// kind-2 slab whose two ghost layers hold the side's state.  A grid with no
+// slabs on a handle that has none keeps the state buffers.
+static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
+                      const std::vector<double> &cs,
+                      const std::vector<int> &userSlabs) {
+  const int nB = h->cfg.nBlock;
+  std::vector<int> nei = userNei, slabs = userSlabs;
+  const int nRes = (int)(userSlabs.size() / 8);
+  for (int b = 0; b < nB; b++)
+    for (int s = 0; s < 6; s++) {
+      if (nei[6 * b + s] != -1) continue;
+      const int t = h->bcType[s];
+      if (t == BATS_BC_PERIODIC) return BATS_EINVAL;
+      if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW) continue;
+      nei[6 * b + s] = -2 - (int)(slabs.size() / 8);
+      const int sl[8] = {b, s, 2, 0, 0, 0, 0, 0};
+      slabs.insert(slabs.end(), sl, sl + 8);
+    }
+  const int nSlab = (int)(slabs.size() / 8);
+  if (nSlab == 0 && h->nSlab == 0) {
+    h->hNei = nei;
+    HIPCHK(hipMemcpyAsync(h->nei, nei.data(), sizeof(int) * 6 * nB,
+                          hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->cellsize, cs.data(), sizeof(double) * 3 * nB,
+                          hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  } else {
+    int st = install_slab_grid(h, nei.data(), cs.data(), nSlab, slabs.data(),
+                               nRes);
+    if (st) return st;
+    // ghost layers of the fixed sides in both state buffers (the stages
+    // refill them with every other slab)
+    for (int u = 0; u < 2 && nSlab > 0; u++) {
+      st = amr_fill_ghosts(h, h->U[u]);
+      if (st) return st;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  h->userNei = userNei;
+  h->userCellSize = cs;
+  h->userSlabs = userSlabs;
+  return BATS_OK;
+}
+
+extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
+                                      const double *CellState_VI) {
+  if (!h || !iTypeCellBc_I) return BATS_EINVAL;
+  bool fixed = false;
+  for (int s = 0; s < 6; s++) {
+    const int t = iTypeCellBc_I[s];
+    if (t != BATS_BC_PERIODIC && t != BATS_BC_FLOAT && t != BATS_BC_FIXED &&
+        t != BATS_BC_INFLOW)
+      return BATS_EBC;
+    fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW;
+  }
+  if (fixed && !CellState_VI) return BATS_EINVAL;
+  if (fixed)
+    for (int t = 0; t < 6 * NVAR; t++)
+      if (!std::isfinite(CellState_VI[t])) return BATS_EINVAL;
+  drop_graphs(h);
+  int oldType[6];
+  double oldState[6 * NVAR];
+  std::memcpy(oldType, h->bcType, sizeof oldType);
+  std::memcpy(oldState, h->bcState, sizeof oldState);
+  std::memcpy(h->bcType, iTypeCellBc_I, sizeof h->bcType);
+  for (int t = 0; t < 6 * NVAR; t++)
+    h->bcState[t] = CellState_VI ? CellState_VI[t] : 0.0;
+  HIPCHK(hipMemcpy(h->dBcState, h->bcState, sizeof h->bcState,
+                   hipMemcpyHostToDevice));
+  if (h->userNei.empty()) return BATS_OK;  // applied by the next grid call
+  const int st = apply_grid(h, h->userNei, h->userCellSize, h->userSlabs);
+  if (st) {
+    std::memcpy(h->bcType, oldType, sizeof oldType);
+    std::memcpy(h->bcState, oldState, sizeof oldState);
+    HIPCHK(hipMemcpy(h->dBcState, h->bcState, sizeof h->bcState,
+                     hipMemcpyHostToDevice));
+  }
+  return st;
 }
 
 extern "C" int bats_synchronize(bats_handle *h) {

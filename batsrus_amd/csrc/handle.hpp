@@ -71,10 +71,21 @@ struct bats_handle {
   int graphValid[2];
   double graphTLimit[2];
   // two-level grids: resolution-change slabs (iNeighbor_EB = -2 - slab)
+  // and the ghost slabs of fixed outer sides (kind 2) that follow them
   int nSlab, nFine;
+  int nResSlab;             // resolution-change slabs (kinds 0 and 1)
   std::vector<int> hSlabs;  // 8 ints per slab
   int *slabs, *slabKind, *fineSlabs;
   void *corr;               // summed fine fluxes of the finer sides
+  // the grid as the caller gave it (outer sides -1), kept so that a change
+  // of the outer boundary types can rebuild the slab list
+  std::vector<int> userNei, userSlabs;
+  std::vector<double> userCellSize;
+  // outer boundary type of each side (BATS_BC_*) and the ghost state of the
+  // fixed/inflow sides, CellState_VI(8, 6) column-major; device copy
+  int bcType[6];
+  double bcState[6 * NVAR];
+  double *dBcState;
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };

@@ -546,6 +546,7 @@ struct AmrArgs {
   void *corr;           // type T, [slab][q][p][sub][9]
   int *err;
   const void *lowCrit;  // low order face weights (type T) or null
+  const double *bcState; // CellState_VI(8, 6) of the fixed outer sides
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -565,7 +566,9 @@ __device__ __forceinline__ T minmod2(T a, T b) {
 // of the 2x2x2 fine cells it covers; the fine ghost is the coarse cell plus
 // its slopes (minmod of the coarse block's one-sided differences, the one
 // available difference at the block edge) times the +-1/4 offsets of the
-// fine centre.  One thread per ghost cell.
+// fine centre.  Kind-2 slabs (fixed outer sides) take the side's state,
+// as set_boundary_for_cell does (src/ModUpdateStateFast.f90:1262-1268).
+// One thread per ghost cell.
 template <typename T>
 __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -580,7 +583,9 @@ __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   auto cell = [&](int blk, int x, int y, int z) {
     return (((long)blk * NB + z) * NB + y) * NB + x;
   };
-  if (kind == 1) {
+  if (kind == 2) {
+    for (int v = 0; v < NVAR; v++) val[v] = T(a.bcState[side * NVAR + v]);
+  } else if (kind == 1) {
     const int F = sl[3 + (p / H) + 2 * (q / H)];
     for (int v = 0; v < NVAR; v++) val[v] = T(0);
     for (int ai = 0; ai < 2; ai++)

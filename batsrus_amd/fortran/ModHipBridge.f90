@@ -42,7 +42,7 @@ module ModHipBridge
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
        bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
        bats_set_amr_grid, bats_set_low_order_face, bats_get_low_order_crit, &
-       bats_set_step_graph, &
+       bats_set_step_graph, bats_set_cell_boundary, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
        bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
        bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
@@ -53,9 +53,13 @@ module ModHipBridge
        update_state, &
        calc_timestep, set_global_timestep, advance_explicit, &
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
-       set_low_order_face, &
+       set_low_order_face, set_outer_boundary, &
        init_timestep, get_time
+  public:: BATS_BC_PERIODIC, BATS_BC_FLOAT, BATS_BC_FIXED, BATS_BC_INFLOW
 
+  ! iTypeCellBc values (src/ModCellBoundary.f90:23-41)
+  integer(c_int), parameter:: BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, &
+       BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10
   integer(c_int), parameter:: BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, &
        BATS_LIM_MC3 = 3, BATS_LIM_BETA = 4, BATS_LIM_NO = 5
   integer(c_int), parameter:: BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, &
@@ -251,6 +255,13 @@ module ModHipBridge
        real(c_double), intent(in):: CellSize_DB(*)
        integer(c_int), value:: nSlab
      end function bats_set_amr_grid
+     integer(c_int) function bats_set_cell_boundary(Handle, iTypeCellBc_I, &
+          CellState_VI) bind(C, name="bats_set_cell_boundary")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), intent(in):: iTypeCellBc_I(6)
+       real(c_double), intent(in):: CellState_VI(8,6)
+     end function bats_set_cell_boundary
      integer(c_int) function bats_set_step_graph(Handle, UseGraph) &
           bind(C, name="bats_set_step_graph")
        import:: c_int, c_ptr
@@ -454,6 +465,17 @@ contains
          merge(1, 0, DoResChangeOnly), merge(1, 0, DoMonotoneRestrict)), &
          'calc_face_value')
   end subroutine calc_face_value
+  !============================================================================
+  subroutine set_outer_boundary(iTypeCellBc_I, CellState_VI)
+    ! outer boundary types (#OUTERBOUNDARY) and the ghost states of the
+    ! fixed/inflow sides, CellState_VI of src/ModPhysics.f90:227,679;
+    ! reallocates the state, so call before sync_state_to_gpu
+    integer(c_int), intent(in):: iTypeCellBc_I(6)
+    real(c_double), intent(in):: CellState_VI(8,6)
+    !--------------------------------------------------------------------------
+    call bats_check(bats_set_cell_boundary(hBats, iTypeCellBc_I, &
+         CellState_VI), 'set_outer_boundary')
+  end subroutine set_outer_boundary
   !============================================================================
   subroutine set_low_order_face
     ! low order face weights of all blocks (src/ModFaceValue.f90:2699)

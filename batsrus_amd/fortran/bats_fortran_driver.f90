@@ -26,9 +26,13 @@ program bats_fortran_driver
   logical:: UseAdaptiveLowOrder = .false.
   integer:: nLowOrder = 1
   real(c_double):: pCritLow = 2.0, pCritHigh = 1.5, VelCrit = 2.0
+  ! outer boundaries of the sides where iNeighbor_EB = -1 (float by default)
+  integer(c_int):: iTypeCellBc(6) = 5
+  real(c_double):: CellState_VI(8,6) = 0.0
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
        iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
-       UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit
+       UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit, &
+       iTypeCellBc, CellState_VI
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -72,6 +76,8 @@ program bats_fortran_driver
   Config%VelCrit = VelCrit
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
+  if(any(iTypeCellBc /= BATS_BC_FLOAT)) &
+       call set_outer_boundary(iTypeCellBc, CellState_VI)
   call sync_state_to_gpu(State_VGB)
 
   if(TypeLoop == 'blocks')then

@@ -102,8 +102,9 @@ int bats_finalize(bats_handle *h);
 
 /* Grid description for same-level blocks.  iNeighbor_EB[6*b + side]
  * (side = x-,x+,y-,y+,z-,z+) is the 0-based index of the neighbouring
- * block or -1 for a float (zero-gradient) outer boundary
- * (src/ModCellBoundary.f90:518-559); periodic sides point at the wrapped
+ * block or -1 for an outer boundary: float (zero-gradient,
+ * src/ModCellBoundary.f90:518-559) unless bats_set_cell_boundary makes the
+ * side fixed; periodic sides point at the wrapped
  * neighbour.  CellSize_DB[3*b + d] = cell size of block b in dimension d.
  * Replaces the BATL tree/geometry queries (DiLevel_EB, CellSize_DB,
  * CellFace_DB, CellVolume_B). */
@@ -124,10 +125,31 @@ int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
  * the sum of the four fine face fluxes, as save_cons_flux / apply_cons_flux
  * do (src/ModConserveFlux.f90:79-441, src/ModAdvanceExplicit.f90:95-128).
  * Reallocates the state: call before uploading it.  Not with the 8-wave
- * source; the per-block routines need a same-level grid. */
+ * source; the per-block routines need a grid without resolution changes. */
 int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const double *CellSize_DB, int nSlab,
                       const int *iSlab_IS);
+
+/* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41) */
+enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_FIXED = 9,
+       BATS_BC_INFLOW = 10 };
+
+/* Outer boundary conditions: iTypeCellBc_I[side] for the sides x-, x+, y-,
+ * y+, z-, z+ of the domain, applied where iNeighbor_EB = -1.  Float copies
+ * the boundary cell into both ghost layers; fixed and inflow set them to
+ * CellState_VI[8*side .. 8*side+7] (rho, rho u, B, p), as
+ * set_boundary_for_cell does in set_boundary_fast
+ * (src/ModUpdateStateFast.f90:1145-1275, CellState_VI of
+ * src/ModPhysics.f90:227,679); a periodic side must have no -1 neighbour.
+ * Other types return BATS_EBC (the reference stops there, :1272; time
+ * varying inflow needs the solar wind input, which is out of scope).  The
+ * default is float on every side.  Fixed sides keep their ghost layers as
+ * slabs after the interior cells of the state, so setting them (and any
+ * later bats_set_grid / bats_set_amr_grid on such a handle) reallocates the
+ * state: call before uploading it.  CellState_VI may be null when no side
+ * is fixed or inflow. */
+int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
+                           const double *CellState_VI);
 
 /* With UseGraph = 1, bats_advance captures one time step (the global time
  * step and every stage's kernels) into a hipGraph on its first step and

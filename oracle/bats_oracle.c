@@ -47,6 +47,12 @@
  *        prolongation = coarse cell + minmod slopes x 1/4) stand in for the
  *        absent BATL message passing -- pinned by closed forms only (exact on
  *        linear data, uniform state kept): parity unpinned against BATL.
+ *   fixed/inflow outer boundaries (ghost = CellState_VI of the side)
+ *        pinned: ghost cells equal the side's state exactly; a uniform state
+ *        equal to the boundary state stays unchanged; the boundary face flux
+ *        of a first order run equals the numerical flux of (interior cell,
+ *        boundary state); a fixed state equal to the float copy of a frozen
+ *        boundary gives the float run (test_boundary_oracle).
  *   HLLD guard constant cTiny (share/ModNumConst, absent): parity unpinned;
  *        configurable, default 1e-6.
  */
@@ -1105,9 +1111,10 @@ void ob_calc_timestep(const ob_config *c, const double *CellSize,
 
 /* ---------------------------------------------------------------------- */
 /* exchange_messages for same-level blocks (periodic neighbours and float */
-/* outer boundaries), face ghost slabs only.                              */
+/* or fixed outer boundaries), face ghost slabs only.                     */
 /* src/ModMessagePass.f90:27-290 (BATL message_pass_cell, absent),         */
-/* src/ModCellBoundary.f90:518-559                                         */
+/* src/ModCellBoundary.f90:518-559 (float), set_boundary_for_cell of       */
+/* src/ModUpdateStateFast.f90:1251-1275 (fixed/inflow: CellState_VI)       */
 /* ---------------------------------------------------------------------- */
 void ob_exchange_messages(const ob_config *c, int nBlock,
                           const int *iNeighbor_EB, double *State) {
@@ -1124,6 +1131,22 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
       if (nb <= -2) continue; /* resolution change: ob_exchange_amr */
       int dim = side / 2, hi = side % 2;
       int nG = c->nG;
+      if (nb == -1 && (c->iTypeCellBc_I[side] == OB_BC_FIXED ||
+                       c->iTypeCellBc_I[side] == OB_BC_INFLOW)) {
+        const double *St = c->CellState_VI + side * OB_NVAR;
+        int n1 = dim == 0 ? s.nJ : s.nI, n2 = dim == 2 ? s.nJ : s.nK;
+        int nd = dim == 0 ? s.nI : dim == 1 ? s.nJ : s.nK, p, q;
+        for (g = 1; g <= nG; g++) {
+          int ig = hi ? nd + g : 1 - g;
+          for (q = 1; q <= n2; q++)
+            for (p = 1; p <= n1; p++) {
+              long o = dim == 0 ? cidx(&s, ig, p, q)
+                     : dim == 1 ? cidx(&s, p, ig, q) : cidx(&s, p, q, ig);
+              memcpy(Blk + o, St, NV * 8);
+            }
+        }
+        continue;
+      }
       for (g = 1; g <= nG; g++) {
         if (dim == 0) {
           int ig = hi ? s.nI + g : 1 - g;

@@ -44,6 +44,12 @@ enum { OB_LIM_MINMOD = 1, OB_LIM_MC = 2, OB_LIM_MC3 = 3, OB_LIM_BETA = 4,
 /* TypeFlux values (src/ModFaceFlux.f90:396-404) */
 enum { OB_FLUX_RUSANOV = 1, OB_FLUX_LINDE = 2, OB_FLUX_HLLD = 3 };
 
+/* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41);
+ * 0 (unset) means float.  Fixed and inflow take the side's CellState_VI
+ * state in the ghost cells (src/ModUpdateStateFast.f90:1262-1268). */
+enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_FIXED = 9,
+       OB_BC_INFLOW = 10 };
+
 /* Error codes (the reference calls stop_mpi at these points) */
 enum { OB_OK = 0, OB_ENEGFAST2 = 2, OB_EHLLD = 3, OB_EBC = 5 };
 
@@ -72,6 +78,11 @@ typedef struct {
   int UseAdaptiveLowOrder;
   int nLowOrder;       /* 1 or 2 */
   double pCritLow, pCritHigh, VelCrit;
+  /* outer boundary type of each side (x-,x+,y-,y+,z-,z+) and the ghost
+   * state of the fixed/inflow sides, CellState_VI(nVar, 6) column-major
+   * (src/ModPhysics.f90:227,679) */
+  int iTypeCellBc_I[6];
+  double CellState_VI[6 * OB_NVAR];
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */

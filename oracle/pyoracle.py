@@ -30,7 +30,12 @@ class ObConfig(C.Structure):
                 ("UseDivbSource", C.c_int),
                 ("UseAdaptiveLowOrder", C.c_int), ("nLowOrder", C.c_int),
                 ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
-                ("VelCrit", C.c_double)]
+                ("VelCrit", C.c_double),
+                ("iTypeCellBc_I", C.c_int * 6),
+                ("CellState_VI", C.c_double * 48)]
+
+# iTypeCellBc values (src/ModCellBoundary.f90:23-41)
+BC_TYPES = {"periodic": 4, "float": 5, "fixed": 9, "inflow": 10}
 
 
 class ObSession(C.Structure):
@@ -108,7 +113,11 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 gamma=5.0 / 3.0, pMin=-1.0, UseDbTrick=False,
                 UseDtFixed=False, DtFixed=0.0, cTiny=1e-6,
                 UseDivbSource=False, UseAdaptiveLowOrder=False, nLowOrder=1,
-                pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0):
+                pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0, TypeCellBc=None,
+                CellState_VI=None):
+    """TypeCellBc: six names or iTypeCellBc codes (x-, x+, y-, y+, z-, z+;
+    default float); CellState_VI: (6, 8) ghost states of the fixed sides
+    in conservative variables (rho, rho u, B, p)."""
     c = ObConfig()
     c.nI, c.nJ, c.nK, c.nG, c.nDim, c.nOrder = nI, nJ, nK, nG, nDim, nOrder
     c.iLimiter = LIMITERS[limiter] if isinstance(limiter, str) else limiter
@@ -120,6 +129,12 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.UseDivbSource = int(UseDivbSource)
     c.UseAdaptiveLowOrder, c.nLowOrder = int(UseAdaptiveLowOrder), nLowOrder
     c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit
+    for side, t in enumerate(TypeCellBc or ["float"] * 6):
+        c.iTypeCellBc_I[side] = BC_TYPES[t] if isinstance(t, str) else int(t)
+    if CellState_VI is not None:
+        st = np.asarray(CellState_VI, dtype=np.float64).reshape(48)
+        for t in range(48):
+            c.CellState_VI[t] = st[t]
     return c
 
 

@@ -24,15 +24,27 @@ LOW = dict(UseAdaptiveLowOrder=True, nLowOrder=1, pCritLow=1.3,
            pCritHigh=1.0, VelCrit=0.0)
 
 
-@pytest.mark.parametrize("option", ["none", "powell", "loworder"])
+# fixed x- and inflow x+ sides (iTypeCellBc 9, 10) with their ghost states
+BC_STATES = np.array([[1.0 + 0.1 * s, 0.2, -0.1 * s, 0.05, 0.4, -0.3, 0.1,
+                       0.8 + 0.05 * s] for s in range(6)])
+
+
+@pytest.mark.parametrize("option", ["none", "powell", "loworder", "fixedbc"])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
 def test_fortran_driver_matches_oracle(loop, option):
     print(f"case fortran_driver loop={loop} option={option}", flush=True)
     powell = option == "powell"
     low = LOW if option == "loworder" else {}
+    bc = {}
+    extra = ""
+    if option == "fixedbc":
+        bc = dict(TypeCellBc=[9, 10, 4, 4, 4, 4], CellState_VI=BC_STATES)
+        extra = (", iTypeCellBc=9,10,4,4,4,4, CellState_VI=" +
+                 ",".join("%.17g" % v for v in BC_STATES.reshape(-1)))
     if not os.path.exists(EXE):
         pytest.fail("Fortran driver not built (build() compiles it)")
-    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8))
+    grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8),
+                     periodic=(not bc, True, True))
     U0 = smooth_mhd_soa(grid, seed=21)
     S0 = grid.soa_to_state_vgb(U0)
     grid.fill_ghosts(S0)
@@ -42,11 +54,11 @@ def test_fortran_driver_matches_oracle(loop, option):
             f.write("&RUN nRootI=2, nRootJ=2, nRootK=2, nCell=8, nStep=%d, "
                     "nOrder=2, nStage=2, iLimiter=3, iFlux=3, "
                     "LimiterBeta=1.5, Cfl=0.8, TypeLoop='%s', "
-                    "UseDivbSource=%s%s /\n"
+                    "UseDivbSource=%s%s%s /\n"
                     % (nstep, loop, ".true." if powell else ".false.",
                        (", UseAdaptiveLowOrder=.true., nLowOrder=1, "
                         "pCritLow=1.3, pCritHigh=1.0, VelCrit=0.0")
-                       if low else ""))
+                       if low else "", extra))
         with open(os.path.join(d, "grid.bin"), "wb") as f:
             f.write(np.ascontiguousarray(grid.iNeighbor_EB, np.int32).tobytes())
             f.write(np.ascontiguousarray(grid.CellSize_DB).tobytes())
@@ -57,7 +69,8 @@ def test_fortran_driver_matches_oracle(loop, option):
         assert r.returncode == 0, r.stdout + r.stderr
         S = np.fromfile(os.path.join(d, "state_out.bin")).reshape(S0.shape)
     cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3", beta=1.5,
-                                   cfl=0.8, UseDivbSource=powell, **low),
+                                   cfl=0.8, UseDivbSource=powell, **low,
+                                   **bc),
                               grid)
     o = po.Session(grid, cfg, S0)
     o.init_timestep()
