@@ -68,6 +68,22 @@ struct StageArgs {
                        // crit_index); read only by the low order variant
 };
 
+// the cyclic frame rotation (x, y, z) -> (y, z, x) that turns a y face into
+// an x face: rotated component d of a vector is the original component
+// (d + 1) % 3; rot_src gives the original variable of each rotated slot
+__host__ __device__ constexpr int rot_src(int v) {
+  return (v >= MX && v <= MZ) ? MX + (v - MX + 1) % 3
+       : (v >= BX && v <= BZ) ? BX + (v - BX + 1) % 3 : v;
+}
+// flux of the rotated frame back to the original components
+template <typename T>
+__device__ __forceinline__ void unrotate_flux(T *F) {
+  const T m0 = F[MX], m1 = F[MY], m2 = F[MZ];
+  F[MY] = m0; F[MZ] = m1; F[MX] = m2;
+  const T b0 = F[BX], b1 = F[BY], b2 = F[BZ];
+  F[BY] = b0; F[BZ] = b1; F[BX] = b2;
+}
+
 // Stage kernel variants: the 8-wave div B source and the adaptive low order
 // face values are compile-time options so the plain kernel carries neither.
 enum { OPT_POWELL = 1, OPT_LOWORDER = 2 };
@@ -367,39 +383,34 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         T F[NFLUX], vdt, unA;
         const double dxf = a.cellsize[3 * bf], dyf = a.cellsize[3 * bf + 1],
                      dzf = a.cellsize[3 * bf + 2];
-        if (dir == 0) {
+        // y faces are evaluated as x faces of the cyclically rotated frame
+        // (x, y, z) -> (y, z, x): the same normal / tangent1 / tangent2
+        // order as the y-normal flux, so a wave whose faces mix x and y runs
+        // one flux evaluation instead of both
+        // the four cells of the stencil along the face normal: a row of the
+        // plane for x faces, a column (stride NS) for y faces
+        const int po = dir == 0 ? (row + 2) * NS + fp : fp * NS + row + 2;
+        const int st = dir == 0 ? 1 : NS;
 #pragma unroll
-          for (int v = 0; v < NVAR; v++) {
-            c0[v] = sP[sPi(lbf, v, row + 2, fp + 0)];
-            c1[v] = sP[sPi(lbf, v, row + 2, fp + 1)];
-            c2[v] = sP[sPi(lbf, v, row + 2, fp + 2)];
-            c3[v] = sP[sPi(lbf, v, row + 2, fp + 3)];
-          }
-          if (LOWORD)
-            err |= face_flux_w<0, FLUX, LIM>(
-                ph, T(dyf * dzf), beta,
-                T(1) - crit[crit_index<NB>(bf, 0, fp, row, k)], c0, c1, c2,
-                c3, F, vdt, unA);
-          else
-            err |= face_flux<0, FLUX, LIM>(ph, T(dyf * dzf), beta, c0, c1,
-                                           c2, c3, F, vdt, unA);
-        } else {
-#pragma unroll
-          for (int v = 0; v < NVAR; v++) {
-            c0[v] = sP[sPi(lbf, v, fp + 0, row + 2)];
-            c1[v] = sP[sPi(lbf, v, fp + 1, row + 2)];
-            c2[v] = sP[sPi(lbf, v, fp + 2, row + 2)];
-            c3[v] = sP[sPi(lbf, v, fp + 3, row + 2)];
-          }
-          if (LOWORD)
-            err |= face_flux_w<1, FLUX, LIM>(
-                ph, T(dxf * dzf), beta,
-                T(1) - crit[crit_index<NB>(bf, 1, row, fp, k)], c0, c1, c2,
-                c3, F, vdt, unA);
-          else
-            err |= face_flux<1, FLUX, LIM>(ph, T(dxf * dzf), beta, c0, c1,
-                                           c2, c3, F, vdt, unA);
+        for (int v = 0; v < NVAR; v++) {
+          const int w = dir == 0 ? v : rot_src(v);
+          const int o = (lbf * NVAR + w) * NR * NS + po;
+          c0[v] = sP[o];
+          c1[v] = sP[o + st];
+          c2[v] = sP[o + 2 * st];
+          c3[v] = sP[o + 3 * st];
         }
+        const T area = T(dir == 0 ? dyf * dzf : dxf * dzf);
+        if (LOWORD)
+          err |= face_flux_w<0, FLUX, LIM>(
+              ph, area, beta,
+              T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
+                                   : crit_index<NB>(bf, 1, row, fp, k)],
+              c0, c1, c2, c3, F, vdt, unA);
+        else
+          err |= face_flux<0, FLUX, LIM>(ph, area, beta, c0, c1, c2, c3, F,
+                                         vdt, unA);
+        if (dir != 0) unrotate_flux(F);
         F[PP] = vdt;
         if (a.corr && (fp == 0 || fp == NB))
           corrected(bf, 2 * dir + (fp == NB), row, k, F);
