@@ -481,16 +481,17 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             update_cell(a.ph, uc, dF, Ucd, Uod, Un, Srcd);
           } else {
             // the stage input at the cell is used only by the SSP-RK2
-            // interpolation and for a zero time step; otherwise re-read the
-            // just-loaded StateOld lines (cache hits) instead of the U*
-            // stream
-            load_state((uc.dtZero || !uc.halfStep) ? U : Uold, vs, off, Uc);
+            // interpolation and for a zero time step
 #pragma unroll
-            for (int v = 0; v < NVAR; v++) {
-              Ucd[v] = double(Uc[v]);
-              Uod[v] = double(Uo[v]);
+            for (int v = 0; v < NVAR; v++) Uod[v] = double(Uo[v]);
+            if (uc.dtZero || !uc.halfStep) {
+              load_state(U, vs, off, Uc);
+#pragma unroll
+              for (int v = 0; v < NVAR; v++) Ucd[v] = double(Uc[v]);
+              update_cell(a.ph, uc, dF, Ucd, Uod, Un);
+            } else {
+              update_cell(a.ph, uc, dF, Uod, Uod, Un);
             }
-            update_cell(a.ph, uc, dF, Ucd, Uod, Un);
           }
 #pragma unroll
           for (int v = 0; v < NVAR; v++) Uout[v * vs + off] = T(Un[v]);
