@@ -638,13 +638,17 @@ extern "C" int bats_download_state_soa(bats_handle *h, double *U) {
 
 // ---- per-block routines ---------------------------------------------
 static int ensure_block_buffers(bats_handle *h) {
-  if (h->Usnap) return BATS_OK;
+  // the exchange snapshot follows the state layout (a grid change frees
+  // it); the face arrays depend on the block size only
+  if (!h->Usnap) {
+    HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->vs));
+    HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
+                          (size_t)h->elemSize * NVAR * h->vs,
+                          hipMemcpyDeviceToDevice, h->stream));
+  }
+  if (h->Src) return BATS_OK;
   const int NB = h->NB;
   const size_t nF = (size_t)(NB + 1) * NB * NB;
-  HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->vs));
-  HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
-                        (size_t)h->elemSize * NVAR * h->vs,
-                        hipMemcpyDeviceToDevice, h->stream));
   double **fs[] = {&h->LX, &h->RX, &h->LY, &h->RY, &h->LZ, &h->RZ};
   for (double **p : fs) {
     HIPCHK(hipMalloc(p, sizeof(double) * NVAR * nF));
@@ -1321,41 +1325,63 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
     if (kind[t] == 0) fine.push_back(t);
   }
   HIPCHK(hipStreamSynchronize(h->stream));
-  // state buffers with the slab region after the interior cells
+  // every new buffer is allocated and filled before the old ones are
+  // released, so a failed allocation leaves the installed grid unchanged
   const long vs = h->nCell + 2L * NB * NB * nSlab;
-  for (int u = 0; u < 2; u++) {
-    HIPCHK(hipFree(h->U[u]));
-    h->U[u] = nullptr;
-    HIPCHK(hipMalloc(&h->U[u], (size_t)h->elemSize * NVAR * vs));
-    HIPCHK(hipMemset(h->U[u], 0, (size_t)h->elemSize * NVAR * vs));
+  const size_t stateBytes = (size_t)h->elemSize * NVAR * vs;
+  const size_t nc = (size_t)nSlab * NB * NB * 4 * 9;
+  void *nU[2] = {nullptr, nullptr}, *nCorr = nullptr;
+  int *nSlabs = nullptr, *nKind = nullptr, *nFine = nullptr;
+  auto fill = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (int u = 0; u < 2 && e == hipSuccess; u++) {
+      e = hipMalloc(&nU[u], stateBytes);
+      if (e == hipSuccess) e = hipMemset(nU[u], 0, stateBytes);
+    }
+    if (nSlab > 0) {
+      if (e == hipSuccess) e = hipMalloc(&nSlabs, sizeof(int) * 8 * nSlab);
+      if (e == hipSuccess) e = hipMalloc(&nKind, sizeof(int) * nSlab);
+      if (e == hipSuccess)
+        e = hipMemcpy(nSlabs, iSlab_IS, sizeof(int) * 8 * nSlab,
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(nKind, kind.data(), sizeof(int) * nSlab,
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMalloc(&nCorr, (size_t)h->elemSize * nc);
+      if (e == hipSuccess) e = hipMemset(nCorr, 0, (size_t)h->elemSize * nc);
+      if (e == hipSuccess && !fine.empty())
+        e = hipMalloc(&nFine, sizeof(int) * fine.size());
+      if (e == hipSuccess && !fine.empty())
+        e = hipMemcpy(nFine, fine.data(), sizeof(int) * fine.size(),
+                      hipMemcpyHostToDevice);
+    }
+    return e;
+  };
+  const hipError_t e = fill();
+  if (e != hipSuccess) {
+    void *made[] = {nU[0], nU[1], nSlabs, nKind, nCorr, nFine};
+    for (void *p : made)
+      if (p) (void)hipFree(p);
+    fprintf(stderr, "bats_hip: grid allocation failed: %s\n",
+            hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
   }
-  if (h->Usnap) { HIPCHK(hipFree(h->Usnap)); h->Usnap = nullptr; }
-  void *old[] = {h->slabs, h->slabKind, h->fineSlabs, h->corr};
+  void *old[] = {h->U[0], h->U[1], h->Usnap, h->slabs, h->slabKind,
+                 h->fineSlabs, h->corr};
   for (void *p : old)
     if (p) HIPCHK(hipFree(p));
-  h->slabs = h->slabKind = h->fineSlabs = nullptr;
-  h->corr = nullptr;
+  h->U[0] = nU[0];
+  h->U[1] = nU[1];
+  h->Usnap = nullptr;
+  h->slabs = nSlabs;
+  h->slabKind = nKind;
+  h->fineSlabs = nFine;
+  h->corr = nCorr;
   h->vs = vs;
   h->nSlab = nSlab;
   h->nResSlab = nResSlab;
   h->nFine = (int)fine.size();
   h->hSlabs.assign(iSlab_IS, iSlab_IS + 8 * nSlab);
-  if (nSlab > 0) {
-    HIPCHK(hipMalloc(&h->slabs, sizeof(int) * 8 * nSlab));
-    HIPCHK(hipMalloc(&h->slabKind, sizeof(int) * nSlab));
-    HIPCHK(hipMemcpy(h->slabs, iSlab_IS, sizeof(int) * 8 * nSlab,
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->slabKind, kind.data(), sizeof(int) * nSlab,
-                     hipMemcpyHostToDevice));
-    const size_t nc = (size_t)nSlab * NB * NB * 4 * 9;
-    HIPCHK(hipMalloc(&h->corr, (size_t)h->elemSize * nc));
-    HIPCHK(hipMemset(h->corr, 0, (size_t)h->elemSize * nc));
-    if (!fine.empty()) {
-      HIPCHK(hipMalloc(&h->fineSlabs, sizeof(int) * fine.size()));
-      HIPCHK(hipMemcpy(h->fineSlabs, fine.data(), sizeof(int) * fine.size(),
-                       hipMemcpyHostToDevice));
-    }
-  }
   h->hNei.assign(iNeighbor_EB, iNeighbor_EB + 6 * nB);
   HIPCHK(hipMemcpy(h->nei, iNeighbor_EB, sizeof(int) * 6 * nB,
                    hipMemcpyHostToDevice));
@@ -1363,7 +1389,6 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
                    hipMemcpyHostToDevice));
   return BATS_OK;
 }
-
 
 // Install the caller's grid (outer sides -1, resolution-change slabs) with
 // the outer boundary types: every -1 side of a fixed/inflow type becomes a

@@ -240,3 +240,43 @@ def test_refused_boundary_types():
         with pytest.raises(StopMpi) as ei:
             s.set_cell_boundary(["fixed"] * 6)            # no state
         assert ei.value.code == 1
+
+
+def test_block_routines_across_a_boundary_change():
+    """Per-block routines, then a change of the outer boundaries (the state
+    and the exchange snapshot are reallocated), a new upload and the
+    per-block routines again: each run matches a fresh oracle run."""
+    print("case boundary block routines regrid", flush=True)
+    g = _grid()
+    U0 = random_mhd_soa(g, seed=48)
+    float_bc = dict(TypeCellBc=["float"] * 4 + ["periodic"] * 2)
+    with Session(g, flux="rusanov", limiter="mc", beta=1.5, cfl=0.8) as s:
+        for bc in (float_bc, BC, float_bc):
+            s.set_cell_boundary(bc["TypeCellBc"], bc.get("CellState_VI"))
+            s.upload_soa(U0)
+            s.exchange_messages()
+            s.set_stage(2)
+            for b in range(g.nBlock):
+                s.calc_face_value(b + 1)
+                s.calc_face_flux(False, b + 1)
+                s.calc_timestep(b + 1)
+            dtg = s.set_global_timestep()
+            for stage in (1, 2):
+                s.set_stage(stage)
+                for b in range(g.nBlock):
+                    s.calc_face_value(b + 1)
+                    s.calc_face_flux(False, b + 1)
+                    s.update_state(b + 1)
+                    if stage == 2:
+                        s.calc_timestep(b + 1)
+                s.exchange_messages()
+            cfg = po.config_from_dict(dict(flux="rusanov", limiter="mc",
+                                           beta=1.5, cfl=0.8, **bc), g)
+            o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
+            o.init_timestep()
+            o.s.nStep += 1
+            dto = o.set_global_timestep()
+            assert abs(dtg - dto) <= 1e-12 * dto
+            assert o.advance_stage(1) == 0
+            assert o.advance_stage(2) == 0
+            assert rel_err(s.download_soa(), o.soa()) <= 1e-12
