@@ -58,7 +58,9 @@ class BatsConfig(C.Structure):
                 ("UseDivbSource", C.c_int), ("StatePrecision", C.c_int),
                 ("UseAdaptiveLowOrder", C.c_int), ("nLowOrder", C.c_int),
                 ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
-                ("VelCrit", C.c_double)]
+                ("VelCrit", C.c_double),
+                ("BetaLimiterResChange", C.c_double),
+                ("nFaceLimiterResChange", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -163,7 +165,8 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        pMin=-1.0, UseDbTrick=False, UseDtFixed=False,
                        DtFixed=0.0, cTiny=1e-6, UseDivbSource=False,
                        precision="f64", UseAdaptiveLowOrder=False,
-                       nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0)
+                       nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0,
+                       BetaLimiterResChange=2.0, nFaceLimiterResChange=2)
 
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
@@ -201,6 +204,8 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.nLowOrder = int(s["nLowOrder"])
     c.pCritLow, c.pCritHigh = s["pCritLow"], s["pCritHigh"]
     c.VelCrit = s["VelCrit"]
+    c.BetaLimiterResChange = s["BetaLimiterResChange"]
+    c.nFaceLimiterResChange = int(s["nFaceLimiterResChange"])
     return c
 
 

@@ -99,6 +99,8 @@ struct BlockArgs {
   double *Src;                          // Source_VC(9, nI, nJ, nK)
   int useSrc;
   const double *lowCrit;                // low order weights or null
+  double betaRC;                        // BetaLimiterResChange
+  int nFaceRC;                          // its faces (0: not used)
   double *dtmaxCB, *dtmaxB;
   int *err;
   int nBlock, iBlock;
@@ -140,12 +142,21 @@ __global__ void k_block_face_value(BlockArgs a) {
     to_primitive(q[s]);
   }
   double Lv[NVAR], Rv[NVAR];
+  // betas of the face's two cells (f-1, f along the normal)
+  double bL = a.beta, bR = a.beta;
+  if (a.nFaceRC > 0) {
+    const int f = dir == 0 ? fi : dir == 1 ? fj : fk;
+    const bool lo = a.nei[6 * a.iBlock + 2 * dir] < 0;
+    const bool hi = a.nei[6 * a.iBlock + 2 * dir + 1] < 0;
+    bL = cell_beta(a.beta, a.betaRC, a.nFaceRC, lo, hi, f - 1, NB);
+    bR = cell_beta(a.beta, a.betaRC, a.nFaceRC, lo, hi, f, NB);
+  }
   if (a.lowCrit)
-    face_states_w_rt(a.lim, a.beta,
+    face_states_w_rt(a.lim, bL, bR,
                      1.0 - a.lowCrit[crit_index<NB>(a.iBlock, dir, fi, fj, fk)],
                      q[0], q[1], q[2], q[3], Lv, Rv);
   else
-    face_states_rt(a.lim, a.beta, q[0], q[1], q[2], q[3], Lv, Rv);
+    face_states_rt(a.lim, bL, bR, q[0], q[1], q[2], q[3], Lv, Rv);
   for (int v = 0; v < NVAR; v++) {
     L[fo * NVAR + v] = Lv[v];
     R[fo * NVAR + v] = Rv[v];
@@ -403,6 +414,10 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (c.UseAdaptiveLowOrder &&
       ((c.nLowOrder != 1 && c.nLowOrder != 2) ||
        c.pCritLow < c.pCritHigh - 1e-15 || !(c.VelCrit >= 0.0)))
+    return BATS_EINVAL;
+  // #RESOLUTIONCHANGE limiter parameters
+  if (c.nFaceLimiterResChange < 0 ||
+      (c.nFaceLimiterResChange > 0 && !std::isfinite(c.BetaLimiterResChange)))
     return BATS_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.iDevice || c.iDevice < 0)
@@ -676,6 +691,17 @@ static bool low_order_set(const bats_config &c) {
 static bool low_order_used(const bats_config &c) {
   return low_order_set(c) && c.nLowOrder == 1;
 }
+// the sharper BetaLimiterResChange applies next to sides with DiLevel_EB
+// /= 0 (src/ModFaceValue.f90:1982-2015): the second order scheme with a
+// limiter that reads beta, LimiterBeta above it, and a grid with such sides
+static bool res_beta_used(const bats_handle *h) {
+  const bats_config &c = h->cfg;
+  return c.nOrder == 2 && c.nFaceLimiterResChange > 0 &&
+         c.LimiterBeta > c.BetaLimiterResChange &&
+         (c.iLimiter == BATS_LIM_MC || c.iLimiter == BATS_LIM_MC3 ||
+          c.iLimiter == BATS_LIM_BETA) &&
+         h->anySideRC;
+}
 
 static int launch_low_order_crit(bats_handle *h, const void *U) {
   LowOrderArgs a;
@@ -717,6 +743,8 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.useSrc = c.UseDivbSource;
   a.lowCrit = low_order_used(c) ? static_cast<const double *>(h->lowCrit)
                                 : nullptr;
+  a.betaRC = c.BetaLimiterResChange;
+  a.nFaceRC = res_beta_used(h) ? c.nFaceLimiterResChange : 0;
   a.dtmaxCB = h->dtmaxCB;
   a.dtmaxB = h->dtmaxB;
   a.err = h->err;
@@ -990,11 +1018,14 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
     HIPCHK(hipEventRecord(h->evStart[ev], h->stream));
   }
   const bool lowUse = low_order_used(c);
-  // the surrogate's face pass has no low order weights
-  if (lowUse && h->sur.mode != BATS_SUR_OFF) return BATS_EINVAL;
+  const bool resBeta = res_beta_used(h);
+  // the surrogate's face pass has no low order weights and one beta
+  if ((lowUse || resBeta) && h->sur.mode != BATS_SUR_OFF) return BATS_EINVAL;
   a.lowCrit = lowUse ? h->lowCrit : nullptr;
+  a.betaRC = c.BetaLimiterResChange;
+  a.nFaceRC = resBeta ? c.nFaceLimiterResChange : 0;
   const int opt = (c.UseDivbSource ? OPT_POWELL : 0) |
-                  (lowUse ? OPT_LOWORDER : 0);
+                  (lowUse ? OPT_LOWORDER : 0) | (resBeta ? OPT_RESBETA : 0);
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
   // resolution changes: ghost slabs of the stage input
@@ -1018,6 +1049,8 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
       m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = a.ph;
       m.beta = a.beta; m.lim = a.lim; m.corr = h->corr; m.err = h->err;
       m.lowCrit = a.lowCrit;
+      m.betaRC = a.betaRC;
+      m.nFaceRC = a.nFaceRC;
 #define BATS_FPICK(NBv, FL)                                                  \
   (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
        : launch_fineflux_##NBv##_##FL##_f64(m, h->stream))
@@ -1433,6 +1466,8 @@ static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
   h->userNei = userNei;
   h->userCellSize = cs;
   h->userSlabs = userSlabs;
+  h->anySideRC = false;
+  for (int n : nei) h->anySideRC = h->anySideRC || n < 0;
   return BATS_OK;
 }
 

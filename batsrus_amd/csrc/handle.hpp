@@ -86,6 +86,7 @@ struct bats_handle {
   int bcType[6];
   double bcState[6 * NVAR];
   double *dBcState;
+  bool anySideRC;           // some block side has DiLevel_EB /= 0
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };

@@ -149,17 +149,20 @@ __device__ __forceinline__ void slopes(Real beta, Real p0, Real p1,
 // L = P(f-1) + dL(f-1), R = P(f) - dR(f)  (src/ModFaceValue.f90:2021-2027)
 // left and right states of one variable at the face between cells 1 and 2
 // of the 4-cell stencil q0..q3 (src/ModFaceValue.f90:2021-2027)
+// The slopes of the two cells use their own beta (betaL for cell 1, betaR
+// for cell 2): the sharper BetaLimiterResChange applies per cell next to a
+// resolution change or an outer boundary (src/ModFaceValue.f90:1982-2015).
 template <int LIM, typename Real>
-__device__ __forceinline__ void face_lr(Real beta, Real q0, Real q1,
-                                        Real q2, Real q3, Real &L,
+__device__ __forceinline__ void face_lr(Real betaL, Real betaR, Real q0,
+                                        Real q1, Real q2, Real q3, Real &L,
                                         Real &R) {
   if (LIM == LIM_FIRST) {
     L = q1;
     R = q2;
   } else {
     Real dl, dr, t;
-    slopes<LIM>(beta, q0, q1, q2, dl, t);
-    slopes<LIM>(beta, q1, q2, q3, t, dr);
+    slopes<LIM>(betaL, q0, q1, q2, dl, t);
+    slopes<LIM>(betaR, q1, q2, q3, t, dr);
     L = q1 + dl;
     R = q2 - dr;
   }
@@ -168,73 +171,84 @@ __device__ __forceinline__ void face_lr(Real beta, Real q0, Real q1,
 // face_lr with the low order weight of the face: the limited slopes scaled
 // by w = 1 - LowOrderCrit (src/ModFaceValue.f90:2021-2027)
 template <int LIM, typename Real>
-__device__ __forceinline__ void face_lr_w(Real beta, Real w, Real q0, Real q1,
-                                          Real q2, Real q3, Real &L,
-                                          Real &R) {
+__device__ __forceinline__ void face_lr_w(Real betaL, Real betaR, Real w,
+                                          Real q0, Real q1, Real q2, Real q3,
+                                          Real &L, Real &R) {
   if (LIM == LIM_FIRST) {
     L = q1;
     R = q2;
   } else {
     Real dl, dr, t;
-    slopes<LIM>(beta, q0, q1, q2, dl, t);
-    slopes<LIM>(beta, q1, q2, q3, t, dr);
+    slopes<LIM>(betaL, q0, q1, q2, dl, t);
+    slopes<LIM>(betaR, q1, q2, q3, t, dr);
     L = q1 + dl * w;
     R = q2 - dr * w;
   }
 }
 
 template <int LIM, typename Real>
-__device__ __forceinline__ void face_states_w(Real beta, Real w,
+__device__ __forceinline__ void face_states_w(Real betaL, Real betaR, Real w,
                                               const Real *q0, const Real *q1,
                                               const Real *q2, const Real *q3,
                                               Real *L, Real *R) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++)
-    face_lr_w<LIM>(beta, w, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
+    face_lr_w<LIM>(betaL, betaR, w, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
 }
 
 template <int LIM, typename Real>
-__device__ __forceinline__ void face_states(Real beta, const Real *q0,
-                                            const Real *q1, const Real *q2,
-                                            const Real *q3, Real *L,
-                                            Real *R) {
+__device__ __forceinline__ void face_states(Real betaL, Real betaR,
+                                            const Real *q0, const Real *q1,
+                                            const Real *q2, const Real *q3,
+                                            Real *L, Real *R) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++)
-    face_lr<LIM>(beta, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
+    face_lr<LIM>(betaL, betaR, q0[v], q1[v], q2[v], q3[v], L[v], R[v]);
 }
 
-// runtime-limiter variant used by the per-block parity routines
+// runtime-limiter variants used by the per-block parity routines
 template <typename Real>
-__device__ __forceinline__ void face_states_rt(int lim, Real beta,
-                                               const Real *q0,
+__device__ __forceinline__ void face_states_rt(int lim, Real betaL,
+                                               Real betaR, const Real *q0,
                                                const Real *q1,
                                                const Real *q2,
                                                const Real *q3, Real *L,
                                                Real *R) {
   switch (lim) {
-  case LIM_MINMOD: face_states<LIM_MINMOD>(beta, q0, q1, q2, q3, L, R); break;
-  case LIM_MC: face_states<LIM_MC>(beta, q0, q1, q2, q3, L, R); break;
-  case LIM_MC3: face_states<LIM_MC3>(beta, q0, q1, q2, q3, L, R); break;
-  case LIM_BETA: face_states<LIM_BETA>(beta, q0, q1, q2, q3, L, R); break;
-  case LIM_NO: face_states<LIM_NO>(beta, q0, q1, q2, q3, L, R); break;
-  default: face_states<LIM_FIRST>(beta, q0, q1, q2, q3, L, R); break;
+  case LIM_MINMOD: face_states<LIM_MINMOD>(betaL, betaR, q0, q1, q2, q3, L, R); break;
+  case LIM_MC: face_states<LIM_MC>(betaL, betaR, q0, q1, q2, q3, L, R); break;
+  case LIM_MC3: face_states<LIM_MC3>(betaL, betaR, q0, q1, q2, q3, L, R); break;
+  case LIM_BETA: face_states<LIM_BETA>(betaL, betaR, q0, q1, q2, q3, L, R); break;
+  case LIM_NO: face_states<LIM_NO>(betaL, betaR, q0, q1, q2, q3, L, R); break;
+  default: face_states<LIM_FIRST>(betaL, betaR, q0, q1, q2, q3, L, R); break;
   }
 }
 template <typename Real>
-__device__ __forceinline__ void face_states_w_rt(int lim, Real beta, Real w,
+__device__ __forceinline__ void face_states_w_rt(int lim, Real betaL,
+                                                 Real betaR, Real w,
                                                  const Real *q0,
                                                  const Real *q1,
                                                  const Real *q2,
                                                  const Real *q3, Real *L,
                                                  Real *R) {
   switch (lim) {
-  case LIM_MINMOD: face_states_w<LIM_MINMOD>(beta, w, q0, q1, q2, q3, L, R); break;
-  case LIM_MC: face_states_w<LIM_MC>(beta, w, q0, q1, q2, q3, L, R); break;
-  case LIM_MC3: face_states_w<LIM_MC3>(beta, w, q0, q1, q2, q3, L, R); break;
-  case LIM_BETA: face_states_w<LIM_BETA>(beta, w, q0, q1, q2, q3, L, R); break;
-  case LIM_NO: face_states_w<LIM_NO>(beta, w, q0, q1, q2, q3, L, R); break;
-  default: face_states_w<LIM_FIRST>(beta, w, q0, q1, q2, q3, L, R); break;
+  case LIM_MINMOD: face_states_w<LIM_MINMOD>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
+  case LIM_MC: face_states_w<LIM_MC>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
+  case LIM_MC3: face_states_w<LIM_MC3>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
+  case LIM_BETA: face_states_w<LIM_BETA>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
+  case LIM_NO: face_states_w<LIM_NO>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
+  default: face_states_w<LIM_FIRST>(betaL, betaR, w, q0, q1, q2, q3, L, R); break;
   }
+}
+
+// beta of the slopes of cell c (0-based, -1..NB along one direction) of a
+// block whose low / high side in that direction has DiLevel_EB /= 0: the
+// cells within nFace of such a side take betaRC
+// (src/ModFaceValue.f90:1982-2015 with iMin = 1, iMax = nI+1)
+template <typename Real>
+__device__ __forceinline__ Real cell_beta(Real beta, Real betaRC, int nFace,
+                                          bool lo, bool hi, int c, int NB) {
+  return ((lo && c <= nFace - 1) || (hi && c >= NB - nFace)) ? betaRC : beta;
 }
 
 // conserved (p stored) -> primitive, u = (1/rho)*rho*u

@@ -66,6 +66,8 @@ struct StageArgs {
   int nWorkgroup;      // launched workgroups (< nGroup: persistent grid)
   const void *lowCrit; // low order weight of every face (type T, see
                        // crit_index); read only by the low order variant
+  double betaRC;       // BetaLimiterResChange, nFaceLimiterResChange: read
+  int nFaceRC;         // only by the OPT_RESBETA variant
 };
 
 // the cyclic frame rotation (x, y, z) -> (y, z, x) that turns a y face into
@@ -86,7 +88,9 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 
 // Stage kernel variants: the 8-wave div B source and the adaptive low order
 // face values are compile-time options so the plain kernel carries neither.
-enum { OPT_POWELL = 1, OPT_LOWORDER = 2 };
+// OPT_RESBETA: the sharper limiter next to sides with DiLevel_EB /= 0
+// (outer boundaries and resolution changes).
+enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4 };
 
 // index of the low order weight of a face of block b: the face normal to
 // dir at face coordinates (i, j, k), whose dir component is the face index
@@ -166,22 +170,23 @@ __device__ __forceinline__ double wave_min(double x) {
 }
 
 template <int DIR, int FLUX, int LIM, typename T>
-__device__ __forceinline__ int face_flux_w(const PhysT<T> &ph, T area, T beta,
-                                           T w, const T *q0, const T *q1,
+__device__ __forceinline__ int face_flux_w(const PhysT<T> &ph, T area,
+                                           T betaL, T betaR, T w,
+                                           const T *q0, const T *q1,
                                            const T *q2, const T *q3, T *F,
                                            T &vdt, T &unA) {
   T L[NVAR], R[NVAR];
-  face_states_w<LIM>(beta, w, q0, q1, q2, q3, L, R);
+  face_states_w<LIM>(betaL, betaR, w, q0, q1, q2, q3, L, R);
   return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
 }
 
 template <int DIR, int FLUX, int LIM, typename T>
-__device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T beta,
-                                         const T *q0, const T *q1,
+__device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T betaL,
+                                         T betaR, const T *q0, const T *q1,
                                          const T *q2, const T *q3, T *F,
                                          T &vdt, T &unA) {
   T L[NVAR], R[NVAR];
-  face_states<LIM>(beta, q0, q1, q2, q3, L, R);
+  face_states<LIM>(betaL, betaR, q0, q1, q2, q3, L, R);
   return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
 }
 
@@ -196,6 +201,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   constexpr bool POWELL = (OPT & OPT_POWELL) != 0;
   // adaptive low order faces: slopes scaled by 1 - weight of the face
   constexpr bool LOWORD = (OPT & OPT_LOWORDER) != 0 && LIM != LIM_FIRST;
+  // per-cell limiter beta near sides with DiLevel_EB /= 0
+  constexpr bool RESB = (OPT & OPT_RESBETA) != 0 && LIM != LIM_FIRST;
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -241,6 +248,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   const long vs = a.vs;
   const PhysT<T> ph = phys_as<T>(a.ph);
   const T beta = T(a.beta);
+  const T betaRC = T(a.betaRC);
+  const int nFaceRC = a.nFaceRC;
   const T *crit = static_cast<const T *>(a.lowCrit);
   int err = 0;
 
@@ -275,6 +284,12 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     const double dxb = a.cellsize[3 * bb], dyb = a.cellsize[3 * bb + 1],
                  dzb = a.cellsize[3 * bb + 2];
     const T AreaZ = T(dxb * dyb);
+    // z sides of this block with DiLevel_EB /= 0 (wave-uniform)
+    const bool zLo = RESB && a.nei[6 * bb + 4] < 0;
+    const bool zHi = RESB && a.nei[6 * bb + 5] < 0;
+    auto betaZ = [&](int c) {
+      return RESB ? cell_beta(beta, betaRC, nFaceRC, zLo, zHi, c, NB) : beta;
+    };
     const double CellVolume = dxb * dyb * dzb;
     T InvHalf[3];
     if (POWELL) {
@@ -313,10 +328,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
       T w = T(1);
       if (LOWORD) w = T(1) - crit[crit_index<NB>(bb, 2, i, j, fz)];
+      const T bz = betaZ(fz);
 #pragma unroll
       for (int v = 0; v < NVAR; v++) {
         T dl, dr;
-        slopes<LIM>(beta, Q1[v], Q2[v], Q3[v], dl, dr);
+        slopes<LIM>(bz, Q1[v], Q2[v], Q3[v], dl, dr);
         if (LOWORD) {
           L[v] = Q1[v] + dLk[v] * w;
           R[v] = Q2[v] - dr * w;
@@ -340,10 +356,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       load_prim(-1, Q1);
       load_prim(0, Q2);
       load_prim(1, Q3);
+      const T bz = betaZ(-1);
 #pragma unroll
       for (int v = 0; v < NVAR; v++) {
         T dl, dr;
-        slopes<LIM>(beta, Pm2[v], Q1[v], Q2[v], dl, dr);
+        slopes<LIM>(bz, Pm2[v], Q1[v], Q2[v], dl, dr);
         dLk[v] = dl;
       }
       zface(FzLo, 0);
@@ -401,14 +418,21 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           c3[v] = sP[o + 3 * st];
         }
         const T area = T(dir == 0 ? dyf * dzf : dxf * dzf);
+        T bL = beta, bR = beta;
+        if (RESB) {
+          const bool lo = a.nei[6 * bf + 2 * dir] < 0;
+          const bool hi = a.nei[6 * bf + 2 * dir + 1] < 0;
+          bL = cell_beta(beta, betaRC, nFaceRC, lo, hi, fp - 1, NB);
+          bR = cell_beta(beta, betaRC, nFaceRC, lo, hi, fp, NB);
+        }
         if (LOWORD)
           err |= face_flux_w<0, FLUX, LIM>(
-              ph, area, beta,
+              ph, area, bL, bR,
               T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
                                    : crit_index<NB>(bf, 1, row, fp, k)],
               c0, c1, c2, c3, F, vdt, unA);
         else
-          err |= face_flux<0, FLUX, LIM>(ph, area, beta, c0, c1, c2, c3, F,
+          err |= face_flux<0, FLUX, LIM>(ph, area, bL, bR, c0, c1, c2, c3, F,
                                          vdt, unA);
         if (dir != 0) unrotate_flux(F);
         F[PP] = vdt;
@@ -442,29 +466,42 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             // cell state
             load_state(U, vs, off, Uc);
             T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
+            // betas of the cells i-1, i, i+1 (x) and j-1, j, j+1 (y)
+            T bx[3] = {beta, beta, beta}, by[3] = {beta, beta, beta};
+            if (RESB) {
+              const int *n6 = a.nei + 6 * b;
+              for (int d = 0; d < 3; d++) {
+                bx[d] = cell_beta(beta, betaRC, nFaceRC, n6[0] < 0, n6[1] < 0,
+                                  i - 1 + d, NB);
+                by[d] = cell_beta(beta, betaRC, nFaceRC, n6[2] < 0, n6[3] < 0,
+                                  j - 1 + d, NB);
+              }
+            }
 #pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
             if (LOWORD) {
-              face_lr_w<LIM>(beta, T(1) - crit[crit_index<NB>(b, 0, i, j, k)],
+              face_lr_w<LIM>(bx[0], bx[1],
+                             T(1) - crit[crit_index<NB>(b, 0, i, j, k)],
                              p[0], p[1], p[2], p[3], LxLo, RxLo);
-              face_lr_w<LIM>(beta,
+              face_lr_w<LIM>(bx[1], bx[2],
                              T(1) - crit[crit_index<NB>(b, 0, i + 1, j, k)],
                              p[1], p[2], p[3], p[4], LxHi, RxHi);
             } else {
-              face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LxLo, RxLo);
-              face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LxHi, RxHi);
+              face_lr<LIM>(bx[0], bx[1], p[0], p[1], p[2], p[3], LxLo, RxLo);
+              face_lr<LIM>(bx[1], bx[2], p[1], p[2], p[3], p[4], LxHi, RxHi);
             }
 #pragma unroll
             for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
             if (LOWORD) {
-              face_lr_w<LIM>(beta, T(1) - crit[crit_index<NB>(b, 1, i, j, k)],
+              face_lr_w<LIM>(by[0], by[1],
+                             T(1) - crit[crit_index<NB>(b, 1, i, j, k)],
                              p[0], p[1], p[2], p[3], LyLo, RyLo);
-              face_lr_w<LIM>(beta,
+              face_lr_w<LIM>(by[1], by[2],
                              T(1) - crit[crit_index<NB>(b, 1, i, j + 1, k)],
                              p[1], p[2], p[3], p[4], LyHi, RyHi);
             } else {
-              face_lr<LIM>(beta, p[0], p[1], p[2], p[3], LyLo, RyLo);
-              face_lr<LIM>(beta, p[1], p[2], p[3], p[4], LyHi, RyHi);
+              face_lr<LIM>(by[0], by[1], p[0], p[1], p[2], p[3], LyLo, RyLo);
+              face_lr<LIM>(by[1], by[2], p[1], p[2], p[3], p[4], LyHi, RyHi);
             }
             T Src[NFLUX];
             powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
@@ -559,6 +596,8 @@ struct AmrArgs {
   int *err;
   const void *lowCrit;  // low order face weights (type T) or null
   const double *bcState; // CellState_VI(8, 6) of the fixed outer sides
+  double betaRC;        // sharper limiter next to sides with DiLevel_EB /= 0
+  int nFaceRC;          // (0: off)
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -671,14 +710,23 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
   const double *cz = a.cellsize + 3 * b;
   const T area = T(d == 0 ? cz[1] * cz[2] : d == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);
   T L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
+  // the face's two cells, f-1 and f along the normal, with their betas
+  // (the fine block's side here is a resolution change)
+  const int f0 = hi ? NB : 0;
+  T bL = T(a.beta), bR = T(a.beta);
+  if (a.nFaceRC > 0) {
+    const bool lo = a.nei[6 * b + 2 * d] < 0, up = a.nei[6 * b + 2 * d + 1] < 0;
+    bL = cell_beta(T(a.beta), T(a.betaRC), a.nFaceRC, lo, up, f0 - 1, NB);
+    bR = cell_beta(T(a.beta), T(a.betaRC), a.nFaceRC, lo, up, f0, NB);
+  }
   if (a.lowCrit) {
     int x, y, z;
-    side_xyz(d, hi ? NB : 0, p, q, x, y, z);
+    side_xyz(d, f0, p, q, x, y, z);
     const T w = T(1) - static_cast<const T *>(a.lowCrit)[crit_index<NB>(
                            b, d, x, y, z)];
-    face_states_w<LIM>(T(a.beta), w, c[0], c[1], c[2], c[3], L, R);
+    face_states_w<LIM>(bL, bR, w, c[0], c[1], c[2], c[3], L, R);
   } else {
-    face_states<LIM>(T(a.beta), c[0], c[1], c[2], c[3], L, R);
+    face_states<LIM>(bL, bR, c[0], c[1], c[2], c[3], L, R);
   }
   int err;
   if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
@@ -713,26 +761,38 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
                                       hipStream_t stream) {
   const dim3 grid(a.nGroup < a.nWorkgroup ? a.nGroup : a.nWorkgroup),
       block(kThreads);
-#define BATS_LAUNCH(LM)                                                     \
-  switch (opt) {                                                            \
-  case 0: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 0, T>), grid, block, 0, \
-                             stream, a); break;                             \
-  case 1: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 1, T>), grid, block, 0, \
-                             stream, a); break;                             \
-  case 2: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 2, T>), grid, block, 0, \
-                             stream, a); break;                             \
-  default: hipLaunchKernelGGL((k_stage<NB, FLUX, LM, 3, T>), grid, block,   \
-                              0, stream, a); break;                         \
+#define BATS_K(LM, O)                                                        \
+  hipLaunchKernelGGL((k_stage<NB, FLUX, LM, O, T>), grid, block, 0, stream, a)
+  // the limiters without beta never take OPT_RESBETA (no instances)
+#define BATS_LAUNCH3(LM)                                                     \
+  switch (opt & 3) {                                                         \
+  case 0: BATS_K(LM, 0); break;                                              \
+  case 1: BATS_K(LM, 1); break;                                              \
+  case 2: BATS_K(LM, 2); break;                                              \
+  default: BATS_K(LM, 3); break;                                             \
+  }
+#define BATS_LAUNCH7(LM)                                                     \
+  switch (opt) {                                                             \
+  case 0: BATS_K(LM, 0); break;                                              \
+  case 1: BATS_K(LM, 1); break;                                              \
+  case 2: BATS_K(LM, 2); break;                                              \
+  case 3: BATS_K(LM, 3); break;                                              \
+  case 4: BATS_K(LM, 4); break;                                              \
+  case 5: BATS_K(LM, 5); break;                                              \
+  case 6: BATS_K(LM, 6); break;                                              \
+  default: BATS_K(LM, 7); break;                                             \
   }
   switch (a.lim) {
-  case LIM_MINMOD: BATS_LAUNCH(LIM_MINMOD); break;
-  case LIM_MC: BATS_LAUNCH(LIM_MC); break;
-  case LIM_MC3: BATS_LAUNCH(LIM_MC3); break;
-  case LIM_BETA: BATS_LAUNCH(LIM_BETA); break;
-  case LIM_NO: BATS_LAUNCH(LIM_NO); break;
-  default: BATS_LAUNCH(LIM_FIRST); break;
+  case LIM_MINMOD: BATS_LAUNCH3(LIM_MINMOD); break;
+  case LIM_MC: BATS_LAUNCH7(LIM_MC); break;
+  case LIM_MC3: BATS_LAUNCH7(LIM_MC3); break;
+  case LIM_BETA: BATS_LAUNCH7(LIM_BETA); break;
+  case LIM_NO: BATS_LAUNCH3(LIM_NO); break;
+  default: BATS_LAUNCH3(LIM_FIRST); break;
   }
-#undef BATS_LAUNCH
+#undef BATS_K
+#undef BATS_LAUNCH3
+#undef BATS_LAUNCH7
   return hipGetLastError();
 }
 

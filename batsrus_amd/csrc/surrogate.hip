@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_harvest(HarvestArgs a) {
           to_primitive(q[s]);
         }
         double L[NVAR], R[NVAR], rhos, cs, jn;
-        face_states<LIM>(a.beta, q[0], q[1], q[2], q[3], L, R);
+        face_states<LIM>(a.beta, a.beta, q[0], q[1], q[2], q[3], L, R);
         bool ok;
         if (dir == 0) ok = face_inputs<0>(a.ph, L, R, x, rhos, cs, jn);
         else if (dir == 1) ok = face_inputs<1>(a.ph, L, R, x, rhos, cs, jn);
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(256) k_sur_faces(SurArgs a) {
         to_primitive(q[s]);
       }
       float L[NVAR], R[NVAR];
-      face_states<LIM>(beta, q[0], q[1], q[2], q[3], L, R);
+      face_states<LIM>(beta, beta, q[0], q[1], q[2], q[3], L, R);
       const double *cz = a.cellsize + 3 * b;
       area = float(dir == 0 ? cz[1] * cz[2]
                             : dir == 1 ? cz[0] * cz[2] : cz[0] * cz[1]);

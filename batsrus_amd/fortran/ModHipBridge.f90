@@ -78,6 +78,10 @@ module ModHipBridge
      integer(c_int):: StatePrecision
      integer(c_int):: UseAdaptiveLowOrder, nLowOrder
      real(c_double):: pCritLow, pCritHigh, VelCrit
+     ! #RESOLUTIONCHANGE BetaLimiterResChange, nFaceLimiterResChange
+     ! (src/ModFaceValue.f90:75-76; reference defaults 2.0 and 2)
+     real(c_double):: BetaLimiterResChange = 2.0_c_double
+     integer(c_int):: nFaceLimiterResChange = 2
   end type bats_config
 
   interface

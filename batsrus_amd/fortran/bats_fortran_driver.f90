@@ -29,10 +29,13 @@ program bats_fortran_driver
   ! outer boundaries of the sides where iNeighbor_EB = -1 (float by default)
   integer(c_int):: iTypeCellBc(6) = 5
   real(c_double):: CellState_VI(8,6) = 0.0
+  ! #RESOLUTIONCHANGE limiter (reference defaults)
+  real(c_double):: BetaLimiterResChange = 2.0
+  integer:: nFaceLimiterResChange = 2
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
        iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
        UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit, &
-       iTypeCellBc, CellState_VI
+       iTypeCellBc, CellState_VI, BetaLimiterResChange, nFaceLimiterResChange
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -74,6 +77,8 @@ program bats_fortran_driver
   Config%nLowOrder = nLowOrder
   Config%pCritLow = pCritLow; Config%pCritHigh = pCritHigh
   Config%VelCrit = VelCrit
+  Config%BetaLimiterResChange = BetaLimiterResChange
+  Config%nFaceLimiterResChange = nFaceLimiterResChange
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   if(any(iTypeCellBc /= BATS_BC_FLOAT)) &

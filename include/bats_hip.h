@@ -90,6 +90,14 @@ typedef struct bats_config {
   int UseAdaptiveLowOrder;
   int nLowOrder;        /* 1 or 2 */
   double pCritLow, pCritHigh, VelCrit;
+  /* #RESOLUTIONCHANGE (src/ModFaceValue.f90:75-76,168-177): when
+   * LimiterBeta > BetaLimiterResChange, the slopes of the cells within
+   * nFaceLimiterResChange faces of a block side whose DiLevel_EB is not 0
+   * (an outer boundary or a resolution change) use BetaLimiterResChange
+   * (:1982-2015).  The reference's defaults are 2.0 and 2;
+   * nFaceLimiterResChange = 0 turns it off. */
+  double BetaLimiterResChange;
+  int nFaceLimiterResChange;
 } bats_config;
 
 typedef struct bats_handle bats_handle;

@@ -281,12 +281,54 @@ void ob_calc_face_value(const ob_config *c, const double *StateBlk,
                         RightY, LeftZ, RightZ);
 }
 
+/* The limiter of one pencil of n cells (faces 1..n+1), with the sharper
+ * BetaLimiterResChange on the cells next to a side whose DiLevel_EB is not
+ * zero (resolution change or outer boundary; lo/hi) when LimiterBeta
+ * exceeds it: src/ModFaceValue.f90:1982-2015 (x), :2047-2080 (y),
+ * :2112-2145 (z). */
+static void limit_pencil(const ob_config *c, int n, int lo, int hi,
+                         double *Pen, double *dL, double *dR) {
+  const int iMin = 1, iMax = n + 1, nF = c->nFaceLimiterResChange;
+  int iMinSharp = iMin, iMaxSharp = iMax;
+  if (c->LimiterBeta > c->BetaLimiterResChange) {
+    if (lo) {
+      const int t = iMax + 1 < 1 + nF ? iMax + 1 : 1 + nF;
+      iMinSharp = iMin > t ? iMin : t;
+    }
+    if (hi) {
+      const int t = iMin - 1 > n + 1 - nF ? iMin - 1 : n + 1 - nF;
+      iMaxSharp = iMax < t ? iMax : t;
+    }
+  }
+  if (iMinSharp <= iMaxSharp)
+    ob_limiter(c->iLimiter, iMinSharp, iMaxSharp, c->LimiterBeta, Pen, -1,
+               dL, dR);
+  if (iMin < iMinSharp)
+    ob_limiter(c->iLimiter, iMin, iMinSharp - 1, c->BetaLimiterResChange,
+               Pen, -1, dL, dR);
+  if (iMax > iMaxSharp)
+    ob_limiter(c->iLimiter, iMaxSharp + 1, iMax, c->BetaLimiterResChange,
+               Pen, -1, dL, dR);
+}
+
 void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
                            const double *CritX, const double *CritY,
                            const double *CritZ, double *LeftX,
                            double *RightX, double *LeftY, double *RightY,
                            double *LeftZ, double *RightZ) {
+  ob_calc_face_value_rc(c, StateBlk, CritX, CritY, CritZ, NULL, LeftX,
+                        RightX, LeftY, RightY, LeftZ, RightZ);
+}
+
+void ob_calc_face_value_rc(const ob_config *c, const double *StateBlk,
+                           const double *CritX, const double *CritY,
+                           const double *CritZ, const int *IsResChange_S,
+                           double *LeftX, double *RightX, double *LeftY,
+                           double *RightY, double *LeftZ, double *RightZ) {
   ob_shape s = shape_of(c);
+  int side_rc[6] = {0, 0, 0, 0, 0, 0};
+  if (IsResChange_S)
+    for (int t = 0; t < 6; t++) side_rc[t] = IsResChange_S[t] != 0;
   /* LowOrderCrit_I = 0 unless UseLowOrder and nLowOrder == 1 (:2017-2019) */
   if (!ob_low_order_active(c)) CritX = CritY = CritZ = NULL;
   const int nI = s.nI, nJ = s.nJ, nK = s.nK;
@@ -365,7 +407,7 @@ void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
         for (l = -1; l <= nI + 2; l++)
           for (v = 0; v < NV; v++)
             Pen[(l + 1) * NV + v] = Prim[cidx(&s, l, j, k) + v];
-        ob_limiter(c->iLimiter, 1, nI + 1, c->LimiterBeta, Pen, -1, dL, dR);
+        limit_pencil(c, nI, side_rc[0], side_rc[1], Pen, dL, dR);
         for (i = 1; i <= nI + 1; i++) {
           const double W =
               1 - (CritX ? CritX[(((long)k - 1) * nJ + j - 1) * (nI + 1) +
@@ -386,7 +428,7 @@ void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
           for (l = -1; l <= nJ + 2; l++)
             for (v = 0; v < NV; v++)
               Pen[(l + 1) * NV + v] = Prim[cidx(&s, i, l, k) + v];
-          ob_limiter(c->iLimiter, 1, nJ + 1, c->LimiterBeta, Pen, -1, dL, dR);
+          limit_pencil(c, nJ, side_rc[2], side_rc[3], Pen, dL, dR);
           for (j = 1; j <= nJ + 1; j++) {
             const double W =
                 1 - (CritY ? CritY[(((long)k - 1) * (nJ + 1) + j - 1) * nI +
@@ -407,7 +449,7 @@ void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
           for (l = -1; l <= nK + 2; l++)
             for (v = 0; v < NV; v++)
               Pen[(l + 1) * NV + v] = Prim[cidx(&s, i, j, l) + v];
-          ob_limiter(c->iLimiter, 1, nK + 1, c->LimiterBeta, Pen, -1, dL, dR);
+          limit_pencil(c, nK, side_rc[4], side_rc[5], Pen, dL, dR);
           for (k = 1; k <= nK + 1; k++) {
             const double W =
                 1 - (CritZ ? CritZ[(((long)k - 1) * nJ + j - 1) * nI + i - 1]
@@ -1390,6 +1432,13 @@ static void scratch_free(ob_scratch *w) {
   free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
 }
 
+/* DiLevel_EB /= 0 on each side of block b: an outer boundary (-1) or a
+ * resolution change (a slab); same-level and periodic neighbours are 0 */
+static void block_res_change(const ob_session *s, int b, int *rc) {
+  int side;
+  for (side = 0; side < 6; side++) rc[side] = s->iNeighbor_EB[6 * b + side] < 0;
+}
+
 static int has_kind(const ob_session *s, int b, int kind) {
   int side;
   for (side = 0; side < 6; side++) {
@@ -1460,9 +1509,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
         const double *cs = s->CellSize_DB + 3L * b;
         const double *CX, *CY, *CZ;
         block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
-        ob_calc_face_value_lo(c, s->State + (long)b * len, CX, CY, CZ, w.LX,
-                              w.RX, w.LY,
-                           w.RY, w.LZ, w.RZ);
+        int rc[6];
+        block_res_change(s, b, rc);
+        ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc,
+                              w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
         e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                               w.FX, w.FY, w.FZ);
         if (e) {
@@ -1493,8 +1543,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       const double *cs = s->CellSize_DB + 3L * b;
       const double *CX, *CY, *CZ;
       block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
-      ob_calc_face_value_lo(c, Blk, CX, CY, CZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
-                            w.RZ);
+      int rc[6];
+      block_res_change(s, b, rc);
+      ob_calc_face_value_rc(c, Blk, CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY,
+                            w.LZ, w.RZ);
       e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
                             w.FY, w.FZ);
       if (s->nSlab > 0) apply_cons_flux(s, b, w.FX, w.FY, w.FZ);

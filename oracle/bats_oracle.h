@@ -83,6 +83,12 @@ typedef struct {
    * (src/ModPhysics.f90:227,679) */
   int iTypeCellBc_I[6];
   double CellState_VI[6 * OB_NVAR];
+  /* #RESOLUTIONCHANGE (src/ModFaceValue.f90:75-76,168-177): when
+   * LimiterBeta > BetaLimiterResChange, the slopes of the cells within
+   * nFaceLimiterResChange faces of a side with DiLevel_EB /= 0 use
+   * BetaLimiterResChange (:1982-2015); reference defaults 2.0 and 2 */
+  double BetaLimiterResChange;
+  int nFaceLimiterResChange;
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */
@@ -122,6 +128,14 @@ void ob_calc_face_value_lo(const ob_config *c, const double *StateBlk,
                            const double *CritZ, double *LeftX,
                            double *RightX, double *LeftY, double *RightY,
                            double *LeftZ, double *RightZ);
+/* calc_face_value with the low order weights and the sides of the block
+ * whose DiLevel_EB is not zero (IsResChange_S[6], x-,x+,y-,y+,z-,z+; NULL:
+ * none), where the sharper BetaLimiterResChange applies */
+void ob_calc_face_value_rc(const ob_config *c, const double *StateBlk,
+                           const double *CritX, const double *CritY,
+                           const double *CritZ, const int *IsResChange_S,
+                           double *LeftX, double *RightX, double *LeftY,
+                           double *RightY, double *LeftZ, double *RightZ);
 /* set_phys_based_low_order_face (src/ModFaceValue.f90:2839-2911) for one
  * ghosted block: CritX (nI+1,nJ,nK), CritY (nI,nJ+1,nK), CritZ
  * (nI,nJ,nK+1), each face from the four cells iFace-2 .. iFace+1 along its

@@ -32,7 +32,9 @@ class ObConfig(C.Structure):
                 ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
                 ("VelCrit", C.c_double),
                 ("iTypeCellBc_I", C.c_int * 6),
-                ("CellState_VI", C.c_double * 48)]
+                ("CellState_VI", C.c_double * 48),
+                ("BetaLimiterResChange", C.c_double),
+                ("nFaceLimiterResChange", C.c_int)]
 
 # iTypeCellBc values (src/ModCellBoundary.f90:23-41)
 BC_TYPES = {"periodic": 4, "float": 5, "fixed": 9, "inflow": 10}
@@ -74,6 +76,9 @@ def lib():
         L.ob_calc_face_value.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 6
         L.ob_calc_face_value_lo.argtypes = [C.POINTER(ObConfig), dp] + \
             [dp] * 9
+        L.ob_calc_face_value_rc.argtypes = [C.POINTER(ObConfig), dp, dp,
+                                            dp, dp, C.POINTER(C.c_int)] + \
+            [dp] * 6
         L.ob_set_low_order_face.argtypes = [C.POINTER(ObConfig), dp, dp, dp,
                                             dp]
         L.ob_calc_face_flux.argtypes = [C.POINTER(ObConfig), dp] + [dp] * 9
@@ -114,7 +119,8 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 UseDtFixed=False, DtFixed=0.0, cTiny=1e-6,
                 UseDivbSource=False, UseAdaptiveLowOrder=False, nLowOrder=1,
                 pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0, TypeCellBc=None,
-                CellState_VI=None):
+                CellState_VI=None, BetaLimiterResChange=2.0,
+                nFaceLimiterResChange=2):
     """TypeCellBc: six names or iTypeCellBc codes (x-, x+, y-, y+, z-, z+;
     default float); CellState_VI: (6, 8) ghost states of the fixed sides
     in conservative variables (rho, rho u, B, p)."""
@@ -129,6 +135,8 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.UseDivbSource = int(UseDivbSource)
     c.UseAdaptiveLowOrder, c.nLowOrder = int(UseAdaptiveLowOrder), nLowOrder
     c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit
+    c.BetaLimiterResChange = BetaLimiterResChange
+    c.nFaceLimiterResChange = nFaceLimiterResChange
     for side, t in enumerate(TypeCellBc or ["float"] * 6):
         c.iTypeCellBc_I[side] = BC_TYPES[t] if isinstance(t, str) else int(t)
     if CellState_VI is not None:
@@ -200,20 +208,28 @@ def face_shapes(cfg):
     return ((nK, nJ, nI + 1), (nK, nJ + 1, nI), (nK + 1, nJ, nI))
 
 
-def calc_face_value(cfg, block_state, crit=None):
+def calc_face_value(cfg, block_state, crit=None, res_change=None):
     """Face states (LX, RX, LY, RY, LZ, RZ); crit = the block's low order
-    weights (X, Y, Z) from set_low_order_face, or None."""
+    weights (X, Y, Z) from set_low_order_face, or None; res_change = six
+    flags (x-, x+, y-, y+, z-, z+) of the sides with DiLevel_EB /= 0, or
+    None."""
     blk = np.ascontiguousarray(block_state, dtype=np.float64)
     sx, sy, sz = face_shapes(cfg)
     arrs = [np.zeros(s + (8,)) for s in (sx, sx, sy, sy, sz, sz)]
-    if crit is None:
+    if crit is None and res_change is None:
         lib().ob_calc_face_value(C.byref(cfg), _dp(blk),
                                  *[_dp(a) for a in arrs])
     else:
-        cr = [np.ascontiguousarray(a, dtype=np.float64) for a in crit]
-        lib().ob_calc_face_value_lo(C.byref(cfg), _dp(blk),
-                                    *[_dp(a) for a in cr],
-                                    *[_dp(a) for a in arrs])
+        dp = C.POINTER(C.c_double)
+        cr = [None] * 3 if crit is None else \
+            [np.ascontiguousarray(a, dtype=np.float64) for a in crit]
+        rc = None if res_change is None else \
+            np.ascontiguousarray(res_change, dtype=np.int32)
+        lib().ob_calc_face_value_rc(
+            C.byref(cfg), _dp(blk),
+            *[dp() if a is None else _dp(a) for a in cr],
+            None if rc is None else rc.ctypes.data_as(C.POINTER(C.c_int)),
+            *[_dp(a) for a in arrs])
     return arrs
 
 

@@ -29,7 +29,8 @@ BC_STATES = np.array([[1.0 + 0.1 * s, 0.2, -0.1 * s, 0.05, 0.4, -0.3, 0.1,
                        0.8 + 0.05 * s] for s in range(6)])
 
 
-@pytest.mark.parametrize("option", ["none", "powell", "loworder", "fixedbc"])
+@pytest.mark.parametrize("option", ["none", "powell", "loworder", "fixedbc",
+                                    "reschange"])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
 def test_fortran_driver_matches_oracle(loop, option):
     print(f"case fortran_driver loop={loop} option={option}", flush=True)
@@ -37,6 +38,13 @@ def test_fortran_driver_matches_oracle(loop, option):
     low = LOW if option == "loworder" else {}
     bc = {}
     extra = ""
+    beta = 1.5
+    if option == "reschange":
+        # the sharper limiter at the float x sides
+        beta = 1.9
+        bc = dict(TypeCellBc=[5, 5, 4, 4, 4, 4], BetaLimiterResChange=1.0,
+                  nFaceLimiterResChange=2)
+        extra = ", BetaLimiterResChange=1.0, nFaceLimiterResChange=2"
     if option == "fixedbc":
         bc = dict(TypeCellBc=[9, 10, 4, 4, 4, 4], CellState_VI=BC_STATES)
         extra = (", iTypeCellBc=9,10,4,4,4,4, CellState_VI=" +
@@ -53,9 +61,9 @@ def test_fortran_driver_matches_oracle(loop, option):
         with open(os.path.join(d, "PARAM.nml"), "w") as f:
             f.write("&RUN nRootI=2, nRootJ=2, nRootK=2, nCell=8, nStep=%d, "
                     "nOrder=2, nStage=2, iLimiter=3, iFlux=3, "
-                    "LimiterBeta=1.5, Cfl=0.8, TypeLoop='%s', "
+                    "LimiterBeta=%g, Cfl=0.8, TypeLoop='%s', "
                     "UseDivbSource=%s%s%s /\n"
-                    % (nstep, loop, ".true." if powell else ".false.",
+                    % (nstep, beta, loop, ".true." if powell else ".false.",
                        (", UseAdaptiveLowOrder=.true., nLowOrder=1, "
                         "pCritLow=1.3, pCritHigh=1.0, VelCrit=0.0")
                        if low else "", extra))
@@ -68,7 +76,7 @@ def test_fortran_driver_matches_oracle(loop, option):
                            timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         S = np.fromfile(os.path.join(d, "state_out.bin")).reshape(S0.shape)
-    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3", beta=1.5,
+    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc3", beta=beta,
                                    cfl=0.8, UseDivbSource=powell, **low,
                                    **bc),
                               grid)
