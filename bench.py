@@ -18,9 +18,10 @@ reference's Fortran path) on a bounded sample of the same workload.
 
 The surrogate leg (on by default, --no-surrogate skips it) runs the
 two-level Kelvin-Helmholtz workload (32,256 blocks of 8^3, 200 steps) three
-ways: plain FP64 HLLD (the judge), FP32 state with the trained network on
-the smooth faces, and FP32 state with the Rusanov flux on the same faces
-(the method without the network); it reports each rate, the network's
+ways: plain FP64 HLLD (the reference for the errors), FP32 state with the
+trained network on the smooth faces, and FP32 state with the Rusanov flux
+on the same faces (the method without the network); it reports each rate,
+the network's
 relative L1/L-infinity errors against the FP64 run, and the training
 statistics.  The network is trained first, on the GPU, on faces harvested
 from FP64 runs.
@@ -231,8 +232,8 @@ def main():
                          "(1.07 GB, for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding "
-              "is designed but not built", file=sys.stderr)
+        print("bench.py: runs on one GPU only; multi-GPU sharding is "
+              "designed but not built", file=sys.stderr)
         sys.exit(2)
 
     # HBM traffic first, in child processes, before this process touches

@@ -8,7 +8,7 @@ Brio-Wu shock tube with gamma=2 on 1024 cells (16 blocks of 64) over
 boundaries, 512 steps to t = 25.6; the reference checks it with
 DiffNum.pl -r=1e-5 -a=1e-11.
 
-Run in the build container (the reference tree is not on the GPU box):
+Run where the reference tree is present (it reads the reference's output):
     python tests/golden/make_shocktube_fixture.py
 """
 import os
