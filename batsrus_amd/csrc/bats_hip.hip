@@ -486,6 +486,7 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   HIPCHK(hipMemset(h->dBcState, 0, sizeof h->bcState));
   // default grid: all sides float until bats_set_grid
   h->hNei.assign(6 * c.nBlock, -1);
+  h->anySideRC = true;
   HIPCHK(hipMemcpy(h->nei, h->hNei.data(), sizeof(int) * 6 * c.nBlock,
                    hipMemcpyHostToDevice));
   *out = h;
