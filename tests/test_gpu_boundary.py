@@ -280,3 +280,29 @@ def test_block_routines_across_a_boundary_change():
             assert o.advance_stage(1) == 0
             assert o.advance_stage(2) == 0
             assert rel_err(s.download_soa(), o.soa()) <= 1e-12
+
+
+def test_surrogate_stage_keeps_a_uniform_flow_at_fixed_sides():
+    """FP32 state, the gated surrogate stage (Rusanov and network modes) on
+    a grid whose six sides are fixed at the uniform state: the flow stays
+    uniform to FP32 rounding, so the surrogate's face pass reads the fixed
+    ghost layers."""
+    print("case boundary surrogate uniform", flush=True)
+    from batsrus_amd.core import Mlp
+    from batsrus_amd.workloads import init_weights
+    g = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8), periodic=(False,) * 3)
+    U = STATES[2]
+    U0 = np.broadcast_to(U[:, None, None, None, None],
+                         (8, g.nBlock, 8, 8, 8)).copy()
+    bc = dict(TypeCellBc=["fixed"] * 6, CellState_VI=np.tile(U, (6, 1)))
+    with Mlp(init_weights(0)) as m:
+        for mode in ("rusanov", "network"):
+            with Session(g, precision="f32", flux="hlld", limiter="mc",
+                         beta=1.5, cfl=0.8, **bc) as s:
+                s.set_surrogate(m, mode)
+                s.upload_soa(U0)
+                s.init_timestep()
+                s.advance(3)
+                got = s.download_soa()
+            err = np.abs(got - U0).max() / np.abs(U0).max()
+            assert err < 1e-6, (mode, err)
