@@ -170,7 +170,8 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
 
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
-BC_TYPES = {"periodic": 4, "float": 5, "fixed": 9, "inflow": 10}
+BC_TYPES = {"periodic": 4, "float": 5, "reflect": 7, "fixed": 9,
+            "inflow": 10}
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):

@@ -571,10 +571,13 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
   for (long b = 0; b < nB; b++)
     for (int side = 0; side < 6; side++) {
       const int n = nei[6 * b + side], dim = side / 2, hi = side % 2;
-      const bool fixedSide = n <= -2 && slabs[8 * (-2 - n) + 2] == 2;
+      const int kindS = n <= -2 ? slabs[8 * (-2 - n) + 2] : -1;
+      const bool fixedSide = kindS == 2, reflectSide = kindS == 3;
       for (int layer = 1; layer <= g; layer++) {
         const int dst = hi ? NB - 1 + layer : -layer;
-        const int src = n >= 0 ? (hi ? layer - 1 : NB - layer) : (hi ? NB - 1 : 0);
+        const int src = n >= 0 ? (hi ? layer - 1 : NB - layer)
+                        : reflectSide ? (hi ? NB - layer : layer - 1)
+                                      : (hi ? NB - 1 : 0);
         const long sb = n >= 0 ? n : b;
         for (int p = 0; p < NB; p++)
           for (int q = 0; q < NB; q++) {
@@ -584,8 +587,12 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
             else { dk = dst; sk = src; di = si = q; dj = sj = p; }
             const long off = ((sb * NB + sk) * NB + sj) * NB + si;
             double *d = at(b, di, dj, dk);
-            for (int v = 0; v < NVAR; v++)
-              d[v] = fixedSide ? bcState[side * NVAR + v] : U[v * vs + off];
+            for (int v = 0; v < NVAR; v++) {
+              const double sg =
+                  reflectSide && (v == MX + dim || v == BX + dim) ? -1.0 : 1.0;
+              d[v] = fixedSide ? bcState[side * NVAR + v]
+                               : sg * U[v * vs + off];
+            }
           }
       }
     }
@@ -658,6 +665,11 @@ static int ensure_block_buffers(bats_handle *h) {
   // it); the face arrays depend on the block size only
   if (!h->Usnap) {
     HIPCHK(hipMalloc(&h->Usnap, (size_t)h->elemSize * NVAR * h->vs));
+    // ghost slabs of the current state (reflecting sides mirror it)
+    if (h->nSlab > 0) {
+      const int st = amr_fill_ghosts(h, h->U[h->stateBuf]);
+      if (st) return st;
+    }
     HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
                           (size_t)h->elemSize * NVAR * h->vs,
                           hipMemcpyDeviceToDevice, h->stream));
@@ -772,6 +784,13 @@ extern "C" int bats_exchange_messages(bats_handle *h) {
   if (!h) return BATS_EINVAL;
   int st = ensure_block_buffers(h);
   if (st) return st;
+  // outer boundaries and resolution changes of the current state
+  // (exchange_messages applies the cell boundaries, src/ModMessagePass.f90:
+  // 120-140), then the snapshot the per-block routines read
+  if (h->nSlab > 0) {
+    st = amr_fill_ghosts(h, h->U[h->stateBuf]);
+    if (st) return st;
+  }
   HIPCHK(hipMemcpyAsync(h->Usnap, h->U[h->stateBuf],
                         (size_t)h->elemSize * NVAR * h->vs,
                         hipMemcpyDeviceToDevice, h->stream));
@@ -1426,8 +1445,9 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
 
 // Install the caller's grid (outer sides -1, resolution-change slabs) with
 // the outer boundary types: every -1 side of a fixed/inflow type becomes a
-// kind-2 slab whose two ghost layers hold the side's state.  A grid with no
-// slabs on a handle that has none keeps the state buffers.
+// kind-2 slab whose two ghost layers hold the side's state, of a reflect
+// type a kind-3 slab (the mirrored cells, refilled with every ghost fill).
+// A grid with no slabs on a handle that has none keeps the state buffers.
 static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
                       const std::vector<double> &cs,
                       const std::vector<int> &userSlabs) {
@@ -1439,9 +1459,10 @@ static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
       if (nei[6 * b + s] != -1) continue;
       const int t = h->bcType[s];
       if (t == BATS_BC_PERIODIC) return BATS_EINVAL;
-      if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW) continue;
+      if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW && t != BATS_BC_REFLECT)
+        continue;
       nei[6 * b + s] = -2 - (int)(slabs.size() / 8);
-      const int sl[8] = {b, s, 2, 0, 0, 0, 0, 0};
+      const int sl[8] = {b, s, t == BATS_BC_REFLECT ? 3 : 2, 0, 0, 0, 0, 0};
       slabs.insert(slabs.end(), sl, sl + 8);
     }
   const int nSlab = (int)(slabs.size() / 8);
@@ -1479,7 +1500,7 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
   for (int s = 0; s < 6; s++) {
     const int t = iTypeCellBc_I[s];
     if (t != BATS_BC_PERIODIC && t != BATS_BC_FLOAT && t != BATS_BC_FIXED &&
-        t != BATS_BC_INFLOW)
+        t != BATS_BC_INFLOW && t != BATS_BC_REFLECT)
       return BATS_EBC;
     fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW;
   }

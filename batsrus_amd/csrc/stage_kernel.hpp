@@ -618,8 +618,10 @@ __device__ __forceinline__ T minmod2(T a, T b) {
 // its slopes (minmod of the coarse block's one-sided differences, the one
 // available difference at the block edge) times the +-1/4 offsets of the
 // fine centre.  Kind-2 slabs (fixed outer sides) take the side's state,
-// as set_boundary_for_cell does (src/ModUpdateStateFast.f90:1262-1268).
-// One thread per ghost cell.
+// as set_boundary_for_cell does (src/ModUpdateStateFast.f90:1262-1268);
+// kind-3 slabs (reflecting sides) mirror the block's first two cells with
+// the normal momentum and field negated (set_symm_bc,
+// src/ModCellBoundary.f90:308-331,624-666).  One thread per ghost cell.
 template <typename T>
 __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -636,6 +638,14 @@ __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   };
   if (kind == 2) {
     for (int v = 0; v < NVAR; v++) val[v] = T(a.bcState[side * NVAR + v]);
+  } else if (kind == 3) {
+    int x, y, z;
+    side_xyz(d, hi ? NB - 1 - L : L, p, q, x, y, z);
+    const long o = cell(sl[0], x, y, z);
+    for (int v = 0; v < NVAR; v++) {
+      const T u = U[v * a.vs + o];
+      val[v] = (v == MX + d || v == BX + d) ? -u : u;
+    }
   } else if (kind == 1) {
     const int F = sl[3 + (p / H) + 2 * (q / H)];
     for (int v = 0; v < NVAR; v++) val[v] = T(0);

@@ -55,11 +55,12 @@ module ModHipBridge
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
        set_low_order_face, set_outer_boundary, &
        init_timestep, get_time
-  public:: BATS_BC_PERIODIC, BATS_BC_FLOAT, BATS_BC_FIXED, BATS_BC_INFLOW
+  public:: BATS_BC_PERIODIC, BATS_BC_FLOAT, BATS_BC_REFLECT, BATS_BC_FIXED, &
+       BATS_BC_INFLOW
 
   ! iTypeCellBc values (src/ModCellBoundary.f90:23-41)
   integer(c_int), parameter:: BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, &
-       BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10
+       BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10
   integer(c_int), parameter:: BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, &
        BATS_LIM_MC3 = 3, BATS_LIM_BETA = 4, BATS_LIM_NO = 5
   integer(c_int), parameter:: BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, &

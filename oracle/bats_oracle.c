@@ -47,6 +47,10 @@
  *        prolongation = coarse cell + minmod slopes x 1/4) stand in for the
  *        absent BATL message passing -- pinned by closed forms only (exact on
  *        linear data, uniform state kept): parity unpinned against BATL.
+ *   reflecting outer boundaries (mirrored cells, normal components
+ *        negated): pinned by a half domain with reflecting walls equal, bit
+ *        for bit, to the matching half of a doubled periodic domain whose
+ *        second half is the mirror image (test_boundary_oracle).
  *   fixed/inflow outer boundaries (ghost = CellState_VI of the side)
  *        pinned: ghost cells equal the side's state exactly; a uniform state
  *        equal to the boundary state stays unchanged; the boundary face flux
@@ -1185,6 +1189,28 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
               long o = dim == 0 ? cidx(&s, ig, p, q)
                      : dim == 1 ? cidx(&s, p, ig, q) : cidx(&s, p, q, ig);
               memcpy(Blk + o, St, NV * 8);
+            }
+        }
+        continue;
+      }
+      if (nb == -1 && c->iTypeCellBc_I[side] == OB_BC_REFLECT) {
+        /* set_symm_bc: ghost layer g mirrors cell g (low side) or nd+1-g
+         * (high side), the normal vector components change sign */
+        int n1 = dim == 0 ? s.nJ : s.nI, n2 = dim == 2 ? s.nJ : s.nK;
+        int nd = dim == 0 ? s.nI : dim == 1 ? s.nJ : s.nK, p, q, v;
+        for (g = 1; g <= nG; g++) {
+          int ig = hi ? nd + g : 1 - g, is = hi ? nd + 1 - g : g;
+          for (q = 1; q <= n2; q++)
+            for (p = 1; p <= n1; p++) {
+              long o = dim == 0 ? cidx(&s, ig, p, q)
+                     : dim == 1 ? cidx(&s, p, ig, q) : cidx(&s, p, q, ig);
+              long os = dim == 0 ? cidx(&s, is, p, q)
+                      : dim == 1 ? cidx(&s, p, is, q) : cidx(&s, p, q, is);
+              for (v = 0; v < NV; v++) {
+                const double sg =
+                    (v == OB_RHOUX + dim || v == OB_BX + dim) ? -1.0 : 1.0;
+                Blk[o + v] = sg * Blk[os + v];
+              }
             }
         }
         continue;

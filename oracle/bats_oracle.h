@@ -46,9 +46,11 @@ enum { OB_FLUX_RUSANOV = 1, OB_FLUX_LINDE = 2, OB_FLUX_HLLD = 3 };
 
 /* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41);
  * 0 (unset) means float.  Fixed and inflow take the side's CellState_VI
- * state in the ghost cells (src/ModUpdateStateFast.f90:1262-1268). */
-enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_FIXED = 9,
-       OB_BC_INFLOW = 10 };
+ * state in the ghost cells (src/ModUpdateStateFast.f90:1262-1268); reflect
+ * mirrors the block's cells with the normal momentum and field negated
+ * (src/ModCellBoundary.f90:308-331,624-666, Cartesian grid). */
+enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_REFLECT = 7,
+       OB_BC_FIXED = 9, OB_BC_INFLOW = 10 };
 
 /* Error codes (the reference calls stop_mpi at these points) */
 enum { OB_OK = 0, OB_ENEGFAST2 = 2, OB_EHLLD = 3, OB_EBC = 5 };

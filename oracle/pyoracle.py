@@ -37,7 +37,8 @@ class ObConfig(C.Structure):
                 ("nFaceLimiterResChange", C.c_int)]
 
 # iTypeCellBc values (src/ModCellBoundary.f90:23-41)
-BC_TYPES = {"periodic": 4, "float": 5, "fixed": 9, "inflow": 10}
+BC_TYPES = {"periodic": 4, "float": 5, "reflect": 7, "fixed": 9,
+            "inflow": 10}
 
 
 class ObSession(C.Structure):

@@ -149,3 +149,58 @@ def test_fixed_equal_to_float_copy_gives_the_float_stage():
     o.new_step()
     assert o.advance_stage(1) == 0
     assert not np.array_equal(res[0], o.soa())
+
+
+def _mirror_x(U):
+    """Mirror image of an SoA state about the high x side of the domain:
+    cells reversed along x, x momentum and B_x negated (blocks reversed
+    too, for a 1-block-high row of blocks)."""
+    M = U[:, ::-1, :, :, ::-1].copy()
+    M[1] *= -1
+    M[4] *= -1
+    return M
+
+
+def test_reflecting_walls_equal_a_mirrored_periodic_domain():
+    """[0,1] with reflecting x sides against [0,2] periodic whose second
+    half is the mirror image of the first: the halves agree bit for bit
+    after a stage (the ghosts are exact mirrors) and to rounding after
+    several steps; the ghost layers hold the mirrored cells."""
+    nb = 2
+    half = BlockGrid(nRoot=(nb, 1, 1), nCell=(NC,) * 3,
+                     xyzMax=(1.0, 1.0 / nb, 1.0 / nb),
+                     periodic=(False, True, True))
+    full = BlockGrid(nRoot=(2 * nb, 1, 1), nCell=(NC,) * 3,
+                     xyzMax=(2.0, 1.0 / nb, 1.0 / nb),
+                     periodic=(True, True, True))
+    U = random_mhd_soa(half, seed=31)
+    Uf = np.concatenate([U, _mirror_x(U)], axis=1)
+    scheme = dict(nI=NC, nJ=NC, nK=NC, flux="hlld", limiter="mc3", beta=1.5)
+    cfg_h = po.make_config(**scheme, TypeCellBc=["reflect", "reflect"] +
+                           ["periodic"] * 4)
+    cfg_f = po.make_config(**scheme)
+    oh = po.Session(half, cfg_h, half.soa_to_state_vgb(U))
+    of = po.Session(full, cfg_f, full.soa_to_state_vgb(Uf))
+    oh.exchange()
+    S = oh.State
+    for b, side, ghost, src in ((0, 0, (1, 0), (2, 3)),
+                                (nb - 1, 1, (NC + 2, NC + 3), (NC + 1, NC))):
+        for gl, sl in zip(ghost, src):
+            want = S[b, 2:2 + NC, 2:2 + NC, sl].copy()
+            want[..., [1, 4]] *= -1
+            assert np.array_equal(S[b, 2:2 + NC, 2:2 + NC, gl], want)
+    for o in (oh, of):
+        o.init_timestep()
+        o.new_step()
+    # the same time step: the mirrored half has the same speeds
+    assert oh.Dt == of.Dt
+    assert oh.advance_stage(1) == 0 and of.advance_stage(1) == 0
+    assert np.array_equal(oh.soa(), of.soa()[:, :nb])
+    assert oh.advance_stage(2) == 0 and of.advance_stage(2) == 0
+    oh.exchange()
+    of.exchange()
+    for _ in range(3):
+        assert oh.advance_step() == 0 and of.advance_step() == 0
+    a, b = oh.soa(), of.soa()[:, :nb]
+    assert (np.abs(a - b).max(axis=(1, 2, 3, 4)) /
+            np.abs(b).max(axis=(1, 2, 3, 4))).max() < 1e-13

@@ -306,3 +306,98 @@ def test_surrogate_stage_keeps_a_uniform_flow_at_fixed_sides():
                 got = s.download_soa()
             err = np.abs(got - U0).max() / np.abs(U0).max()
             assert err < 1e-6, (mode, err)
+
+
+REFLECT = dict(TypeCellBc=["reflect", "reflect", "fixed", "float",
+                           "periodic", "periodic"], CellState_VI=STATES)
+
+
+@pytest.mark.parametrize("flux,limiter,powell", [("hlld", "mc3", False),
+                                                 ("rusanov", "mc", True)])
+def test_reflecting_sides_stage_parity(flux, limiter, powell):
+    print(f"case boundary reflect flux={flux} powell={powell}", flush=True)
+    g = _grid()
+    _stages(g, random_mhd_soa(g, seed=49),
+            dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
+                 UseDivbSource=powell, **REFLECT))
+
+
+def test_reflecting_walls_equal_a_mirrored_periodic_domain():
+    """GPU: [0,1] with reflecting x sides against [0,2] periodic whose
+    second half is the mirror image: equal bit for bit after the first
+    stage, to rounding after three steps."""
+    print("case boundary reflect mirror", flush=True)
+    nb = 2
+    half = BlockGrid(nRoot=(nb, 1, 1), nCell=(8, 8, 8),
+                     xyzMax=(1.0, 1.0 / nb, 1.0 / nb),
+                     periodic=(False, True, True))
+    full = BlockGrid(nRoot=(2 * nb, 1, 1), nCell=(8, 8, 8),
+                     xyzMax=(2.0, 1.0 / nb, 1.0 / nb))
+    U = random_mhd_soa(half, seed=50)
+    M = U[:, ::-1, :, :, ::-1].copy()
+    M[1] *= -1
+    M[4] *= -1
+    Uf = np.concatenate([U, M], axis=1)
+    scheme = dict(flux="hlld", limiter="mc3", beta=1.5, cfl=0.8)
+    out = []
+    for grid, U0, kw in ((half, U, dict(TypeCellBc=["reflect"] * 2 +
+                                        ["periodic"] * 4)),
+                         (full, Uf, {})):
+        with Session(grid, **scheme, **kw) as s:
+            s.upload_soa(U0)
+            s.init_timestep()
+            s.set_global_timestep()
+            s.advance_stage(1)
+            first = s.download_soa()
+            s.advance_stage(2)
+            s.advance(3)
+            out.append((first, s.download_soa()))
+    assert np.array_equal(out[0][0], out[1][0][:, :nb])
+    a, b = out[0][1], out[1][1][:, :nb]
+    assert (np.abs(a - b).max(axis=(1, 2, 3, 4)) /
+            np.abs(b).max(axis=(1, 2, 3, 4))).max() < 1e-13
+
+
+def test_reflecting_sides_block_routines():
+    print("case boundary reflect block routines", flush=True)
+    g = _grid()
+    U0 = random_mhd_soa(g, seed=51)
+    scheme = dict(flux="linde", limiter="mc", beta=1.5, cfl=0.8, **REFLECT)
+    cfg = po.config_from_dict(scheme, g)
+    o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
+    o.init_timestep()
+    with Session(g, **scheme) as s:
+        s.upload_soa(U0)
+        s.exchange_messages()
+        s.set_stage(2)
+        for b in range(g.nBlock):
+            s.calc_face_value(b + 1)
+            s.calc_face_flux(False, b + 1)
+            s.calc_timestep(b + 1)
+        for step in range(2):
+            o.s.nStep += 1
+            dto = o.set_global_timestep()
+            dtg = s.set_global_timestep()
+            assert abs(dtg - dto) <= 1e-12 * dto
+            for stage in (1, 2):
+                assert o.advance_stage(stage) == 0
+                s.set_stage(stage)
+                for b in range(g.nBlock):
+                    s.calc_face_value(b + 1)
+                    s.calc_face_flux(False, b + 1)
+                    s.update_state(b + 1)
+                    if stage == 2:
+                        s.calc_timestep(b + 1)
+                s.exchange_messages()
+                e = rel_err(s.download_soa(), o.soa())
+                assert e <= 1e-12, (step, stage, e)
+            o.exchange()
+        # the ghosted download holds the mirrored cells
+        S = s.download_state()
+    o.exchange()
+    n, gg = g.nI, g.nG
+    mask = np.zeros(S.shape[1:4], bool)
+    mask[gg:gg + n, gg:gg + n, :] = True
+    mask[gg:gg + n, :, gg:gg + n] = True
+    mask[:, gg:gg + n, gg:gg + n] = True
+    assert rel_err(S[:, mask], o.State[:, mask], axis_var=-1) <= 1e-12
