@@ -60,7 +60,8 @@ class BatsConfig(C.Structure):
                 ("pCritLow", C.c_double), ("pCritHigh", C.c_double),
                 ("VelCrit", C.c_double),
                 ("BetaLimiterResChange", C.c_double),
-                ("nFaceLimiterResChange", C.c_int)]
+                ("nFaceLimiterResChange", C.c_int),
+                ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -166,12 +167,13 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        DtFixed=0.0, cTiny=1e-6, UseDivbSource=False,
                        precision="f64", UseAdaptiveLowOrder=False,
                        nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0,
-                       BetaLimiterResChange=2.0, nFaceLimiterResChange=2)
+                       BetaLimiterResChange=2.0, nFaceLimiterResChange=2,
+                       UseOutflowPressure=False, pOutflow=0.0)
 
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
-BC_TYPES = {"periodic": 4, "float": 5, "reflect": 7, "fixed": 9,
-            "inflow": 10}
+BC_TYPES = {"periodic": 4, "float": 5, "outflow": 6, "reflect": 7,
+            "fixed": 9, "inflow": 10}
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):
@@ -207,6 +209,8 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.VelCrit = s["VelCrit"]
     c.BetaLimiterResChange = s["BetaLimiterResChange"]
     c.nFaceLimiterResChange = int(s["nFaceLimiterResChange"])
+    c.UseOutflowPressure = int(s["UseOutflowPressure"])
+    c.pOutflow = s["pOutflow"]
     return c
 
 

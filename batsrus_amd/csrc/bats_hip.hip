@@ -415,6 +415,9 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
       ((c.nLowOrder != 1 && c.nLowOrder != 2) ||
        c.pCritLow < c.pCritHigh - 1e-15 || !(c.VelCrit >= 0.0)))
     return BATS_EINVAL;
+  if (c.UseOutflowPressure != 0 && c.UseOutflowPressure != 1)
+    return BATS_EINVAL;
+  if (c.UseOutflowPressure && !std::isfinite(c.pOutflow)) return BATS_EINVAL;
   // #RESOLUTIONCHANGE limiter parameters
   if (c.nFaceLimiterResChange < 0 ||
       (c.nFaceLimiterResChange > 0 && !std::isfinite(c.BetaLimiterResChange)))
@@ -573,6 +576,7 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
       const int n = nei[6 * b + side], dim = side / 2, hi = side % 2;
       const int kindS = n <= -2 ? slabs[8 * (-2 - n) + 2] : -1;
       const bool fixedSide = kindS == 2, reflectSide = kindS == 3;
+      const bool outflowP = kindS == 4;
       for (int layer = 1; layer <= g; layer++) {
         const int dst = hi ? NB - 1 + layer : -layer;
         const int src = n >= 0 ? (hi ? layer - 1 : NB - layer)
@@ -593,6 +597,7 @@ static void soa_to_aos(const bats_config &c, const std::vector<int> &nei,
               d[v] = fixedSide ? bcState[side * NVAR + v]
                                : sg * U[v * vs + off];
             }
+            if (outflowP) d[PP] = c.pOutflow;
           }
       }
     }
@@ -1312,6 +1317,7 @@ int amr_fill_ghosts(bats_handle *h, const void *U) {
   m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);
   m.beta = h->cfg.LimiterBeta; m.lim = limiter_code(h->cfg);
   m.corr = h->corr; m.err = h->err; m.bcState = h->dBcState;
+  m.pOutflow = h->cfg.pOutflow;
   const long n = 2L * h->NB * h->NB * h->nSlab;
   const dim3 g((unsigned)((n + 255) / 256)), t(256);
   if (h->elemSize == 8)
@@ -1459,10 +1465,13 @@ static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
       if (nei[6 * b + s] != -1) continue;
       const int t = h->bcType[s];
       if (t == BATS_BC_PERIODIC) return BATS_EINVAL;
-      if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW && t != BATS_BC_REFLECT)
+      const bool outP = t == BATS_BC_OUTFLOW && h->cfg.UseOutflowPressure;
+      if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW && t != BATS_BC_REFLECT &&
+          !outP)
         continue;
       nei[6 * b + s] = -2 - (int)(slabs.size() / 8);
-      const int sl[8] = {b, s, t == BATS_BC_REFLECT ? 3 : 2, 0, 0, 0, 0, 0};
+      const int kindS = t == BATS_BC_REFLECT ? 3 : outP ? 4 : 2;
+      const int sl[8] = {b, s, kindS, 0, 0, 0, 0, 0};
       slabs.insert(slabs.end(), sl, sl + 8);
     }
   const int nSlab = (int)(slabs.size() / 8);
@@ -1500,7 +1509,7 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
   for (int s = 0; s < 6; s++) {
     const int t = iTypeCellBc_I[s];
     if (t != BATS_BC_PERIODIC && t != BATS_BC_FLOAT && t != BATS_BC_FIXED &&
-        t != BATS_BC_INFLOW && t != BATS_BC_REFLECT)
+        t != BATS_BC_INFLOW && t != BATS_BC_REFLECT && t != BATS_BC_OUTFLOW)
       return BATS_EBC;
     fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW;
   }

@@ -598,6 +598,7 @@ struct AmrArgs {
   const double *bcState; // CellState_VI(8, 6) of the fixed outer sides
   double betaRC;        // sharper limiter next to sides with DiLevel_EB /= 0
   int nFaceRC;          // (0: off)
+  double pOutflow;      // ghost pressure of outflow sides (kind-4 slabs)
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -621,7 +622,9 @@ __device__ __forceinline__ T minmod2(T a, T b) {
 // as set_boundary_for_cell does (src/ModUpdateStateFast.f90:1262-1268);
 // kind-3 slabs (reflecting sides) mirror the block's first two cells with
 // the normal momentum and field negated (set_symm_bc,
-// src/ModCellBoundary.f90:308-331,624-666).  One thread per ghost cell.
+// src/ModCellBoundary.f90:308-331,624-666); kind-4 slabs (outflow sides
+// with a fixed pressure) copy the boundary cell and set the pressure
+// (src/ModCellBoundary.f90:273-277).  One thread per ghost cell.
 template <typename T>
 __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -638,6 +641,12 @@ __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   };
   if (kind == 2) {
     for (int v = 0; v < NVAR; v++) val[v] = T(a.bcState[side * NVAR + v]);
+  } else if (kind == 4) {
+    int x, y, z;
+    side_xyz(d, hi ? NB - 1 : 0, p, q, x, y, z);
+    const long o = cell(sl[0], x, y, z);
+    for (int v = 0; v < NVAR; v++) val[v] = U[v * a.vs + o];
+    val[PP] = T(a.pOutflow);
   } else if (kind == 3) {
     int x, y, z;
     side_xyz(d, hi ? NB - 1 - L : L, p, q, x, y, z);

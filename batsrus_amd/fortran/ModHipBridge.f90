@@ -55,12 +55,13 @@ module ModHipBridge
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
        set_low_order_face, set_outer_boundary, &
        init_timestep, get_time
-  public:: BATS_BC_PERIODIC, BATS_BC_FLOAT, BATS_BC_REFLECT, BATS_BC_FIXED, &
-       BATS_BC_INFLOW
+  public:: BATS_BC_PERIODIC, BATS_BC_FLOAT, BATS_BC_OUTFLOW, BATS_BC_REFLECT, &
+       BATS_BC_FIXED, BATS_BC_INFLOW
 
   ! iTypeCellBc values (src/ModCellBoundary.f90:23-41)
   integer(c_int), parameter:: BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, &
-       BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10
+       BATS_BC_OUTFLOW = 6, BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, &
+       BATS_BC_INFLOW = 10
   integer(c_int), parameter:: BATS_LIM_MINMOD = 1, BATS_LIM_MC = 2, &
        BATS_LIM_MC3 = 3, BATS_LIM_BETA = 4, BATS_LIM_NO = 5
   integer(c_int), parameter:: BATS_FLUX_RUSANOV = 1, BATS_FLUX_LINDE = 2, &
@@ -83,6 +84,9 @@ module ModHipBridge
      ! (src/ModFaceValue.f90:75-76; reference defaults 2.0 and 2)
      real(c_double):: BetaLimiterResChange = 2.0_c_double
      integer(c_int):: nFaceLimiterResChange = 2
+     ! #OUTFLOWPRESSURE
+     integer(c_int):: UseOutflowPressure = 0
+     real(c_double):: pOutflow = 0.0_c_double
   end type bats_config
 
   interface

@@ -98,6 +98,10 @@ typedef struct bats_config {
    * nFaceLimiterResChange = 0 turns it off. */
   double BetaLimiterResChange;
   int nFaceLimiterResChange;
+  /* #OUTFLOWPRESSURE: outflow sides (BATS_BC_OUTFLOW) float except for the
+   * ghost pressure, which is pOutflow (src/ModCellBoundary.f90:273-277) */
+  int UseOutflowPressure;
+  double pOutflow;
 } bats_config;
 
 typedef struct bats_handle bats_handle;
@@ -139,8 +143,8 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const int *iSlab_IS);
 
 /* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41) */
-enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_REFLECT = 7,
-       BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10 };
+enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
+       BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10 };
 
 /* Outer boundary conditions: iTypeCellBc_I[side] for the sides x-, x+, y-,
  * y+, z-, z+ of the domain, applied where iNeighbor_EB = -1.  Float copies
@@ -148,7 +152,9 @@ enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_REFLECT = 7,
  * CellState_VI[8*side .. 8*side+7] (rho, rho u, B, p), as
  * set_boundary_for_cell does in set_boundary_fast
  * (src/ModUpdateStateFast.f90:1145-1275, CellState_VI of
- * src/ModPhysics.f90:227,679); reflect mirrors the first two cells with the
+ * src/ModPhysics.f90:227,679); outflow floats, with the ghost pressure
+ * set to bats_config.pOutflow when UseOutflowPressure (:273-277 of
+ * src/ModCellBoundary.f90); reflect mirrors the first two cells with the
  * normal momentum and field negated (set_symm_bc on a Cartesian grid,
  * src/ModCellBoundary.f90:308-331,624-666), refilled before every stage
  * and at exchange_messages; a periodic side must have no -1 neighbour.

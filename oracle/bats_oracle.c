@@ -1215,6 +1215,8 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
         }
         continue;
       }
+      const int outP = nb == -1 && c->UseOutflowPressure &&
+                       c->iTypeCellBc_I[side] == OB_BC_OUTFLOW;
       for (g = 1; g <= nG; g++) {
         if (dim == 0) {
           int ig = hi ? s.nI + g : 1 - g;
@@ -1237,6 +1239,17 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
             for (i = 1; i <= s.nI; i++)
               memcpy(Blk + cidx(&s, i, j, kg),
                      (Nbr ? Nbr : Blk) + cidx(&s, i, j, ksrc), NV * 8);
+        }
+        if (outP) { /* set_fixed_bc(p_, p_, [pOutflow]) after the float copy */
+          int n1 = dim == 0 ? s.nJ : s.nI, n2 = dim == 2 ? s.nJ : s.nK;
+          int nd = dim == 0 ? s.nI : dim == 1 ? s.nJ : s.nK, p, q;
+          int ig = hi ? nd + g : 1 - g;
+          for (q = 1; q <= n2; q++)
+            for (p = 1; p <= n1; p++) {
+              long o = dim == 0 ? cidx(&s, ig, p, q)
+                     : dim == 1 ? cidx(&s, p, ig, q) : cidx(&s, p, q, ig);
+              Blk[o + OB_P] = c->pOutflow;
+            }
         }
       }
     }

@@ -48,9 +48,11 @@ enum { OB_FLUX_RUSANOV = 1, OB_FLUX_LINDE = 2, OB_FLUX_HLLD = 3 };
  * 0 (unset) means float.  Fixed and inflow take the side's CellState_VI
  * state in the ghost cells (src/ModUpdateStateFast.f90:1262-1268); reflect
  * mirrors the block's cells with the normal momentum and field negated
- * (src/ModCellBoundary.f90:308-331,624-666, Cartesian grid). */
-enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_REFLECT = 7,
-       OB_BC_FIXED = 9, OB_BC_INFLOW = 10 };
+ * (src/ModCellBoundary.f90:308-331,624-666, Cartesian grid); outflow is
+ * float with the ghost pressure set to pOutflow when UseOutflowPressure
+ * (:273-277). */
+enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_OUTFLOW = 6,
+       OB_BC_REFLECT = 7, OB_BC_FIXED = 9, OB_BC_INFLOW = 10 };
 
 /* Error codes (the reference calls stop_mpi at these points) */
 enum { OB_OK = 0, OB_ENEGFAST2 = 2, OB_EHLLD = 3, OB_EBC = 5 };
@@ -91,6 +93,10 @@ typedef struct {
    * BetaLimiterResChange (:1982-2015); reference defaults 2.0 and 2 */
   double BetaLimiterResChange;
   int nFaceLimiterResChange;
+  /* #OUTFLOWPRESSURE (src/ModPhysics.f90:731-738 area): outflow sides fix
+   * the ghost pressure */
+  int UseOutflowPressure;
+  double pOutflow;
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */

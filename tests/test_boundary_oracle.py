@@ -204,3 +204,31 @@ def test_reflecting_walls_equal_a_mirrored_periodic_domain():
     a, b = oh.soa(), of.soa()[:, :nb]
     assert (np.abs(a - b).max(axis=(1, 2, 3, 4)) /
             np.abs(b).max(axis=(1, 2, 3, 4))).max() < 1e-13
+
+
+def test_outflow_sides_float_with_a_fixed_pressure():
+    """OutFlowBC_ is float; with UseOutflowPressure the ghost pressure is
+    pOutflow in both layers and every other variable still floats
+    (src/ModCellBoundary.f90:273-277)."""
+    g = BlockGrid(nRoot=(2, 1, 1), nCell=(NC,) * 3,
+                  periodic=(False, True, True))
+    U0 = random_mhd_soa(g, seed=32)
+    types = ["outflow", "outflow"] + ["periodic"] * 4
+    res = {}
+    for usep in (False, True):
+        cfg = po.make_config(nI=NC, nJ=NC, nK=NC, TypeCellBc=types,
+                             UseOutflowPressure=usep, pOutflow=0.37)
+        o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
+        o.exchange()
+        res[usep] = o.State.copy()
+    cfgf = po.make_config(nI=NC, nJ=NC, nK=NC)      # all float
+    of = po.Session(g, cfgf, g.soa_to_state_vgb(U0))
+    of.exchange()
+    assert np.array_equal(res[False], of.State)
+    inner = slice(2, 2 + NC)
+    for b, layers in ((0, (0, 1)), (1, (NC + 2, NC + 3))):
+        for gl in layers:
+            got = res[True][b, inner, inner, gl]
+            want = of.State[b, inner, inner, gl].copy()
+            want[..., 7] = 0.37
+            assert np.array_equal(got, want)

@@ -401,3 +401,13 @@ def test_reflecting_sides_block_routines():
     mask[gg:gg + n, :, gg:gg + n] = True
     mask[:, gg:gg + n, gg:gg + n] = True
     assert rel_err(S[:, mask], o.State[:, mask], axis_var=-1) <= 1e-12
+
+
+def test_outflow_pressure_sides_stage_parity():
+    print("case boundary outflow pressure", flush=True)
+    g = _grid()
+    bc = dict(TypeCellBc=["outflow", "outflow", "reflect", "outflow",
+                          "periodic", "periodic"], UseOutflowPressure=True,
+              pOutflow=0.9)
+    _stages(g, random_mhd_soa(g, seed=52),
+            dict(flux="hlld", limiter="mc3", beta=1.5, cfl=0.8, **bc))
