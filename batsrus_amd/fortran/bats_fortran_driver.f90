@@ -32,10 +32,14 @@ program bats_fortran_driver
   ! #RESOLUTIONCHANGE limiter (reference defaults)
   real(c_double):: BetaLimiterResChange = 2.0
   integer:: nFaceLimiterResChange = 2
+  ! #OUTFLOWPRESSURE for outflow sides
+  logical:: UseOutflowPressure = .false.
+  real(c_double):: pOutflow = 0.0
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
        iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
        UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit, &
-       iTypeCellBc, CellState_VI, BetaLimiterResChange, nFaceLimiterResChange
+       iTypeCellBc, CellState_VI, BetaLimiterResChange, nFaceLimiterResChange, &
+       UseOutflowPressure, pOutflow
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -79,6 +83,8 @@ program bats_fortran_driver
   Config%VelCrit = VelCrit
   Config%BetaLimiterResChange = BetaLimiterResChange
   Config%nFaceLimiterResChange = nFaceLimiterResChange
+  Config%UseOutflowPressure = merge(1, 0, UseOutflowPressure)
+  Config%pOutflow = pOutflow
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   if(any(iTypeCellBc /= BATS_BC_FLOAT)) &

@@ -30,7 +30,7 @@ BC_STATES = np.array([[1.0 + 0.1 * s, 0.2, -0.1 * s, 0.05, 0.4, -0.3, 0.1,
 
 
 @pytest.mark.parametrize("option", ["none", "powell", "loworder", "fixedbc",
-                                    "reschange"])
+                                    "reschange", "outflow"])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
 def test_fortran_driver_matches_oracle(loop, option):
     print(f"case fortran_driver loop={loop} option={option}", flush=True)
@@ -45,6 +45,12 @@ def test_fortran_driver_matches_oracle(loop, option):
         bc = dict(TypeCellBc=[5, 5, 4, 4, 4, 4], BetaLimiterResChange=1.0,
                   nFaceLimiterResChange=2)
         extra = ", BetaLimiterResChange=1.0, nFaceLimiterResChange=2"
+    if option == "outflow":
+        # reflecting x- side, outflow x+ side with a fixed pressure
+        bc = dict(TypeCellBc=[7, 6, 4, 4, 4, 4], UseOutflowPressure=True,
+                  pOutflow=0.9)
+        extra = (", iTypeCellBc=7,6,4,4,4,4, UseOutflowPressure=.true., "
+                 "pOutflow=0.9")
     if option == "fixedbc":
         bc = dict(TypeCellBc=[9, 10, 4, 4, 4, 4], CellState_VI=BC_STATES)
         extra = (", iTypeCellBc=9,10,4,4,4,4, CellState_VI=" +
