@@ -61,7 +61,8 @@ class BatsConfig(C.Structure):
                 ("VelCrit", C.c_double),
                 ("BetaLimiterResChange", C.c_double),
                 ("nFaceLimiterResChange", C.c_int),
-                ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double)]
+                ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double),
+                ("IsSteadyState", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -168,7 +169,8 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        precision="f64", UseAdaptiveLowOrder=False,
                        nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0,
                        BetaLimiterResChange=2.0, nFaceLimiterResChange=2,
-                       UseOutflowPressure=False, pOutflow=0.0)
+                       UseOutflowPressure=False, pOutflow=0.0,
+                       IsSteadyState=False)
 
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
@@ -211,6 +213,7 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.nFaceLimiterResChange = int(s["nFaceLimiterResChange"])
     c.UseOutflowPressure = int(s["UseOutflowPressure"])
     c.pOutflow = s["pOutflow"]
+    c.IsSteadyState = int(s["IsSteadyState"])
     return c
 
 

@@ -101,6 +101,7 @@ struct BlockArgs {
   const double *lowCrit;                // low order weights or null
   double betaRC;                        // BetaLimiterResChange
   int nFaceRC;                          // its faces (0: not used)
+  int steady;                           // steady state: per-cell steps
   double *dtmaxCB, *dtmaxB;
   int *err;
   int nBlock, iBlock;
@@ -241,13 +242,14 @@ __global__ void k_block_update(BlockArgs a) {
   load_state(a.StateOld, a.vs, off, So);
   const double *cs = a.cellsize + 3 * a.iBlock;
   UpdateCtl u;
-  u.DtLocal = a.DtFactor * a.scal[S_DTCB];
+  // steady state: the cell's own step, set by calc_timestep at stage 1
+  u.DtLocal = a.DtFactor * (a.steady ? a.dtmaxCB[off] : a.scal[S_DTCB]);
   u.InvVolume = 1.0 / (cs[0] * cs[1] * cs[2]);
   u.halfStep = a.halfStep;
   u.iStage = a.iStage;
   u.nStage = a.nStage;
   u.dbTrick = a.dbTrick;
-  u.dtZero = (a.scal[S_DT] == 0.0);
+  u.dtZero = !a.steady && (a.scal[S_DT] == 0.0);
   u.roundTripOld = 0;  // StateOld is round-tripped in memory below
   const long fx0 = ((long)(k * NB + j) * (NB + 1) + i) * 14;
   const long fy0 = ((long)(k * (NB + 1) + j) * NB + i) * 14;
@@ -417,6 +419,7 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
     return BATS_EINVAL;
   if (c.UseOutflowPressure != 0 && c.UseOutflowPressure != 1)
     return BATS_EINVAL;
+  if (c.IsSteadyState != 0 && c.IsSteadyState != 1) return BATS_EINVAL;
   if (c.UseOutflowPressure && !std::isfinite(c.pOutflow)) return BATS_EINVAL;
   // #RESOLUTIONCHANGE limiter parameters
   if (c.nFaceLimiterResChange < 0 ||
@@ -470,6 +473,10 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   }
   h->Usnap = nullptr;
   h->dtmaxCB = nullptr;
+  if (c.IsSteadyState) {  // the per-cell steps carry over between stages
+    HIPCHK(hipMalloc(&h->dtmaxCB, sizeof(double) * h->nCell));
+    HIPCHK(hipMemset(h->dtmaxCB, 0, sizeof(double) * h->nCell));
+  }
   h->LX = h->RX = h->LY = h->RY = h->LZ = h->RZ = nullptr;
   h->FX = h->FY = h->FZ = nullptr;
   double s0[S_NSCAL] = {0};
@@ -763,6 +770,7 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
                                 : nullptr;
   a.betaRC = c.BetaLimiterResChange;
   a.nFaceRC = res_beta_used(h) ? c.nFaceLimiterResChange : 0;
+  a.steady = c.IsSteadyState;
   a.dtmaxCB = h->dtmaxCB;
   a.dtmaxB = h->dtmaxB;
   a.err = h->err;
@@ -987,11 +995,17 @@ extern "C" int bats_set_global_timestep(bats_handle *h, double TimeSimulationLim
   return BATS_OK;
 }
 
-static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
+// calcDt: DoCalcTimestep of the stage loop.  The local time steps are set
+// after the last stage's update (time accurate, :217-219), before the first
+// stage's update (steady state, :184-195), and always by the no-update pass.
+static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
                         int advanceTime, bool timed) {
   const bats_config &c = h->cfg;
   StageArgs a{};
   const int last = (iStage == c.nStage);
+  const bool steady = c.IsSteadyState != 0;
+  const int doDt = noUpdate ? 1 : steady ? (calcDt && iStage == 1)
+                                         : (calcDt && last);
   if (c.nStage == 1) {
     a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
   } else if (iStage == 1) {
@@ -1026,7 +1040,7 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   a.doDt = doDt;
   a.roundTripOld = 1;
   a.noUpdate = noUpdate;
-  a.advanceTime = advanceTime;
+  a.advanceTime = advanceTime && !steady;
   const int NB = h->NB;
   a.nGroup = (c.nBlock + kThreads / (NB * NB) - 1) / (kThreads / (NB * NB));
   a.nWorkgroup = h->nWorkgroupMax > 0 ? h->nWorkgroupMax : a.nGroup;
@@ -1045,12 +1059,14 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
   const bool lowUse = low_order_used(c);
   const bool resBeta = res_beta_used(h);
   // the surrogate's face pass has no low order weights and one beta
-  if ((lowUse || resBeta) && h->sur.mode != BATS_SUR_OFF) return BATS_EINVAL;
+  if ((lowUse || resBeta || steady) && h->sur.mode != BATS_SUR_OFF)
+    return BATS_EINVAL;
   a.lowCrit = lowUse ? h->lowCrit : nullptr;
   a.betaRC = c.BetaLimiterResChange;
   a.nFaceRC = resBeta ? c.nFaceLimiterResChange : 0;
   const int opt = (c.UseDivbSource ? OPT_POWELL : 0) |
-                  (lowUse ? OPT_LOWORDER : 0) | (resBeta ? OPT_RESBETA : 0);
+                  (lowUse ? OPT_LOWORDER : 0) | (resBeta ? OPT_RESBETA : 0) |
+                  (steady ? OPT_STEADY : 0);
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
   // resolution changes: ghost slabs of the stage input
@@ -1125,14 +1141,13 @@ static int launch_stage(bats_handle *h, int iStage, int doDt, int noUpdate,
 
 extern "C" int bats_advance_stage(bats_handle *h, int iStage, int DoCalcTimestep) {
   if (!h || iStage < 1 || iStage > h->cfg.nStage) return BATS_EINVAL;
-  const int doDt = DoCalcTimestep && iStage == h->cfg.nStage;
-  return launch_stage(h, iStage, doDt, 0, 1, false);
+  return launch_stage(h, iStage, DoCalcTimestep, 0, 1, false);
 }
 
 extern "C" int bats_advance_explicit(bats_handle *h, int DoCalcTimestep) {
   if (!h) return BATS_EINVAL;
   for (int s = 1; s <= h->cfg.nStage; s++) {
-    int st = launch_stage(h, s, DoCalcTimestep && s == h->cfg.nStage, 0, 1, false);
+    int st = launch_stage(h, s, DoCalcTimestep, 0, 1, false);
     if (st) return st;
   }
   return BATS_OK;
@@ -1152,7 +1167,7 @@ static int capture_step(bats_handle *h, double TimeSimulationLimit) {
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int st = bats_set_global_timestep(h, TimeSimulationLimit, nullptr);
   for (int s = 1; !st && s <= h->cfg.nStage; s++)
-    st = launch_stage(h, s, s == h->cfg.nStage, 0, 1, false);
+    st = launch_stage(h, s, 1, 0, 1, false);
   const hipError_t e = hipStreamEndCapture(h->stream, &g);
   h->cur = cur0;
   h->stateBuf = buf0;
@@ -1217,7 +1232,7 @@ extern "C" int bats_advance(bats_handle *h, int nStep, double TimeSimulationLimi
     int st = bats_set_global_timestep(h, TimeSimulationLimit, nullptr);
     if (st) return st;
     for (int s = 1; s <= h->cfg.nStage; s++) {
-      st = launch_stage(h, s, s == h->cfg.nStage, 0, 1, true);
+      st = launch_stage(h, s, 1, 0, 1, true);
       if (st) return st;
     }
   }

@@ -90,7 +90,9 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 // face values are compile-time options so the plain kernel carries neither.
 // OPT_RESBETA: the sharper limiter next to sides with DiLevel_EB /= 0
 // (outer boundaries and resolution changes).
-enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4 };
+// OPT_STEADY: steady state, every cell with its own time step (dtmaxCB,
+// from this launch's fluxes when doDt).
+enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8 };
 
 // index of the low order weight of a face of block b: the face normal to
 // dir at face coordinates (i, j, k), whose dir component is the face index
@@ -203,6 +205,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   constexpr bool LOWORD = (OPT & OPT_LOWORDER) != 0 && LIM != LIM_FIRST;
   // per-cell limiter beta near sides with DiLevel_EB /= 0
   constexpr bool RESB = (OPT & OPT_RESBETA) != 0 && LIM != LIM_FIRST;
+  constexpr bool STEADY = (OPT & OPT_STEADY) != 0;
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -302,13 +305,13 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     const double *rs =
         a.scal + (a.runOfBlock ? a.runOfBlock[bb] : 0) * S_NSCAL;
     UpdateCtl uc;
-    uc.DtLocal = a.DtFactor * rs[S_DTCB];
+    uc.DtLocal = a.DtFactor * rs[S_DTCB];   // time accurate: DtMax_CB = Dt
     uc.InvVolume = 1.0 / CellVolume;
     uc.halfStep = a.halfStep;
     uc.iStage = a.iStage;
     uc.nStage = a.nStage;
     uc.dbTrick = a.dbTrick;
-    uc.dtZero = (rs[S_DT] == 0.0);
+    uc.dtZero = !STEADY && (rs[S_DT] == 0.0);
     uc.roundTripOld = a.roundTripOld;
 
     // z window of the column: Q1 = P(k), Q2 = P(k+1), Q3 = P(k+2) and the
@@ -455,6 +458,24 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           dF[v] = double(d);
         }
         const long off = (((long)b * NB + k) * NB + j) * NB + i;
+        // local time step of the cell (calc_timestep :157-168); in steady
+        // state it is also this update's step (from this launch's fluxes
+        // when doDt, else from the launch that set dtmaxCB), so it is taken
+        // before the update there and after it otherwise
+        double dtc = 0.0;
+        auto cell_dt = [&]() {
+          const T Vdt =
+              fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
+              fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
+              fmax2(FzLo[PP], FzHi[PP]);
+          dtc = CellVolume / double(Vdt);
+          if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
+          dtmin = fmin2(dtmin, dtc);
+        };
+        if (STEADY && a.doDt) cell_dt();
+        UpdateCtl ucc = uc;
+        if (STEADY && !a.noUpdate)
+          ucc.DtLocal = a.DtFactor * (a.doDt ? dtc : a.dtmaxCB[off]);
         if (!a.noUpdate) {
           T Uc[NVAR], Uo[NVAR];
           double Ucd[NVAR], Uod[NVAR], Un[NVAR];
@@ -515,7 +536,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             }
 #pragma unroll
             for (int v = 0; v < NFLUX; v++) Srcd[v] = double(Src[v]);
-            update_cell(a.ph, uc, dF, Ucd, Uod, Un, Srcd);
+            update_cell(a.ph, ucc, dF, Ucd, Uod, Un, Srcd);
           } else {
             // the stage input at the cell is used only by the SSP-RK2
             // interpolation and for a zero time step
@@ -525,23 +546,15 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               load_state(U, vs, off, Uc);
 #pragma unroll
               for (int v = 0; v < NVAR; v++) Ucd[v] = double(Uc[v]);
-              update_cell(a.ph, uc, dF, Ucd, Uod, Un);
+              update_cell(a.ph, ucc, dF, Ucd, Uod, Un);
             } else {
-              update_cell(a.ph, uc, dF, Uod, Uod, Un);
+              update_cell(a.ph, ucc, dF, Uod, Uod, Un);
             }
           }
 #pragma unroll
           for (int v = 0; v < NVAR; v++) Uout[v * vs + off] = T(Un[v]);
         }
-        if (a.doDt) {
-          const T Vdt =
-              fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
-              fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
-              fmax2(FzLo[PP], FzHi[PP]);
-          const double dtc = CellVolume / double(Vdt);
-          if (a.dtmaxCB) a.dtmaxCB[off] = dtc;
-          dtmin = fmin2(dtmin, dtc);
-        }
+        if (!STEADY && a.doDt) cell_dt();
         // shift the z window
 #pragma unroll
         for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
@@ -784,11 +797,15 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   hipLaunchKernelGGL((k_stage<NB, FLUX, LM, O, T>), grid, block, 0, stream, a)
   // the limiters without beta never take OPT_RESBETA (no instances)
 #define BATS_LAUNCH3(LM)                                                     \
-  switch (opt & 3) {                                                         \
+  switch (opt & 11) {                                                        \
   case 0: BATS_K(LM, 0); break;                                              \
   case 1: BATS_K(LM, 1); break;                                              \
   case 2: BATS_K(LM, 2); break;                                              \
-  default: BATS_K(LM, 3); break;                                             \
+  case 3: BATS_K(LM, 3); break;                                              \
+  case 8: BATS_K(LM, 8); break;                                              \
+  case 9: BATS_K(LM, 9); break;                                              \
+  case 10: BATS_K(LM, 10); break;                                            \
+  default: BATS_K(LM, 11); break;                                            \
   }
 #define BATS_LAUNCH7(LM)                                                     \
   switch (opt) {                                                             \
@@ -799,7 +816,15 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 4: BATS_K(LM, 4); break;                                              \
   case 5: BATS_K(LM, 5); break;                                              \
   case 6: BATS_K(LM, 6); break;                                              \
-  default: BATS_K(LM, 7); break;                                             \
+  case 7: BATS_K(LM, 7); break;                                              \
+  case 8: BATS_K(LM, 8); break;                                              \
+  case 9: BATS_K(LM, 9); break;                                              \
+  case 10: BATS_K(LM, 10); break;                                            \
+  case 11: BATS_K(LM, 11); break;                                            \
+  case 12: BATS_K(LM, 12); break;                                            \
+  case 13: BATS_K(LM, 13); break;                                            \
+  case 14: BATS_K(LM, 14); break;                                            \
+  default: BATS_K(LM, 15); break;                                            \
   }
   switch (a.lim) {
   case LIM_MINMOD: BATS_LAUNCH3(LIM_MINMOD); break;

@@ -87,6 +87,8 @@ module ModHipBridge
      ! #OUTFLOWPRESSURE
      integer(c_int):: UseOutflowPressure = 0
      real(c_double):: pOutflow = 0.0_c_double
+     ! #TIMEACCURATE F: steady state with per-cell time steps
+     integer(c_int):: IsSteadyState = 0
   end type bats_config
 
   interface

@@ -35,11 +35,13 @@ program bats_fortran_driver
   ! #OUTFLOWPRESSURE for outflow sides
   logical:: UseOutflowPressure = .false.
   real(c_double):: pOutflow = 0.0
+  ! #TIMEACCURATE (false: steady state, per-cell time steps)
+  logical:: IsTimeAccurate = .true.
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
        iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
        UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit, &
        iTypeCellBc, CellState_VI, BetaLimiterResChange, nFaceLimiterResChange, &
-       UseOutflowPressure, pOutflow
+       UseOutflowPressure, pOutflow, IsTimeAccurate
 
   type(bats_config):: Config
   character(len=512):: NameDir
@@ -85,6 +87,7 @@ program bats_fortran_driver
   Config%nFaceLimiterResChange = nFaceLimiterResChange
   Config%UseOutflowPressure = merge(1, 0, UseOutflowPressure)
   Config%pOutflow = pOutflow
+  Config%IsSteadyState = merge(0, 1, IsTimeAccurate)
 
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   if(any(iTypeCellBc /= BATS_BC_FLOAT)) &
@@ -111,8 +114,12 @@ program bats_fortran_driver
               call calc_face_value(iBlock, .false., .true.)
               call calc_face_flux(.false., iBlock)
               call calc_source(iBlock)
+              ! src/ModAdvanceExplicit.f90:184-195 and :217-219
+              if(.not.IsTimeAccurate .and. iStage == 1) &
+                   call calc_timestep(iBlock)
               call update_state(iBlock)
-              if(iStage == nStage) call calc_timestep(iBlock)
+              if(IsTimeAccurate .and. iStage == nStage) &
+                   call calc_timestep(iBlock)
            end do
            call exchange_messages
         end do STAGELOOP

@@ -102,6 +102,14 @@ typedef struct bats_config {
    * ghost pressure, which is pOutflow (src/ModCellBoundary.f90:273-277) */
   int UseOutflowPressure;
   double pOutflow;
+  /* #TIMEACCURATE F (the reference's IsTimeAccurate = .false.): steady
+   * state; every cell advances with Cfl*iStage/nStage times its own
+   * DtMax_CB, set at the first stage from that stage's fluxes before the
+   * update (src/ModAdvanceExplicit.f90:184-195, src/ModUpdateState.f90:332;
+   * the fast path's :562-567); the simulation time does not advance.  The
+   * per-block routines follow the reference's order: calc_timestep before
+   * update_state at stage 1. */
+  int IsSteadyState;
 } bats_config;
 
 typedef struct bats_handle bats_handle;

@@ -47,6 +47,10 @@
  *        prolongation = coarse cell + minmod slopes x 1/4) stand in for the
  *        absent BATL message passing -- pinned by closed forms only (exact on
  *        linear data, uniform state kept): parity unpinned against BATL.
+ *   steady state (#TIMEACCURATE F: per-cell time step from the first
+ *        stage's fluxes)  pinned: a one-stage steady update equals the
+ *        time-accurate update scaled cell by cell by DtMax_CB/min(DtMax_CB);
+ *        a two-stage step by the same ratio per stage (test_steady_oracle).
  *   reflecting outer boundaries (mirrored cells, normal components
  *        negated): pinned by a half domain with reflecting walls equal, bit
  *        for bit, to the matching half of a doubled periodic domain whose
@@ -1047,7 +1051,7 @@ void ob_update_state(const ob_config *c, int iStage, double Dt,
     memcpy(StateOldBlk, StateBlk, sizeof(double) * block_len(&s));
 
   /* update_state_normal (:191-450) */
-  if (Dt == 0.0) goto limit; /* :238 (time accurate) */
+  if (!c->IsSteadyState && Dt == 0.0) goto limit; /* :241 (time accurate) */
   if (c->UseHalfStep)
     DtFactor = (c->Cfl * iStage) / c->nStage; /* :270-271 */
   else
@@ -1595,6 +1599,11 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
 #endif
         if (!err) err = e;
       }
+      /* steady state: the local time step of the first stage's fluxes,
+       * before the update (src/ModAdvanceExplicit.f90:184-195) */
+      if (DoUpdate && DoCalcTimestep && c->IsSteadyState && iStage == 1)
+        ob_calc_timestep(c, cs, w.FX, w.FY, w.FZ, s->DtMax_CB + nCell * b,
+                         s->DtMax_B + b);
       if (DoUpdate) {
         /* calc_source (src/ModAdvanceExplicit.f90:166) */
         if (c->UseDivbSource)
@@ -1604,7 +1613,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
                         w.FY, w.FZ, c->UseDivbSource ? w.Src : NULL, Blk,
                         Old);
       }
-      if (DoCalcTimestep)
+      /* time accurate: after the last stage's update (:217-219); the
+       * face-speed pass without an update always */
+      if (DoCalcTimestep && (!DoUpdate || (!c->IsSteadyState &&
+                                           iStage == c->nStage)))
         ob_calc_timestep(c, cs, w.FX, w.FY, w.FZ, s->DtMax_CB + nCell * b,
                          s->DtMax_B + b);
     }
@@ -1615,7 +1627,7 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
 
 int ob_advance_stage(ob_session *s, int iStage, int DoCalcTimestep) {
   const ob_config *c = &s->c;
-  int err = run_blocks(s, iStage, 1, DoCalcTimestep && iStage == c->nStage);
+  int err = run_blocks(s, iStage, 1, DoCalcTimestep);
   s->tSimulation = s->tSimulation + s->Dt / c->nStage;        /* :274 */
   if (iStage < c->nStage)                                      /* :309-312 */
     exchange_all(s);
@@ -1634,7 +1646,8 @@ int ob_advance_explicit(ob_session *s, int DoCalcTimestep) {
 int ob_advance_step(ob_session *s, double TimeSimulationLimit) {
   int err;
   s->nStep += 1;                                    /* ModBatsrusMethods :566 */
-  ob_set_global_timestep(s, TimeSimulationLimit);   /* :572 */
+  if (!s->c.IsSteadyState)
+    ob_set_global_timestep(s, TimeSimulationLimit); /* :569-572 */
   err = ob_advance_explicit(s, 1);                  /* :606 */
   exchange_all(s);                                  /* :653 */
   return err;

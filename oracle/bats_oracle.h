@@ -97,6 +97,11 @@ typedef struct {
    * the ghost pressure */
   int UseOutflowPressure;
   double pOutflow;
+  /* #TIMEACCURATE F (IsTimeAccurate = .false.): steady state, every cell
+   * advances with its own CFL step, calc_timestep runs at the first stage
+   * before the update (src/ModAdvanceExplicit.f90:184-195) and
+   * set_global_timestep is not called (src/ModBatsrusMethods.f90:569-572) */
+  int IsSteadyState;
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */

@@ -35,7 +35,8 @@ class ObConfig(C.Structure):
                 ("CellState_VI", C.c_double * 48),
                 ("BetaLimiterResChange", C.c_double),
                 ("nFaceLimiterResChange", C.c_int),
-                ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double)]
+                ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double),
+                ("IsSteadyState", C.c_int)]
 
 # iTypeCellBc values (src/ModCellBoundary.f90:23-41)
 BC_TYPES = {"periodic": 4, "float": 5, "outflow": 6, "reflect": 7,
@@ -123,7 +124,7 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0, TypeCellBc=None,
                 CellState_VI=None, BetaLimiterResChange=2.0,
                 nFaceLimiterResChange=2, UseOutflowPressure=False,
-                pOutflow=0.0):
+                pOutflow=0.0, IsSteadyState=False):
     """TypeCellBc: six names or iTypeCellBc codes (x-, x+, y-, y+, z-, z+;
     default float); CellState_VI: (6, 8) ghost states of the fixed sides
     in conservative variables (rho, rho u, B, p)."""
@@ -139,6 +140,7 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.UseAdaptiveLowOrder, c.nLowOrder = int(UseAdaptiveLowOrder), nLowOrder
     c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit
     c.UseOutflowPressure, c.pOutflow = int(UseOutflowPressure), pOutflow
+    c.IsSteadyState = int(IsSteadyState)
     c.BetaLimiterResChange = BetaLimiterResChange
     c.nFaceLimiterResChange = nFaceLimiterResChange
     for side, t in enumerate(TypeCellBc or ["float"] * 6):

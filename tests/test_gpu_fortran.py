@@ -30,7 +30,7 @@ BC_STATES = np.array([[1.0 + 0.1 * s, 0.2, -0.1 * s, 0.05, 0.4, -0.3, 0.1,
 
 
 @pytest.mark.parametrize("option", ["none", "powell", "loworder", "fixedbc",
-                                    "reschange", "outflow"])
+                                    "reschange", "outflow", "steady"])
 @pytest.mark.parametrize("loop", ["blocks", "fused"])
 def test_fortran_driver_matches_oracle(loop, option):
     print(f"case fortran_driver loop={loop} option={option}", flush=True)
@@ -45,6 +45,9 @@ def test_fortran_driver_matches_oracle(loop, option):
         bc = dict(TypeCellBc=[5, 5, 4, 4, 4, 4], BetaLimiterResChange=1.0,
                   nFaceLimiterResChange=2)
         extra = ", BetaLimiterResChange=1.0, nFaceLimiterResChange=2"
+    if option == "steady":
+        bc = dict(IsSteadyState=True)
+        extra = ", IsTimeAccurate=.false."
     if option == "outflow":
         # reflecting x- side, outflow x+ side with a fixed pressure
         bc = dict(TypeCellBc=[7, 6, 4, 4, 4, 4], UseOutflowPressure=True,
@@ -58,7 +61,7 @@ def test_fortran_driver_matches_oracle(loop, option):
     if not os.path.exists(EXE):
         pytest.fail("Fortran driver not built (build() compiles it)")
     grid = BlockGrid(nRoot=(2, 2, 2), nCell=(8, 8, 8),
-                     periodic=(not bc, True, True))
+                     periodic=(not bc or option == "steady", True, True))
     U0 = smooth_mhd_soa(grid, seed=21)
     S0 = grid.soa_to_state_vgb(U0)
     grid.fill_ghosts(S0)
