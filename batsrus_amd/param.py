@@ -1,0 +1,266 @@
+"""A reader for the subset of BATS-R-US PARAM.in commands that set up this
+solver, so the reference's parameter files drive it directly.
+
+File format (SWMF ModReadParam, share/Library, not in the reference tree;
+restated from the reference's PARAM files): a line that starts with '#'
+opens a command, the following lines up to a blank line are its
+parameters, one per line, the value first and a comment after a TAB or
+three spaces; other lines are text. #END ends the input, #RUN ends the
+first session (only the first session is read).
+
+Commands and the variables they read (src/ModSetParameters.f90):
+  #CHECKGRIDSIZE  nI nJ nK MinBlockAll                     :2289-2299
+  #GRID           nRootBlock1..3 xMin xMax yMin yMax zMin zMax :2270-2281
+  #GAMMA          Gamma                                    :2314-2318
+  #TIMESTEPPING, #RUNGEKUTTA, #RK  nStage CflExpl; UseHalfStep for
+                  #TIMESTEPPING with nStage <= 2           :628-633
+  #FIXEDTIMESTEP  UseDtFixed [DtFixedDim]; a fixed step sets Cfl = 1
+                                                           :648-650,4314-4319
+  #SCHEME         nOrder TypeFlux [TypeLimiter [LimiterBeta]]; nStage =
+                  nOrder, UseHalfStep                      :1745-1761
+  #DIVB           UseDivbSource UseDivbDiffusion UseProjection UseConstrainB
+                                                           :1869-1874
+  #TIMEACCURATE   IsTimeAccurate                           :579-581
+  #STOP           MaxIteration tSimulationMax              :561-564
+  #UPDATE         TypeUpdate                               :1839-1840
+  #OUTFLOWPRESSURE UseOutflowPressure [pOutflowSi]         :2432-2434
+  #OUTERBOUNDARY  TypeCellBc for 2*nDim sides (src/ModBoundaryGeometry.f90:96-98)
+  #RESOLUTIONCHANGE UseAccurateResChange UseTvdResChange BetaLimiterResChange
+                  nFaceLimiterResChange (src/ModFaceValue.f90:169-175)
+  #ADAPTIVELOWORDER UseAdaptiveLowOrder [nLowOrder pCritLow pCritHigh
+                  VelCrit] (src/ModFaceValue.f90:214-221)
+  #SHOCKTUBE      the primitive left states, then the right states
+                  (src/ModSetInitialCondition.f90:81-88)
+  #SHOCKPOSITION  ShockPosition ShockSlope (:91-93)
+Dimensional values are taken as normalized (#NORMALIZATION NONE); other
+commands are listed in Param.ignored.
+"""
+from dataclasses import dataclass, field
+import re
+
+import numpy as np
+
+from .grid import BlockGrid
+
+_SCHEME_LIMITERS = {"minmod": "minmod", "mc": "mc", "mc3": "mc3",
+                    "koren": "mc3", "beta": "beta", "no": "no"}
+
+
+def _logical(s):
+    t = s.strip().lower().lstrip(".")
+    if t[:1] in ("t", "f"):
+        return t[0] == "t"
+    raise ValueError(f"not a logical: {s!r}")
+
+
+def _value(line):
+    return re.split(r"\t| {3,}", line.strip())[0].strip()
+
+
+@dataclass
+class Param:
+    """Values read from a PARAM file."""
+    nCell: tuple = (8, 8, 8)
+    nRoot: tuple = (1, 1, 1)
+    xyzMin: tuple = (-0.5, -0.5, -0.5)
+    xyzMax: tuple = (0.5, 0.5, 0.5)
+    TypeCellBc: list = field(default_factory=list)
+    scheme: dict = field(default_factory=dict)
+    MaxIteration: int = -1
+    tSimulationMax: float = -1.0
+    ShockLeft: np.ndarray = None
+    ShockRight: np.ndarray = None
+    ShockPosition: float = 0.0
+    ShockSlope: float = 0.0
+    TypeUpdate: str = "orig"
+    ignored: list = field(default_factory=list)
+    commands: list = field(default_factory=list)
+
+    @property
+    def nDim(self):
+        return 3 - sum(1 for n in self.nCell if n == 1)
+
+
+def _commands(text):
+    block, out = None, []
+    for raw in text.splitlines():
+        line = raw.rstrip()
+        if line.startswith("#"):
+            name = line.split()[0]
+            if name in ("#END", "#RUN"):
+                break
+            block = (name, [])
+            out.append(block)
+        elif not line.strip():
+            block = None
+        elif block is not None:
+            block[1].append(_value(line))
+    return out
+
+
+def read_param(text):
+    """Parse PARAM text (not a file name) into a Param."""
+    p = Param()
+    s = p.scheme
+    for name, vals in _commands(text):
+        p.commands.append(name)
+        it = iter(vals)
+
+        def nxt():
+            v = next(it, None)
+            if v is None:
+                raise ValueError(f"{name}: too few parameters")
+            return v
+        if name == "#CHECKGRIDSIZE":
+            p.nCell = tuple(int(nxt()) for _ in range(3))
+        elif name == "#GRID":
+            p.nRoot = tuple(int(nxt()) for _ in range(3))
+            b = [float(nxt()) for _ in range(6)]
+            p.xyzMin, p.xyzMax = tuple(b[0::2]), tuple(b[1::2])
+        elif name == "#GAMMA":
+            s["gamma"] = float(nxt())
+        elif name in ("#TIMESTEPPING", "#RUNGEKUTTA", "#RK"):
+            s["nStage"] = int(nxt())
+            s["cfl"] = float(nxt())
+            s["UseHalfStep"] = name == "#TIMESTEPPING" and s["nStage"] <= 2
+        elif name == "#FIXEDTIMESTEP":
+            s["UseDtFixed"] = _logical(nxt())
+            if s["UseDtFixed"]:
+                s["DtFixed"] = float(nxt())
+        elif name == "#SCHEME":
+            s["nOrder"] = int(nxt())
+            s["nStage"] = s["nOrder"]
+            s["UseHalfStep"] = s["nStage"] <= 2
+            flux = nxt().upper()
+            s["flux"] = {"RUSANOV": "rusanov", "TVDLF": "rusanov",
+                         "LINDE": "linde", "HLLE": "linde",
+                         "HLLD": "hlld"}.get(flux)
+            if s["flux"] is None:
+                raise ValueError(f"#SCHEME TypeFlux {flux} not supported")
+            s["beta"] = 1.0
+            if s["nOrder"] > 1:
+                lim = nxt()
+                if lim not in _SCHEME_LIMITERS:
+                    raise ValueError(f"#SCHEME TypeLimiter {lim} not supported")
+                s["limiter"] = _SCHEME_LIMITERS[lim]
+                if lim not in ("minmod", "no"):
+                    s["beta"] = float(nxt())
+            else:
+                s["limiter"] = "no"
+        elif name == "#DIVB":
+            src, diff, proj, cons = (_logical(nxt()) for _ in range(4))
+            if diff or proj or cons:
+                raise ValueError("#DIVB: only UseDivbSource is supported")
+            s["UseDivbSource"] = src
+        elif name == "#TIMEACCURATE":
+            s["IsSteadyState"] = not _logical(nxt())
+        elif name == "#STOP":
+            p.MaxIteration = int(nxt())
+            p.tSimulationMax = float(nxt())
+        elif name == "#UPDATE":
+            p.TypeUpdate = nxt().lower()
+        elif name == "#OUTFLOWPRESSURE":
+            s["UseOutflowPressure"] = _logical(nxt())
+            if s["UseOutflowPressure"]:
+                s["pOutflow"] = float(nxt())
+        elif name == "#OUTERBOUNDARY":
+            p.TypeCellBc = [v.lower() for v in vals]
+        elif name == "#RESOLUTIONCHANGE":
+            if _logical(nxt()):
+                raise ValueError("UseAccurateResChange is not supported")
+            _logical(nxt())  # UseTvdResChange: not applied
+            s["BetaLimiterResChange"] = float(nxt())
+            s["nFaceLimiterResChange"] = int(nxt())
+        elif name == "#ADAPTIVELOWORDER":
+            s["UseAdaptiveLowOrder"] = _logical(nxt())
+            if s["UseAdaptiveLowOrder"]:
+                s["nLowOrder"] = int(nxt())
+                s["pCritLow"] = float(nxt())
+                s["pCritHigh"] = float(nxt())
+                s["VelCrit"] = float(nxt())
+        elif name == "#SHOCKTUBE":
+            v = np.array([float(x) for x in vals])
+            p.ShockLeft, p.ShockRight = v[:len(v) // 2], v[len(v) // 2:]
+        elif name == "#SHOCKPOSITION":
+            p.ShockPosition = float(nxt())
+            p.ShockSlope = float(nxt())
+        else:
+            p.ignored.append(name)
+    if s.get("UseDtFixed"):
+        s["cfl"] = 1.0
+    return p
+
+
+def read_param_file(NameFile):
+    with open(NameFile) as f:
+        return read_param(f.read())
+
+
+def grid_from_param(p, nCell=None):
+    """The BlockGrid of a Param. With nCell (this library runs cubic blocks
+    of 8 or 16 cells), the root blocks are re-cut so that every direction
+    keeps its number of cells; a direction that has one cell in the file
+    (nDim < 3) becomes one periodic block of nCell cells."""
+    nCell = tuple(nCell or p.nCell)
+    nRoot, periodic = [], []
+    bc = p.TypeCellBc
+    for d in range(3):
+        total = p.nRoot[d] * p.nCell[d]
+        if p.nCell[d] == 1:
+            nRoot.append(1)
+            periodic.append(True)
+            continue
+        if total % nCell[d]:
+            raise ValueError(f"{total} cells in direction {d} do not split "
+                             f"into blocks of {nCell[d]}")
+        nRoot.append(total // nCell[d])
+        sides = bc[2 * d:2 * d + 2] if len(bc) >= 2 * d + 2 else []
+        periodic.append(bool(sides) and all(t == "periodic" for t in sides))
+    return BlockGrid(nRoot=tuple(nRoot), nCell=nCell, xyzMin=p.xyzMin,
+                     xyzMax=p.xyzMax, periodic=tuple(periodic))
+
+
+def session_args(p, grid):
+    """Keyword arguments for batsrus_amd.core.Session(grid, **args): the
+    scheme, and the outer boundary types of the non-periodic sides."""
+    args = dict(p.scheme)
+    types = []
+    for d in range(3):
+        sides = p.TypeCellBc[2 * d:2 * d + 2]
+        if grid.periodic[d] or len(sides) < 2:
+            types += ["periodic", "periodic"] if grid.periodic[d] \
+                else ["float", "float"]
+        else:
+            types += sides
+    if any(t != "periodic" for t in types):
+        args["TypeCellBc"] = types
+    return args
+
+
+def shocktube_state(p, grid):
+    """SoA state (rho, rho*u, B, p) of #SHOCKTUBE: the left primitive state
+    where x < ShockPosition (ShockSlope 0), the right one elsewhere. A ninth
+    primitive (Hyp, the hyperbolic cleaning scalar) must be zero."""
+    if p.ShockLeft is None:
+        raise ValueError("no #SHOCKTUBE in the parameters")
+    if p.ShockSlope != 0.0:
+        raise ValueError("ShockSlope /= 0 is not supported")
+    L, R = p.ShockLeft, p.ShockRight
+    if len(L) == 9:
+        if L[7] != 0.0 or R[7] != 0.0:
+            raise ValueError("#SHOCKTUBE Hyp must be 0")
+        L, R = np.delete(L, 7), np.delete(R, 7)
+    if len(L) != 8:
+        raise ValueError("#SHOCKTUBE needs 8 primitive values per side")
+    nI, nJ, nK = grid.nI, grid.nJ, grid.nK
+    x = grid.xyzMin[0] + (grid.iRoot_DB[:, 0, None] * nI + np.arange(nI)
+                          + 0.5) * np.asarray(grid.CellSize_DB)[:, 0, None]
+    left = np.broadcast_to((x < p.ShockPosition)[:, None, None, :],
+                           (grid.nBlock, nK, nJ, nI))
+    U = np.empty((8, grid.nBlock, nK, nJ, nI))
+    for v in range(8):
+        a, b = L[v], R[v]
+        if v in (1, 2, 3):  # momentum
+            a, b = a * L[0], b * R[0]
+        U[v] = np.where(left, a, b)
+    return U

@@ -9,6 +9,8 @@ file and log file formats of batsrus_amd.plotfile:
                              (real4 binary, 3-D, no parameters)
   log_2bodyplot.log.gz       Param/2BODYPLOT/TestOutput/REFlog_n000001.log
                              (log file, StringTime 'step time')
+  PARAM.in.hyp.1d.gz         Param/SHOCKTUBE/PARAM.in.hyp.1d
+                             (the parameter file of mhdhyp_1d_ref.out)
 
 They are data files of the reference's own tests, copied unchanged.
 Run where the reference tree is present:
@@ -25,6 +27,7 @@ FILES = {
     "idl_fastwave_2d_ref.out.gz": "SHOCKTUBE/TestOutput/fastwave_2d_ref.out",
     "idl_b0local_real4.out.gz": "CORONA/B0local.out",
     "log_2bodyplot.log.gz": "2BODYPLOT/TestOutput/REFlog_n000001.log",
+    "PARAM.in.hyp.1d.gz": "SHOCKTUBE/PARAM.in.hyp.1d",
 }
 
 

@@ -77,3 +77,34 @@ def test_gpu_shocktube_idl_and_log(tmp_path):
                                                             p.min(), p.max()]
         # the log prints 6 significant digits; By and Uz sum to about 0
         assert np.allclose(rows[k, 2:], want, rtol=6e-6, atol=1e-12), k
+
+
+def test_gpu_shocktube_from_param_file(tmp_path):
+    """The same run set up by the reference's own parameter file
+    (Param/SHOCKTUBE/PARAM.in.hyp.1d) through batsrus_amd.param: grid,
+    scheme, fixed step, float ends and the #SHOCKTUBE state; 64-cell 1-D
+    blocks re-cut into 8^3 blocks (periodic in y and z)."""
+    from batsrus_amd import param
+    print("case gpu_shocktube_from_param_file", flush=True)
+    with gzip.open(os.path.join(GOLDEN, "PARAM.in.hyp.1d.gz"), "rt") as f:
+        p = param.read_param(f.read())
+    grid = param.grid_from_param(p, nCell=(8, 8, 8))
+    nStep = int(round(p.tSimulationMax / p.scheme["DtFixed"]))
+    with Session(grid, **param.session_args(p, grid)) as s:
+        s.upload_soa(param.shocktube_state(p, grid))
+        s.advance(nStep, p.tSimulationMax)
+        dt, t, n = s.time()
+        U = s.download_soa()
+    assert n == 512 and abs(t - 25.6) < 1e-12
+    col = U[:, :, 0, 0, :].reshape(8, -1)
+    ref_path = tmp_path / "ref.out"
+    with gzip.open(os.path.join(GOLDEN, "idl_mhdhyp_1d_ref.out.gz")) as f:
+        ref_path.write_bytes(f.read())
+    ref = pf.read_plot_file(ref_path)
+    names = ref.NameVar.split()[1:1 + ref.nVar]
+    mine = {"Rho": col[0], "Ux": col[1] / col[0], "By": col[5],
+            "Bx": col[4], "P": col[7]}
+    for nm, a in mine.items():
+        r = ref.Var[:, names.index(nm)]
+        err = np.abs(a - r)
+        assert np.all((err <= 1e-11) | (err <= 1e-5 * np.abs(r))), nm
