@@ -19,7 +19,10 @@ GAMMA = 5.0 / 3.0
 
 def _prim(S):
     P = S.copy()
-    P[..., 1:4] = S[..., 1:4] / S[..., :1]
+    # edge and corner ghost cells are never filled (the dimension-split
+    # stencil does not read them) and hold rho = 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        P[..., 1:4] = S[..., 1:4] / S[..., :1]
     return P
 
 
