@@ -221,7 +221,11 @@ class Session:
     """One GPU-resident solution: the reference's State_VGB plus the
     hot-path routines, with reference names and argument order."""
 
-    def __init__(self, grid: BlockGrid, device=0, **scheme):
+    def __init__(self, grid: BlockGrid, device=0, default_grid=False,
+                 **scheme):
+        """default_grid: keep bats_init's default grid (each block its own
+        domain of unit cells, six outer sides) instead of setting grid's
+        neighbours and cell sizes; grid then only gives the shapes."""
         self.lib = load_library()
         self.grid = grid
         self.cfg = make_config(grid, device, **scheme)
@@ -229,14 +233,12 @@ class Session:
         self._check(self.lib.bats_init(C.byref(self.cfg), C.byref(h)),
                     "bats_init")
         self.h = h
-        if scheme.get("TypeCellBc") is not None:
-            # stored now, applied with the grid below
-            self.set_cell_boundary(scheme["TypeCellBc"],
-                                   scheme.get("CellState_VI"))
         nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
         cs = np.ascontiguousarray(grid.CellSize_DB, dtype=np.float64)
         slabs = getattr(grid, "iSlab_IS", None)
-        if slabs is not None and len(slabs):
+        if default_grid:
+            pass
+        elif slabs is not None and len(slabs):
             sl = np.ascontiguousarray(slabs, dtype=np.int32)
             self._check(self.lib.bats_set_amr_grid(
                 self.h, _ptr(nei, C.c_int), _ptr(cs), len(sl),
@@ -244,6 +246,10 @@ class Session:
         else:
             self._check(self.lib.bats_set_grid(self.h, _ptr(nei, C.c_int),
                                                _ptr(cs)), "bats_set_grid")
+        if scheme.get("TypeCellBc") is not None:
+            # applied to the grid just set (before any state is uploaded)
+            self.set_cell_boundary(scheme["TypeCellBc"],
+                                   scheme.get("CellState_VI"))
 
     # ------------------------------------------------------------------
     def _check(self, code, where):

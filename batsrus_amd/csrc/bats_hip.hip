@@ -405,7 +405,11 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   if (c.nOrder == 2 && (c.iLimiter < BATS_LIM_MINMOD || c.iLimiter > BATS_LIM_NO))
     return BATS_EINVAL;
   if (c.iFlux < BATS_FLUX_RUSANOV || c.iFlux > BATS_FLUX_HLLD) return BATS_EINVAL;
-  if (c.nStage != 1 && c.nStage != 2) return BATS_EINVAL;
+  // one or two stages (half step or SSP-RK2), or SSP-RK3, which the
+  // reference runs without the half step scheme
+  // (src/ModSetParameters.f90:628-633)
+  if (c.nStage < 1 || c.nStage > 3) return BATS_EINVAL;
+  if (c.nStage == 3 && c.UseHalfStep) return BATS_EINVAL;
   if (!(c.Gamma > 1.0) || !(c.Cfl > 0.0)) return BATS_EINVAL;
   if (c.UseDivbSource != 0 && c.UseDivbSource != 1) return BATS_EINVAL;
   if (c.StatePrecision != 0 && c.StatePrecision != 4 && c.StatePrecision != 8)
@@ -438,6 +442,8 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   HIPCHK(hipSetDevice(c.iDevice));
   bats_handle *h = new bats_handle();
   h->cfg = c;
+  // a fixed time step runs with Cfl = 1 (src/ModSetParameters.f90:4314-4321)
+  if (c.UseDtFixed) h->cfg.Cfl = 1.0;
   h->NB = c.nI;
   h->nCellBlk = (long)c.nI * c.nJ * c.nK;
   h->nCell = h->nCellBlk * c.nBlock;
@@ -451,8 +457,9 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   hipError_t e = hipStreamCreate(&h->stream);
   h->elemSize = (c.StatePrecision == 4) ? 4 : 8;
   const size_t sz = (size_t)h->elemSize * NVAR * h->nCell;
-  if (e == hipSuccess) e = hipMalloc(&h->U[0], sz);
-  if (e == hipSuccess) e = hipMalloc(&h->U[1], sz);
+  h->nBuf = c.nStage == 3 ? 3 : 2;
+  for (int u = 0; u < h->nBuf; u++)
+    if (e == hipSuccess) e = hipMalloc(&h->U[u], sz);
   if (e == hipSuccess) e = hipMalloc(&h->nei, sizeof(int) * 6 * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->cellsize, sizeof(double) * 3 * c.nBlock);
   if (e == hipSuccess) e = hipMalloc(&h->dtmaxB, sizeof(double) * c.nBlock);
@@ -494,11 +501,18 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
   for (int side = 0; side < 6; side++) h->bcType[side] = BATS_BC_FLOAT;
   HIPCHK(hipMalloc(&h->dBcState, sizeof h->bcState));
   HIPCHK(hipMemset(h->dBcState, 0, sizeof h->bcState));
-  // default grid: all sides float until bats_set_grid
+  // default grid until bats_set_grid: every block its own domain of unit
+  // cells with six outer sides; kept as the caller's grid, so that
+  // bats_set_cell_boundary applies its types to it like to any other grid
   h->hNei.assign(6 * c.nBlock, -1);
   h->anySideRC = true;
+  h->userNei = h->hNei;
+  h->userCellSize.assign(3 * c.nBlock, 1.0);
+  h->userSlabs.clear();
   HIPCHK(hipMemcpy(h->nei, h->hNei.data(), sizeof(int) * 6 * c.nBlock,
                    hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->cellsize, h->userCellSize.data(),
+                   sizeof(double) * 3 * c.nBlock, hipMemcpyHostToDevice));
   *out = h;
   return BATS_OK;
 }
@@ -506,7 +520,7 @@ extern "C" int bats_init(const bats_config *cfg, bats_handle **out) {
 extern "C" int bats_finalize(bats_handle *h) {
   if (!h) return BATS_EINVAL;
   (void)hipStreamSynchronize(h->stream);
-  void *ptrs[] = {h->U[0], h->U[1], h->Usnap, h->nei, h->cellsize, h->dtmaxB,
+  void *ptrs[] = {h->U[0], h->U[1], h->U[2], h->Usnap, h->nei, h->cellsize, h->dtmaxB,
                   h->dtmaxCB, h->scal, h->runFirst, h->runOfBlock, h->err,
                   h->partial, h->LX, h->RX,
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
@@ -758,7 +772,7 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   const bats_config &c = h->cfg;
   a.Usnap = h->Usnap;
   a.State = static_cast<double *>(h->U[h->stateBuf]);
-  a.StateOld = static_cast<double *>(h->U[1 - h->stateBuf]);
+  a.StateOld = static_cast<double *>(h->U[h->stateBuf == 0 ? 1 : 0]);
   a.nei = h->nei;
   a.cellsize = h->cellsize;
   a.scal = h->scal + (long)h->hRunOfBlock[iBlock] * S_NSCAL;
@@ -787,7 +801,7 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.iStage = h->iStageBlock;
   a.nStage = c.nStage;
   a.halfStep = halfStep;
-  a.dbTrick = c.UseDbTrick;
+  a.dbTrick = db_trick_at(c, h->iStageBlock);
   a.useDtFixed = c.UseDtFixed;
   a.DtFixed = c.DtFixed;
   return a;
@@ -1006,13 +1020,16 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   const bool steady = c.IsSteadyState != 0;
   const int doDt = noUpdate ? 1 : steady ? (calcDt && iStage == 1)
                                          : (calcDt && last);
-  if (c.nStage == 1) {
-    a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
-  } else if (iStage == 1) {
-    a.Uin = h->U[h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[1 - h->cur];
-  } else {
-    a.Uin = h->U[1 - h->cur]; a.Uold = h->U[h->cur]; a.Uout = h->U[h->cur];
-  }
+  // stage buffers: the step's start state U[cur] stays the old state of
+  // every stage; stage s reads stage_in(s) and writes stage_in(s+1); the
+  // last stage of a multi-stage step writes U[cur] in place (each cell
+  // reads its own old value before writing it, no halo reads U[cur])
+  const int o1 = (h->cur + 1) % h->nBuf, o2 = (h->cur + 2) % h->nBuf;
+  const int inBuf = iStage == 1 ? h->cur : iStage == 2 ? o1 : o2;
+  const int outBuf = c.nStage == 1 ? o1
+                   : last ? h->cur
+                   : iStage == 1 ? o1 : o2;
+  a.Uin = h->U[inBuf]; a.Uold = h->U[h->cur]; a.Uout = h->U[outBuf];
   if (noUpdate) {  // face-speed pass on the current state only
     a.Uin = h->U[h->stateBuf]; a.Uold = a.Uin; a.Uout = nullptr;
   }
@@ -1036,7 +1053,7 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   a.iStage = iStage;
   a.nStage = c.nStage;
   a.halfStep = c.UseHalfStep || c.nStage == 1;
-  a.dbTrick = c.UseDbTrick;
+  a.dbTrick = db_trick_at(c, iStage);
   a.doDt = doDt;
   a.roundTripOld = 1;
   a.noUpdate = noUpdate;
@@ -1058,8 +1075,10 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   }
   const bool lowUse = low_order_used(c);
   const bool resBeta = res_beta_used(h);
-  // the surrogate's face pass has no low order weights and one beta
-  if ((lowUse || resBeta || steady) && h->sur.mode != BATS_SUR_OFF)
+  // the surrogate's face pass has no low order weights, one beta and no
+  // 8-wave source
+  if ((lowUse || resBeta || steady || c.UseDivbSource) &&
+      h->sur.mode != BATS_SUR_OFF)
     return BATS_EINVAL;
   a.lowCrit = lowUse ? h->lowCrit : nullptr;
   a.betaRC = c.BetaLimiterResChange;
@@ -1129,12 +1148,8 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   HIPCHK(e);
   if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));
   if (!noUpdate) {
-    if (c.nStage == 1) {
-      h->cur = 1 - h->cur;
-      h->stateBuf = h->cur;
-    } else {
-      h->stateBuf = last ? h->cur : 1 - h->cur;
-    }
+    if (c.nStage == 1) h->cur = outBuf;
+    h->stateBuf = outBuf;
   }
   return BATS_OK;
 }
@@ -1404,11 +1419,11 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
   const long vs = h->nCell + 2L * NB * NB * nSlab;
   const size_t stateBytes = (size_t)h->elemSize * NVAR * vs;
   const size_t nc = (size_t)nSlab * NB * NB * 4 * 9;
-  void *nU[2] = {nullptr, nullptr}, *nCorr = nullptr;
+  void *nU[3] = {nullptr, nullptr, nullptr}, *nCorr = nullptr;
   int *nSlabs = nullptr, *nKind = nullptr, *nFine = nullptr;
   auto fill = [&]() -> hipError_t {
     hipError_t e = hipSuccess;
-    for (int u = 0; u < 2 && e == hipSuccess; u++) {
+    for (int u = 0; u < h->nBuf && e == hipSuccess; u++) {
       e = hipMalloc(&nU[u], stateBytes);
       if (e == hipSuccess) e = hipMemset(nU[u], 0, stateBytes);
     }
@@ -1433,19 +1448,18 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
   };
   const hipError_t e = fill();
   if (e != hipSuccess) {
-    void *made[] = {nU[0], nU[1], nSlabs, nKind, nCorr, nFine};
+    void *made[] = {nU[0], nU[1], nU[2], nSlabs, nKind, nCorr, nFine};
     for (void *p : made)
       if (p) (void)hipFree(p);
     fprintf(stderr, "bats_hip: grid allocation failed: %s\n",
             hipGetErrorString(e));
     return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
   }
-  void *old[] = {h->U[0], h->U[1], h->Usnap, h->slabs, h->slabKind,
-                 h->fineSlabs, h->corr};
+  void *old[] = {h->U[0], h->U[1], h->U[2], h->Usnap, h->slabs,
+                 h->slabKind, h->fineSlabs, h->corr};
   for (void *p : old)
     if (p) HIPCHK(hipFree(p));
-  h->U[0] = nU[0];
-  h->U[1] = nU[1];
+  for (int u = 0; u < 3; u++) h->U[u] = nU[u];
   h->Usnap = nullptr;
   h->slabs = nSlabs;
   h->slabKind = nKind;
@@ -1503,7 +1517,7 @@ static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
     if (st) return st;
     // ghost layers of the fixed sides in both state buffers (the stages
     // refill them with every other slab)
-    for (int u = 0; u < 2 && nSlab > 0; u++) {
+    for (int u = 0; u < h->nBuf && nSlab > 0; u++) {
       st = amr_fill_ghosts(h, h->U[u]);
       if (st) return st;
     }
@@ -1542,7 +1556,6 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
     h->bcState[t] = CellState_VI ? CellState_VI[t] : 0.0;
   HIPCHK(hipMemcpy(h->dBcState, h->bcState, sizeof h->bcState,
                    hipMemcpyHostToDevice));
-  if (h->userNei.empty()) return BATS_OK;  // applied by the next grid call
   const int st = apply_grid(h, h->userNei, h->userCellSize, h->userSlabs);
   if (st) {
     std::memcpy(h->bcType, oldType, sizeof oldType);

@@ -33,9 +33,11 @@ struct bats_handle {
   int NB;
   long nCellBlk, nCell;
   hipStream_t stream;
-  void *U[2];    // state buffers of elemSize-byte reals, vs per variable
+  void *U[3];    // state buffers of elemSize-byte reals, vs per variable;
+                 // U[2] only for three stages (SSP-RK3)
   long vs;       // variable stride: nCell interior cells + ghost slabs
   int elemSize;  // 8: FP64 state, 4: FP32 state
+  int nBuf;      // state buffers: 2, or 3 for three stages
   int cur;       // buffer holding the state at the start of a step
   int stateBuf;  // buffer holding the current State_VGB (differs from cur
                  // between the stages of a 2-stage step)
@@ -125,6 +127,19 @@ inline Phys make_phys(const bats_config &c) {
 
 inline int limiter_code(const bats_config &c) {
   return c.nOrder == 1 ? LIM_FIRST : c.iLimiter;
+}
+
+// Whether stage iStage adds |dB|^2/2 to the energy.  UseDbTrickNow is off
+// for a one-stage time-accurate run and without the half step scheme
+// (src/ModSetParameters.f90:3903-3906); when on, the trick applies at the
+// first of two stages and at the single stage of a steady-state run
+// (src/ModUpdateState.f90:916-918).
+inline int db_trick_at(const bats_config &c, int iStage) {
+  const bool timeAccurate = c.IsSteadyState == 0;
+  const bool now = c.UseDbTrick && c.UseHalfStep &&
+                   !(c.nStage == 1 && timeAccurate);
+  return now && ((c.nStage == 2 && iStage == 1) ||
+                 (c.nStage == 1 && !timeAccurate));
 }
 
 inline int err_from_bits(int bits) {

@@ -590,9 +590,12 @@ struct UpdateCtl {
   double InvVolume;   // 1/CellVolume
   int halfStep;       // UseHalfStep or nStage==1
   int iStage, nStage;
-  int dbTrick;
-  int dtZero;         // time accurate and Dt == 0: leave the state alone
-  int roundTripOld;   // emulate StateOld's p->E->p round trip at stage 2
+  int dbTrick;        // add |dB|^2/2 to the energy at this stage (the host
+                      // decides: src/ModSetParameters.f90:3903-3906,
+                      // src/ModUpdateState.f90:916-918)
+  int dtZero;        // time accurate and Dt == 0: leave the state alone
+  int roundTripOld;   // emulate StateOld's p->E->p round trips (one per
+                      // earlier stage of the step)
 };
 
 __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
@@ -616,9 +619,10 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
 #pragma unroll
   for (int v = 0; v < NVAR; v++) Old[v] = Uold[v];
   const double InvRhoOld = frcp(Old[RHO]);
-  if (u.roundTripOld && u.iStage > 1)
-    Old[PP] = energy_to_pressure(ph, Old, pressure_to_energy(ph, Old, InvRhoOld),
-                                 InvRhoOld);
+  if (u.roundTripOld)
+    for (int r = 1; r < u.iStage; r++)
+      Old[PP] = energy_to_pressure(
+          ph, Old, pressure_to_energy(ph, Old, InvRhoOld), InvRhoOld);
   const double Eold = pressure_to_energy(ph, Old, InvRhoOld);
   double Base[NVAR];
   double Ebase;
@@ -635,14 +639,17 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
   for (int v = 0; v < NVAR; v++)
     Unew[v] = (v == PP) ? Ebase + Source[EN] : Base[v] + Source[v];
   if (!u.halfStep && u.iStage > 1) {
-    const double Coeff1 = 0.5, Coeff2 = 1 - Coeff1;
+    // Runge-Kutta interpolation weights (src/ModUpdateState.f90:748-756)
+    const double Coeff1 =
+        u.nStage == 3 ? (u.iStage == 2 ? 0.75 : 1.0 / 3.0) : 0.5;
+    const double Coeff2 = 1 - Coeff1;
 #pragma unroll
     for (int v = 0; v < NVAR; v++) {
       const double o = (v == PP) ? Eold : Old[v];
       Unew[v] = Coeff1 * o + Coeff2 * Unew[v];
     }
   }
-  if (u.dbTrick && u.nStage == 2 && u.iStage == 1)
+  if (u.dbTrick)
     Unew[PP] = Unew[PP] + 0.5 * (Source[BX] * Source[BX] +
                                  Source[BY] * Source[BY] +
                                  Source[BZ] * Source[BZ]);

@@ -693,6 +693,8 @@ extern "C" int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode,
   if (Mode == BATS_SUR_NETWORK && !m) return BATS_EINVAL;
   drop_graphs(h);
   if (Mode != BATS_SUR_OFF && h->elemSize != 4) return BATS_EINVAL;
+  // the surrogate stage has no 8-wave div B source
+  if (Mode != BATS_SUR_OFF && h->cfg.UseDivbSource) return BATS_EINVAL;
   // the gated face pass carries no adaptive low order weights
   if (Mode != BATS_SUR_OFF && h->cfg.UseAdaptiveLowOrder &&
       h->cfg.nLowOrder == 1 && h->cfg.nOrder == 2)

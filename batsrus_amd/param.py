@@ -23,6 +23,8 @@ Commands and the variables they read (src/ModSetParameters.f90):
   #TIMEACCURATE   IsTimeAccurate                           :579-581
   #STOP           MaxIteration tSimulationMax              :561-564
   #UPDATE         TypeUpdate                               :1839-1840
+  #DBTRICK        UseDbTrick (default true; off unless TypeUpdate is
+                  'orig' or unset, the CPU default)       :1908-1909,3873-3892
   #OUTFLOWPRESSURE UseOutflowPressure [pOutflowSi]         :2432-2434
   #OUTERBOUNDARY  TypeCellBc for 2*nDim sides (src/ModBoundaryGeometry.f90:96-98)
   #RESOLUTIONCHANGE UseAccurateResChange UseTvdResChange BetaLimiterResChange
@@ -159,6 +161,8 @@ def read_param(text):
             p.tSimulationMax = float(nxt())
         elif name == "#UPDATE":
             p.TypeUpdate = nxt().lower()
+        elif name == "#DBTRICK":
+            s["UseDbTrick"] = _logical(nxt())
         elif name == "#OUTFLOWPRESSURE":
             s["UseOutflowPressure"] = _logical(nxt())
             if s["UseOutflowPressure"]:
@@ -188,6 +192,14 @@ def read_param(text):
             p.ignored.append(name)
     if s.get("UseDtFixed"):
         s["cfl"] = 1.0
+    # UseDbTrick defaults to true (src/ModAdvance.f90:41); a TypeUpdate other
+    # than 'orig' (the CPU default, which any other value falls back to)
+    # switches it off (src/ModSetParameters.f90:3873-3892).  Whether a stage
+    # applies it (half step scheme, stage count, time accuracy) is decided
+    # by the solver, as UseDbTrickNow is.
+    s.setdefault("UseDbTrick", True)
+    if p.TypeUpdate in ("slow", "fast"):
+        s["UseDbTrick"] = False
     return p
 
 

@@ -62,14 +62,22 @@ typedef struct bats_config {
   int iLimiter;         /* BATS_LIM_* (ignored for nOrder=1) */
   double LimiterBeta;
   int iFlux;            /* BATS_FLUX_* */
-  int nStage;           /* 1 or 2 */
+  int nStage;           /* 1, 2 or 3 (3: SSP-RK3, needs UseHalfStep = 0,
+                           src/ModSetParameters.f90:628-633; the interpolation
+                           weights of src/ModUpdateState.f90:748-756) */
   int UseHalfStep;      /* 1: midpoint "half step" scheme (default);
                            0: SSP-RK2 interpolation (#RK) */
   double Cfl;
   double Gamma;
   double pMin;          /* pressure floor, < 0 disables (#MINIMUMPRESSURE) */
-  int UseDbTrick;       /* #DBTRICK */
-  int UseDtFixed;       /* #FIXEDTIMESTEP */
+  int UseDbTrick;       /* #DBTRICK: add |dB|^2/2 to the energy at stage 1
+                           of a two-stage half step scheme and at the one
+                           stage of a steady-state run; never for SSP-RK
+                           (UseHalfStep = 0) or a one-stage time-accurate run
+                           (UseDbTrickNow, src/ModSetParameters.f90:
+                           3903-3906; src/ModUpdateState.f90:916-918) */
+  int UseDtFixed;       /* #FIXEDTIMESTEP; forces Cfl = 1
+                           (src/ModSetParameters.f90:4314-4321) */
   double DtFixed;
   double cTiny;         /* HLLD degeneracy guard (share ModNumConst) */
   int iDevice;          /* HIP device ordinal */
@@ -168,7 +176,10 @@ enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
  * and at exchange_messages; a periodic side must have no -1 neighbour.
  * Other types return BATS_EBC (the reference stops there, :1272; time
  * varying inflow needs the solar wind input, which is out of scope).  The
- * default is float on every side.  Fixed sides keep their ghost layers as
+ * default is float on every side.  The types apply to the current grid at
+ * once and to every later one; until bats_set_grid the grid is the default
+ * one of bats_init (each block its own domain of unit cells, all six sides
+ * outer), so a periodic type needs the grid set first.  Fixed sides keep their ghost layers as
  * slabs after the interior cells of the state, so setting them (and any
  * later bats_set_grid / bats_set_amr_grid on such a handle) reallocates the
  * state: call before uploading it.  CellState_VI may be null when no side
@@ -373,7 +384,10 @@ int bats_mlp_train(bats_mlp *m, int nEpoch, int Batch, double Lr0,
  * plasma beta 2p/B^2 of both face states is at least BetaMin.
  * BATS_SUR_RUSANOV: the same gate with the Rusanov flux on the smooth faces
  * (the method without the network).  Every face still carries one flux, so
- * the update stays conservative.  The model must outlive its use. */
+ * the update stays conservative.  The model must outlive its use.  A
+ * surrogate mode returns BATS_EINVAL with the 8-wave div B source
+ * (UseDivbSource), adaptive low order faces, a sharper limiter next to
+ * resolution changes or steady state: its stage carries none of them. */
 enum { BATS_SUR_OFF = 0, BATS_SUR_NETWORK = 1, BATS_SUR_RUSANOV = 2 };
 int bats_set_surrogate(bats_handle *h, bats_mlp *m, int Mode, double BetaMin,
                        double JumpTol);

@@ -1035,6 +1035,20 @@ void ob_calc_source(const ob_config *c, const double *CellSize,
 /* ---------------------------------------------------------------------- */
 /* update_state -- src/ModUpdateState.f90:83-965                           */
 /* ---------------------------------------------------------------------- */
+/* Whether stage iStage adds |dB|^2/2 to the energy: UseDbTrickNow is off
+ * for a one-stage time-accurate run and without the half step scheme
+ * (src/ModSetParameters.f90:3903-3906); when on, it applies at stage 1 of a
+ * two-stage step and at the single stage of a steady-state run
+ * (src/ModUpdateState.f90:916-918). */
+static int db_trick_at(const ob_config *c, int iStage) {
+  const int IsTimeAccurate = !c->IsSteadyState;
+  int UseDbTrickNow = c->UseDbTrick;
+  if ((c->nStage == 1 && IsTimeAccurate) || !c->UseHalfStep)
+    UseDbTrickNow = 0;
+  return UseDbTrickNow && ((c->nStage == 2 && iStage == 1) ||
+                           (c->nStage == 1 && !IsTimeAccurate));
+}
+
 void ob_update_state(const ob_config *c, int iStage, double Dt,
                      const double *CellSize, const double *DtMax_CB,
                      const double *FluxX, const double *FluxY,
@@ -1096,13 +1110,14 @@ void ob_update_state(const ob_config *c, int iStage, double Dt,
           for (v = 0; v < NV; v++) St[v] = So[v] + Source[v]; /* :680-686 */
         } else {
           for (v = 0; v < NV; v++) St[v] = St[v] + Source[v]; /* :697-699 */
-          if (iStage > 1) { /* :744-773, nStage == 2 */
-            const double Coeff1 = 0.5, Coeff2 = 1 - Coeff1;
+          if (iStage > 1) { /* :744-773 (cThird of ModNumConst = 1/3) */
+            double Coeff1 = 0.5;
+            if (c->nStage == 3) Coeff1 = iStage == 2 ? 0.75 : 1.0 / 3.0;
+            const double Coeff2 = 1 - Coeff1;
             for (v = 0; v < NV; v++) St[v] = Coeff1 * So[v] + Coeff2 * St[v];
           }
         }
-        if (c->UseDbTrick &&
-            (c->nStage == 2 && iStage == 1)) /* :916-937 */
+        if (db_trick_at(c, iStage)) /* :916-937 */
           St[OB_P] = St[OB_P] + 0.5 * (Source[OB_BX] * Source[OB_BX] +
                                        Source[OB_BY] * Source[OB_BY] +
                                        Source[OB_BZ] * Source[OB_BZ]);

@@ -136,6 +136,8 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.nStage, c.UseHalfStep, c.Cfl, c.Gamma = nStage, int(UseHalfStep), cfl, gamma
     c.pMin, c.UseDbTrick = pMin, int(UseDbTrick)
     c.UseDtFixed, c.DtFixed, c.cTiny = int(UseDtFixed), DtFixed, cTiny
+    if UseDtFixed:  # src/ModSetParameters.f90:4314-4321
+        c.Cfl = 1.0
     c.UseDivbSource = int(UseDivbSource)
     c.UseAdaptiveLowOrder, c.nLowOrder = int(UseAdaptiveLowOrder), nLowOrder
     c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit

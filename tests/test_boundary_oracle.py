@@ -85,8 +85,8 @@ def test_first_order_boundary_update_closed_form():
                              (8, g.nBlock, NC, NC, NC)).copy()
         o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
         o.init_timestep()
-        dt = o.new_step()                  # Cfl * DtFixed
-        assert 0 < dt < 0.02
+        dt = o.new_step()                  # DtFixed (Cfl forced to 1)
+        assert dt == 0.02
         assert o.advance_stage(1) == 0
         U1 = o.soa()
         dx = g.CellSize_DB[0, 0]

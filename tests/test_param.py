@@ -25,9 +25,12 @@ def test_reads_shocktube_param():
     assert p.TypeCellBc == ["float", "float"]
     # #SCHEME after #TIMESTEPPING sets nStage = nOrder; #FIXEDTIMESTEP
     # sets Cfl = 1 (src/ModSetParameters.f90:1748-1750,4314-4319)
+    # #UPDATE fast switches the dB trick off (src/ModSetParameters.f90:
+    # 3873-3892)
     assert p.scheme == dict(gamma=2.0, nStage=1, cfl=1.0, UseHalfStep=True,
                             UseDtFixed=True, DtFixed=0.05, nOrder=1,
-                            flux="rusanov", beta=1.0, limiter="no")
+                            flux="rusanov", beta=1.0, limiter="no",
+                            UseDbTrick=False)
     assert (p.MaxIteration, p.tSimulationMax, p.TypeUpdate) == (-1, 25.6,
                                                                  "fast")
     assert np.array_equal(p.ShockLeft, [1, 0, 0, 0, 0.75, 1, 0, 0, 1])
@@ -72,3 +75,16 @@ def test_scheme_rules():
         param.read_param("#DIVB\nF\nF\nT\nF\n")
     with pytest.raises(ValueError):  # too few parameters
         param.read_param("#GRID\n2\n2\n")
+
+
+def test_dbtrick_default_and_update_type():
+    """UseDbTrick defaults to true (src/ModAdvance.f90:41), #DBTRICK sets
+    it, and a TypeUpdate of 'fast' or 'slow' switches it off."""
+    assert param.read_param("#GAMMA\n1.4\tGamma\n")\
+        .scheme["UseDbTrick"] is True
+    assert param.read_param("#DBTRICK\nF\tUseDbTrick\n")\
+        .scheme["UseDbTrick"] is False
+    for t, want in (("orig", True), ("slow", False), ("fast", False)):
+        p = param.read_param(f"#UPDATE\n{t}\tTypeUpdate\n\n"
+                             "#DBTRICK\nT\tUseDbTrick\n")
+        assert p.scheme["UseDbTrick"] is want, t
