@@ -19,19 +19,61 @@ def _params(seed):
     return w.astype(np.float32)
 
 
+def _bf16_ulp(x):
+    """spacing of the BF16 values around x (8 significant bits)"""
+    e = np.floor(np.log2(np.maximum(np.abs(x), 1e-38)))
+    return 2.0 ** (e - 7)
+
+
+def _forward_bound(w, X):
+    """Per row and output, a bound on |GPU - oracle| of the forward pass.
+    Each layer's pre-activation differs by the FP32 accumulation error
+    (at most n * 2^-24 * sum|a w| for n terms of exact BF16 products) plus
+    what earlier differences carry through |W|; the activation silu(z),
+    rounded to BF16, can then differ by one BF16 spacing, and only where
+    the oracle's value lies within that difference (plus the fast sigmoid's
+    3e-7 relative error) of a BF16 rounding midpoint."""
+    Y, As, Zs = mo.forward(w, X, rounding=True, keep=True)
+    p = mo.unflatten(np.asarray(w, np.float64))
+    u = 2.0 ** -24
+    dA = np.zeros_like(As[0])
+    nl = len(mo.DIMS) - 1
+    for l in range(1, nl + 1):
+        Wa = np.abs(mo.bf16(p["W%d" % l]))
+        n = Wa.shape[0] + 1
+        dZ = n * u * (np.abs(As[l - 1]) @ Wa + np.abs(p["b%d" % l])) + dA @ Wa
+        if l == nl:
+            return Y, dZ
+        z = Zs[l - 1]
+        s = mo.silu(z)
+        delta = 1.1 * dZ + 3e-7 * (np.abs(s) + np.abs(z))
+        ulp = _bf16_ulp(s)
+        # distance of s to the nearest midpoint between BF16 neighbours
+        k = np.floor(np.abs(s) / ulp)
+        dist = np.abs(np.abs(s) - (k + 0.5) * ulp)
+        dist = np.minimum(dist, np.abs(np.abs(s) - (k - 0.5) * ulp))
+        flip = dist <= delta
+        dA = np.where(flip, ulp, 0.0)
+
+
 @pytest.mark.parametrize("rows", [1, 16, 1003])
 def test_forward_matches_oracle(rows):
+    """Every row within the bound that FP32 accumulation and BF16 rounding
+    allow (_forward_bound); most rows carry no BF16 flip at all."""
     print(f"case mlp forward rows={rows}", flush=True)
     rng = np.random.default_rng(rows)
     w = _params(1)
     X = rng.normal(0, 1, (rows, 16)).astype(np.float32)
     with Mlp(w) as m:
         Y = m.forward(X)
-    Yo = mo.forward(w, X, rounding=True)
+    Yo, bound = _forward_bound(w, X)
     err = np.abs(Y - Yo)
     scale = np.abs(Yo).max()
+    ratio = err / (bound + 1e-7 * scale)
+    print(f"  max err {err.max():.2e}, max err/bound {ratio.max():.3f}, "
+          f"median bound {np.median(bound):.1e}")
+    assert np.all(ratio <= 1.0)
     assert np.median(err) <= 1e-4 * scale
-    assert err.max() <= 2e-2 * scale
 
 
 def test_gradient_matches_oracle():
