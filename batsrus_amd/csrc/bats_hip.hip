@@ -1366,7 +1366,6 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
     return BATS_EINVAL;
   drop_graphs(h);
   const int nB = h->cfg.nBlock, NB = h->NB, H = NB / 2;
-  if (nSlab > 0 && h->cfg.UseDivbSource) return BATS_EINVAL;
   for (int t = 0; t < 3 * nB; t++)
     if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
   for (int b = 0; b < nB; b++)
@@ -1418,7 +1417,7 @@ static int install_slab_grid(bats_handle *h, const int *iNeighbor_EB,
   // released, so a failed allocation leaves the installed grid unchanged
   const long vs = h->nCell + 2L * NB * NB * nSlab;
   const size_t stateBytes = (size_t)h->elemSize * NVAR * vs;
-  const size_t nc = (size_t)nSlab * NB * NB * 4 * 9;
+  const size_t nc = (size_t)nSlab * NB * NB * 4 * NCORR;
   void *nU[3] = {nullptr, nullptr, nullptr}, *nCorr = nullptr;
   int *nSlabs = nullptr, *nKind = nullptr, *nFine = nullptr;
   auto fill = [&]() -> hipError_t {

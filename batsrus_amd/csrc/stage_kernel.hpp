@@ -36,6 +36,10 @@ namespace bats {
 
 enum { S_DTCB = 0, S_DT = 1, S_TSIM = 2, S_NSTEP = 3, S_DTMIN = 4, S_NSCAL = 8 };
 constexpr int kThreads = 256;
+// record of one fine sub-face at a resolution change: flux slots 0..6,
+// PP <- Vdt, EN, then the normal field of the face's left and right states
+// times 1/4 (save_cons_flux, src/ModConserveFlux.f90:129-140)
+enum { CORR_BNL = 9, CORR_BNR = 10, NCORR = 11 };
 
 struct StageArgs {
   // state buffers, double or float (the kernel's storage type T)
@@ -60,8 +64,8 @@ struct StageArgs {
   int advanceTime;     // add Dt/nStage to tSimulation when done
   long nCellTot;       // interior cells; ghost slabs follow in each plane
   const int *slabKind; // resolution-change slabs: 0 coarser, 1 finer side
-  const void *corr;    // summed fine fluxes of the finer sides (type T),
-                       // [slab][q][p][sub 0..3][9 slots]; null: no slabs
+  const void *corr;    // fine sub-face records of the finer sides (type
+                       // T), [slab][q][p][sub 0..3][NCORR]; null: no slabs
   int nGroup;          // workgroup-sized groups of blocks (work items)
   int nWorkgroup;      // launched workgroups (< nGroup: persistent grid)
   const void *lowCrit; // low order weight of every face (type T, see
@@ -261,14 +265,28 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   };
   // a face on a side with finer neighbours takes the sum of the four fine
   // face fluxes (apply_cons_flux, src/ModConserveFlux.f90:260-441)
-  auto corrected = [&](int blk, int side, int p, int q, T *F) {
+  auto fine_record = [&](int blk, int side, int p, int q) -> const T * {
     const int n = a.nei[6 * blk + side];
-    if (n > -2 || a.slabKind[-2 - n] != 1) return;
-    const T *c = static_cast<const T *>(a.corr) +
-                 ((((long)(-2 - n)) * NB + q) * NB + p) * 4 * NFS;
+    if (n > -2 || a.slabKind[-2 - n] != 1) return nullptr;
+    return static_cast<const T *>(a.corr) +
+           ((((long)(-2 - n)) * NB + q) * NB + p) * 4 * NCORR;
+  };
+  auto sum4 = [&](const T *c, int v) {
+    return ((c[v] + c[NCORR + v]) + c[2 * NCORR + v]) + c[3 * NCORR + v];
+  };
+  auto corrected = [&](int blk, int side, int p, int q, T *F) {
+    const T *c = fine_record(blk, side, p, q);
+    if (!c) return;
 #pragma unroll
-    for (int v = 0; v < NFS; v++)
-      F[v] = ((c[v] + c[NFS + v]) + c[2 * NFS + v]) + c[3 * NFS + v];
+    for (int v = 0; v < NFS; v++) F[v] = sum4(c, v);
+  };
+  // and, for the 8-wave source, so do the normal field of its left and
+  // right face states (:355-372)
+  auto corrected_bn = [&](int blk, int side, int p, int q, T &L, T &R) {
+    const T *c = fine_record(blk, side, p, q);
+    if (!c) return;
+    L = sum4(c, CORR_BNL);
+    R = sum4(c, CORR_BNR);
   };
   auto sFi = [&](int l, int dir, int v, int q) {
     return ((l * 2 + dir) * NFS + v) * NFX + q;
@@ -367,7 +385,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         dLk[v] = dl;
       }
       zface(FzLo, 0);
-      if (a.corr) corrected(bb, 4, i, j, FzLo);
+      if (a.corr) {
+        corrected(bb, 4, i, j, FzLo);
+        if (POWELL) corrected_bn(bb, 4, i, j, FzLo[NFZ - 2], FzLo[NFZ - 1]);
+      }
 #pragma unroll
       for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
     }
@@ -377,7 +398,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       if (act) {
         load_prim(k + 2, Q3);
         zface(FzHi, k + 1);
-        if (a.corr && k == NB - 1) corrected(bb, 5, i, j, FzHi);
+        if (a.corr && k == NB - 1) {
+          corrected(bb, 5, i, j, FzHi);
+          if (POWELL) corrected_bn(bb, 5, i, j, FzHi[NFZ - 2], FzHi[NFZ - 1]);
+        }
         // own primitive of plane k
 #pragma unroll
         for (int v = 0; v < NVAR; v++) sP[sPi(lb, v, j + 2, i + 2)] = Q1[v];
@@ -524,6 +548,12 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               face_lr<LIM>(by[0], by[1], p[0], p[1], p[2], p[3], LyLo, RyLo);
               face_lr<LIM>(by[1], by[2], p[1], p[2], p[3], p[4], LyHi, RyHi);
             }
+            if (a.corr) {
+              if (i == 0) corrected_bn(b, 0, j, k, LxLo, RxLo);
+              if (i == NB - 1) corrected_bn(b, 1, j, k, LxHi, RxHi);
+              if (j == 0) corrected_bn(b, 2, i, k, LyLo, RyLo);
+              if (j == NB - 1) corrected_bn(b, 3, i, k, LyHi, RyHi);
+            }
             T Src[NFLUX];
             powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
                           RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
@@ -605,7 +635,7 @@ struct AmrArgs {
   Phys ph;
   double beta;
   int lim;
-  void *corr;           // type T, [slab][q][p][sub][9]
+  void *corr;           // type T, [slab][q][p][sub][NCORR]
   int *err;
   const void *lowCrit;  // low order face weights (type T) or null
   const double *bcState; // CellState_VI(8, 6) of the fixed outer sides
@@ -767,8 +797,12 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
   F[PP] = vdt;
   const int pc = sl[4] + p / 2, qc = sl[5] + q / 2, sub = (p % 2) + 2 * (q % 2);
   T *dst = static_cast<T *>(a.corr) +
-           ((((long)sl[7]) * NB + qc) * NB + pc) * 4 * 9 + sub * 9;
+           ((((long)sl[7]) * NB + qc) * NB + pc) * 4 * NCORR + sub * NCORR;
   for (int v = 0; v < 9; v++) dst[v] = F[v];
+  // the normal field of the face states times FaceRatio = 1/4
+  // (src/ModConserveFlux.f90:136-140)
+  dst[CORR_BNL] = L[BX + d] * T(0.25);
+  dst[CORR_BNR] = R[BX + d] * T(0.25);
   if (err) atomicOr(a.err, err);
 }
 

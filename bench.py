@@ -78,15 +78,44 @@ def _time_oracle(nroot_z, nthreads, target_seconds):
     return g, n, el
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads this process may use: the CPUs it is allowed to run on,
+    capped by OMP_NUM_THREADS when the environment sets the job's CPU share
+    (a one-GPU job on a shared host gets a share of the host's cores)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return n
+
+
 def cpu_baseline(target_seconds):
     """Time the CPU oracle on a bounded sample of the same workload
-    (a slab of the 256^3 box) on all host threads (up to 16), and on one
-    thread for a smaller slab."""
-    nthreads = min(os.cpu_count() or 1, 16)
+    (a slab of the 256^3 box) on every host thread this job may use, and on
+    one thread for a smaller slab."""
+    nthreads = cpu_threads()
     g, n, el = _time_oracle(2, nthreads, target_seconds)
     g1, n1, el1 = _time_oracle(1, 1, target_seconds / 2)
     return {"value": g.nCellTotal * n / el, "unit": "cell-updates/s",
             "cores": nthreads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "threads_note": "OpenMP threads = the CPUs this job may use "
+                            "(affinity, capped by OMP_NUM_THREADS, the "
+                            "job's share of the host)",
             "sample": "CPU oracle (C restatement of the reference Fortran "
                       "orig path, OpenMP over blocks) on 512 of the 4096 "
                       "blocks (16x16x2 slab, 2.1M cells), %d steps, "
@@ -126,8 +155,8 @@ def measure_traffic(timeout=150):
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d,
                    "-o", "run", "--", sys.executable,
                    os.path.abspath(__file__), "--steps", "2", "--warmup",
-                   "0", "--no-cpu", "--no-surrogate", "--no-traffic",
-                   "--copy-calibration"]
+                   "0", "--no-cpu", "--no-surrogate", "--no-ensemble",
+                   "--no-traffic", "--copy-calibration"]
             r = subprocess.run(cmd, stdout=subprocess.DEVNULL,
                                stderr=subprocess.DEVNULL, timeout=timeout)
             if r.returncode != 0:
@@ -163,6 +192,42 @@ def measure_traffic(timeout=150):
                    "write_scale": per["WRITE_SIZE"][0],
                    "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
                              "4 stage launches, scaled by a 1.07 GB copy"}
+
+
+def ensemble_leg(steps=20):
+    """The batched ensemble: 48 independent blast-wave runs of 8 to 4096
+    blocks of 8^3 (seeded log-uniform counts), MC beta 1.5, HLLD, per-run
+    CFL time steps, all runs advanced by one launch per stage; FP64 and
+    FP32 state."""
+    from batsrus_amd.core import Session
+    from batsrus_amd.ensemble import blast_ensemble
+    from batsrus_amd.workloads import ENS_SCHEME
+    g, U0 = blast_ensemble(nRun=48)
+    out = {"workload": "48 blast-wave runs, %d blocks of 8^3 in all (%d "
+                       "cells), MC beta=1.5, HLLD+Linde, 2-stage, CFL 0.8, "
+                       "per-run dt, %d steps" % (g.nBlock, g.nCellTotal,
+                                                 steps),
+           "unit": "cell-updates/s", "runs": g.nRun}
+    for prec in ("f64", "f32"):
+        with Session(g, precision=prec, **ENS_SCHEME) as s:
+            s.set_runs(g.iBlockFirst_R)
+            s.upload_soa(U0)
+            s.init_timestep()
+            s.advance(3)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.advance(steps)
+            s.synchronize()
+            el = time.perf_counter() - t0
+            kms, _ = s.last_stage_kernel_ms()
+            s.check_nan()
+        rate = g.nCellTotal * steps / el
+        bpc = 160.0 if prec == "f64" else 80.0
+        out[prec] = {"value": rate, "ms_per_step": el * 1e3 / steps,
+                     "stage_kernel_ms": kms,
+                     "hbm_frac": g.nCellTotal * bpc / (kms * 1e-3) / 1e9 /
+                     HBM_PEAK_GBS}
+    return out
 
 
 def surrogate_leg(steps=200, samples=20_000_000, epochs=100, batch=8192,
@@ -224,6 +289,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that "
                          "measure HBM bytes per stage launch")
@@ -314,6 +380,11 @@ def main():
             s2.synchronize()
         out["copy_calibration_bytes"] = {"read": ncell * 64,
                                          "write": ncell * 64}
+    if not args.no_ensemble:
+        try:
+            out["ensemble"] = ensemble_leg()
+        except Exception as e:            # reported, not fatal to the line
+            out["ensemble"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not args.no_surrogate:
         try:
             out["surrogate"] = surrogate_leg()

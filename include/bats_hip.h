@@ -152,8 +152,10 @@ int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
  * prolongation for fine ghosts) and the coarse faces of kind-1 sides take
  * the sum of the four fine face fluxes, as save_cons_flux / apply_cons_flux
  * do (src/ModConserveFlux.f90:79-441, src/ModAdvanceExplicit.f90:95-128).
- * Reallocates the state: call before uploading it.  Not with the 8-wave
- * source; the per-block routines need a grid without resolution changes. */
+ * With the 8-wave source, the normal field of the coarse face's left and
+ * right states is likewise the mean of the four fine faces' (:355-372),
+ * which the div B of the coarse cells next to the face reads.
+ * Reallocates the state: call before uploading it. */
 int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const double *CellSize_DB, int nSlab,
                       const int *iSlab_IS);

@@ -1386,13 +1386,29 @@ static double *side_face_flux(const ob_config *c, int side, int p, int q,
   return FZ + idx * OB_NFACEVALUE;
 }
 
+/* the face state (8 values) of the boundary face (p, q) of side `side` in
+ * the block's face state arrays LeftState_V[XYZ] / RightState_V[XYZ] */
+static double *side_face_state(const ob_config *c, int side, int p, int q,
+                               double *X, double *Y, double *Z) {
+  const int NB = c->nI, d = side / 2, f = (side % 2) ? NB : 0;
+  long idx;
+  if (d == 0) { idx = ((long)q * NB + p) * (NB + 1) + f; return X + idx * NV; }
+  if (d == 1) { idx = ((long)q * (NB + 1) + f) * NB + p; return Y + idx * NV; }
+  idx = ((long)f * NB + q) * NB + p;
+  return Z + idx * NV;
+}
+
 /* save_cons_flux (src/ModConserveFlux.f90:79-187) for one fine block: its
  * face fluxes at coarser sides go to the coarse slab's corrected-flux store
- * Corr[(slab*NB*NB + qc*NB + pc)*4 + sub][0..Vdt] */
+ * Corr[(slab*NB*NB + qc*NB + pc)*4 + sub][0..Vdt], and the normal field of
+ * the face's left and right states times FaceRatio = 1/4 to the BnL and BnR
+ * slots (:136-140, Cartesian) */
 static void save_cons_flux(const ob_session *s, int b, double *FX, double *FY,
-                           double *FZ) {
+                           double *FZ, double *LX, double *RX, double *LY,
+                           double *RY, double *LZ, double *RZ) {
   const ob_config *c = &s->c;
   const int NB = c->nI;
+  const double FaceRatio = 0.25;  /* 1/2**(nDim-1), :54 */
   int side, p, q, v;
   for (side = 0; side < 6; side++) {
     const int n = s->iNeighbor_EB[6 * b + side];
@@ -1402,19 +1418,26 @@ static void save_cons_flux(const ob_session *s, int b, double *FX, double *FY,
     for (q = 0; q < NB; q++)
       for (p = 0; p < NB; p++) {
         const double *F = side_face_flux(c, side, p, q, FX, FY, FZ);
+        const double *L = side_face_state(c, side, p, q, LX, LY, LZ);
+        const double *R = side_face_state(c, side, p, q, RX, RY, RZ);
         const int pc = sl[4] + p / 2, qc = sl[5] + q / 2;
         const int sub = (p % 2) + 2 * (q % 2);
         double *dst = s->Corr + (((long)sl[7] * NB + qc) * NB + pc) * 4 * OB_NFACEVALUE +
                       sub * OB_NFACEVALUE;
         for (v = 0; v <= OB_VDT; v++) dst[v] = F[v];
+        dst[OB_BNL] = L[OB_BX + side / 2] * FaceRatio;
+        dst[OB_BNR] = R[OB_BX + side / 2] * FaceRatio;
       }
   }
 }
 
 /* apply_cons_flux (src/ModConserveFlux.f90:260-441) for one coarse block:
- * the face fluxes at finer sides become the sums of the four fine ones */
+ * the face fluxes at finer sides become the sums of the four fine ones, and
+ * so do the normal field of the face's left and right states (:355-372,
+ * Cartesian), which the 8-wave source reads */
 static void apply_cons_flux(const ob_session *s, int b, double *FX, double *FY,
-                            double *FZ) {
+                            double *FZ, double *LX, double *RX, double *LY,
+                            double *RY, double *LZ, double *RZ) {
   const ob_config *c = &s->c;
   const int NB = c->nI;
   int side, p, q, v;
@@ -1426,10 +1449,18 @@ static void apply_cons_flux(const ob_session *s, int b, double *FX, double *FY,
     for (q = 0; q < NB; q++)
       for (p = 0; p < NB; p++) {
         double *F = side_face_flux(c, side, p, q, FX, FY, FZ);
+        double *L = side_face_state(c, side, p, q, LX, LY, LZ);
+        double *R = side_face_state(c, side, p, q, RX, RY, RZ);
         const double *src = s->Corr + (((long)t * NB + q) * NB + p) * 4 * OB_NFACEVALUE;
         for (v = 0; v <= OB_VDT; v++)
           F[v] = ((src[v] + src[OB_NFACEVALUE + v]) + src[2 * OB_NFACEVALUE + v]) +
                  src[3 * OB_NFACEVALUE + v];
+        L[OB_BX + side / 2] =
+            ((src[OB_BNL] + src[OB_NFACEVALUE + OB_BNL]) +
+             src[2 * OB_NFACEVALUE + OB_BNL]) + src[3 * OB_NFACEVALUE + OB_BNL];
+        R[OB_BX + side / 2] =
+            ((src[OB_BNR] + src[OB_NFACEVALUE + OB_BNR]) +
+             src[2 * OB_NFACEVALUE + OB_BNR]) + src[3 * OB_NFACEVALUE + OB_BNR];
       }
   }
 }
@@ -1579,7 +1610,8 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
 #endif
           if (!err) err = e;
         }
-        save_cons_flux(s, b, w.FX, w.FY, w.FZ);
+        save_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                       w.RZ);
       }
       scratch_free(&w);
     }
@@ -1607,7 +1639,9 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
                             w.LZ, w.RZ);
       e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
                             w.FY, w.FZ);
-      if (s->nSlab > 0) apply_cons_flux(s, b, w.FX, w.FY, w.FZ);
+      if (s->nSlab > 0)
+        apply_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                        w.RZ);
       if (e) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -1637,6 +1671,52 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
     }
     scratch_free(&w);
   }
+  return err;
+}
+
+/* The 8-wave source Source_VC of every block (nBlock x nCell x 9) from the
+ * current state, computed as the stage computes it: fine blocks next to
+ * coarser ones save their face fluxes and normal-field face states, every
+ * block's face states and fluxes at finer sides take the sums
+ * (apply_cons_flux), then calc_source.  No state changes. */
+int ob_session_sources(ob_session *s, double *Source) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const long len = block_len(&sh);
+  const long nCell = (long)c->nI * c->nJ * c->nK;
+  int b, err = OB_OK;
+  ob_scratch w;
+  scratch_alloc(c, &w);
+  for (b = 0; b < s->nBlock && s->nSlab > 0; b++) {
+    int rc[6];
+    const double *CX, *CY, *CZ;
+    if (!has_kind(s, b, 0)) continue;
+    block_crit(s, b, &CX, &CY, &CZ);
+    block_res_change(s, b, rc);
+    ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w.LX,
+                          w.RX, w.LY, w.RY, w.LZ, w.RZ);
+    err |= ob_calc_face_flux(c, s->CellSize_DB + 3L * b, w.LX, w.RX, w.LY,
+                             w.RY, w.LZ, w.RZ, w.FX, w.FY, w.FZ);
+    save_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                   w.RZ);
+  }
+  for (b = 0; b < s->nBlock; b++) {
+    int rc[6];
+    const double *CX, *CY, *CZ;
+    const double *cs = s->CellSize_DB + 3L * b;
+    block_crit(s, b, &CX, &CY, &CZ);
+    block_res_change(s, b, rc);
+    ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w.LX,
+                          w.RX, w.LY, w.RY, w.LZ, w.RZ);
+    err |= ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
+                             w.FY, w.FZ);
+    if (s->nSlab > 0)
+      apply_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                      w.RZ);
+    ob_calc_source(c, cs, s->State + (long)b * len, w.LX, w.RX, w.LY, w.RY,
+                   w.LZ, w.RZ, Source + (long)b * nCell * OB_NFLUX);
+  }
+  scratch_free(&w);
   return err;
 }
 

@@ -231,6 +231,9 @@ void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit);
 /* one stage of advance_explicit: all blocks, time advance, ghost exchange
  * between stages (src/ModAdvanceExplicit.f90:74-312) */
 int ob_advance_stage(ob_session *s, int iStage, int DoCalcTimestep);
+/* every block's 8-wave source from the current state, as a stage computes
+ * it (with the resolution-change correction of the face fields) */
+int ob_session_sources(ob_session *s, double *Source);
 /* advance_explicit (src/ModAdvanceExplicit.f90:18-325), orig CPU path */
 int ob_advance_explicit(ob_session *s, int DoCalcTimestep);
 /* BATS_advance-shaped time step (src/ModBatsrusMethods.f90:566-653) */

@@ -101,6 +101,8 @@ def lib():
         L.ob_advance_stage.restype = C.c_int
         L.ob_advance_step.argtypes = [C.POINTER(ObSession), C.c_double]
         L.ob_advance_step.restype = C.c_int
+        L.ob_session_sources.argtypes = [C.POINTER(ObSession), dp]
+        L.ob_session_sources.restype = C.c_int
         L.ob_init_timestep.argtypes = [C.POINTER(ObSession)]
         L.ob_init_timestep.restype = C.c_int
         L.ob_exchange_amr.argtypes = [C.POINTER(ObConfig), C.c_int, C.c_int,
@@ -345,6 +347,17 @@ class Session:
 
     def init_timestep(self):
         return lib().ob_init_timestep(C.byref(self.s))
+
+    def sources(self):
+        """Source_VC of every block (nBlock, nK, nJ, nI, 9) from the current
+        state (ghost cells as last exchanged), with the resolution-change
+        correction of the face fields."""
+        n = self.grid.nBlock * self.cfg.nI * self.cfg.nJ * self.cfg.nK
+        src = np.zeros(n * 9)
+        err = lib().ob_session_sources(C.byref(self.s), _dp(src))
+        assert err == 0
+        return src.reshape(self.grid.nBlock, self.cfg.nK, self.cfg.nJ,
+                           self.cfg.nI, 9)
 
     def set_global_timestep(self, tlimit=-1.0):
         lib().ob_set_global_timestep(C.byref(self.s), tlimit)

@@ -121,3 +121,69 @@ def test_two_level_uniform_state():
     for _ in range(2):
         o.advance_step()
     assert np.allclose(o.soa(), U0, rtol=1e-13, atol=1e-15)
+
+
+def test_powell_source_takes_fine_face_fields():
+    """The 8-wave source of a coarse cell next to finer blocks uses, at the
+    resolution-change face, the mean of the four fine faces' normal field
+    (left and right states separately, src/ModConserveFlux.f90:136-140,
+    355-372): recomputed here from the fine blocks' own face values and the
+    plain per-block source routine, it equals the session's source; every
+    other block's source is the plain per-block one; and the correction
+    changes the coarse cells' source (the check is not vacuous)."""
+    g = _grid()
+    U0 = smooth_mhd_soa(g, seed=31, amp=0.3)
+    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc", beta=1.5,
+                                   UseDivbSource=True), g)
+    o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
+    o.exchange()
+    src = o.sources()
+    S = o.State
+    nb = g.nI
+    H = nb // 2
+
+    def rc_flags(b):
+        return [int(g.iNeighbor_EB[b, s] < -1) for s in range(6)]
+
+    def faces_of(b):
+        return po.calc_face_value(cfg, np.ascontiguousarray(S[b]),
+                                  res_change=rc_flags(b))
+
+    def face_view(F, d, f):
+        # the face layer f normal to d, indexed [q, p] as the slab table
+        return F[:, :, f] if d == 0 else F[:, f, :] if d == 1 else F[f]
+
+    n_corrected = 0
+    for b in range(g.nBlock):
+        faces = faces_of(b)
+        plain = po.calc_source(cfg, g.CellSize_DB[b], S[b],
+                               [f.copy() for f in faces])
+        for side in range(6):
+            t = -2 - g.iNeighbor_EB[b, side]
+            if t < 0 or g.iSlab_IS[t][2] != 1:
+                continue
+            d, hi = side // 2, side % 2
+            kids = g.iSlab_IS[t][3:7]
+            L = face_view(faces[2 * d], d, nb if hi else 0)
+            R = face_view(faces[2 * d + 1], d, nb if hi else 0)
+            for q in range(nb):
+                for p in range(nb):
+                    kid = kids[(p // H) + 2 * (q // H)]
+                    kf = faces_of(kid)
+                    KL = face_view(kf[2 * d], d, 0 if hi else nb)
+                    KR = face_view(kf[2 * d + 1], d, 0 if hi else nb)
+                    pf, qf = 2 * (p % H), 2 * (q % H)
+                    for X, KX in ((L, KL), (R, KR)):
+                        v = [KX[qf, pf, 4 + d], KX[qf, pf + 1, 4 + d],
+                             KX[qf + 1, pf, 4 + d], KX[qf + 1, pf + 1, 4 + d]]
+                        X[q, p, 4 + d] = ((0.25 * v[0] + 0.25 * v[1]) +
+                                          0.25 * v[2]) + 0.25 * v[3]
+            n_corrected += 1
+        want = po.calc_source(cfg, g.CellSize_DB[b], S[b],
+                              [f.copy() for f in faces])
+        assert np.array_equal(src[b], want), b
+        if any(g.iNeighbor_EB[b, s] < -1 and
+               g.iSlab_IS[-2 - g.iNeighbor_EB[b, s]][2] == 1
+               for s in range(6)):
+            assert np.abs(want - plain).max() > 1e-6
+    assert n_corrected > 0

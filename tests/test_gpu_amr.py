@@ -17,13 +17,17 @@ def _grid():
     return AmrGrid((4, 4, 4), sphere_refinement((4, 4, 4), 0.3))
 
 
-@pytest.mark.parametrize("flux,limiter", [("rusanov", "mc"), ("hlld", "mc3"),
-                                          ("linde", "minmod")])
-def test_two_level_stages_match_oracle(flux, limiter):
-    print(f"case amr flux={flux} lim={limiter}", flush=True)
+@pytest.mark.parametrize("flux,limiter,powell", [
+    ("rusanov", "mc", False), ("hlld", "mc3", False),
+    ("linde", "minmod", False), ("hlld", "mc", True), ("rusanov", "mc3", True)])
+def test_two_level_stages_match_oracle(flux, limiter, powell):
+    """With the 8-wave source the coarse cells next to a resolution change
+    take the fine faces' normal field at that face."""
+    print(f"case amr flux={flux} lim={limiter} powell={powell}", flush=True)
     g = _grid()
     U0 = smooth_mhd_soa(g, seed=71, amp=0.2)
-    scheme = dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8)
+    scheme = dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
+                  UseDivbSource=powell)
     cfg = po.config_from_dict(scheme, g)
     S = g.soa_to_state_vgb(U0)
     o = po.Session(g, cfg, S)
