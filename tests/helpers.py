@@ -49,3 +49,20 @@ def rel_err(a, b, axis_var=0):
     b2 = np.moveaxis(b, axis_var, 0).reshape(b.shape[axis_var], -1)
     scale = np.maximum(np.abs(b2).max(axis=1), 1e-300)
     return (np.abs(a2 - b2).max(axis=1) / scale).max()
+
+
+def rel_err_vec(a, b):
+    """max |a-b| per variable divided by the largest |b| of the physical
+    quantity the variable belongs to: rho, the momentum vector (rho u_x,
+    rho u_y, rho u_z), the field vector (B_x, B_y, B_z), p.  A vector
+    component that stays at rounding level (B_z of the Orszag-Tang vortex,
+    whose flux divergence cancels exactly) is measured against the
+    vector's size, not against its own rounding noise.  SoA arrays
+    U[var, ...]."""
+    a = np.asarray(a).reshape(8, -1)
+    b = np.asarray(b).reshape(8, -1)
+    mx = np.abs(b).max(axis=1)
+    group = [[0], [1, 2, 3], [1, 2, 3], [1, 2, 3], [4, 5, 6], [4, 5, 6],
+             [4, 5, 6], [7]]
+    scale = np.array([max(mx[g].max(), 1e-300) for g in group])
+    return (np.abs(a - b).max(axis=1) / scale).max()

@@ -5,7 +5,12 @@ r < 0.25, MC beta 1.5, HLLD + Linde fix).  On reduced grids their initial
 states advance stage by stage within 1e-12 of the oracle; on the full grids
 every conserved quantity (mass, momentum, magnetic flux, energy) is kept to
 rounding (FP64, FP32 state, and the FP32 surrogate stage with the network
-on the smooth faces)."""
+on the smooth faces).
+
+The stage differences are measured per physical quantity (rho, momentum
+vector, field vector, p): the vortex's B_z has an exactly cancelling flux
+divergence and stays at rounding level, so against its own size any two
+correct evaluations differ by O(1)."""
 import numpy as np
 import pytest
 
@@ -17,7 +22,7 @@ from batsrus_amd.core import Mlp, Session
 from batsrus_amd.grid import BlockGrid
 from batsrus_amd.workloads import KH_SCHEME, OT_SCHEME, kh_two_level, \
     orszag_tang_256
-from helpers import rel_err
+from helpers import rel_err_vec
 
 pytestmark = pytest.mark.gpu
 
@@ -40,7 +45,7 @@ def _stage_parity(grid, U0, scheme, nstep=2, tol=1e-12):
                 if stage == cfg.nStage:
                     o.exchange()
                 s.advance_stage(stage)
-                e = rel_err(s.download_soa(), o.soa())
+                e = rel_err_vec(s.download_soa(), o.soa())
                 worst = max(worst, e)
                 assert e <= tol, (step, stage, e)
     print(f"  worst stage difference {worst:.2e}")
@@ -64,7 +69,8 @@ def test_kelvin_helmholtz_two_level_reduced_grid():
 
 def _conserved_scale(grid, U):
     """sum over cells of |U| * volume per conserved quantity (energy from
-    the pressure), the scale the integrals' drift is measured against."""
+    the pressure; summed over the components for the momentum and field
+    vectors), the scale the integrals' drift is measured against."""
     V = np.prod(np.asarray(grid.CellSize_DB), axis=1)
     w = V[None, :, None, None, None]
     gm1 = 5.0 / 3.0 - 1
@@ -72,6 +78,10 @@ def _conserved_scale(grid, U):
         0.5 * (U[4] ** 2 + U[5] ** 2 + U[6] ** 2)
     sc = np.array([np.sum(np.abs(U[v]) * w[0]) for v in range(7)] +
                   [np.sum(E * w[0])])
+    # a vector's components share the vector's scale (B_z of the vortex and
+    # B_y, B_z of the shear layer start at zero)
+    sc[1:4] = sc[1:4].sum()
+    sc[4:7] = sc[4:7].sum()
     return sc
 
 
