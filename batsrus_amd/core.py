@@ -109,6 +109,7 @@ ABI_FUNCTIONS = {
     "bats_mlp_set_params": (C.c_int, [_hp, _fp]),
     "bats_mlp_get_params": (C.c_int, [_hp, _fp, _fp]),
     "bats_mlp_forward": (C.c_int, [_hp, _fp, _fp, C.c_int]),
+    "bats_mlp_forward_noalloc": (C.c_int, [_hp, _fp, _fp, C.c_int]),
     "bats_mlp_loss_grad": (C.c_int, [_hp, _fp, _fp, C.c_int, _dp, _fp]),
     "bats_mlp_set_data": (C.c_int, [_hp, _fp, _fp, C.c_long]),
     "bats_mlp_get_data": (C.c_int, [_hp, C.POINTER(C.c_long), _fp, _fp,
@@ -538,6 +539,16 @@ class Mlp:
         self._check(self.lib.bats_mlp_forward(self.h, _ptr(X, C.c_float),
                                               _ptr(Y, C.c_float), X.shape[0]),
                     "mlp_forward")
+        return Y
+
+    def forward_noalloc(self, X):
+        """forward() through the entry point that allocates no device
+        memory (rows staged through the library's fixed array)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 8), np.float32)
+        self._check(self.lib.bats_mlp_forward_noalloc(
+            self.h, _ptr(X, C.c_float), _ptr(Y, C.c_float), X.shape[0]),
+            "mlp_forward_noalloc")
         return Y
 
     def loss_grad(self, X, T):

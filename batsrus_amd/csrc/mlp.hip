@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bats_hip.h"
@@ -265,6 +266,7 @@ __global__ void k_gather_rows_f32(const float *dataX, const float *dataT,
 // =====================================================================
 struct bats_mlp {
   hipStream_t stream;
+  int dev;                   // device the model (and its stream) lives on
   float *w, *m, *v, *grad;   // FP32 master weights, Adam moments, gradient
   __bf16 *wb;                // BF16 copy of w (flat layout)
   __bf16 *frag;              // fragment-ordered BF16 weights for inference
@@ -294,6 +296,45 @@ struct bats_mlp {
 
 static const int kChunkRows = 256;  // rows per weight-gradient split
 
+// ---- staging memory of bats_mlp_forward_noalloc -----------------------
+// bats_mlp_forward allocates device buffers for the rows it stages in and
+// out on every call; the no-allocation variant stages them through this
+// fixed array instead: the inputs (BATS_MLP_NOALLOC_ROWS x 16 floats), then
+// the outputs (BATS_MLP_NOALLOC_ROWS x 8), 6 MiB in all.  Each device that
+// runs the library holds its own copy until the process ends.  In host code
+// the array's name is only a stand-in; its device address on each device
+// comes from hipGetSymbolAddress, taken once per device in bats_mlp_create
+// with that device current, and only that address is passed to the copies
+// and kernels.
+static __device__ __align__(256) float
+    scratch[(size_t)BATS_MLP_NOALLOC_ROWS * (BATS_MLP_NIN + BATS_MLP_NOUT)];
+static const int kMaxDevices = 64;
+static std::mutex g_scratchLock;
+static float *g_scratchDev[kMaxDevices];  // scratch's address per device
+
+// record the address of this device's copy of scratch (the current device
+// must be dev)
+static int noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return BATS_ENODEV;
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  if (g_scratchDev[dev]) return BATS_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "bats_mlp: hipGetSymbolAddress(scratch) failed on "
+                    "device %d: %s\n", dev, hipGetErrorString(e));
+    return BATS_EHIP;
+  }
+  g_scratchDev[dev] = static_cast<float *>(p);
+  return BATS_OK;
+}
+
+static float *noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  return g_scratchDev[dev];
+}
+
 static int repack(bats_mlp *m) {
   if (!m->fragStale) return BATS_OK;
   hipLaunchKernelGGL(k_pack_frag, dim3((NFRAG + 255) / 256), dim3(256), 0,
@@ -319,6 +360,12 @@ extern "C" int bats_mlp_create(bats_mlp **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return BATS_ENODEV;
   bats_mlp *m = new bats_mlp();
   memset(m, 0, sizeof *m);
+  MCHK(hipGetDevice(&m->dev));
+  const int st = noalloc_setup(m->dev);
+  if (st) {
+    delete m;
+    return st;
+  }
   MCHK(hipStreamCreate(&m->stream));
   MCHK(hipMalloc(&m->w, sizeof(float) * NPARAM));
   MCHK(hipMalloc(&m->m, sizeof(float) * NPARAM));
@@ -547,6 +594,29 @@ extern "C" int bats_mlp_forward(bats_mlp *m, const float *X, float *Y, int rows)
   MCHK(hipStreamSynchronize(m->stream));
   (void)hipFree(dX);
   (void)hipFree(dY);
+  return st;
+}
+
+extern "C" int bats_mlp_forward_noalloc(bats_mlp *m, const float *X,
+                                        float *Y, int rows) {
+  if (!m || !X || !Y || rows <= 0) return BATS_EINVAL;
+  if (rows > BATS_MLP_NOALLOC_ROWS) {
+    fprintf(stderr, "bats_mlp_forward_noalloc: %d rows exceed the staging "
+                    "limit of %d rows\n", rows, (int)BATS_MLP_NOALLOC_ROWS);
+    return BATS_ELIMIT;
+  }
+  float *dX = noalloc_scratch(m->dev);
+  if (!dX) return BATS_EHIP;
+  float *dY = dX + (size_t)BATS_MLP_NOALLOC_ROWS * BATS_MLP_NIN;
+  // every row written by the copy in is read, every row read by the copy
+  // out is written by the kernel: nothing in scratch needs clearing
+  MCHK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)rows * BATS_MLP_NIN,
+                      hipMemcpyHostToDevice, m->stream));
+  const int st = bats::mlp::forward_device(m, dX, dY, rows, m->stream);
+  if (st == BATS_OK)
+    MCHK(hipMemcpyAsync(Y, dY, sizeof(float) * (size_t)rows * BATS_MLP_NOUT,
+                        hipMemcpyDeviceToHost, m->stream));
+  MCHK(hipStreamSynchronize(m->stream));
   return st;
 }
 

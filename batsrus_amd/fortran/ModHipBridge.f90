@@ -44,7 +44,8 @@ module ModHipBridge
        bats_set_amr_grid, bats_set_low_order_face, bats_get_low_order_crit, &
        bats_set_step_graph, bats_set_cell_boundary, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
-       bats_mlp_get_params, bats_mlp_forward, bats_mlp_loss_grad, &
+       bats_mlp_get_params, bats_mlp_forward, bats_mlp_forward_noalloc, &
+       bats_mlp_loss_grad, &
        bats_mlp_set_data, bats_mlp_get_data, bats_mlp_harvest, &
        bats_mlp_train, bats_set_surrogate, bats_surrogate_counts
   ! reference-named routines on the module-level handle
@@ -324,6 +325,16 @@ module ModHipBridge
        real(c_float), intent(out):: Y(*)
        integer(c_int), value:: nRow
      end function bats_mlp_forward
+     ! the same without device allocation (at most BATS_MLP_NOALLOC_ROWS
+     ! rows, staged through the library's fixed per-device array)
+     integer(c_int) function bats_mlp_forward_noalloc(Model, X, Y, nRow) &
+          bind(C, name="bats_mlp_forward_noalloc")
+       import:: c_int, c_ptr, c_float
+       type(c_ptr), value:: Model
+       real(c_float), intent(in):: X(*)
+       real(c_float), intent(out):: Y(*)
+       integer(c_int), value:: nRow
+     end function bats_mlp_forward_noalloc
      integer(c_int) function bats_mlp_loss_grad(Model, X, T, nRow, Loss, &
           Grad) bind(C, name="bats_mlp_loss_grad")
        import:: c_int, c_ptr, c_float, c_double

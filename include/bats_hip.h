@@ -343,6 +343,19 @@ int bats_mlp_set_params(bats_mlp *m, const float *W);
 int bats_mlp_get_params(bats_mlp *m, float *W, float *Grad);
 /* Y(nRow x 8) = network(X(nRow x 16)), host arrays */
 int bats_mlp_forward(bats_mlp *m, const float *X, float *Y, int nRow);
+/* bats_mlp_forward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The rows are staged through a fixed device array of the
+ * library (BATS_MLP_NOALLOC_ROWS rows of inputs and outputs, one copy per
+ * device, held until the process ends) instead of per-call buffers; nRow
+ * above BATS_MLP_NOALLOC_ROWS returns BATS_ELIMIT.  Results are bitwise
+ * those of bats_mlp_forward.  All calls on one device share its array:
+ * calls that overlap in time on one device (from several host threads or
+ * models) race. */
+enum { BATS_MLP_NOALLOC_ROWS = 65536 };
+int bats_mlp_forward_noalloc(bats_mlp *m, const float *X, float *Y,
+                             int nRow);
 /* mean-square loss of the network on (X, T) and its gradient (27,464) */
 int bats_mlp_loss_grad(bats_mlp *m, const float *X, const float *T, int nRow,
                        double *Loss, float *Grad);
