@@ -88,6 +88,9 @@ ABI_FUNCTIONS = {
     "bats_set_stage": (C.c_int, [_hp, C.c_int]),
     "bats_calc_face_value": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int]),
     "bats_calc_face_flux": (C.c_int, [_hp, C.c_int, C.c_int]),
+    "bats_save_cons_flux": (C.c_int, [_hp, C.c_int]),
+    "bats_message_pass_face": (C.c_int, [_hp]),
+    "bats_apply_cons_flux": (C.c_int, [_hp, C.c_int]),
     "bats_calc_source": (C.c_int, [_hp, C.c_int]),
     "bats_update_state": (C.c_int, [_hp, C.c_int]),
     "bats_calc_timestep": (C.c_int, [_hp, C.c_int, C.c_int]),
@@ -354,6 +357,18 @@ class Session:
     def calc_face_flux(self, DoResChangeOnly, iBlock):
         self._check(self.lib.bats_calc_face_flux(
             self.h, int(DoResChangeOnly), iBlock), "calc_face_flux")
+
+    def save_cons_flux(self, iBlock):
+        self._check(self.lib.bats_save_cons_flux(self.h, iBlock),
+                    "save_cons_flux")
+
+    def message_pass_face(self):
+        self._check(self.lib.bats_message_pass_face(self.h),
+                    "message_pass_face")
+
+    def apply_cons_flux(self, iBlock):
+        self._check(self.lib.bats_apply_cons_flux(self.h, iBlock),
+                    "apply_cons_flux")
 
     def calc_source(self, iBlock):
         self._check(self.lib.bats_calc_source(self.h, iBlock), "calc_source")

@@ -98,6 +98,13 @@ struct BlockArgs {
   double *FX, *FY, *FZ;                 // Flux_V?I(14, ...)
   double *Src;                          // Source_VC(9, nI, nJ, nK)
   int useSrc;
+  // resolution changes: slab list (8 ints per slab), slab kinds, the
+  // fine sub-face records of the coarse sides; resOnly restricts the face
+  // routines to the faces of sides whose neighbour is coarser
+  // (DoResChangeOnly)
+  const int *slabs, *slabKind;
+  double *corr;
+  int resOnly;
   const double *lowCrit;                // low order weights or null
   double betaRC;                        // BetaLimiterResChange
   int nFaceRC;                          // its faces (0: not used)
@@ -113,6 +120,17 @@ struct BlockArgs {
   int iStage, nStage, halfStep, dbTrick, useDtFixed;
   double DtFixed;
 };
+
+// whether face (fi, fj, fk) normal to dir lies on a side of the block whose
+// neighbour is coarser (DiLevel_EB = 1: a kind-0 slab), the faces that
+// calc_face_value / calc_face_flux visit with DoResChangeOnly
+__device__ inline bool coarser_side_face(const BlockArgs &a, int dir, int fi,
+                                         int fj, int fk, int NB) {
+  const int f = dir == 0 ? fi : dir == 1 ? fj : fk;
+  if (f != 0 && f != NB) return false;
+  const int n = a.nei[6 * a.iBlock + 2 * dir + (f == NB)];
+  return n <= -2 && a.slabKind[-2 - n] == 0;
+}
 
 template <int NB>
 __global__ void k_block_face_value(BlockArgs a) {
@@ -133,6 +151,7 @@ __global__ void k_block_face_value(BlockArgs a) {
     fi = r % NB; fj = (r / NB) % NB; fk = r / (NB * NB);
     fo = r; L = a.LZ; R = a.RZ;
   }
+  if (a.resOnly && !coarser_side_face(a, dir, fi, fj, fk, NB)) return;
   double q[4][NVAR];
   for (int s = 0; s < 4; s++) {
     int ii = fi, jj = fj, kk = fk;
@@ -184,6 +203,14 @@ __global__ void k_block_face_flux(BlockArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * nFx) return;
   const int dir = t / nFx, r = t % nFx;
+  if (a.resOnly) {
+    const int fi = dir == 0 ? r % (NB + 1) : r % NB;
+    const int fj = dir == 0 ? (r / (NB + 1)) % NB
+                 : dir == 1 ? (r / NB) % (NB + 1) : (r / NB) % NB;
+    const int fk = dir == 0 ? r / ((NB + 1) * NB)
+                 : dir == 1 ? r / (NB * (NB + 1)) : r / (NB * NB);
+    if (!coarser_side_face(a, dir, fi, fj, fk, NB)) return;
+  }
   const double *cs = a.cellsize + 3 * a.iBlock;
   double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
   const double *Ls = dir == 0 ? a.LX : (dir == 1 ? a.LY : a.LZ);
@@ -198,6 +225,71 @@ __global__ void k_block_face_flux(BlockArgs a) {
   for (int v = 0; v < NFLUX; v++) o[v] = F[v];
   o[9] = unA; o[10] = 0.0; o[11] = vdt; o[12] = 0.0; o[13] = 0.0;
   if (err) atomicOr(a.err, err);
+}
+
+// index of the face (p, q) of block side `side` in the per-block face
+// arrays (LeftState_VX(nVar, nI+1, nJ, nK) etc.); the in-side order (p, q)
+// is (j, k) / (i, k) / (i, j) on x / y / z sides
+__device__ inline long side_face(int side, int p, int q, int NB) {
+  const int d = side / 2, f = (side % 2) ? NB : 0;
+  if (d == 0) return ((long)q * NB + p) * (NB + 1) + f;
+  if (d == 1) return ((long)q * (NB + 1) + f) * NB + p;
+  return ((long)f * NB + q) * NB + p;
+}
+
+// save_cons_flux (src/ModConserveFlux.f90:79-187) for one fine block: the
+// fluxes of its faces on coarser sides (slots 0..6, Vdt, energy) and the
+// normal field of their left and right states times FaceRatio = 1/4
+// (:136-140) go to the coarse side's record of the face's sub-face, the
+// store that the fused path's fine-flux kernel writes too
+template <int NB>
+__global__ void k_block_save_cons_flux(BlockArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * NB * NB) return;
+  const int side = t / (NB * NB), p = t % NB, q = (t / NB) % NB;
+  const int n = a.nei[6 * a.iBlock + side];
+  if (n > -2 || a.slabKind[-2 - n] != 0) return;
+  const int *sl = a.slabs + 8 * (-2 - n);
+  const int d = side / 2;
+  const long f = side_face(side, p, q, NB);
+  const double *F = (d == 0 ? a.FX : d == 1 ? a.FY : a.FZ) + f * 14;
+  const double *L = (d == 0 ? a.LX : d == 1 ? a.LY : a.LZ) + f * NVAR;
+  const double *R = (d == 0 ? a.RX : d == 1 ? a.RY : a.RZ) + f * NVAR;
+  const int pc = sl[4] + p / 2, qc = sl[5] + q / 2, sub = (p % 2) + 2 * (q % 2);
+  double *dst = a.corr + (((long)sl[7] * NB + qc) * NB + pc) * 4 * NCORR +
+                sub * NCORR;
+  for (int v = 0; v < 7; v++) dst[v] = F[v];
+  dst[PP] = F[11];   // Vdt
+  dst[EN] = F[EN];
+  dst[CORR_BNL] = L[BX + d] * 0.25;
+  dst[CORR_BNR] = R[BX + d] * 0.25;
+}
+
+// apply_cons_flux (src/ModConserveFlux.f90:260-441) for one coarse block:
+// the faces of its sides with finer neighbours take the sums of the four
+// fine sub-face fluxes and Vdt, and their left and right states the sums
+// of the fine normal field (:350-384)
+template <int NB>
+__global__ void k_block_apply_cons_flux(BlockArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * NB * NB) return;
+  const int side = t / (NB * NB), p = t % NB, q = (t / NB) % NB;
+  const int n = a.nei[6 * a.iBlock + side];
+  if (n > -2 || a.slabKind[-2 - n] != 1) return;
+  const int d = side / 2;
+  const long f = side_face(side, p, q, NB);
+  double *F = (d == 0 ? a.FX : d == 1 ? a.FY : a.FZ) + f * 14;
+  double *L = (d == 0 ? a.LX : d == 1 ? a.LY : a.LZ) + f * NVAR;
+  double *R = (d == 0 ? a.RX : d == 1 ? a.RY : a.RZ) + f * NVAR;
+  const double *c = a.corr + (((long)(-2 - n) * NB + q) * NB + p) * 4 * NCORR;
+  auto sum4 = [&](int v) {
+    return ((c[v] + c[NCORR + v]) + c[2 * NCORR + v]) + c[3 * NCORR + v];
+  };
+  for (int v = 0; v < 7; v++) F[v] = sum4(v);
+  F[EN] = sum4(EN);
+  F[11] = sum4(PP);
+  L[BX + d] = sum4(CORR_BNL);
+  R[BX + d] = sum4(CORR_BNR);
 }
 
 // calc_source for one block: the 8-wave div B source from the block's face
@@ -780,6 +872,10 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.FX = h->FX; a.FY = h->FY; a.FZ = h->FZ;
   a.Src = h->Src;
   a.useSrc = c.UseDivbSource;
+  a.slabs = h->slabs;
+  a.slabKind = h->slabKind;
+  a.corr = static_cast<double *>(h->corr);
+  a.resOnly = 0;
   a.lowCrit = low_order_used(c) ? static_cast<const double *>(h->lowCrit)
                                 : nullptr;
   a.betaRC = c.BetaLimiterResChange;
@@ -832,8 +928,6 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
-  // the per-block routines work on grids without resolution changes
-  if (h->nResSlab > 0) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only
   if (h->elemSize != 8) return BATS_EINVAL;
   return ensure_block_buffers(h);
@@ -841,11 +935,15 @@ static int check_block(bats_handle *h, int iBlock1) {
 
 extern "C" int bats_calc_face_value(bats_handle *h, int iBlock,
                                     int DoResChangeOnly, int DoMonotoneRestrict) {
-  (void)DoMonotoneRestrict;  // no resolution changes on a same-level grid
+  // the ghost slabs of resolution changes are filled by exchange_messages
+  // (mean of the fine cells, limited prolongation of the coarse ones)
+  (void)DoMonotoneRestrict;
   int st = check_block(h, iBlock);
   if (st) return st;
-  if (DoResChangeOnly) return BATS_OK;  // nothing to do without res changes
+  // DoResChangeOnly: only the faces of sides with a coarser neighbour
+  if (DoResChangeOnly && h->nResSlab == 0) return BATS_OK;
   BlockArgs a = block_args(h, iBlock - 1);
+  a.resOnly = DoResChangeOnly != 0;
   const int n = 3 * (h->NB + 1) * h->NB * h->NB;
   if (h->NB == 8)
     hipLaunchKernelGGL(k_block_face_value<8>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
@@ -900,13 +998,48 @@ extern "C" int bats_get_low_order_crit(bats_handle *h, int iBlock,
 extern "C" int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock) {
   int st = check_block(h, iBlock);
   if (st) return st;
-  if (DoResChangeOnly) return BATS_OK;
+  if (DoResChangeOnly && h->nResSlab == 0) return BATS_OK;
   BlockArgs a = block_args(h, iBlock - 1);
+  a.resOnly = DoResChangeOnly != 0;
   const int n = 3 * (h->NB + 1) * h->NB * h->NB;
   if (h->NB == 8)
     hipLaunchKernelGGL(k_block_face_flux<8>, dim3((n + 127) / 128), dim3(128), 0, h->stream, a);
   else
     hipLaunchKernelGGL(k_block_face_flux<16>, dim3((n + 127) / 128), dim3(128), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_save_cons_flux(bats_handle *h, int iBlock) {
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  if (h->nResSlab == 0) return BATS_OK;
+  BlockArgs a = block_args(h, iBlock - 1);
+  const int n = 6 * h->NB * h->NB;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_save_cons_flux<8>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_save_cons_flux<16>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  return hip_status(hipGetLastError());
+}
+
+extern "C" int bats_message_pass_face(bats_handle *h) {
+  // save_cons_flux already writes every fine sub-face record into the
+  // coarse side's store, in stream order before any apply_cons_flux, and
+  // apply_cons_flux adds the four sub-faces: nothing is left to move
+  if (!h) return BATS_EINVAL;
+  return BATS_OK;
+}
+
+extern "C" int bats_apply_cons_flux(bats_handle *h, int iBlock) {
+  int st = check_block(h, iBlock);
+  if (st) return st;
+  if (h->nResSlab == 0) return BATS_OK;
+  BlockArgs a = block_args(h, iBlock - 1);
+  const int n = 6 * h->NB * h->NB;
+  if (h->NB == 8)
+    hipLaunchKernelGGL(k_block_apply_cons_flux<8>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_block_apply_cons_flux<16>, dim3((n + 255) / 256), dim3(256), 0, h->stream, a);
   return hip_status(hipGetLastError());
 }
 

@@ -5,6 +5,9 @@
 !    calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
 !                                          src/ModFaceValue.f90:854
 !    calc_face_flux(DoResChangeOnly, iBlock) src/ModFaceFlux.f90:483
+!    save_cons_flux(iBlock)                src/ModConserveFlux.f90:79
+!    message_pass_face                     src/ModAdvanceExplicit.f90:127
+!    apply_cons_flux(iBlock)               src/ModConserveFlux.f90:260
 !    calc_source(iBlock)                   src/ModCalcSource.f90:245
 !    update_state(iBlock)                  src/ModUpdateState.f90:83
 !    calc_timestep(iBlock, IsPartLocal)    src/ModTimeStepControl.f90:111
@@ -35,6 +38,7 @@ module ModHipBridge
        bats_download_state, bats_upload_state_soa, bats_download_state_soa, &
        bats_exchange_messages, bats_set_stage, bats_calc_face_value, &
        bats_calc_face_flux, bats_calc_source, bats_update_state, &
+       bats_save_cons_flux, bats_message_pass_face, bats_apply_cons_flux, &
        bats_calc_timestep, &
        bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
@@ -51,6 +55,7 @@ module ModHipBridge
   ! reference-named routines on the module-level handle
   public:: init_hip_bridge, clean_hip_bridge, sync_state_to_gpu, &
        sync_state_to_cpu, calc_face_value, calc_face_flux, calc_source, &
+       save_cons_flux, message_pass_face, apply_cons_flux, &
        update_state, &
        calc_timestep, set_global_timestep, advance_explicit, &
        exchange_messages, set_stage, check_nan, bats_check, bridge_handle, &
@@ -159,6 +164,23 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: DoResChangeOnly, iBlock
      end function bats_calc_face_flux
+     integer(c_int) function bats_save_cons_flux(Handle, iBlock) &
+          bind(C, name="bats_save_cons_flux")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock
+     end function bats_save_cons_flux
+     integer(c_int) function bats_message_pass_face(Handle) &
+          bind(C, name="bats_message_pass_face")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+     end function bats_message_pass_face
+     integer(c_int) function bats_apply_cons_flux(Handle, iBlock) &
+          bind(C, name="bats_apply_cons_flux")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: iBlock
+     end function bats_apply_cons_flux
      integer(c_int) function bats_calc_source(Handle, iBlock) &
           bind(C, name="bats_calc_source")
        import:: c_int, c_ptr
@@ -512,6 +534,26 @@ contains
     call bats_check(bats_calc_face_flux(hBats, merge(1, 0, DoResChangeOnly), &
          iBlock), 'calc_face_flux')
   end subroutine calc_face_flux
+  !============================================================================
+  subroutine save_cons_flux(iBlock)
+    ! fine-side face fluxes at resolution changes (ModConserveFlux)
+    integer, intent(in):: iBlock
+    !--------------------------------------------------------------------------
+    call bats_check(bats_save_cons_flux(hBats, iBlock), 'save_cons_flux')
+  end subroutine save_cons_flux
+  !============================================================================
+  subroutine message_pass_face
+    ! the stored fine-face records already sit at the coarse sides
+    !--------------------------------------------------------------------------
+    call bats_check(bats_message_pass_face(hBats), 'message_pass_face')
+  end subroutine message_pass_face
+  !============================================================================
+  subroutine apply_cons_flux(iBlock)
+    ! coarse-side faces take the sums of the fine face fluxes
+    integer, intent(in):: iBlock
+    !--------------------------------------------------------------------------
+    call bats_check(bats_apply_cons_flux(hBats, iBlock), 'apply_cons_flux')
+  end subroutine apply_cons_flux
   !============================================================================
   subroutine calc_source(iBlock)
     integer, intent(in):: iBlock

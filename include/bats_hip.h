@@ -229,8 +229,14 @@ int bats_download_state_soa(bats_handle *h, double *U);
 /* ------------------------------------------------------------------------
  * Per-block routines with the reference's names and arguments.  They work
  * on one block at a time through one set of face/flux buffers, like the
- * reference's threadprivate arrays, and exist for per-stage parity dumps.
- * Ghost values come from the snapshot taken by bats_exchange_messages. */
+ * reference's threadprivate arrays, and exist for per-stage parity dumps
+ * and hosts that keep the reference's per-block loop.  Ghost values come
+ * from the snapshot taken by bats_exchange_messages.  On a two-level grid
+ * the loop is the reference's (src/ModAdvanceExplicit.f90:93-199): for the
+ * blocks with a coarser neighbour calc_face_value(iBlock, 1, 1),
+ * calc_face_flux(1, iBlock) and save_cons_flux(iBlock); then
+ * message_pass_face; then for every block calc_face_value, calc_face_flux,
+ * apply_cons_flux, calc_source, update_state (and calc_timestep). */
 
 /* exchange_messages (src/ModMessagePass.f90:27-290): snapshot the current
  * state as the halo source of the next stage. */
@@ -238,13 +244,32 @@ int bats_exchange_messages(bats_handle *h);
 /* iStage (ModMain) for the per-block routines */
 int bats_set_stage(bats_handle *h, int iStage);
 /* calc_face_value(iBlock, DoResChangeOnly, DoMonotoneRestrict)
- * (src/ModFaceValue.f90:854).  DoResChangeOnly must be 0 on a same-level
- * grid (no resolution changes). */
+ * (src/ModFaceValue.f90:854).  With DoResChangeOnly only the faces of the
+ * block's sides with a coarser neighbour are set (a no-op on a same-level
+ * grid).  The ghost cells of resolution changes are the mean of the 2x2x2
+ * fine cells or the limited prolongation of the coarse cell (set by
+ * bats_exchange_messages); DoMonotoneRestrict is not read. */
 int bats_calc_face_value(bats_handle *h, int iBlock, int DoResChangeOnly,
                          int DoMonotoneRestrict);
 /* calc_face_flux(DoResChangeOnly, iBlock) -- reference argument order
- * (src/ModFaceFlux.f90:483). */
+ * (src/ModFaceFlux.f90:483); DoResChangeOnly as in calc_face_value. */
 int bats_calc_face_flux(bats_handle *h, int DoResChangeOnly, int iBlock);
+/* save_cons_flux(iBlock) (src/ModConserveFlux.f90:79-187): the face fluxes
+ * (and Vdt) of the block's sides with a coarser neighbour, and the normal
+ * field of their left and right states times 1/4, stored for the coarse
+ * neighbour's face; a no-op for other blocks and on same-level grids. */
+int bats_save_cons_flux(bats_handle *h, int iBlock);
+/* message_pass_face(nCorrectedFaceValues, CorrectedFlux_V[XYZ]B) as
+ * advance_explicit calls it (src/ModAdvanceExplicit.f90:127-128): the
+ * stored sub-face records already sit at the coarse side (one device, one
+ * stream), so nothing moves; kept for the reference's call sequence. */
+int bats_message_pass_face(bats_handle *h);
+/* apply_cons_flux(iBlock) (src/ModConserveFlux.f90:260-441): the faces of
+ * the block's sides with finer neighbours take the sum of the four fine
+ * sub-face fluxes and Vdt, and their face states the sums of the fine
+ * normal field (read by calc_source).  Call after calc_face_flux(0,
+ * iBlock). */
+int bats_apply_cons_flux(bats_handle *h, int iBlock);
 /* calc_source(iBlock) (src/ModCalcSource.f90:245-1025): the 8-wave div B
  * source from the last calc_face_value block's face states; zero unless
  * UseDivbSource.  Call it between calc_face_flux and update_state, as
