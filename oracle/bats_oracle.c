@@ -482,11 +482,15 @@ static const double kNormal[3][3] = {{1.0, 0.0, 0.0},
                                      {0.0, 1.0, 0.0},
                                      {0.0, 0.0, 1.0}};
 
-void ob_physical_flux(const ob_config *c, int iDim, const double *State_V,
-                      double *StateCons_V, double *Flux_V, double *UnOut) {
+void ob_physical_flux_b0(const ob_config *c, int iDim, const double *State_V,
+                         const double *B0_D, double *StateCons_V,
+                         double *Flux_V, double *UnOut) {
   const double NormalX = kNormal[iDim][0], NormalY = kNormal[iDim][1],
                NormalZ = kNormal[iDim][2];
-  const double B0x = 0.0, B0y = 0.0, B0z = 0.0;
+  /* the face's B0 (B0x, B0y, B0z of get_physical_flux, set from B0_DX/Y/Z
+   * of the face, src/ModFaceFlux.f90:580-590) */
+  const double B0x = B0_D ? B0_D[0] : 0.0, B0y = B0_D ? B0_D[1] : 0.0,
+               B0z = B0_D ? B0_D[2] : 0.0;
   const double InvGammaMinus1 = 1.0 / (c->Gamma - 1.0);
   double Bx, By, Bz, FullBx, FullBy, FullBz, Bn, B0n, FullBn;
   double Rho, Ux, Uy, Uz, p, Un, pPerp, e, pExtra, B2, B0B1, pTotal;
@@ -544,13 +548,18 @@ void ob_physical_flux(const ob_config *c, int iDim, const double *State_V,
                       Flux_V[OB_BY] * By + Flux_V[OB_BZ] * Bz;
   *UnOut = Un;
 }
+void ob_physical_flux(const ob_config *c, int iDim, const double *State_V,
+                      double *StateCons_V, double *Flux_V, double *UnOut) {
+  ob_physical_flux_b0(c, iDim, State_V, NULL, StateCons_V, Flux_V, UnOut);
+}
 
 /* ---------------------------------------------------------------------- */
 /* get_speed_max -> get_mhd_speed                                          */
 /* src/ModFaceFlux.f90:3545-3722, 3853-4252 (ideal MHD, no Hall, no B0)    */
 /* ---------------------------------------------------------------------- */
-int ob_speed_max(const ob_config *c, int iDim, const double *State_V,
-                 double *Cmax, double *Cleft, double *Cright) {
+int ob_speed_max_b0(const ob_config *c, int iDim, const double *State_V,
+                    const double *B0_D, double *Cmax, double *Cleft,
+                    double *Cright) {
   const double *Normal = kNormal[iDim];
   double Rho, Sound2, Un, UnMin, UnMax, InvRho, FullBx, FullBy, FullBz;
   double Alfven2, FullBn, Alfven2Normal, Fast2, Discr, Fast;
@@ -560,9 +569,10 @@ int ob_speed_max(const ob_config *c, int iDim, const double *State_V,
        State_V[OB_RHOUZ] * Normal[2];                          /* :3903 */
   UnMin = Un; UnMax = Un;
   InvRho = 1.0 / Rho;                                          /* :3921 */
-  FullBx = State_V[OB_BX] + 0.0;
-  FullBy = State_V[OB_BY] + 0.0;
-  FullBz = State_V[OB_BZ] + 0.0;
+  /* full field B1 + B0 of the face (:3912-3920) */
+  FullBx = State_V[OB_BX] + (B0_D ? B0_D[0] : 0.0);
+  FullBy = State_V[OB_BY] + (B0_D ? B0_D[1] : 0.0);
+  FullBz = State_V[OB_BZ] + (B0_D ? B0_D[2] : 0.0);
   Alfven2 = (FullBx * FullBx + FullBy * FullBy + FullBz * FullBz) * InvRho;
   FullBn = Normal[0] * FullBx + Normal[1] * FullBy + Normal[2] * FullBz;
   Alfven2Normal = InvRho * (FullBn * FullBn);
@@ -575,21 +585,28 @@ int ob_speed_max(const ob_config *c, int iDim, const double *State_V,
   if (Cright) *Cright = UnMax + Fast;                          /* :4248 */
   return OB_OK;
 }
+int ob_speed_max(const ob_config *c, int iDim, const double *State_V,
+                 double *Cmax, double *Cleft, double *Cright) {
+  return ob_speed_max_b0(c, iDim, State_V, NULL, Cmax, Cleft, Cright);
+}
 
 /* ---------------------------------------------------------------------- */
 /* hlld_flux -- src/ModFaceFlux.f90:2826-3160 (B0 = 0, no resistivity)     */
 /* StateLeft/Right are the Linde-corrected face states.                    */
 /* ---------------------------------------------------------------------- */
 static int ob_hlld(const ob_config *c, int iDim, const double *StateLeft_V,
-                   const double *StateRight_V, double *Flux_V, double *Cmax,
-                   double *CmaxDt, double *Unormal) {
+                   const double *StateRight_V, const double *B0_D,
+                   double *Flux_V, double *Cmax, double *CmaxDt,
+                   double *Unormal) {
   const double GammaMinus1 = c->Gamma - 1.0;
   const double InvGammaMinus1 = 1.0 / (c->Gamma - 1.0);
   const double cTiny = c->cTiny;
   /* rotation for a Cartesian face: (n, t1, t2) = (x,y,z),(y,z,x),(z,x,y)
    * (rotate_state_vectors :4493-4552) */
   const int in = iDim, it1 = (iDim + 1) % 3, it2 = (iDim + 2) % 3;
-  const double B0n = 0.0, B0t1 = 0.0, B0t2 = 0.0;
+  /* the face's B0 in the rotated frame (rotate_state_vectors :4528-4540) */
+  const double B0n = B0_D ? B0_D[in] : 0.0, B0t1 = B0_D ? B0_D[it1] : 0.0,
+               B0t2 = B0_D ? B0_D[it2] : 0.0;
   double CleftStateLeft, CrightStateLeft, CleftStateRight, CrightStateRight;
   double sL, sR;
   double RhoL, pL, RhoR, pR, UnL, Ut1L, Ut2L, B1nL, B1t1L, B1t2L;
@@ -604,11 +621,11 @@ static int ob_hlld(const ob_config *c, int iDim, const double *StateLeft_V,
   double FluxRot[6];
   int ierr;
 
-  ierr = ob_speed_max(c, iDim, StateLeft_V, NULL, &CleftStateLeft,
-                      &CrightStateLeft);
+  ierr = ob_speed_max_b0(c, iDim, StateLeft_V, B0_D, NULL, &CleftStateLeft,
+                         &CrightStateLeft);
   if (ierr) return ierr;
-  ierr = ob_speed_max(c, iDim, StateRight_V, NULL, &CleftStateRight,
-                      &CrightStateRight);
+  ierr = ob_speed_max_b0(c, iDim, StateRight_V, B0_D, NULL, &CleftStateRight,
+                         &CrightStateRight);
   if (ierr) return ierr;
   sL = fmin2(CleftStateLeft, CleftStateRight);                 /* :2868 */
   sR = fmax2(CrightStateLeft, CrightStateRight);               /* :2869 */
@@ -617,12 +634,12 @@ static int ob_hlld(const ob_config *c, int iDim, const double *StateLeft_V,
 
   if (sL >= 0.) { /* :2880-2886 */
     double Cons[OB_NFLUX];
-    ob_physical_flux(c, iDim, StateLeft_V, Cons, Flux_V, Unormal);
+    ob_physical_flux_b0(c, iDim, StateLeft_V, B0_D, Cons, Flux_V, Unormal);
     return OB_OK;
   }
   if (sR <= 0.) { /* :2888-2893 */
     double Cons[OB_NFLUX];
-    ob_physical_flux(c, iDim, StateRight_V, Cons, Flux_V, Unormal);
+    ob_physical_flux_b0(c, iDim, StateRight_V, B0_D, Cons, Flux_V, Unormal);
     return OB_OK;
   }
 
@@ -781,9 +798,10 @@ static int ob_hlld(const ob_config *c, int iDim, const double *StateLeft_V,
 /* get_numerical_flux -- src/ModFaceFlux.f90:2130-2473 (ideal MHD subset)  */
 /* + the per-face epilogue of get_flux_x (:651-658)                        */
 /* ---------------------------------------------------------------------- */
-int ob_numerical_flux(const ob_config *c, int iDim, double Area,
-                      const double *StateLeftIn_V,
-                      const double *StateRightIn_V, double *Flux_V) {
+int ob_numerical_flux_b0(const ob_config *c, int iDim, double Area,
+                         const double *StateLeftIn_V,
+                         const double *StateRightIn_V, const double *B0_D,
+                         double *Flux_V) {
   const double *Normal_D = kNormal[iDim];
   double StateLeft_V[NV], StateRight_V[NV], State_V[NV];
   double StateLeftCons_V[OB_NFLUX], StateRightCons_V[OB_NFLUX];
@@ -820,16 +838,16 @@ int ob_numerical_flux(const ob_config *c, int iDim, double Area,
     State_V[v] = 0.5 * (StateLeft_V[v] + StateRight_V[v]); /* :2326 */
 
   if (c->iFlux == OB_FLUX_RUSANOV || c->iFlux == OB_FLUX_LINDE) {
-    ob_physical_flux(c, iDim, StateLeft_V, StateLeftCons_V, FluxLeft_V,
-                     &UnLeft);
-    ob_physical_flux(c, iDim, StateRight_V, StateRightCons_V, FluxRight_V,
-                     &UnRight);
+    ob_physical_flux_b0(c, iDim, StateLeft_V, B0_D, StateLeftCons_V,
+                        FluxLeft_V, &UnLeft);
+    ob_physical_flux_b0(c, iDim, StateRight_V, B0_D, StateRightCons_V,
+                        FluxRight_V, &UnRight);
   }
 
   CmaxDt = 0.0; /* :2360 */
   if (c->iFlux == OB_FLUX_RUSANOV) {
     /* lax_friedrichs_flux :2513-2559 */
-    ierr = ob_speed_max(c, iDim, State_V, &Cmax, NULL, NULL);
+    ierr = ob_speed_max_b0(c, iDim, State_V, B0_D, &Cmax, NULL, NULL);
     if (ierr) return ierr;
     CmaxDt = fmax2(CmaxDt, Cmax); /* get_speed_max :3697 */
     for (v = 0; v < OB_NFLUX; v++)
@@ -840,12 +858,13 @@ int ob_numerical_flux(const ob_config *c, int iDim, double Area,
     /* harten_lax_vanleer_flux :2561-2645 */
     double CleftHat, CrightHat, CleftL, CrightR, Cleft, Cright;
     double WeightLeft, WeightRight, Diffusion;
-    ierr = ob_speed_max(c, iDim, State_V, &Cmax, &CleftHat, &CrightHat);
+    ierr = ob_speed_max_b0(c, iDim, State_V, B0_D, &Cmax, &CleftHat,
+                           &CrightHat);
     if (ierr) return ierr;
     CmaxDt = fmax2(CmaxDt, Cmax);
-    ierr = ob_speed_max(c, iDim, StateLeft_V, NULL, &CleftL, NULL);
+    ierr = ob_speed_max_b0(c, iDim, StateLeft_V, B0_D, NULL, &CleftL, NULL);
     if (ierr) return ierr;
-    ierr = ob_speed_max(c, iDim, StateRight_V, NULL, NULL, &CrightR);
+    ierr = ob_speed_max_b0(c, iDim, StateRight_V, B0_D, NULL, NULL, &CrightR);
     if (ierr) return ierr;
     Cleft = fmin2(fmin2(0.0, CleftL), CleftHat);
     Cright = fmax2(fmax2(0.0, CrightR), CrightHat);
@@ -857,8 +876,8 @@ int ob_numerical_flux(const ob_config *c, int iDim, double Area,
                    Diffusion * (StateRightCons_V[v] - StateLeftCons_V[v]));
     Unormal = WeightRight * UnRight + WeightLeft * UnLeft;
   } else {
-    ierr = ob_hlld(c, iDim, StateLeft_V, StateRight_V, Flux_V, &Cmax, &CmaxDt,
-                   &Unormal);
+    ierr = ob_hlld(c, iDim, StateLeft_V, StateRight_V, B0_D, Flux_V, &Cmax,
+                   &CmaxDt, &Unormal);
     if (ierr) return ierr;
   }
 
@@ -874,14 +893,22 @@ int ob_numerical_flux(const ob_config *c, int iDim, double Area,
   Flux_V[OB_UNFIRST] = Unormal * Area;  /* get_flux_x :658 */
   return OB_OK;
 }
+int ob_numerical_flux(const ob_config *c, int iDim, double Area,
+                      const double *StateLeftIn_V,
+                      const double *StateRightIn_V, double *Flux_V) {
+  return ob_numerical_flux_b0(c, iDim, Area, StateLeftIn_V, StateRightIn_V,
+                              NULL, Flux_V);
+}
 
 /* ---------------------------------------------------------------------- */
 /* calc_face_flux -- src/ModFaceFlux.f90:483-977 (Cartesian, no body)      */
 /* ---------------------------------------------------------------------- */
-int ob_calc_face_flux(const ob_config *c, const double *CellSize,
-                      double *LeftX, double *RightX, double *LeftY,
-                      double *RightY, double *LeftZ, double *RightZ,
-                      double *FluxX, double *FluxY, double *FluxZ) {
+int ob_calc_face_flux_b0(const ob_config *c, const double *CellSize,
+                         double *LeftX, double *RightX, double *LeftY,
+                         double *RightY, double *LeftZ, double *RightZ,
+                         const double *B0X, const double *B0Y,
+                         const double *B0Z, double *FluxX, double *FluxY,
+                         double *FluxZ) {
   const int nI = c->nI, nJ = c->nJ, nK = c->nK;
   /* CellFace_DB (BATL, absent): face area = product of the two transverse
    * cell sizes */
@@ -893,8 +920,9 @@ int ob_calc_face_flux(const ob_config *c, const double *CellSize,
     for (j = 1; j <= nJ; j++)
       for (i = 1; i <= nI + 1; i++) {
         long f = (((long)(k - 1) * nJ + (j - 1)) * (nI + 1) + (i - 1));
-        ierr = ob_numerical_flux(c, 0, AreaX, LeftX + f * NV, RightX + f * NV,
-                                 FluxX + f * OB_NFACEVALUE);
+        ierr = ob_numerical_flux_b0(c, 0, AreaX, LeftX + f * NV,
+                                    RightX + f * NV, B0X ? B0X + 3 * f : NULL,
+                                    FluxX + f * OB_NFACEVALUE);
         if (ierr && !first) first = ierr;
       }
   if (c->nDim >= 2)
@@ -902,8 +930,10 @@ int ob_calc_face_flux(const ob_config *c, const double *CellSize,
       for (j = 1; j <= nJ + 1; j++)
         for (i = 1; i <= nI; i++) {
           long f = (((long)(k - 1) * (nJ + 1) + (j - 1)) * nI + (i - 1));
-          ierr = ob_numerical_flux(c, 1, AreaY, LeftY + f * NV,
-                                   RightY + f * NV, FluxY + f * OB_NFACEVALUE);
+          ierr = ob_numerical_flux_b0(c, 1, AreaY, LeftY + f * NV,
+                                      RightY + f * NV,
+                                      B0Y ? B0Y + 3 * f : NULL,
+                                      FluxY + f * OB_NFACEVALUE);
           if (ierr && !first) first = ierr;
         }
   if (c->nDim >= 3)
@@ -911,11 +941,85 @@ int ob_calc_face_flux(const ob_config *c, const double *CellSize,
       for (j = 1; j <= nJ; j++)
         for (i = 1; i <= nI; i++) {
           long f = (((long)(k - 1) * nJ + (j - 1)) * nI + (i - 1));
-          ierr = ob_numerical_flux(c, 2, AreaZ, LeftZ + f * NV,
-                                   RightZ + f * NV, FluxZ + f * OB_NFACEVALUE);
+          ierr = ob_numerical_flux_b0(c, 2, AreaZ, LeftZ + f * NV,
+                                      RightZ + f * NV,
+                                      B0Z ? B0Z + 3 * f : NULL,
+                                      FluxZ + f * OB_NFACEVALUE);
           if (ierr && !first) first = ierr;
         }
   return first;
+}
+int ob_calc_face_flux(const ob_config *c, const double *CellSize,
+                      double *LeftX, double *RightX, double *LeftY,
+                      double *RightY, double *LeftZ, double *RightZ,
+                      double *FluxX, double *FluxY, double *FluxZ) {
+  return ob_calc_face_flux_b0(c, CellSize, LeftX, RightX, LeftY, RightY,
+                              LeftZ, RightZ, NULL, NULL, NULL, FluxX, FluxY,
+                              FluxZ);
+}
+
+/* ---------------------------------------------------------------------- */
+/* B0 of the faces and the B0 source terms -- src/ModB0.f90               */
+/* ---------------------------------------------------------------------- */
+/* set_b0_face (:314-336, no constrained transport): B0_DX(3, nI+1, nJ, nK)
+ * = mean of the cell-centred B0 of the face's two cells, likewise B0_DY,
+ * B0_DZ; B0Blk is the block's B0_DGB(3, MinI:MaxI, MinJ:MaxJ, MinK:MaxK)
+ * with its ghost cells. */
+void ob_set_b0_face(const ob_config *c, const double *B0Blk, double *B0X,
+                    double *B0Y, double *B0Z) {
+  ob_shape s = shape_of(c);
+  const int nI = s.nI, nJ = s.nJ, nK = s.nK;
+  int i, j, k, d;
+#define B0AT(ii, jj, kk) (B0Blk + cidx(&s, ii, jj, kk) / NV * 3)
+  for (k = 1; k <= nK; k++)
+    for (j = 1; j <= nJ; j++)
+      for (i = 1; i <= nI + 1; i++) {
+        const long f = ((long)(k - 1) * nJ + (j - 1)) * (nI + 1) + (i - 1);
+        for (d = 0; d < 3; d++)
+          B0X[3 * f + d] = 0.5 * (B0AT(i - 1, j, k)[d] + B0AT(i, j, k)[d]);
+      }
+  for (k = 1; k <= nK; k++)
+    for (j = 1; j <= nJ + 1; j++)
+      for (i = 1; i <= nI; i++) {
+        const long f = ((long)(k - 1) * (nJ + 1) + (j - 1)) * nI + (i - 1);
+        for (d = 0; d < 3; d++)
+          B0Y[3 * f + d] = 0.5 * (B0AT(i, j - 1, k)[d] + B0AT(i, j, k)[d]);
+      }
+  for (k = 1; k <= nK + 1; k++)
+    for (j = 1; j <= nJ; j++)
+      for (i = 1; i <= nI; i++) {
+        const long f = ((long)(k - 1) * nJ + (j - 1)) * nI + (i - 1);
+        for (d = 0; d < 3; d++)
+          B0Z[3 * f + d] = 0.5 * (B0AT(i, j, k - 1)[d] + B0AT(i, j, k)[d]);
+      }
+#undef B0AT
+}
+
+/* set_b0_source (:545-590, Cartesian 3-D): DivB0_C(nI, nJ, nK) and
+ * CurlB0_DC(3, nI, nJ, nK) from the face B0 */
+void ob_set_b0_source(const ob_config *c, const double *CellSize,
+                      const double *B0X, const double *B0Y, const double *B0Z,
+                      double *DivB0, double *CurlB0) {
+  const int nI = c->nI, nJ = c->nJ, nK = c->nK;
+  const double DxInv = 1 / CellSize[0], DyInv = 1 / CellSize[1],
+               DzInv = 1 / CellSize[2];
+  int i, j, k;
+  for (k = 1; k <= nK; k++)
+    for (j = 1; j <= nJ; j++)
+      for (i = 1; i <= nI; i++) {
+        const long c0 = ((long)(k - 1) * nJ + (j - 1)) * nI + (i - 1);
+        const double *X0 = B0X + 3 * (((long)(k - 1) * nJ + (j - 1)) * (nI + 1) + (i - 1));
+        const double *X1 = X0 + 3;
+        const double *Y0 = B0Y + 3 * (((long)(k - 1) * (nJ + 1) + (j - 1)) * nI + (i - 1));
+        const double *Y1 = Y0 + 3 * nI;
+        const double *Z0 = B0Z + 3 * c0;
+        const double *Z1 = Z0 + 3L * nI * nJ;
+        DivB0[c0] = DxInv * (X1[0] - X0[0]) + DyInv * (Y1[1] - Y0[1]) +
+                    DzInv * (Z1[2] - Z0[2]);
+        CurlB0[3 * c0 + 0] = DyInv * (Y1[2] - Y0[2]) - DzInv * (Z1[1] - Z0[1]);
+        CurlB0[3 * c0 + 1] = DzInv * (Z1[0] - Z0[0]) - DxInv * (X1[2] - X0[2]);
+        CurlB0[3 * c0 + 2] = DxInv * (X1[1] - X0[1]) - DyInv * (Y1[0] - Y0[0]);
+      }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -952,11 +1056,15 @@ static inline void limit_pressure_cell(const ob_config *c, double *S) {
 /* (Powell) div B source on a Cartesian grid without B0 (calc_divb_source  */
 /* :1694-1889 with DoCorrectFace false, then :908-951 and :1016-1022).     */
 /* ---------------------------------------------------------------------- */
-void ob_calc_source(const ob_config *c, const double *CellSize,
-                    const double *StateBlk, const double *LeftX,
-                    const double *RightX, const double *LeftY,
-                    const double *RightY, const double *LeftZ,
-                    const double *RightZ, double *Source_VC) {
+void ob_calc_source_b0(const ob_config *c, const double *CellSize,
+                       const double *StateBlk, const double *LeftX,
+                       const double *RightX, const double *LeftY,
+                       const double *RightY, const double *LeftZ,
+                       const double *RightZ, const double *B0Blk,
+                       const double *B0X, const double *B0Y,
+                       const double *B0Z, const double *DivB0,
+                       const double *CurlB0, int UseB0Source,
+                       double *Source_VC) {
   ob_shape s = shape_of(c);
   const int nI = s.nI, nJ = s.nJ, nK = s.nK;
   const long nCell = (long)nI * nJ * nK;
@@ -1005,6 +1113,24 @@ void ob_calc_source(const ob_config *c, const double *CellSize,
                            2 * DzInvHalf * (LeftZ[fz1 * NV + OB_BZ] -
                                             RightZ[fz * NV + OB_BZ]);
           }
+          /* with B0 (UseMhdMomentumFlux, orig update, UseDivFullBSource
+           * false): the face B0 times the face's normal-field jump
+           * (:1792-1808), then the cell B0 times the internal divergence
+           * (:1880-1885) */
+          double SourceMhdB0[3] = {0, 0, 0};
+          if (B0Blk) {
+            const long fy = ((long)(k - 1) * (nJ + 1) + (j - 1)) * nI + (i - 1);
+            const double *X0 = B0X + 3 * fx, *X1 = X0 + 3;
+            const double *Y0 = B0Y + 3 * fy, *Y1 = Y0 + 3L * nI;
+            const double *Z0 = B0Z + 3 * c0, *Z1 = Z0 + 3L * nI * nJ;
+            const double *B0c = B0Blk + cidx(&s, i, j, k) / NV * 3;
+            for (v = 0; v < 3; v++) {
+              double t = SourceMhdB0[v] - X0[v] * dB1nFace1 - X1[v] * dB1nFace2;
+              t = t - Y0[v] * dB1nFace3 - Y1[v] * dB1nFace4;
+              t = t - Z0[v] * dB1nFace5 - Z1[v] * dB1nFace6;
+              SourceMhdB0[v] = t - DivBInternal * B0c[v];
+            }
+          }
           /* :1865-1873 */
           DivB1 = DivBInternal + dB1nFace1 + dB1nFace2;
           if (c->nDim >= 2) DivB1 = DivB1 + dB1nFace3 + dB1nFace4;
@@ -1019,17 +1145,35 @@ void ob_calc_source(const ob_config *c, const double *CellSize,
             for (v = 0; v < 3; v++)
               Src[OB_BX + v] = Src[OB_BX + v] - DivB1 * u_D[v];
             for (v = 0; v < 3; v++)
-              SourceMhd[v] = 0 - DivB1 * St[OB_BX + v];
+              SourceMhd[v] = SourceMhdB0[v] - DivB1 * St[OB_BX + v];
             Src[OB_ENERGY] =
                 Src[OB_ENERGY] - DivB1 * (St[OB_BX] * u_D[0] +
                                           St[OB_BY] * u_D[1] +
                                           St[OB_BZ] * u_D[2]);
+            if (B0Blk && UseB0Source) {
+              /* -B1 div(B0) - curl(B0) x B1 (:953-967) */
+              const double *cB = CurlB0 + 3 * c0;
+              const double Bx = St[OB_BX], By = St[OB_BY], Bz = St[OB_BZ];
+              SourceMhd[0] = SourceMhd[0] - Bx * DivB0[c0] - (cB[1] * Bz - cB[2] * By);
+              SourceMhd[1] = SourceMhd[1] - By * DivB0[c0] - (cB[2] * Bx - cB[0] * Bz);
+              SourceMhd[2] = SourceMhd[2] - Bz * DivB0[c0] - (cB[0] * By - cB[1] * Bx);
+            }
             /* :1016-1022 */
             for (v = 0; v < 3; v++)
               Src[OB_RHOUX + v] = Src[OB_RHOUX + v] + SourceMhd[v];
           }
         }
   }
+}
+
+void ob_calc_source(const ob_config *c, const double *CellSize,
+                    const double *StateBlk, const double *LeftX,
+                    const double *RightX, const double *LeftY,
+                    const double *RightY, const double *LeftZ,
+                    const double *RightZ, double *Source_VC) {
+  ob_calc_source_b0(c, CellSize, StateBlk, LeftX, RightX, LeftY, RightY,
+                    LeftZ, RightZ, NULL, NULL, NULL, NULL, NULL, NULL, 0,
+                    Source_VC);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1502,6 +1646,8 @@ void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit) {
 /* ---------------------------------------------------------------------- */
 typedef struct {
   double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ, *Src;
+  /* B0 of the block's faces and its div / curl (B0 splitting only) */
+  double *B0X, *B0Y, *B0Z, *DivB0, *CurlB0;
 } ob_scratch;
 
 static void scratch_alloc(const ob_config *c, ob_scratch *w) {
@@ -1515,10 +1661,30 @@ static void scratch_alloc(const ob_config *c, ob_scratch *w) {
   w->FY = (double *)calloc(nfy * OB_NFACEVALUE, 8);
   w->FZ = (double *)calloc(nfz * OB_NFACEVALUE, 8);
   w->Src = (double *)calloc((long)c->nI * c->nJ * c->nK * OB_NFLUX, 8);
+  w->B0X = (double *)calloc(nfx * 3, 8);
+  w->B0Y = (double *)calloc(nfy * 3, 8);
+  w->B0Z = (double *)calloc(nfz * 3, 8);
+  w->DivB0 = (double *)calloc((long)c->nI * c->nJ * c->nK, 8);
+  w->CurlB0 = (double *)calloc((long)c->nI * c->nJ * c->nK * 3, 8);
 }
 static void scratch_free(ob_scratch *w) {
   free(w->LX); free(w->RX); free(w->LY); free(w->RY); free(w->LZ);
   free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
+  free(w->B0X); free(w->B0Y); free(w->B0Z); free(w->DivB0); free(w->CurlB0);
+}
+
+/* B0 of block b's faces and its div / curl into the scratch (set_b0_face
+ * and set_b0_source for the block, src/ModAdvanceExplicit.f90:147,
+ * src/ModCalcSource.f90:262); returns the block's ghosted B0 or NULL */
+static const double *block_b0(const ob_session *s, int b, ob_scratch *w) {
+  if (!s->B0) return NULL;
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const double *B0Blk = s->B0 + (long)b * (block_len(&sh) / NV * 3);
+  ob_set_b0_face(c, B0Blk, w->B0X, w->B0Y, w->B0Z);
+  ob_set_b0_source(c, s->CellSize_DB + 3L * b, w->B0X, w->B0Y, w->B0Z,
+                   w->DivB0, w->CurlB0);
+  return B0Blk;
 }
 
 /* DiLevel_EB /= 0 on each side of block b: an outer boundary (-1) or a
@@ -1637,8 +1803,10 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       block_res_change(s, b, rc);
       ob_calc_face_value_rc(c, Blk, CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY,
                             w.LZ, w.RZ);
-      e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
-                            w.FY, w.FZ);
+      const double *B0Blk = block_b0(s, b, &w);
+      e = ob_calc_face_flux_b0(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
+                               B0Blk ? w.B0X : NULL, B0Blk ? w.B0Y : NULL,
+                               B0Blk ? w.B0Z : NULL, w.FX, w.FY, w.FZ);
       if (s->nSlab > 0)
         apply_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
                         w.RZ);
@@ -1656,8 +1824,9 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       if (DoUpdate) {
         /* calc_source (src/ModAdvanceExplicit.f90:166) */
         if (c->UseDivbSource)
-          ob_calc_source(c, cs, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
-                         w.Src);
+          ob_calc_source_b0(c, cs, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
+                            B0Blk, w.B0X, w.B0Y, w.B0Z, w.DivB0, w.CurlB0,
+                            s->UseB0Source, w.Src);
         ob_update_state(c, iStage, s->Dt, cs, s->DtMax_CB + nCell * b, w.FX,
                         w.FY, w.FZ, c->UseDivbSource ? w.Src : NULL, Blk,
                         Old);
@@ -1708,13 +1877,17 @@ int ob_session_sources(ob_session *s, double *Source) {
     block_res_change(s, b, rc);
     ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w.LX,
                           w.RX, w.LY, w.RY, w.LZ, w.RZ);
-    err |= ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.FX,
-                             w.FY, w.FZ);
+    const double *B0Blk = block_b0(s, b, &w);
+    err |= ob_calc_face_flux_b0(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
+                                B0Blk ? w.B0X : NULL, B0Blk ? w.B0Y : NULL,
+                                B0Blk ? w.B0Z : NULL, w.FX, w.FY, w.FZ);
     if (s->nSlab > 0)
       apply_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
                       w.RZ);
-    ob_calc_source(c, cs, s->State + (long)b * len, w.LX, w.RX, w.LY, w.RY,
-                   w.LZ, w.RZ, Source + (long)b * nCell * OB_NFLUX);
+    ob_calc_source_b0(c, cs, s->State + (long)b * len, w.LX, w.RX, w.LY,
+                      w.RY, w.LZ, w.RZ, B0Blk, w.B0X, w.B0Y, w.B0Z, w.DivB0,
+                      w.CurlB0, s->UseB0Source,
+                      Source + (long)b * nCell * OB_NFLUX);
   }
   scratch_free(&w);
   return err;

@@ -105,6 +105,43 @@ typedef struct {
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */
+/* variants with the face's B0 (B0_D[3], NULL: none) for B0 splitting */
+void ob_physical_flux_b0(const ob_config *c, int iDim, const double *State_V,
+                         const double *B0_D, double *StateCons_V,
+                         double *Flux_V, double *UnOut);
+int ob_speed_max_b0(const ob_config *c, int iDim, const double *State_V,
+                    const double *B0_D, double *Cmax, double *Cleft,
+                    double *Cright);
+int ob_numerical_flux_b0(const ob_config *c, int iDim, double Area,
+                         const double *StateLeft_V,
+                         const double *StateRight_V, const double *B0_D,
+                         double *Flux_V);
+int ob_calc_face_flux_b0(const ob_config *c, const double *CellSize,
+                         double *LeftX, double *RightX, double *LeftY,
+                         double *RightY, double *LeftZ, double *RightZ,
+                         const double *B0X, const double *B0Y,
+                         const double *B0Z, double *FluxX, double *FluxY,
+                         double *FluxZ);
+/* set_b0_face (src/ModB0.f90:314-336): face B0 B0_DX(3, nI+1, nJ, nK) etc.
+ * from one block's ghosted B0_DGB(3, ...) */
+void ob_set_b0_face(const ob_config *c, const double *B0Blk, double *B0X,
+                    double *B0Y, double *B0Z);
+/* set_b0_source (src/ModB0.f90:545-590, Cartesian): DivB0_C(nI,nJ,nK),
+ * CurlB0_DC(3,nI,nJ,nK) */
+void ob_set_b0_source(const ob_config *c, const double *CellSize,
+                      const double *B0X, const double *B0Y, const double *B0Z,
+                      double *DivB0, double *CurlB0);
+/* calc_source with B0: the 8-wave source plus the B0 terms
+ * (src/ModCalcSource.f90:1694-1889 with UseB0, :930-967) */
+void ob_calc_source_b0(const ob_config *c, const double *CellSize,
+                       const double *StateBlk, const double *LeftX,
+                       const double *RightX, const double *LeftY,
+                       const double *RightY, const double *LeftZ,
+                       const double *RightZ, const double *B0Blk,
+                       const double *B0X, const double *B0Y,
+                       const double *B0Z, const double *DivB0,
+                       const double *CurlB0, int UseB0Source,
+                       double *Source_VC);
 /* limiter on a 1-D pencil: P holds primitives for l = lMin-2 .. lMax+1 with
  * P[(l - lOff)*8 + v]; slopes are written for l = lMin-1 .. lMax into
  * dL/dR[(l - lOff)*8 + v].  Mirrors src/ModFaceValue.f90:3689-3791. */
@@ -224,6 +261,13 @@ typedef struct {
    * (X, Y, Z); set at stage 1 of each step when the adaptive low order
    * scheme is on (src/ModAdvanceExplicit.f90:87-91), else unused */
   double *LowOrderCrit;
+  /* B0 splitting (UseB0, src/ModB0.f90): the state's field is B1 = B - B0;
+   * B0 is B0_DGB(3, MinI:MaxI, MinJ:MaxJ, MinK:MaxK, nBlock), the static
+   * field at the cell centres and ghost cells (NULL: no B0); UseB0Source
+   * adds -B1 div(B0) - curl(B0) x B1 to the momentum with the 8-wave
+   * source (src/ModCalcSource.f90:953-967) */
+  const double *B0;
+  int UseB0Source;
 } ob_session;
 
 /* set_global_timestep (src/ModTimeStepControl.f90:306-524) */

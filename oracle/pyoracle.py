@@ -55,7 +55,8 @@ class ObSession(C.Structure):
                 ("nStep", C.c_int), ("nThread", C.c_int),
                 ("nSlab", C.c_int), ("iSlab_IS", C.POINTER(C.c_int)),
                 ("Corr", C.POINTER(C.c_double)),
-                ("LowOrderCrit", C.POINTER(C.c_double))]
+                ("LowOrderCrit", C.POINTER(C.c_double)),
+                ("B0", C.POINTER(C.c_double)), ("UseB0Source", C.c_int)]
 
 
 def lib():
@@ -107,6 +108,22 @@ def lib():
         L.ob_init_timestep.restype = C.c_int
         L.ob_exchange_amr.argtypes = [C.POINTER(ObConfig), C.c_int, C.c_int,
                                       C.POINTER(C.c_int), dp]
+        L.ob_physical_flux_b0.argtypes = [C.POINTER(ObConfig), C.c_int, dp,
+                                          dp, dp, dp, dp]
+        L.ob_speed_max_b0.argtypes = [C.POINTER(ObConfig), C.c_int, dp, dp,
+                                      dp, dp, dp]
+        L.ob_speed_max_b0.restype = C.c_int
+        L.ob_numerical_flux_b0.argtypes = [C.POINTER(ObConfig), C.c_int,
+                                           C.c_double, dp, dp, dp, dp]
+        L.ob_numerical_flux_b0.restype = C.c_int
+        L.ob_calc_face_flux_b0.argtypes = [C.POINTER(ObConfig), dp] + \
+            [dp] * 12
+        L.ob_calc_face_flux_b0.restype = C.c_int
+        L.ob_set_b0_face.argtypes = [C.POINTER(ObConfig), dp, dp, dp, dp]
+        L.ob_set_b0_source.argtypes = [C.POINTER(ObConfig), dp, dp, dp, dp,
+                                       dp, dp]
+        L.ob_calc_source_b0.argtypes = [C.POINTER(ObConfig), dp, dp] + \
+            [dp] * 6 + [dp] * 6 + [C.c_int, dp]
         L.ob_pressure_to_energy_cell.argtypes = [C.POINTER(ObConfig), dp]
         L.ob_energy_to_pressure_cell.argtypes = [C.POINTER(ObConfig), dp]
         _LIB = L
@@ -203,6 +220,82 @@ def numerical_flux(cfg, iDim, area, left, right):
     return err, F
 
 
+def physical_flux_b0(cfg, iDim, state, b0):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    b = np.ascontiguousarray(b0, dtype=np.float64)
+    cons = np.zeros(9)
+    flux = np.zeros(9)
+    un = C.c_double()
+    lib().ob_physical_flux_b0(C.byref(cfg), iDim, _dp(s), _dp(b), _dp(cons),
+                              _dp(flux), C.byref(un))
+    return cons, flux, un.value
+
+
+def speed_max_b0(cfg, iDim, state, b0):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    b = np.ascontiguousarray(b0, dtype=np.float64)
+    a, l, r = C.c_double(), C.c_double(), C.c_double()
+    err = lib().ob_speed_max_b0(C.byref(cfg), iDim, _dp(s), _dp(b),
+                                C.byref(a), C.byref(l), C.byref(r))
+    return err, a.value, l.value, r.value
+
+
+def numerical_flux_b0(cfg, iDim, area, left, right, b0):
+    L = np.ascontiguousarray(left, dtype=np.float64)
+    R = np.ascontiguousarray(right, dtype=np.float64)
+    b = np.ascontiguousarray(b0, dtype=np.float64)
+    F = np.zeros(14)
+    err = lib().ob_numerical_flux_b0(C.byref(cfg), iDim, area, _dp(L),
+                                     _dp(R), _dp(b), _dp(F))
+    return err, F
+
+
+def set_b0_face(cfg, b0_block):
+    """Face B0 (B0X (nK, nJ, nI+1, 3), B0Y, B0Z) of one block from its
+    ghosted B0_DGB (nKg, nJg, nIg, 3)."""
+    blk = np.ascontiguousarray(b0_block, dtype=np.float64)
+    out = [np.zeros(sh + (3,)) for sh in face_shapes(cfg)]
+    lib().ob_set_b0_face(C.byref(cfg), _dp(blk), *[_dp(a) for a in out])
+    return out
+
+
+def set_b0_source(cfg, cell_size, b0_faces):
+    """DivB0 (nK, nJ, nI) and CurlB0 (nK, nJ, nI, 3) from face B0."""
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    div = np.zeros((cfg.nK, cfg.nJ, cfg.nI))
+    curl = np.zeros((cfg.nK, cfg.nJ, cfg.nI, 3))
+    lib().ob_set_b0_source(C.byref(cfg), _dp(cs),
+                           *[_dp(np.ascontiguousarray(a)) for a in b0_faces],
+                           _dp(div), _dp(curl))
+    return div, curl
+
+
+def calc_face_flux_b0(cfg, cell_size, faces, b0_faces):
+    sx, sy, sz = face_shapes(cfg)
+    fl = [np.zeros(s + (14,)) for s in (sx, sy, sz)]
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    err = lib().ob_calc_face_flux_b0(
+        C.byref(cfg), _dp(cs), *[_dp(a) for a in faces],
+        *[_dp(np.ascontiguousarray(a)) for a in b0_faces],
+        *[_dp(a) for a in fl])
+    return err, fl
+
+
+def calc_source_b0(cfg, cell_size, block_state, faces, b0_block, b0_faces,
+                   div_b0, curl_b0, UseB0Source=True):
+    cs = np.ascontiguousarray(cell_size, dtype=np.float64)
+    blk = np.ascontiguousarray(block_state, dtype=np.float64)
+    src = np.zeros((cfg.nK, cfg.nJ, cfg.nI, 9))
+    lib().ob_calc_source_b0(
+        C.byref(cfg), _dp(cs), _dp(blk),
+        *[_dp(np.ascontiguousarray(a)) for a in faces],
+        _dp(np.ascontiguousarray(b0_block, dtype=np.float64)),
+        *[_dp(np.ascontiguousarray(a)) for a in b0_faces],
+        _dp(np.ascontiguousarray(div_b0)), _dp(np.ascontiguousarray(curl_b0)),
+        int(UseB0Source), _dp(src))
+    return src
+
+
 def energy_round_trip(cfg, state):
     s = np.array(state, dtype=np.float64)
     lib().ob_pressure_to_energy_cell(C.byref(cfg), _dp(s))
@@ -297,7 +390,10 @@ def calc_timestep(cfg, cell_size, fluxes):
 class Session:
     """Whole-grid oracle run (BATS_advance-shaped time loop)."""
 
-    def __init__(self, grid, cfg, state_vgb, nThread=0):
+    def __init__(self, grid, cfg, state_vgb, nThread=0, B0_DGB=None,
+                 UseB0Source=True):
+        """B0_DGB: B0 splitting, the static field (nBlock, nKg, nJg, nIg, 3)
+        at the cell centres and ghost cells (same-level grids)."""
         self.grid = grid
         self.cfg = cfg
         self.nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
@@ -328,6 +424,12 @@ class Session:
             s.nSlab = self.nSlab
             s.iSlab_IS = self.slabs.ctypes.data_as(C.POINTER(C.c_int))
             s.Corr = _dp(self.Corr)
+        if B0_DGB is not None:
+            assert self.nSlab == 0, "B0 on same-level grids only"
+            self.B0 = np.ascontiguousarray(B0_DGB, dtype=np.float64)
+            assert self.B0.shape[-1] == 3 and self.B0.shape[0] == grid.nBlock
+            s.B0 = _dp(self.B0)
+            s.UseB0Source = int(UseB0Source)
         if cfg.UseAdaptiveLowOrder:
             sx, sy, sz = face_shapes(cfg)
             self.LowOrderCrit = np.zeros(
