@@ -78,6 +78,7 @@ ABI_FUNCTIONS = {
     "bats_set_amr_grid": (C.c_int, [_hp, _ip, _dp, C.c_int, _ip]),
     "bats_set_low_order_face": (C.c_int, [_hp]),
     "bats_set_cell_boundary": (C.c_int, [_hp, _ip, _dp]),
+    "bats_set_b0": (C.c_int, [_hp, _dp, C.c_int]),
     "bats_set_step_graph": (C.c_int, [_hp, C.c_int]),
     "bats_get_low_order_crit": (C.c_int, [_hp, C.c_int, _dp, _dp, _dp]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
@@ -335,6 +336,20 @@ class Session:
         self._check(self.lib.bats_set_cell_boundary(
             self.h, _ptr(t, C.c_int), None if st is None else _ptr(st)),
             "set_cell_boundary")
+
+    def set_b0(self, B0_DGB, UseB0Source=True):
+        """B0 splitting: B0_DGB of shape (nBlock, nKg, nJg, nIg, 3), the
+        static field at the cell centres and ghost cells (None drops it).
+        The state then holds B1 = B - B0."""
+        if B0_DGB is None:
+            self._check(self.lib.bats_set_b0(self.h, None, 0), "set_b0")
+            return
+        b = np.ascontiguousarray(B0_DGB, dtype=np.float64)
+        g = self.grid
+        assert b.shape == (g.nBlock, g.nK + 2 * g.nG, g.nJ + 2 * g.nG,
+                           g.nI + 2 * g.nG, 3), b.shape
+        self._check(self.lib.bats_set_b0(self.h, _ptr(b), int(UseB0Source)),
+                    "set_b0")
 
     def set_step_graph(self, use=True):
         """bats_advance replays one captured hipGraph per time step."""

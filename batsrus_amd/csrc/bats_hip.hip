@@ -105,6 +105,9 @@ struct BlockArgs {
   const int *slabs, *slabKind;
   double *corr;
   int resOnly;
+  // B0 splitting: face B0 and cell B0 / div / curl (null: no B0)
+  const double *b0F, *b0C;
+  int useB0Src;
   const double *lowCrit;                // low order weights or null
   double betaRC;                        // BetaLimiterResChange
   int nFaceRC;                          // its faces (0: not used)
@@ -186,7 +189,15 @@ __global__ void k_block_face_value(BlockArgs a) {
 template <int DIR>
 __device__ int block_flux_dir(const BlockArgs &a, double area, const double *L,
                               const double *R, double *F, double &vdt,
-                              double &unA) {
+                              double &unA, const double *B0) {
+  if (B0) switch (a.iFlux) {
+  case FLUX_RUSANOV:
+    return numerical_flux<DIR, FLUX_RUSANOV, true, true>(a.ph, area, L, R, F, vdt, unA, B0);
+  case FLUX_LINDE:
+    return numerical_flux<DIR, FLUX_LINDE, true, true>(a.ph, area, L, R, F, vdt, unA, B0);
+  default:
+    return numerical_flux<DIR, FLUX_HLLD, true, true>(a.ph, area, L, R, F, vdt, unA, B0);
+  }
   switch (a.iFlux) {
   case FLUX_RUSANOV:
     return numerical_flux<DIR, FLUX_RUSANOV, true>(a.ph, area, L, R, F, vdt, unA);
@@ -217,10 +228,19 @@ __global__ void k_block_face_flux(BlockArgs a) {
   const double *Rs = dir == 0 ? a.RX : (dir == 1 ? a.RY : a.RZ);
   double *Fo = dir == 0 ? a.FX : (dir == 1 ? a.FY : a.FZ);
   for (int v = 0; v < NVAR; v++) { L[v] = Ls[r * NVAR + v]; R[v] = Rs[r * NVAR + v]; }
+  // the face's B0 (B0 splitting); r is the face's index within the block
+  double B0f[3];
+  const double *pB0 = nullptr;
+  if (a.b0F) {
+    const long nFaceTot = (long)a.nBlock * nFx;
+    for (int c = 0; c < 3; c++)
+      B0f[c] = a.b0F[(dir * 3 + c) * nFaceTot + (long)a.iBlock * nFx + r];
+    pB0 = B0f;
+  }
   int err;
-  if (dir == 0) err = block_flux_dir<0>(a, cs[1] * cs[2], L, R, F, vdt, unA);
-  else if (dir == 1) err = block_flux_dir<1>(a, cs[0] * cs[2], L, R, F, vdt, unA);
-  else err = block_flux_dir<2>(a, cs[0] * cs[1], L, R, F, vdt, unA);
+  if (dir == 0) err = block_flux_dir<0>(a, cs[1] * cs[2], L, R, F, vdt, unA, pB0);
+  else if (dir == 1) err = block_flux_dir<1>(a, cs[0] * cs[2], L, R, F, vdt, unA, pB0);
+  else err = block_flux_dir<2>(a, cs[0] * cs[1], L, R, F, vdt, unA, pB0);
   double *o = Fo + (long)r * 14;
   for (int v = 0; v < NFLUX; v++) o[v] = F[v];
   o[9] = unA; o[10] = 0.0; o[11] = vdt; o[12] = 0.0; o[13] = 0.0;
@@ -314,10 +334,31 @@ __global__ void k_block_source(BlockArgs a) {
   const long fy1 = fy + (long)NB * NVAR;
   const long fz = ((long)(k * NB + j) * NB + i) * NVAR;
   const long fz1 = fz + (long)NB * NB * NVAR;
+  CellB0<double> cb;
+  if (a.b0F) {
+    const long nFaceTot = (long)a.nBlock * (NB + 1) * NB * NB;
+    auto face = [&](int dir, int fi, int fj, int fk, double *o) {
+      const long f = b0_face_index<NB>(a.iBlock, dir, fi, fj, fk);
+      for (int q = 0; q < 3; q++) o[q] = a.b0F[(dir * 3 + q) * nFaceTot + f];
+    };
+    face(0, i, j, k, cb.face[0]);
+    face(0, i + 1, j, k, cb.face[1]);
+    face(1, i, j, k, cb.face[2]);
+    face(1, i, j + 1, k, cb.face[3]);
+    face(2, i, j, k, cb.face[4]);
+    face(2, i, j, k + 1, cb.face[5]);
+    for (int q = 0; q < 3; q++) {
+      cb.cell[q] = a.b0C[q * a.nCellTot + off];
+      cb.curl[q] = a.b0C[(4 + q) * a.nCellTot + off];
+    }
+    cb.div = a.b0C[3 * a.nCellTot + off];
+    cb.useSource = a.useB0Src;
+  }
   powell_source(InvHalf, a.LX[fx + BX], a.RX[fx + BX], a.LX[fx + NVAR + BX],
                 a.RX[fx + NVAR + BX], a.LY[fy + BY], a.RY[fy + BY],
                 a.LY[fy1 + BY], a.RY[fy1 + BY], a.LZ[fz + BZ], a.RZ[fz + BZ],
-                a.LZ[fz1 + BZ], a.RZ[fz1 + BZ], St, Src);
+                a.LZ[fz1 + BZ], a.RZ[fz1 + BZ], St, Src,
+                a.b0F ? &cb : nullptr);
 }
 
 template <int NB>
@@ -618,7 +659,7 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
                   h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
-                  h->corr, h->lowCrit, h->dBcState};
+                  h->corr, h->lowCrit, h->dBcState, h->b0F, h->b0C};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   drop_graphs(h);
@@ -633,10 +674,20 @@ static int apply_grid(bats_handle *h, const std::vector<int> &nei,
                       const std::vector<double> &cs,
                       const std::vector<int> &slabs);
 
+// a new grid drops the B0 field (it belongs to the old block geometry)
+static void drop_b0(bats_handle *h) {
+  if (h->b0F) (void)hipFree(h->b0F);
+  if (h->b0C) (void)hipFree(h->b0C);
+  h->b0F = h->b0C = nullptr;
+  h->useB0 = 0;
+}
+
 extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
                              const double *CellSize_DB) {
   if (!h || !iNeighbor_EB || !CellSize_DB) return BATS_EINVAL;
   drop_graphs(h);
+  (void)hipStreamSynchronize(h->stream);
+  drop_b0(h);
   const int nB = h->cfg.nBlock;
   for (int t = 0; t < 6 * nB; t++)
     if (iNeighbor_EB[t] < -1 || iNeighbor_EB[t] >= nB) return BATS_EINVAL;
@@ -876,6 +927,9 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.slabKind = h->slabKind;
   a.corr = static_cast<double *>(h->corr);
   a.resOnly = 0;
+  a.b0F = h->useB0 ? h->b0F : nullptr;
+  a.b0C = h->b0C;
+  a.useB0Src = h->useB0Src;
   a.lowCrit = low_order_used(c) ? static_cast<const double *>(h->lowCrit)
                                 : nullptr;
   a.betaRC = c.BetaLimiterResChange;
@@ -1218,7 +1272,10 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   a.nFaceRC = resBeta ? c.nFaceLimiterResChange : 0;
   const int opt = (c.UseDivbSource ? OPT_POWELL : 0) |
                   (lowUse ? OPT_LOWORDER : 0) | (resBeta ? OPT_RESBETA : 0) |
-                  (steady ? OPT_STEADY : 0);
+                  (steady ? OPT_STEADY : 0) | (h->useB0 ? OPT_B0 : 0);
+  a.b0F = h->b0F;
+  a.b0C = h->b0C;
+  a.useB0Src = h->useB0Src;
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
   // resolution changes: ghost slabs of the stage input
@@ -1498,6 +1555,8 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
       (nSlab > 0 && !iSlab_IS))
     return BATS_EINVAL;
   drop_graphs(h);
+  (void)hipStreamSynchronize(h->stream);
+  drop_b0(h);
   const int nB = h->cfg.nBlock, NB = h->NB, H = NB / 2;
   for (int t = 0; t < 3 * nB; t++)
     if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
@@ -1675,6 +1734,9 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
     fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW;
   }
   if (fixed && !CellState_VI) return BATS_EINVAL;
+  // with B0 splitting a fixed ghost field would be CellState minus the
+  // ghost cells' B0 (src/ModUpdateStateFast.f90:1264-1266): not supported
+  if (fixed && h->useB0) return BATS_EINVAL;
   if (fixed)
     for (int t = 0; t < 6 * NVAR; t++)
       if (!std::isfinite(CellState_VI[t])) return BATS_EINVAL;
@@ -1696,6 +1758,105 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
                      hipMemcpyHostToDevice));
   }
   return st;
+}
+
+// B0 splitting: face B0 (set_b0_face, src/ModB0.f90:326-335), and per cell
+// the B0 itself, div B0 and curl B0 from the face values (set_b0_source,
+// :568-590), computed once on the host from B0_DGB
+extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
+                           int UseB0Source) {
+  if (!h) return BATS_EINVAL;
+  drop_graphs(h);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (!B0_DGB) {
+    drop_b0(h);
+    return BATS_OK;
+  }
+  const bats_config &c = h->cfg;
+  // FP64 state, same-level grid, no fixed or inflow ghost states, no
+  // low order faces or steady state (the B0 stage variants carry neither)
+  if (h->elemSize != 8 || h->nResSlab > 0) return BATS_EINVAL;
+  for (int s = 0; s < 6; s++)
+    if (h->bcType[s] == BATS_BC_FIXED || h->bcType[s] == BATS_BC_INFLOW)
+      return BATS_EINVAL;
+  if (low_order_used(c) || c.IsSteadyState) return BATS_EINVAL;
+  const int NB = h->NB, g = c.nG, NG = NB + 2 * g, nB = c.nBlock;
+  for (long t = 0; t < 3L * NG * NG * NG * nB; t++)
+    if (!std::isfinite(B0_DGB[t])) return BATS_EINVAL;
+  const long NF = (long)(NB + 1) * NB * NB, nFaceTot = (long)nB * NF;
+  const long nCell = h->nCell;
+  std::vector<double> F(9 * nFaceTot), C(7 * nCell);
+  // B0_DGB(3, MinI:MaxI, MinJ:MaxJ, MinK:MaxK, nBlock), variable fastest
+  auto B0 = [&](int b, int i, int j, int k, int d) {
+    return B0_DGB[((((long)b * NG + (k + g)) * NG + (j + g)) * NG + (i + g)) * 3 + d];
+  };
+  for (int b = 0; b < nB; b++)
+    for (int k = 0; k <= NB; k++)
+      for (int j = 0; j <= NB; j++)
+        for (int i = 0; i <= NB; i++)
+          for (int d = 0; d < 3; d++) {
+            if (j < NB && k < NB)
+              F[(0 * 3 + d) * nFaceTot +
+                (NB == 8 ? b0_face_index<8>(b, 0, i, j, k)
+                         : b0_face_index<16>(b, 0, i, j, k))] =
+                  0.5 * (B0(b, i - 1, j, k, d) + B0(b, i, j, k, d));
+            if (i < NB && k < NB)
+              F[(1 * 3 + d) * nFaceTot +
+                (NB == 8 ? b0_face_index<8>(b, 1, i, j, k)
+                         : b0_face_index<16>(b, 1, i, j, k))] =
+                  0.5 * (B0(b, i, j - 1, k, d) + B0(b, i, j, k, d));
+            if (i < NB && j < NB)
+              F[(2 * 3 + d) * nFaceTot +
+                (NB == 8 ? b0_face_index<8>(b, 2, i, j, k)
+                         : b0_face_index<16>(b, 2, i, j, k))] =
+                  0.5 * (B0(b, i, j, k - 1, d) + B0(b, i, j, k, d));
+          }
+  auto face = [&](int dir, int b, int i, int j, int k, int d) {
+    return F[(dir * 3 + d) * nFaceTot +
+             (NB == 8 ? b0_face_index<8>(b, dir, i, j, k)
+                      : b0_face_index<16>(b, dir, i, j, k))];
+  };
+  for (int b = 0; b < nB; b++) {
+    const double *cs = h->userCellSize.data() + 3 * b;
+    const double DxInv = 1 / cs[0], DyInv = 1 / cs[1], DzInv = 1 / cs[2];
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++) {
+          const long off = (((long)b * NB + k) * NB + j) * NB + i;
+          for (int d = 0; d < 3; d++) C[d * nCell + off] = B0(b, i, j, k, d);
+          C[3 * nCell + off] =
+              DxInv * (face(0, b, i + 1, j, k, 0) - face(0, b, i, j, k, 0)) +
+              DyInv * (face(1, b, i, j + 1, k, 1) - face(1, b, i, j, k, 1)) +
+              DzInv * (face(2, b, i, j, k + 1, 2) - face(2, b, i, j, k, 2));
+          C[4 * nCell + off] =
+              DyInv * (face(1, b, i, j + 1, k, 2) - face(1, b, i, j, k, 2)) -
+              DzInv * (face(2, b, i, j, k + 1, 1) - face(2, b, i, j, k, 1));
+          C[5 * nCell + off] =
+              DzInv * (face(2, b, i, j, k + 1, 0) - face(2, b, i, j, k, 0)) -
+              DxInv * (face(0, b, i + 1, j, k, 2) - face(0, b, i, j, k, 2));
+          C[6 * nCell + off] =
+              DxInv * (face(0, b, i + 1, j, k, 1) - face(0, b, i, j, k, 1)) -
+              DyInv * (face(1, b, i, j + 1, k, 0) - face(1, b, i, j, k, 0));
+        }
+  }
+  double *dF = nullptr, *dC = nullptr;
+  hipError_t e = hipMalloc(&dF, sizeof(double) * F.size());
+  if (e == hipSuccess) e = hipMalloc(&dC, sizeof(double) * C.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(dF, F.data(), sizeof(double) * F.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dC, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (dF) (void)hipFree(dF);
+    if (dC) (void)hipFree(dC);
+    return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
+  }
+  drop_b0(h);
+  h->b0F = dF;
+  h->b0C = dC;
+  h->useB0 = 1;
+  h->useB0Src = UseB0Source != 0;
+  return BATS_OK;
 }
 
 extern "C" int bats_synchronize(bats_handle *h) {

@@ -89,6 +89,10 @@ struct bats_handle {
   double bcState[6 * NVAR];
   double *dBcState;
   bool anySideRC;           // some block side has DiLevel_EB /= 0
+  // B0 splitting (bats_set_b0): face B0 [dir][comp][block][face], cell B0,
+  // div B0 and curl B0 [7][cell]; dropped by any grid change
+  double *b0F, *b0C;
+  int useB0, useB0Src;
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };

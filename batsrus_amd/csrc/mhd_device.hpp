@@ -265,14 +265,28 @@ __device__ __forceinline__ void to_primitive(Real *q) {
 // physical flux of a primitive state for a face normal to axis DIR
 // (src/ModFaceFlux.f90:1194-1212,1649-1876,1574-1601).  Cons[9] is the
 // conserved state with total energy in slot 8.
-template <int DIR, bool FULL, typename Real>
+//
+// With B0 splitting (B0F, the face's B0 in B0): S holds B1; the magnetic
+// force is -B1n (B1 + B0) - B0n B1 + (B1^2/2 + B0.B1) n, the induction flux
+// uses the full field, the energy flux the Poynting term of B1
+// (src/ModFaceFlux.f90:1203-1212,1798-1805).
+template <int DIR, bool FULL, bool B0F = false, typename Real>
 __device__ __forceinline__ void phys_flux(const PhysT<Real> &ph, const Real *S,
                                           Real *Cons, Real *F,
-                                          Real &UnOut) {
+                                          Real &UnOut,
+                                          const Real *B0 = nullptr) {
   const Real Rho = S[RHO], p = S[PP];
   const Real u[3] = {S[MX], S[MY], S[MZ]};
   const Real b[3] = {S[BX], S[BY], S[BZ]};
   const Real Un = u[DIR], Bn = b[DIR];
+  Real fullB[3] = {b[0], b[1], b[2]};
+  Real B0n = Real(0), FullBn = Bn;
+  if (B0F) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) fullB[d] = b[d] + B0[d];
+    B0n = B0[DIR];
+    FullBn = B0n + Bn;
+  }
   F[RHO] = Rho * Un;
   Cons[RHO] = Rho;
   Cons[MX] = Rho * u[0]; Cons[MY] = Rho * u[1]; Cons[MZ] = Rho * u[2];
@@ -281,19 +295,22 @@ __device__ __forceinline__ void phys_flux(const PhysT<Real> &ph, const Real *S,
   const Real e = ph.invgm1 * p + Real(0.5) * Rho * (u[0] * u[0] + u[1] * u[1] +
                                                 u[2] * u[2]);
   const Real B2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-  const Real pTotal = Real(0.5) * B2;
+  Real pTotal = Real(0.5) * B2;
+  if (B0F) pTotal = pTotal + (B0[0] * b[0] + B0[1] * b[1] + B0[2] * b[2]);
   const Real UnRho = Un * Rho;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     Real f = UnRho * u[d];
     if (d == DIR) f = f + p;
-    Real mf = -Bn * b[d];
+    Real mf = -Bn * fullB[d];
+    if (B0F) mf = mf - B0n * b[d];
     if (d == DIR) mf = mf + pTotal;
     F[MX + d] = f + mf;
   }
   Real fb[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) fb[d] = (d == DIR) ? Real(0.0) : Un * b[d] - u[d] * Bn;
+  for (int d = 0; d < 3; d++)
+    fb[d] = (d == DIR) ? Real(0.0) : Un * fullB[d] - u[d] * FullBn;
   F[BX] = fb[0]; F[BY] = fb[1]; F[BZ] = fb[2];
   Real fe = Un * (e + p);
 #pragma unroll
@@ -308,15 +325,19 @@ __device__ __forceinline__ void phys_flux(const PhysT<Real> &ph, const Real *S,
 // fast magnetosonic speed of state S for a face normal to DIR
 // (src/ModFaceFlux.f90:3900-3921,3990-4134).  Returns false on a negative
 // fast speed squared.
-template <int DIR, typename Real>
+// With B0 splitting the speeds use the full field B1 + B0 (:3912-3920).
+template <int DIR, bool B0F = false, typename Real>
 __device__ __forceinline__ bool fast_speed(const PhysT<Real> &ph, const Real *S,
-                                           Real &Un, Real &Fast) {
+                                           Real &Un, Real &Fast,
+                                           const Real *B0 = nullptr) {
   const Real InvRho = frcp(S[RHO]);
   const Real Sound2 = S[PP] * ph.gamma * InvRho;
   Un = S[MX + DIR];
-  const Real bx = S[BX], by = S[BY], bz = S[BZ];
+  Real bx = S[BX], by = S[BY], bz = S[BZ];
+  if (B0F) { bx = bx + B0[0]; by = by + B0[1]; bz = bz + B0[2]; }
   const Real Alfven2 = (bx * bx + by * by + bz * bz) * InvRho;
-  const Real FullBn = S[BX + DIR];
+  const Real FullBn = B0F ? (DIR == 0 ? bx : DIR == 1 ? by : bz)
+                          : S[BX + DIR];
   const Real Alfven2Normal = InvRho * (FullBn * FullBn);
   const Real Fast2 = Sound2 + Alfven2;
   const Real Discr =
@@ -332,15 +353,18 @@ __device__ __forceinline__ bool fast_speed(const PhysT<Real> &ph, const Real *S,
 // (Evaluating the Un < 0 branch as the reflected Un >= 0 branch removes the
 // branch divergence but keeps about 30 more values live; on gfx950 the
 // resulting register spills cost more than the divergence.)
-template <int DIR, bool FULL, typename Real>
+// With B0 splitting (B0F) the states hold B1 and the face's B0 enters the
+// total field and total pressure (src/ModFaceFlux.f90:2905-2921,2955-2958,
+// 3115-3129).
+template <int DIR, bool FULL, bool B0F = false, typename Real>
 __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
                                     const Real *SR, Real *F, Real &Cmax,
-                                    Real &UnOut) {
+                                    Real &UnOut, const Real *B0 = nullptr) {
   constexpr int in = DIR, it1 = (DIR + 1) % 3, it2 = (DIR + 2) % 3;
   int err = 0;
   Real UnLs, FastL, UnRs, FastR;
-  if (!fast_speed<DIR>(ph, SL, UnLs, FastL)) err |= ERRB_NEGFAST2;
-  if (!fast_speed<DIR>(ph, SR, UnRs, FastR)) err |= ERRB_NEGFAST2;
+  if (!fast_speed<DIR, B0F>(ph, SL, UnLs, FastL, B0)) err |= ERRB_NEGFAST2;
+  if (!fast_speed<DIR, B0F>(ph, SR, UnRs, FastR, B0)) err |= ERRB_NEGFAST2;
   const Real sL = fmin2(UnLs - FastL, UnRs - FastR);
   const Real sR = fmax2(UnLs + FastL, UnRs + FastR);
   Cmax = fmax2(sR, -sL);
@@ -349,7 +373,7 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
     Real S[NVAR], Cons[NFLUX];
 #pragma unroll
     for (int v = 0; v < NVAR; v++) S[v] = left ? SL[v] : SR[v];
-    phys_flux<DIR, FULL>(ph, S, Cons, F, UnOut);
+    phys_flux<DIR, FULL, B0F>(ph, S, Cons, F, UnOut, B0);
     return err;
   }
   const Real cTiny = ph.cTiny;
@@ -359,15 +383,21 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
   const Real UnR = SR[MX + in], Ut1R = SR[MX + it1], Ut2R = SR[MX + it2];
   const Real B1nR = SR[BX + in], B1t1R = SR[BX + it1], B1t2R = SR[BX + it2];
 
+  const Real B0n = B0F ? B0[in] : Real(0), B0t1 = B0F ? B0[it1] : Real(0),
+             B0t2 = B0F ? B0[it2] : Real(0);
   const Real B1n = Real(0.5) * (B1nL + B1nR);
   const Real B1n2 = B1n * B1n;
-  const Real Bn = B1n;
+  const Real Bn = B0F ? B1n + B0n : B1n;
   const Real Bn2 = Bn * Bn;
   const Real SignBn = fsign(Real(1.), Bn);
   const Real PbL = Real(0.5) * (B1n2 + B1t1L * B1t1L + B1t2L * B1t2L);
   const Real PbR = Real(0.5) * (B1n2 + B1t1R * B1t1R + B1t2R * B1t2R);
-  const Real PtotL = pL + PbL;
-  const Real PtotR = pR + PbR;
+  Real PtotL = pL + PbL;
+  Real PtotR = pR + PbR;
+  if (B0F) {
+    PtotL = PtotL + B1n * B0n + B1t1L * B0t1 + B1t2L * B0t2;
+    PtotR = PtotR + B1n * B0n + B1t1R * B0t1 + B1t2R * B0t2;
+  }
   const Real DsL = sL - UnL, DsR = sR - UnR;
   const Real DsRhoL = DsL * RhoL, DsRhoR = DsR * RhoR;
   const Real Tmp0 = frcp(DsRhoR - DsRhoL);
@@ -375,7 +405,10 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
   const Real Ptot12 =
       (DsRhoR * PtotL - DsRhoL * PtotR + DsRhoR * DsRhoL * (UnR - UnL)) * Tmp0;
 
-  const Real Bt1L = B1t1L, Bt2L = B1t2L, Bt1R = B1t1R, Bt2R = B1t2R;
+  const Real Bt1L = B0F ? B1t1L + B0t1 : B1t1L;
+  const Real Bt2L = B0F ? B1t2L + B0t2 : B1t2L;
+  const Real Bt1R = B0F ? B1t1R + B0t1 : B1t1R;
+  const Real Bt2R = B0F ? B1t2R + B0t2 : B1t2R;
   Real Rho, Ut1, Ut2, B1t1, B1t2, uDotB1, e;
   Real Ut1L1, Ut2L1, B1t1L1, B1t2L1, Ut1R1, Ut2R1, B1t1R1, B1t2R1;
   auto outer_left = [&]() {
@@ -388,6 +421,7 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
       Ut2L1 = Ut2L - Bn * Bt2L * (Un - UnL) * T;
       B1t1L1 = Bt1L * (DsRhoL * DsL - Bn2) * T;
       B1t2L1 = Bt2L * (DsRhoL * DsL - Bn2) * T;
+      if (B0F) { B1t1L1 = B1t1L1 - B0t1; B1t2L1 = B1t2L1 - B0t2; }
     }
   };
   auto outer_right = [&]() {
@@ -400,6 +434,7 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
       Ut2R1 = Ut2R - Bn * Bt2R * (Un - UnR) * T;
       B1t1R1 = Bt1R * (DsRhoR * DsR - Bn2) * T;
       B1t2R1 = Bt2R * (DsRhoR * DsR - Bn2) * T;
+      if (B0F) { B1t1R1 = B1t1R1 - B0t1; B1t2R1 = B1t2R1 - B0t2; }
     }
   };
   if (Un >= Real(0.)) {
@@ -463,11 +498,17 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
   }
   // flux of the HLLD state in the original frame (:3111-3131)
   const Real RhoUn = Rho * Un;
-  const Real Bt1 = B1t1, Bt2 = B1t2;
+  const Real Bt1 = B0F ? B1t1 + B0t1 : B1t1, Bt2 = B0F ? B1t2 + B0t2 : B1t2;
   F[RHO] = RhoUn;
-  F[MX + in] = RhoUn * Un - B1n * Bn + Ptot12;
-  F[MX + it1] = RhoUn * Ut1 - B1n * Bt1;
-  F[MX + it2] = RhoUn * Ut2 - B1n * Bt2;
+  if (B0F) {
+    F[MX + in] = RhoUn * Un - B1n * Bn - B0n * B1n + Ptot12;
+    F[MX + it1] = RhoUn * Ut1 - B1n * Bt1 - B0n * B1t1;
+    F[MX + it2] = RhoUn * Ut2 - B1n * Bt2 - B0n * B1t2;
+  } else {
+    F[MX + in] = RhoUn * Un - B1n * Bn + Ptot12;
+    F[MX + it1] = RhoUn * Ut1 - B1n * Bt1;
+    F[MX + it2] = RhoUn * Ut2 - B1n * Bt2;
+  }
   F[BX + in] = Real(0.0);
   F[BX + it1] = Un * Bt1 - Ut1 * Bn;
   F[BX + it2] = Un * Bt2 - Ut2 * Bn;
@@ -483,11 +524,13 @@ __device__ __forceinline__ int hlld(const PhysT<Real> &ph, const Real *SL,
 // numerical flux for a face normal to DIR from the left/right primitive
 // face states (src/ModFaceFlux.f90:2130-2473).  Out: F[0..8] already times
 // Area, Vdt = CmaxDt*Area, Un*Area.
-template <int DIR, int FLUX, bool FULL, typename Real>
+// With B0 splitting (B0F) B0 is the face's B0 in the face's frame.
+template <int DIR, int FLUX, bool FULL, bool B0F = false, typename Real>
 __device__ __forceinline__ int numerical_flux(const PhysT<Real> &ph, Real Area,
                                               const Real *Lin,
                                               const Real *Rin, Real *F,
-                                              Real &Vdt, Real &UnArea) {
+                                              Real &Vdt, Real &UnArea,
+                                              const Real *B0 = nullptr) {
   Real L[NVAR], R[NVAR];
 #pragma unroll
   for (int v = 0; v < NVAR; v++) { L[v] = Lin[v]; R[v] = Rin[v]; }
@@ -503,16 +546,16 @@ __device__ __forceinline__ int numerical_flux(const PhysT<Real> &ph, Real Area,
     DiffE = Real(0.5) * ((R[BX + DIR] + L[BX + DIR]) * dB);
   }
   if (FLUX == FLUX_HLLD) {
-    err |= hlld<DIR, FULL>(ph, L, R, F, Cmax, Un);
+    err |= hlld<DIR, FULL, B0F>(ph, L, R, F, Cmax, Un, B0);
   } else {
     Real S[NVAR];
 #pragma unroll
     for (int v = 0; v < NVAR; v++) S[v] = Real(0.5) * (L[v] + R[v]);
     Real CL[NFLUX], CR[NFLUX], FL[NFLUX], FR[NFLUX], UnL, UnR;
-    phys_flux<DIR, FULL>(ph, L, CL, FL, UnL);
-    phys_flux<DIR, FULL>(ph, R, CR, FR, UnR);
+    phys_flux<DIR, FULL, B0F>(ph, L, CL, FL, UnL, B0);
+    phys_flux<DIR, FULL, B0F>(ph, R, CR, FR, UnR, B0);
     Real UnHat, FastHat;
-    if (!fast_speed<DIR>(ph, S, UnHat, FastHat)) err |= ERRB_NEGFAST2;
+    if (!fast_speed<DIR, B0F>(ph, S, UnHat, FastHat, B0)) err |= ERRB_NEGFAST2;
     Cmax = fabsr(UnHat) + FastHat;
     if (FLUX == FLUX_RUSANOV) {
 #pragma unroll
@@ -521,8 +564,8 @@ __device__ __forceinline__ int numerical_flux(const PhysT<Real> &ph, Real Area,
       Un = Real(0.5) * (UnL + UnR);
     } else {
       Real UnLs, FastL, UnRs, FastR;
-      if (!fast_speed<DIR>(ph, L, UnLs, FastL)) err |= ERRB_NEGFAST2;
-      if (!fast_speed<DIR>(ph, R, UnRs, FastR)) err |= ERRB_NEGFAST2;
+      if (!fast_speed<DIR, B0F>(ph, L, UnLs, FastL, B0)) err |= ERRB_NEGFAST2;
+      if (!fast_speed<DIR, B0F>(ph, R, UnRs, FastR, B0)) err |= ERRB_NEGFAST2;
       const Real Cleft = fmin2(fmin2(Real(0.0), UnLs - FastL), UnHat - FastHat);
       const Real Cright = fmax2(fmax2(Real(0.0), UnRs + FastR), UnHat + FastHat);
       const Real WeightLeft = Cright * frcp(Cright - Cleft);
@@ -656,16 +699,27 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
   Unew[PP] = limit_pressure(ph, energy_to_pressure(ph, Unew, Unew[PP]));
 }
 
-// 8-wave (Powell) div B source of one cell on a Cartesian grid without B0
+// B0 of a cell for the 8-wave source with B0 splitting: the B0 of its six
+// faces, its own B0, and div B0 / curl B0 (set_b0_source) when UseB0Source
+template <typename Real> struct CellB0 {
+  Real face[6][3];
+  Real cell[3];
+  Real div, curl[3];
+  int useSource;
+};
+
+// 8-wave (Powell) div B source of one cell on a Cartesian grid
 // (src/ModCalcSource.f90:1694-1889 with DoCorrectFace false, :930-951,
-// :1016-1022).  B?lo/B?hi are the normal-field left (L) and right (R) face
-// states at the lower and upper faces of the cell in each direction;
-// InvHalf = 0.5/CellSize; St is the cell's stage-input state.
+// :1016-1022), with the B0 terms when B0s is given.  B?lo/B?hi are the
+// normal-field left (L) and right (R) face states at the lower and upper
+// faces of the cell in each direction; InvHalf = 0.5/CellSize; St is the
+// cell's stage-input state.
 template <typename Real>
 __device__ __forceinline__ void powell_source(
     const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
     Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
-    Real LzHi, Real RzHi, const Real *St, Real *Src) {
+    Real LzHi, Real RzHi, const Real *St, Real *Src,
+    const CellB0<Real> *B0s = nullptr) {
   const Real dB1 = InvHalf[0] * (RxLo - LxLo);
   const Real dB2 = InvHalf[0] * (RxHi - LxHi);
   const Real dB3 = InvHalf[1] * (RyLo - LyLo);
@@ -682,9 +736,37 @@ __device__ __forceinline__ void powell_source(
   const Real ux = RhoInv * St[MX], uy = RhoInv * St[MY],
              uz = RhoInv * St[MZ];
   Src[RHO] = Real(0);
-  Src[MX] = -(DivB * St[BX]);
-  Src[MY] = -(DivB * St[BY]);
-  Src[MZ] = -(DivB * St[BZ]);
+  if (B0s) {
+    // B0 splitting: the face B0 times each face's normal-field jump, the
+    // cell B0 times the internal divergence (src/ModCalcSource.f90:
+    // 1792-1808,1880-1885), then -B1 div(B0) - curl(B0) x B1 with
+    // UseB0Source (:953-967); B0s->face[f] holds the B0 of the cell's
+    // faces x-, x+, y-, y+, z-, z+
+    const Real dBf[6] = {dB1, dB2, dB3, dB4, dB5, dB6};
+    Real m[3];
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      Real t = Real(0) - B0s->face[0][v] * dBf[0] - B0s->face[1][v] * dBf[1];
+      t = t - B0s->face[2][v] * dBf[2] - B0s->face[3][v] * dBf[3];
+      t = t - B0s->face[4][v] * dBf[4] - B0s->face[5][v] * dBf[5];
+      t = t - Internal * B0s->cell[v];
+      m[v] = t - DivB * St[BX + v];
+    }
+    if (B0s->useSource) {
+      const Real *cB = B0s->curl;
+      const Real Bx = St[BX], By = St[BY], Bz = St[BZ], dv = B0s->div;
+      m[0] = m[0] - Bx * dv - (cB[1] * Bz - cB[2] * By);
+      m[1] = m[1] - By * dv - (cB[2] * Bx - cB[0] * Bz);
+      m[2] = m[2] - Bz * dv - (cB[0] * By - cB[1] * Bx);
+    }
+    Src[MX] = m[0];
+    Src[MY] = m[1];
+    Src[MZ] = m[2];
+  } else {
+    Src[MX] = -(DivB * St[BX]);
+    Src[MY] = -(DivB * St[BY]);
+    Src[MZ] = -(DivB * St[BZ]);
+  }
   Src[BX] = -(DivB * ux);
   Src[BY] = -(DivB * uy);
   Src[BZ] = -(DivB * uz);

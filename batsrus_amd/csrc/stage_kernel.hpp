@@ -72,6 +72,11 @@ struct StageArgs {
                        // crit_index); read only by the low order variant
   double betaRC;       // BetaLimiterResChange, nFaceLimiterResChange: read
   int nFaceRC;         // only by the OPT_RESBETA variant
+  // B0 splitting (OPT_B0 variant, FP64): B0 of every face, [dir][comp]
+  // [block][face in the block's LeftState_V? order], and per cell B0,
+  // div B0, curl B0 ([7][interior cell]); useB0Src: UseB0Source
+  const double *b0F, *b0C;
+  int useB0Src;
 };
 
 // the cyclic frame rotation (x, y, z) -> (y, z, x) that turns a y face into
@@ -96,7 +101,23 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 // (outer boundaries and resolution changes).
 // OPT_STEADY: steady state, every cell with its own time step (dtmaxCB,
 // from this launch's fluxes when doDt).
-enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8 };
+// OPT_B0: B0 splitting, the state holds B1 = B - B0 (FP64 only).
+enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8,
+       OPT_B0 = 16 };
+
+// index of a face of block b in the B0 face arrays (per direction, the
+// reference's LeftState_VX(nI+1,nJ,nK) / _VY(nI,nJ+1,nK) / _VZ(nI,nJ,nK+1)
+// order within a block); (fi, fj, fk) are the face's cell-index coordinates
+template <int NB>
+__host__ __device__ __forceinline__ long b0_face_index(int b, int dir, int fi,
+                                                       int fj, int fk) {
+  constexpr long NF = (long)(NB + 1) * NB * NB;
+  long l;
+  if (dir == 0) l = ((long)fk * NB + fj) * (NB + 1) + fi;
+  else if (dir == 1) l = ((long)fk * (NB + 1) + fj) * NB + fi;
+  else l = ((long)fk * NB + fj) * NB + fi;
+  return (long)b * NF + l;
+}
 
 // index of the low order weight of a face of block b: the face normal to
 // dir at face coordinates (i, j, k), whose dir component is the face index
@@ -175,25 +196,27 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
-template <int DIR, int FLUX, int LIM, typename T>
+template <int DIR, int FLUX, int LIM, bool B0F, typename T>
 __device__ __forceinline__ int face_flux_w(const PhysT<T> &ph, T area,
                                            T betaL, T betaR, T w,
                                            const T *q0, const T *q1,
                                            const T *q2, const T *q3, T *F,
-                                           T &vdt, T &unA) {
+                                           T &vdt, T &unA, const T *B0) {
   T L[NVAR], R[NVAR];
   face_states_w<LIM>(betaL, betaR, w, q0, q1, q2, q3, L, R);
-  return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  return numerical_flux<DIR, FLUX, false, B0F>(ph, area, L, R, F, vdt, unA,
+                                               B0);
 }
 
-template <int DIR, int FLUX, int LIM, typename T>
+template <int DIR, int FLUX, int LIM, bool B0F, typename T>
 __device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T betaL,
                                          T betaR, const T *q0, const T *q1,
                                          const T *q2, const T *q3, T *F,
-                                         T &vdt, T &unA) {
+                                         T &vdt, T &unA, const T *B0) {
   T L[NVAR], R[NVAR];
   face_states<LIM>(betaL, betaR, q0, q1, q2, q3, L, R);
-  return numerical_flux<DIR, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  return numerical_flux<DIR, FLUX, false, B0F>(ph, area, L, R, F, vdt, unA,
+                                               B0);
 }
 
 // ---------------------------------------------------------------------
@@ -210,6 +233,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // per-cell limiter beta near sides with DiLevel_EB /= 0
   constexpr bool RESB = (OPT & OPT_RESBETA) != 0 && LIM != LIM_FIRST;
   constexpr bool STEADY = (OPT & OPT_STEADY) != 0;
+  constexpr bool B0S = (OPT & OPT_B0) != 0;
+  static_assert(!B0S || sizeof(T) == 8, "B0 splitting is FP64 only");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -291,6 +316,13 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   auto sFi = [&](int l, int dir, int v, int q) {
     return ((l * 2 + dir) * NFS + v) * NFX + q;
   };
+  // B0 of a face (B0 splitting)
+  const long nFaceTot = (long)a.nBlock * (NB + 1) * NB * NB;
+  auto b0face = [&](int dir, int blk, int fi, int fj, int fk, T *out) {
+    const long f = b0_face_index<NB>(blk, dir, fi, fj, fk);
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = T(a.b0F[(dir * 3 + c) * nFaceTot + f]);
+  };
 
   // a persistent grid walks the flat work list of block groups; with one
   // workgroup per group the loop runs once
@@ -363,7 +395,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         }
         dLk[v] = dl;
       }
-      err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
+      if (B0S) {
+        T B0f[3];
+        b0face(2, bb, i, j, fz, B0f);
+        err |= numerical_flux<2, FLUX, false, true>(ph, AreaZ, L, R, F, vdt,
+                                                    unA, B0f);
+      } else {
+        err |= numerical_flux<2, FLUX, false>(ph, AreaZ, L, R, F, vdt, unA);
+      }
 #pragma unroll
       for (int v = 0; v < NFLUX; v++) Fz[v] = F[v];
       Fz[PP] = vdt;
@@ -452,15 +491,24 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           bL = cell_beta(beta, betaRC, nFaceRC, lo, hi, fp - 1, NB);
           bR = cell_beta(beta, betaRC, nFaceRC, lo, hi, fp, NB);
         }
+        // the face's B0, in the rotated frame for a y face
+        T B0r[3];
+        if (B0S) {
+          T B0f[3];
+          if (dir == 0) b0face(0, bf, fp, row, k, B0f);
+          else b0face(1, bf, row, fp, k, B0f);
+#pragma unroll
+          for (int d = 0; d < 3; d++) B0r[d] = dir == 0 ? B0f[d] : B0f[(d + 1) % 3];
+        }
         if (LOWORD)
-          err |= face_flux_w<0, FLUX, LIM>(
+          err |= face_flux_w<0, FLUX, LIM, B0S>(
               ph, area, bL, bR,
               T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
                                    : crit_index<NB>(bf, 1, row, fp, k)],
-              c0, c1, c2, c3, F, vdt, unA);
+              c0, c1, c2, c3, F, vdt, unA, B0r);
         else
-          err |= face_flux<0, FLUX, LIM>(ph, area, bL, bR, c0, c1, c2, c3, F,
-                                         vdt, unA);
+          err |= face_flux<0, FLUX, LIM, B0S>(ph, area, bL, bR, c0, c1, c2,
+                                              c3, F, vdt, unA, B0r);
         if (dir != 0) unrotate_flux(F);
         F[PP] = vdt;
         if (a.corr && (fp == 0 || fp == NB))
@@ -555,9 +603,26 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               if (j == NB - 1) corrected_bn(b, 3, i, k, LyHi, RyHi);
             }
             T Src[NFLUX];
+            CellB0<T> cb;
+            if (B0S) {
+              b0face(0, b, i, j, k, cb.face[0]);
+              b0face(0, b, i + 1, j, k, cb.face[1]);
+              b0face(1, b, i, j, k, cb.face[2]);
+              b0face(1, b, i, j + 1, k, cb.face[3]);
+              b0face(2, b, i, j, k, cb.face[4]);
+              b0face(2, b, i, j, k + 1, cb.face[5]);
+              const long nc = a.nCellTot;
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                cb.cell[c] = T(a.b0C[c * nc + off]);
+                cb.curl[c] = T(a.b0C[(4 + c) * nc + off]);
+              }
+              cb.div = T(a.b0C[3 * nc + off]);
+              cb.useSource = a.useB0Src;
+            }
             powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
                           RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
-                          FzHi[NFZ - 1], Uc, Src);
+                          FzHi[NFZ - 1], Uc, Src, B0S ? &cb : nullptr);
             double Srcd[NFLUX];
 #pragma unroll
             for (int v = 0; v < NVAR; v++) {
@@ -860,6 +925,31 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 14: BATS_K(LM, 14); break;                                            \
   default: BATS_K(LM, 15); break;                                            \
   }
+  // B0 splitting (FP64): with or without the 8-wave source and the sharper
+  // resolution-change limiter; no low order faces, no steady state
+#define BATS_LAUNCH_B0(LM, RB)                                               \
+  switch (opt & (RB ? 5 : 1)) {                                              \
+  case 0: BATS_K(LM, 16); break;                                             \
+  case 1: BATS_K(LM, 17); break;                                             \
+  case 4: BATS_K(LM, 20); break;                                             \
+  default: BATS_K(LM, 21); break;                                            \
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (opt & OPT_B0) {
+      if (opt & (OPT_LOWORDER | OPT_STEADY)) return hipErrorInvalidValue;
+      switch (a.lim) {
+      case LIM_MINMOD: BATS_LAUNCH_B0(LIM_MINMOD, false); break;
+      case LIM_MC: BATS_LAUNCH_B0(LIM_MC, true); break;
+      case LIM_MC3: BATS_LAUNCH_B0(LIM_MC3, true); break;
+      case LIM_BETA: BATS_LAUNCH_B0(LIM_BETA, true); break;
+      case LIM_NO: BATS_LAUNCH_B0(LIM_NO, false); break;
+      default: BATS_LAUNCH_B0(LIM_FIRST, false); break;
+      }
+      return hipGetLastError();
+    }
+  } else {
+    if (opt & OPT_B0) return hipErrorInvalidValue;
+  }
   switch (a.lim) {
   case LIM_MINMOD: BATS_LAUNCH3(LIM_MINMOD); break;
   case LIM_MC: BATS_LAUNCH7(LIM_MC); break;
@@ -871,6 +961,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
 #undef BATS_K
 #undef BATS_LAUNCH3
 #undef BATS_LAUNCH7
+#undef BATS_LAUNCH_B0
   return hipGetLastError();
 }
 

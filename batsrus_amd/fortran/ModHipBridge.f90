@@ -39,6 +39,7 @@ module ModHipBridge
        bats_exchange_messages, bats_set_stage, bats_calc_face_value, &
        bats_calc_face_flux, bats_calc_source, bats_update_state, &
        bats_save_cons_flux, bats_message_pass_face, bats_apply_cons_flux, &
+       bats_set_b0, &
        bats_calc_timestep, &
        bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
@@ -164,6 +165,13 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: DoResChangeOnly, iBlock
      end function bats_calc_face_flux
+     integer(c_int) function bats_set_b0(Handle, B0_DGB, UseB0Source) &
+          bind(C, name="bats_set_b0")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       real(c_double), intent(in):: B0_DGB(*)
+       integer(c_int), value:: UseB0Source
+     end function bats_set_b0
      integer(c_int) function bats_save_cons_flux(Handle, iBlock) &
           bind(C, name="bats_save_cons_flux")
        import:: c_int, c_ptr

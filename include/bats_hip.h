@@ -160,6 +160,21 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
                       const double *CellSize_DB, int nSlab,
                       const int *iSlab_IS);
 
+/* B0 splitting (UseB0, src/ModB0.f90): from here on the state's field is
+ * B1 = B - B0 for the static field B0 given at the cell centres and ghost
+ * cells, B0_DGB(3, MinI:MaxI, MinJ:MaxJ, MinK:MaxK, nBlock) (Fortran order,
+ * nG ghost layers, as the reference's B0_DGB).  The face B0 is the mean of
+ * the two cells' B0 (set_b0_face, :314-336); the fluxes, wave speeds and
+ * HLLD use the full field where the reference does
+ * (src/ModFaceFlux.f90:1203-1212,1798-1805,2905-2958,3111-3131,3912-3920);
+ * with UseDivbSource the 8-wave source takes the B0 terms
+ * (src/ModCalcSource.f90:1792-1808,1880-1885) and, with UseB0Source,
+ * -B1 div(B0) - curl(B0) x B1 from the face B0 (:953-967, set_b0_source
+ * :568-590).  FP64 handles on same-level grids only; not with fixed or
+ * inflow outer sides, adaptive low order faces or steady state
+ * (BATS_EINVAL).  A new grid drops B0; B0_DGB = NULL drops it too. */
+int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
+
 /* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41) */
 enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
        BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10 };

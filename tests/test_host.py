@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _header_functions():
     txt = open(os.path.join(ROOT, "include", "bats_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(bats_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(bats_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_abi_exports_every_declared_symbol():
