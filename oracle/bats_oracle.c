@@ -1648,6 +1648,9 @@ typedef struct {
   double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ, *Src;
   /* B0 of the block's faces and its div / curl (B0 splitting only) */
   double *B0X, *B0Y, *B0Z, *DivB0, *CurlB0;
+  /* faces between two body cells (inner body only) */
+  unsigned char *SkipX, *SkipY, *SkipZ;
+  double *Keep;   /* the block's state before the update (body cells) */
 } ob_scratch;
 
 static void scratch_alloc(const ob_config *c, ob_scratch *w) {
@@ -1666,11 +1669,199 @@ static void scratch_alloc(const ob_config *c, ob_scratch *w) {
   w->B0Z = (double *)calloc(nfz * 3, 8);
   w->DivB0 = (double *)calloc((long)c->nI * c->nJ * c->nK, 8);
   w->CurlB0 = (double *)calloc((long)c->nI * c->nJ * c->nK * 3, 8);
+  w->SkipX = (unsigned char *)calloc(nfx, 1);
+  w->SkipY = (unsigned char *)calloc(nfy, 1);
+  w->SkipZ = (unsigned char *)calloc(nfz, 1);
+  {
+    ob_shape sh = shape_of(c);
+    w->Keep = (double *)calloc(block_len(&sh), 8);
+  }
 }
 static void scratch_free(ob_scratch *w) {
   free(w->LX); free(w->RX); free(w->LY); free(w->RY); free(w->LZ);
   free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
   free(w->B0X); free(w->B0Y); free(w->B0Z); free(w->DivB0); free(w->CurlB0);
+  free(w->SkipX); free(w->SkipY); free(w->SkipZ); free(w->Keep);
+}
+
+/* ---------------------------------------------------------------------- */
+/* inner body: true cells and the 'ionosphere' face boundary condition,    */
+/* as the fast update applies them (src/ModUpdateStateFast.f90:230-265,    */
+/* 343-345, 924-937, 1277-1327) with calc_timestep's mask                  */
+/* (src/ModTimeStepControl.f90:158-159, 255)                               */
+/* ---------------------------------------------------------------------- */
+void ob_body_face_state(const double *True_V, const double *XyzFace_D,
+                        double BodyRho, double BodyP, double B1rCoef,
+                        double *Ghost_V) {
+  const double DensityJumpLimit = 0.1;
+  /* FaceState_VI(:, body1_): DefaultState_V with the body density and
+   * pressure (src/ModPhysics.f90:535-541) */
+  const double FaceState_V[NV] = {BodyRho, 0, 0, 0, 0, 0, 0, BodyP};
+  int v;
+  for (v = 0; v < NV; v++) Ghost_V[v] = True_V[v];
+  /* where(DefaultState_V > 0): rho and p, the jump toward the body state
+   * limited to DensityJumpLimit times the true value */
+  for (v = 0; v < NV; v++) {
+    if (v != OB_RHO && v != OB_P) continue;
+    Ghost_V[v] = True_V[v] + fsign(1.0, FaceState_V[v] - True_V[v]) *
+                                 fmin2(fabs(FaceState_V[v] - True_V[v]),
+                                       DensityJumpLimit * True_V[v]);
+  }
+  Ghost_V[OB_P] = True_V[OB_P];                     /* pressure floats */
+  for (v = 0; v < 3; v++) Ghost_V[OB_RHOUX + v] = -True_V[OB_RHOUX + v];
+  if (B1rCoef != 1.0) { /* reflect (part of) the radial B1 */
+    const double Coef = (1 - B1rCoef) /
+                        (XyzFace_D[0] * XyzFace_D[0] +
+                         XyzFace_D[1] * XyzFace_D[1] +
+                         XyzFace_D[2] * XyzFace_D[2]);
+    const double BdotR = True_V[OB_BX] * XyzFace_D[0] +
+                         True_V[OB_BY] * XyzFace_D[1] +
+                         True_V[OB_BZ] * XyzFace_D[2];
+    for (v = 0; v < 3; v++)
+      Ghost_V[OB_BX + v] = True_V[OB_BX + v] - Coef * XyzFace_D[v] * BdotR;
+  }
+}
+
+static int block_has_body(const ob_session *s, int b) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const unsigned char *U = s->Used + (long)b * (block_len(&sh) / NV);
+  int i, j, k;
+  for (k = 1; k <= c->nK; k++)
+    for (j = 1; j <= c->nJ; j++)
+      for (i = 1; i <= c->nI; i++)
+        if (!U[cidx(&sh, i, j, k) / NV]) return 1;
+  return 0;
+}
+
+/* The body's face states of one block (get_face_x/y/z of the fast update,
+ * src/ModUpdateStateFast.f90:924-937): a face value whose limiter stencil
+ * meets a body cell falls back to first order, and a face between a true
+ * and a body cell takes the ionosphere ghost state on the body side.
+ * Faces between two body cells are flagged in Skip (their flux is zero,
+ * :237-240). */
+static void body_faces(const ob_session *s, int b, double *LX, double *RX,
+                       double *LY, double *RY, double *LZ, double *RZ,
+                       unsigned char *SkipX, unsigned char *SkipY,
+                       unsigned char *SkipZ) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const int nI = c->nI, nJ = c->nJ, nK = c->nK;
+  const double *Blk = s->State + (long)b * block_len(&sh);
+  const unsigned char *U = s->Used + (long)b * (block_len(&sh) / NV);
+  const double *x0 = s->XyzStart_DB + 3L * b, *dx = s->CellSize_DB + 3L * b;
+  int dir;
+  for (dir = 0; dir < 3; dir++) {
+    const int ni = nI + (dir == 0), nj = nJ + (dir == 1), nk = nK + (dir == 2);
+    double *L = dir == 0 ? LX : dir == 1 ? LY : LZ;
+    double *R = dir == 0 ? RX : dir == 1 ? RY : RZ;
+    unsigned char *Skip = dir == 0 ? SkipX : dir == 1 ? SkipY : SkipZ;
+    int i, j, k, v, t;
+    for (k = 1; k <= nk; k++)
+      for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++) {
+          const long f = ((long)(k - 1) * nj + (j - 1)) * ni + (i - 1);
+          int ci[4][3], u[4];
+          double P1[NV], P2[NV];
+          for (t = 0; t < 4; t++) { /* cells f-2 .. f+1 along dir */
+            ci[t][0] = i; ci[t][1] = j; ci[t][2] = k;
+            ci[t][dir] += t - 2;
+            u[t] = U[cidx(&sh, ci[t][0], ci[t][1], ci[t][2]) / NV];
+          }
+          Skip[f] = !u[1] && !u[2];
+          if (Skip[f]) continue;
+          for (t = 1; t <= 2; t++) {
+            const double *S = Blk + cidx(&sh, ci[t][0], ci[t][1], ci[t][2]);
+            double *P = t == 1 ? P1 : P2;
+            const double RhoInv = 1.0 / S[OB_RHO];
+            for (v = 0; v < NV; v++) P[v] = S[v];
+            for (v = 0; v < 3; v++) P[OB_RHOUX + v] = RhoInv * S[OB_RHOUX + v];
+          }
+          if (c->nOrder == 2) { /* first order where the stencil meets the body */
+            if (!(u[0] && u[1] && u[2]))
+              for (v = 0; v < NV; v++) L[f * NV + v] = P1[v];
+            if (!(u[1] && u[2] && u[3]))
+              for (v = 0; v < NV; v++) R[f * NV + v] = P2[v];
+          }
+          if (u[1] != u[2]) {
+            const int tt = u[1] ? 1 : 2, tb = u[1] ? 2 : 1;
+            double XyzFace_D[3], Ghost[NV];
+            for (v = 0; v < 3; v++)
+              XyzFace_D[v] = 0.5 * ((x0[v] + (ci[tt][v] - 1) * dx[v]) +
+                                    (x0[v] + (ci[tb][v] - 1) * dx[v]));
+            if (u[1]) {
+              ob_body_face_state(L + f * NV, XyzFace_D, s->BodyRho, s->BodyP,
+                                 s->B1rCoef, Ghost);
+              for (v = 0; v < NV; v++) R[f * NV + v] = Ghost[v];
+            } else {
+              ob_body_face_state(R + f * NV, XyzFace_D, s->BodyRho, s->BodyP,
+                                 s->B1rCoef, Ghost);
+              for (v = 0; v < NV; v++) L[f * NV + v] = Ghost[v];
+            }
+          }
+        }
+  }
+}
+
+static void body_zero_fluxes(const ob_config *c, const unsigned char *SkipX,
+                             const unsigned char *SkipY,
+                             const unsigned char *SkipZ, double *FX,
+                             double *FY, double *FZ) {
+  const long nfx = (long)(c->nI + 1) * c->nJ * c->nK;
+  const long nfy = (long)c->nI * (c->nJ + 1) * c->nK;
+  const long nfz = (long)c->nI * c->nJ * (c->nK + 1);
+  long f;
+  int v;
+  for (f = 0; f < nfx; f++)
+    if (SkipX[f]) for (v = 0; v < OB_NFACEVALUE; v++) FX[f * OB_NFACEVALUE + v] = 0;
+  for (f = 0; f < nfy; f++)
+    if (SkipY[f]) for (v = 0; v < OB_NFACEVALUE; v++) FY[f * OB_NFACEVALUE + v] = 0;
+  for (f = 0; f < nfz; f++)
+    if (SkipZ[f]) for (v = 0; v < OB_NFACEVALUE; v++) FZ[f * OB_NFACEVALUE + v] = 0;
+}
+
+/* calc_timestep's mask: body cells get DtMax_CB = 0 and do not enter the
+ * block minimum (src/ModTimeStepControl.f90:158-159,255) */
+static void body_timestep(const ob_session *s, int b, double *DtMax_CB,
+                          double *DtMax_B) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const unsigned char *U = s->Used + (long)b * (block_len(&sh) / NV);
+  double m = 1.7976931348623157e308;
+  int i, j, k;
+  for (k = 1; k <= c->nK; k++)
+    for (j = 1; j <= c->nJ; j++)
+      for (i = 1; i <= c->nI; i++) {
+        const long q = ((long)(k - 1) * c->nJ + (j - 1)) * c->nI + (i - 1);
+        if (!U[cidx(&sh, i, j, k) / NV]) DtMax_CB[q] = 0;
+        else m = fmin2(m, DtMax_CB[q]);
+      }
+  *DtMax_B = m;
+}
+
+/* Face states of block b as the stage forms them (face values of the
+ * current state, then the body's treatment when the session has a body);
+ * Skip (3 arrays, may be NULL) flags faces between two body cells */
+void ob_session_face_values(const ob_session *s, int b, double *LX,
+                            double *RX, double *LY, double *RY, double *LZ,
+                            double *RZ, unsigned char *SkipX,
+                            unsigned char *SkipY, unsigned char *SkipZ) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  ob_calc_face_value(c, s->State + (long)b * block_len(&sh), LX, RX, LY, RY,
+                     LZ, RZ);
+  if (s->Used && block_has_body(s, b)) {
+    const long nfx = (long)(c->nI + 1) * c->nJ * c->nK;
+    const long nfy = (long)c->nI * (c->nJ + 1) * c->nK;
+    const long nfz = (long)c->nI * c->nJ * (c->nK + 1);
+    unsigned char *kx = SkipX ? SkipX : (unsigned char *)calloc(nfx, 1);
+    unsigned char *ky = SkipY ? SkipY : (unsigned char *)calloc(nfy, 1);
+    unsigned char *kz = SkipZ ? SkipZ : (unsigned char *)calloc(nfz, 1);
+    body_faces(s, b, LX, RX, LY, RY, LZ, RZ, kx, ky, kz);
+    if (!SkipX) free(kx);
+    if (!SkipY) free(ky);
+    if (!SkipZ) free(kz);
+  }
 }
 
 /* B0 of block b's faces and its div / curl into the scratch (set_b0_face
@@ -1803,10 +1994,16 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       block_res_change(s, b, rc);
       ob_calc_face_value_rc(c, Blk, CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY,
                             w.LZ, w.RZ);
+      const int body = s->Used && block_has_body(s, b);
+      if (body)
+        body_faces(s, b, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.SkipX,
+                   w.SkipY, w.SkipZ);
       const double *B0Blk = block_b0(s, b, &w);
       e = ob_calc_face_flux_b0(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                                B0Blk ? w.B0X : NULL, B0Blk ? w.B0Y : NULL,
                                B0Blk ? w.B0Z : NULL, w.FX, w.FY, w.FZ);
+      if (body)
+        body_zero_fluxes(c, w.SkipX, w.SkipY, w.SkipZ, w.FX, w.FY, w.FZ);
       if (s->nSlab > 0)
         apply_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
                         w.RZ);
@@ -1818,25 +2015,36 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       }
       /* steady state: the local time step of the first stage's fluxes,
        * before the update (src/ModAdvanceExplicit.f90:184-195) */
-      if (DoUpdate && DoCalcTimestep && c->IsSteadyState && iStage == 1)
+      if (DoUpdate && DoCalcTimestep && c->IsSteadyState && iStage == 1) {
         ob_calc_timestep(c, cs, w.FX, w.FY, w.FZ, s->DtMax_CB + nCell * b,
                          s->DtMax_B + b);
+        if (body) body_timestep(s, b, s->DtMax_CB + nCell * b, s->DtMax_B + b);
+      }
       if (DoUpdate) {
         /* calc_source (src/ModAdvanceExplicit.f90:166) */
         if (c->UseDivbSource)
           ob_calc_source_b0(c, cs, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                             B0Blk, w.B0X, w.B0Y, w.B0Z, w.DivB0, w.CurlB0,
                             s->UseB0Source, w.Src);
+        if (body) memcpy(w.Keep, Blk, sizeof(double) * len);
         ob_update_state(c, iStage, s->Dt, cs, s->DtMax_CB + nCell * b, w.FX,
                         w.FY, w.FZ, c->UseDivbSource ? w.Src : NULL, Blk,
                         Old);
+        if (body) { /* body cells are not updated (:343-345) */
+          const unsigned char *Ub = s->Used + (long)b * (len / NV);
+          long q;
+          for (q = 0; q < len / NV; q++)
+            if (!Ub[q]) memcpy(Blk + q * NV, w.Keep + q * NV, NV * 8);
+        }
       }
       /* time accurate: after the last stage's update (:217-219); the
        * face-speed pass without an update always */
       if (DoCalcTimestep && (!DoUpdate || (!c->IsSteadyState &&
-                                           iStage == c->nStage)))
+                                           iStage == c->nStage))) {
         ob_calc_timestep(c, cs, w.FX, w.FY, w.FZ, s->DtMax_CB + nCell * b,
                          s->DtMax_B + b);
+        if (body) body_timestep(s, b, s->DtMax_CB + nCell * b, s->DtMax_B + b);
+      }
     }
     scratch_free(&w);
   }

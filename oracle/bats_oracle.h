@@ -268,7 +268,30 @@ typedef struct {
    * source (src/ModCalcSource.f90:953-967) */
   const double *B0;
   int UseB0Source;
+  /* inner body (UseBody) with the 'ionosphere' face boundary condition:
+   * Used_GB of every block (nBlock ghosted blocks of 1 = true cell, 0 =
+   * inside the body; NULL: no body), the centre of cell (1,1,1) of each
+   * block (XyzStart_DB, 3 per block, body centre at the origin), the body
+   * density and pressure (FaceState_VI(:, body1_)) and B1rCoef */
+  const unsigned char *Used;
+  const double *XyzStart_DB;
+  double BodyRho, BodyP, B1rCoef;
 } ob_session;
+
+/* set_face, 'ionosphere' type (src/ModUpdateStateFast.f90:1277-1327,
+ * src/ModFaceBoundary.f90:621-648,876-884): the ghost face state of a face
+ * between a true cell and a body cell from the true side's face state;
+ * XyzFace_D is the face centre relative to the body centre */
+void ob_body_face_state(const double *True_V, const double *XyzFace_D,
+                        double BodyRho, double BodyP, double B1rCoef,
+                        double *Ghost_V);
+
+/* face states of block b as a stage forms them (with the body's first
+ * order fallback and ghost face states); Skip flags body-body faces */
+void ob_session_face_values(const ob_session *s, int b, double *LX,
+                            double *RX, double *LY, double *RY, double *LZ,
+                            double *RZ, unsigned char *SkipX,
+                            unsigned char *SkipY, unsigned char *SkipZ);
 
 /* set_global_timestep (src/ModTimeStepControl.f90:306-524) */
 void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit);

@@ -56,7 +56,11 @@ class ObSession(C.Structure):
                 ("nSlab", C.c_int), ("iSlab_IS", C.POINTER(C.c_int)),
                 ("Corr", C.POINTER(C.c_double)),
                 ("LowOrderCrit", C.POINTER(C.c_double)),
-                ("B0", C.POINTER(C.c_double)), ("UseB0Source", C.c_int)]
+                ("B0", C.POINTER(C.c_double)), ("UseB0Source", C.c_int),
+                ("Used", C.POINTER(C.c_ubyte)),
+                ("XyzStart_DB", C.POINTER(C.c_double)),
+                ("BodyRho", C.c_double), ("BodyP", C.c_double),
+                ("B1rCoef", C.c_double)]
 
 
 def lib():
@@ -124,6 +128,11 @@ def lib():
                                        dp, dp]
         L.ob_calc_source_b0.argtypes = [C.POINTER(ObConfig), dp, dp] + \
             [dp] * 6 + [dp] * 6 + [C.c_int, dp]
+        L.ob_body_face_state.argtypes = [dp, dp, C.c_double, C.c_double,
+                                         C.c_double, dp]
+        ub = C.POINTER(C.c_ubyte)
+        L.ob_session_face_values.argtypes = [C.POINTER(ObSession), C.c_int] \
+            + [dp] * 6 + [ub] * 3
         L.ob_pressure_to_energy_cell.argtypes = [C.POINTER(ObConfig), dp]
         L.ob_energy_to_pressure_cell.argtypes = [C.POINTER(ObConfig), dp]
         _LIB = L
@@ -296,6 +305,14 @@ def calc_source_b0(cfg, cell_size, block_state, faces, b0_block, b0_faces,
     return src
 
 
+def body_face_state(true_state, xyz_face, BodyRho, BodyP, B1rCoef):
+    t = np.ascontiguousarray(true_state, dtype=np.float64)
+    x = np.ascontiguousarray(xyz_face, dtype=np.float64)
+    g = np.zeros(8)
+    lib().ob_body_face_state(_dp(t), _dp(x), BodyRho, BodyP, B1rCoef, _dp(g))
+    return g
+
+
 def energy_round_trip(cfg, state):
     s = np.array(state, dtype=np.float64)
     lib().ob_pressure_to_energy_cell(C.byref(cfg), _dp(s))
@@ -391,9 +408,14 @@ class Session:
     """Whole-grid oracle run (BATS_advance-shaped time loop)."""
 
     def __init__(self, grid, cfg, state_vgb, nThread=0, B0_DGB=None,
-                 UseB0Source=True):
+                 UseB0Source=True, Used_GB=None, XyzStart_DB=None,
+                 BodyRho=1.0, BodyP=1.0, B1rCoef=-1.0):
         """B0_DGB: B0 splitting, the static field (nBlock, nKg, nJg, nIg, 3)
-        at the cell centres and ghost cells (same-level grids)."""
+        at the cell centres and ghost cells (same-level grids).  Used_GB:
+        an inner body, the true cells (nBlock, nKg, nJg, nIg) of bool;
+        XyzStart_DB (nBlock, 3) the centre of each block's first cell
+        relative to the body centre; BodyRho, BodyP, B1rCoef the body's
+        face state and radial field coefficient."""
         self.grid = grid
         self.cfg = cfg
         self.nei = np.ascontiguousarray(grid.iNeighbor_EB, dtype=np.int32)
@@ -430,6 +452,16 @@ class Session:
             assert self.B0.shape[-1] == 3 and self.B0.shape[0] == grid.nBlock
             s.B0 = _dp(self.B0)
             s.UseB0Source = int(UseB0Source)
+        if Used_GB is not None:
+            assert self.nSlab == 0, "a body on same-level grids only"
+            self.Used = np.ascontiguousarray(Used_GB, dtype=np.uint8)
+            assert self.Used.shape == self.State.shape[:-1]
+            self.XyzStart = np.ascontiguousarray(XyzStart_DB,
+                                                 dtype=np.float64)
+            assert self.XyzStart.shape == (grid.nBlock, 3)
+            s.Used = self.Used.ctypes.data_as(C.POINTER(C.c_ubyte))
+            s.XyzStart_DB = _dp(self.XyzStart)
+            s.BodyRho, s.BodyP, s.B1rCoef = BodyRho, BodyP, B1rCoef
         if cfg.UseAdaptiveLowOrder:
             sx, sy, sz = face_shapes(cfg)
             self.LowOrderCrit = np.zeros(
@@ -449,6 +481,17 @@ class Session:
 
     def init_timestep(self):
         return lib().ob_init_timestep(C.byref(self.s))
+
+    def face_values(self, b):
+        """Face states (LX, RX, LY, RY, LZ, RZ) of block b (0-based) as the
+        stage forms them, and the body-body face flags (SX, SY, SZ)."""
+        sx, sy, sz = face_shapes(self.cfg)
+        arrs = [np.zeros(s + (8,)) for s in (sx, sx, sy, sy, sz, sz)]
+        skips = [np.zeros(s, np.uint8) for s in (sx, sy, sz)]
+        lib().ob_session_face_values(
+            C.byref(self.s), b, *[_dp(a) for a in arrs],
+            *[a.ctypes.data_as(C.POINTER(C.c_ubyte)) for a in skips])
+        return arrs, skips
 
     def sources(self):
         """Source_VC of every block (nBlock, nK, nJ, nI, 9) from the current
