@@ -79,6 +79,8 @@ ABI_FUNCTIONS = {
     "bats_set_low_order_face": (C.c_int, [_hp]),
     "bats_set_cell_boundary": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_b0": (C.c_int, [_hp, _dp, C.c_int]),
+    "bats_set_body": (C.c_int, [_hp, C.POINTER(C.c_ubyte), _dp, C.c_double,
+                                C.c_double, C.c_double]),
     "bats_set_step_graph": (C.c_int, [_hp, C.c_int]),
     "bats_get_low_order_crit": (C.c_int, [_hp, C.c_int, _dp, _dp, _dp]),
     "bats_upload_state": (C.c_int, [_hp, _dp]),
@@ -350,6 +352,27 @@ class Session:
                            g.nI + 2 * g.nG, 3), b.shape
         self._check(self.lib.bats_set_b0(self.h, _ptr(b), int(UseB0Source)),
                     "set_b0")
+
+    def set_body(self, Used_GB, XyzStart_DB=None, BodyRho=1.0, BodyP=1.0,
+                 B1rCoef=-1.0):
+        """Inner body: Used_GB of shape (nBlock, nKg, nJg, nIg), True at the
+        true cells of every block and its ghost cells, XyzStart_DB
+        (nBlock, 3) the first cell centres relative to the body centre, the
+        body density and pressure of the ghost face state and B1rCoef
+        (None drops the body)."""
+        if Used_GB is None:
+            self._check(self.lib.bats_set_body(self.h, None, None, 0.0, 0.0,
+                                               0.0), "set_body")
+            return
+        g = self.grid
+        u = np.ascontiguousarray(Used_GB, dtype=np.uint8)
+        assert u.shape == (g.nBlock, g.nK + 2 * g.nG, g.nJ + 2 * g.nG,
+                           g.nI + 2 * g.nG), u.shape
+        x = np.ascontiguousarray(XyzStart_DB, dtype=np.float64)
+        assert x.shape == (g.nBlock, 3), x.shape
+        self._check(self.lib.bats_set_body(
+            self.h, _ptr(u, C.c_ubyte), _ptr(x), float(BodyRho), float(BodyP),
+            float(B1rCoef)), "set_body")
 
     def set_step_graph(self, use=True):
         """bats_advance replays one captured hipGraph per time step."""

@@ -5,6 +5,7 @@
 // stage_kernel.hpp and is instantiated in stage_inst.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -659,7 +660,8 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->LY, h->RY, h->LZ, h->RZ, h->FX, h->FY, h->FZ, h->Src,
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
                   h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
-                  h->corr, h->lowCrit, h->dBcState, h->b0F, h->b0C};
+                  h->corr, h->lowCrit, h->dBcState, h->b0F, h->b0C,
+                  h->bodySlot, h->bodyUsed, h->bodyXyz0, h->groupList};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   drop_graphs(h);
@@ -682,12 +684,26 @@ static void drop_b0(bats_handle *h) {
   h->useB0 = 0;
 }
 
+// and the inner body (its masks belong to the old blocks)
+static void drop_body(bats_handle *h) {
+  void *ptrs[] = {h->bodySlot, h->bodyUsed, h->bodyXyz0, h->groupList};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  h->bodySlot = nullptr;
+  h->bodyUsed = nullptr;
+  h->bodyXyz0 = nullptr;
+  h->groupList = nullptr;
+  h->useBody = 0;
+  h->nPlainGroup = h->nBodyGroup = 0;
+}
+
 extern "C" int bats_set_grid(bats_handle *h, const int *iNeighbor_EB,
                              const double *CellSize_DB) {
   if (!h || !iNeighbor_EB || !CellSize_DB) return BATS_EINVAL;
   drop_graphs(h);
   (void)hipStreamSynchronize(h->stream);
   drop_b0(h);
+  drop_body(h);
   const int nB = h->cfg.nBlock;
   for (int t = 0; t < 6 * nB; t++)
     if (iNeighbor_EB[t] < -1 || iNeighbor_EB[t] >= nB) return BATS_EINVAL;
@@ -982,8 +998,9 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
-  // the reference-shaped per-block routines run on FP64 state only
-  if (h->elemSize != 8) return BATS_EINVAL;
+  // the reference-shaped per-block routines run on FP64 state only, and
+  // without the inner body (which the fused stage carries)
+  if (h->elemSize != 8 || h->useBody) return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
 
@@ -1262,6 +1279,9 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   }
   const bool lowUse = low_order_used(c);
   const bool resBeta = res_beta_used(h);
+  // the body variant carries no low order faces, resolution-change beta or
+  // steady state
+  if (h->useBody && (lowUse || resBeta || steady)) return BATS_EINVAL;
   // the surrogate's face pass has no low order weights, one beta and no
   // 8-wave source
   if ((lowUse || resBeta || steady || c.UseDivbSource) &&
@@ -1322,18 +1342,44 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
     const int st = sur_stage(h, a);
     if (st) return st;
   } else {
+    auto launch = [&](const StageArgs &la, int lopt) {
 #define BATS_PICK(NBv, FL)                                                   \
-  (f32 ? launch_stage_##NBv##_##FL##_f32(a, opt, h->stream)                  \
-       : launch_stage_##NBv##_##FL##_f64(a, opt, h->stream))
-  if (NB == 8)
-    e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(8, rusanov)
-      : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(8, linde)
-                                     : BATS_PICK(8, hlld);
-  else
-    e = c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(16, rusanov)
-      : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(16, linde)
-                                     : BATS_PICK(16, hlld);
+  (f32 ? launch_stage_##NBv##_##FL##_f32(la, lopt, h->stream)                \
+       : launch_stage_##NBv##_##FL##_f64(la, lopt, h->stream))
+      if (NB == 8)
+        return c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(8, rusanov)
+             : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(8, linde)
+                                            : BATS_PICK(8, hlld);
+      return c.iFlux == BATS_FLUX_RUSANOV ? BATS_PICK(16, rusanov)
+           : c.iFlux == BATS_FLUX_LINDE   ? BATS_PICK(16, linde)
+                                          : BATS_PICK(16, hlld);
 #undef BATS_PICK
+    };
+    if (h->useBody) {
+      // the groups without a body block run the plain variant, then those
+      // with one the body variant (which also advances the time)
+      const int wg = h->nWorkgroupMax;
+      StageArgs ap = a;
+      ap.groupList = h->groupList;
+      ap.nGroup = h->nPlainGroup;
+      ap.nWorkgroup = wg > 0 ? wg : ap.nGroup;
+      ap.advanceTime = 0;
+      if (ap.nGroup > 0) e = launch(ap, opt);
+      if (e == hipSuccess) {
+        StageArgs ab = a;
+        ab.groupList = h->groupList + h->nPlainGroup;
+        ab.nGroup = h->nBodyGroup;
+        ab.nWorkgroup = wg > 0 ? wg : ab.nGroup;
+        ab.bodySlot = h->bodySlot;
+        ab.used = h->bodyUsed;
+        ab.xyz0 = h->bodyXyz0;
+        ab.bodyRho = h->bodyRho;
+        ab.b1rCoef = h->b1rCoef;
+        e = launch(ab, opt | OPT_BODY);
+      }
+    } else {
+      e = launch(a, opt);
+    }
   }
   HIPCHK(e);
   if (timed) HIPCHK(hipEventRecord(h->evStop[ev], h->stream));
@@ -1557,6 +1603,7 @@ extern "C" int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
   drop_graphs(h);
   (void)hipStreamSynchronize(h->stream);
   drop_b0(h);
+  drop_body(h);
   const int nB = h->cfg.nBlock, NB = h->NB, H = NB / 2;
   for (int t = 0; t < 3 * nB; t++)
     if (!(CellSize_DB[t] > 0.0)) return BATS_EINVAL;
@@ -1856,6 +1903,98 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
   h->b0C = dC;
   h->useB0 = 1;
   h->useB0Src = UseB0Source != 0;
+  return BATS_OK;
+}
+
+// Inner body: the true cells Used_GB (with the ghost cells) and the
+// 'ionosphere' face boundary condition of the fast update
+// (src/ModUpdateStateFast.f90:230-265,343-345,924-937,1277-1327).  Blocks
+// with a body cell among their cells or ghost cells (IsBody_B,
+// src/ModBoundaryGeometry.f90:226) get a mask slot; the groups of blocks
+// holding one run the body variant of the stage kernel.
+extern "C" int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
+                             const double *XyzStart_DB, double BodyRho,
+                             double BodyP, double B1rCoef) {
+  if (!h) return BATS_EINVAL;
+  drop_graphs(h);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (!Used_GB) {
+    drop_body(h);
+    return BATS_OK;
+  }
+  const bats_config &c = h->cfg;
+  // FP64 state, same-level grid, no low order faces, resolution-change
+  // beta or steady state (the body variant carries none of them)
+  if (h->elemSize != 8 || h->nResSlab > 0 || low_order_used(c) ||
+      res_beta_used(h) || c.IsSteadyState || !XyzStart_DB)
+    return BATS_EINVAL;
+  // the ghost pressure floats, so BodyP only has to be a number
+  if (!(BodyRho > 0.0) || !std::isfinite(BodyRho) || !std::isfinite(BodyP) ||
+      !std::isfinite(B1rCoef))
+    return BATS_EINVAL;
+  const int NB = h->NB, g = c.nG, NGc = NB + 2 * g, nB = c.nBlock;
+  const int NG = NB + 4;  // the mask over the block and its 2-cell halo
+  for (long t = 0; t < 3L * nB; t++)
+    if (!std::isfinite(XyzStart_DB[t])) return BATS_EINVAL;
+  const long nMask = (long)NG * NG * NG;
+  std::vector<int> slot(nB, -1);
+  std::vector<unsigned char> masks;
+  for (int b = 0; b < nB; b++) {
+    const unsigned char *U = Used_GB + (long)b * NGc * NGc * NGc;
+    bool any = false;
+    for (long q = 0; q < (long)NGc * NGc * NGc && !any; q++) any = !U[q];
+    if (!any) continue;
+    slot[b] = (int)(masks.size() / nMask);
+    for (int k = -2; k < NB + 2; k++)
+      for (int j = -2; j < NB + 2; j++)
+        for (int i = -2; i < NB + 2; i++)
+          masks.push_back(
+              U[(((long)(k + g) * NGc) + (j + g)) * NGc + (i + g)] ? 1 : 0);
+  }
+  drop_body(h);
+  if (masks.empty()) return BATS_OK;  // no body cell anywhere
+  // work list: groups without a body block first, then those with one
+  const int BPW = kThreads / (NB * NB);
+  const int nGroup = (nB + BPW - 1) / BPW;
+  std::vector<int> plain, body;
+  for (int q = 0; q < nGroup; q++) {
+    bool any = false;
+    for (int b = q * BPW; b < std::min(nB, (q + 1) * BPW); b++)
+      any = any || slot[b] >= 0;
+    (any ? body : plain).push_back(q);
+  }
+  std::vector<int> list(plain);
+  list.insert(list.end(), body.begin(), body.end());
+  int *dSlot = nullptr, *dList = nullptr;
+  unsigned char *dUsed = nullptr;
+  double *dXyz = nullptr;
+  hipError_t e = hipMalloc(&dSlot, sizeof(int) * nB);
+  if (e == hipSuccess) e = hipMalloc(&dList, sizeof(int) * list.size());
+  if (e == hipSuccess) e = hipMalloc(&dUsed, masks.size());
+  if (e == hipSuccess) e = hipMalloc(&dXyz, sizeof(double) * 3 * nB);
+  if (e == hipSuccess)
+    e = hipMemcpy(dSlot, slot.data(), sizeof(int) * nB, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dList, list.data(), sizeof(int) * list.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dUsed, masks.data(), masks.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dXyz, XyzStart_DB, sizeof(double) * 3 * nB,
+                  hipMemcpyHostToDevice);
+  h->bodySlot = dSlot;
+  h->groupList = dList;
+  h->bodyUsed = dUsed;
+  h->bodyXyz0 = dXyz;
+  if (e != hipSuccess) {
+    drop_body(h);
+    return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
+  }
+  h->nPlainGroup = (int)plain.size();
+  h->nBodyGroup = (int)body.size();
+  h->bodyRho = BodyRho;
+  h->b1rCoef = B1rCoef;
+  h->useBody = 1;
   return BATS_OK;
 }
 

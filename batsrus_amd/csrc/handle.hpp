@@ -93,6 +93,18 @@ struct bats_handle {
   // div B0 and curl B0 [7][cell]; dropped by any grid change
   double *b0F, *b0C;
   int useB0, useB0Src;
+  // inner body (bats_set_body): per block the slot of its true cell mask
+  // (-1: none), the masks over block + 2-cell halo, the blocks' first cell
+  // centres relative to the body centre, the body density and B1rCoef; the
+  // work list holds the groups without a body block, then those with one;
+  // dropped by any grid change
+  int useBody;
+  int *bodySlot;
+  unsigned char *bodyUsed;
+  double *bodyXyz0;
+  int *groupList;
+  int nPlainGroup, nBodyGroup;
+  double bodyRho, b1rCoef;
   // learned face-flux surrogate (surrogate.hip), FP32 state only
   SurrogateState sur;
 };

@@ -699,6 +699,37 @@ __device__ __forceinline__ void update_cell(const Phys &ph, const UpdateCtl &u,
   Unew[PP] = limit_pressure(ph, energy_to_pressure(ph, Unew, Unew[PP]));
 }
 
+// Ghost face state of the inner body ('ionosphere' type, set_face in
+// src/ModUpdateStateFast.f90:1277-1327): from the true side's face state
+// T, the density moves toward the body density by at most 10 % of the true
+// value, the pressure floats, the velocity reverses, and 1 - B1rCoef of the
+// radial field (along r, the face centre relative to the body centre) is
+// removed twice over: B1rCoef = -1 reflects B1r, 0 removes it.
+template <typename Real>
+__device__ __forceinline__ void body_ghost(const Real *T, const Real *r,
+                                           Real BodyRho, Real B1rCoef,
+                                           Real *G) {
+  const Real DensityJumpLimit = Real(0.1);
+  const Real d = BodyRho - T[RHO];
+  G[RHO] = T[RHO] + fsign(Real(1), d) *
+                        fmin2(fabsr(d), DensityJumpLimit * T[RHO]);
+  G[MX] = -T[MX];
+  G[MY] = -T[MY];
+  G[MZ] = -T[MZ];
+  G[PP] = T[PP];
+  if (B1rCoef != Real(1)) {
+    const Real Coef = (1 - B1rCoef) / (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const Real BdotR = T[BX] * r[0] + T[BY] * r[1] + T[BZ] * r[2];
+    G[BX] = T[BX] - Coef * r[0] * BdotR;
+    G[BY] = T[BY] - Coef * r[1] * BdotR;
+    G[BZ] = T[BZ] - Coef * r[2] * BdotR;
+  } else {
+    G[BX] = T[BX];
+    G[BY] = T[BY];
+    G[BZ] = T[BZ];
+  }
+}
+
 // B0 of a cell for the 8-wave source with B0 splitting: the B0 of its six
 // faces, its own B0, and div B0 / curl B0 (set_b0_source) when UseB0Source
 template <typename Real> struct CellB0 {

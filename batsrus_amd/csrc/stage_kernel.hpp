@@ -77,6 +77,17 @@ struct StageArgs {
   // div B0, curl B0 ([7][interior cell]); useB0Src: UseB0Source
   const double *b0F, *b0C;
   int useB0Src;
+  // groups of blocks this launch processes (null: groups 0..nGroup-1)
+  const int *groupList;
+  // inner body (OPT_BODY variant, FP64): per block the slot of its true
+  // cell mask (-1: no body cell within 2 cells of the block), the masks
+  // [slot][k+2][j+2][i+2] over the block and its 2-cell halo (1: true
+  // cell), each block's first cell centre relative to the body centre, and
+  // the body density and B1rCoef of the ghost face state
+  const int *bodySlot;
+  const unsigned char *used;
+  const double *xyz0;
+  double bodyRho, b1rCoef;
 };
 
 // the cyclic frame rotation (x, y, z) -> (y, z, x) that turns a y face into
@@ -102,8 +113,10 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 // OPT_STEADY: steady state, every cell with its own time step (dtmaxCB,
 // from this launch's fluxes when doDt).
 // OPT_B0: B0 splitting, the state holds B1 = B - B0 (FP64 only).
+// OPT_BODY: the inner body's true cells and face boundary condition (FP64,
+// launched only on the groups of blocks that reach a body cell).
 enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8,
-       OPT_B0 = 16 };
+       OPT_B0 = 16, OPT_BODY = 32 };
 
 // index of a face of block b in the B0 face arrays (per direction, the
 // reference's LeftState_VX(nI+1,nJ,nK) / _VY(nI,nJ+1,nK) / _VZ(nI,nJ,nK+1)
@@ -219,6 +232,33 @@ __device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T betaL,
                                                B0);
 }
 
+// The inner body's treatment of one face (get_face_x/y/z of the fast
+// update, src/ModUpdateStateFast.f90:924-937): u0..u3 flag the true cells
+// of the stencil f-2 .. f+1 along the normal, P1 / P2 are the primitive
+// states of the cells left and right of the face.  A second-order face
+// value whose stencil meets a body cell becomes first order; a face between
+// a true and a body cell takes the ghost state on the body side (r: the
+// face centre relative to the body centre, in the face's frame).  Returns
+// true for a face between two body cells, whose flux is zero (:237-240).
+template <int LIM, typename T>
+__device__ __forceinline__ bool body_face(bool u0, bool u1, bool u2, bool u3,
+                                          const T *P1, const T *P2, T *L,
+                                          T *R, const T *r, T BodyRho,
+                                          T B1rCoef) {
+  if (!u1 && !u2) return true;
+  if (LIM != LIM_FIRST) {
+    if (!(u0 && u1 && u2))
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) L[v] = P1[v];
+    if (!(u1 && u2 && u3))
+#pragma unroll
+      for (int v = 0; v < NVAR; v++) R[v] = P2[v];
+  }
+  if (u1 && !u2) body_ghost(L, r, BodyRho, B1rCoef, R);
+  else if (u2 && !u1) body_ghost(R, r, BodyRho, B1rCoef, L);
+  return false;
+}
+
 // ---------------------------------------------------------------------
 // fused stage kernel.  T is the storage and face-arithmetic type (double:
 // plain path; float: FP32-state path); the stage update, the pressure fix
@@ -235,6 +275,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   constexpr bool STEADY = (OPT & OPT_STEADY) != 0;
   constexpr bool B0S = (OPT & OPT_B0) != 0;
   static_assert(!B0S || sizeof(T) == 8, "B0 splitting is FP64 only");
+  constexpr bool BODY = (OPT & OPT_BODY) != 0;
+  static_assert(!BODY || (sizeof(T) == 8 && !LOWORD && !RESB && !STEADY),
+                "the body variant is FP64, without low order faces, the "
+                "resolution-change beta or steady state");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -244,8 +288,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // z-face fluxes of the lower and upper face of the cell; with the 8-wave
   // scheme two more slots carry the normal-field face states Bz_L, Bz_R
   constexpr int NFZ = POWELL ? NFS + 2 : NFS;
+  // LDS slots of the x/y faces: the body variant with the 8-wave scheme
+  // keeps the normal-field face states too (the body changes them)
+  constexpr int NFL = (BODY && POWELL) ? NFS + 2 : NFS;
   __shared__ T sP[BPW * NVAR * NR * NS];
-  __shared__ T sF[BPW * 2 * NFS * NFX];
+  __shared__ T sF[BPW * 2 * NFL * NFX];
   __shared__ double sRed[kThreads / 64];
 
   const int tid = threadIdx.x;
@@ -314,7 +361,29 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     R = sum4(c, CORR_BNR);
   };
   auto sFi = [&](int l, int dir, int v, int q) {
-    return ((l * 2 + dir) * NFS + v) * NFX + q;
+    return ((l * 2 + dir) * NFL + v) * NFX + q;
+  };
+  // the body: true cell mask of a block (null: all cells true) and the
+  // flag of its cell (ii, jj, kk), each index in -2 .. NB+1
+  constexpr int NG = NB + 4;
+  auto body_mask = [&](int blk) -> const unsigned char * {
+    if (!BODY) return nullptr;
+    const int sl = a.bodySlot[blk];
+    return sl < 0 ? nullptr : a.used + (long)sl * NG * NG * NG;
+  };
+  auto is_true = [&](const unsigned char *m, int ii, int jj, int kk) {
+    return !m || m[((kk + 2) * NG + (jj + 2)) * NG + (ii + 2)] != 0;
+  };
+  // centre of the face between cells c and c + e_dir of block blk,
+  // relative to the body centre (Xyz_DGB of the two cells averaged)
+  auto face_xyz = [&](int blk, int dir, int ci, int cj, int ck, T *r) {
+    const double *x0 = a.xyz0 + 3 * blk;
+    const int c[3] = {ci, cj, ck};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double h = a.cellsize[3 * blk + d];
+      r[d] = T(0.5 * ((x0[d] + c[d] * h) + (x0[d] + (c[d] + (d == dir)) * h)));
+    }
   };
   // B0 of a face (B0 splitting)
   const long nFaceTot = (long)a.nBlock * (NB + 1) * NB * NB;
@@ -327,7 +396,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // a persistent grid walks the flat work list of block groups; with one
   // workgroup per group the loop runs once
   for (int grp = blockIdx.x; grp < a.nGroup; grp += gridDim.x) {
-    const int blk0 = grp * BPW;
+    const int blk0 = (a.groupList ? a.groupList[grp] : grp) * BPW;
     const int b = blk0 + lb;
     const bool act = b < a.nBlock;
     const int bb = act ? b : 0;
@@ -344,6 +413,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       return RESB ? cell_beta(beta, betaRC, nFaceRC, zLo, zHi, c, NB) : beta;
     };
     const double CellVolume = dxb * dyb * dzb;
+    const unsigned char *Ub = body_mask(bb);
     T InvHalf[3];
     if (POWELL) {
       InvHalf[0] = T(0.5 / dxb);
@@ -394,6 +464,18 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           R[v] = Q2[v] - dr;
         }
         dLk[v] = dl;
+      }
+      if (BODY && Ub) {
+        T r[3];
+        const bool u1 = is_true(Ub, i, j, fz - 1), u2 = is_true(Ub, i, j, fz);
+        if (u1 != u2) face_xyz(bb, 2, i, j, fz - 1, r);
+        if (body_face<LIM>(is_true(Ub, i, j, fz - 2), u1, u2,
+                           is_true(Ub, i, j, fz + 1), Q1, Q2, L, R, r,
+                           T(a.bodyRho), T(a.b1rCoef))) {
+#pragma unroll
+          for (int v = 0; v < NFZ; v++) Fz[v] = T(0);
+          return;
+        }
       }
       if (B0S) {
         T B0f[3];
@@ -500,7 +582,41 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
 #pragma unroll
           for (int d = 0; d < 3; d++) B0r[d] = dir == 0 ? B0f[d] : B0f[(d + 1) % 3];
         }
-        if (LOWORD)
+        bool skip = false;
+        if (BODY) {
+          const unsigned char *m = body_mask(bf);
+          T L[NVAR], R[NVAR];
+          face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
+          if (m) {
+            // stencil flags and face centre in the original frame; the
+            // face centre rotates with the y face
+            bool u[4];
+            for (int t = 0; t < 4; t++)
+              u[t] = dir == 0 ? is_true(m, fp - 2 + t, row, k)
+                              : is_true(m, row, fp - 2 + t, k);
+            T r[3], rr[3];
+            if (u[1] != u[2]) {
+              if (dir == 0) face_xyz(bf, 0, fp - 1, row, k, r);
+              else face_xyz(bf, 1, row, fp - 1, k, r);
+#pragma unroll
+              for (int d = 0; d < 3; d++) rr[d] = dir == 0 ? r[d] : r[(d + 1) % 3];
+            }
+            skip = body_face<LIM>(u[0], u[1], u[2], u[3], c1, c2, L, R, rr,
+                                  T(a.bodyRho), T(a.b1rCoef));
+          }
+          if (skip) {
+#pragma unroll
+            for (int v = 0; v < NFLUX; v++) F[v] = T(0);
+            vdt = T(0);
+          } else {
+            err |= numerical_flux<0, FLUX, false, B0S>(ph, area, L, R, F, vdt,
+                                                       unA, B0r);
+          }
+          if (POWELL) {
+            sF[sFi(lbf, dir, NFS, q)] = skip ? T(0) : L[BX];
+            sF[sFi(lbf, dir, NFS + 1, q)] = skip ? T(0) : R[BX];
+          }
+        } else if (LOWORD)
           err |= face_flux_w<0, FLUX, LIM, B0S>(
               ph, area, bL, bR,
               T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
@@ -535,7 +651,16 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         // when doDt, else from the launch that set dtmaxCB), so it is taken
         // before the update there and after it otherwise
         double dtc = 0.0;
+        // body cells are not updated and hold a zero local time step that
+        // stays out of the block minimum (src/ModUpdateStateFast.f90:343-345,
+        // src/ModTimeStepControl.f90:158-159,255)
+        const bool trueCell = !BODY || is_true(Ub, i, j, k);
         auto cell_dt = [&]() {
+          if (!trueCell) {
+            dtc = 0.0;
+            if (a.dtmaxCB) a.dtmaxCB[off] = 0.0;
+            return;
+          }
           const T Vdt =
               fmax2(sF[sFi(lb, 0, PP, qx0)], sF[sFi(lb, 0, PP, qx0 + 1)]) +
               fmax2(sF[sFi(lb, 1, PP, qy0)], sF[sFi(lb, 1, PP, qy0 + 1)]) +
@@ -548,7 +673,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         UpdateCtl ucc = uc;
         if (STEADY && !a.noUpdate)
           ucc.DtLocal = a.DtFactor * (a.doDt ? dtc : a.dtmaxCB[off]);
-        if (!a.noUpdate) {
+        if (!a.noUpdate && !trueCell) {
+          // the body cell keeps its stage-input state
+#pragma unroll
+          for (int v = 0; v < NVAR; v++) Uout[v * vs + off] = U[v * vs + off];
+        } else if (!a.noUpdate) {
           T Uc[NVAR], Uo[NVAR];
           double Ucd[NVAR], Uod[NVAR], Un[NVAR];
           load_state(Uold, vs, off, Uo);
@@ -559,42 +688,54 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             // cell state
             load_state(U, vs, off, Uc);
             T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
-            // betas of the cells i-1, i, i+1 (x) and j-1, j, j+1 (y)
-            T bx[3] = {beta, beta, beta}, by[3] = {beta, beta, beta};
-            if (RESB) {
-              const int *n6 = a.nei + 6 * b;
-              for (int d = 0; d < 3; d++) {
-                bx[d] = cell_beta(beta, betaRC, nFaceRC, n6[0] < 0, n6[1] < 0,
-                                  i - 1 + d, NB);
-                by[d] = cell_beta(beta, betaRC, nFaceRC, n6[2] < 0, n6[3] < 0,
-                                  j - 1 + d, NB);
+            if (BODY) {
+              // the face states as the body left them, from the face loop
+              LxLo = sF[sFi(lb, 0, NFS, qx0)];
+              RxLo = sF[sFi(lb, 0, NFS + 1, qx0)];
+              LxHi = sF[sFi(lb, 0, NFS, qx0 + 1)];
+              RxHi = sF[sFi(lb, 0, NFS + 1, qx0 + 1)];
+              LyLo = sF[sFi(lb, 1, NFS, qy0)];
+              RyLo = sF[sFi(lb, 1, NFS + 1, qy0)];
+              LyHi = sF[sFi(lb, 1, NFS, qy0 + 1)];
+              RyHi = sF[sFi(lb, 1, NFS + 1, qy0 + 1)];
+            } else {
+              // betas of the cells i-1, i, i+1 (x) and j-1, j, j+1 (y)
+              T bx[3] = {beta, beta, beta}, by[3] = {beta, beta, beta};
+              if (RESB) {
+                const int *n6 = a.nei + 6 * b;
+                for (int d = 0; d < 3; d++) {
+                  bx[d] = cell_beta(beta, betaRC, nFaceRC, n6[0] < 0,
+                                    n6[1] < 0, i - 1 + d, NB);
+                  by[d] = cell_beta(beta, betaRC, nFaceRC, n6[2] < 0,
+                                    n6[3] < 0, j - 1 + d, NB);
+                }
               }
-            }
 #pragma unroll
-            for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
-            if (LOWORD) {
-              face_lr_w<LIM>(bx[0], bx[1],
-                             T(1) - crit[crit_index<NB>(b, 0, i, j, k)],
-                             p[0], p[1], p[2], p[3], LxLo, RxLo);
-              face_lr_w<LIM>(bx[1], bx[2],
-                             T(1) - crit[crit_index<NB>(b, 0, i + 1, j, k)],
-                             p[1], p[2], p[3], p[4], LxHi, RxHi);
-            } else {
-              face_lr<LIM>(bx[0], bx[1], p[0], p[1], p[2], p[3], LxLo, RxLo);
-              face_lr<LIM>(bx[1], bx[2], p[1], p[2], p[3], p[4], LxHi, RxHi);
-            }
+              for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BX, j + 2, i + d)];
+              if (LOWORD) {
+                face_lr_w<LIM>(bx[0], bx[1],
+                               T(1) - crit[crit_index<NB>(b, 0, i, j, k)],
+                               p[0], p[1], p[2], p[3], LxLo, RxLo);
+                face_lr_w<LIM>(
+                    bx[1], bx[2], T(1) - crit[crit_index<NB>(b, 0, i + 1, j, k)],
+                    p[1], p[2], p[3], p[4], LxHi, RxHi);
+              } else {
+                face_lr<LIM>(bx[0], bx[1], p[0], p[1], p[2], p[3], LxLo, RxLo);
+                face_lr<LIM>(bx[1], bx[2], p[1], p[2], p[3], p[4], LxHi, RxHi);
+              }
 #pragma unroll
-            for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
-            if (LOWORD) {
-              face_lr_w<LIM>(by[0], by[1],
-                             T(1) - crit[crit_index<NB>(b, 1, i, j, k)],
-                             p[0], p[1], p[2], p[3], LyLo, RyLo);
-              face_lr_w<LIM>(by[1], by[2],
-                             T(1) - crit[crit_index<NB>(b, 1, i, j + 1, k)],
-                             p[1], p[2], p[3], p[4], LyHi, RyHi);
-            } else {
-              face_lr<LIM>(by[0], by[1], p[0], p[1], p[2], p[3], LyLo, RyLo);
-              face_lr<LIM>(by[1], by[2], p[1], p[2], p[3], p[4], LyHi, RyHi);
+              for (int d = 0; d < 5; d++) p[d] = sP[sPi(lb, BY, j + d, i + 2)];
+              if (LOWORD) {
+                face_lr_w<LIM>(by[0], by[1],
+                               T(1) - crit[crit_index<NB>(b, 1, i, j, k)],
+                               p[0], p[1], p[2], p[3], LyLo, RyLo);
+                face_lr_w<LIM>(
+                    by[1], by[2], T(1) - crit[crit_index<NB>(b, 1, i, j + 1, k)],
+                    p[1], p[2], p[3], p[4], LyHi, RyHi);
+              } else {
+                face_lr<LIM>(by[0], by[1], p[0], p[1], p[2], p[3], LyLo, RyLo);
+                face_lr<LIM>(by[1], by[2], p[1], p[2], p[3], p[4], LyHi, RyHi);
+              }
             }
             if (a.corr) {
               if (i == 0) corrected_bn(b, 0, j, k, LxLo, RxLo);
@@ -934,7 +1075,28 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 4: BATS_K(LM, 20); break;                                             \
   default: BATS_K(LM, 21); break;                                            \
   }
+  // the inner body (FP64): with or without the 8-wave source and B0
+#define BATS_LAUNCH_BODY(LM)                                                 \
+  switch (opt & (OPT_POWELL | OPT_B0)) {                                     \
+  case 0: BATS_K(LM, 32); break;                                             \
+  case 1: BATS_K(LM, 33); break;                                             \
+  case 16: BATS_K(LM, 48); break;                                            \
+  default: BATS_K(LM, 49); break;                                            \
+  }
   if constexpr (sizeof(T) == 8) {
+    if (opt & OPT_BODY) {
+      if (opt & (OPT_LOWORDER | OPT_RESBETA | OPT_STEADY))
+        return hipErrorInvalidValue;
+      switch (a.lim) {
+      case LIM_MINMOD: BATS_LAUNCH_BODY(LIM_MINMOD); break;
+      case LIM_MC: BATS_LAUNCH_BODY(LIM_MC); break;
+      case LIM_MC3: BATS_LAUNCH_BODY(LIM_MC3); break;
+      case LIM_BETA: BATS_LAUNCH_BODY(LIM_BETA); break;
+      case LIM_NO: BATS_LAUNCH_BODY(LIM_NO); break;
+      default: BATS_LAUNCH_BODY(LIM_FIRST); break;
+      }
+      return hipGetLastError();
+    }
     if (opt & OPT_B0) {
       if (opt & (OPT_LOWORDER | OPT_STEADY)) return hipErrorInvalidValue;
       switch (a.lim) {
@@ -948,7 +1110,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
       return hipGetLastError();
     }
   } else {
-    if (opt & OPT_B0) return hipErrorInvalidValue;
+    if (opt & (OPT_B0 | OPT_BODY)) return hipErrorInvalidValue;
   }
   switch (a.lim) {
   case LIM_MINMOD: BATS_LAUNCH3(LIM_MINMOD); break;
@@ -962,6 +1124,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
 #undef BATS_LAUNCH3
 #undef BATS_LAUNCH7
 #undef BATS_LAUNCH_B0
+#undef BATS_LAUNCH_BODY
   return hipGetLastError();
 }
 

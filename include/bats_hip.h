@@ -175,6 +175,29 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
  * (BATS_EINVAL).  A new grid drops B0; B0_DGB = NULL drops it too. */
 int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
 
+/* Inner body (UseBody with the 'ionosphere' face boundary condition, as the
+ * fast update applies it: src/ModUpdateStateFast.f90:230-265,343-345,
+ * 924-937,1277-1327; replaces the body part of set_face_boundary,
+ * src/ModFaceBoundary.f90:621-648,876-884).  Used_GB(MinI:MaxI, MinJ:MaxJ,
+ * MinK:MaxK, nBlock) (Fortran order, one byte per cell, nonzero: true cell)
+ * flags the true cells of every block and its ghost cells (Used_GB of
+ * src/ModGeometry.f90); XyzStart_DB(3, nBlock) is the centre of each
+ * block's first cell relative to the body centre.  On faces whose limiter
+ * stencil meets a body cell the face value becomes first order; a face
+ * between a true and a body cell takes the ghost state: density toward
+ * BodyRho by at most 10 % of the true value, floating pressure, reversed
+ * velocity, B1r times B1rCoef (src/ModFaceBoundary.f90:30, -1 by
+ * default); faces between two body cells carry no flux; body cells are not
+ * updated and hold DtMax_CB = 0 outside the block minimum
+ * (src/ModTimeStepControl.f90:158-159,255).  Works with B0 and the 8-wave
+ * source; FP64 same-level grids only, not with adaptive low order faces,
+ * the resolution-change beta or steady state, and not through the
+ * per-block routines (BATS_EINVAL).  A new grid drops the body;
+ * Used_GB = NULL drops it too. */
+int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
+                  const double *XyzStart_DB, double BodyRho, double BodyP,
+                  double B1rCoef);
+
 /* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41) */
 enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
        BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10 };

@@ -1722,15 +1722,15 @@ void ob_body_face_state(const double *True_V, const double *XyzFace_D,
   }
 }
 
+/* IsBody_B: some cell of the block, ghost cells included, is a body cell
+ * (src/ModBoundaryGeometry.f90:226) */
 static int block_has_body(const ob_session *s, int b) {
-  const ob_config *c = &s->c;
-  ob_shape sh = shape_of(c);
-  const unsigned char *U = s->Used + (long)b * (block_len(&sh) / NV);
-  int i, j, k;
-  for (k = 1; k <= c->nK; k++)
-    for (j = 1; j <= c->nJ; j++)
-      for (i = 1; i <= c->nI; i++)
-        if (!U[cidx(&sh, i, j, k) / NV]) return 1;
+  ob_shape sh = shape_of(&s->c);
+  const long n = block_len(&sh) / NV;
+  const unsigned char *U = s->Used + (long)b * n;
+  long q;
+  for (q = 0; q < n; q++)
+    if (!U[q]) return 1;
   return 0;
 }
 

@@ -66,3 +66,24 @@ def rel_err_vec(a, b):
              [4, 5, 6], [7]]
     scale = np.array([max(mx[g].max(), 1e-300) for g in group])
     return (np.abs(a - b).max(axis=1) / scale).max()
+
+
+def body_mask(grid: BlockGrid, rBody, center=(0.5, 0.5, 0.5)):
+    """True cells outside a sphere of radius rBody at `center`, over every
+    block and its ghost cells (cell centres of the ghosted blocks), shape
+    (nBlock, nKg, nJg, nIg), and each block's first cell centre relative
+    to the sphere's centre, shape (nBlock, 3)."""
+    g = grid.nG
+    n = np.array([grid.nI, grid.nJ, grid.nK])
+    dx = np.asarray(grid.CellSize_DB[0])
+    org = grid.xyzMin[None, :] + grid.iRoot_DB * n[None, :] * dx[None, :]
+    c = np.asarray(center, dtype=np.float64)
+    ax = [org[:, d, None] + (np.arange(-g, n[d] + g) + 0.5)[None, :] * dx[d]
+          - c[d] for d in range(3)]
+    x = ax[0][:, None, None, :]
+    y = ax[1][:, None, :, None]
+    z = ax[2][:, :, None, None]
+    used = (x * x + y * y + z * z) > rBody ** 2
+    xyz0 = org + 0.5 * dx[None, :] - c[None, :]
+    return np.ascontiguousarray(used), np.ascontiguousarray(xyz0)
+

@@ -18,6 +18,7 @@ import pytest
 
 import pyoracle as po
 from batsrus_amd.grid import BlockGrid
+from helpers import body_mask
 
 RNG = np.random.default_rng(20261204)
 
@@ -51,21 +52,12 @@ def test_ghost_face_state(brho):
 
 def _body_setup(nRoot=(2, 2, 2), nCell=8, rBody=0.18, center=(0.5, 0.5, 0.5)):
     """A grid on [0,1]^3 with float sides, the true cells outside a sphere
-    at `center`, ghost cells included (cell centres of the ghosted blocks),
-    and each block's first cell centre relative to the body centre."""
+    at `center` (ghost cells included) and each block's first cell centre
+    relative to the sphere's centre."""
     grid = BlockGrid(nRoot=nRoot, nCell=(nCell,) * 3,
                      periodic=(False, False, False), order="morton")
-    g = grid.nG
-    dx = grid.CellSize_DB[0]
-    org = grid.xyzMin[None, :] + grid.iRoot_DB * nCell * dx[None, :]
-    idx = np.arange(-g, nCell + g) + 0.5
-    c = np.asarray(center)
-    x = org[:, 0, None, None, None] + idx[None, None, None, :] * dx[0] - c[0]
-    y = org[:, 1, None, None, None] + idx[None, None, :, None] * dx[1] - c[1]
-    z = org[:, 2, None, None, None] + idx[None, :, None, None] * dx[2] - c[2]
-    used = (x * x + y * y + z * z) > rBody ** 2
-    xyz0 = org + 0.5 * dx[None, :] - c[None, :]
-    return grid, np.ascontiguousarray(used), np.ascontiguousarray(xyz0)
+    used, xyz0 = body_mask(grid, rBody, center)
+    return grid, used, xyz0
 
 
 def _run(grid, cfg, U, nstep, **body):
@@ -192,3 +184,44 @@ def test_face_states_next_to_the_body():
                 np.testing.assert_allclose(L, exp_L, rtol=0, atol=1e-15)
                 np.testing.assert_allclose(R, exp_R, rtol=0, atol=1e-15)
     assert checked == {"skip", "ghostR", "ghostL", "first"}
+
+
+def test_body_in_ghost_cells_only():
+    """A block whose own cells are all true but whose ghost cells reach a
+    body cell of its neighbour is a body block (IsBody_B covers the ghost
+    cells): its face next to that cell takes the ghost face state."""
+    dx = 1.0 / 16
+    c = (0.5 - 0.5 * dx, 0.25 + 0.5 * dx, 0.25 + 0.5 * dx)
+    grid, used, xyz0 = _body_setup(nCell=8, rBody=0.3 * dx, center=c)
+    g = grid.nG
+    inner_true = used[:, g:-g, g:-g, g:-g].all(axis=(1, 2, 3))
+    ghost_body = ~used.all(axis=(1, 2, 3))
+    blocks = np.nonzero(inner_true & ghost_body)[0]
+    assert len(blocks) >= 1
+    cfg = po.config_from_dict(dict(flux="rusanov", limiter="mc", beta=1.5),
+                              grid)
+    S = grid.soa_to_state_vgb(_smooth(grid))
+    grid.fill_ghosts(S)
+    o = po.Session(grid, cfg, S, Used_GB=used, XyzStart_DB=xyz0, BodyRho=3.0,
+                   BodyP=2.0, B1rCoef=-1.0)
+    hits = 0
+    for b in blocks:
+        (LX, RX, *_), _ = o.face_values(int(b))
+        plain = po.calc_face_value(cfg, S[b])
+        u = used[b]
+        for k in range(grid.nI):
+            for j in range(grid.nI):
+                if u[k + g, j + g, g - 1]:
+                    continue
+                # face 0 of this row: a body ghost cell on its left
+                R = plain[1][k, j, 0] if u[k + g, j + g, g:g + 2].all() \
+                    else None
+                xf = xyz0[b] + np.array([-0.5, j, k]) * dx
+                exp = po.body_face_state(RX[k, j, 0], xf, 3.0, 2.0, -1.0)
+                np.testing.assert_allclose(LX[k, j, 0], exp, rtol=0,
+                                           atol=1e-15)
+                if R is not None:
+                    np.testing.assert_array_equal(RX[k, j, 0], R)
+                assert np.abs(LX[k, j, 0] - plain[0][k, j, 0]).max() > 1e-3
+                hits += 1
+    assert hits >= 1
