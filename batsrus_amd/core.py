@@ -110,6 +110,7 @@ ABI_FUNCTIONS = {
     "bats_set_runs": (C.c_int, [_hp, C.c_int, _ip]),
     "bats_get_run_times": (C.c_int, [_hp, _dp, _dp, _ip]),
     "bats_set_persistent_grid": (C.c_int, [_hp, C.c_int]),
+    "bats_set_stage_shape": (C.c_int, [_hp, C.c_int, C.c_int]),
     "bats_mlp_create": (C.c_int, [C.POINTER(_hp)]),
     "bats_mlp_free": (C.c_int, [_hp]),
     "bats_mlp_set_params": (C.c_int, [_hp, _fp]),
@@ -509,6 +510,15 @@ class Session:
     def set_persistent_grid(self, nWorkgroup):
         self._check(self.lib.bats_set_persistent_grid(self.h, nWorkgroup),
                     "set_persistent_grid")
+
+    def set_stage_shape(self, CellsPerThread=None, UseXcdOrder=False):
+        """Cells per thread of the stage kernel (NB, NB/2 or NB/4; None:
+        NB) and the XCD-contiguous work order (the same bits; a split
+        rounds its part-boundary faces within an ulp)."""
+        cpt = self.grid.nI if CellsPerThread is None else int(CellsPerThread)
+        self._check(self.lib.bats_set_stage_shape(self.h, cpt,
+                                                  int(bool(UseXcdOrder))),
+                    "set_stage_shape")
 
     def set_surrogate(self, model, mode="network", BetaMin=0.0, JumpTol=0.05):
         """Gated face fluxes on an FP32 handle: mode 'network' (the trained

@@ -468,6 +468,13 @@ k_global_dt(const double *dtmaxB, const int *runFirst, double *scalAll,
   }
 }
 
+// fill n doubles with v (a kernel rather than a memset, so that a captured
+// step graph holds no memset node)
+__global__ void __launch_bounds__(256) k_fill(double *p, long n, double v) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
 template <typename T>
 __global__ void k_check_nan(const T *U, long n, int *err) {
   int bad = 0;
@@ -1264,7 +1271,10 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   a.advanceTime = advanceTime && !steady;
   const int NB = h->NB;
   a.nGroup = (c.nBlock + kThreads / (NB * NB) - 1) / (kThreads / (NB * NB));
-  a.nWorkgroup = h->nWorkgroupMax > 0 ? h->nWorkgroupMax : a.nGroup;
+  a.splitShift = h->splitShift;
+  a.xcdOrder = h->xcdOrder;
+  a.nWorkgroup = h->nWorkgroupMax > 0 ? h->nWorkgroupMax
+                                      : a.nGroup << a.splitShift;
   int ev = -1;
   if (timed) {
     ev = h->nTimed++;
@@ -1342,6 +1352,12 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
     const int st = sur_stage(h, a);
     if (st) return st;
   } else {
+    // the parts of a split block meet in an atomic minimum of dtmaxB
+    if (a.doDt && a.splitShift) {
+      const long n = c.nBlock;
+      hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                         0, h->stream, h->dtmaxB, n, 1.7976931348623157e308);
+    }
     auto launch = [&](const StageArgs &la, int lopt) {
 #define BATS_PICK(NBv, FL)                                                   \
   (f32 ? launch_stage_##NBv##_##FL##_f32(la, lopt, h->stream)                \
@@ -1362,14 +1378,14 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
       StageArgs ap = a;
       ap.groupList = h->groupList;
       ap.nGroup = h->nPlainGroup;
-      ap.nWorkgroup = wg > 0 ? wg : ap.nGroup;
+      ap.nWorkgroup = wg > 0 ? wg : ap.nGroup << ap.splitShift;
       ap.advanceTime = 0;
       if (ap.nGroup > 0) e = launch(ap, opt);
       if (e == hipSuccess) {
         StageArgs ab = a;
         ab.groupList = h->groupList + h->nPlainGroup;
         ab.nGroup = h->nBodyGroup;
-        ab.nWorkgroup = wg > 0 ? wg : ab.nGroup;
+        ab.nWorkgroup = wg > 0 ? wg : ab.nGroup << ab.splitShift;
         ab.bodySlot = h->bodySlot;
         ab.used = h->bodyUsed;
         ab.xyz0 = h->bodyXyz0;
@@ -1573,6 +1589,22 @@ extern "C" int bats_set_persistent_grid(bats_handle *h, int nWorkgroup) {
   if (!h || nWorkgroup < 0) return BATS_EINVAL;
   drop_graphs(h);
   h->nWorkgroupMax = nWorkgroup;
+  return BATS_OK;
+}
+
+// cells per thread of the stage kernel (NB, NB/2 or NB/4 planes of each
+// column per work item) and the XCD-contiguous work order (the same bits;
+// a split rounds the part-boundary faces within an ulp)
+extern "C" int bats_set_stage_shape(bats_handle *h, int CellsPerThread,
+                                    int UseXcdOrder) {
+  if (!h) return BATS_EINVAL;
+  int shift = -1;
+  for (int s = 0; s <= 2; s++)
+    if (CellsPerThread == (h->NB >> s)) shift = s;
+  if (shift < 0 || (UseXcdOrder != 0 && UseXcdOrder != 1)) return BATS_EINVAL;
+  drop_graphs(h);
+  h->splitShift = shift;
+  h->xcdOrder = UseXcdOrder;
   return BATS_OK;
 }
 

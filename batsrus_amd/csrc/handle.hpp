@@ -54,6 +54,9 @@ struct bats_handle {
   int *runFirst;     // nRun+1 block offsets
   int *runOfBlock;   // nBlock
   int nWorkgroupMax; // > 0: persistent grid of at most this many workgroups
+  int splitShift;    // each block's planes split over 2^splitShift work
+                     // items (cells per thread NB >> splitShift)
+  int xcdOrder;      // XCD-contiguous order of the work items
   int *err;
   double *partial;
   double *LX, *RX, *LY, *RY, *LZ, *RZ, *FX, *FY, *FZ;

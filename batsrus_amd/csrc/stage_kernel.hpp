@@ -79,6 +79,11 @@ struct StageArgs {
   int useB0Src;
   // groups of blocks this launch processes (null: groups 0..nGroup-1)
   const int *groupList;
+  // launch shape: each block group splits into 2^splitShift work items of
+  // NB >> splitShift planes (cells per thread); xcdOrder hands each XCD a
+  // contiguous range of work items (workgroups go round-robin over the 8
+  // XCDs), so neighbouring blocks share an L2
+  int splitShift, xcdOrder;
   // inner body (OPT_BODY variant, FP64): per block the slot of its true
   // cell mask (-1: no body cell within 2 cells of the block), the masks
   // [slot][k+2][j+2][i+2] over the block and its 2-cell halo (1: true
@@ -201,6 +206,14 @@ __device__ __forceinline__ void load_state(const T *__restrict__ U, long vs,
                                            long off, T *q) {
 #pragma unroll
   for (int v = 0; v < NVAR; v++) q[v] = U[v * vs + off];
+}
+
+// work item of launch slot `item` of n: with the XCD order, the slots that
+// land on XCD x (item % 8 == x) take the x-th contiguous eighth of the
+// items (a bijection of [0, n))
+__device__ __forceinline__ int xcd_item(int item, int n) {
+  const int x = item & 7, r = item >> 3, q = n >> 3, rem = n & 7;
+  return x * q + (x < rem ? x : rem) + r;
 }
 
 __device__ __forceinline__ double wave_min(double x) {
@@ -395,7 +408,13 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
 
   // a persistent grid walks the flat work list of block groups; with one
   // workgroup per group the loop runs once
-  for (int grp = blockIdx.x; grp < a.nGroup; grp += gridDim.x) {
+  // a work item is one part (planes k0 .. k1-1) of one group of blocks
+  const int nItem = a.nGroup << a.splitShift;
+  for (int item = blockIdx.x; item < nItem; item += gridDim.x) {
+    const int w = a.xcdOrder ? xcd_item(item, nItem) : item;
+    const int grp = w >> a.splitShift;
+    const int kLen = NB >> a.splitShift;
+    const int k0 = (w & ((1 << a.splitShift) - 1)) * kLen, k1 = k0 + kLen;
     const int blk0 = (a.groupList ? a.groupList[grp] : grp) * BPW;
     const int b = blk0 + lb;
     const bool act = b < a.nBlock;
@@ -456,13 +475,15 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       for (int v = 0; v < NVAR; v++) {
         T dl, dr;
         slopes<LIM>(bz, Q1[v], Q2[v], Q3[v], dl, dr);
-        if (LOWORD) {
-          L[v] = Q1[v] + dLk[v] * w;
-          R[v] = Q2[v] - dr * w;
-        } else {
-          L[v] = Q1[v] + dLk[v];
-          R[v] = Q2[v] - dr;
+        {
+          // the left state adds the slope carried from the previous face
+          // as a rounded value (no FMA with the product that formed it,
+          // as the oracle rounds it), whether the slope comes from the
+          // window's prologue or from the previous plane
+#pragma clang fp contract(off)
+          L[v] = LOWORD ? Q1[v] + dLk[v] * w : Q1[v] + dLk[v];
         }
+        R[v] = LOWORD ? Q2[v] - dr * w : Q2[v] - dr;
         dLk[v] = dl;
       }
       if (BODY && Ub) {
@@ -492,21 +513,21 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
     };
 
     if (act) {
-      // prologue: slopes of cell -1 and the face below cell 0
+      // prologue: slopes of cell k0-1 and the face below cell k0
       T Pm2[NVAR];
-      load_prim(-2, Pm2);
-      load_prim(-1, Q1);
-      load_prim(0, Q2);
-      load_prim(1, Q3);
-      const T bz = betaZ(-1);
+      load_prim(k0 - 2, Pm2);
+      load_prim(k0 - 1, Q1);
+      load_prim(k0, Q2);
+      load_prim(k0 + 1, Q3);
+      const T bz = betaZ(k0 - 1);
 #pragma unroll
       for (int v = 0; v < NVAR; v++) {
         T dl, dr;
         slopes<LIM>(bz, Pm2[v], Q1[v], Q2[v], dl, dr);
         dLk[v] = dl;
       }
-      zface(FzLo, 0);
-      if (a.corr) {
+      zface(FzLo, k0);
+      if (a.corr && k0 == 0) {
         corrected(bb, 4, i, j, FzLo);
         if (POWELL) corrected_bn(bb, 4, i, j, FzLo[NFZ - 2], FzLo[NFZ - 1]);
       }
@@ -514,7 +535,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
       for (int v = 0; v < NVAR; v++) { Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
     }
 
-    for (int k = 0; k < NB; k++) {
+    for (int k = k0; k < k1; k++) {
       // z face k+1 of this column
       if (act) {
         load_prim(k + 2, Q3);
@@ -812,7 +833,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         constexpr int WPB = NC >= 64 ? NC / 64 : 1;  // waves per block
         double r = sRed[(lb * NC) / 64];
         for (int q = 1; q < WPB; q++) r = fmin2(r, sRed[(lb * NC) / 64 + q]);
-        a.dtmaxB[b] = r;
+        // the parts of a split block meet in an atomic minimum over the
+        // bit patterns (an order-preserving map for non-negative doubles;
+        // the launcher fills dtmaxB with DBL_MAX first)
+        if (a.splitShift)
+          atomicMin(reinterpret_cast<unsigned long long *>(a.dtmaxB + b),
+                    (unsigned long long)__double_as_longlong(r));
+        else
+          a.dtmaxB[b] = r;
       }
     }
     __syncthreads();  // LDS is reused by the next group
@@ -1031,7 +1059,8 @@ inline hipError_t launch_amr_fineflux(const AmrArgs &a, hipStream_t stream) {
 template <int NB, int FLUX, typename T>
 inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
                                       hipStream_t stream) {
-  const dim3 grid(a.nGroup < a.nWorkgroup ? a.nGroup : a.nWorkgroup),
+  const int nItem = a.nGroup << a.splitShift;
+  const dim3 grid(nItem < a.nWorkgroup ? nItem : a.nWorkgroup),
       block(kThreads);
 #define BATS_K(LM, O)                                                        \
   hipLaunchKernelGGL((k_stage<NB, FLUX, LM, O, T>), grid, block, 0, stream, a)

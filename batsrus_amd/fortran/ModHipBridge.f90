@@ -46,6 +46,7 @@ module ModHipBridge
        bats_advance, bats_init_timestep, bats_get_time, bats_synchronize, &
        bats_check_nan, bats_integrate_domain, bats_last_stage_kernel_ms, &
        bats_set_runs, bats_get_run_times, bats_set_persistent_grid, &
+       bats_set_stage_shape, &
        bats_set_amr_grid, bats_set_low_order_face, bats_get_low_order_crit, &
        bats_set_step_graph, bats_set_cell_boundary, &
        bats_mlp_create, bats_mlp_free, bats_mlp_set_params, &
@@ -297,6 +298,12 @@ module ModHipBridge
        type(c_ptr), value:: Handle
        integer(c_int), value:: nWorkgroup
      end function bats_set_persistent_grid
+     integer(c_int) function bats_set_stage_shape(Handle, CellsPerThread, &
+          UseXcdOrder) bind(C, name="bats_set_stage_shape")
+       import:: c_int, c_ptr
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: CellsPerThread, UseXcdOrder
+     end function bats_set_stage_shape
      integer(c_int) function bats_set_amr_grid(Handle, iNeighbor_EB, &
           CellSize_DB, nSlab, iSlab_IS) bind(C, name="bats_set_amr_grid")
        import:: c_int, c_ptr, c_double

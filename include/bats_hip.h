@@ -380,6 +380,17 @@ int bats_get_run_times(bats_handle *h, double *Dt_R, double *tSimulation_R,
  * group of blocks; n > 0 launches at most n workgroups that walk the flat
  * block work list persistently. */
 int bats_set_persistent_grid(bats_handle *h, int nWorkgroup);
+/* Cells per thread of the stage kernel and the order of its work items
+ * (the reference fixes its loop shape at compile time: Config.pl -g and
+ * the OpenACC gang/vector loops of src/ModUpdateStateFast.f90:219-311).
+ * CellsPerThread = NB (default), NB/2 or NB/4: each block's columns are
+ * split into that many planes per work item, so small grids fill more of
+ * the chip.  UseXcdOrder = 1 hands each of the 8 XCDs a contiguous range
+ * of the work list, so neighbouring blocks share an L2.  The work order
+ * gives the same bits; a split block evaluates the z face at each part
+ * boundary in its window prologue, which rounds within an ulp of the
+ * unsplit kernel's face there.  Other values return BATS_EINVAL. */
+int bats_set_stage_shape(bats_handle *h, int CellsPerThread, int UseXcdOrder);
 /* ------------------------------------------------------------------------
  * Learned face-flux surrogate.  No counterpart in the reference.  A small
  * network (16 -> 128 -> 128 -> 64 -> 8, SiLU; FP32 master weights, BF16

@@ -4,7 +4,8 @@ through its JSON file."""
 import os
 import tempfile
 
-from batsrus_amd.autotune import Autotuner, candidates, workload_key
+from batsrus_amd.autotune import (Autotuner, candidates, domain_key,
+                                  workload_key)
 from batsrus_amd.grid import BlockGrid
 
 
@@ -16,11 +17,18 @@ def test_keys_and_candidates():
     assert k != workload_key(g8, dict(flux="hlld", precision="f32"))
     assert k != workload_key(g8, dict(flux="hlld"), nRun=3)
     assert k == workload_key(g8, dict(flux="hlld", beta=1.2))
-    c = candidates(g8)                 # 128 groups: no persistent size fits
-    assert {x["nWorkgroup"] for x in c} == {0}
+    c = candidates(g8)                 # 128 groups: 128, 256, 512 items
+    assert {(x["cells_per_thread"], x["nWorkgroup"]) for x in c} == {
+        (8, 0), (4, 0), (2, 0), (2, 256)}
     assert {x["graph"] for x in c} == {False, True}
+    assert {x["xcd"] for x in c} == {False, True}
+    assert len(c) == len({tuple(sorted(x.items())) for x in c})
     big = BlockGrid(nRoot=(16, 16, 16), nCell=(8, 8, 8))   # 1024 groups
-    assert {x["nWorkgroup"] for x in candidates(big)} == {0, 256, 512}
+    assert {x["nWorkgroup"] for x in candidates(big)
+            if x["cells_per_thread"] == 8} == {0, 256, 512}
+    assert {x["cells_per_thread"] for x in candidates(g16)} == {16, 8, 4}
+    assert domain_key(8 ** 6, dict(flux="hlld")) != \
+        domain_key(8 ** 6, dict(flux="hlld", precision="f32"))
 
 
 def test_table_round_trip():
