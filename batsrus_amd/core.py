@@ -62,7 +62,7 @@ class BatsConfig(C.Structure):
                 ("BetaLimiterResChange", C.c_double),
                 ("nFaceLimiterResChange", C.c_int),
                 ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double),
-                ("IsSteadyState", C.c_int)]
+                ("IsSteadyState", C.c_int), ("UseTvdResChange", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -178,7 +178,7 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
                        nLowOrder=1, pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0,
                        BetaLimiterResChange=2.0, nFaceLimiterResChange=2,
                        UseOutflowPressure=False, pOutflow=0.0,
-                       IsSteadyState=False)
+                       IsSteadyState=False, UseTvdResChange=False)
 
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
@@ -222,6 +222,7 @@ def make_config(grid: BlockGrid, device=0, **scheme):
     c.UseOutflowPressure = int(s["UseOutflowPressure"])
     c.pOutflow = s["pOutflow"]
     c.IsSteadyState = int(s["IsSteadyState"])
+    c.UseTvdResChange = int(s["UseTvdResChange"])
     return c
 
 

@@ -97,6 +97,8 @@ module ModHipBridge
      real(c_double):: pOutflow = 0.0_c_double
      ! #TIMEACCURATE F: steady state with per-cell time steps
      integer(c_int):: IsSteadyState = 0
+     ! #RESOLUTIONCHANGE UseTvdResChange (first-order prolongation)
+     integer(c_int):: UseTvdResChange = 0
   end type bats_config
 
   interface

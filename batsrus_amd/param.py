@@ -172,7 +172,7 @@ def read_param(text):
         elif name == "#RESOLUTIONCHANGE":
             if _logical(nxt()):
                 raise ValueError("UseAccurateResChange is not supported")
-            _logical(nxt())  # UseTvdResChange: not applied
+            s["UseTvdResChange"] = _logical(nxt())
             s["BetaLimiterResChange"] = float(nxt())
             s["nFaceLimiterResChange"] = int(nxt())
         elif name == "#ADAPTIVELOWORDER":

@@ -118,6 +118,17 @@ typedef struct bats_config {
    * per-block routines follow the reference's order: calc_timestep before
    * update_state at stage 1. */
   int IsSteadyState;
+  /* #RESOLUTIONCHANGE UseTvdResChange with first-order prolongation (the
+   * reference's default treatment at resolution changes, nOrderProlong =
+   * 1): the fine block's ghost layers hold the two coarse cells behind the
+   * face, the coarse block's first ghost layer is restricted from the fine
+   * cells at the face and moved to the coarse cell distance
+   * (correct_monotone_restrict, src/ModFaceValue.f90:233-408), and the
+   * fine block's face values at the coarse side and one face in come from
+   * tvd_reschange (:680-852,2269-2343).  Two-level grids, fused entry
+   * points, no 8-wave source, low order faces or surrogate (BATS_EINVAL);
+   * 0 keeps this project's second-order prolongation. */
+  int UseTvdResChange;
 } bats_config;
 
 typedef struct bats_handle bats_handle;

@@ -1465,7 +1465,39 @@ void ob_exchange_amr(const ob_config *c, int nBlock, int nSlab,
           int x, y, z;
           double val[NV];
           side_xyz(d, hi ? NB + L : -1 - L, p, q, &x, &y, &z);
-          if (kind == 1) { /* restriction from the fine neighbour */
+          if (kind == 1 && c->UseTvdResChange) {
+            /* face restriction (DoRestrictFace with nOrderProlong = 1,
+             * src/ModMessagePass.f90:105): the mean of the 2x2 fine cells
+             * of the fine layer nearest to the ghost layer; layer 0 is
+             * then moved to the coarse distance by correct_monotone_restrict
+             * when the face values are formed.  (Layer 1 enters no result:
+             * the coarse face it serves takes the fine fluxes.) */
+            const int F = sl[3 + (p / H) + 2 * (q / H)];
+            const double *Fb = State + (long)F * len;
+            const int nf = hi ? 2 * L : NB - 1 - 2 * L;
+            int bq, cp;
+            for (v = 0; v < NV; v++) val[v] = 0.0;
+            for (bq = 0; bq < 2; bq++)
+              for (cp = 0; cp < 2; cp++) {
+                int fx, fy, fz;
+                side_xyz(d, nf, 2 * (p % H) + cp, 2 * (q % H) + bq, &fx, &fy,
+                         &fz);
+                const double *src = Fb + cidx(&s, fx + 1, fy + 1, fz + 1);
+                for (v = 0; v < NV; v++) val[v] += src[v];
+              }
+            for (v = 0; v < NV; v++) val[v] = 0.25 * val[v];
+          } else if (kind == 0 && c->UseTvdResChange) {
+            /* first-order prolongation: the coarse cell itself; with the
+             * second-order scheme the two ghost layers take the two coarse
+             * layers behind the face (DoTwoCoarseLayers, :108-109) */
+            const int C = sl[3], p0 = sl[4], q0 = sl[5];
+            const int Lc = c->nOrder > 1 ? L : 0;
+            int cx, cy, cz;
+            side_xyz(d, hi ? Lc : NB - 1 - Lc, p0 + p / 2, q0 + q / 2, &cx,
+                     &cy, &cz);
+            memcpy(val, State + (long)C * len + cidx(&s, cx + 1, cy + 1, cz + 1),
+                   NV * 8);
+          } else if (kind == 1) { /* restriction from the fine neighbour */
             const int F = sl[3 + (p / H) + 2 * (q / H)];
             const double *Fb = State + (long)F * len;
             int a, bq, cp;
@@ -1515,6 +1547,175 @@ void ob_exchange_amr(const ob_config *c, int nBlock, int nSlab,
           }
           memcpy(Blk + cidx(&s, x + 1, y + 1, z + 1), val, NV * 8);
         }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* TVD face values at resolution changes (UseTvdResChange)                 */
+/* ---------------------------------------------------------------------- */
+static const double cThird = 1.0 / 3.0, cTwoThird = 2.0 / 3.0,
+                    cSixth = 1.0 / 6.0;
+
+/* src/ModFaceValue.f90:287-294 (one ghost cell, primitive variables) */
+void ob_correct_monotone_restrict(const double *C, double *G) {
+  int v;
+  for (v = 0; v < NV; v++) G[v] = G[v] + cThird * (G[v] - C[v]);
+  /* where(DefaultState_V > 0): density and pressure */
+  G[OB_RHO] = fmax2(G[OB_RHO], 1e-30);
+  G[OB_P] = fmax2(G[OB_P], 1e-30);
+}
+
+/* src/ModFaceValue.f90:2269-2343 */
+void ob_tvd_reschange(const ob_config *c, const double *C2, const double *C1,
+                      const double *F1, const double *F2, double *CtoF,
+                      double *FtoC, double *FF) {
+  const double Beta = fmin2(c->BetaLimiterResChange, c->LimiterBeta);
+  int v, f;
+  for (v = 0; v < NV; v++) {
+    const double Avg =
+        0.25 * (((F1[0 * NV + v] + F1[1 * NV + v]) + F1[2 * NV + v]) +
+                F1[3 * NV + v]);
+    const double Grad = Avg - C1[v];
+    const double Sgn = fsign(1.0, Grad);
+    const double dC = C1[v] - C2[v];
+    const double Ltd =
+        Sgn * fmax2(0.0, fmin2(fmin2(Beta * cTwoThird * fabs(Grad),
+                                     Beta * 0.5 * Sgn * dC),
+                               cThird * fabs(Grad) + 0.25 * Sgn * dC));
+    for (f = 0; f < 4; f++) {
+      const double t = F1[f * NV + v] - Avg;
+      CtoF[f * NV + v] = C1[v] + Ltd + fsign(fmin2(fabs(Ltd), fabs(t)), t);
+    }
+    for (f = 0; f < 4; f++) {
+      const double dF = F2[f * NV + v] - F1[f * NV + v];
+      const double Ltd2 =
+          Sgn * fmax2(0.0,
+                      fmin2(fmin2(Sgn * (F1[f * NV + v] - C1[v]),
+                                  Beta * cThird * fabs(Grad)),
+                            fmin2(Beta * 0.5 * Sgn * dF,
+                                  cSixth * fabs(Grad) + 0.25 * Sgn * dF)));
+      FtoC[f * NV + v] = F1[f * NV + v] - Ltd2;
+      FF[f * NV + v] = F1[f * NV + v] + Ltd2;
+    }
+  }
+}
+
+static void to_prim(const double *S, double *P) {
+  const double RhoInv = 1.0 / S[OB_RHO];
+  int v;
+  for (v = 0; v < NV; v++) P[v] = S[v];
+  for (v = 0; v < 3; v++) P[OB_RHOUX + v] = RhoInv * S[OB_RHOUX + v];
+}
+
+/* 1-based cell (i, j, k) of a side's (normal, t1, t2) coordinates */
+static inline void side_ijk(int d, int n, int t1, int t2, int *i, int *j,
+                            int *k) {
+  if (d == 0) { *i = n; *j = t1; *k = t2; }
+  else if (d == 1) { *i = t1; *j = n; *k = t2; }
+  else { *i = t1; *j = t2; *k = n; }
+}
+
+/* correct_monotone_restrict (src/ModFaceValue.f90:233-408) on a copy of
+ * the coarse block b whose sides with finer neighbours hold the face
+ * restriction in ghost layer 0: each such ghost cell moves to the coarse
+ * cell distance in primitive variables and returns to conservative ones.
+ * (The reference converts the momenta of the whole block in place and back,
+ * which changes them by at most an ulp; the copy leaves them as they are.)
+ * Returns Blk itself when the block has no finer side. */
+static const double *monotone_restrict_copy(const ob_session *s, int b,
+                                            double *Copy) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const long len = block_len(&sh);
+  const double *Blk = s->State + (long)b * len;
+  const int n[3] = {c->nI, c->nJ, c->nK};
+  int side, copied = 0;
+  for (side = 0; side < 6; side++) {
+    const int nb = s->iNeighbor_EB[6 * b + side];
+    if (nb > -2 || s->iSlab_IS[8 * (-2 - nb) + 2] != 1) continue;
+    if (!copied) { memcpy(Copy, Blk, sizeof(double) * len); copied = 1; }
+    const int d = side / 2, hi = side % 2;
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    int t1, t2, v;
+    for (t2 = 1; t2 <= n[d2]; t2++)
+      for (t1 = 1; t1 <= n[d1]; t1++) {
+        int i, j, k, ii, jj, kk;
+        side_ijk(d, hi ? n[d] + 1 : 0, t1, t2, &i, &j, &k);
+        side_ijk(d, hi ? n[d] : 1, t1, t2, &ii, &jj, &kk);
+        double *G = Copy + cidx(&sh, i, j, k);
+        double Pg[NV], Pc[NV];
+        to_prim(G, Pg);
+        to_prim(Copy + cidx(&sh, ii, jj, kk), Pc);
+        ob_correct_monotone_restrict(Pc, Pg);
+        for (v = 0; v < NV; v++) G[v] = Pg[v];
+        for (v = 0; v < 3; v++) G[OB_RHOUX + v] = Pg[OB_RHOUX + v] * Pg[OB_RHO];
+      }
+  }
+  return copied ? Copy : Blk;
+}
+
+/* face arrays' index of face f (0-based, normal to d) at transverse
+ * (t1, t2) (0-based) */
+static inline long face_at(const ob_config *c, int d, int f, int t1, int t2) {
+  if (d == 0) return ((long)t2 * c->nJ + t1) * (c->nI + 1) + f;
+  if (d == 1) return ((long)t2 * (c->nJ + 1) + f) * c->nI + t1;
+  return ((long)f * c->nJ + t2) * c->nI + t1;
+}
+
+/* get_face_tvd (src/ModFaceValue.f90:680-852) on every side of block b
+ * with a coarser neighbour: from the two coarse ghost layers and the first
+ * two fine layers, the face values at the coarse side (both states) and
+ * the fine-side state one face in */
+static void tvd_faces(const ob_session *s, int b, const double *Blk,
+                      double *LX, double *RX, double *LY, double *RY,
+                      double *LZ, double *RZ) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const int n[3] = {c->nI, c->nJ, c->nK};
+  int side;
+  for (side = 0; side < 6; side++) {
+    const int nb = s->iNeighbor_EB[6 * b + side];
+    if (nb > -2 || s->iSlab_IS[8 * (-2 - nb) + 2] != 0) continue;
+    const int d = side / 2, hi = side % 2;
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    double *L = d == 0 ? LX : d == 1 ? LY : LZ;
+    double *R = d == 0 ? RX : d == 1 ? RY : RZ;
+    const int N = n[d];
+    int t1, t2, f, v;
+    for (t2 = 0; t2 < n[d2]; t2 += 2)
+      for (t1 = 0; t1 < n[d1]; t1 += 2) {
+        double C2[NV], C1[NV], F1[4 * NV], F2[4 * NV], CtoF[4 * NV],
+            FtoC[4 * NV], FF[4 * NV];
+        int i, j, k;
+        side_ijk(d, hi ? N + 2 : -1, t1 + 1, t2 + 1, &i, &j, &k);
+        to_prim(Blk + cidx(&sh, i, j, k), C2);
+        side_ijk(d, hi ? N + 1 : 0, t1 + 1, t2 + 1, &i, &j, &k);
+        to_prim(Blk + cidx(&sh, i, j, k), C1);
+        for (f = 0; f < 4; f++) {
+          const int u1 = t1 + 1 + f % 2, u2 = t2 + 1 + f / 2;
+          side_ijk(d, hi ? N : 1, u1, u2, &i, &j, &k);
+          to_prim(Blk + cidx(&sh, i, j, k), F1 + f * NV);
+          side_ijk(d, hi ? N - 1 : 2, u1, u2, &i, &j, &k);
+          to_prim(Blk + cidx(&sh, i, j, k), F2 + f * NV);
+        }
+        ob_tvd_reschange(c, C2, C1, F1, F2, CtoF, FtoC, FF);
+        for (f = 0; f < 4; f++) {
+          const int u1 = t1 + f % 2, u2 = t2 + f / 2;
+          const long f0 = face_at(c, d, hi ? N : 0, u1, u2);
+          const long f1 = face_at(c, d, hi ? N - 1 : 1, u1, u2);
+          for (v = 0; v < NV; v++) {
+            if (hi) {
+              R[f0 * NV + v] = CtoF[f * NV + v];
+              L[f0 * NV + v] = FtoC[f * NV + v];
+              R[f1 * NV + v] = FF[f * NV + v];
+            } else {
+              L[f0 * NV + v] = CtoF[f * NV + v];
+              R[f0 * NV + v] = FtoC[f * NV + v];
+              L[f1 * NV + v] = FF[f * NV + v];
+            }
+          }
+        }
+      }
   }
 }
 
@@ -1651,6 +1852,7 @@ typedef struct {
   /* faces between two body cells (inner body only) */
   unsigned char *SkipX, *SkipY, *SkipZ;
   double *Keep;   /* the block's state before the update (body cells) */
+  double *Tvd;    /* the block with its monotone-restricted ghosts */
 } ob_scratch;
 
 static void scratch_alloc(const ob_config *c, ob_scratch *w) {
@@ -1675,6 +1877,7 @@ static void scratch_alloc(const ob_config *c, ob_scratch *w) {
   {
     ob_shape sh = shape_of(c);
     w->Keep = (double *)calloc(block_len(&sh), 8);
+    w->Tvd = (double *)calloc(block_len(&sh), 8);
   }
 }
 static void scratch_free(ob_scratch *w) {
@@ -1682,6 +1885,7 @@ static void scratch_free(ob_scratch *w) {
   free(w->RZ); free(w->FX); free(w->FY); free(w->FZ); free(w->Src);
   free(w->B0X); free(w->B0Y); free(w->B0Z); free(w->DivB0); free(w->CurlB0);
   free(w->SkipX); free(w->SkipY); free(w->SkipZ); free(w->Keep);
+  free(w->Tvd);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1848,8 +2052,13 @@ void ob_session_face_values(const ob_session *s, int b, double *LX,
                             unsigned char *SkipY, unsigned char *SkipZ) {
   const ob_config *c = &s->c;
   ob_shape sh = shape_of(c);
-  ob_calc_face_value(c, s->State + (long)b * block_len(&sh), LX, RX, LY, RY,
-                     LZ, RZ);
+  const double *Blk = s->State + (long)b * block_len(&sh);
+  const int tvd = c->UseTvdResChange && c->nOrder > 1;
+  double *Copy = tvd ? (double *)malloc(sizeof(double) * block_len(&sh)) : NULL;
+  ob_calc_face_value(c, tvd ? monotone_restrict_copy(s, b, Copy) : Blk, LX, RX,
+                     LY, RY, LZ, RZ);
+  if (tvd) tvd_faces(s, b, Blk, LX, RX, LY, RY, LZ, RZ);
+  free(Copy);
   if (s->Used && block_has_body(s, b)) {
     const long nfx = (long)(c->nI + 1) * c->nJ * c->nK;
     const long nfy = (long)c->nI * (c->nJ + 1) * c->nK;
@@ -1959,6 +2168,9 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
         block_res_change(s, b, rc);
         ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc,
                               w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
+        if (c->UseTvdResChange && c->nOrder > 1)
+          tvd_faces(s, b, s->State + (long)b * len, w.LX, w.RX, w.LY, w.RY,
+                    w.LZ, w.RZ);
         e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                               w.FX, w.FY, w.FZ);
         if (e) {
@@ -1992,8 +2204,13 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
       block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
       int rc[6];
       block_res_change(s, b, rc);
-      ob_calc_face_value_rc(c, Blk, CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY,
-                            w.LZ, w.RZ);
+      /* calc_face_value with DoMonotoneRestrict (src/ModFaceValue.f90:
+       * 965-969,1118-1123): TVD face values at resolution changes */
+      const int tvd = c->UseTvdResChange && c->nOrder > 1;
+      ob_calc_face_value_rc(c, tvd ? monotone_restrict_copy(s, b, w.Tvd) : Blk,
+                            CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY, w.LZ,
+                            w.RZ);
+      if (tvd) tvd_faces(s, b, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
       const int body = s->Used && block_has_body(s, b);
       if (body)
         body_faces(s, b, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ, w.SkipX,

@@ -102,9 +102,26 @@ typedef struct {
    * before the update (src/ModAdvanceExplicit.f90:184-195) and
    * set_global_timestep is not called (src/ModBatsrusMethods.f90:569-572) */
   int IsSteadyState;
+  /* #RESOLUTIONCHANGE UseTvdResChange with nOrderProlong = 1: two coarse
+   * ghost layers on fine blocks, face restriction + correct_monotone_restrict
+   * on coarse blocks, tvd_reschange face values on fine blocks
+   * (src/ModFaceValue.f90:233-408,680-852,1096-1140,2269-2343) */
+  int UseTvdResChange;
 } ob_config;
 
 /* --- point routines (exposed for unit tests) ------------------------------ */
+/* tvd_reschange (src/ModFaceValue.f90:2269-2343) on primitive states: C2,
+ * C1 the two coarse cells behind the face, F1[4][8] / F2[4][8] the fine
+ * cells of the first and second layer (subface f = i2 + 2 j2); out: the
+ * coarse-side and fine-side values at the four subfaces and the fine
+ * cells' values at their next face */
+void ob_tvd_reschange(const ob_config *c, const double *C2, const double *C1,
+                      const double *F1, const double *F2, double *CtoF,
+                      double *FtoC, double *FF);
+/* correct_monotone_restrict on one ghost cell: G (primitive, the restricted
+ * value at the fine distance) moved to the coarse cell distance from the
+ * coarse block's own cell C (src/ModFaceValue.f90:287-294) */
+void ob_correct_monotone_restrict(const double *C, double *G);
 /* variants with the face's B0 (B0_D[3], NULL: none) for B0 splitting */
 void ob_physical_flux_b0(const ob_config *c, int iDim, const double *State_V,
                          const double *B0_D, double *StateCons_V,

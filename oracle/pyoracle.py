@@ -36,7 +36,7 @@ class ObConfig(C.Structure):
                 ("BetaLimiterResChange", C.c_double),
                 ("nFaceLimiterResChange", C.c_int),
                 ("UseOutflowPressure", C.c_int), ("pOutflow", C.c_double),
-                ("IsSteadyState", C.c_int)]
+                ("IsSteadyState", C.c_int), ("UseTvdResChange", C.c_int)]
 
 # iTypeCellBc values (src/ModCellBoundary.f90:23-41)
 BC_TYPES = {"periodic": 4, "float": 5, "outflow": 6, "reflect": 7,
@@ -130,6 +130,8 @@ def lib():
             [dp] * 6 + [dp] * 6 + [C.c_int, dp]
         L.ob_body_face_state.argtypes = [dp, dp, C.c_double, C.c_double,
                                          C.c_double, dp]
+        L.ob_tvd_reschange.argtypes = [C.POINTER(ObConfig)] + [dp] * 7
+        L.ob_correct_monotone_restrict.argtypes = [dp, dp]
         ub = C.POINTER(C.c_ubyte)
         L.ob_session_face_values.argtypes = [C.POINTER(ObSession), C.c_int] \
             + [dp] * 6 + [ub] * 3
@@ -152,7 +154,7 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
                 pCritLow=2.0, pCritHigh=1.5, VelCrit=2.0, TypeCellBc=None,
                 CellState_VI=None, BetaLimiterResChange=2.0,
                 nFaceLimiterResChange=2, UseOutflowPressure=False,
-                pOutflow=0.0, IsSteadyState=False):
+                pOutflow=0.0, IsSteadyState=False, UseTvdResChange=False):
     """TypeCellBc: six names or iTypeCellBc codes (x-, x+, y-, y+, z-, z+;
     default float); CellState_VI: (6, 8) ghost states of the fixed sides
     in conservative variables (rho, rho u, B, p)."""
@@ -171,6 +173,7 @@ def make_config(nI=8, nJ=8, nK=8, nG=2, nDim=3, nOrder=2, limiter="mc",
     c.pCritLow, c.pCritHigh, c.VelCrit = pCritLow, pCritHigh, VelCrit
     c.UseOutflowPressure, c.pOutflow = int(UseOutflowPressure), pOutflow
     c.IsSteadyState = int(IsSteadyState)
+    c.UseTvdResChange = int(UseTvdResChange)
     c.BetaLimiterResChange = BetaLimiterResChange
     c.nFaceLimiterResChange = nFaceLimiterResChange
     for side, t in enumerate(TypeCellBc or ["float"] * 6):
@@ -200,6 +203,25 @@ def limiter(iLimiter, P, beta):
     lib().ob_limiter(LIMITERS.get(iLimiter, iLimiter), lMin, lMax, beta,
                      _dp(P), 0, _dp(dL), _dp(dR))
     return dL, dR
+
+
+def tvd_reschange(cfg, C2, C1, F1, F2):
+    """tvd_reschange on primitive states: C2, C1 (8,), F1, F2 (4, 8) with
+    subface f = i2 + 2 j2.  Returns (CoarseToFineF, FineToCoarseF, FineF),
+    each (4, 8)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (C2, C1, F1, F2)]
+    out = [np.zeros((4, 8)) for _ in range(3)]
+    lib().ob_tvd_reschange(C.byref(cfg), *[_dp(x) for x in a],
+                           *[_dp(x) for x in out])
+    return tuple(out)
+
+
+def correct_monotone_restrict(Cell, Ghost):
+    """The ghost's primitive state moved to the coarse cell distance."""
+    c = np.ascontiguousarray(Cell, dtype=np.float64)
+    g = np.array(Ghost, dtype=np.float64)
+    lib().ob_correct_monotone_restrict(_dp(c), _dp(g))
+    return g
 
 
 def physical_flux(cfg, iDim, state):
