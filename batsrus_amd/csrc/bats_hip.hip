@@ -1006,8 +1006,11 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only, and
-  // without the inner body (which the fused stage carries)
-  if (h->elemSize != 8 || h->useBody) return BATS_EINVAL;
+  // without the inner body or the TVD resolution-change faces (which the
+  // fused stage carries)
+  if (h->elemSize != 8 || h->useBody ||
+      (h->cfg.UseTvdResChange && h->nResSlab > 0))
+    return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
 
@@ -1223,6 +1226,8 @@ extern "C" int bats_set_global_timestep(bats_handle *h, double TimeSimulationLim
 // calcDt: DoCalcTimestep of the stage loop.  The local time steps are set
 // after the last stage's update (time accurate, :217-219), before the first
 // stage's update (steady state, :184-195), and always by the no-update pass.
+static void tvd_args(const bats_handle *h, AmrArgs &m);
+
 static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
                         int advanceTime, bool timed) {
   const bats_config &c = h->cfg;
@@ -1292,6 +1297,13 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   // the body variant carries no low order faces, resolution-change beta or
   // steady state
   if (h->useBody && (lowUse || resBeta || steady)) return BATS_EINVAL;
+  // the TVD treatment at resolution changes: second-order faces of the
+  // fine sides from the records k_amr_fineflux leaves; not with the 8-wave
+  // source, low order faces, steady state or the surrogate
+  const bool tvd = c.UseTvdResChange && h->nResSlab > 0;
+  if (tvd && (c.UseDivbSource || low_order_set(c) || steady ||
+              h->sur.mode != BATS_SUR_OFF))
+    return BATS_EINVAL;
   // the surrogate's face pass has no low order weights, one beta and no
   // 8-wave source
   if ((lowUse || resBeta || steady || c.UseDivbSource) &&
@@ -1302,7 +1314,8 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   a.nFaceRC = resBeta ? c.nFaceLimiterResChange : 0;
   const int opt = (c.UseDivbSource ? OPT_POWELL : 0) |
                   (lowUse ? OPT_LOWORDER : 0) | (resBeta ? OPT_RESBETA : 0) |
-                  (steady ? OPT_STEADY : 0) | (h->useB0 ? OPT_B0 : 0);
+                  (steady ? OPT_STEADY : 0) | (h->useB0 ? OPT_B0 : 0) |
+                  (tvd && c.nOrder > 1 ? OPT_TVD : 0);
   a.b0F = h->b0F;
   a.b0C = h->b0C;
   a.useB0Src = h->useB0Src;
@@ -1331,6 +1344,7 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
       m.lowCrit = a.lowCrit;
       m.betaRC = a.betaRC;
       m.nFaceRC = a.nFaceRC;
+      tvd_args(h, m);
 #define BATS_FPICK(NBv, FL)                                                  \
   (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
        : launch_fineflux_##NBv##_##FL##_f64(m, h->stream))
@@ -1608,8 +1622,18 @@ extern "C" int bats_set_stage_shape(bats_handle *h, int CellsPerThread,
   return BATS_OK;
 }
 
+// the TVD treatment at resolution changes (UseTvdResChange, the
+// reference's default with nOrderProlong = 1): ghost fill and fine faces
+static void tvd_args(const bats_handle *h, AmrArgs &m) {
+  const bats_config &c = h->cfg;
+  m.tvd = c.UseTvdResChange != 0;
+  m.twoLayers = c.nOrder > 1;
+  m.tvdBeta = std::min(c.BetaLimiterResChange, c.LimiterBeta);
+}
+
 int amr_fill_ghosts(bats_handle *h, const void *U) {
   AmrArgs m{};
+  tvd_args(h, m);
   m.U = U; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
   m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
   m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);

@@ -120,8 +120,52 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 // OPT_B0: B0 splitting, the state holds B1 = B - B0 (FP64 only).
 // OPT_BODY: the inner body's true cells and face boundary condition (FP64,
 // launched only on the groups of blocks that reach a body cell).
+// OPT_TVD: the TVD face values at resolution changes (UseTvdResChange): a
+// fine block's faces at a coarser side take the flux, and the faces one
+// cell in the fine-side state, that k_amr_fineflux left in the side's
+// record (the record layout is below).
 enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8,
-       OPT_B0 = 16, OPT_BODY = 32 };
+       OPT_B0 = 16, OPT_BODY = 32, OPT_TVD = 64 };
+// record of one fine face (p, q) of a coarser side with the TVD treatment,
+// in the side's slab store: the face flux (slots 0..6, PP <- Vdt, EN) and
+// the fine-side primitive state of the face one cell in
+enum { TVD_FLUX = 0, TVD_FINEF = 9, NTVD = 17 };
+static_assert(NTVD <= 4 * NCORR, "the TVD record fits a slab cell's store");
+
+// tvd_reschange (src/ModFaceValue.f90:2269-2343) for subface f of a 2x2
+// group, on primitive states: C2, C1 the coarse cells behind the face, F1
+// the group's four first-layer fine cells (f = i2 + 2 j2), F2 this
+// subface's second-layer fine cell.  Out: the coarse-side and fine-side
+// values at the subface and the fine value at the next face.
+template <typename T>
+__device__ __forceinline__ void tvd_subface(T Beta, const T *C2, const T *C1,
+                                            const T (*F1)[NVAR], const T *F2,
+                                            int f, T *CtoF, T *FtoC, T *FF) {
+  const T cThird = T(1.0 / 3.0), cTwoThird = T(2.0 / 3.0),
+          cSixth = T(1.0 / 6.0);
+#pragma unroll
+  for (int v = 0; v < NVAR; v++) {
+    const T Avg = T(0.25) * (((F1[0][v] + F1[1][v]) + F1[2][v]) + F1[3][v]);
+    const T Grad = Avg - C1[v];
+    const T Sgn = fsign(T(1), Grad);
+    const T dC = C1[v] - C2[v];
+    const T Ltd =
+        Sgn * fmax2(T(0), fmin2(fmin2(Beta * cTwoThird * fabsr(Grad),
+                                      Beta * T(0.5) * Sgn * dC),
+                                cThird * fabsr(Grad) + T(0.25) * Sgn * dC));
+    const T t = F1[f][v] - Avg;
+    CtoF[v] = C1[v] + Ltd + fsign(fmin2(fabsr(Ltd), fabsr(t)), t);
+    const T dF = F2[v] - F1[f][v];
+    const T Ltd2 =
+        Sgn * fmax2(T(0), fmin2(fmin2(Sgn * (F1[f][v] - C1[v]),
+                                      Beta * cThird * fabsr(Grad)),
+                                fmin2(Beta * T(0.5) * Sgn * dF,
+                                      cSixth * fabsr(Grad) +
+                                          T(0.25) * Sgn * dF)));
+    FtoC[v] = F1[f][v] - Ltd2;
+    FF[v] = F1[f][v] + Ltd2;
+  }
+}
 
 // index of a face of block b in the B0 face arrays (per direction, the
 // reference's LeftState_VX(nI+1,nJ,nK) / _VY(nI,nJ+1,nK) / _VZ(nI,nJ,nK+1)
@@ -292,6 +336,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   static_assert(!BODY || (sizeof(T) == 8 && !LOWORD && !RESB && !STEADY),
                 "the body variant is FP64, without low order faces, the "
                 "resolution-change beta or steady state");
+  constexpr bool TVD = (OPT & OPT_TVD) != 0;
+  static_assert(!TVD || (!POWELL && !LOWORD && !STEADY && !B0S && !BODY),
+                "the TVD variant carries no 8-wave source, low order faces, "
+                "steady state, B0 or body");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -375,6 +423,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   };
   auto sFi = [&](int l, int dir, int v, int q) {
     return ((l * 2 + dir) * NFL + v) * NFX + q;
+  };
+  // the TVD record of face (p, q) of a side with a coarser neighbour
+  auto tvd_rec = [&](int blk, int side, int p, int q) -> const T * {
+    if (!TVD) return nullptr;
+    const int n = a.nei[6 * blk + side];
+    if (n > -2 || a.slabKind[-2 - n] != 0) return nullptr;
+    return static_cast<const T *>(a.corr) +
+           ((((long)(-2 - n)) * NB + q) * NB + p) * 4 * NCORR;
   };
   // the body: true cell mask of a block (null: all cells true) and the
   // flag of its cell (ii, jj, kk), each index in -2 .. NB+1
@@ -485,6 +541,21 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         }
         R[v] = LOWORD ? Q2[v] - dr * w : Q2[v] - dr;
         dLk[v] = dl;
+      }
+      if (TVD && (fz <= 1 || fz >= NB - 1)) {
+        // faces at and one cell in from a coarser z side
+        const T *rec = tvd_rec(bb, fz <= 1 ? 4 : 5, i, j);
+        if (rec && (fz == 0 || fz == NB)) {
+#pragma unroll
+          for (int v = 0; v < NFS; v++) Fz[v] = rec[TVD_FLUX + v];
+          return;
+        }
+        if (rec && fz == 1)
+#pragma unroll
+          for (int v = 0; v < NVAR; v++) L[v] = rec[TVD_FINEF + v];
+        if (rec && fz == NB - 1)
+#pragma unroll
+          for (int v = 0; v < NVAR; v++) R[v] = rec[TVD_FINEF + v];
       }
       if (BODY && Ub) {
         T r[3];
@@ -604,7 +675,29 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           for (int d = 0; d < 3; d++) B0r[d] = dir == 0 ? B0f[d] : B0f[(d + 1) % 3];
         }
         bool skip = false;
-        if (BODY) {
+        // TVD: the face at a coarser side takes its record's flux (in the
+        // original frame), the face one cell in its fine-side state
+        const T *trec = nullptr;
+        bool recFlux = false;
+        if (TVD && (fp <= 1 || fp >= NB - 1)) {
+          trec = tvd_rec(bf, 2 * dir + (fp >= NB - 1), row, k);
+          recFlux = trec && (fp == 0 || fp == NB);
+        }
+        if (TVD && trec) {
+          if (recFlux) {
+#pragma unroll
+            for (int v = 0; v < NFLUX; v++) F[v] = trec[TVD_FLUX + v];
+            vdt = trec[TVD_FLUX + PP];
+          } else {
+            T L[NVAR], R[NVAR];
+            face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
+            T *S = fp == 1 ? L : R;
+#pragma unroll
+            for (int v = 0; v < NVAR; v++)
+              S[v] = trec[TVD_FINEF + (dir == 0 ? v : rot_src(v))];
+            err |= numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
+          }
+        } else if (BODY) {
           const unsigned char *m = body_mask(bf);
           T L[NVAR], R[NVAR];
           face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
@@ -646,7 +739,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         else
           err |= face_flux<0, FLUX, LIM, B0S>(ph, area, bL, bR, c0, c1, c2,
                                               c3, F, vdt, unA, B0r);
-        if (dir != 0) unrotate_flux(F);
+        if (dir != 0 && !recFlux) unrotate_flux(F);
         F[PP] = vdt;
         if (a.corr && (fp == 0 || fp == NB))
           corrected(bf, 2 * dir + (fp == NB), row, k, F);
@@ -876,6 +969,12 @@ struct AmrArgs {
   double betaRC;        // sharper limiter next to sides with DiLevel_EB /= 0
   int nFaceRC;          // (0: off)
   double pOutflow;      // ghost pressure of outflow sides (kind-4 slabs)
+  // TVD treatment at resolution changes (UseTvdResChange): first-order
+  // prolongation, with the two coarse layers when twoLayers (second-order
+  // scheme); face restriction and correct_monotone_restrict on the coarse
+  // side; tvdBeta = min(BetaLimiterResChange, LimiterBeta)
+  int tvd, twoLayers;
+  double tvdBeta;
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -932,6 +1031,45 @@ __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
       const T u = U[v * a.vs + o];
       val[v] = (v == MX + d || v == BX + d) ? -u : u;
     }
+  } else if (kind == 1 && a.tvd) {
+    // face restriction: the 2x2 fine cells of the fine layer nearest to
+    // the ghost layer (DoRestrictFace, src/ModMessagePass.f90:105); layer 0
+    // moves to the coarse cell distance from the block's own cell in
+    // primitive variables (correct_monotone_restrict,
+    // src/ModFaceValue.f90:267-294,390-403).  Layer 1 enters no result: the
+    // coarse face it serves takes the fine fluxes.
+    const int F = sl[3 + (p / H) + 2 * (q / H)];
+    const int nf = hi ? 2 * L : NB - 1 - 2 * L;
+    for (int v = 0; v < NVAR; v++) val[v] = T(0);
+    for (int bq = 0; bq < 2; bq++)
+      for (int cp = 0; cp < 2; cp++) {
+        int x, y, z;
+        side_xyz(d, nf, 2 * (p % H) + cp, 2 * (q % H) + bq, x, y, z);
+        const long o = cell(F, x, y, z);
+        for (int v = 0; v < NVAR; v++) val[v] += U[v * a.vs + o];
+      }
+    for (int v = 0; v < NVAR; v++) val[v] = T(0.25) * val[v];
+    if (L == 0 && a.twoLayers) {
+      int x, y, z;
+      side_xyz(d, hi ? NB - 1 : 0, p, q, x, y, z);
+      T c[NVAR];
+      load_state(U, a.vs, cell(sl[0], x, y, z), c);
+      to_primitive(c);
+      to_primitive(val);
+      const T cThird = T(1.0 / 3.0);
+      for (int v = 0; v < NVAR; v++) val[v] = val[v] + cThird * (val[v] - c[v]);
+      val[RHO] = fmax2(val[RHO], T(1e-30));
+      val[PP] = fmax2(val[PP], T(1e-30));
+      for (int v = MX; v <= MZ; v++) val[v] = val[v] * val[RHO];
+    }
+  } else if (kind == 0 && a.tvd) {
+    // first-order prolongation: the coarse cell itself; with the
+    // second-order scheme the two ghost layers take the two coarse layers
+    // behind the face (DoTwoCoarseLayers, src/ModMessagePass.f90:108-109)
+    const int Lc = a.twoLayers ? L : 0;
+    int x, y, z;
+    side_xyz(d, hi ? Lc : NB - 1 - Lc, sl[4] + p / 2, sl[5] + q / 2, x, y, z);
+    load_state(U, a.vs, cell(sl[3], x, y, z), val);
   } else if (kind == 1) {
     const int F = sl[3 + (p / H) + 2 * (q / H)];
     for (int v = 0; v < NVAR; v++) val[v] = T(0);
@@ -1015,7 +1153,33 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
     bL = cell_beta(T(a.beta), T(a.betaRC), a.nFaceRC, lo, up, f0 - 1, NB);
     bR = cell_beta(T(a.beta), T(a.betaRC), a.nFaceRC, lo, up, f0, NB);
   }
-  if (a.lowCrit) {
+  T FF[NVAR];
+  if (a.tvd && LIM != LIM_FIRST) {
+    // get_face_tvd (src/ModFaceValue.f90:680-852): c holds C2, C1, F1, F2
+    // (low side) or F2, F1, C1, C2 (high side) of this face; the group's
+    // other three first-layer fine cells complete the 2x2 average
+    const int f = (p % 2) + 2 * (q % 2);
+    T F1[4][NVAR];
+    for (int g = 0; g < 4; g++) {
+      if (g == f) {
+        for (int v = 0; v < NVAR; v++) F1[g][v] = c[hi ? 1 : 2][v];
+        continue;
+      }
+      int x, y, z;
+      side_xyz(d, hi ? NB - 1 : 0, (p & ~1) + g % 2, (q & ~1) + g / 2, x, y,
+               z);
+      load_state(U, a.vs, cell_off<NB>(a.nei, b, x, y, z, a.nCellTot),
+                 F1[g]);
+      to_primitive(F1[g]);
+    }
+    T CtoF[NVAR], FtoC[NVAR];
+    tvd_subface(T(a.tvdBeta), c[hi ? 3 : 0], c[hi ? 2 : 1], F1,
+                c[hi ? 0 : 3], f, CtoF, FtoC, FF);
+    for (int v = 0; v < NVAR; v++) {
+      L[v] = hi ? FtoC[v] : CtoF[v];
+      R[v] = hi ? CtoF[v] : FtoC[v];
+    }
+  } else if (a.lowCrit) {
     int x, y, z;
     side_xyz(d, f0, p, q, x, y, z);
     const T w = T(1) - static_cast<const T *>(a.lowCrit)[crit_index<NB>(
@@ -1037,6 +1201,14 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
   // (src/ModConserveFlux.f90:136-140)
   dst[CORR_BNL] = L[BX + d] * T(0.25);
   dst[CORR_BNR] = R[BX + d] * T(0.25);
+  if (a.tvd && LIM != LIM_FIRST) {
+    // the fine block's record: this face's flux and the fine value of the
+    // face one cell in
+    T *rec = static_cast<T *>(a.corr) +
+             ((((long)slab) * NB + q) * NB + p) * 4 * NCORR;
+    for (int v = 0; v < 9; v++) rec[TVD_FLUX + v] = F[v];
+    for (int v = 0; v < NVAR; v++) rec[TVD_FINEF + v] = FF[v];
+  }
   if (err) atomicOr(a.err, err);
 }
 
@@ -1104,6 +1276,26 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 4: BATS_K(LM, 20); break;                                             \
   default: BATS_K(LM, 21); break;                                            \
   }
+  // the TVD treatment at resolution changes (second-order limiters), with
+  // or without the resolution-change beta
+#define BATS_LAUNCH_TVD(LM, RB)                                              \
+  switch (opt & (RB ? OPT_RESBETA : 0)) {                                    \
+  case 0: BATS_K(LM, 64); break;                                             \
+  default: BATS_K(LM, 68); break;                                            \
+  }
+  if (opt & OPT_TVD) {
+    if (opt & (OPT_POWELL | OPT_LOWORDER | OPT_STEADY | OPT_B0 | OPT_BODY))
+      return hipErrorInvalidValue;
+    switch (a.lim) {
+    case LIM_MINMOD: BATS_LAUNCH_TVD(LIM_MINMOD, false); break;
+    case LIM_MC: BATS_LAUNCH_TVD(LIM_MC, true); break;
+    case LIM_MC3: BATS_LAUNCH_TVD(LIM_MC3, true); break;
+    case LIM_BETA: BATS_LAUNCH_TVD(LIM_BETA, true); break;
+    case LIM_NO: BATS_LAUNCH_TVD(LIM_NO, false); break;
+    default: return hipErrorInvalidValue;  // first order needs no record
+    }
+    return hipGetLastError();
+  }
   // the inner body (FP64): with or without the 8-wave source and B0
 #define BATS_LAUNCH_BODY(LM)                                                 \
   switch (opt & (OPT_POWELL | OPT_B0)) {                                     \
@@ -1154,6 +1346,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
 #undef BATS_LAUNCH7
 #undef BATS_LAUNCH_B0
 #undef BATS_LAUNCH_BODY
+#undef BATS_LAUNCH_TVD
   return hipGetLastError();
 }
 
