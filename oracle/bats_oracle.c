@@ -1341,7 +1341,8 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
       int dim = side / 2, hi = side % 2;
       int nG = c->nG;
       if (nb == -1 && (c->iTypeCellBc_I[side] == OB_BC_FIXED ||
-                       c->iTypeCellBc_I[side] == OB_BC_INFLOW)) {
+                       c->iTypeCellBc_I[side] == OB_BC_INFLOW ||
+                       c->iTypeCellBc_I[side] == OB_BC_VARY)) {
         const double *St = c->CellState_VI + side * OB_NVAR;
         int n1 = dim == 0 ? s.nJ : s.nI, n2 = dim == 2 ? s.nJ : s.nK;
         int nd = dim == 0 ? s.nI : dim == 1 ? s.nJ : s.nK, p, q;
@@ -1810,11 +1811,67 @@ static void apply_cons_flux(const ob_session *s, int b, double *FX, double *FY,
   }
 }
 
-static void exchange_all(ob_session *s) {
-  ob_exchange_messages(&s->c, s->nBlock, s->iNeighbor_EB, s->State);
-  if (s->nSlab > 0)
-    ob_exchange_amr(&s->c, s->nBlock, s->nSlab, s->iSlab_IS, s->State);
+void ob_vary_state(int nPoint, const double *Time_I, const double *State_VI,
+                   double t, double *Cons_V) {
+  int v, i;
+  if (t <= Time_I[0] || nPoint == 1) {
+    for (v = 0; v < NV; v++) Cons_V[v] = State_VI[v];
+  } else if (t >= Time_I[nPoint - 1]) {
+    for (v = 0; v < NV; v++) Cons_V[v] = State_VI[(nPoint - 1) * NV + v];
+  } else {
+    for (i = 0; i < nPoint - 2 && Time_I[i + 1] <= t; i++) continue;
+    const double Dx = (t - Time_I[i]) / (Time_I[i + 1] - Time_I[i]);
+    for (v = 0; v < NV; v++)
+      Cons_V[v] = State_VI[i * NV + v] +
+                  Dx * (State_VI[(i + 1) * NV + v] - State_VI[i * NV + v]);
+  }
+  for (v = 0; v < 3; v++) Cons_V[OB_RHOUX + v] = Cons_V[OB_RHOUX + v] * Cons_V[OB_RHO];
 }
+
+/* exchange_messages: the ghost cells of every block.  The x+ side of type
+ * vary takes the inflow table's state at the current time first (time
+ * accurate runs, src/ModUpdateStateFast.f90:1151-1158); with B0 the ghost
+ * cells of fixed, inflow and vary sides hold B1 = CellState B - B0
+ * (set_boundary_for_cell, :1262-1268). */
+static void exchange_all(ob_session *s) {
+  ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  if (s->nVary > 0 && !c->IsSteadyState && c->iTypeCellBc_I[1] == OB_BC_VARY)
+    ob_vary_state(s->nVary, s->VaryTime_I, s->VaryState_VI, s->tSimulation,
+                  c->CellState_VI + 1 * NV);
+  ob_exchange_messages(c, s->nBlock, s->iNeighbor_EB, s->State);
+  if (s->B0) {
+    const long len = block_len(&sh), nc = len / NV;
+    const int n[3] = {c->nI, c->nJ, c->nK};
+    int b;
+    for (b = 0; b < s->nBlock; b++) {
+      int side;
+      for (side = 0; side < 6; side++) {
+        const int t = c->iTypeCellBc_I[side];
+        if (s->iNeighbor_EB[6 * b + side] != -1 ||
+            (t != OB_BC_FIXED && t != OB_BC_INFLOW && t != OB_BC_VARY))
+          continue;
+        const int d = side / 2, hi = side % 2;
+        const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+        int g, t1, t2, v;
+        for (g = 1; g <= c->nG; g++)
+          for (t2 = 1; t2 <= n[d2]; t2++)
+            for (t1 = 1; t1 <= n[d1]; t1++) {
+              int i, j, k;
+              side_ijk(d, hi ? n[d] + g : 1 - g, t1, t2, &i, &j, &k);
+              const long q = cidx(&sh, i, j, k) / NV;
+              double *S = s->State + (long)b * len + q * NV;
+              const double *B0 = s->B0 + ((long)b * nc + q) * 3;
+              for (v = 0; v < 3; v++) S[OB_BX + v] = S[OB_BX + v] - B0[v];
+            }
+      }
+    }
+  }
+  if (s->nSlab > 0)
+    ob_exchange_amr(c, s->nBlock, s->nSlab, s->iSlab_IS, s->State);
+}
+
+void ob_session_exchange(ob_session *s) { exchange_all(s); }
 
 /* ---------------------------------------------------------------------- */
 /* set_global_timestep -- src/ModTimeStepControl.f90:306-524               */

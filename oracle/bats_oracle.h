@@ -52,7 +52,8 @@ enum { OB_FLUX_RUSANOV = 1, OB_FLUX_LINDE = 2, OB_FLUX_HLLD = 3 };
  * float with the ghost pressure set to pOutflow when UseOutflowPressure
  * (:273-277). */
 enum { OB_BC_PERIODIC = 4, OB_BC_FLOAT = 5, OB_BC_OUTFLOW = 6,
-       OB_BC_REFLECT = 7, OB_BC_FIXED = 9, OB_BC_INFLOW = 10 };
+       OB_BC_REFLECT = 7, OB_BC_FIXED = 9, OB_BC_INFLOW = 10,
+       OB_BC_VARY = 11 };
 
 /* Error codes (the reference calls stop_mpi at these points) */
 enum { OB_OK = 0, OB_ENEGFAST2 = 2, OB_EHLLD = 3, OB_EBC = 5 };
@@ -251,6 +252,13 @@ void ob_exchange_messages(const ob_config *c, int nBlock,
  * [block, side, kind, n0, n1, n2, n3, target] (kind 0: coarser neighbour
  * n0 at in-plane offset (n1, n2) coarse cells; kind 1: finer neighbours
  * n0..n3 by quadrant). */
+/* the time-varying inflow state at time t (get_solar_wind_point without a
+ * satellite offset, src/ModSolarwind.f90:591-740): the table's primitive
+ * states interpolated linearly in time, the end values outside, then the
+ * velocity turned into momentum (src/ModUpdateStateFast.f90:1155-1157) */
+void ob_vary_state(int nPoint, const double *Time_I, const double *State_VI,
+                   double t, double *Cons_V);
+
 void ob_exchange_amr(const ob_config *c, int nBlock, int nSlab,
                      const int *iSlab_IS, double *State);
 
@@ -293,7 +301,19 @@ typedef struct {
   const unsigned char *Used;
   const double *XyzStart_DB;
   double BodyRho, BodyP, B1rCoef;
+  /* time-varying inflow (VaryBC_ on the x+ side, set_boundary_fast,
+   * src/ModUpdateStateFast.f90:1151-1158): nVary points of the solar wind
+   * table, times VaryTime_I and primitive states VaryState_VI (8 per
+   * point); 0: the side keeps its CellState_VI */
+  int nVary;
+  const double *VaryTime_I, *VaryState_VI;
 } ob_session;
+
+/* exchange_messages of a session: same-level and outer ghost cells (with
+ * the vary table and B0 at fixed/inflow/vary sides), then the
+ * resolution-change ghosts */
+void ob_session_exchange(ob_session *s);
+
 
 /* set_face, 'ionosphere' type (src/ModUpdateStateFast.f90:1277-1327,
  * src/ModFaceBoundary.f90:621-648,876-884): the ghost face state of a face

@@ -40,7 +40,7 @@ class ObConfig(C.Structure):
 
 # iTypeCellBc values (src/ModCellBoundary.f90:23-41)
 BC_TYPES = {"periodic": 4, "float": 5, "outflow": 6, "reflect": 7,
-            "fixed": 9, "inflow": 10}
+            "fixed": 9, "inflow": 10, "vary": 11}
 
 
 class ObSession(C.Structure):
@@ -60,7 +60,9 @@ class ObSession(C.Structure):
                 ("Used", C.POINTER(C.c_ubyte)),
                 ("XyzStart_DB", C.POINTER(C.c_double)),
                 ("BodyRho", C.c_double), ("BodyP", C.c_double),
-                ("B1rCoef", C.c_double)]
+                ("B1rCoef", C.c_double), ("nVary", C.c_int),
+                ("VaryTime_I", C.POINTER(C.c_double)),
+                ("VaryState_VI", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -131,6 +133,8 @@ def lib():
         L.ob_body_face_state.argtypes = [dp, dp, C.c_double, C.c_double,
                                          C.c_double, dp]
         L.ob_tvd_reschange.argtypes = [C.POINTER(ObConfig)] + [dp] * 7
+        L.ob_vary_state.argtypes = [C.c_int, dp, dp, C.c_double, dp]
+        L.ob_session_exchange.argtypes = [C.POINTER(ObSession)]
         L.ob_correct_monotone_restrict.argtypes = [dp, dp]
         ub = C.POINTER(C.c_ubyte)
         L.ob_session_face_values.argtypes = [C.POINTER(ObSession), C.c_int] \
@@ -214,6 +218,16 @@ def tvd_reschange(cfg, C2, C1, F1, F2):
     lib().ob_tvd_reschange(C.byref(cfg), *[_dp(x) for x in a],
                            *[_dp(x) for x in out])
     return tuple(out)
+
+
+def vary_state(Time_I, State_VI, t):
+    """The time-varying inflow's conservative state at time t from a table
+    of primitive states State_VI (nPoint, 8)."""
+    T = np.ascontiguousarray(Time_I, dtype=np.float64)
+    S = np.ascontiguousarray(State_VI, dtype=np.float64)
+    out = np.zeros(8)
+    lib().ob_vary_state(len(T), _dp(T), _dp(S), float(t), _dp(out))
+    return out
 
 
 def correct_monotone_restrict(Cell, Ghost):
@@ -431,7 +445,7 @@ class Session:
 
     def __init__(self, grid, cfg, state_vgb, nThread=0, B0_DGB=None,
                  UseB0Source=True, Used_GB=None, XyzStart_DB=None,
-                 BodyRho=1.0, BodyP=1.0, B1rCoef=-1.0):
+                 BodyRho=1.0, BodyP=1.0, B1rCoef=-1.0, VaryTable=None):
         """B0_DGB: B0 splitting, the static field (nBlock, nKg, nJg, nIg, 3)
         at the cell centres and ghost cells (same-level grids).  Used_GB:
         an inner body, the true cells (nBlock, nKg, nJg, nIg) of bool;
@@ -484,6 +498,14 @@ class Session:
             s.Used = self.Used.ctypes.data_as(C.POINTER(C.c_ubyte))
             s.XyzStart_DB = _dp(self.XyzStart)
             s.BodyRho, s.BodyP, s.B1rCoef = BodyRho, BodyP, B1rCoef
+        if VaryTable is not None:
+            # (times, primitive states (nPoint, 8)) of the x+ vary side
+            self.VaryTime = np.ascontiguousarray(VaryTable[0], np.float64)
+            self.VaryState = np.ascontiguousarray(VaryTable[1], np.float64)
+            assert self.VaryState.shape == (len(self.VaryTime), 8)
+            s.nVary = len(self.VaryTime)
+            s.VaryTime_I = _dp(self.VaryTime)
+            s.VaryState_VI = _dp(self.VaryState)
         if cfg.UseAdaptiveLowOrder:
             sx, sy, sz = face_shapes(cfg)
             self.LowOrderCrit = np.zeros(
@@ -492,14 +514,7 @@ class Session:
         self.s = s
 
     def exchange(self):
-        lib().ob_exchange_messages(C.byref(self.cfg), self.grid.nBlock,
-                                   self.nei.ctypes.data_as(C.POINTER(C.c_int)),
-                                   _dp(self.State))
-        if self.nSlab:
-            lib().ob_exchange_amr(C.byref(self.cfg), self.grid.nBlock,
-                                  self.nSlab,
-                                  self.slabs.ctypes.data_as(C.POINTER(C.c_int)),
-                                  _dp(self.State))
+        lib().ob_session_exchange(C.byref(self.s))
 
     def init_timestep(self):
         return lib().ob_init_timestep(C.byref(self.s))
