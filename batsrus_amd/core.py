@@ -79,6 +79,7 @@ ABI_FUNCTIONS = {
     "bats_set_low_order_face": (C.c_int, [_hp]),
     "bats_set_cell_boundary": (C.c_int, [_hp, _ip, _dp]),
     "bats_set_b0": (C.c_int, [_hp, _dp, C.c_int]),
+    "bats_set_inflow_table": (C.c_int, [_hp, C.c_int, _dp, _dp]),
     "bats_set_body": (C.c_int, [_hp, C.POINTER(C.c_ubyte), _dp, C.c_double,
                                 C.c_double, C.c_double]),
     "bats_set_step_graph": (C.c_int, [_hp, C.c_int]),
@@ -183,7 +184,7 @@ SCHEME_DEFAULTS = dict(nOrder=2, limiter="mc", beta=1.5, flux="rusanov",
 
 # iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41)
 BC_TYPES = {"periodic": 4, "float": 5, "outflow": 6, "reflect": 7,
-            "fixed": 9, "inflow": 10}
+            "fixed": 9, "inflow": 10, "vary": 11}
 
 
 def make_config(grid: BlockGrid, device=0, **scheme):
@@ -340,6 +341,20 @@ class Session:
         self._check(self.lib.bats_set_cell_boundary(
             self.h, _ptr(t, C.c_int), None if st is None else _ptr(st)),
             "set_cell_boundary")
+
+    def set_inflow_table(self, Time_I, State_VI=None):
+        """Time-varying inflow of the x+ side (type 'vary'): table times
+        (nPoint,) and primitive states (nPoint, 8); None drops it."""
+        if Time_I is None:
+            self._check(self.lib.bats_set_inflow_table(self.h, 0, None, None),
+                        "set_inflow_table")
+            return
+        t = np.ascontiguousarray(Time_I, dtype=np.float64)
+        st = np.ascontiguousarray(State_VI, dtype=np.float64)
+        assert st.shape == (len(t), 8), st.shape
+        self._check(self.lib.bats_set_inflow_table(self.h, len(t), _ptr(t),
+                                                   _ptr(st)),
+                    "set_inflow_table")
 
     def set_b0(self, B0_DGB, UseB0Source=True):
         """B0 splitting: B0_DGB of shape (nBlock, nKg, nJg, nIg, 3), the

@@ -112,6 +112,8 @@ struct BlockArgs {
   const double *lowCrit;                // low order weights or null
   double betaRC;                        // BetaLimiterResChange
   int nFaceRC;                          // its faces (0: not used)
+  int tvd;                              // UseTvdResChange (second order)
+  double tvdBeta;                       // min(BetaLimiterResChange, beta)
   int steady;                           // steady state: per-cell steps
   double *dtmaxCB, *dtmaxB;
   int *err;
@@ -181,6 +183,44 @@ __global__ void k_block_face_value(BlockArgs a) {
                      q[0], q[1], q[2], q[3], Lv, Rv);
   else
     face_states_rt(a.lim, bL, bR, q[0], q[1], q[2], q[3], Lv, Rv);
+  // the TVD treatment (get_face_tvd, src/ModFaceValue.f90:680-852): at a
+  // side with a coarser neighbour the face and the fine-side state of the
+  // face one in come from tvd_reschange of the two coarse ghost layers and
+  // the first two fine layers of the subface's 2x2 group
+  const int f = dir == 0 ? fi : dir == 1 ? fj : fk;
+  if (a.tvd && (f <= 1 || f >= NB - 1)) {
+    const int hi = f >= NB - 1;
+    const int n = a.nei[6 * a.iBlock + 2 * dir + hi];
+    if (n <= -2 && a.slabKind[-2 - n] == 0) {
+      const int t1 = dir == 0 ? fj : fi, t2 = dir == 2 ? fj : fk;
+      auto cell = [&](int nrm, int u1, int u2, double *P) {
+        const int ii = dir == 0 ? nrm : u1;
+        const int jj = dir == 1 ? nrm : dir == 0 ? u1 : u2;
+        const int kk = dir == 2 ? nrm : u2;
+        load_state(a.Usnap, a.vs,
+                   cell_off<NB>(a.nei, a.iBlock, ii, jj, kk, a.nCellTot), P);
+        to_primitive(P);
+      };
+      double C2[NVAR], C1[NVAR], F1[4][NVAR], F2[NVAR];
+      cell(hi ? NB + 1 : -2, t1, t2, C2);
+      cell(hi ? NB : -1, t1, t2, C1);
+      for (int g = 0; g < 4; g++)
+        cell(hi ? NB - 1 : 0, (t1 & ~1) + g % 2, (t2 & ~1) + g / 2, F1[g]);
+      cell(hi ? NB - 2 : 1, t1, t2, F2);
+      double CtoF[NVAR], FtoC[NVAR], FF[NVAR];
+      tvd_subface(a.tvdBeta, C2, C1, F1, F2, (t1 % 2) + 2 * (t2 % 2), CtoF,
+                  FtoC, FF);
+      if (f == 0) {
+        for (int v = 0; v < NVAR; v++) { Lv[v] = CtoF[v]; Rv[v] = FtoC[v]; }
+      } else if (f == NB) {
+        for (int v = 0; v < NVAR; v++) { Lv[v] = FtoC[v]; Rv[v] = CtoF[v]; }
+      } else if (f == 1) {
+        for (int v = 0; v < NVAR; v++) Lv[v] = FF[v];
+      } else {
+        for (int v = 0; v < NVAR; v++) Rv[v] = FF[v];
+      }
+    }
+  }
   for (int v = 0; v < NVAR; v++) {
     L[fo * NVAR + v] = Lv[v];
     R[fo * NVAR + v] = Rv[v];
@@ -668,7 +708,8 @@ extern "C" int bats_finalize(bats_handle *h) {
                   h->sur.face, h->sur.rowX, h->sur.rowFace, h->sur.rowScale,
                   h->sur.counters, h->slabs, h->slabKind, h->fineSlabs,
                   h->corr, h->lowCrit, h->dBcState, h->b0F, h->b0C,
-                  h->bodySlot, h->bodyUsed, h->bodyXyz0, h->groupList};
+                  h->bodySlot, h->bodyUsed, h->bodyXyz0, h->groupList,
+                  h->b0Side, h->b0SideSlot, h->varyT, h->varyS};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   drop_graphs(h);
@@ -685,9 +726,11 @@ static int apply_grid(bats_handle *h, const std::vector<int> &nei,
 
 // a new grid drops the B0 field (it belongs to the old block geometry)
 static void drop_b0(bats_handle *h) {
-  if (h->b0F) (void)hipFree(h->b0F);
-  if (h->b0C) (void)hipFree(h->b0C);
-  h->b0F = h->b0C = nullptr;
+  void *ptrs[] = {h->b0F, h->b0C, h->b0Side, h->b0SideSlot};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  h->b0F = h->b0C = h->b0Side = nullptr;
+  h->b0SideSlot = nullptr;
   h->useB0 = 0;
 }
 
@@ -957,6 +1000,8 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
                                 : nullptr;
   a.betaRC = c.BetaLimiterResChange;
   a.nFaceRC = res_beta_used(h) ? c.nFaceLimiterResChange : 0;
+  a.tvd = c.UseTvdResChange && h->nResSlab > 0 && c.nOrder > 1;
+  a.tvdBeta = std::min(c.BetaLimiterResChange, c.LimiterBeta);
   a.steady = c.IsSteadyState;
   a.dtmaxCB = h->dtmaxCB;
   a.dtmaxB = h->dtmaxB;
@@ -1006,10 +1051,13 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only, and
-  // without the inner body or the TVD resolution-change faces (which the
-  // fused stage carries)
-  if (h->elemSize != 8 || h->useBody ||
-      (h->cfg.UseTvdResChange && h->nResSlab > 0))
+  // without the inner body (which the fused stage carries); the TVD
+  // resolution-change faces not with the 8-wave source, low order faces or
+  // steady state
+  if (h->elemSize != 8 || h->useBody) return BATS_EINVAL;
+  const bats_config &c = h->cfg;
+  if (c.UseTvdResChange && h->nResSlab > 0 &&
+      (c.UseDivbSource || low_order_set(c) || c.IsSteadyState))
     return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
@@ -1634,6 +1682,13 @@ static void tvd_args(const bats_handle *h, AmrArgs &m) {
 int amr_fill_ghosts(bats_handle *h, const void *U) {
   AmrArgs m{};
   tvd_args(h, m);
+  m.b0Side = h->useB0 ? h->b0Side : nullptr;
+  m.b0SideSlot = h->useB0 ? h->b0SideSlot : nullptr;
+  m.varyT = h->varyT;
+  m.varyS = h->varyS;
+  m.nVary = h->cfg.IsSteadyState ? 0 : h->nVary;
+  m.scal = h->scal;
+  m.runOfBlock = h->nRun > 1 ? h->runOfBlock : nullptr;
   m.U = U; m.vs = h->vs; m.nCellTot = h->nCell; m.nei = h->nei;
   m.cellsize = h->cellsize; m.slabs = h->slabs; m.nSlab = h->nSlab;
   m.fineSlabs = h->fineSlabs; m.nFine = h->nFine; m.ph = make_phys(h->cfg);
@@ -1790,10 +1845,12 @@ static int apply_grid(bats_handle *h, const std::vector<int> &userNei,
       if (t == BATS_BC_PERIODIC) return BATS_EINVAL;
       const bool outP = t == BATS_BC_OUTFLOW && h->cfg.UseOutflowPressure;
       if (t != BATS_BC_FIXED && t != BATS_BC_INFLOW && t != BATS_BC_REFLECT &&
-          !outP)
+          t != BATS_BC_VARY && !outP)
         continue;
       nei[6 * b + s] = -2 - (int)(slabs.size() / 8);
-      const int kindS = t == BATS_BC_REFLECT ? 3 : outP ? 4 : 2;
+      // kind 5: the x+ vary side, refilled from the inflow table
+      const int kindS = t == BATS_BC_REFLECT ? 3 : outP ? 4
+                      : (t == BATS_BC_VARY && s == 1) ? 5 : 2;
       const int sl[8] = {b, s, kindS, 0, 0, 0, 0, 0};
       slabs.insert(slabs.end(), sl, sl + 8);
     }
@@ -1832,14 +1889,13 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
   for (int s = 0; s < 6; s++) {
     const int t = iTypeCellBc_I[s];
     if (t != BATS_BC_PERIODIC && t != BATS_BC_FLOAT && t != BATS_BC_FIXED &&
-        t != BATS_BC_INFLOW && t != BATS_BC_REFLECT && t != BATS_BC_OUTFLOW)
+        t != BATS_BC_INFLOW && t != BATS_BC_REFLECT && t != BATS_BC_OUTFLOW &&
+        t != BATS_BC_VARY)
       return BATS_EBC;
-    fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW;
+    fixed = fixed || t == BATS_BC_FIXED || t == BATS_BC_INFLOW ||
+            t == BATS_BC_VARY;
   }
   if (fixed && !CellState_VI) return BATS_EINVAL;
-  // with B0 splitting a fixed ghost field would be CellState minus the
-  // ghost cells' B0 (src/ModUpdateStateFast.f90:1264-1266): not supported
-  if (fixed && h->useB0) return BATS_EINVAL;
   if (fixed)
     for (int t = 0; t < 6 * NVAR; t++)
       if (!std::isfinite(CellState_VI[t])) return BATS_EINVAL;
@@ -1863,6 +1919,50 @@ extern "C" int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
   return st;
 }
 
+// the time-varying inflow table of the x+ side (set_boundary_fast,
+// src/ModUpdateStateFast.f90:1151-1158)
+extern "C" int bats_set_inflow_table(bats_handle *h, int nPoint,
+                                     const double *Time_I,
+                                     const double *State_VI) {
+  if (!h || nPoint < 0) return BATS_EINVAL;
+  drop_graphs(h);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (nPoint == 0) {
+    if (h->varyT) (void)hipFree(h->varyT);
+    if (h->varyS) (void)hipFree(h->varyS);
+    h->varyT = h->varyS = nullptr;
+    h->nVary = 0;
+    return BATS_OK;
+  }
+  if (!Time_I || !State_VI) return BATS_EINVAL;
+  for (int i = 0; i < nPoint; i++) {
+    if (!std::isfinite(Time_I[i]) || (i > 0 && !(Time_I[i] > Time_I[i - 1])))
+      return BATS_EINVAL;
+    for (int v = 0; v < NVAR; v++)
+      if (!std::isfinite(State_VI[(long)i * NVAR + v])) return BATS_EINVAL;
+    if (!(State_VI[(long)i * NVAR + RHO] > 0.0)) return BATS_EINVAL;
+  }
+  double *dT = nullptr, *dS = nullptr;
+  hipError_t e = hipMalloc(&dT, sizeof(double) * nPoint);
+  if (e == hipSuccess) e = hipMalloc(&dS, sizeof(double) * NVAR * nPoint);
+  if (e == hipSuccess)
+    e = hipMemcpy(dT, Time_I, sizeof(double) * nPoint, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dS, State_VI, sizeof(double) * NVAR * nPoint,
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (dT) (void)hipFree(dT);
+    if (dS) (void)hipFree(dS);
+    return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
+  }
+  if (h->varyT) (void)hipFree(h->varyT);
+  if (h->varyS) (void)hipFree(h->varyS);
+  h->varyT = dT;
+  h->varyS = dS;
+  h->nVary = nPoint;
+  return BATS_OK;
+}
+
 // B0 splitting: face B0 (set_b0_face, src/ModB0.f90:326-335), and per cell
 // the B0 itself, div B0 and curl B0 from the face values (set_b0_source,
 // :568-590), computed once on the host from B0_DGB
@@ -1879,9 +1979,6 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
   // FP64 state, same-level grid, no fixed or inflow ghost states, no
   // low order faces or steady state (the B0 stage variants carry neither)
   if (h->elemSize != 8 || h->nResSlab > 0) return BATS_EINVAL;
-  for (int s = 0; s < 6; s++)
-    if (h->bcType[s] == BATS_BC_FIXED || h->bcType[s] == BATS_BC_INFLOW)
-      return BATS_EINVAL;
   if (low_order_used(c) || c.IsSteadyState) return BATS_EINVAL;
   const int NB = h->NB, g = c.nG, NG = NB + 2 * g, nB = c.nBlock;
   for (long t = 0; t < 3L * NG * NG * NG * nB; t++)
@@ -1942,21 +2039,53 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
               DyInv * (face(1, b, i, j + 1, k, 0) - face(1, b, i, j, k, 0));
         }
   }
-  double *dF = nullptr, *dC = nullptr;
+  // the ghost B0 of every outer side (the caller's -1 sides), for the
+  // fixed, inflow and vary ghost cells: slot per (block, side), layer L =
+  // 0, 1 away from the side, in-plane (p, q) as the ghost slabs
+  std::vector<int> slot(6L * nB, -1);
+  std::vector<double> SB;
+  for (int b = 0; b < nB; b++)
+    for (int s = 0; s < 6; s++) {
+      if (h->userNei[6 * b + s] != -1) continue;
+      slot[6 * b + s] = (int)(SB.size() / (6L * NB * NB));
+      const int d = s / 2, hi = s % 2;
+      for (int L = 0; L < 2; L++)
+        for (int q = 0; q < NB; q++)
+          for (int p = 0; p < NB; p++) {
+            const int nrm = hi ? NB + L : -1 - L;
+            const int i = d == 0 ? nrm : p, j = d == 1 ? nrm : d == 0 ? p : q,
+                      k = d == 2 ? nrm : q;
+            for (int v = 0; v < 3; v++) SB.push_back(B0(b, i, j, k, v));
+          }
+    }
+  double *dF = nullptr, *dC = nullptr, *dSB = nullptr;
+  int *dSlot = nullptr;
   hipError_t e = hipMalloc(&dF, sizeof(double) * F.size());
   if (e == hipSuccess) e = hipMalloc(&dC, sizeof(double) * C.size());
+  if (e == hipSuccess) e = hipMalloc(&dSlot, sizeof(int) * slot.size());
+  if (e == hipSuccess && !SB.empty())
+    e = hipMalloc(&dSB, sizeof(double) * SB.size());
   if (e == hipSuccess)
     e = hipMemcpy(dF, F.data(), sizeof(double) * F.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(dC, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dSlot, slot.data(), sizeof(int) * slot.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && dSB)
+    e = hipMemcpy(dSB, SB.data(), sizeof(double) * SB.size(),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    if (dF) (void)hipFree(dF);
-    if (dC) (void)hipFree(dC);
+    void *made[] = {dF, dC, dSB, dSlot};
+    for (void *p : made)
+      if (p) (void)hipFree(p);
     return e == hipErrorOutOfMemory ? BATS_ENOMEM : BATS_EHIP;
   }
   drop_b0(h);
   h->b0F = dF;
   h->b0C = dC;
+  h->b0Side = dSB;
+  h->b0SideSlot = dSlot;
   h->useB0 = 1;
   h->useB0Src = UseB0Source != 0;
   return BATS_OK;

@@ -96,6 +96,15 @@ struct bats_handle {
   // div B0 and curl B0 [7][cell]; dropped by any grid change
   double *b0F, *b0C;
   int useB0, useB0Src;
+  // B0 of the two ghost layers of every outer side (-1 neighbour) [slot]
+  // [layer][q][p][3], slot per (block, side) (-1: none), for the fixed,
+  // inflow and vary ghost cells
+  double *b0Side;
+  int *b0SideSlot;
+  // time-varying inflow of the x+ side: nVary table points (times and
+  // primitive states, 8 per point)
+  double *varyT, *varyS;
+  int nVary;
   // inner body (bats_set_body): per block the slot of its true cell mask
   // (-1: none), the masks over block + 2-cell halo, the blocks' first cell
   // centres relative to the body centre, the body density and B1rCoef; the

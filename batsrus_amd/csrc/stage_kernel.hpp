@@ -975,6 +975,16 @@ struct AmrArgs {
   // side; tvdBeta = min(BetaLimiterResChange, LimiterBeta)
   int tvd, twoLayers;
   double tvdBeta;
+  // B0 at the outer sides' ghost cells (fixed/inflow/vary: B1 = CellState
+  // B - B0), [slot][layer][q][p][3] with a slot per (block, side) or null
+  const double *b0Side;
+  const int *b0SideSlot;
+  // time-varying inflow of the x+ side (kind-5 slabs): nVary table points
+  // (times, primitive states), read at the slab block's run time
+  const double *varyT, *varyS;
+  int nVary;
+  const double *scal;
+  const int *runOfBlock;
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -1015,8 +1025,44 @@ __global__ void __launch_bounds__(256) k_amr_ghosts(AmrArgs a, int NB) {
   auto cell = [&](int blk, int x, int y, int z) {
     return (((long)blk * NB + z) * NB + y) * NB + x;
   };
-  if (kind == 2) {
-    for (int v = 0; v < NVAR; v++) val[v] = T(a.bcState[side * NVAR + v]);
+  if (kind == 2 || kind == 5) {
+    if (kind == 5 && a.nVary > 0) {
+      // the inflow table at the run's time: V_i + w (V_i+1 - V_i), the end
+      // values outside, then momentum (set_boundary_fast,
+      // src/ModUpdateStateFast.f90:1151-1158)
+      const double t =
+          a.scal[(a.runOfBlock ? a.runOfBlock[sl[0]] : 0) * S_NSCAL + S_TSIM];
+      const int n = a.nVary;
+      double w[NVAR];
+      if (t <= a.varyT[0] || n == 1) {
+        for (int v = 0; v < NVAR; v++) w[v] = a.varyS[v];
+      } else if (t >= a.varyT[n - 1]) {
+        for (int v = 0; v < NVAR; v++) w[v] = a.varyS[(n - 1) * NVAR + v];
+      } else {
+        int lo = 0, hi2 = n - 1;  // varyT[lo] <= t < varyT[hi2]
+        while (hi2 - lo > 1) {
+          const int mid = (lo + hi2) / 2;
+          if (a.varyT[mid] <= t) lo = mid; else hi2 = mid;
+        }
+        const double Dx = (t - a.varyT[lo]) / (a.varyT[lo + 1] - a.varyT[lo]);
+        for (int v = 0; v < NVAR; v++)
+          w[v] = a.varyS[lo * NVAR + v] +
+                 Dx * (a.varyS[(lo + 1) * NVAR + v] - a.varyS[lo * NVAR + v]);
+      }
+      for (int v = MX; v <= MZ; v++) w[v] = w[v] * w[RHO];
+      for (int v = 0; v < NVAR; v++) val[v] = T(w[v]);
+    } else {
+      for (int v = 0; v < NVAR; v++) val[v] = T(a.bcState[side * NVAR + v]);
+    }
+    if (a.b0SideSlot) {
+      const int slot = a.b0SideSlot[6 * sl[0] + side];
+      if (slot >= 0) {
+        const double *b0 = a.b0Side + (((long)slot * 2 + L) * NB * NB +
+                                       (long)q * NB + p) * 3;
+        for (int c = 0; c < 3; c++)
+          val[BX + c] = T(double(val[BX + c]) - b0[c]);
+      }
+    }
   } else if (kind == 4) {
     int x, y, z;
     side_xyz(d, hi ? NB - 1 : 0, p, q, x, y, z);

@@ -39,7 +39,7 @@ module ModHipBridge
        bats_exchange_messages, bats_set_stage, bats_calc_face_value, &
        bats_calc_face_flux, bats_calc_source, bats_update_state, &
        bats_save_cons_flux, bats_message_pass_face, bats_apply_cons_flux, &
-       bats_set_b0, bats_set_body, &
+       bats_set_b0, bats_set_body, bats_set_inflow_table, &
        bats_calc_timestep, &
        bats_get_face_values, bats_get_face_fluxes, bats_get_dtmax, &
        bats_set_global_timestep, bats_advance_explicit, bats_advance_stage, &
@@ -175,6 +175,13 @@ module ModHipBridge
        real(c_double), intent(in):: B0_DGB(*)
        integer(c_int), value:: UseB0Source
      end function bats_set_b0
+     integer(c_int) function bats_set_inflow_table(Handle, nPoint, Time_I, &
+          State_VI) bind(C, name="bats_set_inflow_table")
+       import:: c_int, c_ptr, c_double
+       type(c_ptr), value:: Handle
+       integer(c_int), value:: nPoint
+       real(c_double), intent(in):: Time_I(*), State_VI(*)
+     end function bats_set_inflow_table
      integer(c_int) function bats_set_body(Handle, Used_GB, XyzStart_DB, &
           BodyRho, BodyP, B1rCoef) bind(C, name="bats_set_body")
        import:: c_int, c_ptr, c_double, c_signed_char

@@ -181,9 +181,10 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
  * with UseDivbSource the 8-wave source takes the B0 terms
  * (src/ModCalcSource.f90:1792-1808,1880-1885) and, with UseB0Source,
  * -B1 div(B0) - curl(B0) x B1 from the face B0 (:953-967, set_b0_source
- * :568-590).  FP64 handles on same-level grids only; not with fixed or
- * inflow outer sides, adaptive low order faces or steady state
- * (BATS_EINVAL).  A new grid drops B0; B0_DGB = NULL drops it too. */
+ * :568-590).  The ghost B0 of the outer sides serves the fixed, inflow and
+ * vary ghost cells.  FP64 handles on same-level grids only; not with
+ * adaptive low order faces or steady state (BATS_EINVAL).  A new grid
+ * drops B0; B0_DGB = NULL drops it too. */
 int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
 
 /* Inner body (UseBody with the 'ionosphere' face boundary condition, as the
@@ -211,7 +212,8 @@ int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
 
 /* iTypeCellBc values of the outer boundaries (src/ModCellBoundary.f90:23-41) */
 enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
-       BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10 };
+       BATS_BC_REFLECT = 7, BATS_BC_FIXED = 9, BATS_BC_INFLOW = 10,
+       BATS_BC_VARY = 11 };
 
 /* Outer boundary conditions: iTypeCellBc_I[side] for the sides x-, x+, y-,
  * y+, z-, z+ of the domain, applied where iNeighbor_EB = -1.  Float copies
@@ -225,18 +227,33 @@ enum { BATS_BC_PERIODIC = 4, BATS_BC_FLOAT = 5, BATS_BC_OUTFLOW = 6,
  * normal momentum and field negated (set_symm_bc on a Cartesian grid,
  * src/ModCellBoundary.f90:308-331,624-666), refilled before every stage
  * and at exchange_messages; a periodic side must have no -1 neighbour.
- * Other types return BATS_EBC (the reference stops there, :1272; time
- * varying inflow needs the solar wind input, which is out of scope).  The
- * default is float on every side.  The types apply to the current grid at
+ * Vary (BATS_BC_VARY) is fixed with the inflow table of
+ * bats_set_inflow_table on the x+ side.  With B0 splitting the ghost
+ * cells of fixed, inflow and vary sides hold B1 = CellState B - B0 of the
+ * ghost cell (:1264-1266).  Other types return BATS_EBC (the reference
+ * stops there, :1272).  The default is float on every side.  The types apply to the current grid at
  * once and to every later one; until bats_set_grid the grid is the default
  * one of bats_init (each block its own domain of unit cells, all six sides
  * outer), so a periodic type needs the grid set first.  Fixed sides keep their ghost layers as
  * slabs after the interior cells of the state, so setting them (and any
  * later bats_set_grid / bats_set_amr_grid on such a handle) reallocates the
  * state: call before uploading it.  CellState_VI may be null when no side
- * is fixed or inflow. */
+ * is fixed, inflow or vary. */
 int bats_set_cell_boundary(bats_handle *h, const int *iTypeCellBc_I,
                            const double *CellState_VI);
+
+/* The time-varying inflow (VaryBC_) of the x+ side in time-accurate runs
+ * (set_boundary_fast, src/ModUpdateStateFast.f90:1151-1158, with
+ * get_solar_wind_point, src/ModSolarwind.f90:591-740, for a satellite on
+ * the boundary): before every stage the side's ghost cells take the table's
+ * primitive state at the run's simulation time, linear between the points
+ * (V_i + w (V_i+1 - V_i)), the end values outside, velocity turned into
+ * momentum.  State_VI(8, nPoint) Fortran order (variable fastest),
+ * Time_I strictly increasing.  Vary sides other than x+ (or without a
+ * table, or in steady state) keep their CellState_VI, as fixed sides do.
+ * nPoint = 0 drops the table. */
+int bats_set_inflow_table(bats_handle *h, int nPoint, const double *Time_I,
+                          const double *State_VI);
 
 /* With UseGraph = 1, bats_advance captures one time step (the global time
  * step and every stage's kernels) into a hipGraph on its first step and

@@ -227,8 +227,11 @@ def test_refused_boundary_types():
     g = _grid()
     with Session(g) as s:
         with pytest.raises(StopMpi) as ei:
-            s.set_cell_boundary(["float"] * 5 + [11])     # VaryBC_
+            s.set_cell_boundary(["float"] * 5 + [14])     # ShearBC_
         assert ei.value.code == 5
+        with pytest.raises(StopMpi) as ei:                # vary: a state
+            s.set_cell_boundary(["float"] * 5 + [11])     # is needed
+        assert ei.value.code == 1
         with pytest.raises(StopMpi) as ei:
             s.set_cell_boundary(["periodic"] * 6)         # x, y sides are -1
         assert ei.value.code == 1

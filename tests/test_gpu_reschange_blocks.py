@@ -49,16 +49,20 @@ def _per_block_stage(s, g, stage, last, apply=True):
             s.calc_timestep(b)
 
 
-@pytest.mark.parametrize("flux,limiter,powell", [
-    ("rusanov", "mc", False), ("hlld", "mc3", True), ("linde", "minmod", False)])
-def test_per_block_two_level_stages_match_oracle(flux, limiter, powell):
+@pytest.mark.parametrize("flux,limiter,powell,tvd", [
+    ("rusanov", "mc", False, False), ("hlld", "mc3", True, False),
+    ("linde", "minmod", False, False), ("hlld", "mc", False, True),
+    ("rusanov", "mc3", False, True)])
+def test_per_block_two_level_stages_match_oracle(flux, limiter, powell, tvd):
+    """tvd: the TVD faces at the resolution changes (UseTvdResChange),
+    through calc_face_value of the fine blocks in both passes."""
     print(f"case per-block two-level flux={flux} lim={limiter} "
-          f"powell={powell}", flush=True)
+          f"powell={powell} tvd={tvd}", flush=True)
     g = _grid()
     assert len(_fine_blocks(g)) > 0
     U0 = smooth_mhd_soa(g, seed=81, amp=0.2)
     scheme = dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
-                  UseDivbSource=powell)
+                  UseDivbSource=powell, UseTvdResChange=tvd)
     cfg = po.config_from_dict(scheme, g)
     o = po.Session(g, cfg, g.soa_to_state_vgb(U0))
     o.exchange()

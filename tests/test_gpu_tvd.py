@@ -6,7 +6,8 @@ within 1e-12 per stage over two steps (Rusanov, Linde, HLLD; MC, Koren,
 minmod, beta, with the sharper resolution-change beta, and first order),
 FP32 state within its rounding, the grid conserves to rounding, the step
 graph replays the eager stages, and the combinations the TVD variant does
-not carry are refused."""
+not carry are refused.  The per-block routines with the TVD faces are
+checked in test_gpu_reschange_blocks."""
 import numpy as np
 import pytest
 
@@ -125,8 +126,9 @@ def test_tvd_refusals():
             s.upload_soa(U0)
             with pytest.raises(StopMpi):
                 s.init_timestep()
-    # the per-block routines do not carry the TVD faces
-    with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
+    # nor do the per-block routines with the 8-wave source
+    with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True,
+                 UseDivbSource=True) as s:
         s.upload_soa(U0)
         s.exchange_messages()
         s.set_stage(1)
