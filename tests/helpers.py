@@ -51,6 +51,19 @@ def rel_err(a, b, axis_var=0):
     return (np.abs(a2 - b2).max(axis=1) / scale).max()
 
 
+def rel_err_point(a, b, floor=1e-3, axis_var=0):
+    """Elementwise relative error: max over every element of |a-b| divided
+    by |b| itself, or by floor times the variable's max |b| where |b| is
+    smaller than that (elements that cross zero)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    a2 = np.moveaxis(a, axis_var, 0).reshape(a.shape[axis_var], -1)
+    b2 = np.moveaxis(b, axis_var, 0).reshape(b.shape[axis_var], -1)
+    big = np.abs(b2).max(axis=1, keepdims=True)
+    den = np.maximum(np.abs(b2), np.maximum(floor * big, 1e-300))
+    return (np.abs(a2 - b2) / den).max()
+
+
 def rel_err_vec(a, b):
     """max |a-b| per variable divided by the largest |b| of the physical
     quantity the variable belongs to: rho, the momentum vector (rho u_x,

@@ -13,11 +13,12 @@ import pyoracle as po
 from batsrus_amd.core import Session
 from batsrus_amd.grid import BlockGrid
 from batsrus_amd import ics
-from helpers import random_mhd_soa, smooth_mhd_soa, rel_err
+from helpers import random_mhd_soa, smooth_mhd_soa, rel_err, rel_err_point
 
 pytestmark = pytest.mark.gpu
 
 TOL_STAGE = 1e-12
+TOL_POINT = 1e-10   # elementwise, floor 1e-3 of the variable's max
 TOL_FACE = 1e-13
 
 SCHEMES = [
@@ -129,9 +130,15 @@ def test_fused_stages(flux, limiter, nOrder, NB):
                 if stage == 2:
                     o.exchange()   # BATS_advance's exchange after the step
                 s.advance_stage(stage)
-                e = rel_err(s.download_soa(), o.soa())
-                print(f"  step {step} stage {stage} rel err {e:.2e}")
+                Ug, Uo = s.download_soa(), o.soa()
+                e = rel_err(Ug, Uo)
+                # and element by element (elements below 1e-3 of their
+                # variable's largest value against that 1e-3)
+                ep = rel_err_point(Ug, Uo)
+                print(f"  step {step} stage {stage} rel err {e:.2e}, "
+                      f"pointwise {ep:.2e}")
                 assert e <= TOL_STAGE, (step, stage, e)
+                assert ep <= TOL_POINT, (step, stage, ep)
         s.synchronize()
         _, t, n = s.time()
         assert n == 2
