@@ -1342,9 +1342,8 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   }
   const bool lowUse = low_order_used(c);
   const bool resBeta = res_beta_used(h);
-  // the body variant carries no low order faces, resolution-change beta or
-  // steady state
-  if (h->useBody && (lowUse || resBeta || steady)) return BATS_EINVAL;
+  // the body variant carries no low order faces or resolution-change beta
+  if (h->useBody && (lowUse || resBeta)) return BATS_EINVAL;
   // the TVD treatment at resolution changes: second-order faces of the
   // fine sides from the records k_amr_fineflux leaves; not with the 8-wave
   // source, low order faces, steady state or the surrogate
@@ -1367,6 +1366,7 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   a.b0F = h->b0F;
   a.b0C = h->b0C;
   a.useB0Src = h->useB0Src;
+  a.steady = steady;
   const bool f32 = h->elemSize == 4;
   hipError_t e = hipSuccess;
   // resolution changes: ghost slabs of the stage input
@@ -1976,10 +1976,10 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
     return BATS_OK;
   }
   const bats_config &c = h->cfg;
-  // FP64 state, same-level grid, no fixed or inflow ghost states, no
-  // low order faces or steady state (the B0 stage variants carry neither)
+  // FP64 state, same-level grid, no low order faces (the B0 stage
+  // variants do not carry them)
   if (h->elemSize != 8 || h->nResSlab > 0) return BATS_EINVAL;
-  if (low_order_used(c) || c.IsSteadyState) return BATS_EINVAL;
+  if (low_order_used(c)) return BATS_EINVAL;
   const int NB = h->NB, g = c.nG, NG = NB + 2 * g, nB = c.nBlock;
   for (long t = 0; t < 3L * NG * NG * NG * nB; t++)
     if (!std::isfinite(B0_DGB[t])) return BATS_EINVAL;
@@ -2108,10 +2108,10 @@ extern "C" int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
     return BATS_OK;
   }
   const bats_config &c = h->cfg;
-  // FP64 state, same-level grid, no low order faces, resolution-change
-  // beta or steady state (the body variant carries none of them)
+  // FP64 state, same-level grid, no low order faces or resolution-change
+  // beta (the body variant carries neither)
   if (h->elemSize != 8 || h->nResSlab > 0 || low_order_used(c) ||
-      res_beta_used(h) || c.IsSteadyState || !XyzStart_DB)
+      res_beta_used(h) || !XyzStart_DB)
     return BATS_EINVAL;
   // the ghost pressure floats, so BodyP only has to be a number
   if (!(BodyRho > 0.0) || !std::isfinite(BodyRho) || !std::isfinite(BodyP) ||

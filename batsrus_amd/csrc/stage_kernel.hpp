@@ -79,6 +79,9 @@ struct StageArgs {
   int useB0Src;
   // groups of blocks this launch processes (null: groups 0..nGroup-1)
   const int *groupList;
+  // steady state in the B0 and body variants (the plain variants carry it
+  // as OPT_STEADY)
+  int steady;
   // launch shape: each block group splits into 2^splitShift work items of
   // NB >> splitShift planes (cells per thread); xcdOrder hands each XCD a
   // contiguous range of work items (workgroups go round-robin over the 8
@@ -329,15 +332,15 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   constexpr bool LOWORD = (OPT & OPT_LOWORDER) != 0 && LIM != LIM_FIRST;
   // per-cell limiter beta near sides with DiLevel_EB /= 0
   constexpr bool RESB = (OPT & OPT_RESBETA) != 0 && LIM != LIM_FIRST;
-  constexpr bool STEADY = (OPT & OPT_STEADY) != 0;
+  constexpr bool STEADY_BIT = (OPT & OPT_STEADY) != 0;
   constexpr bool B0S = (OPT & OPT_B0) != 0;
   static_assert(!B0S || sizeof(T) == 8, "B0 splitting is FP64 only");
   constexpr bool BODY = (OPT & OPT_BODY) != 0;
-  static_assert(!BODY || (sizeof(T) == 8 && !LOWORD && !RESB && !STEADY),
-                "the body variant is FP64, without low order faces, the "
-                "resolution-change beta or steady state");
+  static_assert(!BODY || (sizeof(T) == 8 && !LOWORD && !RESB && !STEADY_BIT),
+                "the body variant is FP64, without low order faces or the "
+                "resolution-change beta (steady state: the run-time flag)");
   constexpr bool TVD = (OPT & OPT_TVD) != 0;
-  static_assert(!TVD || (!POWELL && !LOWORD && !STEADY && !B0S && !BODY),
+  static_assert(!TVD || (!POWELL && !LOWORD && !STEADY_BIT && !B0S && !BODY),
                 "the TVD variant carries no 8-wave source, low order faces, "
                 "steady state, B0 or body");
   constexpr int NC = NB * NB;          // columns per block
@@ -356,6 +359,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   __shared__ T sF[BPW * 2 * NFL * NFX];
   __shared__ double sRed[kThreads / 64];
 
+  // steady state: a compile-time option of the plain variants, a run-time
+  // flag of the B0 and body variants (which would otherwise double)
+  const bool STEADY = STEADY_BIT || ((B0S || BODY) && a.steady != 0);
   const int tid = threadIdx.x;
   // a wave never spans two blocks (NC >= 64): the block index, and with it
   // every per-block value (cell size, run, time step), is wave-uniform and
@@ -1352,8 +1358,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   }
   if constexpr (sizeof(T) == 8) {
     if (opt & OPT_BODY) {
-      if (opt & (OPT_LOWORDER | OPT_RESBETA | OPT_STEADY))
-        return hipErrorInvalidValue;
+      if (opt & (OPT_LOWORDER | OPT_RESBETA)) return hipErrorInvalidValue;
       switch (a.lim) {
       case LIM_MINMOD: BATS_LAUNCH_BODY(LIM_MINMOD); break;
       case LIM_MC: BATS_LAUNCH_BODY(LIM_MC); break;
@@ -1365,7 +1370,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
       return hipGetLastError();
     }
     if (opt & OPT_B0) {
-      if (opt & (OPT_LOWORDER | OPT_STEADY)) return hipErrorInvalidValue;
+      if (opt & OPT_LOWORDER) return hipErrorInvalidValue;
       switch (a.lim) {
       case LIM_MINMOD: BATS_LAUNCH_B0(LIM_MINMOD, false); break;
       case LIM_MC: BATS_LAUNCH_B0(LIM_MC, true); break;

@@ -182,9 +182,9 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
  * (src/ModCalcSource.f90:1792-1808,1880-1885) and, with UseB0Source,
  * -B1 div(B0) - curl(B0) x B1 from the face B0 (:953-967, set_b0_source
  * :568-590).  The ghost B0 of the outer sides serves the fixed, inflow and
- * vary ghost cells.  FP64 handles on same-level grids only; not with
- * adaptive low order faces or steady state (BATS_EINVAL).  A new grid
- * drops B0; B0_DGB = NULL drops it too. */
+ * vary ghost cells.  FP64 handles on same-level grids only (time accurate
+ * or steady state); not with adaptive low order faces (BATS_EINVAL).  A
+ * new grid drops B0; B0_DGB = NULL drops it too. */
 int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
 
 /* Inner body (UseBody with the 'ionosphere' face boundary condition, as the
@@ -202,9 +202,9 @@ int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
  * default); faces between two body cells carry no flux; body cells are not
  * updated and hold DtMax_CB = 0 outside the block minimum
  * (src/ModTimeStepControl.f90:158-159,255).  Works with B0 and the 8-wave
- * source; FP64 same-level grids only, not with adaptive low order faces,
- * the resolution-change beta or steady state, and not through the
- * per-block routines (BATS_EINVAL).  A new grid drops the body;
+ * source, time accurate or steady state; FP64 same-level grids only, not
+ * with adaptive low order faces or the resolution-change beta, and not
+ * through the per-block routines (BATS_EINVAL).  A new grid drops the body;
  * Used_GB = NULL drops it too. */
 int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
                   const double *XyzStart_DB, double BodyRho, double BodyP,

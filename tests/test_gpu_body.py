@@ -104,6 +104,47 @@ def test_fused_stages_with_body_match_oracle(flux, limiter, order, powell,
         np.testing.assert_allclose(s.get_dtmax_b(), o.DtMax_B, rtol=1e-12)
 
 
+def test_steady_state_with_body_and_b0_matches_oracle():
+    """Local time stepping (#TIMEACCURATE F) with the body and a dipole B0:
+    every true cell with its own step, body cells at rest."""
+    print("case body steady", flush=True)
+    grid, used, xyz0 = _grid(8)
+    g = grid.nG
+    inner = used[:, g:-g, g:-g, g:-g]
+    U0 = smooth_mhd_soa(grid, seed=79, amp=0.2)
+    scheme = dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8,
+                  UseDivbSource=True, IsSteadyState=True)
+    cfg = po.config_from_dict(scheme, grid)
+    B0 = b0_dgb(grid, dipole)
+    body = dict(Used_GB=used, XyzStart_DB=xyz0, **BODY)
+    o = po.Session(grid, cfg, grid.soa_to_state_vgb(U0), B0_DGB=B0,
+                   UseB0Source=True, **body)
+    o.init_timestep()
+    with Session(grid, **scheme) as s:
+        s.set_b0(B0, UseB0Source=True)
+        s.set_body(used, xyz0, **BODY)
+        s.upload_soa(U0)
+        s.init_timestep()
+        for step in range(2):
+            o.s.nStep += 1
+            s.set_global_timestep()
+            for stage in (1, 2):
+                assert o.advance_stage(stage) == 0
+                if stage == 2:
+                    o.exchange()
+                s.advance_stage(stage)
+                Ug = s.download_soa()
+                e = rel_err(Ug, o.soa())
+                print(f"  step {step} stage {stage} rel err {e:.2e}")
+                assert e <= 1e-12, (step, stage, e)
+                np.testing.assert_array_equal(Ug[:, ~inner], U0[:, ~inner])
+        cb, _ = s.get_dtmax()
+        assert np.all(cb[~inner] == 0) and np.all(cb[inner] > 0)
+        np.testing.assert_allclose(cb[inner],
+                                   o.DtMax_CB.reshape(inner.shape)[inner],
+                                   rtol=1e-12)
+
+
 def test_body_changes_the_result_and_graph_agrees():
     """Without the body the run differs near it; the captured step graph
     (both launches of each stage) gives the stage-by-stage state."""
@@ -135,9 +176,6 @@ def test_body_refusals():
     print("case body refusals", flush=True)
     grid, used, xyz0 = _grid(8)
     with Session(grid, flux="hlld", limiter="mc", precision="f32") as s:
-        with pytest.raises(StopMpi):
-            s.set_body(used, xyz0, **BODY)
-    with Session(grid, flux="hlld", limiter="mc", IsSteadyState=True) as s:
         with pytest.raises(StopMpi):
             s.set_body(used, xyz0, **BODY)
     with Session(grid, flux="hlld", limiter="mc",
