@@ -2130,6 +2130,58 @@ void ob_session_face_values(const ob_session *s, int b, double *LX,
   }
 }
 
+/* B0 (3 values) of the boundary face (p, q) of side `side` in the block's
+ * face B0 arrays B0_DX(3, nI+1, nJ, nK), B0_DY, B0_DZ */
+static double *side_face_b0(const ob_config *c, int side, int p, int q,
+                            double *X, double *Y, double *Z) {
+  const int NB = c->nI, d = side / 2, f = (side % 2) ? NB : 0;
+  long idx;
+  if (d == 0) { idx = ((long)q * NB + p) * (NB + 1) + f; return X + idx * 3; }
+  if (d == 1) { idx = ((long)q * (NB + 1) + f) * NB + p; return Y + idx * 3; }
+  idx = ((long)f * NB + q) * NB + p;
+  return Z + idx * 3;
+}
+
+/* set_b0_reschange and its use in set_b0_face (src/ModB0.f90:338-351,
+ * 366-455, Cartesian 3-D): a face of a side with finer neighbours takes
+ * Coef = 1/8 times the sum, over the four fine faces it covers, of the fine
+ * block's B0 in its ghost cell and its first cell at that face (the fine
+ * faces' sum formed in the order of their quadrant index) */
+static void b0_reschange(const ob_session *s, int b, double *B0X,
+                         double *B0Y, double *B0Z) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const int NB = c->nI, H = NB / 2;
+  const long nc3 = block_len(&sh) / NV * 3;
+  int side, p, q, a, d;
+  for (side = 0; side < 6; side++) {
+    const int n = s->iNeighbor_EB[6 * b + side];
+    if (n > -2) continue;
+    const int *sl = s->iSlab_IS + 8 * (-2 - n);
+    if (sl[2] != 1) continue;
+    const int dn = side / 2, hi = side % 2;
+    /* the fine block's side facing this block: ghost index 0 and first
+     * cell 1 (its low side) or nI+1 and nI (its high side) */
+    const int ig0 = hi ? 0 : NB + 1, ic1 = hi ? 1 : NB;
+    for (q = 0; q < NB; q++)
+      for (p = 0; p < NB; p++) {
+        const double *B0f = s->B0 + (long)sl[3 + p / H + 2 * (q / H)] * nc3;
+        double sum[3] = {0, 0, 0};
+        for (a = 0; a < 4; a++) {
+          const int pf = 2 * (p % H) + a % 2, qf = 2 * (q % H) + a / 2;
+          int ig, jg, kg, ic, jc, kc;
+          side_ijk(dn, ig0, pf + 1, qf + 1, &ig, &jg, &kg);
+          side_ijk(dn, ic1, pf + 1, qf + 1, &ic, &jc, &kc);
+          const double *Bg = B0f + cidx(&sh, ig, jg, kg) / NV * 3;
+          const double *Bc = B0f + cidx(&sh, ic, jc, kc) / NV * 3;
+          for (d = 0; d < 3; d++) sum[d] = sum[d] + 0.125 * (Bg[d] + Bc[d]);
+        }
+        double *F = side_face_b0(c, side, p, q, B0X, B0Y, B0Z);
+        for (d = 0; d < 3; d++) F[d] = sum[d];
+      }
+  }
+}
+
 /* B0 of block b's faces and its div / curl into the scratch (set_b0_face
  * and set_b0_source for the block, src/ModAdvanceExplicit.f90:147,
  * src/ModCalcSource.f90:262); returns the block's ghosted B0 or NULL */
@@ -2139,6 +2191,7 @@ static const double *block_b0(const ob_session *s, int b, ob_scratch *w) {
   ob_shape sh = shape_of(c);
   const double *B0Blk = s->B0 + (long)b * (block_len(&sh) / NV * 3);
   ob_set_b0_face(c, B0Blk, w->B0X, w->B0Y, w->B0Z);
+  if (s->nSlab > 0) b0_reschange(s, b, w->B0X, w->B0Y, w->B0Z);
   ob_set_b0_source(c, s->CellSize_DB + 3L * b, w->B0X, w->B0Y, w->B0Z,
                    w->DivB0, w->CurlB0);
   return B0Blk;
@@ -2175,6 +2228,39 @@ static void block_crit(const ob_session *s, int b, const double **X,
   *X = s->LowOrderCrit + crit_len(c) * b;
   *Y = *X + (long)(c->nI + 1) * c->nJ * c->nK;
   *Z = *Y + (long)c->nI * (c->nJ + 1) * c->nK;
+}
+
+/* the resolution-change pass of a fine block next to coarser ones
+ * (src/ModAdvanceExplicit.f90:95-118): its face values (with the TVD faces
+ * and the body's face states), face fluxes with the block's B0, and
+ * save_cons_flux */
+static int fine_prepass(ob_session *s, int b, ob_scratch *w) {
+  const ob_config *c = &s->c;
+  ob_shape sh = shape_of(c);
+  const long len = block_len(&sh);
+  const double *cs = s->CellSize_DB + 3L * b;
+  const double *CX, *CY, *CZ;
+  int rc[6];
+  block_crit(s, b, &CX, &CY, &CZ);
+  block_res_change(s, b, rc);
+  ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w->LX,
+                        w->RX, w->LY, w->RY, w->LZ, w->RZ);
+  if (c->UseTvdResChange && c->nOrder > 1)
+    tvd_faces(s, b, s->State + (long)b * len, w->LX, w->RX, w->LY, w->RY,
+              w->LZ, w->RZ);
+  const int body = s->Used && block_has_body(s, b);
+  if (body)
+    body_faces(s, b, w->LX, w->RX, w->LY, w->RY, w->LZ, w->RZ, w->SkipX,
+               w->SkipY, w->SkipZ);
+  const double *B0Blk = block_b0(s, b, w);
+  const int e = ob_calc_face_flux_b0(
+      c, cs, w->LX, w->RX, w->LY, w->RY, w->LZ, w->RZ, B0Blk ? w->B0X : NULL,
+      B0Blk ? w->B0Y : NULL, B0Blk ? w->B0Z : NULL, w->FX, w->FY, w->FZ);
+  if (body)
+    body_zero_fluxes(c, w->SkipX, w->SkipY, w->SkipZ, w->FX, w->FY, w->FZ);
+  save_cons_flux(s, b, w->FX, w->FY, w->FZ, w->LX, w->RX, w->LY, w->RY, w->LZ,
+                 w->RZ);
+  return e;
 }
 
 static int run_blocks(ob_session *s, int iStage, int DoUpdate,
@@ -2218,26 +2304,13 @@ static int run_blocks(ob_session *s, int iStage, int DoUpdate,
 #endif
       for (b = 0; b < s->nBlock; b++) {
         if (!has_kind(s, b, 0)) continue;
-        const double *cs = s->CellSize_DB + 3L * b;
-        const double *CX, *CY, *CZ;
-        block_crit(s, UseCrit ? b : -1, &CX, &CY, &CZ);
-        int rc[6];
-        block_res_change(s, b, rc);
-        ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc,
-                              w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
-        if (c->UseTvdResChange && c->nOrder > 1)
-          tvd_faces(s, b, s->State + (long)b * len, w.LX, w.RX, w.LY, w.RY,
-                    w.LZ, w.RZ);
-        e = ob_calc_face_flux(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
-                              w.FX, w.FY, w.FZ);
+        e = fine_prepass(s, b, &w);
         if (e) {
 #ifdef _OPENMP
 #pragma omp critical
 #endif
           if (!err) err = e;
         }
-        save_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
-                       w.RZ);
       }
       scratch_free(&w);
     }
@@ -2338,19 +2411,8 @@ int ob_session_sources(ob_session *s, double *Source) {
   int b, err = OB_OK;
   ob_scratch w;
   scratch_alloc(c, &w);
-  for (b = 0; b < s->nBlock && s->nSlab > 0; b++) {
-    int rc[6];
-    const double *CX, *CY, *CZ;
-    if (!has_kind(s, b, 0)) continue;
-    block_crit(s, b, &CX, &CY, &CZ);
-    block_res_change(s, b, rc);
-    ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w.LX,
-                          w.RX, w.LY, w.RY, w.LZ, w.RZ);
-    err |= ob_calc_face_flux(c, s->CellSize_DB + 3L * b, w.LX, w.RX, w.LY,
-                             w.RY, w.LZ, w.RZ, w.FX, w.FY, w.FZ);
-    save_cons_flux(s, b, w.FX, w.FY, w.FZ, w.LX, w.RX, w.LY, w.RY, w.LZ,
-                   w.RZ);
-  }
+  for (b = 0; b < s->nBlock && s->nSlab > 0; b++)
+    if (has_kind(s, b, 0)) err |= fine_prepass(s, b, &w);
   for (b = 0; b < s->nBlock; b++) {
     int rc[6];
     const double *CX, *CY, *CZ;
@@ -2373,6 +2435,23 @@ int ob_session_sources(ob_session *s, double *Source) {
   }
   scratch_free(&w);
   return err;
+}
+
+int ob_session_b0(const ob_session *s, int b, double *B0X, double *B0Y,
+                  double *B0Z, double *DivB0, double *CurlB0) {
+  const ob_config *c = &s->c;
+  if (!s->B0) return -1;
+  const long nI = c->nI, nJ = c->nJ, nK = c->nK;
+  ob_scratch w;
+  scratch_alloc(c, &w);
+  block_b0(s, b, &w);
+  memcpy(B0X, w.B0X, sizeof(double) * 3 * (nI + 1) * nJ * nK);
+  memcpy(B0Y, w.B0Y, sizeof(double) * 3 * nI * (nJ + 1) * nK);
+  memcpy(B0Z, w.B0Z, sizeof(double) * 3 * nI * nJ * (nK + 1));
+  memcpy(DivB0, w.DivB0, sizeof(double) * nI * nJ * nK);
+  memcpy(CurlB0, w.CurlB0, sizeof(double) * 3 * nI * nJ * nK);
+  scratch_free(&w);
+  return 0;
 }
 
 int ob_advance_stage(ob_session *s, int iStage, int DoCalcTimestep) {

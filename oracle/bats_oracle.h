@@ -330,6 +330,13 @@ void ob_session_face_values(const ob_session *s, int b, double *LX,
                             double *RZ, unsigned char *SkipX,
                             unsigned char *SkipY, unsigned char *SkipZ);
 
+/* the face B0 (B0_DX(3, nI+1, nJ, nK), B0_DY, B0_DZ), DivB0_C and
+ * CurlB0_DC of block b as a stage uses them, with the resolution-change
+ * correction on sides with finer neighbours (set_b0_face, set_b0_source);
+ * returns -1 without B0 */
+int ob_session_b0(const ob_session *s, int b, double *B0X, double *B0Y,
+                  double *B0Z, double *DivB0, double *CurlB0);
+
 /* set_global_timestep (src/ModTimeStepControl.f90:306-524) */
 void ob_set_global_timestep(ob_session *s, double TimeSimulationLimit);
 /* one stage of advance_explicit: all blocks, time advance, ghost exchange

@@ -110,6 +110,9 @@ def lib():
         L.ob_advance_step.restype = C.c_int
         L.ob_session_sources.argtypes = [C.POINTER(ObSession), dp]
         L.ob_session_sources.restype = C.c_int
+        L.ob_session_b0.argtypes = [C.POINTER(ObSession), C.c_int, dp, dp,
+                                    dp, dp, dp]
+        L.ob_session_b0.restype = C.c_int
         L.ob_init_timestep.argtypes = [C.POINTER(ObSession)]
         L.ob_init_timestep.restype = C.c_int
         L.ob_exchange_amr.argtypes = [C.POINTER(ObConfig), C.c_int, C.c_int,
@@ -447,7 +450,7 @@ class Session:
                  UseB0Source=True, Used_GB=None, XyzStart_DB=None,
                  BodyRho=1.0, BodyP=1.0, B1rCoef=-1.0, VaryTable=None):
         """B0_DGB: B0 splitting, the static field (nBlock, nKg, nJg, nIg, 3)
-        at the cell centres and ghost cells (same-level grids).  Used_GB:
+        at the cell centres and ghost cells.  Used_GB:
         an inner body, the true cells (nBlock, nKg, nJg, nIg) of bool;
         XyzStart_DB (nBlock, 3) the centre of each block's first cell
         relative to the body centre; BodyRho, BodyP, B1rCoef the body's
@@ -483,13 +486,11 @@ class Session:
             s.iSlab_IS = self.slabs.ctypes.data_as(C.POINTER(C.c_int))
             s.Corr = _dp(self.Corr)
         if B0_DGB is not None:
-            assert self.nSlab == 0, "B0 on same-level grids only"
             self.B0 = np.ascontiguousarray(B0_DGB, dtype=np.float64)
             assert self.B0.shape[-1] == 3 and self.B0.shape[0] == grid.nBlock
             s.B0 = _dp(self.B0)
             s.UseB0Source = int(UseB0Source)
         if Used_GB is not None:
-            assert self.nSlab == 0, "a body on same-level grids only"
             self.Used = np.ascontiguousarray(Used_GB, dtype=np.uint8)
             assert self.Used.shape == self.State.shape[:-1]
             self.XyzStart = np.ascontiguousarray(XyzStart_DB,
@@ -540,6 +541,21 @@ class Session:
         assert err == 0
         return src.reshape(self.grid.nBlock, self.cfg.nK, self.cfg.nJ,
                            self.cfg.nI, 9)
+
+    def b0(self, b):
+        """Block b's face B0 as a stage uses it (B0X (nK, nJ, nI+1, 3), B0Y
+        (nK, nJ+1, nI, 3), B0Z (nK+1, nJ, nI, 3)), DivB0 (nK, nJ, nI) and
+        CurlB0 (nK, nJ, nI, 3)."""
+        nI, nJ, nK = self.cfg.nI, self.cfg.nJ, self.cfg.nK
+        X = np.zeros((nK, nJ, nI + 1, 3))
+        Y = np.zeros((nK, nJ + 1, nI, 3))
+        Z = np.zeros((nK + 1, nJ, nI, 3))
+        D = np.zeros((nK, nJ, nI))
+        Cu = np.zeros((nK, nJ, nI, 3))
+        err = lib().ob_session_b0(C.byref(self.s), b, _dp(X), _dp(Y), _dp(Z),
+                                  _dp(D), _dp(Cu))
+        assert err == 0
+        return X, Y, Z, D, Cu
 
     def set_global_timestep(self, tlimit=-1.0):
         lib().ob_set_global_timestep(C.byref(self.s), tlimit)

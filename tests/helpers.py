@@ -81,22 +81,54 @@ def rel_err_vec(a, b):
     return (np.abs(a - b).max(axis=1) / scale).max()
 
 
-def body_mask(grid: BlockGrid, rBody, center=(0.5, 0.5, 0.5)):
+def block_geometry(grid):
+    """Lower corner and cell size of every block, arrays (nBlock, 3), for a
+    same-level grid (root positions) or a two-level grid (positions in
+    fine-block units)."""
+    n = np.array([grid.nI, grid.nJ, grid.nK])
+    dx = np.asarray(grid.CellSize_DB, dtype=np.float64).reshape(-1, 3)
+    if hasattr(grid, "pos"):
+        dxf = (grid.xyzMax - grid.xyzMin) / (np.array(grid.nRoot) * 2 * n)
+        org = grid.xyzMin[None, :] + grid.pos * n * dxf[None, :]
+    else:
+        org = grid.xyzMin[None, :] + grid.iRoot_DB * n[None, :] * dx
+    return org, dx
+
+
+def ghost_axes(grid):
+    """Cell-centre coordinates of every block's ghosted cells along x, y and
+    z, arrays (nBlock, n + 2 nG)."""
+    g = grid.nG
+    n = np.array([grid.nI, grid.nJ, grid.nK])
+    org, dx = block_geometry(grid)
+    return [org[:, d, None] + (np.arange(-g, n[d] + g) + 0.5)[None, :] *
+            dx[:, d, None] for d in range(3)]
+
+
+def b0_dgb(grid, field):
+    """B0_DGB (nBlock, nKg, nJg, nIg, 3) of field(x, y, z) at the cell
+    centres and ghost cells of every block."""
+    ax = ghost_axes(grid)
+    x = ax[0][:, None, None, :]
+    y = ax[1][:, None, :, None]
+    z = ax[2][:, :, None, None]
+    shp = (grid.nBlock, ax[2].shape[1], ax[1].shape[1], ax[0].shape[1])
+    x, y, z = (np.broadcast_to(a, shp) for a in (x, y, z))
+    return np.ascontiguousarray(np.stack(field(x, y, z), axis=-1))
+
+
+def body_mask(grid, rBody, center=(0.5, 0.5, 0.5)):
     """True cells outside a sphere of radius rBody at `center`, over every
     block and its ghost cells (cell centres of the ghosted blocks), shape
     (nBlock, nKg, nJg, nIg), and each block's first cell centre relative
     to the sphere's centre, shape (nBlock, 3)."""
-    g = grid.nG
-    n = np.array([grid.nI, grid.nJ, grid.nK])
-    dx = np.asarray(grid.CellSize_DB[0])
-    org = grid.xyzMin[None, :] + grid.iRoot_DB * n[None, :] * dx[None, :]
     c = np.asarray(center, dtype=np.float64)
-    ax = [org[:, d, None] + (np.arange(-g, n[d] + g) + 0.5)[None, :] * dx[d]
-          - c[d] for d in range(3)]
+    ax = [a - c[d] for d, a in enumerate(ghost_axes(grid))]
     x = ax[0][:, None, None, :]
     y = ax[1][:, None, :, None]
     z = ax[2][:, :, None, None]
     used = (x * x + y * y + z * z) > rBody ** 2
-    xyz0 = org + 0.5 * dx[None, :] - c[None, :]
+    org, dx = block_geometry(grid)
+    xyz0 = org + 0.5 * dx - c[None, :]
     return np.ascontiguousarray(used), np.ascontiguousarray(xyz0)
 

@@ -10,7 +10,7 @@ import pytest
 import pyoracle as po
 from batsrus_amd.core import Session, StopMpi
 from batsrus_amd.grid import BlockGrid
-from helpers import smooth_mhd_soa, rel_err
+from helpers import smooth_mhd_soa, rel_err, b0_dgb
 
 pytestmark = pytest.mark.gpu
 
@@ -33,24 +33,6 @@ def b0_dipole(x, y, z, c=(0.5, 0.5, -0.6), m=(0.0, 0.3, 1.0)):
     return ((3 * mr * rx - m[0] * r2) / r5 * 0.05,
             (3 * mr * ry - m[1] * r2) / r5 * 0.05,
             (3 * mr * rz - m[2] * r2) / r5 * 0.05)
-
-
-def b0_dgb(grid, field):
-    """B0_DGB (nBlock, nKg, nJg, nIg, 3) at the cell centres and ghost
-    cells of every block."""
-    g = grid.nG
-    dx = np.asarray(grid.CellSize_DB[0])
-    org = grid.xyzMin[None, :] + grid.iRoot_DB * np.asarray(
-        [grid.nI, grid.nJ, grid.nK]) * dx[None, :]
-    i = np.arange(-g, grid.nI + g) + 0.5
-    j = np.arange(-g, grid.nJ + g) + 0.5
-    k = np.arange(-g, grid.nK + g) + 0.5
-    x = org[:, 0, None, None, None] + i[None, None, None, :] * dx[0]
-    y = org[:, 1, None, None, None] + j[None, None, :, None] * dx[1]
-    z = org[:, 2, None, None, None] + k[None, :, None, None] * dx[2]
-    shp = (grid.nBlock, grid.nK + 2 * g, grid.nJ + 2 * g, grid.nI + 2 * g)
-    x, y, z = (np.broadcast_to(a, shp) for a in (x, y, z))
-    return np.ascontiguousarray(np.stack(field(x, y, z), axis=-1))
 
 
 CASES = [
