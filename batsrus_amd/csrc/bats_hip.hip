@@ -1052,12 +1052,12 @@ static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only, and
   // without the inner body (which the fused stage carries); the TVD
-  // resolution-change faces not with the 8-wave source, low order faces or
-  // steady state
+  // resolution-change faces not with the 8-wave source, low order faces,
+  // steady state or B0
   if (h->elemSize != 8 || h->useBody) return BATS_EINVAL;
   const bats_config &c = h->cfg;
   if (c.UseTvdResChange && h->nResSlab > 0 &&
-      (c.UseDivbSource || low_order_set(c) || c.IsSteadyState))
+      (c.UseDivbSource || low_order_set(c) || c.IsSteadyState || h->useB0))
     return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
@@ -1349,7 +1349,7 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   // source, low order faces, steady state or the surrogate
   const bool tvd = c.UseTvdResChange && h->nResSlab > 0;
   if (tvd && (c.UseDivbSource || low_order_set(c) || steady ||
-              h->sur.mode != BATS_SUR_OFF))
+              h->sur.mode != BATS_SUR_OFF || h->useB0 || h->useBody))
     return BATS_EINVAL;
   // the surrogate's face pass has no low order weights, one beta and no
   // 8-wave source
@@ -1392,6 +1392,8 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
       m.lowCrit = a.lowCrit;
       m.betaRC = a.betaRC;
       m.nFaceRC = a.nFaceRC;
+      m.b0F = h->useB0 ? h->b0F : nullptr;
+      m.nBlock = c.nBlock;
       tvd_args(h, m);
 #define BATS_FPICK(NBv, FL)                                                  \
   (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
@@ -1963,8 +1965,10 @@ extern "C" int bats_set_inflow_table(bats_handle *h, int nPoint,
   return BATS_OK;
 }
 
-// B0 splitting: face B0 (set_b0_face, src/ModB0.f90:326-335), and per cell
-// the B0 itself, div B0 and curl B0 from the face values (set_b0_source,
+// B0 splitting: face B0 (set_b0_face, src/ModB0.f90:326-335; on two-level
+// grids the faces of sides with finer neighbours take the mean of the fine
+// faces' B0, set_b0_reschange, :338-351,366-455), and per cell the B0
+// itself, div B0 and curl B0 from the face values (set_b0_source,
 // :568-590), computed once on the host from B0_DGB
 extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
                            int UseB0Source) {
@@ -1976,9 +1980,9 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
     return BATS_OK;
   }
   const bats_config &c = h->cfg;
-  // FP64 state, same-level grid, no low order faces (the B0 stage
-  // variants do not carry them)
-  if (h->elemSize != 8 || h->nResSlab > 0) return BATS_EINVAL;
+  // FP64 state, no low order faces (the B0 stage variants do not carry
+  // them)
+  if (h->elemSize != 8) return BATS_EINVAL;
   if (low_order_used(c)) return BATS_EINVAL;
   const int NB = h->NB, g = c.nG, NG = NB + 2 * g, nB = c.nBlock;
   for (long t = 0; t < 3L * NG * NG * NG * nB; t++)
@@ -2011,11 +2015,45 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
                          : b0_face_index<16>(b, 2, i, j, k))] =
                   0.5 * (B0(b, i, j, k - 1, d) + B0(b, i, j, k, d));
           }
-  auto face = [&](int dir, int b, int i, int j, int k, int d) {
+  auto face = [&](int dir, int b, int i, int j, int k, int d) -> double & {
     return F[(dir * 3 + d) * nFaceTot +
              (NB == 8 ? b0_face_index<8>(b, dir, i, j, k)
                       : b0_face_index<16>(b, dir, i, j, k))];
   };
+  // a side with finer neighbours (kind-1 slab): the face (p, q) takes 1/8
+  // times the sum over the four fine faces it covers of the fine block's
+  // B0 in its ghost cell and first cell there, summed in the order of the
+  // fine faces' quadrant index
+  const int H = NB / 2;
+  for (size_t t = 0; t < h->userSlabs.size() / 8; t++) {
+    const int *sl = h->userSlabs.data() + 8 * t;
+    if (sl[2] != 1) continue;
+    const int b = sl[0], d = sl[1] / 2, hi = sl[1] % 2;
+    // the fine block's side facing b: ghost -1 and first cell 0 (its low
+    // side) or NB and NB-1 (its high side)
+    const int ng = hi ? -1 : NB, nc = hi ? 0 : NB - 1;
+    auto at = [&](int nrm, int p, int q, int &i, int &j, int &k) {
+      i = d == 0 ? nrm : p;
+      j = d == 1 ? nrm : d == 0 ? p : q;
+      k = d == 2 ? nrm : q;
+    };
+    for (int q = 0; q < NB; q++)
+      for (int p = 0; p < NB; p++) {
+        const int f = sl[3 + p / H + 2 * (q / H)];
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 4; a++) {
+          const int pf = 2 * (p % H) + a % 2, qf = 2 * (q % H) + a / 2;
+          int ig, jg, kg, ic, jc, kc;
+          at(ng, pf, qf, ig, jg, kg);
+          at(nc, pf, qf, ic, jc, kc);
+          for (int v = 0; v < 3; v++)
+            sum[v] = sum[v] + 0.125 * (B0(f, ig, jg, kg, v) + B0(f, ic, jc, kc, v));
+        }
+        int i, j, k;
+        at(hi ? NB : 0, p, q, i, j, k);
+        for (int v = 0; v < 3; v++) face(d, b, i, j, k, v) = sum[v];
+      }
+  }
   for (int b = 0; b < nB; b++) {
     const double *cs = h->userCellSize.data() + 3 * b;
     const double DxInv = 1 / cs[0], DyInv = 1 / cs[1], DzInv = 1 / cs[2];
@@ -2108,10 +2146,10 @@ extern "C" int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
     return BATS_OK;
   }
   const bats_config &c = h->cfg;
-  // FP64 state, same-level grid, no low order faces or resolution-change
-  // beta (the body variant carries neither)
-  if (h->elemSize != 8 || h->nResSlab > 0 || low_order_used(c) ||
-      res_beta_used(h) || !XyzStart_DB)
+  // FP64 state, no low order faces or resolution-change beta (the body
+  // variant carries neither)
+  if (h->elemSize != 8 || low_order_used(c) || res_beta_used(h) ||
+      !XyzStart_DB)
     return BATS_EINVAL;
   // the ghost pressure floats, so BodyP only has to be a number
   if (!(BodyRho > 0.0) || !std::isfinite(BodyRho) || !std::isfinite(BodyP) ||
@@ -2129,6 +2167,10 @@ extern "C" int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
     bool any = false;
     for (long q = 0; q < (long)NGc * NGc * NGc && !any; q++) any = !U[q];
     if (!any) continue;
+    // a block that reaches the body has no resolution-change side (the
+    // fine-face pass carries no body faces)
+    for (int s = 0; s < 6; s++)
+      if (h->userNei[6 * b + s] <= -2) return BATS_EINVAL;
     slot[b] = (int)(masks.size() / nMask);
     for (int k = -2; k < NB + 2; k++)
       for (int j = -2; j < NB + 2; j++)

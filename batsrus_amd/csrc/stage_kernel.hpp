@@ -991,6 +991,9 @@ struct AmrArgs {
   int nVary;
   const double *scal;
   const int *runOfBlock;
+  // B0 splitting: the face B0 of every block (as StageArgs::b0F) or null
+  const double *b0F;
+  int nBlock;
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -1241,9 +1244,27 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
     face_states<LIM>(bL, bR, c[0], c[1], c[2], c[3], L, R);
   }
   int err;
-  if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
-  else if (d == 1) err = numerical_flux<1, FLUX, false>(ph, area, L, R, F, vdt, unA);
-  else err = numerical_flux<2, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  bool withB0 = false;
+  if constexpr (sizeof(T) == 8) {
+    if (a.b0F) {
+      // the fine block's own face B0 (B0 splitting)
+      withB0 = true;
+      int x, y, z;
+      side_xyz(d, f0, p, q, x, y, z);
+      const long nFaceTot = (long)a.nBlock * (NB + 1) * NB * NB;
+      const long fi = b0_face_index<NB>(b, d, x, y, z);
+      T B0f[3];
+      for (int v = 0; v < 3; v++) B0f[v] = a.b0F[(d * 3 + v) * nFaceTot + fi];
+      if (d == 0) err = numerical_flux<0, FLUX, false, true>(ph, area, L, R, F, vdt, unA, B0f);
+      else if (d == 1) err = numerical_flux<1, FLUX, false, true>(ph, area, L, R, F, vdt, unA, B0f);
+      else err = numerical_flux<2, FLUX, false, true>(ph, area, L, R, F, vdt, unA, B0f);
+    }
+  }
+  if (!withB0) {
+    if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
+    else if (d == 1) err = numerical_flux<1, FLUX, false>(ph, area, L, R, F, vdt, unA);
+    else err = numerical_flux<2, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  }
   F[PP] = vdt;
   const int pc = sl[4] + p / 2, qc = sl[5] + q / 2, sub = (p % 2) + 2 * (q % 2);
   T *dst = static_cast<T *>(a.corr) +

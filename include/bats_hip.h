@@ -182,9 +182,13 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
  * (src/ModCalcSource.f90:1792-1808,1880-1885) and, with UseB0Source,
  * -B1 div(B0) - curl(B0) x B1 from the face B0 (:953-967, set_b0_source
  * :568-590).  The ghost B0 of the outer sides serves the fixed, inflow and
- * vary ghost cells.  FP64 handles on same-level grids only (time accurate
- * or steady state); not with adaptive low order faces (BATS_EINVAL).  A
- * new grid drops B0; B0_DGB = NULL drops it too. */
+ * vary ghost cells.  On two-level grids a face of a side with finer
+ * neighbours takes the mean of the four fine faces' B0 (set_b0_reschange,
+ * src/ModB0.f90:338-351,366-455) and the fine blocks' face fluxes that
+ * replace the coarse ones carry their B0.  FP64 handles, time accurate or
+ * steady state; not with adaptive low order faces or the TVD
+ * resolution-change faces (BATS_EINVAL at the stage).  A new grid drops
+ * B0; B0_DGB = NULL drops it too. */
 int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
 
 /* Inner body (UseBody with the 'ionosphere' face boundary condition, as the
@@ -202,10 +206,11 @@ int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
  * default); faces between two body cells carry no flux; body cells are not
  * updated and hold DtMax_CB = 0 outside the block minimum
  * (src/ModTimeStepControl.f90:158-159,255).  Works with B0 and the 8-wave
- * source, time accurate or steady state; FP64 same-level grids only, not
- * with adaptive low order faces or the resolution-change beta, and not
- * through the per-block routines (BATS_EINVAL).  A new grid drops the body;
- * Used_GB = NULL drops it too. */
+ * source, time accurate or steady state; FP64 handles, on two-level grids
+ * only where no block that reaches the body has a resolution-change side;
+ * not with adaptive low order faces, the resolution-change beta or the TVD
+ * resolution-change faces, and not through the per-block routines
+ * (BATS_EINVAL).  A new grid drops the body; Used_GB = NULL drops it too. */
 int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
                   const double *XyzStart_DB, double BodyRho, double BodyP,
                   double B1rCoef);
