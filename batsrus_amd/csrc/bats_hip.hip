@@ -1394,6 +1394,13 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
       m.nFaceRC = a.nFaceRC;
       m.b0F = h->useB0 ? h->b0F : nullptr;
       m.nBlock = c.nBlock;
+      if (h->useBody) {
+        m.bodySlot = h->bodySlot;
+        m.used = h->bodyUsed;
+        m.xyz0 = h->bodyXyz0;
+        m.bodyRho = h->bodyRho;
+        m.b1rCoef = h->b1rCoef;
+      }
       tvd_args(h, m);
 #define BATS_FPICK(NBv, FL)                                                  \
   (f32 ? launch_fineflux_##NBv##_##FL##_f32(m, h->stream)                    \
@@ -2167,10 +2174,6 @@ extern "C" int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
     bool any = false;
     for (long q = 0; q < (long)NGc * NGc * NGc && !any; q++) any = !U[q];
     if (!any) continue;
-    // a block that reaches the body has no resolution-change side (the
-    // fine-face pass carries no body faces)
-    for (int s = 0; s < 6; s++)
-      if (h->userNei[6 * b + s] <= -2) return BATS_EINVAL;
     slot[b] = (int)(masks.size() / nMask);
     for (int k = -2; k < NB + 2; k++)
       for (int j = -2; j < NB + 2; j++)

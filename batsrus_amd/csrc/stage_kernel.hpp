@@ -994,6 +994,12 @@ struct AmrArgs {
   // B0 splitting: the face B0 of every block (as StageArgs::b0F) or null
   const double *b0F;
   int nBlock;
+  // the inner body (as StageArgs: mask slot per block, masks, first cell
+  // centres, ghost face state) or used = null
+  const int *bodySlot;
+  const unsigned char *used;
+  const double *xyz0;
+  double bodyRho, b1rCoef;
 };
 
 __device__ __forceinline__ void side_xyz(int d, int n, int p, int q, int &x,
@@ -1243,7 +1249,35 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
   } else {
     face_states<LIM>(bL, bR, c[0], c[1], c[2], c[3], L, R);
   }
-  int err;
+  // the inner body: the fine block's face as the stage treats it (first
+  // order next to body cells, the ghost face state between a true and a
+  // body cell, no flux between two body cells); the record keeps the face
+  // states' normal field either way
+  bool skip = false;
+  if (a.used && a.bodySlot[b] >= 0) {
+    constexpr int NG = NB + 4;
+    const unsigned char *m = a.used + (long)a.bodySlot[b] * NG * NG * NG;
+    bool u[4];
+    for (int s4 = 0; s4 < 4; s4++) {
+      int x, y, z;
+      side_xyz(d, (hi ? NB : 0) + s4 - 2, p, q, x, y, z);
+      u[s4] = m[((z + 2) * NG + (y + 2)) * NG + (x + 2)] != 0;
+    }
+    T r[3] = {T(0), T(0), T(0)};
+    if (u[1] != u[2]) {
+      // centre of the face between cells f0-1 and f0 along the normal
+      int x, y, z;
+      side_xyz(d, f0 - 1, p, q, x, y, z);
+      const int ci[3] = {x, y, z};
+      for (int v = 0; v < 3; v++) {
+        const double h = a.cellsize[3 * b + v], x0 = a.xyz0[3 * b + v];
+        r[v] = T(0.5 * ((x0 + ci[v] * h) + (x0 + (ci[v] + (v == d)) * h)));
+      }
+    }
+    skip = body_face<LIM>(u[0], u[1], u[2], u[3], c[1], c[2], L, R, r,
+                          T(a.bodyRho), T(a.b1rCoef));
+  }
+  int err = 0;
   bool withB0 = false;
   if constexpr (sizeof(T) == 8) {
     if (a.b0F) {
@@ -1264,6 +1298,11 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
     if (d == 0) err = numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
     else if (d == 1) err = numerical_flux<1, FLUX, false>(ph, area, L, R, F, vdt, unA);
     else err = numerical_flux<2, FLUX, false>(ph, area, L, R, F, vdt, unA);
+  }
+  if (skip) {
+    for (int v = 0; v < NFLUX; v++) F[v] = T(0);
+    vdt = T(0);
+    err = 0;
   }
   F[PP] = vdt;
   const int pc = sl[4] + p / 2, qc = sl[5] + q / 2, sub = (p % 2) + 2 * (q % 2);

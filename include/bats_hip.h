@@ -206,11 +206,12 @@ int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
  * default); faces between two body cells carry no flux; body cells are not
  * updated and hold DtMax_CB = 0 outside the block minimum
  * (src/ModTimeStepControl.f90:158-159,255).  Works with B0 and the 8-wave
- * source, time accurate or steady state; FP64 handles, on two-level grids
- * only where no block that reaches the body has a resolution-change side;
- * not with adaptive low order faces, the resolution-change beta or the TVD
- * resolution-change faces, and not through the per-block routines
- * (BATS_EINVAL).  A new grid drops the body; Used_GB = NULL drops it too. */
+ * source, time accurate or steady state, on same-level and two-level
+ * grids (the fine faces that replace coarse ones get the same treatment);
+ * FP64 handles, not with adaptive low order faces, the resolution-change
+ * beta or the TVD resolution-change faces, and not through the per-block
+ * routines (BATS_EINVAL).  A new grid drops the body; Used_GB = NULL drops
+ * it too. */
 int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
                   const double *XyzStart_DB, double BodyRho, double BodyP,
                   double B1rCoef);

@@ -5,9 +5,9 @@ src/ModB0.f90:338-351,366-455) and the per-block routines in the
 reference's advance_explicit order, stage by stage over two steps against
 the oracle's stage update within 1e-12; time accurate and steady state,
 with and without the 8-wave source and its B0 terms; an ionosphere body
-inside the fine level with a dipole B0 (the body's blocks meet no
-resolution change); and a body block at a resolution change, or the
-TVD resolution-change faces with B0, are refused."""
+with a dipole B0, inside the fine level and across resolution changes
+(the fine-face pass with the body's face treatment); and the TVD
+resolution-change faces with B0 are refused."""
 import numpy as np
 import pytest
 
@@ -123,35 +123,55 @@ def _body_grid():
     return g, used, xyz0
 
 
-@pytest.mark.parametrize("steady", [False, True])
-def test_fused_two_level_body_with_b0_matches_oracle(steady):
-    print(f"case fused two-level body steady={steady}", flush=True)
-    g, used, xyz0 = _body_grid()
+def _across_grid():
+    """A body of radius 0.1 at the corner the refined and unrefined roots
+    share: fine blocks with coarser sides and coarse blocks with finer
+    sides reach it."""
+    g = _grid()
+    used, xyz0 = body_mask(g, 0.1)
+    return g, used, xyz0
+
+
+BODY_CASES = [("inside", "hlld", "mc", False), ("inside", "hlld", "mc", True),
+              ("across", "hlld", "mc3", False), ("across", "hlld", "mc", True),
+              ("across", "linde", "minmod", False)]
+
+
+@pytest.mark.parametrize("where,flux,limiter,steady", BODY_CASES)
+def test_fused_two_level_body_with_b0_matches_oracle(where, flux, limiter,
+                                                     steady):
+    print(f"case fused two-level body {where} {flux} {limiter} "
+          f"steady={steady}", flush=True)
+    g, used, xyz0 = _body_grid() if where == "inside" else _across_grid()
     nb = ~used.reshape(g.nBlock, -1).all(axis=1)
-    assert nb.sum() == 8
     nei = g.iNeighbor_EB.reshape(-1, 6)
-    assert (nei[nb] >= 0).all()
+    kinds = {int(g.iSlab_IS[-2 - n][2]) for n in nei[nb].ravel() if n <= -2}
+    if where == "inside":
+        assert nb.sum() == 8 and not kinds
+    else:
+        assert kinds == {0, 1}
     U0 = smooth_mhd_soa(g, seed=94, amp=0.2)
     B0 = b0_dgb(g, lambda x, y, z: _dipole(x, y, z, c=(0.5, 0.5, 0.5),
                                            m=(0.0, 0.0, -1.0), scale=2e-4))
-    _compare(g, dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8,
+    _compare(g, dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
                      UseDivbSource=True, IsSteadyState=steady), U0, B0,
              body=(used, xyz0, dict(BodyRho=3.0, BodyP=0.5, B1rCoef=-1.0)),
              steady=steady)
 
 
-def test_two_level_b0_and_body_refusals():
-    print("case two-level b0/body refusals", flush=True)
+def test_two_level_tvd_refusals():
+    print("case two-level tvd with b0/body refusals", flush=True)
     g = _grid()
-    # a body block with a resolution-change side
-    used, xyz0 = body_mask(g, 0.1, center=(0.5, 0.5, 0.5))
-    with Session(g, flux="hlld", limiter="mc") as s:
-        with pytest.raises(StopMpi):
-            s.set_body(used, xyz0)
-    # the TVD resolution-change faces are not combined with B0
+    # the TVD resolution-change faces are not combined with B0 or a body
     U0 = smooth_mhd_soa(g, seed=95)
     with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
         s.set_b0(b0_dgb(g, _dipole))
+        s.upload_soa(U0)
+        with pytest.raises(StopMpi):
+            s.init_timestep()
+    used, xyz0 = body_mask(g, 0.1)
+    with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
+        s.set_body(used, xyz0)
         s.upload_soa(U0)
         with pytest.raises(StopMpi):
             s.init_timestep()

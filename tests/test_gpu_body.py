@@ -193,13 +193,14 @@ def test_body_refusals():
         # dropping the body gives them back
         s.set_body(None)
         s.calc_face_value(1)
-    # a body block at a resolution change of a two-level grid
+    # the sharper resolution-change limiter on a two-level grid
     ag = AmrGrid((4, 4, 4), sphere_refinement((4, 4, 4), 0.3))
     n = ag.nI + 2 * ag.nG
-    b = int(np.nonzero((ag.iNeighbor_EB.reshape(-1, 6) <= -2).any(1))[0][0])
     au = np.ones((ag.nBlock, n, n, n), bool)
-    au[b, 4, 4, 4] = False
+    au[0, 4, 4, 4] = False
     ax = np.zeros((ag.nBlock, 3))
-    with Session(ag, flux="hlld", limiter="mc") as s:
+    with Session(ag, flux="hlld", limiter="mc", beta=1.5,
+                 BetaLimiterResChange=1.2,
+                 nFaceLimiterResChange=2) as s:
         with pytest.raises(StopMpi):
             s.set_body(au, ax, **BODY)
