@@ -115,6 +115,13 @@ struct BlockArgs {
   int tvd;                              // UseTvdResChange (second order)
   double tvdBeta;                       // min(BetaLimiterResChange, beta)
   int steady;                           // steady state: per-cell steps
+  // the inner body: mask slot per block (-1: no body cell within the
+  // block's halo), the true-cell masks over the block and its 2-cell halo,
+  // first cell centres, the ghost face state (used = null: no body)
+  const int *bodySlot;
+  const unsigned char *used;
+  const double *xyz0;
+  double bodyRho, b1rCoef;
   double *dtmaxCB, *dtmaxB;
   int *err;
   int nBlock, iBlock;
@@ -136,6 +143,21 @@ __device__ inline bool coarser_side_face(const BlockArgs &a, int dir, int fi,
   if (f != 0 && f != NB) return false;
   const int n = a.nei[6 * a.iBlock + 2 * dir + (f == NB)];
   return n <= -2 && a.slabKind[-2 - n] == 0;
+}
+
+// the true-cell mask of the block (null: no body cell within its halo)
+// and the flag of its cell (ii, jj, kk), each index in -2 .. NB+1
+template <int NB>
+__device__ inline const unsigned char *block_body_mask(const BlockArgs &a) {
+  if (!a.used) return nullptr;
+  const int sl = a.bodySlot[a.iBlock];
+  return sl < 0 ? nullptr : a.used + (long)sl * (NB + 4) * (NB + 4) * (NB + 4);
+}
+template <int NB>
+__device__ inline bool block_true(const unsigned char *m, int ii, int jj,
+                                  int kk) {
+  constexpr int NG = NB + 4;
+  return !m || m[((kk + 2) * NG + (jj + 2)) * NG + (ii + 2)] != 0;
 }
 
 template <int NB>
@@ -221,6 +243,33 @@ __global__ void k_block_face_value(BlockArgs a) {
       }
     }
   }
+  // the inner body: the fast update's face treatment, as the fused stage
+  // applies it (src/ModUpdateStateFast.f90:924-937,1277-1327)
+  if (const unsigned char *m = block_body_mask<NB>(a)) {
+    bool u[4];
+    for (int s = 0; s < 4; s++)
+      u[s] = block_true<NB>(m, fi - 2 * (dir == 0) + s * (dir == 0),
+                            fj - 2 * (dir == 1) + s * (dir == 1),
+                            fk - 2 * (dir == 2) + s * (dir == 2));
+    double r[3] = {0.0, 0.0, 0.0};
+    if (u[1] != u[2]) {
+      // the face centre: the mean of its two cells' centres
+      const int c[3] = {fi - (dir == 0), fj - (dir == 1), fk - (dir == 2)};
+      for (int v = 0; v < 3; v++) {
+        const double h = a.cellsize[3 * a.iBlock + v];
+        const double x0 = a.xyz0[3 * a.iBlock + v];
+        r[v] = 0.5 * ((x0 + c[v] * h) + (x0 + (c[v] + (v == dir)) * h));
+      }
+    }
+    // the limiter only decides whether the face values fall back to
+    // first order next to body cells
+    if (a.lim == LIM_FIRST)
+      body_face<LIM_FIRST>(u[0], u[1], u[2], u[3], q[1], q[2], Lv, Rv, r,
+                           a.bodyRho, a.b1rCoef);
+    else
+      body_face<LIM_MC>(u[0], u[1], u[2], u[3], q[1], q[2], Lv, Rv, r,
+                        a.bodyRho, a.b1rCoef);
+  }
   for (int v = 0; v < NVAR; v++) {
     L[fo * NVAR + v] = Lv[v];
     R[fo * NVAR + v] = Rv[v];
@@ -255,14 +304,12 @@ __global__ void k_block_face_flux(BlockArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * nFx) return;
   const int dir = t / nFx, r = t % nFx;
-  if (a.resOnly) {
-    const int fi = dir == 0 ? r % (NB + 1) : r % NB;
-    const int fj = dir == 0 ? (r / (NB + 1)) % NB
-                 : dir == 1 ? (r / NB) % (NB + 1) : (r / NB) % NB;
-    const int fk = dir == 0 ? r / ((NB + 1) * NB)
-                 : dir == 1 ? r / (NB * (NB + 1)) : r / (NB * NB);
-    if (!coarser_side_face(a, dir, fi, fj, fk, NB)) return;
-  }
+  const int fi = dir == 0 ? r % (NB + 1) : r % NB;
+  const int fj = dir == 0 ? (r / (NB + 1)) % NB
+               : dir == 1 ? (r / NB) % (NB + 1) : (r / NB) % NB;
+  const int fk = dir == 0 ? r / ((NB + 1) * NB)
+               : dir == 1 ? r / (NB * (NB + 1)) : r / (NB * NB);
+  if (a.resOnly && !coarser_side_face(a, dir, fi, fj, fk, NB)) return;
   const double *cs = a.cellsize + 3 * a.iBlock;
   double L[NVAR], R[NVAR], F[NFLUX], vdt, unA;
   const double *Ls = dir == 0 ? a.LX : (dir == 1 ? a.LY : a.LZ);
@@ -285,6 +332,15 @@ __global__ void k_block_face_flux(BlockArgs a) {
   double *o = Fo + (long)r * 14;
   for (int v = 0; v < NFLUX; v++) o[v] = F[v];
   o[9] = unA; o[10] = 0.0; o[11] = vdt; o[12] = 0.0; o[13] = 0.0;
+  // a face between two body cells carries nothing
+  // (src/ModUpdateStateFast.f90:237-240)
+  if (const unsigned char *m = block_body_mask<NB>(a)) {
+    if (!block_true<NB>(m, fi - (dir == 0), fj - (dir == 1), fk - (dir == 2)) &&
+        !block_true<NB>(m, fi, fj, fk)) {
+      for (int v = 0; v < 14; v++) o[v] = 0.0;
+      err = 0;
+    }
+  }
   if (err) atomicOr(a.err, err);
 }
 
@@ -437,7 +493,9 @@ __global__ void k_block_update(BlockArgs a) {
   }
   update_cell(a.ph, u, dF, St, So, Un,
               a.useSrc ? a.Src + (long)c * NFLUX : nullptr);
-  for (int v = 0; v < NVAR; v++) a.State[v * a.vs + off] = Un[v];
+  // body cells are not updated (src/ModUpdateStateFast.f90:343-345)
+  if (block_true<NB>(block_body_mask<NB>(a), i, j, k))
+    for (int v = 0; v < NVAR; v++) a.State[v * a.vs + off] = Un[v];
   if (!u.dtZero)  // p -> E -> p of StateOld (src/ModUpdateState.f90:606,965)
     a.StateOld[PP * a.vs + off] =
         energy_to_pressure(a.ph, So, pressure_to_energy(a.ph, So));
@@ -459,6 +517,12 @@ __global__ void __launch_bounds__(512) k_block_timestep(BlockArgs a) {
                        fmax2(a.FZ[fz0 + 11], a.FZ[fz0 + (long)NB * NB * 14 + 11]);
     const double dtc = V / Vdt;
     const long off = (((long)a.iBlock * NB + k) * NB + j) * NB + i;
+    // body cells hold a zero step outside the block minimum
+    // (src/ModTimeStepControl.f90:158-159,255)
+    if (!block_true<NB>(block_body_mask<NB>(a), i, j, k)) {
+      a.dtmaxCB[off] = 0.0;
+      continue;
+    }
     m = fmin2(m, dtc);
     a.dtmaxCB[off] = a.useDtFixed ? a.DtFixed : dtc;
   }
@@ -1003,6 +1067,11 @@ static BlockArgs block_args(bats_handle *h, int iBlock) {
   a.tvd = c.UseTvdResChange && h->nResSlab > 0 && c.nOrder > 1;
   a.tvdBeta = std::min(c.BetaLimiterResChange, c.LimiterBeta);
   a.steady = c.IsSteadyState;
+  a.used = h->useBody ? h->bodyUsed : nullptr;
+  a.bodySlot = h->bodySlot;
+  a.xyz0 = h->bodyXyz0;
+  a.bodyRho = h->bodyRho;
+  a.b1rCoef = h->b1rCoef;
   a.dtmaxCB = h->dtmaxCB;
   a.dtmaxB = h->dtmaxB;
   a.err = h->err;
@@ -1050,14 +1119,14 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
-  // the reference-shaped per-block routines run on FP64 state only, and
-  // without the inner body (which the fused stage carries); the TVD
-  // resolution-change faces not with the 8-wave source, low order faces,
-  // steady state or B0
-  if (h->elemSize != 8 || h->useBody) return BATS_EINVAL;
+  // the reference-shaped per-block routines run on FP64 state only; the
+  // TVD resolution-change faces not with the 8-wave source, low order
+  // faces, steady state, B0 or the body
+  if (h->elemSize != 8) return BATS_EINVAL;
   const bats_config &c = h->cfg;
   if (c.UseTvdResChange && h->nResSlab > 0 &&
-      (c.UseDivbSource || low_order_set(c) || c.IsSteadyState || h->useB0))
+      (c.UseDivbSource || low_order_set(c) || c.IsSteadyState || h->useB0 ||
+       h->useBody))
     return BATS_EINVAL;
   return ensure_block_buffers(h);
 }

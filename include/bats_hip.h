@@ -207,11 +207,14 @@ int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
  * updated and hold DtMax_CB = 0 outside the block minimum
  * (src/ModTimeStepControl.f90:158-159,255).  Works with B0 and the 8-wave
  * source, time accurate or steady state, on same-level and two-level
- * grids (the fine faces that replace coarse ones get the same treatment);
- * FP64 handles, not with adaptive low order faces, the resolution-change
- * beta or the TVD resolution-change faces, and not through the per-block
- * routines (BATS_EINVAL).  A new grid drops the body; Used_GB = NULL drops
- * it too. */
+ * grids (the fine faces that replace coarse ones get the same treatment),
+ * through the fused entry points and the per-block routines
+ * (bats_calc_face_value applies the face treatment that set_face_boundary
+ * applies after calc_face_value in the reference, calc_face_flux zeroes
+ * the body-body faces, update_state keeps body cells, calc_timestep masks
+ * them); FP64 handles, not with adaptive low order faces, the
+ * resolution-change beta or the TVD resolution-change faces (BATS_EINVAL).
+ * A new grid drops the body; Used_GB = NULL drops it too. */
 int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
                   const double *XyzStart_DB, double BodyRho, double BodyP,
                   double B1rCoef);

@@ -123,6 +123,18 @@ def _body_grid():
     return g, used, xyz0
 
 
+def test_per_block_two_level_body_matches_oracle():
+    print("case per-block two-level body across resolution changes",
+          flush=True)
+    g, used, xyz0 = _across_grid()
+    U0 = smooth_mhd_soa(g, seed=96, amp=0.2)
+    B0 = b0_dgb(g, lambda x, y, z: _dipole(x, y, z, c=(0.5, 0.5, 0.5),
+                                           m=(0.0, 0.0, -1.0), scale=2e-4))
+    _compare(g, dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8,
+                     UseDivbSource=True), U0, B0, per_block=True,
+             body=(used, xyz0, dict(BodyRho=3.0, BodyP=0.5, B1rCoef=-1.0)))
+
+
 def _across_grid():
     """A body of radius 0.1 at the corner the refined and unrefined roots
     share: fine blocks with coarser sides and coarse blocks with finer
