@@ -8,6 +8,10 @@
 !    PARAM.nml      namelist &RUN (grid shape, scheme, nStep, TypeLoop)
 !    grid.bin       iNeighbor_EB(6,nBlock) int32, CellSize_DB(3,nBlock) real8
 !    state_in.bin   State_VGB(8,-1:nI+2,-1:nJ+2,-1:nK+2,nBlock) real8
+!  and, with UseB0 / UseBody,
+!    b0.bin         B0_DGB(3,-1:nI+2,-1:nJ+2,-1:nK+2,nBlock) real8
+!    body.bin       Used_GB(-1:nI+2,-1:nJ+2,-1:nK+2,nBlock) int8 (1: true
+!                   cell), then XyzStart_DB(3,nBlock) real8
 !  and receives state_out.bin (same layout) and log.txt.
 !  TypeLoop='blocks' calls the per-block routines in the reference's order;
 !  TypeLoop='fused' calls set_global_timestep + advance_explicit.
@@ -37,17 +41,24 @@ program bats_fortran_driver
   real(c_double):: pOutflow = 0.0
   ! #TIMEACCURATE (false: steady state, per-cell time steps)
   logical:: IsTimeAccurate = .true.
+  ! B0 splitting (B0_DGB from b0.bin) and the inner body (Used_GB and
+  ! XyzStart_DB from body.bin) with its face state and B1rCoef
+  logical:: UseB0 = .false., UseB0Source = .true., UseBody = .false.
+  real(c_double):: BodyRho = 1.0, BodyP = 1.0, B1rCoef = -1.0
   namelist /RUN/ nRootI, nRootJ, nRootK, nCell, nStep, nOrder, nStage, &
        iLimiter, iFlux, LimiterBeta, Cfl, Gamma, TypeLoop, UseDivbSource, &
        UseAdaptiveLowOrder, nLowOrder, pCritLow, pCritHigh, VelCrit, &
        iTypeCellBc, CellState_VI, BetaLimiterResChange, nFaceLimiterResChange, &
-       UseOutflowPressure, pOutflow, IsTimeAccurate
+       UseOutflowPressure, pOutflow, IsTimeAccurate, UseB0, UseB0Source, &
+       UseBody, BodyRho, BodyP, B1rCoef
 
   type(bats_config):: Config
   character(len=512):: NameDir
   integer:: nBlock, iBlock, iStep, iStage, iUnit, nG
   integer(c_int), allocatable:: iNeighbor_EB(:,:)
   real(c_double), allocatable:: CellSize_DB(:,:), State_VGB(:,:,:,:,:)
+  real(c_double), allocatable:: B0_DGB(:,:,:,:,:), XyzStart_DB(:,:)
+  integer(c_signed_char), allocatable:: Used_GB(:,:,:,:)
   real(c_double):: Dt, tSimulation
   integer(c_int):: nStepDone
   !----------------------------------------------------------------------------
@@ -92,6 +103,28 @@ program bats_fortran_driver
   call init_hip_bridge(Config, iNeighbor_EB, CellSize_DB)
   if(any(iTypeCellBc /= BATS_BC_FLOAT)) &
        call set_outer_boundary(iTypeCellBc, CellState_VI)
+  ! B0 and the body after the grid and boundaries (a new grid drops them)
+  if(UseB0)then
+     allocate(B0_DGB(3,1-nG:nCell+nG,1-nG:nCell+nG,1-nG:nCell+nG,nBlock))
+     open(newunit=iUnit, file=trim(NameDir)//'/b0.bin', access='stream', &
+          form='unformatted', status='old')
+     read(iUnit) B0_DGB
+     close(iUnit)
+     call bats_check(bats_set_b0(bridge_handle(), B0_DGB, &
+          merge(1, 0, UseB0Source)), 'set_b0')
+  end if
+  if(UseBody)then
+     allocate(Used_GB(1-nG:nCell+nG,1-nG:nCell+nG,1-nG:nCell+nG,nBlock), &
+          XyzStart_DB(3,nBlock))
+     open(newunit=iUnit, file=trim(NameDir)//'/body.bin', access='stream', &
+          form='unformatted', status='old')
+     read(iUnit) Used_GB, XyzStart_DB
+     close(iUnit)
+     ! the body's face treatment is part of calc_face_value here, so the
+     ! loop below has no set_face_boundary call
+     call bats_check(bats_set_body(bridge_handle(), Used_GB, XyzStart_DB, &
+          BodyRho, BodyP, B1rCoef), 'set_body')
+  end if
   call sync_state_to_gpu(State_VGB)
 
   if(TypeLoop == 'blocks')then
