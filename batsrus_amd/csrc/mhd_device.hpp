@@ -741,54 +741,56 @@ template <typename Real> struct CellB0 {
 
 // 8-wave (Powell) div B source of one cell on a Cartesian grid
 // (src/ModCalcSource.f90:1694-1889 with DoCorrectFace false, :930-951,
-// :1016-1022), with the B0 terms when B0s is given.  B?lo/B?hi are the
-// normal-field left (L) and right (R) face states at the lower and upper
-// faces of the cell in each direction; InvHalf = 0.5/CellSize; St is the
-// cell's stage-input state.
-template <typename Real>
-__device__ __forceinline__ void powell_source(
+// :1016-1022), with the B0 terms.  B?lo/B?hi are the normal-field left (L)
+// and right (R) face states at the lower and upper faces of the cell in
+// each direction; InvHalf = 0.5/CellSize; St is the cell's stage-input
+// state.  B0 splitting: faceB0(f, out) gives the B0 of the cell's face f
+// (x-, x+, y-, y+, z-, z+), cellB0(q) the cell's B0 (q = 0..2), div B0
+// (q = 3) and curl B0 (q = 4..6); the faces are fetched one at a time, each
+// consumed before the next is asked for, which keeps the fused stage's
+// register use down.
+template <bool B0S, typename Real, typename FaceB0, typename CellV>
+__device__ __forceinline__ void powell_source_with(
     const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
     Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
-    Real LzHi, Real RzHi, const Real *St, Real *Src,
-    const CellB0<Real> *B0s = nullptr) {
-  const Real dB1 = InvHalf[0] * (RxLo - LxLo);
-  const Real dB2 = InvHalf[0] * (RxHi - LxHi);
-  const Real dB3 = InvHalf[1] * (RyLo - LyLo);
-  const Real dB4 = InvHalf[1] * (RyHi - LyHi);
-  const Real dB5 = InvHalf[2] * (RzLo - LzLo);
-  const Real dB6 = InvHalf[2] * (RzHi - LzHi);
+    Real LzHi, Real RzHi, const Real *St, Real *Src, FaceB0 &&faceB0,
+    CellV &&cellB0, int useB0Source) {
+  const Real dBf[6] = {InvHalf[0] * (RxLo - LxLo), InvHalf[0] * (RxHi - LxHi),
+                       InvHalf[1] * (RyLo - LyLo), InvHalf[1] * (RyHi - LyHi),
+                       InvHalf[2] * (RzLo - LzLo), InvHalf[2] * (RzHi - LzHi)};
   Real Internal = 2 * InvHalf[0] * (LxHi - RxLo);
   Internal = Internal + 2 * InvHalf[1] * (LyHi - RyLo);
   Internal = Internal + 2 * InvHalf[2] * (LzHi - RzLo);
-  Real DivB = Internal + dB1 + dB2;
-  DivB = DivB + dB3 + dB4;
-  DivB = DivB + dB5 + dB6;
+  Real DivB = Internal + dBf[0] + dBf[1];
+  DivB = DivB + dBf[2] + dBf[3];
+  DivB = DivB + dBf[4] + dBf[5];
   const Real RhoInv = frcp(St[RHO]);
   const Real ux = RhoInv * St[MX], uy = RhoInv * St[MY],
              uz = RhoInv * St[MZ];
   Src[RHO] = Real(0);
-  if (B0s) {
-    // B0 splitting: the face B0 times each face's normal-field jump, the
-    // cell B0 times the internal divergence (src/ModCalcSource.f90:
-    // 1792-1808,1880-1885), then -B1 div(B0) - curl(B0) x B1 with
-    // UseB0Source (:953-967); B0s->face[f] holds the B0 of the cell's
-    // faces x-, x+, y-, y+, z-, z+
-    const Real dBf[6] = {dB1, dB2, dB3, dB4, dB5, dB6};
-    Real m[3];
+  if (B0S) {
+    // the face B0 times each face's normal-field jump, the cell B0 times
+    // the internal divergence (src/ModCalcSource.f90:1792-1808,
+    // 1880-1885), then -B1 div(B0) - curl(B0) x B1 with UseB0Source
+    // (:953-967)
+    Real m[3] = {Real(0), Real(0), Real(0)};
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-      Real t = Real(0) - B0s->face[0][v] * dBf[0] - B0s->face[1][v] * dBf[1];
-      t = t - B0s->face[2][v] * dBf[2] - B0s->face[3][v] * dBf[3];
-      t = t - B0s->face[4][v] * dBf[4] - B0s->face[5][v] * dBf[5];
-      t = t - Internal * B0s->cell[v];
-      m[v] = t - DivB * St[BX + v];
+    for (int f = 0; f < 6; f++) {
+      Real B0f[3];
+      faceB0(f, B0f);
+#pragma unroll
+      for (int v = 0; v < 3; v++) m[v] = m[v] - B0f[v] * dBf[f];
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (B0s->useSource) {
-      const Real *cB = B0s->curl;
-      const Real Bx = St[BX], By = St[BY], Bz = St[BZ], dv = B0s->div;
-      m[0] = m[0] - Bx * dv - (cB[1] * Bz - cB[2] * By);
-      m[1] = m[1] - By * dv - (cB[2] * Bx - cB[0] * Bz);
-      m[2] = m[2] - Bz * dv - (cB[0] * By - cB[1] * Bx);
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+      m[v] = (m[v] - Internal * cellB0(v)) - DivB * St[BX + v];
+    if (useB0Source) {
+      const Real Bx = St[BX], By = St[BY], Bz = St[BZ], dv = cellB0(3);
+      const Real c0 = cellB0(4), c1 = cellB0(5), c2 = cellB0(6);
+      m[0] = m[0] - Bx * dv - (c1 * Bz - c2 * By);
+      m[1] = m[1] - By * dv - (c2 * Bx - c0 * Bz);
+      m[2] = m[2] - Bz * dv - (c0 * By - c1 * Bx);
     }
     Src[MX] = m[0];
     Src[MY] = m[1];
@@ -803,6 +805,30 @@ __device__ __forceinline__ void powell_source(
   Src[BZ] = -(DivB * uz);
   Src[PP] = Real(0);
   Src[EN] = -(DivB * (St[BX] * ux + St[BY] * uy + St[BZ] * uz));
+}
+
+// the same with the B0 values of the cell in a CellB0 (null: no B0)
+template <typename Real>
+__device__ __forceinline__ void powell_source(
+    const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
+    Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
+    Real LzHi, Real RzHi, const Real *St, Real *Src,
+    const CellB0<Real> *B0s = nullptr) {
+  auto face = [&](int f, Real *o) {
+#pragma unroll
+    for (int v = 0; v < 3; v++) o[v] = B0s->face[f][v];
+  };
+  auto cell = [&](int q) {
+    return q < 3 ? B0s->cell[q] : q == 3 ? B0s->div : B0s->curl[q - 4];
+  };
+  if (B0s)
+    powell_source_with<true>(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo,
+                             LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, St, Src,
+                             face, cell, B0s->useSource);
+  else
+    powell_source_with<false>(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo,
+                              LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, St, Src,
+                              face, cell, 0);
 }
 
 } // namespace bats

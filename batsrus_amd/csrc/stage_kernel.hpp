@@ -269,14 +269,19 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
+// face values and face flux of one face; Bn (when given) receives the
+// normal field of the left and right face states, which the 8-wave source
+// reads
 template <int DIR, int FLUX, int LIM, bool B0F, typename T>
 __device__ __forceinline__ int face_flux_w(const PhysT<T> &ph, T area,
                                            T betaL, T betaR, T w,
                                            const T *q0, const T *q1,
                                            const T *q2, const T *q3, T *F,
-                                           T &vdt, T &unA, const T *B0) {
+                                           T &vdt, T &unA, const T *B0,
+                                           T *Bn = nullptr) {
   T L[NVAR], R[NVAR];
   face_states_w<LIM>(betaL, betaR, w, q0, q1, q2, q3, L, R);
+  if (Bn) { Bn[0] = L[BX + DIR]; Bn[1] = R[BX + DIR]; }
   return numerical_flux<DIR, FLUX, false, B0F>(ph, area, L, R, F, vdt, unA,
                                                B0);
 }
@@ -285,9 +290,11 @@ template <int DIR, int FLUX, int LIM, bool B0F, typename T>
 __device__ __forceinline__ int face_flux(const PhysT<T> &ph, T area, T betaL,
                                          T betaR, const T *q0, const T *q1,
                                          const T *q2, const T *q3, T *F,
-                                         T &vdt, T &unA, const T *B0) {
+                                         T &vdt, T &unA, const T *B0,
+                                         T *Bn = nullptr) {
   T L[NVAR], R[NVAR];
   face_states<LIM>(betaL, betaR, q0, q1, q2, q3, L, R);
+  if (Bn) { Bn[0] = L[BX + DIR]; Bn[1] = R[BX + DIR]; }
   return numerical_flux<DIR, FLUX, false, B0F>(ph, area, L, R, F, vdt, unA,
                                                B0);
 }
@@ -352,9 +359,16 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // z-face fluxes of the lower and upper face of the cell; with the 8-wave
   // scheme two more slots carry the normal-field face states Bz_L, Bz_R
   constexpr int NFZ = POWELL ? NFS + 2 : NFS;
-  // LDS slots of the x/y faces: the body variant with the 8-wave scheme
-  // keeps the normal-field face states too (the body changes them)
-  constexpr int NFL = (BODY && POWELL) ? NFS + 2 : NFS;
+  // LDS slots of the x/y faces: with the 8-wave scheme the face loop keeps
+  // the normal-field face states for the source when two workgroups per CU
+  // (three for FP32) still fit in the 160 KiB of LDS with them, and always
+  // in the body variant (the body changes them); otherwise the update
+  // recomputes them from the plane
+  constexpr int LDS_BN = (BPW * NVAR * NR * NS + BPW * 2 * (NFS + 2) * NFX) *
+                             (int)sizeof(T) + (kThreads / 64) * 8;
+  constexpr bool BNLDS =
+      POWELL && (BODY || LDS_BN * (sizeof(T) == 4 ? 3 : 2) <= 160 * 1024);
+  constexpr int NFL = BNLDS ? NFS + 2 : NFS;
   __shared__ T sP[BPW * NVAR * NR * NS];
   __shared__ T sF[BPW * 2 * NFL * NFX];
   __shared__ double sRed[kThreads / 64];
@@ -736,15 +750,23 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             sF[sFi(lbf, dir, NFS, q)] = skip ? T(0) : L[BX];
             sF[sFi(lbf, dir, NFS + 1, q)] = skip ? T(0) : R[BX];
           }
-        } else if (LOWORD)
-          err |= face_flux_w<0, FLUX, LIM, B0S>(
-              ph, area, bL, bR,
-              T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
-                                   : crit_index<NB>(bf, 1, row, fp, k)],
-              c0, c1, c2, c3, F, vdt, unA, B0r);
-        else
-          err |= face_flux<0, FLUX, LIM, B0S>(ph, area, bL, bR, c0, c1, c2,
-                                              c3, F, vdt, unA, B0r);
+        } else {
+          T Bn[2];
+          if (LOWORD)
+            err |= face_flux_w<0, FLUX, LIM, B0S>(
+                ph, area, bL, bR,
+                T(1) - crit[dir == 0 ? crit_index<NB>(bf, 0, fp, row, k)
+                                     : crit_index<NB>(bf, 1, row, fp, k)],
+                c0, c1, c2, c3, F, vdt, unA, B0r, BNLDS ? Bn : nullptr);
+          else
+            err |= face_flux<0, FLUX, LIM, B0S>(ph, area, bL, bR, c0, c1,
+                                                c2, c3, F, vdt, unA, B0r,
+                                                BNLDS ? Bn : nullptr);
+          if (BNLDS) {
+            sF[sFi(lbf, dir, NFS, q)] = Bn[0];
+            sF[sFi(lbf, dir, NFS + 1, q)] = Bn[1];
+          }
+        }
         if (dir != 0 && !recFlux) unrotate_flux(F);
         F[PP] = vdt;
         if (a.corr && (fp == 0 || fp == NB))
@@ -800,7 +822,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         } else if (!a.noUpdate) {
           T Uc[NVAR], Uo[NVAR];
           double Ucd[NVAR], Uod[NVAR], Un[NVAR];
-          load_state(Uold, vs, off, Uo);
+          if (!POWELL) load_state(Uold, vs, off, Uo);
           if (POWELL) {
             // calc_source: the div B source from the normal-field face
             // states of the six faces (x and y recomputed from the LDS plane
@@ -808,8 +830,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             // cell state
             load_state(U, vs, off, Uc);
             T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
-            if (BODY) {
-              // the face states as the body left them, from the face loop
+            if (BNLDS) {
+              // the face states as the face loop formed them (and as the
+              // body left them)
               LxLo = sF[sFi(lb, 0, NFS, qx0)];
               RxLo = sF[sFi(lb, 0, NFS + 1, qx0)];
               LxHi = sF[sFi(lb, 0, NFS, qx0 + 1)];
@@ -864,26 +887,20 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               if (j == NB - 1) corrected_bn(b, 3, i, k, LyHi, RyHi);
             }
             T Src[NFLUX];
-            CellB0<T> cb;
-            if (B0S) {
-              b0face(0, b, i, j, k, cb.face[0]);
-              b0face(0, b, i + 1, j, k, cb.face[1]);
-              b0face(1, b, i, j, k, cb.face[2]);
-              b0face(1, b, i, j + 1, k, cb.face[3]);
-              b0face(2, b, i, j, k, cb.face[4]);
-              b0face(2, b, i, j, k + 1, cb.face[5]);
-              const long nc = a.nCellTot;
-#pragma unroll
-              for (int c = 0; c < 3; c++) {
-                cb.cell[c] = T(a.b0C[c * nc + off]);
-                cb.curl[c] = T(a.b0C[(4 + c) * nc + off]);
-              }
-              cb.div = T(a.b0C[3 * nc + off]);
-              cb.useSource = a.useB0Src;
-            }
-            powell_source(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi,
-                          RyHi, FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2],
-                          FzHi[NFZ - 1], Uc, Src, B0S ? &cb : nullptr);
+            // the B0 of the cell's faces (x-, x+, y-, y+, z-, z+) and the
+            // cell's B0, div B0, curl B0, read as the source asks for them
+            auto faceB0 = [&](int f, T *o) {
+              const int d = f / 2, u = f % 2;
+              b0face(d, b, i + (d == 0) * u, j + (d == 1) * u,
+                     k + (d == 2) * u, o);
+            };
+            auto cellB0 = [&](int q) { return T(a.b0C[q * a.nCellTot + off]); };
+            powell_source_with<B0S>(
+                InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi,
+                FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2], FzHi[NFZ - 1],
+                Uc, Src, faceB0, cellB0, a.useB0Src);
+            // the old state only now, past the source's working set
+            load_state(Uold, vs, off, Uo);
             double Srcd[NFLUX];
 #pragma unroll
             for (int v = 0; v < NVAR; v++) {
@@ -917,9 +934,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
 #pragma unroll
         for (int v = 0; v < NFZ; v++) FzLo[v] = FzHi[v];
       }
-      // the div B source reads neighbouring cells of the LDS plane, which
-      // the next plane overwrites
-      if (POWELL) __syncthreads();
+      // the div B source that recomputes the face states reads
+      // neighbouring cells of the LDS plane, which the next plane
+      // overwrites (the LDS face states are rewritten only after the next
+      // plane's first barrier)
+      if (POWELL && !BNLDS) __syncthreads();
     }
 
     if (a.doDt) {

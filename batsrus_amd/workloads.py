@@ -9,6 +9,10 @@
   kh_two_level      Kelvin-Helmholtz shear layer on a 28^3 root grid of 8^3
                     blocks refined inside r < 0.25 of the centre (32,256
                     blocks), MC beta 1.5, HLLD + Linde, 200 steps
+  magnetosphere_256 4096 blocks of 16^3 with float sides, a dipole B0 split
+                    off the state, an ionosphere body of radius 0.06 at the
+                    centre, solar-wind inflow state, Koren (mc3) beta 1.5,
+                    HLLD + Linde, the 8-wave source with the B0 terms
 The surrogate is trained on faces harvested from FP64 runs of the
 Orszag-Tang and ensemble workloads, then runs in the two-level
 Kelvin-Helmholtz workload on FP32 state against the plain FP64 HLLD run.
@@ -29,6 +33,9 @@ OT_SCHEME = dict(flux="hlld", limiter="mc3", beta=1.5, nStage=2, cfl=0.8)
 ENS_SCHEME = dict(flux="hlld", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
 KH_SCHEME = dict(flux="hlld", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
 TUBE_SCHEME = dict(flux="rusanov", limiter="mc", beta=1.5, nStage=2, cfl=0.8)
+MAG_SCHEME = dict(flux="hlld", limiter="mc3", beta=1.5, nStage=2, cfl=0.8,
+                  UseDivbSource=True)
+MAG_BODY = dict(BodyRho=3.0, BodyP=0.5, B1rCoef=-1.0)
 
 
 def shock_tube_64():
@@ -44,6 +51,44 @@ def orszag_tang_256(nRoot=16):
 def kh_two_level(nRoot=28, radius=0.25):
     g = AmrGrid((nRoot,) * 3, sphere_refinement((nRoot,) * 3, radius))
     return g, ics.kelvin_helmholtz(g)
+
+
+def magnetosphere_256(nRoot=16, rBody=0.06, moment=2e-4):
+    """A magnetosphere-shaped run on the 256^3 box: returns the grid, the
+    state U[var, block, k, j, i] (B1 = B - B0), B0_DGB (nBlock, nKg, nJg,
+    nIg, 3) of a dipole at the centre with moment (0, 0, -moment), Used_GB
+    (nBlock, nKg, nJg, nIg) outside the body of radius rBody and
+    XyzStart_DB (nBlock, 3) relative to the centre.  The state is a
+    southward-field solar wind, slowed and compressed around the body."""
+    g = BlockGrid(nRoot=(nRoot,) * 3, nCell=(16, 16, 16), order="morton",
+                  periodic=(False, False, False))
+    n, nG = 16, g.nG
+    dx = np.asarray(g.CellSize_DB, dtype=np.float64).reshape(-1, 3)
+    org = g.xyzMin[None, :] + g.iRoot_DB * n * dx
+    ax = [org[:, d, None] + (np.arange(-nG, n + nG) + 0.5)[None, :] *
+          dx[:, d, None] - 0.5 for d in range(3)]
+    x = ax[0][:, None, None, :]
+    y = ax[1][:, None, :, None]
+    z = ax[2][:, :, None, None]
+    r2 = x * x + y * y + z * z
+    used = np.ascontiguousarray(r2 > rBody ** 2)
+    r5 = r2 ** 2.5
+    mz = -moment
+    B0 = np.empty(r2.shape + (3,))
+    B0[..., 0] = 3 * mz * z * x / r5
+    B0[..., 1] = 3 * mz * z * y / r5
+    B0[..., 2] = (3 * mz * z * z - mz * r2) / r5
+    del r5
+    xyz0 = np.ascontiguousarray(org + 0.5 * dx - 0.5)
+    xc, yc, zc = g.cell_centers()
+    r = np.sqrt((xc - 0.5) ** 2 + (yc - 0.5) ** 2 + (zc - 0.5) ** 2)
+    f = 1.0 - 0.5 * np.exp(-((r - 2 * rBody) / (2 * rBody)) ** 2)
+    U = np.zeros((8,) + xc.shape)
+    U[0] = 2.0 - f
+    U[1] = U[0] * (-0.5) * f
+    U[6] = -0.1
+    U[7] = 0.3 * (2.0 - f)
+    return g, U, B0, used, xyz0
 
 
 def kh_uniform(nRoot=28, seed=7):

@@ -158,7 +158,8 @@ def measure_traffic(timeout=150):
                    "-o", "run", "--", sys.executable,
                    os.path.abspath(__file__), "--steps", "2", "--warmup",
                    "0", "--no-cpu", "--no-surrogate", "--no-ensemble",
-                   "--no-traffic", "--copy-calibration"]
+                   "--no-magnetosphere", "--no-traffic",
+                   "--copy-calibration"]
             r = subprocess.run(cmd, stdout=subprocess.DEVNULL,
                                stderr=subprocess.DEVNULL, timeout=timeout)
             if r.returncode != 0:
@@ -230,6 +231,42 @@ def ensemble_leg(steps=20):
                      "hbm_frac": g.nCellTotal * bpc / (kms * 1e-3) / 1e9 /
                      HBM_PEAK_GBS}
     return out
+
+
+def magnetosphere_leg(steps=20):
+    """B0 splitting and the inner body on the 256^3 box: a dipole B0 at the
+    centre of an ionosphere body, float sides, HLLD with the 8-wave source
+    and its B0 terms, FP64 (the B0 + body + 8-wave stage variants on the
+    groups of blocks that reach the body, the B0 + 8-wave variant on the
+    rest)."""
+    from batsrus_amd.core import Session
+    from batsrus_amd import workloads as W
+    g, U0, B0, used, xyz0 = W.magnetosphere_256()
+    with Session(g, **W.MAG_SCHEME) as s:
+        s.set_b0(B0, UseB0Source=True)
+        del B0
+        s.set_body(used, xyz0, **W.MAG_BODY)
+        s.upload_soa(U0)
+        s.init_timestep()
+        s.advance(3)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.advance(steps)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        kms, _ = s.last_stage_kernel_ms()
+        s.check_nan()
+    rate = g.nCellTotal * steps / el
+    return {"workload": "4096 blocks of 16^3 (256^3 cells), float sides, "
+                        "dipole B0 at the centre of an ionosphere body of "
+                        "radius 0.06, solar-wind state, mc3 beta=1.5, "
+                        "HLLD+Linde, 8-wave source with the B0 terms, "
+                        "2-stage, CFL 0.8, FP64, %d steps" % steps,
+            "unit": "cell-updates/s", "value": rate,
+            "ms_per_step": el * 1e3 / steps, "stage_kernel_ms": kms,
+            "hbm_frac": g.nCellTotal * BYTES_PER_CELL_LAUNCH /
+            (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "body_blocks": int((~used.reshape(g.nBlock, -1).all(1)).sum())}
 
 
 def _kh_result(W, np, g, U0, steps, ref, r64, m, mode):
@@ -320,6 +357,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-magnetosphere", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that "
                          "measure HBM bytes per stage launch")
@@ -415,6 +453,12 @@ def main():
             out["ensemble"] = ensemble_leg()
         except Exception as e:            # reported, not fatal to the line
             out["ensemble"] = {"error": "%s: %s" % (type(e).__name__, e)}
+    if not args.no_magnetosphere:
+        try:
+            out["magnetosphere"] = magnetosphere_leg()
+        except Exception as e:            # reported, not fatal to the line
+            out["magnetosphere"] = {"error": "%s: %s" % (type(e).__name__,
+                                                         e)}
     if not args.no_surrogate:
         try:
             out["surrogate"] = surrogate_leg()
