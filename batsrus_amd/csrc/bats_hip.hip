@@ -444,11 +444,7 @@ __global__ void k_block_source(BlockArgs a) {
     face(1, i, j + 1, k, cb.face[3]);
     face(2, i, j, k, cb.face[4]);
     face(2, i, j, k + 1, cb.face[5]);
-    for (int q = 0; q < 3; q++) {
-      cb.cell[q] = a.b0C[q * a.nCellTot + off];
-      cb.curl[q] = a.b0C[(4 + q) * a.nCellTot + off];
-    }
-    cb.div = a.b0C[3 * a.nCellTot + off];
+    for (int q = 0; q < 3; q++) cb.cell[q] = a.b0C[q * a.nCellTot + off];
     cb.useSource = a.useB0Src;
   }
   powell_source(InvHalf, a.LX[fx + BX], a.RX[fx + BX], a.LX[fx + NVAR + BX],
@@ -2065,7 +2061,7 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
     if (!std::isfinite(B0_DGB[t])) return BATS_EINVAL;
   const long NF = (long)(NB + 1) * NB * NB, nFaceTot = (long)nB * NF;
   const long nCell = h->nCell;
-  std::vector<double> F(9 * nFaceTot), C(7 * nCell);
+  std::vector<double> F(9 * nFaceTot), C(3 * nCell);
   // B0_DGB(3, MinI:MaxI, MinJ:MaxJ, MinK:MaxK, nBlock), variable fastest
   auto B0 = [&](int b, int i, int j, int k, int d) {
     return B0_DGB[((((long)b * NG + (k + g)) * NG + (j + g)) * NG + (i + g)) * 3 + d];
@@ -2130,29 +2126,15 @@ extern "C" int bats_set_b0(bats_handle *h, const double *B0_DGB,
         for (int v = 0; v < 3; v++) face(d, b, i, j, k, v) = sum[v];
       }
   }
-  for (int b = 0; b < nB; b++) {
-    const double *cs = h->userCellSize.data() + 3 * b;
-    const double DxInv = 1 / cs[0], DyInv = 1 / cs[1], DzInv = 1 / cs[2];
+  // the cell B0 (div B0 and curl B0 are formed from the face values where
+  // the source needs them)
+  for (int b = 0; b < nB; b++)
     for (int k = 0; k < NB; k++)
       for (int j = 0; j < NB; j++)
         for (int i = 0; i < NB; i++) {
           const long off = (((long)b * NB + k) * NB + j) * NB + i;
           for (int d = 0; d < 3; d++) C[d * nCell + off] = B0(b, i, j, k, d);
-          C[3 * nCell + off] =
-              DxInv * (face(0, b, i + 1, j, k, 0) - face(0, b, i, j, k, 0)) +
-              DyInv * (face(1, b, i, j + 1, k, 1) - face(1, b, i, j, k, 1)) +
-              DzInv * (face(2, b, i, j, k + 1, 2) - face(2, b, i, j, k, 2));
-          C[4 * nCell + off] =
-              DyInv * (face(1, b, i, j + 1, k, 2) - face(1, b, i, j, k, 2)) -
-              DzInv * (face(2, b, i, j, k + 1, 1) - face(2, b, i, j, k, 1));
-          C[5 * nCell + off] =
-              DzInv * (face(2, b, i, j, k + 1, 0) - face(2, b, i, j, k, 0)) -
-              DxInv * (face(0, b, i + 1, j, k, 2) - face(0, b, i, j, k, 2));
-          C[6 * nCell + off] =
-              DxInv * (face(0, b, i + 1, j, k, 1) - face(0, b, i, j, k, 1)) -
-              DyInv * (face(1, b, i, j + 1, k, 0) - face(1, b, i, j, k, 0));
         }
-  }
   // the ghost B0 of every outer side (the caller's -1 sides), for the
   // fixed, inflow and vary ghost cells: slot per (block, side), layer L =
   // 0, 1 away from the side, in-plane (p, q) as the ghost slabs

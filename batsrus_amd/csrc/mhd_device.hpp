@@ -731,11 +731,10 @@ __device__ __forceinline__ void body_ghost(const Real *T, const Real *r,
 }
 
 // B0 of a cell for the 8-wave source with B0 splitting: the B0 of its six
-// faces, its own B0, and div B0 / curl B0 (set_b0_source) when UseB0Source
+// faces and its own B0; useSource adds -B1 div(B0) - curl(B0) x B1
 template <typename Real> struct CellB0 {
   Real face[6][3];
   Real cell[3];
-  Real div, curl[3];
   int useSource;
 };
 
@@ -743,17 +742,19 @@ template <typename Real> struct CellB0 {
 // (src/ModCalcSource.f90:1694-1889 with DoCorrectFace false, :930-951,
 // :1016-1022), with the B0 terms.  B?lo/B?hi are the normal-field left (L)
 // and right (R) face states at the lower and upper faces of the cell in
-// each direction; InvHalf = 0.5/CellSize; St is the cell's stage-input
-// state.  B0 splitting: faceB0(f, out) gives the B0 of the cell's face f
-// (x-, x+, y-, y+, z-, z+), cellB0(q) the cell's B0 (q = 0..2), div B0
-// (q = 3) and curl B0 (q = 4..6); the faces are fetched one at a time, each
-// consumed before the next is asked for, which keeps the fused stage's
-// register use down.
+// each direction; InvHalf = 0.5/CellSize; P is the cell's stage-input
+// state in primitive form (velocity u = rho u / rho, field B1).  B0
+// splitting: faceB0(f, out) gives the B0 of the cell's face f (x-, x+, y-,
+// y+, z-, z+), cellB0(q) the cell's own B0 (q = 0..2).  The faces are
+// fetched a direction at a time, each pair consumed before the next is
+// asked for (which keeps the fused stage's register use down), and div B0
+// and curl B0 are formed from them as set_b0_source forms them
+// (src/ModB0.f90:568-590).
 template <bool B0S, typename Real, typename FaceB0, typename CellV>
 __device__ __forceinline__ void powell_source_with(
     const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
     Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
-    Real LzHi, Real RzHi, const Real *St, Real *Src, FaceB0 &&faceB0,
+    Real LzHi, Real RzHi, const Real *P, Real *Src, FaceB0 &&faceB0,
     CellV &&cellB0, int useB0Source) {
   const Real dBf[6] = {InvHalf[0] * (RxLo - LxLo), InvHalf[0] * (RxHi - LxHi),
                        InvHalf[1] * (RyLo - LyLo), InvHalf[1] * (RyHi - LyHi),
@@ -764,9 +765,8 @@ __device__ __forceinline__ void powell_source_with(
   Real DivB = Internal + dBf[0] + dBf[1];
   DivB = DivB + dBf[2] + dBf[3];
   DivB = DivB + dBf[4] + dBf[5];
-  const Real RhoInv = frcp(St[RHO]);
-  const Real ux = RhoInv * St[MX], uy = RhoInv * St[MY],
-             uz = RhoInv * St[MZ];
+  const Real ux = P[MX], uy = P[MY], uz = P[MZ];
+  const Real Bx = P[BX], By = P[BY], Bz = P[BZ];
   Src[RHO] = Real(0);
   if (B0S) {
     // the face B0 times each face's normal-field jump, the cell B0 times
@@ -774,60 +774,74 @@ __device__ __forceinline__ void powell_source_with(
     // 1880-1885), then -B1 div(B0) - curl(B0) x B1 with UseB0Source
     // (:953-967)
     Real m[3] = {Real(0), Real(0), Real(0)};
+    Real div = Real(0), curl[3] = {Real(0), Real(0), Real(0)};
 #pragma unroll
-    for (int f = 0; f < 6; f++) {
-      Real B0f[3];
-      faceB0(f, B0f);
+    for (int d = 0; d < 3; d++) {
+      Real lo[3], hi[3];
+      faceB0(2 * d, lo);
+      faceB0(2 * d + 1, hi);
 #pragma unroll
-      for (int v = 0; v < 3; v++) m[v] = m[v] - B0f[v] * dBf[f];
+      for (int v = 0; v < 3; v++) m[v] = m[v] - lo[v] * dBf[2 * d];
+#pragma unroll
+      for (int v = 0; v < 3; v++) m[v] = m[v] - hi[v] * dBf[2 * d + 1];
+      if (useB0Source) {
+        // DxInv*(B0X(i+1) - B0X(i)) etc.; curl B0 in its x, y, z terms
+        const Real DInv = 2 * InvHalf[d];
+        const int a1 = (d + 1) % 3, a2 = (d + 2) % 3;
+        div = div + DInv * (hi[d] - lo[d]);
+        curl[a2] = curl[a2] + DInv * (hi[a1] - lo[a1]);
+        curl[a1] = curl[a1] - DInv * (hi[a2] - lo[a2]);
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
+    const Real B1[3] = {Bx, By, Bz};
 #pragma unroll
     for (int v = 0; v < 3; v++)
-      m[v] = (m[v] - Internal * cellB0(v)) - DivB * St[BX + v];
+      m[v] = (m[v] - Internal * cellB0(v)) - DivB * B1[v];
     if (useB0Source) {
-      const Real Bx = St[BX], By = St[BY], Bz = St[BZ], dv = cellB0(3);
-      const Real c0 = cellB0(4), c1 = cellB0(5), c2 = cellB0(6);
-      m[0] = m[0] - Bx * dv - (c1 * Bz - c2 * By);
-      m[1] = m[1] - By * dv - (c2 * Bx - c0 * Bz);
-      m[2] = m[2] - Bz * dv - (c0 * By - c1 * Bx);
+      m[0] = m[0] - Bx * div - (curl[1] * Bz - curl[2] * By);
+      m[1] = m[1] - By * div - (curl[2] * Bx - curl[0] * Bz);
+      m[2] = m[2] - Bz * div - (curl[0] * By - curl[1] * Bx);
     }
     Src[MX] = m[0];
     Src[MY] = m[1];
     Src[MZ] = m[2];
   } else {
-    Src[MX] = -(DivB * St[BX]);
-    Src[MY] = -(DivB * St[BY]);
-    Src[MZ] = -(DivB * St[BZ]);
+    Src[MX] = -(DivB * Bx);
+    Src[MY] = -(DivB * By);
+    Src[MZ] = -(DivB * Bz);
   }
   Src[BX] = -(DivB * ux);
   Src[BY] = -(DivB * uy);
   Src[BZ] = -(DivB * uz);
   Src[PP] = Real(0);
-  Src[EN] = -(DivB * (St[BX] * ux + St[BY] * uy + St[BZ] * uz));
+  Src[EN] = -(DivB * (Bx * ux + By * uy + Bz * uz));
 }
 
-// the same with the B0 values of the cell in a CellB0 (null: no B0)
+// the same from the cell's stored state St (rho u) and, with B0 splitting,
+// its B0 values in a CellB0 (null: no B0)
 template <typename Real>
 __device__ __forceinline__ void powell_source(
     const Real *InvHalf, Real LxLo, Real RxLo, Real LxHi, Real RxHi,
     Real LyLo, Real RyLo, Real LyHi, Real RyHi, Real LzLo, Real RzLo,
     Real LzHi, Real RzHi, const Real *St, Real *Src,
     const CellB0<Real> *B0s = nullptr) {
+  Real P[NVAR];
+#pragma unroll
+  for (int v = 0; v < NVAR; v++) P[v] = St[v];
+  to_primitive(P);
   auto face = [&](int f, Real *o) {
 #pragma unroll
     for (int v = 0; v < 3; v++) o[v] = B0s->face[f][v];
   };
-  auto cell = [&](int q) {
-    return q < 3 ? B0s->cell[q] : q == 3 ? B0s->div : B0s->curl[q - 4];
-  };
+  auto cell = [&](int q) { return B0s->cell[q]; };
   if (B0s)
     powell_source_with<true>(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo,
-                             LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, St, Src,
+                             LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, P, Src,
                              face, cell, B0s->useSource);
   else
     powell_source_with<false>(InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo,
-                              LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, St, Src,
+                              LyHi, RyHi, LzLo, RzLo, LzHi, RzHi, P, Src,
                               face, cell, 0);
 }
 

@@ -825,10 +825,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           if (!POWELL) load_state(Uold, vs, off, Uo);
           if (POWELL) {
             // calc_source: the div B source from the normal-field face
-            // states of the six faces (x and y recomputed from the LDS plane
-            // exactly as the face loop computed them) and the stage-input
-            // cell state
-            load_state(U, vs, off, Uc);
+            // states of the six faces (x and y from the face loop, or
+            // recomputed from the LDS plane exactly as the face loop
+            // computed them) and the stage-input cell state
             T p[5], LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi;
             if (BNLDS) {
               // the face states as the face loop formed them (and as the
@@ -895,21 +894,34 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
                      k + (d == 2) * u, o);
             };
             auto cellB0 = [&](int q) { return T(a.b0C[q * a.nCellTot + off]); };
+            // the cell's stage-input state in primitive form: its entry of
+            // the LDS plane (to_primitive of the stage input; the cell's
+            // own entry, which no thread rewrites before this thread's next
+            // plane)
+            T Pc[NVAR];
+#pragma unroll
+            for (int v = 0; v < NVAR; v++) Pc[v] = sP[sPi(lb, v, j + 2, i + 2)];
             powell_source_with<B0S>(
                 InvHalf, LxLo, RxLo, LxHi, RxHi, LyLo, RyLo, LyHi, RyHi,
                 FzLo[NFZ - 2], FzLo[NFZ - 1], FzHi[NFZ - 2], FzHi[NFZ - 1],
-                Uc, Src, faceB0, cellB0, a.useB0Src);
+                Pc, Src, faceB0, cellB0, a.useB0Src);
             // the old state only now, past the source's working set
             load_state(Uold, vs, off, Uo);
             double Srcd[NFLUX];
 #pragma unroll
-            for (int v = 0; v < NVAR; v++) {
-              Ucd[v] = double(Uc[v]);
-              Uod[v] = double(Uo[v]);
-            }
+            for (int v = 0; v < NVAR; v++) Uod[v] = double(Uo[v]);
 #pragma unroll
             for (int v = 0; v < NFLUX; v++) Srcd[v] = double(Src[v]);
-            update_cell(a.ph, ucc, dF, Ucd, Uod, Un, Srcd);
+            // the stage input itself only for the SSP-RK2 interpolation and
+            // a zero time step
+            if (uc.dtZero || !uc.halfStep) {
+              load_state(U, vs, off, Uc);
+#pragma unroll
+              for (int v = 0; v < NVAR; v++) Ucd[v] = double(Uc[v]);
+              update_cell(a.ph, ucc, dF, Ucd, Uod, Un, Srcd);
+            } else {
+              update_cell(a.ph, ucc, dF, Uod, Uod, Un, Srcd);
+            }
           } else {
             // the stage input at the cell is used only by the SSP-RK2
             // interpolation and for a zero time step
