@@ -105,14 +105,14 @@ def harvest_training_set(model, n_target, seed=1, betaMin=0.0, jumpTol=0.05,
     """Faces passing the gate, with their FP64 HLLD fluxes, from FP64 runs of
     the named workloads (Orszag-Tang, the blast ensemble and optionally a
     Kelvin-Helmholtz run of the same family as kh_two_level, "kh_family")
-    at several times, until about n_target samples are held."""
+    at several times, until about n_target samples are held.  Each source
+    is asked for an equal share of what is still missing; the Orszag-Tang
+    run, whose 256^3 box holds the most gated faces, comes last and makes
+    up what the smaller runs could not give."""
     got = 0
     t0 = time.perf_counter()
     wanted = tuple(sources)
     sources = []
-    if "orszag_tang" in wanted:
-        g, U = orszag_tang_256(ot_root)
-        sources.append(("orszag_tang", g, U, OT_SCHEME, None))
     if "blast_ensemble" in wanted:
         eg, Ue = blast_ensemble(nRun=ens_runs)
         sources.append(("blast_ensemble", eg, Ue, ENS_SCHEME,
@@ -120,8 +120,11 @@ def harvest_training_set(model, n_target, seed=1, betaMin=0.0, jumpTol=0.05,
     if "kh_family" in wanted:
         g, U = kh_uniform()
         sources.append(("kh_family", g, U, KH_SCHEME, None))
-    per = n_target / len(sources)
+    if "orszag_tang" in wanted:
+        g, U = orszag_tang_256(ot_root)
+        sources.append(("orszag_tang", g, U, OT_SCHEME, None))
     for si, (name, grid, U0, scheme, runs) in enumerate(sources):
+        per = (n_target - got) / (len(sources) - si)
         with Session(grid, **scheme) as s:
             if runs is not None:
                 s.set_runs(runs)
