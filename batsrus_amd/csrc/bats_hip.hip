@@ -1488,8 +1488,15 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
     const int st = sur_stage(h, a);
     if (st) return st;
   } else {
+    // the few groups with a body block would leave most of the GPU idle in
+    // their own launch: their blocks are cut into z-parts of at least two
+    // planes until about one workgroup per CU runs
+    int bodyShift = a.splitShift;
+    if (h->useBody)
+      while ((h->nBodyGroup << bodyShift) < 256 && (NB >> (bodyShift + 1)) >= 2)
+        bodyShift++;
     // the parts of a split block meet in an atomic minimum of dtmaxB
-    if (a.doDt && a.splitShift) {
+    if (a.doDt && (a.splitShift || bodyShift)) {
       const long n = c.nBlock;
       hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256),
                          0, h->stream, h->dtmaxB, n, 1.7976931348623157e308);
@@ -1521,6 +1528,7 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
         StageArgs ab = a;
         ab.groupList = h->groupList + h->nPlainGroup;
         ab.nGroup = h->nBodyGroup;
+        ab.splitShift = bodyShift;
         ab.nWorkgroup = wg > 0 ? wg : ab.nGroup << ab.splitShift;
         ab.bodySlot = h->bodySlot;
         ab.used = h->bodyUsed;
