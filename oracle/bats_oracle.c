@@ -2417,10 +2417,14 @@ int ob_session_sources(ob_session *s, double *Source) {
     int rc[6];
     const double *CX, *CY, *CZ;
     const double *cs = s->CellSize_DB + 3L * b;
+    double *Blk = s->State + (long)b * len;
     block_crit(s, b, &CX, &CY, &CZ);
     block_res_change(s, b, rc);
-    ob_calc_face_value_rc(c, s->State + (long)b * len, CX, CY, CZ, rc, w.LX,
-                          w.RX, w.LY, w.RY, w.LZ, w.RZ);
+    /* as the stage forms them: the TVD faces at resolution changes */
+    const int tvd = c->UseTvdResChange && c->nOrder > 1;
+    ob_calc_face_value_rc(c, tvd ? monotone_restrict_copy(s, b, w.Tvd) : Blk,
+                          CX, CY, CZ, rc, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
+    if (tvd) tvd_faces(s, b, Blk, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ);
     const double *B0Blk = block_b0(s, b, &w);
     err |= ob_calc_face_flux_b0(c, cs, w.LX, w.RX, w.LY, w.RY, w.LZ, w.RZ,
                                 B0Blk ? w.B0X : NULL, B0Blk ? w.B0Y : NULL,

@@ -259,3 +259,39 @@ def test_fine_ghost_layers_are_the_two_coarse_cells():
                     want = _cell(o.State[C], d, cn, p0 + t1 // 2,
                                  q0 + t2 // 2, G)
                     np.testing.assert_array_equal(got, want)
+
+
+def test_eight_wave_source_reads_the_tvd_faces():
+    """With the TVD faces the 8-wave source sees the divergence of a field
+    linear along its normal, Bx = 0.3 + 0.4 x (div B = 0.4), exactly in the
+    cells next to the resolution changes as everywhere else: the TVD face
+    values of such a field are exact (no jump of the normal field at any
+    face, the internal divergence the field's own), so Source(Bx) =
+    -div B u_x = -0.04 there.  Faces taken without the TVD treatment on the
+    TVD ghost layers (which hold the coarse cells unchanged) miss it next
+    to the x resolution changes.  Cells near the periodic seam, where the
+    field jumps, are left out."""
+    g = AmrGrid((4, 4, 4), sphere_refinement((4, 4, 4), 0.3))
+    x, y, z = g.cell_centers()
+    U = np.zeros((8,) + x.shape)
+    U[0] = 1.0
+    U[1] = 0.1
+    U[7] = 1.0
+    U[4] = 0.3 + 0.4 * x
+    U[5] = -0.2
+    U[6] = 0.1
+    cfg = po.config_from_dict(dict(flux="hlld", limiter="mc", beta=1.5,
+                                   UseDivbSource=True, UseTvdResChange=True),
+                              g)
+    o = po.Session(g, cfg, g.soa_to_state_vgb(U))
+    o.exchange()
+    src = np.moveaxis(o.sources(), -1, 0)     # (9, nBlock, nK, nJ, nI)
+    far = (x > 3 / 32) & (x < 1 - 3 / 32)
+    # the cells of blocks at an x resolution change
+    nei = g.iNeighbor_EB.reshape(-1, 6)
+    rc = (nei[:, :2] <= -2).any(axis=1)
+    sel = far & rc[:, None, None, None]
+    assert sel.sum() > 10000
+    np.testing.assert_allclose(src[4][sel], -0.04, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(src[1][sel], -0.4 * U[4][sel], rtol=0,
+                               atol=1e-12)
