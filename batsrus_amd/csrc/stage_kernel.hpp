@@ -130,9 +130,10 @@ __device__ __forceinline__ void unrotate_flux(T *F) {
 enum { OPT_POWELL = 1, OPT_LOWORDER = 2, OPT_RESBETA = 4, OPT_STEADY = 8,
        OPT_B0 = 16, OPT_BODY = 32, OPT_TVD = 64 };
 // record of one fine face (p, q) of a coarser side with the TVD treatment,
-// in the side's slab store: the face flux (slots 0..6, PP <- Vdt, EN) and
-// the fine-side primitive state of the face one cell in
-enum { TVD_FLUX = 0, TVD_FINEF = 9, NTVD = 17 };
+// in the side's slab store: the face flux (slots 0..6, PP <- Vdt, EN), the
+// fine-side primitive state of the face one cell in, and the normal field
+// of the face's left and right states (for the 8-wave source)
+enum { TVD_FLUX = 0, TVD_FINEF = 9, TVD_BN = 17, NTVD = 19 };
 static_assert(NTVD <= 4 * NCORR, "the TVD record fits a slab cell's store");
 
 // tvd_reschange (src/ModFaceValue.f90:2269-2343) for subface f of a 2x2
@@ -347,9 +348,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
                 "the body variant is FP64, without low order faces or the "
                 "resolution-change beta (steady state: the run-time flag)");
   constexpr bool TVD = (OPT & OPT_TVD) != 0;
-  static_assert(!TVD || (!POWELL && !LOWORD && !STEADY_BIT && !B0S && !BODY),
-                "the TVD variant carries no 8-wave source, low order faces, "
-                "steady state, B0 or body");
+  static_assert(!TVD || (!LOWORD && !STEADY_BIT && !B0S && !BODY),
+                "the TVD variant carries no low order faces, steady state, "
+                "B0 or body");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -362,12 +363,13 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // LDS slots of the x/y faces: with the 8-wave scheme the face loop keeps
   // the normal-field face states for the source when two workgroups per CU
   // (three for FP32) still fit in the 160 KiB of LDS with them, and always
-  // in the body variant (the body changes them); otherwise the update
+  // in the body and TVD variants (which change them); otherwise the update
   // recomputes them from the plane
   constexpr int LDS_BN = (BPW * NVAR * NR * NS + BPW * 2 * (NFS + 2) * NFX) *
                              (int)sizeof(T) + (kThreads / 64) * 8;
   constexpr bool BNLDS =
-      POWELL && (BODY || LDS_BN * (sizeof(T) == 4 ? 3 : 2) <= 160 * 1024);
+      POWELL &&
+      (BODY || TVD || LDS_BN * (sizeof(T) == 4 ? 3 : 2) <= 160 * 1024);
   constexpr int NFL = BNLDS ? NFS + 2 : NFS;
   __shared__ T sP[BPW * NVAR * NR * NS];
   __shared__ T sF[BPW * 2 * NFL * NFX];
@@ -568,6 +570,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
         if (rec && (fz == 0 || fz == NB)) {
 #pragma unroll
           for (int v = 0; v < NFS; v++) Fz[v] = rec[TVD_FLUX + v];
+          if (POWELL) {
+            Fz[NFZ - 2] = rec[TVD_BN];
+            Fz[NFZ - 1] = rec[TVD_BN + 1];
+          }
           return;
         }
         if (rec && fz == 1)
@@ -709,10 +715,16 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           recFlux = trec && (fp == 0 || fp == NB);
         }
         if (TVD && trec) {
+          // (the normal-field face states for the 8-wave source: the
+          // record's, or those of the TVD face values)
           if (recFlux) {
 #pragma unroll
             for (int v = 0; v < NFLUX; v++) F[v] = trec[TVD_FLUX + v];
             vdt = trec[TVD_FLUX + PP];
+            if (BNLDS) {
+              sF[sFi(lbf, dir, NFS, q)] = trec[TVD_BN];
+              sF[sFi(lbf, dir, NFS + 1, q)] = trec[TVD_BN + 1];
+            }
           } else {
             T L[NVAR], R[NVAR];
             face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
@@ -720,6 +732,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
 #pragma unroll
             for (int v = 0; v < NVAR; v++)
               S[v] = trec[TVD_FINEF + (dir == 0 ? v : rot_src(v))];
+            if (BNLDS) {
+              sF[sFi(lbf, dir, NFS, q)] = L[BX];
+              sF[sFi(lbf, dir, NFS + 1, q)] = R[BX];
+            }
             err |= numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
           }
         } else if (BODY) {
@@ -1357,6 +1373,9 @@ __global__ void __launch_bounds__(256) k_amr_fineflux(AmrArgs a) {
              ((((long)slab) * NB + q) * NB + p) * 4 * NCORR;
     for (int v = 0; v < 9; v++) rec[TVD_FLUX + v] = F[v];
     for (int v = 0; v < NVAR; v++) rec[TVD_FINEF + v] = FF[v];
+    // and the face states' normal field, for the 8-wave source
+    rec[TVD_BN] = L[BX + d];
+    rec[TVD_BN + 1] = R[BX + d];
   }
   if (err) atomicOr(a.err, err);
 }
@@ -1428,12 +1447,14 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   // the TVD treatment at resolution changes (second-order limiters), with
   // or without the resolution-change beta
 #define BATS_LAUNCH_TVD(LM, RB)                                              \
-  switch (opt & (RB ? OPT_RESBETA : 0)) {                                    \
+  switch (opt & ((RB ? OPT_RESBETA : 0) | OPT_POWELL)) {                     \
   case 0: BATS_K(LM, 64); break;                                             \
-  default: BATS_K(LM, 68); break;                                            \
+  case 1: BATS_K(LM, 65); break;                                             \
+  case 4: BATS_K(LM, 68); break;                                             \
+  default: BATS_K(LM, 69); break;                                            \
   }
   if (opt & OPT_TVD) {
-    if (opt & (OPT_POWELL | OPT_LOWORDER | OPT_STEADY | OPT_B0 | OPT_BODY))
+    if (opt & (OPT_LOWORDER | OPT_STEADY | OPT_B0 | OPT_BODY))
       return hipErrorInvalidValue;
     switch (a.lim) {
     case LIM_MINMOD: BATS_LAUNCH_TVD(LIM_MINMOD, false); break;

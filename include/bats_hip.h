@@ -126,8 +126,8 @@ typedef struct bats_config {
    * (correct_monotone_restrict, src/ModFaceValue.f90:233-408), and the
    * fine block's face values at the coarse side and one face in come from
    * tvd_reschange (:680-852,2269-2343).  Two-level grids, fused entry
-   * points and per-block routines; not with the 8-wave source, low order
-   * faces, steady state, B0, the inner body or the surrogate
+   * points and per-block routines, with or without the 8-wave source; not
+   * with low order faces, steady state, B0, the inner body or the surrogate
    * (BATS_EINVAL); 0 keeps this project's second-order prolongation. */
   int UseTvdResChange;
 } bats_config;

@@ -52,7 +52,7 @@ def _per_block_stage(s, g, stage, last, apply=True):
 @pytest.mark.parametrize("flux,limiter,powell,tvd", [
     ("rusanov", "mc", False, False), ("hlld", "mc3", True, False),
     ("linde", "minmod", False, False), ("hlld", "mc", False, True),
-    ("rusanov", "mc3", False, True)])
+    ("rusanov", "mc3", False, True), ("hlld", "mc", True, True)])
 def test_per_block_two_level_stages_match_oracle(flux, limiter, powell, tvd):
     """tvd: the TVD faces at the resolution changes (UseTvdResChange),
     through calc_face_value of the fine blocks in both passes."""

@@ -3,7 +3,8 @@ first-order prolongation, the reference's default:
 src/ModFaceValue.f90:233-408,680-852,2269-2343): on a two-level grid the
 fused stages match the oracle's stage update with the same treatment
 within 1e-12 per stage over two steps (Rusanov, Linde, HLLD; MC, Koren,
-minmod, beta, with the sharper resolution-change beta, and first order),
+minmod, beta, with the sharper resolution-change beta, and first order;
+with the 8-wave source, on 8^3 and 16^3 blocks),
 FP32 state within its rounding, the grid conserves to rounding, the step
 graph replays the eager stages, and the combinations the TVD variant does
 not carry are refused.  The per-block routines with the TVD faces are
@@ -30,11 +31,16 @@ CASES = [
     dict(flux="hlld", limiter="beta", beta=1.8, BetaLimiterResChange=1.2,
          nFaceLimiterResChange=2),
     dict(flux="hlld", limiter="mc", nOrder=1),
+    # with the 8-wave source (the reference's default for MHD)
+    dict(flux="hlld", limiter="mc3", UseDivbSource=True),
+    dict(flux="linde", limiter="beta", beta=1.8, BetaLimiterResChange=1.2,
+         nFaceLimiterResChange=2, UseDivbSource=True),
 ]
 
 
-def _run_pair(scheme, precision="f64", steps=2, seed=51, check=None):
-    g = _grid()
+def _run_pair(scheme, precision="f64", steps=2, seed=51, check=None,
+              grid=None):
+    g = grid or _grid()
     U0 = smooth_mhd_soa(g, seed=seed, amp=0.2)
     sch = dict(dict(beta=1.5, cfl=0.8), **scheme, UseTvdResChange=True)
     cfg = po.config_from_dict(sch, g)
@@ -69,6 +75,21 @@ def test_tvd_stages_match_oracle(scheme):
         print(f"  step {step} stage {stage} rel err {e:.2e}")
         assert e <= 1e-12, (step, stage, e)
     _run_pair(scheme, check=check)
+
+
+def test_tvd_eight_wave_16cube_matches_oracle():
+    print("case tvd 8-wave 16^3", flush=True)
+    refine = np.zeros((2, 2, 2), bool)
+    refine[0, 0, 0] = True
+    refine[1, 1, 0] = True
+
+    def check(step, stage, dtg, dto, Ug, Uo):
+        assert abs(dtg - dto) <= 1e-12 * dto, (step, dtg, dto)
+        e = rel_err(Ug, Uo)
+        print(f"  step {step} stage {stage} rel err {e:.2e}")
+        assert e <= 1e-12, (step, stage, e)
+    _run_pair(dict(flux="hlld", limiter="mc", UseDivbSource=True),
+              check=check, grid=AmrGrid((2, 2, 2), refine, nCell=16))
 
 
 def test_tvd_fp32_state_matches_oracle_within_rounding():
@@ -119,16 +140,15 @@ def test_tvd_refusals():
     print("case tvd refusals", flush=True)
     g = _grid()
     U0 = smooth_mhd_soa(g, seed=54)
-    for bad in (dict(UseDivbSource=True), dict(IsSteadyState=True),
-                dict(UseAdaptiveLowOrder=True)):
+    for bad in (dict(IsSteadyState=True), dict(UseAdaptiveLowOrder=True)):
         with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True,
                      **bad) as s:
             s.upload_soa(U0)
             with pytest.raises(StopMpi):
                 s.init_timestep()
-    # nor do the per-block routines with the 8-wave source
+    # nor do the per-block routines in steady state
     with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True,
-                 UseDivbSource=True) as s:
+                 IsSteadyState=True) as s:
         s.upload_soa(U0)
         s.exchange_messages()
         s.set_stage(1)
