@@ -348,9 +348,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
                 "the body variant is FP64, without low order faces or the "
                 "resolution-change beta (steady state: the run-time flag)");
   constexpr bool TVD = (OPT & OPT_TVD) != 0;
-  static_assert(!TVD || (!LOWORD && !STEADY_BIT && !B0S && !BODY),
-                "the TVD variant carries no low order faces, steady state, "
-                "B0 or body");
+  static_assert(!TVD || (!LOWORD && !STEADY_BIT && !BODY),
+                "the TVD variant carries no low order faces or body "
+                "(steady state: the run-time flag)");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -376,8 +376,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   __shared__ double sRed[kThreads / 64];
 
   // steady state: a compile-time option of the plain variants, a run-time
-  // flag of the B0 and body variants (which would otherwise double)
-  const bool STEADY = STEADY_BIT || ((B0S || BODY) && a.steady != 0);
+  // flag of the B0, body and TVD variants (which would otherwise double)
+  const bool STEADY = STEADY_BIT || ((B0S || BODY || TVD) && a.steady != 0);
   const int tid = threadIdx.x;
   // a wave never spans two blocks (NC >= 64): the block index, and with it
   // every per-block value (cell size, run, time step), is wave-uniform and
@@ -736,7 +736,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               sF[sFi(lbf, dir, NFS, q)] = L[BX];
               sF[sFi(lbf, dir, NFS + 1, q)] = R[BX];
             }
-            err |= numerical_flux<0, FLUX, false>(ph, area, L, R, F, vdt, unA);
+            err |= numerical_flux<0, FLUX, false, B0S>(ph, area, L, R, F, vdt,
+                                                       unA, B0r);
           }
         } else if (BODY) {
           const unsigned char *m = body_mask(bf);
@@ -1453,9 +1454,32 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 4: BATS_K(LM, 68); break;                                             \
   default: BATS_K(LM, 69); break;                                            \
   }
+  // and with B0 splitting (FP64)
+#define BATS_LAUNCH_TVD_B0(LM, RB)                                           \
+  switch (opt & ((RB ? OPT_RESBETA : 0) | OPT_POWELL)) {                     \
+  case 0: BATS_K(LM, 80); break;                                             \
+  case 1: BATS_K(LM, 81); break;                                             \
+  case 4: BATS_K(LM, 84); break;                                             \
+  default: BATS_K(LM, 85); break;                                            \
+  }
   if (opt & OPT_TVD) {
-    if (opt & (OPT_LOWORDER | OPT_STEADY | OPT_B0 | OPT_BODY))
-      return hipErrorInvalidValue;
+    // steady state travels as the run-time flag
+    if (opt & (OPT_LOWORDER | OPT_BODY)) return hipErrorInvalidValue;
+    if (opt & OPT_B0) {
+      if constexpr (sizeof(T) == 8) {
+        switch (a.lim) {
+        case LIM_MINMOD: BATS_LAUNCH_TVD_B0(LIM_MINMOD, false); break;
+        case LIM_MC: BATS_LAUNCH_TVD_B0(LIM_MC, true); break;
+        case LIM_MC3: BATS_LAUNCH_TVD_B0(LIM_MC3, true); break;
+        case LIM_BETA: BATS_LAUNCH_TVD_B0(LIM_BETA, true); break;
+        case LIM_NO: BATS_LAUNCH_TVD_B0(LIM_NO, false); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+      } else {
+        return hipErrorInvalidValue;
+      }
+    }
     switch (a.lim) {
     case LIM_MINMOD: BATS_LAUNCH_TVD(LIM_MINMOD, false); break;
     case LIM_MC: BATS_LAUNCH_TVD(LIM_MC, true); break;
@@ -1516,6 +1540,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
 #undef BATS_LAUNCH_B0
 #undef BATS_LAUNCH_BODY
 #undef BATS_LAUNCH_TVD
+#undef BATS_LAUNCH_TVD_B0
   return hipGetLastError();
 }
 

@@ -126,8 +126,9 @@ typedef struct bats_config {
    * (correct_monotone_restrict, src/ModFaceValue.f90:233-408), and the
    * fine block's face values at the coarse side and one face in come from
    * tvd_reschange (:680-852,2269-2343).  Two-level grids, fused entry
-   * points and per-block routines, with or without the 8-wave source; not
-   * with low order faces, steady state, B0, the inner body or the surrogate
+   * points and per-block routines, with or without the 8-wave source and
+   * B0, time accurate or steady state; not with low order faces, the inner
+   * body or the surrogate
    * (BATS_EINVAL); 0 keeps this project's second-order prolongation. */
   int UseTvdResChange;
 } bats_config;
@@ -187,9 +188,9 @@ int bats_set_amr_grid(bats_handle *h, const int *iNeighbor_EB,
  * neighbours takes the mean of the four fine faces' B0 (set_b0_reschange,
  * src/ModB0.f90:338-351,366-455) and the fine blocks' face fluxes that
  * replace the coarse ones carry their B0.  FP64 handles, time accurate or
- * steady state; not with adaptive low order faces or the TVD
- * resolution-change faces (BATS_EINVAL at the stage).  A new grid drops
- * B0; B0_DGB = NULL drops it too. */
+ * steady state, with the default prolongation at resolution changes or
+ * the TVD faces; not with adaptive low order faces (BATS_EINVAL).  A new
+ * grid drops B0; B0_DGB = NULL drops it too. */
 int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
 
 /* Inner body (UseBody with the 'ionosphere' face boundary condition, as the

@@ -6,8 +6,9 @@ reference's advance_explicit order, stage by stage over two steps against
 the oracle's stage update within 1e-12; time accurate and steady state,
 with and without the 8-wave source and its B0 terms; an ionosphere body
 with a dipole B0, inside the fine level and across resolution changes
-(the fine-face pass with the body's face treatment); and the TVD
-resolution-change faces with B0 are refused."""
+(the fine-face pass with the body's face treatment); the TVD
+resolution-change faces with B0 and the 8-wave source, time accurate and
+steady state; and the TVD faces with a body are refused."""
 import numpy as np
 import pytest
 
@@ -80,6 +81,12 @@ FUSED = [
     dict(flux="linde", limiter="beta", beta=1.8, BetaLimiterResChange=1.2,
          nFaceLimiterResChange=2, UseDivbSource=True),
     dict(flux="hlld", limiter="mc", UseDivbSource=True, IsSteadyState=True),
+    # with the TVD resolution-change faces
+    dict(flux="hlld", limiter="mc", UseDivbSource=True, UseTvdResChange=True),
+    dict(flux="hlld", limiter="mc3", UseDivbSource=True, UseTvdResChange=True,
+         IsSteadyState=True),
+    dict(flux="linde", limiter="beta", beta=1.8, BetaLimiterResChange=1.2,
+         nFaceLimiterResChange=2, UseTvdResChange=True),
 ]
 
 
@@ -102,16 +109,17 @@ def test_fused_two_level_b0_16cube_matches_oracle():
                      UseDivbSource=True), U0, b0_dgb(g, _dipole), steps=1)
 
 
-@pytest.mark.parametrize("flux,limiter,powell", [
-    ("hlld", "mc", True), ("linde", "minmod", False)])
-def test_per_block_two_level_b0_matches_oracle(flux, limiter, powell):
-    print(f"case per-block two-level b0 flux={flux} powell={powell}",
-          flush=True)
+@pytest.mark.parametrize("flux,limiter,powell,tvd", [
+    ("hlld", "mc", True, False), ("linde", "minmod", False, False),
+    ("hlld", "mc3", True, True)])
+def test_per_block_two_level_b0_matches_oracle(flux, limiter, powell, tvd):
+    print(f"case per-block two-level b0 flux={flux} powell={powell} "
+          f"tvd={tvd}", flush=True)
     g = AmrGrid((4, 4, 4), sphere_refinement((4, 4, 4), 0.3))
     U0 = smooth_mhd_soa(g, seed=93, amp=0.2)
     _compare(g, dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
-                     UseDivbSource=powell), U0, b0_dgb(g, _dipole),
-             per_block=True)
+                     UseDivbSource=powell, UseTvdResChange=tvd), U0,
+             b0_dgb(g, _dipole), per_block=True)
 
 
 def _body_grid():
@@ -172,15 +180,10 @@ def test_fused_two_level_body_with_b0_matches_oracle(where, flux, limiter,
 
 
 def test_two_level_tvd_refusals():
-    print("case two-level tvd with b0/body refusals", flush=True)
+    print("case two-level tvd with body refusals", flush=True)
     g = _grid()
-    # the TVD resolution-change faces are not combined with B0 or a body
+    # the TVD resolution-change faces are not combined with a body
     U0 = smooth_mhd_soa(g, seed=95)
-    with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
-        s.set_b0(b0_dgb(g, _dipole))
-        s.upload_soa(U0)
-        with pytest.raises(StopMpi):
-            s.init_timestep()
     used, xyz0 = body_mask(g, 0.1)
     with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
         s.set_body(used, xyz0)

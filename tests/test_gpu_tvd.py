@@ -4,7 +4,8 @@ src/ModFaceValue.f90:233-408,680-852,2269-2343): on a two-level grid the
 fused stages match the oracle's stage update with the same treatment
 within 1e-12 per stage over two steps (Rusanov, Linde, HLLD; MC, Koren,
 minmod, beta, with the sharper resolution-change beta, and first order;
-with the 8-wave source, on 8^3 and 16^3 blocks),
+with the 8-wave source, on 8^3 and 16^3 blocks; steady state, fused and
+per-block),
 FP32 state within its rounding, the grid conserves to rounding, the step
 graph replays the eager stages, and the combinations the TVD variant does
 not carry are refused.  The per-block routines with the TVD faces are
@@ -92,6 +93,53 @@ def test_tvd_eight_wave_16cube_matches_oracle():
               check=check, grid=AmrGrid((2, 2, 2), refine, nCell=16))
 
 
+@pytest.mark.parametrize("per_block", [False, True])
+def test_tvd_steady_state_matches_oracle(per_block):
+    """Steady state (per-cell steps) with the TVD faces and the 8-wave
+    source, fused and through the per-block routines in the reference's
+    order (calc_timestep before update_state at the first stage)."""
+    print(f"case tvd steady per_block={per_block}", flush=True)
+    g = _grid()
+    U0 = smooth_mhd_soa(g, seed=57, amp=0.2)
+    sch = dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8,
+               UseDivbSource=True, UseTvdResChange=True, IsSteadyState=True)
+    o = po.Session(g, po.config_from_dict(sch, g), g.soa_to_state_vgb(U0))
+    o.exchange()
+    o.init_timestep()
+    from test_gpu_reschange_blocks import _fine_blocks
+    with Session(g, **sch) as s:
+        s.upload_soa(U0)
+        s.init_timestep()
+        for step in range(2):
+            o.s.nStep += 1
+            s.set_global_timestep()
+            for stage in (1, 2):
+                assert o.advance_stage(stage) == 0
+                if stage == 2:
+                    o.exchange()
+                if per_block:
+                    s.exchange_messages()
+                    s.set_stage(stage)
+                    for b in _fine_blocks(g):
+                        s.calc_face_value(b, True, True)
+                        s.calc_face_flux(True, b)
+                        s.save_cons_flux(b)
+                    s.message_pass_face()
+                    for b in range(1, g.nBlock + 1):
+                        s.calc_face_value(b, False, True)
+                        s.calc_face_flux(False, b)
+                        s.apply_cons_flux(b)
+                        s.calc_source(b)
+                        if stage == 1:
+                            s.calc_timestep(b)
+                        s.update_state(b)
+                else:
+                    s.advance_stage(stage)
+                e = rel_err(s.download_soa(), o.soa())
+                print(f"  step {step} stage {stage} rel err {e:.2e}")
+                assert e <= 1e-12, (step, stage, e)
+
+
 def test_tvd_fp32_state_matches_oracle_within_rounding():
     print("case tvd fp32", flush=True)
 
@@ -140,15 +188,15 @@ def test_tvd_refusals():
     print("case tvd refusals", flush=True)
     g = _grid()
     U0 = smooth_mhd_soa(g, seed=54)
-    for bad in (dict(IsSteadyState=True), dict(UseAdaptiveLowOrder=True)):
+    for bad in (dict(UseAdaptiveLowOrder=True),):
         with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True,
                      **bad) as s:
             s.upload_soa(U0)
             with pytest.raises(StopMpi):
                 s.init_timestep()
-    # nor do the per-block routines in steady state
+    # nor do the per-block routines with low order faces
     with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True,
-                 IsSteadyState=True) as s:
+                 UseAdaptiveLowOrder=True) as s:
         s.upload_soa(U0)
         s.exchange_messages()
         s.set_stage(1)
