@@ -1116,11 +1116,10 @@ extern "C" int bats_set_stage(bats_handle *h, int iStage) {
 static int check_block(bats_handle *h, int iBlock1) {
   if (!h || iBlock1 < 1 || iBlock1 > h->cfg.nBlock) return BATS_EINVAL;
   // the reference-shaped per-block routines run on FP64 state only; the
-  // TVD resolution-change faces not with low order faces or the body
+  // TVD resolution-change faces not with low order faces
   if (h->elemSize != 8) return BATS_EINVAL;
   const bats_config &c = h->cfg;
-  if (c.UseTvdResChange && h->nResSlab > 0 &&
-      (low_order_set(c) || h->useBody))
+  if (c.UseTvdResChange && h->nResSlab > 0 && low_order_set(c))
     return BATS_EINVAL;
   return ensure_block_buffers(h);
 }
@@ -1409,9 +1408,9 @@ static int launch_stage(bats_handle *h, int iStage, int calcDt, int noUpdate,
   if (h->useBody && (lowUse || resBeta)) return BATS_EINVAL;
   // the TVD treatment at resolution changes: second-order faces of the
   // fine sides from the records k_amr_fineflux leaves; not with low order
-  // faces, the body or the surrogate
+  // faces or the surrogate
   const bool tvd = c.UseTvdResChange && h->nResSlab > 0;
-  if (tvd && (low_order_set(c) || h->sur.mode != BATS_SUR_OFF || h->useBody))
+  if (tvd && (low_order_set(c) || h->sur.mode != BATS_SUR_OFF))
     return BATS_EINVAL;
   // the surrogate's face pass has no low order weights, one beta and no
   // 8-wave source

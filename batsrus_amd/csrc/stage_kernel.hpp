@@ -348,9 +348,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
                 "the body variant is FP64, without low order faces or the "
                 "resolution-change beta (steady state: the run-time flag)");
   constexpr bool TVD = (OPT & OPT_TVD) != 0;
-  static_assert(!TVD || (!LOWORD && !STEADY_BIT && !BODY),
-                "the TVD variant carries no low order faces or body "
-                "(steady state: the run-time flag)");
+  static_assert(!TVD || (!LOWORD && !STEADY_BIT),
+                "the TVD variant carries no low order faces (steady state: "
+                "the run-time flag)");
   constexpr int NC = NB * NB;          // columns per block
   constexpr int BPW = kThreads / NC;   // blocks per workgroup
   constexpr int NR = NB + 4;           // rows of the haloed plane
@@ -714,36 +714,32 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
           trec = tvd_rec(bf, 2 * dir + (fp >= NB - 1), row, k);
           recFlux = trec && (fp == 0 || fp == NB);
         }
-        if (TVD && trec) {
-          // (the normal-field face states for the 8-wave source: the
-          // record's, or those of the TVD face values)
-          if (recFlux) {
+        if (TVD && trec && recFlux) {
+          // the face at a coarser side: its record's flux (with the
+          // body's treatment when the fine block reaches the body) and the
+          // normal field of its face states for the 8-wave source
 #pragma unroll
-            for (int v = 0; v < NFLUX; v++) F[v] = trec[TVD_FLUX + v];
-            vdt = trec[TVD_FLUX + PP];
-            if (BNLDS) {
-              sF[sFi(lbf, dir, NFS, q)] = trec[TVD_BN];
-              sF[sFi(lbf, dir, NFS + 1, q)] = trec[TVD_BN + 1];
-            }
-          } else {
-            T L[NVAR], R[NVAR];
-            face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
+          for (int v = 0; v < NFLUX; v++) F[v] = trec[TVD_FLUX + v];
+          vdt = trec[TVD_FLUX + PP];
+          if (BNLDS) {
+            sF[sFi(lbf, dir, NFS, q)] = trec[TVD_BN];
+            sF[sFi(lbf, dir, NFS + 1, q)] = trec[TVD_BN + 1];
+          }
+        } else if ((TVD && trec) || BODY) {
+          // the face one cell in from a coarser side takes its fine-side
+          // state from the record; then the body's treatment (first order
+          // next to body cells, the ghost state between a true and a body
+          // cell, nothing between two body cells), in that order
+          T L[NVAR], R[NVAR];
+          face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
+          if (TVD && trec) {
             T *S = fp == 1 ? L : R;
 #pragma unroll
             for (int v = 0; v < NVAR; v++)
               S[v] = trec[TVD_FINEF + (dir == 0 ? v : rot_src(v))];
-            if (BNLDS) {
-              sF[sFi(lbf, dir, NFS, q)] = L[BX];
-              sF[sFi(lbf, dir, NFS + 1, q)] = R[BX];
-            }
-            err |= numerical_flux<0, FLUX, false, B0S>(ph, area, L, R, F, vdt,
-                                                       unA, B0r);
           }
-        } else if (BODY) {
           const unsigned char *m = body_mask(bf);
-          T L[NVAR], R[NVAR];
-          face_states<LIM>(bL, bR, c0, c1, c2, c3, L, R);
-          if (m) {
+          if (BODY && m) {
             // stencil flags and face centre in the original frame; the
             // face centre rotates with the y face
             bool u[4];
@@ -768,7 +764,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
             err |= numerical_flux<0, FLUX, false, B0S>(ph, area, L, R, F, vdt,
                                                        unA, B0r);
           }
-          if (POWELL) {
+          if (BNLDS) {
             sF[sFi(lbf, dir, NFS, q)] = skip ? T(0) : L[BX];
             sF[sFi(lbf, dir, NFS + 1, q)] = skip ? T(0) : R[BX];
           }
@@ -1462,9 +1458,33 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
   case 4: BATS_K(LM, 84); break;                                             \
   default: BATS_K(LM, 85); break;                                            \
   }
+  // and with the inner body (FP64; no resolution-change beta)
+#define BATS_LAUNCH_TVD_BODY(LM)                                             \
+  switch (opt & (OPT_POWELL | OPT_B0)) {                                     \
+  case 0: BATS_K(LM, 96); break;                                             \
+  case 1: BATS_K(LM, 97); break;                                             \
+  case 16: BATS_K(LM, 112); break;                                           \
+  default: BATS_K(LM, 113); break;                                           \
+  }
   if (opt & OPT_TVD) {
     // steady state travels as the run-time flag
-    if (opt & (OPT_LOWORDER | OPT_BODY)) return hipErrorInvalidValue;
+    if (opt & OPT_LOWORDER) return hipErrorInvalidValue;
+    if (opt & OPT_BODY) {
+      if constexpr (sizeof(T) == 8) {
+        if (opt & OPT_RESBETA) return hipErrorInvalidValue;
+        switch (a.lim) {
+        case LIM_MINMOD: BATS_LAUNCH_TVD_BODY(LIM_MINMOD); break;
+        case LIM_MC: BATS_LAUNCH_TVD_BODY(LIM_MC); break;
+        case LIM_MC3: BATS_LAUNCH_TVD_BODY(LIM_MC3); break;
+        case LIM_BETA: BATS_LAUNCH_TVD_BODY(LIM_BETA); break;
+        case LIM_NO: BATS_LAUNCH_TVD_BODY(LIM_NO); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+      } else {
+        return hipErrorInvalidValue;
+      }
+    }
     if (opt & OPT_B0) {
       if constexpr (sizeof(T) == 8) {
         switch (a.lim) {
@@ -1541,6 +1561,7 @@ inline hipError_t launch_stage_kernel(const StageArgs &a, int opt,
 #undef BATS_LAUNCH_BODY
 #undef BATS_LAUNCH_TVD
 #undef BATS_LAUNCH_TVD_B0
+#undef BATS_LAUNCH_TVD_BODY
   return hipGetLastError();
 }
 

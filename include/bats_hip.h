@@ -126,9 +126,9 @@ typedef struct bats_config {
    * (correct_monotone_restrict, src/ModFaceValue.f90:233-408), and the
    * fine block's face values at the coarse side and one face in come from
    * tvd_reschange (:680-852,2269-2343).  Two-level grids, fused entry
-   * points and per-block routines, with or without the 8-wave source and
-   * B0, time accurate or steady state; not with low order faces, the inner
-   * body or the surrogate
+   * points and per-block routines, with or without the 8-wave source, B0
+   * and the inner body, time accurate or steady state; not with low order
+   * faces or the surrogate
    * (BATS_EINVAL); 0 keeps this project's second-order prolongation. */
   int UseTvdResChange;
 } bats_config;
@@ -214,8 +214,9 @@ int bats_set_b0(bats_handle *h, const double *B0_DGB, int UseB0Source);
  * (bats_calc_face_value applies the face treatment that set_face_boundary
  * applies after calc_face_value in the reference, calc_face_flux zeroes
  * the body-body faces, update_state keeps body cells, calc_timestep masks
- * them); FP64 handles, not with adaptive low order faces, the
- * resolution-change beta or the TVD resolution-change faces (BATS_EINVAL).
+ * them), with the default prolongation at resolution changes or the TVD
+ * faces; FP64 handles, not with adaptive low order faces or the
+ * resolution-change beta (BATS_EINVAL).
  * A new grid drops the body; Used_GB = NULL drops it too. */
 int bats_set_body(bats_handle *h, const unsigned char *Used_GB,
                   const double *XyzStart_DB, double BodyRho, double BodyP,

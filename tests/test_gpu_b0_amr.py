@@ -6,15 +6,15 @@ reference's advance_explicit order, stage by stage over two steps against
 the oracle's stage update within 1e-12; time accurate and steady state,
 with and without the 8-wave source and its B0 terms; an ionosphere body
 with a dipole B0, inside the fine level and across resolution changes
-(the fine-face pass with the body's face treatment); the TVD
-resolution-change faces with B0 and the 8-wave source, time accurate and
-steady state; and the TVD faces with a body are refused."""
+(the fine-face pass with the body's face treatment); and the TVD
+resolution-change faces with B0, the 8-wave source and the body, time
+accurate and steady state."""
 import numpy as np
 import pytest
 
 import pyoracle as po
 from batsrus_amd.amr import AmrGrid, sphere_refinement
-from batsrus_amd.core import Session, StopMpi
+from batsrus_amd.core import Session
 from helpers import b0_dgb, body_mask, smooth_mhd_soa, rel_err
 from test_gpu_reschange_blocks import _per_block_stage
 
@@ -131,15 +131,17 @@ def _body_grid():
     return g, used, xyz0
 
 
-def test_per_block_two_level_body_matches_oracle():
-    print("case per-block two-level body across resolution changes",
-          flush=True)
+@pytest.mark.parametrize("tvd", [False, True])
+def test_per_block_two_level_body_matches_oracle(tvd):
+    print(f"case per-block two-level body across resolution changes "
+          f"tvd={tvd}", flush=True)
     g, used, xyz0 = _across_grid()
     U0 = smooth_mhd_soa(g, seed=96, amp=0.2)
     B0 = b0_dgb(g, lambda x, y, z: _dipole(x, y, z, c=(0.5, 0.5, 0.5),
                                            m=(0.0, 0.0, -1.0), scale=2e-4))
     _compare(g, dict(flux="hlld", limiter="mc", beta=1.5, cfl=0.8,
-                     UseDivbSource=True), U0, B0, per_block=True,
+                     UseDivbSource=True, UseTvdResChange=tvd), U0, B0,
+             per_block=True,
              body=(used, xyz0, dict(BodyRho=3.0, BodyP=0.5, B1rCoef=-1.0)))
 
 
@@ -152,16 +154,23 @@ def _across_grid():
     return g, used, xyz0
 
 
-BODY_CASES = [("inside", "hlld", "mc", False), ("inside", "hlld", "mc", True),
-              ("across", "hlld", "mc3", False), ("across", "hlld", "mc", True),
-              ("across", "linde", "minmod", False)]
+BODY_CASES = [("inside", "hlld", "mc", False, False),
+              ("inside", "hlld", "mc", True, False),
+              ("across", "hlld", "mc3", False, False),
+              ("across", "hlld", "mc", True, False),
+              ("across", "linde", "minmod", False, False),
+              ("across", "hlld", "mc", False, True),
+              ("across", "hlld", "mc3", True, True)]
 
 
-@pytest.mark.parametrize("where,flux,limiter,steady", BODY_CASES)
+@pytest.mark.parametrize("where,flux,limiter,steady,tvd", BODY_CASES)
 def test_fused_two_level_body_with_b0_matches_oracle(where, flux, limiter,
-                                                     steady):
+                                                     steady, tvd):
+    """tvd: with the TVD resolution-change faces (the fine faces that
+    replace coarse ones and the faces one cell in take the TVD values,
+    then the body's treatment)."""
     print(f"case fused two-level body {where} {flux} {limiter} "
-          f"steady={steady}", flush=True)
+          f"steady={steady} tvd={tvd}", flush=True)
     g, used, xyz0 = _body_grid() if where == "inside" else _across_grid()
     nb = ~used.reshape(g.nBlock, -1).all(axis=1)
     nei = g.iNeighbor_EB.reshape(-1, 6)
@@ -174,19 +183,7 @@ def test_fused_two_level_body_with_b0_matches_oracle(where, flux, limiter,
     B0 = b0_dgb(g, lambda x, y, z: _dipole(x, y, z, c=(0.5, 0.5, 0.5),
                                            m=(0.0, 0.0, -1.0), scale=2e-4))
     _compare(g, dict(flux=flux, limiter=limiter, beta=1.5, cfl=0.8,
-                     UseDivbSource=True, IsSteadyState=steady), U0, B0,
+                     UseDivbSource=True, IsSteadyState=steady,
+                     UseTvdResChange=tvd), U0, B0,
              body=(used, xyz0, dict(BodyRho=3.0, BodyP=0.5, B1rCoef=-1.0)),
              steady=steady)
-
-
-def test_two_level_tvd_refusals():
-    print("case two-level tvd with body refusals", flush=True)
-    g = _grid()
-    # the TVD resolution-change faces are not combined with a body
-    U0 = smooth_mhd_soa(g, seed=95)
-    used, xyz0 = body_mask(g, 0.1)
-    with Session(g, flux="hlld", limiter="mc", UseTvdResChange=True) as s:
-        s.set_body(used, xyz0)
-        s.upload_soa(U0)
-        with pytest.raises(StopMpi):
-            s.init_timestep()
