@@ -363,13 +363,12 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
   // LDS slots of the x/y faces: with the 8-wave scheme the face loop keeps
   // the normal-field face states for the source when two workgroups per CU
   // (three for FP32) still fit in the 160 KiB of LDS with them, and always
-  // in the body and TVD variants (which change them); otherwise the update
-  // recomputes them from the plane
+  // in the body variant (which changes them); otherwise the update
+  // recomputes them from the plane (and the TVD faces from their records)
   constexpr int LDS_BN = (BPW * NVAR * NR * NS + BPW * 2 * (NFS + 2) * NFX) *
                              (int)sizeof(T) + (kThreads / 64) * 8;
   constexpr bool BNLDS =
-      POWELL &&
-      (BODY || TVD || LDS_BN * (sizeof(T) == 4 ? 3 : 2) <= 160 * 1024);
+      POWELL && (BODY || LDS_BN * (sizeof(T) == 4 ? 3 : 2) <= 160 * 1024);
   constexpr int NFL = BNLDS ? NFS + 2 : NFS;
   __shared__ T sP[BPW * NVAR * NR * NS];
   __shared__ T sF[BPW * 2 * NFL * NFX];
@@ -896,6 +895,40 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2)
               } else {
                 face_lr<LIM>(by[0], by[1], p[0], p[1], p[2], p[3], LyLo, RyLo);
                 face_lr<LIM>(by[1], by[2], p[1], p[2], p[3], p[4], LyHi, RyHi);
+              }
+              if (TVD) {
+                // the TVD faces of coarser sides as the face loop took
+                // them: the face at the side from its record, the face one
+                // cell in with the record's fine-side state (x faces: the
+                // record (j, k), y faces: (i, k))
+                const T *rx0 = i <= 1 ? tvd_rec(b, 0, j, k) : nullptr;
+                const T *rx1 = i >= NB - 2 ? tvd_rec(b, 1, j, k) : nullptr;
+                const T *ry0 = j <= 1 ? tvd_rec(b, 2, i, k) : nullptr;
+                const T *ry1 = j >= NB - 2 ? tvd_rec(b, 3, i, k) : nullptr;
+                if (rx0 && i == 0) {
+                  LxLo = rx0[TVD_BN];
+                  RxLo = rx0[TVD_BN + 1];
+                  LxHi = rx0[TVD_FINEF + BX];
+                }
+                if (rx0 && i == 1) LxLo = rx0[TVD_FINEF + BX];
+                if (rx1 && i == NB - 1) {
+                  LxHi = rx1[TVD_BN];
+                  RxHi = rx1[TVD_BN + 1];
+                  RxLo = rx1[TVD_FINEF + BX];
+                }
+                if (rx1 && i == NB - 2) RxHi = rx1[TVD_FINEF + BX];
+                if (ry0 && j == 0) {
+                  LyLo = ry0[TVD_BN];
+                  RyLo = ry0[TVD_BN + 1];
+                  LyHi = ry0[TVD_FINEF + BY];
+                }
+                if (ry0 && j == 1) LyLo = ry0[TVD_FINEF + BY];
+                if (ry1 && j == NB - 1) {
+                  LyHi = ry1[TVD_BN];
+                  RyHi = ry1[TVD_BN + 1];
+                  RyLo = ry1[TVD_FINEF + BY];
+                }
+                if (ry1 && j == NB - 2) RyHi = ry1[TVD_FINEF + BY];
               }
             }
             if (a.corr) {
